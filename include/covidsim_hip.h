@@ -1,0 +1,380 @@
+/*
+ * covidsim_hip.h -- C ABI of the MI355X (gfx950) infection-step library.
+ *
+ * The library replaces CovidSim's per-timestep infection step:
+ *   InfectSweep(double t, int run)   src/Sweep.cpp:283-881 (declared src/Sweep.h:4-9)
+ *   IncubRecoverySweep(double t)     src/Sweep.cpp:883-979
+ * together with the hazards of src/CalcInfSusc.cpp, the spatial-kernel lookup
+ * of src/Kernels.cpp, the cell sampling tables of UpdateProbs
+ * (src/CovidSim.cpp:2752-2802) and the state transitions those sweeps call in
+ * src/Update.cpp.  CovidSim itself has no plugin API: its host code calls the
+ * two sweeps as free functions over global arrays (src/CovidSim.cpp:1488,
+ * :1490).  A shim translation unit defines those two symbols and forwards to
+ * the functions below (the binding is shown in INTEGRATION.md).
+ *
+ * Conventions
+ *  - Every function returns 0 on success and a negative CS_E* code on error;
+ *    cs_last_error() gives the message.  Errors are never silent (the
+ *    reference drops overflowing infection-queue entries, src/Sweep.cpp:2043;
+ *    here an overflow is reported and the step is re-run with a larger queue).
+ *  - Plain C types only; every pointer argument is host memory owned by the
+ *    caller and is not retained after the call returns.
+ *  - One caller thread per engine; calls are ordered on the engine's stream.
+ *  - All realisations held by an engine advance together in one launch.
+ */
+#ifndef COVIDSIM_HIP_H_
+#define COVIDSIM_HIP_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- sizes that mirror the reference's compile-time constants ---------- */
+#define CS_NUM_AGE_GROUPS      17     /* src/Constants.h NUM_AGE_GROUPS */
+#define CS_AGE_GROUP_WIDTH     5      /* src/Constants.h AGE_GROUP_WIDTH */
+#define CS_MAX_PLACE_TYPES     5      /* src/Country.h MAX_NUM_PLACE_TYPES */
+#define CS_HOUSE               5      /* src/Constants.h House   (setting index) */
+#define CS_SPATIAL             6      /* src/Constants.h Spatial (setting index) */
+#define CS_NUM_SETTINGS        7      /* MAX_NUM_PLACE_TYPES + 2 */
+#define CS_NUM_NPI             6      /* CI, HQ, PC, SD, ESD, DCT (src/Constants.h:121-126) */
+#define CS_MAX_HOUSEHOLD_SIZE  10     /* src/Country.h */
+#define CS_CDF_RES             20     /* src/Constants.h CDF_RES */
+#define CS_MAX_INFECTIOUS_STEPS 2550  /* src/Constants.h */
+#define CS_DAYS_PER_YEAR       364
+#define CS_INFECT_TYPE_MASK    16
+#define CS_MAX_HOLIDAYS        64
+#define CS_UNSET_TIME          65534  /* USHRT_MAX - 1, the reference's "never" */
+
+/* InfStat codes, src/InfStat.h:7-38 */
+#define CS_SUSCEPTIBLE          0
+#define CS_LATENT               1
+#define CS_INF_ALMOST_SYMPT    (-1)
+#define CS_INF_ASYMPT           2
+#define CS_CASE                (-2)
+#define CS_RECOVERED_ASYMP      3
+#define CS_RECOVERED_SYMP      (-3)
+#define CS_IMMUNE_AT_START      4
+#define CS_DEAD_ASYMP           5
+#define CS_DEAD_SYMP           (-5)
+
+/* Severity codes, src/InfStat.h:41-58 */
+#define CS_SEV_ASYMPTOMATIC 0
+#define CS_SEV_MILD         1
+#define CS_SEV_ILI          2
+#define CS_SEV_SARI         3
+#define CS_SEV_CRITICAL     4
+#define CS_SEV_STEPDOWN     5
+#define CS_SEV_DEAD         6
+#define CS_SEV_RECOVERED    7
+
+/* error codes */
+#define CS_OK              0
+#define CS_EINVAL        (-1)   /* bad argument / inconsistent sizes          */
+#define CS_EHIP          (-2)   /* HIP runtime error                           */
+#define CS_ENOMEM        (-3)   /* device allocation failed                    */
+#define CS_ESTATE        (-4)   /* call out of order (e.g. no population yet)  */
+#define CS_ERANGE        (-5)   /* kernel r^2 beyond the lookup table, src/Kernels.cpp:110-115 */
+#define CS_EUNSUPPORTED  (-6)   /* parameter combination outside this path     */
+#define CS_EOVERFLOW     (-7)   /* event queue full (retried internally first) */
+
+/* ICDF table slots in cs_params.icdf[] (src/Param.h:168-170) */
+enum {
+    CS_ICDF_LATENT = 0, CS_ICDF_INFECTIOUS, CS_ICDF_MILD_RECOV, CS_ICDF_ILI_RECOV,
+    CS_ICDF_SARI_RECOV, CS_ICDF_CRIT_CRITRECOV, CS_ICDF_CRITRECOV_RECOV,
+    CS_ICDF_ILI_SARI, CS_ICDF_SARI_CRIT, CS_ICDF_ILI_DEATH, CS_ICDF_SARI_DEATH,
+    CS_ICDF_STEPDOWN_DEATH, CS_ICDF_CRIT_DEATH, CS_NUM_ICDF
+};
+/* per-age mean slots in cs_params.mean_by_age[] (src/Param.h:173-175) */
+enum {
+    CS_MEAN_MILD_RECOV = 0, CS_MEAN_ILI_RECOV, CS_MEAN_SARI_RECOV, CS_MEAN_CRIT_CRITRECOV,
+    CS_MEAN_CRITRECOV_RECOV, CS_MEAN_STEPDOWN_DEATH, CS_MEAN_ILI_SARI, CS_MEAN_SARI_CRIT,
+    CS_MEAN_CRIT_DEATH, CS_MEAN_SARI_DEATH, CS_MEAN_ILI_DEATH, CS_NUM_MEANS
+};
+
+/*
+ * Derived model constants the path reads.  Field names follow Param
+ * (src/Param.h); values are those after Params::ReadParams and the
+ * run-start updates of InitModel (src/CovidSim.cpp:1140-1187).
+ */
+typedef struct cs_params {
+    /* time (src/ReadParams.cpp:1754-1760) */
+    double time_steps_per_day;         /* P.TimeStepsPerDay */
+    double model_time_step;            /* P.ModelTimeStep   */
+    double false_positive_rate;        /* P.FalsePositiveRate (only 0 is supported) */
+    int    do_seasonality;
+    int    pad0;
+    double seasonality[CS_DAYS_PER_YEAR];
+    /* transmission */
+    double local_beta;                 /* P.LocalBeta (InitTransmissionCoeffs) */
+    double household_trans;            /* P.HouseholdTrans */
+    double household_denom_lookup[CS_MAX_HOUSEHOLD_SIZE];
+    int    do_households, do_places, num_place_types, hotel_place_type;
+    int    place_type_no_air_num, care_home_place_type;
+    int    do_symptoms, do_severity, do_latent, do_infectiousness_profile;
+    int    do_real_sympt_withdrawal, absenteeism_place_closure;
+    int    no_infectiousness_sd_in_hh, do_death, do_ad_units, output_ad_unit_age;
+    int    include_stepdown_to_death, do_digital_contact_tracing;
+    int    do_blanket_move_restr, interventions_active;  /* must be 0: interventions are outside this path */
+    double place_type_trans[CS_MAX_PLACE_TYPES];
+    double place_type_group_size_param1[CS_MAX_PLACE_TYPES];
+    double place_type_prop_between_group_links[CS_MAX_PLACE_TYPES];
+    double sympt_place_type_contact_rate[CS_MAX_PLACE_TYPES];
+    double sympt_place_type_withdrawal_prop[CS_MAX_PLACE_TYPES];
+    double sympt_spatial_contact_rate;
+    double relative_spatial_contact[CS_NUM_AGE_GROUPS];
+    double relative_spatial_contact_susc[CS_NUM_AGE_GROUPS];
+    double waifw[CS_NUM_AGE_GROUPS][CS_NUM_AGE_GROUPS];          /* [susceptible][infector] */
+    double waifw_spatial[CS_NUM_AGE_GROUPS][CS_NUM_AGE_GROUPS];
+    double age_susceptibility[CS_NUM_AGE_GROUPS];
+    double age_infectiousness[CS_NUM_AGE_GROUPS];
+    double efficacies[CS_NUM_NPI][CS_NUM_SETTINGS];              /* P.Efficacies */
+    double care_home_resident_household_scaling, care_home_resident_spatial_scaling;
+    double care_home_worker_group_scaling, care_home_resident_place_scaling, care_home_rel_prob_hosp;
+    double treat_inf_drop, vacc_inf_drop, treat_susc_drop, vacc_susc_drop, vacc_susc_drop2;
+    double treat_sympt_drop, vacc_sympt_drop, treat_death_drop;
+    uint16_t us_vacc_time_to_efficacy, us_vacc_time_efficacy_switch;
+    uint16_t us_hquarantine_house_duration, us_case_isolation_delay;
+    uint16_t us_case_isolation_duration, us_case_absenteeism_delay;
+    uint16_t us_case_absenteeism_duration, pad1;
+    /* natural history */
+    int    n_infectiousness;           /* valid entries of infectiousness[] */
+    int    pad2;
+    double infectiousness[CS_MAX_INFECTIOUS_STEPS]; /* P.infectiousness, per step */
+    double latent_period, infectious_period, latent_to_sympt_delay;
+    double sympt_infectiousness, asympt_infectiousness, infectiousness_sd, susceptibility_sd;
+    double proportion_symptomatic[CS_NUM_AGE_GROUPS];
+    double icdf[CS_NUM_ICDF][CS_CDF_RES + 1];                    /* exp(-x) transformed tables */
+    double mean_by_age[CS_NUM_MEANS][CS_NUM_AGE_GROUPS];
+    double prop_ili[CS_NUM_AGE_GROUPS], prop_sari[CS_NUM_AGE_GROUPS], prop_critical[CS_NUM_AGE_GROUPS];
+    double cfr_critical[CS_NUM_AGE_GROUPS], cfr_sari[CS_NUM_AGE_GROUPS], cfr_ili[CS_NUM_AGE_GROUPS];
+    double cfr_critical_scale, cfr_sari_scale, cfr_ili_scale;
+    double control_prop_cases_id;      /* P.ControlPropCasesId during the run */
+    int    case_absent_child_age_cutoff;
+    int    place_close_round_household;
+    double case_absent_child_prop_adult_carers;
+    double location_initial_infection[2]; /* P.LocationInitialInfection[0] after seeding */
+    /* spatial kernel (src/Kernels.h:75-115); tables are passed separately */
+    int    kernel_size;                /* KernelLookup.size_ */
+    int    kernel_expansion;           /* KernelLookup.expansion_factor_ */
+    double kernel_delta;               /* KernelLookup.delta_ */
+    int    kernel_type;                /* KernelStruct.type_ (1..7) */
+    int    pad3;
+    double kernel_shape, kernel_scale, kernel_p3, kernel_p4;
+    /* geometry (src/Dist.cpp) */
+    int    do_utm_coords, do_periodic_boundaries;
+    int    nch;                        /* P.nch: cells per column */
+    int    pad4;
+    double cell_width, cell_height;    /* P.in_cells_ */
+    double degrees_width, degrees_height; /* P.in_degrees_ */
+    double bbox_bottom_left_y;         /* P.SpatialBoundingBox.bottom_left().y */
+    double sinx[361], cosx[361], asin2sqx[1001];
+    /* holidays (src/Sweep.cpp:889-918) */
+    int    do_holidays, num_holidays;
+    double holiday_effect[CS_MAX_PLACE_TYPES];
+    double holiday_start[CS_MAX_HOLIDAYS], holiday_duration[CS_MAX_HOLIDAYS];
+    double holidays_start_day_simtime;
+} cs_params;
+
+/*
+ * The population a realisation runs on, as CovidSim's SetupModel leaves it
+ * (src/Model.h:295-303).  Person-indexed arrays have n_people entries;
+ * place_links/place_group_links are [n_people][CS_MAX_PLACE_TYPES].
+ * Places of type t: member list place_members[t][place_member_start[t][k] ..
+ * +place_n[t][k]), groups place_group_start[t][place_groups_start[t][k] + g]
+ * (offsets into the member list of that place) and place_group_size likewise.
+ */
+typedef struct cs_population {
+    int32_t n_people, n_households;
+    int32_t n_cells;                  /* NumPopulatedCells */
+    int32_t n_cells_total;            /* NumCells (ncw*nch)  */
+    int32_t n_mcells;                 /* total microcells    */
+    int32_t n_adunits;
+    /* persons */
+    const int32_t*  hh;               /* Person.hh     */
+    const int32_t*  mcell;            /* Person.mcell  */
+    const int32_t*  pcell;            /* Person.pcell  (index into Cells[]) */
+    const uint8_t*  age;              /* Person.age (years) */
+    const uint8_t*  flags;            /* bit0 care_home_resident, bit1 keyworker, bit2 esocdist_comply, bit3 digitalContactTracingUser */
+    const int32_t*  place_links;      /* Person.PlaceLinks        */
+    const uint16_t* place_group_links;/* Person.PlaceGroupLinks   */
+    /* households */
+    const int32_t*  hh_first;         /* Household.FirstPerson */
+    const uint16_t* hh_nh;            /* Household.nh  */
+    const uint16_t* hh_nhr;           /* Household.nhr */
+    const float*    hh_loc;           /* Household.loc, [n_households][2] */
+    /* populated cells, in CellLookup order */
+    const int32_t*  cell_index;       /* CellLookup[j] - Cells      */
+    const int32_t*  cell_n;           /* Cell.n                     */
+    const int32_t*  cell_member_start;/* offset of cell j in cell_members (n_cells+1 entries) */
+    const int32_t*  cell_members;     /* Cell.members (concatenated) */
+    /* microcells */
+    const int32_t*  mcell_adunit;     /* Microcell.adunit */
+    /* places */
+    int32_t         n_places[CS_MAX_PLACE_TYPES];
+    const int32_t*  place_n[CS_MAX_PLACE_TYPES];
+    const int32_t*  place_mcell[CS_MAX_PLACE_TYPES];
+    const int32_t*  place_member_start[CS_MAX_PLACE_TYPES];  /* n_places+1 */
+    const int32_t*  place_members[CS_MAX_PLACE_TYPES];
+    const int32_t*  place_groups_start[CS_MAX_PLACE_TYPES];  /* n_places+1, into group arrays */
+    const int32_t*  place_group_start[CS_MAX_PLACE_TYPES];
+    const int32_t*  place_group_size[CS_MAX_PLACE_TYPES];
+} cs_population;
+
+/*
+ * Dynamic state of one realisation (the mutable fields of Person, Cell and
+ * Place that the path reads or writes).  Used to load a realisation from a
+ * host that initialised it (InitModel + SeedInfection) and to copy it back.
+ * Person arrays: n_people entries.  cell_* : n_cells entries (populated-cell
+ * order).  cell_perm: the permutation array of every populated cell,
+ * concatenated with the offsets of cs_population.cell_member_start
+ * (Cell.susceptible[0..n) = S | L | I | R+D segments).
+ */
+typedef struct cs_state {
+    int8_t*   inf;                    /* InfStat code        */
+    uint8_t*  severity_current;
+    uint8_t*  severity_final;
+    uint8_t*  bits;                   /* bit0 to_die, bit1 detected */
+    int32_t*  listpos;
+    int32_t*  infector;
+    int16_t*  infect_type;
+    float*    infectiousness;
+    float*    susc;
+    float*    prob_care;              /* Person.ProbCare */
+    uint16_t* infection_time;
+    uint16_t* latent_time;
+    uint16_t* recovery_or_death_time;
+    uint16_t* sari_time;
+    uint16_t* critical_time;
+    uint16_t* stepdown_time;
+    uint16_t* detected_time;
+    uint16_t* absent_start_time;
+    uint16_t* absent_stop_time;
+    /* intervention fields read by the hazards (only when interventions_active) */
+    uint16_t* treat_start_time;
+    uint16_t* treat_stop_time;
+    uint16_t* vacc_start_time;
+    uint16_t* isolation_start_time;
+    uint16_t* quar_start_time;        /* HostsQuarantine[].start_time */
+    uint8_t*  quar_comply;            /* HostsQuarantine[].comply     */
+    uint8_t*  dct_traced;             /* Person.digitalContactTraced  */
+    /* cells */
+    int32_t*  cell_S; int32_t* cell_L; int32_t* cell_I; int32_t* cell_R; int32_t* cell_D;
+    int32_t*  cell_cumTC;
+    int32_t*  cell_perm;
+    /* places (per type, n_places entries) */
+    uint16_t* place_close_start[CS_MAX_PLACE_TYPES];
+    uint16_t* place_close_end[CS_MAX_PLACE_TYPES];
+    /* microcells (n_mcells entries; intervention state read by hazards) */
+    uint8_t*  mcell_socdist;          /* TreatStat */
+    uint8_t*  mcell_moverest;
+} cs_state;
+
+/*
+ * Counters a step adds to (the StateT fields of src/Model.h:68-122 that the
+ * path writes).  cs_get_counters returns the totals accumulated since the
+ * last call with reset != 0.  Age arrays have CS_NUM_AGE_GROUPS entries.
+ */
+#define CS_NUM_SEV_KINDS 8
+enum { CS_SK_MILD = 0, CS_SK_ILI, CS_SK_SARI, CS_SK_CRITICAL, CS_SK_CRITRECOV,
+       CS_SK_DEATH_ILI, CS_SK_DEATH_SARI, CS_SK_DEATH_CRITICAL };
+typedef struct cs_counters {
+    int64_t cumI, cumC, cumDC, cumFC, cumTG, cumSI, nTG;
+    int64_t cumAC, cumACS, cumAA;
+    double  sumRad2, maxRad2;
+    int64_t cumItype[CS_INFECT_TYPE_MASK];
+    int64_t cumIa[CS_NUM_AGE_GROUPS], cumCa[CS_NUM_AGE_GROUPS], cumDa[CS_NUM_AGE_GROUPS];
+    int64_t cumI_keyworker[2], cumC_keyworker[2];
+    /* severity prevalence change (signed) and cumulative incidence, total and by age */
+    int64_t sev_prev[CS_NUM_SEV_KINDS], sev_cum[CS_NUM_SEV_KINDS];
+    int64_t sev_prev_age[CS_NUM_SEV_KINDS][CS_NUM_AGE_GROUPS];
+    int64_t sev_cum_age[CS_NUM_SEV_KINDS][CS_NUM_AGE_GROUPS];
+    /* population totals after the step */
+    int64_t S, L, I, R, D;
+} cs_counters;
+
+/* Event-count tallies used for the algorithmic-bytes roofline figure. */
+typedef struct cs_work_counts {
+    int64_t n_inf, n_hh_inf, n_hh_member, n_place_link, n_place_cand;
+    int64_t n_sp_draw, n_sp_susc, n_new, n_active, n_transition, n_cells;
+} cs_work_counts;
+
+typedef struct cs_engine cs_engine;
+
+const char* cs_last_error(const cs_engine* e);
+const char* cs_version(void);
+
+/* lifetime */
+int  cs_create(cs_engine** out, int device);
+void cs_destroy(cs_engine* e);
+
+/* set-up: parameters, kernel tables (KernelLookup lookup_/hi_res_, each
+ * kernel_size+1 doubles; NULL builds them here from the kernel fields of
+ * cs_params, src/Kernels.cpp:17-45), population (copied to HBM).  The cell
+ * max_trans table (src/Kernels.cpp:48-63) is built on the device. */
+int  cs_set_params(cs_engine* e, const cs_params* p, const double* kernel_lookup, const double* kernel_hi_res);
+int  cs_set_population(cs_engine* e, const cs_population* pop);
+
+/* realisations: allocate n_real realisations, each keyed by its own 64-bit
+ * seed.  cs_reset_realisations runs InitModel's per-person reset
+ * (src/CovidSim.cpp:1006-1037) and SeedInfection's random-location branch
+ * (src/CovidSim.cpp:1244-1381) on the device for every realisation, then
+ * UpdateProbs (src/CovidSim.cpp:2752-2802). */
+int  cs_alloc_realisations(cs_engine* e, int n_real, const uint64_t* seeds);
+int  cs_reset_realisations(cs_engine* e, int n_seed_infections);
+
+/* host-initialised state in and out (drop-in use under CovidSim's RunModel) */
+int  cs_load_state(cs_engine* e, int real, const cs_state* s);
+int  cs_store_state(cs_engine* e, int real, cs_state* s);
+
+/* the two sweeps the library replaces; t = CurrSimTime in days */
+int  cs_infect_sweep(cs_engine* e, double t);
+int  cs_incub_recovery_sweep(cs_engine* e, double t);
+/* UpdateProbs(0) for every realisation (src/CovidSim.cpp:2752-2802) */
+int  cs_update_probs(cs_engine* e);
+/* RunModel's day loop without host round trips: n_steps timesteps of
+ * InfectSweep + IncubRecoverySweep + the S-drop UpdateProbs check
+ * (src/CovidSim.cpp:1426-1522), recording daily totals every
+ * steps_per_sample steps.  The loop is captured once into a HIP graph. */
+int  cs_run_steps(cs_engine* e, double t0, int n_steps, int steps_per_sample);
+
+/* results */
+int  cs_get_counters(cs_engine* e, int real, cs_counters* out, int reset);
+/* daily series [n_real][n_samples][CS_TS_FIELDS] recorded by cs_run_steps */
+#define CS_TS_FIELDS 8   /* S, L, I, R, D, incI, incC, incDeath */
+int  cs_get_timeseries(cs_engine* e, int32_t* out, int max_samples, int* n_samples);
+int  cs_get_work_counts(cs_engine* e, cs_work_counts* out, int reset);
+int  cs_sync(cs_engine* e);
+
+/* ---- direct evaluation of path components (parity tests) -------------- */
+/* KernelLookup::num for n squared distances (src/Kernels.cpp:108-129) */
+int  cs_kernel_num(cs_engine* e, const double* r2, double* out, int n);
+/* dist2 between persons (src/Dist.cpp:49-60) */
+int  cs_dist2_persons(cs_engine* e, const int32_t* a, const int32_t* b, double* out, int n);
+/* cell tables of realisation real: max_trans [n_cells][n_cells],
+ * cum_trans [n_cells][n_cells], invcdf [n_cells][1025], tot_prob, S0 */
+int  cs_get_cell_tables(cs_engine* e, int real, float* max_trans, float* cum_trans,
+                        int32_t* invcdf, float* tot_prob, int32_t* S0);
+/* hazards (src/CalcInfSusc.cpp:12-80) of realisation real at TimeStepNow ts.
+ * kind: 0 HouseInf(person) 1 PlaceInf(person,pt) 2 SpatialInf(person)
+ *       3 PersonInf(person) 4 HouseSusc(person,infector) 5 PlaceSusc(person,pt)
+ *       6 SpatialSusc(person) 7 PersonSusc(person,infector) */
+int  cs_eval_hazards(cs_engine* e, int real, int kind, uint16_t ts, const int32_t* person,
+                     const int32_t* other, double* out, int n);
+/* InverseCdf::choose for given uniforms u (src/InverseCdf.cpp:60-74) */
+int  cs_icdf_choose(cs_engine* e, int table, const double* u, const double* mean,
+                    uint16_t* out, int n);
+/* spatial indexing for given uniforms (src/Sweep.cpp:716-768): infector
+ * index within a normalised cumulative list, destination cell from the
+ * InvCDF + cum_trans scan of source cell `cell` in realisation real. */
+int  cs_select_infector(cs_engine* e, const float* cell_inf, int n_inf, const double* u,
+                        int32_t* out, int n);
+int  cs_select_dest_cell(cs_engine* e, int real, int cell, const double* u, int32_t* out, int n);
+/* raw counter-based RNG output (Philox4x32-10) for known-answer tests */
+int  cs_philox(cs_engine* e, const uint32_t* ctr_key, uint32_t* out, int n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* COVIDSIM_HIP_H_ */

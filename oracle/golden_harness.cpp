@@ -1,0 +1,625 @@
+// golden_harness.cpp -- test infrastructure, not product code.
+//
+// Links the unmodified CovidSim objects built by oracle/Makefile.ref with
+// Sweep.cpp compiled as InfectSweep_cpu/IncubRecoverySweep_cpu, and supplies
+// the two sweep symbols itself (src/Sweep.h:4-9).  Each call forwards to the
+// reference sweep unchanged, so a harness run reproduces covidsim_ref's run
+// exactly; around the calls it writes what the parity tests need:
+//
+//   CS_DUMP_DIR=<dir>   required to dump anything
+//   CS_DUMP_STEPS=a,b   TimeStepNow values at which the state of realisation
+//                       0 is written before InfectSweep (default "0")
+//   CS_DUMP_GOLDEN=1    at the first call also write golden vectors: the
+//                       kernel lookup num() (src/Kernels.cpp:108-129), person
+//                       dist2 (src/Dist.cpp:49-60), InverseCdf::choose
+//                       (src/InverseCdf.cpp:60-74) with its uniform, and the
+//                       eight hazards of src/CalcInfSusc.cpp over a grid of
+//                       synthetic person states
+//   CS_SERIES=<file>    append one record per step and realisation:
+//                       run, TimeStepNow, S, L, I, R, D, cumI, cumC, cumD
+//                       (int32 each), read after IncubRecoverySweep
+//
+// Output arrays are .npy files; cs_params is written as its raw bytes
+// (params.bin) with the layout of include/covidsim_hip.h.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+#include <functional>
+#include <limits>
+#include <climits>
+#include <iostream>
+#include <fstream>
+#include <sstream>
+
+// KernelLookup keeps its tables private (src/Kernels.h:79-87); the harness
+// reads them in place, so the class is opened up for this one TU.
+#define private public
+#include "Kernels.h"
+#undef private
+
+#include "CalcInfSusc.h"
+#include "Constants.h"
+#include "Dist.h"
+#include "InverseCdf.h"
+#include "Model.h"
+#include "ModelMacros.h"
+#include "Param.h"
+#include "Rand.h"
+#include "Sweep.h"
+
+#include "../include/covidsim_hip.h"
+
+void InfectSweep_cpu(double t, int run);
+void IncubRecoverySweep_cpu(double t);
+
+namespace {
+
+std::string g_dir;
+std::vector<int> g_steps;
+bool g_golden = false;
+bool g_started = false;
+FILE* g_series = nullptr;
+int g_last_run = -1;
+
+void save_npy(const std::string& name, const char* descr, const std::vector<size_t>& shape,
+              const void* data, size_t elem)
+{
+    std::string path = g_dir + "/" + name + ".npy";
+    FILE* f = fopen(path.c_str(), "wb");
+    if (!f) { perror(path.c_str()); exit(2); }
+    std::string sh = "(";
+    size_t n = 1;
+    for (size_t i = 0; i < shape.size(); i++) {
+        sh += std::to_string(shape[i]);
+        sh += (shape.size() == 1) ? "," : (i + 1 < shape.size() ? ", " : "");
+        n *= shape[i];
+    }
+    sh += ")";
+    std::string hdr = std::string("{'descr': '") + descr + "', 'fortran_order': False, 'shape': " + sh + ", }";
+    size_t total = 10 + hdr.size() + 1;
+    size_t pad = (64 - total % 64) % 64;
+    hdr += std::string(pad, ' ') + "\n";
+    unsigned char magic[8] = {0x93, 'N', 'U', 'M', 'P', 'Y', 1, 0};
+    fwrite(magic, 1, 8, f);
+    uint16_t hl = (uint16_t)hdr.size();
+    fwrite(&hl, 2, 1, f);
+    fwrite(hdr.data(), 1, hdr.size(), f);
+    if (n) fwrite(data, elem, n, f);
+    fclose(f);
+}
+template <class T> const char* npdt();
+template <> const char* npdt<int8_t>() { return "|i1"; }
+template <> const char* npdt<uint8_t>() { return "|u1"; }
+template <> const char* npdt<int16_t>() { return "<i2"; }
+template <> const char* npdt<uint16_t>() { return "<u2"; }
+template <> const char* npdt<int32_t>() { return "<i4"; }
+template <> const char* npdt<uint32_t>() { return "<u4"; }
+template <> const char* npdt<float>() { return "<f4"; }
+template <> const char* npdt<double>() { return "<f8"; }
+template <class T> void save(const std::string& name, const std::vector<T>& v)
+{
+    save_npy(name, npdt<T>(), {v.size()}, v.data(), sizeof(T));
+}
+template <class T> void save2(const std::string& name, const std::vector<T>& v, size_t rows, size_t cols)
+{
+    save_npy(name, npdt<T>(), {rows, cols}, v.data(), sizeof(T));
+}
+
+void copy_icdf(double* dst, InverseCdf& src)
+{
+    for (int i = 0; i <= CDF_RES; i++) dst[i] = src[i];
+}
+
+// cs_params from the reference's Param P (src/Param.h) as the run sees it.
+void fill_params(cs_params* c)
+{
+    memset(c, 0, sizeof(*c));
+    c->time_steps_per_day = P.TimeStepsPerDay;
+    c->model_time_step = P.ModelTimeStep;
+    c->false_positive_rate = P.FalsePositiveRate;
+    c->do_seasonality = P.DoSeasonality;
+    for (int i = 0; i < DAYS_PER_YEAR; i++) c->seasonality[i] = P.DoSeasonality ? P.Seasonality[i] : 1.0;
+    c->local_beta = P.LocalBeta;
+    c->household_trans = P.HouseholdTrans;
+    for (int i = 0; i < MAX_HOUSEHOLD_SIZE; i++) c->household_denom_lookup[i] = P.HouseholdDenomLookup[i];
+    c->do_households = P.DoHouseholds;
+    c->do_places = P.DoPlaces;
+    c->num_place_types = P.NumPlaceTypes;
+    c->hotel_place_type = P.HotelPlaceType;
+    c->place_type_no_air_num = P.PlaceTypeNoAirNum;
+    c->care_home_place_type = P.CareHomePlaceType;
+    c->do_symptoms = P.DoSymptoms;
+    c->do_severity = P.DoSeverity;
+    c->do_latent = P.DoLatent;
+    c->do_infectiousness_profile = P.DoInfectiousnessProfile;
+    c->do_real_sympt_withdrawal = P.DoRealSymptWithdrawal;
+    c->absenteeism_place_closure = P.AbsenteeismPlaceClosure;
+    c->no_infectiousness_sd_in_hh = P.NoInfectiousnessSDinHH;
+    c->do_death = P.DoDeath;
+    c->do_ad_units = P.DoAdUnits;
+    c->output_ad_unit_age = P.OutputAdUnitAge;
+    c->include_stepdown_to_death = P.IncludeStepDownToDeath;
+    c->do_digital_contact_tracing = P.DoDigitalContactTracing;
+    c->do_blanket_move_restr = P.DoBlanketMoveRestr;
+    c->interventions_active = 0;
+    for (int i = 0; i < MAX_NUM_PLACE_TYPES; i++) {
+        c->place_type_trans[i] = P.PlaceTypeTrans[i];
+        c->place_type_group_size_param1[i] = P.PlaceTypeGroupSizeParam1[i];
+        c->place_type_prop_between_group_links[i] = P.PlaceTypePropBetweenGroupLinks[i];
+        c->sympt_place_type_contact_rate[i] = P.SymptPlaceTypeContactRate[i];
+        c->sympt_place_type_withdrawal_prop[i] = P.SymptPlaceTypeWithdrawalProp[i];
+        c->holiday_effect[i] = P.HolidayEffect[i];
+    }
+    c->sympt_spatial_contact_rate = P.SymptSpatialContactRate;
+    for (int a = 0; a < NUM_AGE_GROUPS; a++) {
+        c->relative_spatial_contact[a] = P.RelativeSpatialContact[a];
+        c->relative_spatial_contact_susc[a] = P.RelativeSpatialContactSusc[a];
+        c->age_susceptibility[a] = P.AgeSusceptibility[a];
+        c->age_infectiousness[a] = P.AgeInfectiousness[a];
+        c->proportion_symptomatic[a] = P.ProportionSymptomatic[a];
+        for (int b = 0; b < NUM_AGE_GROUPS; b++) {
+            c->waifw[a][b] = P.WAIFW_Matrix[a][b];
+            c->waifw_spatial[a][b] = P.WAIFW_Matrix_SpatialOnly[a][b];
+        }
+        c->mean_by_age[CS_MEAN_MILD_RECOV][a] = P.Mean_MildToRecovery[a];
+        c->mean_by_age[CS_MEAN_ILI_RECOV][a] = P.Mean_ILIToRecovery[a];
+        c->mean_by_age[CS_MEAN_SARI_RECOV][a] = P.Mean_SARIToRecovery[a];
+        c->mean_by_age[CS_MEAN_CRIT_CRITRECOV][a] = P.Mean_CriticalToCritRecov[a];
+        c->mean_by_age[CS_MEAN_CRITRECOV_RECOV][a] = P.Mean_CritRecovToRecov[a];
+        c->mean_by_age[CS_MEAN_STEPDOWN_DEATH][a] = P.Mean_StepdownToDeath[a];
+        c->mean_by_age[CS_MEAN_ILI_SARI][a] = P.Mean_ILIToSARI[a];
+        c->mean_by_age[CS_MEAN_SARI_CRIT][a] = P.Mean_SARIToCritical[a];
+        c->mean_by_age[CS_MEAN_CRIT_DEATH][a] = P.Mean_CriticalToDeath[a];
+        c->mean_by_age[CS_MEAN_SARI_DEATH][a] = P.Mean_SARIToDeath[a];
+        c->mean_by_age[CS_MEAN_ILI_DEATH][a] = P.Mean_ILIToDeath[a];
+        c->prop_ili[a] = P.Prop_ILI_ByAge[a];
+        c->prop_sari[a] = P.Prop_SARI_ByAge[a];
+        c->prop_critical[a] = P.Prop_Critical_ByAge[a];
+        c->cfr_critical[a] = P.CFR_Critical_ByAge[a];
+        c->cfr_sari[a] = P.CFR_SARI_ByAge[a];
+        c->cfr_ili[a] = P.CFR_ILI_ByAge[a];
+    }
+    for (int i = 0; i < P.NumInterventionClasses && i < CS_NUM_NPI; i++)
+        for (int j = 0; j < P.NumInfectionSettings && j < CS_NUM_SETTINGS; j++)
+            c->efficacies[i][j] = P.Efficacies[i][j];
+    c->care_home_resident_household_scaling = P.CareHomeResidentHouseholdScaling;
+    c->care_home_resident_spatial_scaling = P.CareHomeResidentSpatialScaling;
+    c->care_home_worker_group_scaling = P.CareHomeWorkerGroupScaling;
+    c->care_home_resident_place_scaling = P.CareHomeResidentPlaceScaling;
+    c->care_home_rel_prob_hosp = P.CareHomeRelProbHosp;
+    c->treat_inf_drop = P.TreatInfDrop;
+    c->vacc_inf_drop = P.VaccInfDrop;
+    c->treat_susc_drop = P.TreatSuscDrop;
+    c->vacc_susc_drop = P.VaccSuscDrop;
+    c->vacc_susc_drop2 = P.VaccSuscDrop2;
+    c->treat_sympt_drop = P.TreatSympDrop;
+    c->vacc_sympt_drop = P.VaccSympDrop;
+    c->treat_death_drop = P.TreatDeathDrop;
+    c->us_vacc_time_to_efficacy = P.usVaccTimeToEfficacy;
+    c->us_vacc_time_efficacy_switch = P.usVaccTimeEfficacySwitch;
+    c->us_hquarantine_house_duration = P.usHQuarantineHouseDuration;
+    c->us_case_isolation_delay = P.usCaseIsolationDelay;
+    c->us_case_isolation_duration = P.usCaseIsolationDuration;
+    c->us_case_absenteeism_delay = P.usCaseAbsenteeismDelay;
+    c->us_case_absenteeism_duration = P.usCaseAbsenteeismDuration;
+    c->n_infectiousness = MAX_INFECTIOUS_STEPS;
+    for (int i = 0; i < MAX_INFECTIOUS_STEPS; i++) c->infectiousness[i] = P.infectiousness[i];
+    c->latent_period = P.LatentPeriod;
+    c->infectious_period = P.InfectiousPeriod;
+    c->latent_to_sympt_delay = P.LatentToSymptDelay;
+    c->sympt_infectiousness = P.SymptInfectiousness;
+    c->asympt_infectiousness = P.AsymptInfectiousness;
+    c->infectiousness_sd = P.InfectiousnessSD;
+    c->susceptibility_sd = P.SusceptibilitySD;
+    copy_icdf(c->icdf[CS_ICDF_LATENT], P.latent_icdf);
+    copy_icdf(c->icdf[CS_ICDF_INFECTIOUS], P.infectious_icdf);
+    copy_icdf(c->icdf[CS_ICDF_MILD_RECOV], P.MildToRecovery_icdf);
+    copy_icdf(c->icdf[CS_ICDF_ILI_RECOV], P.ILIToRecovery_icdf);
+    copy_icdf(c->icdf[CS_ICDF_SARI_RECOV], P.SARIToRecovery_icdf);
+    copy_icdf(c->icdf[CS_ICDF_CRIT_CRITRECOV], P.CriticalToCritRecov_icdf);
+    copy_icdf(c->icdf[CS_ICDF_CRITRECOV_RECOV], P.CritRecovToRecov_icdf);
+    copy_icdf(c->icdf[CS_ICDF_ILI_SARI], P.ILIToSARI_icdf);
+    copy_icdf(c->icdf[CS_ICDF_SARI_CRIT], P.SARIToCritical_icdf);
+    copy_icdf(c->icdf[CS_ICDF_ILI_DEATH], P.ILIToDeath_icdf);
+    copy_icdf(c->icdf[CS_ICDF_SARI_DEATH], P.SARIToDeath_icdf);
+    copy_icdf(c->icdf[CS_ICDF_STEPDOWN_DEATH], P.StepdownToDeath_icdf);
+    copy_icdf(c->icdf[CS_ICDF_CRIT_DEATH], P.CriticalToDeath_icdf);
+    c->cfr_critical_scale = P.CFR_Critical_Scale_Current;
+    c->cfr_sari_scale = P.CFR_SARI_Scale_Current;
+    c->cfr_ili_scale = P.CFR_ILI_Scale_Current;
+    c->control_prop_cases_id = P.ControlPropCasesId;
+    c->case_absent_child_age_cutoff = P.CaseAbsentChildAgeCutoff;
+    c->place_close_round_household = P.PlaceCloseRoundHousehold;
+    c->case_absent_child_prop_adult_carers = P.CaseAbsentChildPropAdultCarers;
+    c->location_initial_infection[0] = P.LocationInitialInfection[0][0];
+    c->location_initial_infection[1] = P.LocationInitialInfection[0][1];
+    c->kernel_size = P.KernelLookup.size_;
+    c->kernel_expansion = P.KernelLookup.expansion_factor_;
+    c->kernel_delta = P.KernelLookup.delta_;
+    c->kernel_type = P.MoveKernel.type_;
+    c->kernel_shape = P.MoveKernel.shape_;
+    c->kernel_scale = P.MoveKernel.scale_;
+    c->kernel_p3 = P.MoveKernel.p3_;
+    c->kernel_p4 = P.MoveKernel.p4_;
+    c->do_utm_coords = P.DoUTM_coords;
+    c->do_periodic_boundaries = P.DoPeriodicBoundaries;
+    c->nch = P.nch;
+    c->cell_width = P.in_cells_.width;
+    c->cell_height = P.in_cells_.height;
+    c->degrees_width = P.in_degrees_.width;
+    c->degrees_height = P.in_degrees_.height;
+    c->bbox_bottom_left_y = P.SpatialBoundingBox.bottom_left().y;
+    for (int i = 0; i <= 360; i++) { c->sinx[i] = P.sinx[i]; c->cosx[i] = P.cosx[i]; }
+    for (int i = 0; i <= 1000; i++) c->asin2sqx[i] = P.asin2sqx[i];
+    c->do_holidays = P.DoHolidays;
+    c->num_holidays = std::min(P.NumHolidays, CS_MAX_HOLIDAYS);
+    for (int i = 0; i < c->num_holidays; i++) {
+        c->holiday_start[i] = P.HolidayStartTime[i];
+        c->holiday_duration[i] = P.HolidayDuration[i];
+    }
+    c->holidays_start_day_simtime = P.HolidaysStartDay_SimTime;
+}
+
+void dump_params(const std::string& name)
+{
+    cs_params c;
+    fill_params(&c);
+    std::string path = g_dir + "/" + name;
+    FILE* f = fopen(path.c_str(), "wb");
+    fwrite(&c, sizeof(c), 1, f);
+    fclose(f);
+}
+
+void dump_population()
+{
+    const int N = P.PopSize;
+    std::vector<int32_t> hh(N), mcell(N), pcell(N), plinks((size_t)N * MAX_NUM_PLACE_TYPES);
+    std::vector<uint16_t> pglinks((size_t)N * MAX_NUM_PLACE_TYPES);
+    std::vector<uint8_t> age(N), flags(N);
+    for (int i = 0; i < N; i++) {
+        const Person& a = Hosts[i];
+        hh[i] = a.hh; mcell[i] = a.mcell; pcell[i] = a.pcell; age[i] = a.age;
+        flags[i] = (uint8_t)((a.care_home_resident ? 1 : 0) | (a.keyworker ? 2 : 0) |
+                             (a.esocdist_comply ? 4 : 0) | (a.digitalContactTracingUser ? 8 : 0));
+        for (int t = 0; t < MAX_NUM_PLACE_TYPES; t++) {
+            plinks[(size_t)i * MAX_NUM_PLACE_TYPES + t] = (t < P.NumPlaceTypes) ? a.PlaceLinks[t] : -1;
+            pglinks[(size_t)i * MAX_NUM_PLACE_TYPES + t] = (t < P.NumPlaceTypes) ? a.PlaceGroupLinks[t] : 0;
+        }
+    }
+    save("p_hh", hh); save("p_mcell", mcell); save("p_pcell", pcell);
+    save("p_age", age); save("p_flags", flags);
+    save2("p_place_links", plinks, N, MAX_NUM_PLACE_TYPES);
+    save2("p_place_group_links", pglinks, N, MAX_NUM_PLACE_TYPES);
+
+    std::vector<int32_t> hfirst(P.NumHouseholds);
+    std::vector<uint16_t> hnh(P.NumHouseholds), hnhr(P.NumHouseholds);
+    std::vector<float> hloc((size_t)P.NumHouseholds * 2);
+    for (int i = 0; i < P.NumHouseholds; i++) {
+        hfirst[i] = Households[i].FirstPerson; hnh[i] = Households[i].nh; hnhr[i] = Households[i].nhr;
+        hloc[2 * i] = Households[i].loc.x; hloc[2 * i + 1] = Households[i].loc.y;
+    }
+    save("h_first", hfirst); save("h_nh", hnh); save("h_nhr", hnhr); save2("h_loc", hloc, P.NumHouseholds, 2);
+
+    const int NC = P.NumPopulatedCells;
+    std::vector<int32_t> cidx(NC), cn(NC), cstart(NC + 1), cmem;
+    cstart[0] = 0;
+    for (int j = 0; j < NC; j++) {
+        Cell* c = CellLookup[j];
+        cidx[j] = (int32_t)(c - Cells); cn[j] = c->n;
+        for (int k = 0; k < c->n; k++) cmem.push_back(c->members[k]);
+        cstart[j + 1] = cstart[j] + c->n;
+    }
+    save("c_index", cidx); save("c_n", cn); save("c_member_start", cstart); save("c_members", cmem);
+
+    int nm = P.NMCL * P.NMCL * P.NumCells;
+    std::vector<int32_t> madunit(nm);
+    for (int i = 0; i < nm; i++) madunit[i] = Mcells[i].adunit;
+    save("m_adunit", madunit);
+
+    for (int t = 0; t < P.NumPlaceTypes; t++) {
+        std::vector<int32_t> pn(P.Nplace[t]), pm(P.Nplace[t]), pms(P.Nplace[t] + 1), mem, pgs(P.Nplace[t] + 1), gst, gsz;
+        pms[0] = 0; pgs[0] = 0;
+        for (int k = 0; k < P.Nplace[t]; k++) {
+            const Place& pl = Places[t][k];
+            pn[k] = pl.n; pm[k] = pl.mcell;
+            for (int m = 0; m < pl.n; m++) mem.push_back(pl.members[m]);
+            pms[k + 1] = pms[k] + pl.n;
+            for (int g = 0; g < pl.ng; g++) { gst.push_back(pl.group_start[g]); gsz.push_back(pl.group_size[g]); }
+            pgs[k + 1] = pgs[k] + pl.ng;
+        }
+        std::string s = std::to_string(t);
+        save("pl" + s + "_n", pn); save("pl" + s + "_mcell", pm); save("pl" + s + "_member_start", pms);
+        save("pl" + s + "_members", mem); save("pl" + s + "_groups_start", pgs);
+        save("pl" + s + "_group_start", gst); save("pl" + s + "_group_size", gsz);
+    }
+    std::vector<int32_t> meta = {P.PopSize, P.NumHouseholds, NC, P.NumCells, nm, P.NumAdunits, P.NumPlaceTypes, P.NMCL};
+    save("meta", meta);
+
+    // kernel tables
+    std::vector<double> lk(P.KernelLookup.lookup_), hr(P.KernelLookup.hi_res_);
+    save("kernel_lookup", lk); save("kernel_hi_res", hr);
+}
+
+void dump_state(const std::string& prefix, unsigned short ts)
+{
+    const int N = P.PopSize;
+    std::vector<int8_t> inf(N);
+    std::vector<uint8_t> sevc(N), sevf(N), bits(N), qc(N), dct(N);
+    std::vector<int32_t> listpos(N), infector(N);
+    std::vector<int16_t> itype(N);
+    std::vector<float> infness(N), susc(N), pcare(N);
+    std::vector<uint16_t> t_inf(N), t_lat(N), t_rod(N), t_sari(N), t_crit(N), t_step(N), t_det(N), t_as(N), t_ae(N),
+        t_ts(N), t_te(N), t_vs(N), t_is(N), t_qs(N);
+    for (int i = 0; i < N; i++) {
+        const Person& a = Hosts[i];
+        // the InfStat code is private; read it through the predicates (src/Person.cpp)
+        int code;
+        if (a.is_susceptible()) code = 0;
+        else if (a.is_latent()) code = 1;
+        else if (a.is_infectious_almost_symptomatic()) code = -1;
+        else if (a.is_infectious_asymptomatic_not_case()) code = 2;
+        else if (a.is_case()) code = -2;
+        else if (a.is_immune_at_start()) code = 4;
+        else if (a.is_dead_was_asymp()) code = 5;
+        else if (a.is_dead_was_symp()) code = -5;
+        else if (a.is_recovered_symp()) code = -3;
+        else code = 3;
+        inf[i] = (int8_t)code;
+        sevc[i] = (uint8_t)a.Severity_Current; sevf[i] = (uint8_t)a.Severity_Final;
+        bits[i] = (uint8_t)((a.to_die ? 1 : 0) | (a.detected ? 2 : 0));
+        listpos[i] = a.listpos; infector[i] = a.infector; itype[i] = a.infect_type;
+        infness[i] = a.infectiousness; susc[i] = a.susc; pcare[i] = a.ProbCare;
+        t_inf[i] = a.infection_time; t_lat[i] = a.latent_time; t_rod[i] = a.recovery_or_death_time;
+        t_sari[i] = a.SARI_time; t_crit[i] = a.Critical_time; t_step[i] = a.Stepdown_time; t_det[i] = a.detected_time;
+        t_as[i] = a.absent_start_time; t_ae[i] = a.absent_stop_time;
+        t_ts[i] = a.treat_start_time; t_te[i] = a.treat_stop_time; t_vs[i] = a.vacc_start_time;
+        t_is[i] = a.isolation_start_time;
+        t_qs[i] = HostsQuarantine[i].start_time; qc[i] = HostsQuarantine[i].comply;
+        dct[i] = a.digitalContactTraced;
+    }
+    const std::string p = prefix;
+    save(p + "inf", inf); save(p + "sev_cur", sevc); save(p + "sev_final", sevf); save(p + "bits", bits);
+    save(p + "listpos", listpos); save(p + "infector", infector); save(p + "infect_type", itype);
+    save(p + "infectiousness", infness); save(p + "susc", susc); save(p + "prob_care", pcare);
+    save(p + "t_infection", t_inf); save(p + "t_latent", t_lat); save(p + "t_recovery", t_rod);
+    save(p + "t_sari", t_sari); save(p + "t_critical", t_crit); save(p + "t_stepdown", t_step);
+    save(p + "t_detected", t_det); save(p + "t_absent_start", t_as); save(p + "t_absent_stop", t_ae);
+    save(p + "t_treat_start", t_ts); save(p + "t_treat_stop", t_te); save(p + "t_vacc_start", t_vs);
+    save(p + "t_isolation_start", t_is); save(p + "t_quar_start", t_qs); save(p + "quar_comply", qc);
+    save(p + "dct_traced", dct);
+
+    const int NC = P.NumPopulatedCells;
+    std::vector<int32_t> S(NC), L(NC), I(NC), R(NC), D(NC), cumTC(NC), S0(NC), perm;
+    std::vector<float> totp(NC), maxt((size_t)NC * NC), cumt((size_t)NC * NC);
+    std::vector<int32_t> invcdf((size_t)NC * 1025);
+    for (int j = 0; j < NC; j++) {
+        Cell* c = CellLookup[j];
+        S[j] = c->S; L[j] = c->L; I[j] = c->I; R[j] = c->R; D[j] = c->D; cumTC[j] = c->cumTC; S0[j] = c->S0;
+        totp[j] = c->tot_prob;
+        for (int k = 0; k < c->n; k++) perm.push_back(c->susceptible[k]);
+        for (int m = 0; m < NC; m++) { maxt[(size_t)j * NC + m] = c->max_trans[m]; cumt[(size_t)j * NC + m] = c->cum_trans[m]; }
+        for (int k = 0; k <= 1024; k++) invcdf[(size_t)j * 1025 + k] = c->InvCDF[k];
+    }
+    save(p + "cell_S", S); save(p + "cell_L", L); save(p + "cell_I", I); save(p + "cell_R", R); save(p + "cell_D", D);
+    save(p + "cell_cumTC", cumTC); save(p + "cell_S0", S0); save(p + "cell_perm", perm);
+    save(p + "tot_prob", totp); save2(p + "max_trans", maxt, NC, NC); save2(p + "cum_trans", cumt, NC, NC);
+    save2(p + "invcdf", invcdf, NC, 1025);
+
+    for (int t = 0; t < P.NumPlaceTypes; t++) {
+        std::vector<uint16_t> cs(P.Nplace[t]), ce(P.Nplace[t]);
+        for (int k = 0; k < P.Nplace[t]; k++) { cs[k] = Places[t][k].close_start_time; ce[k] = Places[t][k].close_end_time; }
+        save(p + "pl" + std::to_string(t) + "_close_start", cs);
+        save(p + "pl" + std::to_string(t) + "_close_end", ce);
+    }
+    int nm = P.NMCL * P.NMCL * P.NumCells;
+    std::vector<uint8_t> sd(nm), mr(nm);
+    for (int i = 0; i < nm; i++) { sd[i] = (uint8_t)Mcells[i].socdist; mr[i] = (uint8_t)Mcells[i].moverest; }
+    save(p + "m_socdist", sd); save(p + "m_moverest", mr);
+    std::vector<int32_t> tsv = {(int32_t)ts};
+    save(p + "timestep", tsv);
+    dump_params(p + "params.bin");
+}
+
+// 64-bit LCG so the golden sample choices do not touch the reference RNG
+uint64_t g_lcg = 0x853c49e6748fea9bULL;
+double lcg_u() { g_lcg = g_lcg * 6364136223846793005ULL + 1442695040888963407ULL; return (double)(g_lcg >> 11) * (1.0 / 9007199254740992.0); }
+int lcg_i(int n) { return (int)(lcg_u() * n); }
+
+void dump_golden()
+{
+    // kernel lookup num(r2): uniform over the table, the hi-res region, and the boundaries
+    const CovidSim::TBD1::KernelLookup& K = P.KernelLookup;
+    double rmax = K.delta_ * K.size_;
+    std::vector<double> r2;
+    for (int i = 0; i < 6000; i++) r2.push_back(lcg_u() * rmax);
+    for (int i = 0; i < 3000; i++) r2.push_back(lcg_u() * rmax / K.expansion_factor_);
+    r2.push_back(0.0); r2.push_back(rmax);
+    double hb = K.delta_ * K.size_ / K.expansion_factor_;
+    for (int d = -20; d <= 20; d++) r2.push_back(hb * (1.0 + d * 1e-13));
+    for (int i = 0; i < 500; i++) r2.push_back(K.delta_ * (double)lcg_i(K.size_));
+    std::vector<double> kv(r2.size());
+    for (size_t i = 0; i < r2.size(); i++) kv[i] = K.num(r2[i]);
+    save("g_kernel_r2", r2); save("g_kernel_num", kv);
+
+    // dist2 between random person pairs (src/Dist.cpp:49-60)
+    std::vector<int32_t> da, db;
+    std::vector<double> dv;
+    for (int i = 0; i < 4000; i++) {
+        int a = lcg_i(P.PopSize), b = (i % 7 == 0) ? a : lcg_i(P.PopSize);
+        da.push_back(a); db.push_back(b); dv.push_back(dist2(Hosts + a, Hosts + b));
+    }
+    save("g_dist_a", da); save("g_dist_b", db); save("g_dist2", dv);
+
+    // InverseCdf::choose with its uniform captured from thread 0's stream
+    InverseCdf* tabs[CS_NUM_ICDF] = {&P.latent_icdf, &P.infectious_icdf, &P.MildToRecovery_icdf, &P.ILIToRecovery_icdf,
+        &P.SARIToRecovery_icdf, &P.CriticalToCritRecov_icdf, &P.CritRecovToRecov_icdf, &P.ILIToSARI_icdf,
+        &P.SARIToCritical_icdf, &P.ILIToDeath_icdf, &P.SARIToDeath_icdf, &P.StepdownToDeath_icdf, &P.CriticalToDeath_icdf};
+    std::vector<int32_t> ct;
+    std::vector<double> cu, cm;
+    std::vector<uint16_t> cvv;
+    int32_t s1 = Xcg1[0], s2 = Xcg2[0];
+    for (int tb = 0; tb < CS_NUM_ICDF; tb++)
+        for (int i = 0; i < 400; i++) {
+            double mean = 0.5 + 20.0 * lcg_u();
+            int32_t a1 = Xcg1[0], a2 = Xcg2[0];
+            double u = ranf_mt(0);
+            Xcg1[0] = a1; Xcg2[0] = a2;
+            unsigned short v = tabs[tb]->choose(mean, 0, P.TimeStepsPerDay);
+            ct.push_back(tb); cu.push_back(u); cm.push_back(mean); cvv.push_back(v);
+        }
+    Xcg1[0] = s1; Xcg2[0] = s2;
+    save("g_choose_table", ct); save("g_choose_u", cu); save("g_choose_mean", cm); save("g_choose_value", cvv);
+
+    // hazards over synthetic states.  The harness perturbs the intervention
+    // parameters so that every factor is exercised, gives distinct persons
+    // synthetic states, evaluates the eight reference hazards, writes the
+    // perturbed state (prefix hz_) and restores everything.
+    double** effsave = P.Efficacies;
+    static double effbuf[CS_NUM_NPI][CS_NUM_SETTINGS];
+    static double* effrows[CS_NUM_NPI];
+    for (int i = 0; i < CS_NUM_NPI; i++) {
+        effrows[i] = effbuf[i];
+        for (int j = 0; j < CS_NUM_SETTINGS; j++) effbuf[i][j] = 0.2 + 0.75 * lcg_u();
+    }
+    const double sv_tid = P.TreatInfDrop, sv_vid = P.VaccInfDrop, sv_tsd = P.TreatSuscDrop, sv_vsd = P.VaccSuscDrop,
+        sv_vsd2 = P.VaccSuscDrop2, sv_chh = P.CareHomeResidentHouseholdScaling, sv_chs = P.CareHomeResidentSpatialScaling;
+    const unsigned short sv_u1 = P.usVaccTimeToEfficacy, sv_u2 = P.usVaccTimeEfficacySwitch,
+        sv_u3 = P.usHQuarantineHouseDuration, sv_u4 = P.usCaseIsolationDelay, sv_u5 = P.usCaseIsolationDuration;
+    double rsc[NUM_AGE_GROUPS];
+    for (int a = 0; a < NUM_AGE_GROUPS; a++) rsc[a] = P.RelativeSpatialContactSusc[a];
+    P.Efficacies = effrows;
+    P.TreatInfDrop = 0.61; P.VaccInfDrop = 0.83; P.TreatSuscDrop = 0.47; P.VaccSuscDrop = 0.39; P.VaccSuscDrop2 = 0.71;
+    P.CareHomeResidentHouseholdScaling = 0.9; P.CareHomeResidentSpatialScaling = 0.55;
+    P.usVaccTimeToEfficacy = 8; P.usVaccTimeEfficacySwitch = 120; P.usHQuarantineHouseDuration = 40;
+    P.usCaseIsolationDelay = 2; P.usCaseIsolationDuration = 28;
+    for (int a = 0; a < NUM_AGE_GROUPS; a++) P.RelativeSpatialContactSusc[a] = 0.5 + lcg_u();
+
+    const int NG = std::min(6000, P.PopSize / 3);
+    std::vector<int> order(P.PopSize);
+    for (int i = 0; i < P.PopSize; i++) order[i] = i;
+    for (int i = P.PopSize - 1; i > 0; i--) std::swap(order[i], order[lcg_i(i + 1)]);
+    std::vector<int32_t> hp(NG), ho(NG), hpt(NG);
+    std::vector<uint16_t> hts(NG);
+    std::vector<double> out((size_t)NG * 8);
+    std::vector<Person> saveH(P.PopSize);
+    std::vector<PersonQuarantine> saveQ(HostsQuarantine);
+    for (int i = 0; i < P.PopSize; i++) saveH[i] = Hosts[i];
+    int nm = P.NMCL * P.NMCL * P.NumCells;
+    std::vector<TreatStat> saveSD(nm);
+    for (int i = 0; i < nm; i++) { saveSD[i] = Mcells[i].socdist; Mcells[i].socdist = (lcg_u() < 0.4) ? TreatStat::Treated : TreatStat::Untreated; }
+    for (int g = 0; g < NG; g++) {
+        int a = order[2 * g], b = order[2 * g + 1];
+        int pt = lcg_i(P.NumPlaceTypes);
+        unsigned short ts = (unsigned short)(60 + lcg_i(200));
+        for (int who : {a, b}) {
+            Person& h = Hosts[who];
+            h.latent_time = (unsigned short)(ts - 1 - lcg_i(50));
+            h.infectiousness = (float)((lcg_u() < 0.5 ? -1 : 1) * (0.1 + 2 * lcg_u()));
+            h.susc = (float)(0.2 + lcg_u());
+            h.treat_start_time = (unsigned short)(lcg_u() < 0.3 ? ts - lcg_i(10) : CS_UNSET_TIME);
+            h.treat_stop_time = (unsigned short)(lcg_u() < 0.5 ? ts + 1 + lcg_i(10) : 0);
+            h.vacc_start_time = (unsigned short)(lcg_u() < 0.3 ? lcg_i(ts) : CS_UNSET_TIME);
+            h.isolation_start_time = (unsigned short)(lcg_u() < 0.3 ? ts - lcg_i(30) : CS_UNSET_TIME);
+            h.digitalContactTraced = lcg_u() < 0.2 ? 1 : 0;
+            h.care_home_resident = lcg_u() < 0.2 ? 1 : 0;
+            h.esocdist_comply = lcg_u() < 0.5 ? 1 : 0;
+            if (lcg_u() < 0.3) h.set_case(); else h.set_infectious_asymptomatic_not_case();
+            HostsQuarantine[who].comply = (uint8_t)lcg_i(3);
+            HostsQuarantine[who].start_time = (uint16_t)(lcg_u() < 0.5 ? ts - lcg_i(45) : CS_UNSET_TIME);
+        }
+        hp[g] = a; ho[g] = b; hpt[g] = pt; hts[g] = ts;
+        double* o = &out[(size_t)g * 8];
+        o[0] = CalcHouseInf(a, ts);
+        o[1] = CalcPlaceInf(a, pt, ts);
+        o[2] = CalcSpatialInf(a, ts);
+        o[3] = CalcPersonInf(a, ts);
+        o[4] = CalcHouseSusc(a, ts, b);
+        o[5] = CalcPlaceSusc(a, pt, ts);
+        o[6] = CalcSpatialSusc(a, ts);
+        o[7] = CalcPersonSusc(a, ts, b);
+    }
+    save("g_hz_person", hp); save("g_hz_other", ho); save("g_hz_pt", hpt); save("g_hz_ts", hts);
+    save2("g_hz_out", out, NG, 8);
+    {
+        std::vector<uint8_t> flags(P.PopSize);
+        for (int i = 0; i < P.PopSize; i++) {
+            const Person& a = Hosts[i];
+            flags[i] = (uint8_t)((a.care_home_resident ? 1 : 0) | (a.keyworker ? 2 : 0) |
+                                 (a.esocdist_comply ? 4 : 0) | (a.digitalContactTracingUser ? 8 : 0));
+        }
+        save("hz_p_flags", flags);
+    }
+    dump_state("hz_", 0);
+    // restore
+    for (int i = 0; i < P.PopSize; i++) Hosts[i] = saveH[i];
+    HostsQuarantine = saveQ;
+    for (int i = 0; i < nm; i++) Mcells[i].socdist = saveSD[i];
+    for (int a = 0; a < NUM_AGE_GROUPS; a++) P.RelativeSpatialContactSusc[a] = rsc[a];
+    P.Efficacies = effsave;
+    P.TreatInfDrop = sv_tid; P.VaccInfDrop = sv_vid; P.TreatSuscDrop = sv_tsd; P.VaccSuscDrop = sv_vsd;
+    P.VaccSuscDrop2 = sv_vsd2; P.CareHomeResidentHouseholdScaling = sv_chh; P.CareHomeResidentSpatialScaling = sv_chs;
+    P.usVaccTimeToEfficacy = sv_u1; P.usVaccTimeEfficacySwitch = sv_u2; P.usHQuarantineHouseDuration = sv_u3;
+    P.usCaseIsolationDelay = sv_u4; P.usCaseIsolationDuration = sv_u5;
+}
+
+void init_once()
+{
+    if (g_started) return;
+    g_started = true;
+    const char* d = getenv("CS_DUMP_DIR");
+    if (d) g_dir = d;
+    const char* s = getenv("CS_DUMP_STEPS");
+    std::string steps = s ? s : "0";
+    std::stringstream ss(steps);
+    std::string tok;
+    while (std::getline(ss, tok, ',')) if (!tok.empty()) g_steps.push_back(atoi(tok.c_str()));
+    g_golden = getenv("CS_DUMP_GOLDEN") && atoi(getenv("CS_DUMP_GOLDEN"));
+    const char* se = getenv("CS_SERIES");
+    if (se) g_series = fopen(se, "ab");
+}
+
+int g_run_of_call = -1;
+
+} // namespace
+
+void InfectSweep(double t, int run)
+{
+    init_once();
+    g_run_of_call = run;
+    unsigned short ts = (unsigned short)(P.TimeStepsPerDay * t);
+    if (!g_dir.empty() && run == 0) {
+        if (ts == 0 && g_last_run != run) {
+            dump_population();
+            dump_params("params.bin");
+            if (g_golden) dump_golden();
+        }
+        for (int s : g_steps)
+            if (s == ts) dump_state("s" + std::to_string(ts) + "_", ts);
+    }
+    g_last_run = run;
+    InfectSweep_cpu(t, run);
+}
+
+void IncubRecoverySweep(double t)
+{
+    IncubRecoverySweep_cpu(t);
+    if (g_series) {
+        int32_t rec[10];
+        int S = 0, L = 0, I = 0, R = 0, D = 0;
+        for (int j = 0; j < P.NumPopulatedCells; j++) {
+            S += CellLookup[j]->S; L += CellLookup[j]->L; I += CellLookup[j]->I;
+            R += CellLookup[j]->R; D += CellLookup[j]->D;
+        }
+        int cumI = 0, cumC = 0;
+        for (int k = 0; k < P.NumThreads; k++) { cumI += StateT[k].cumI; cumC += StateT[k].cumC; }
+        rec[0] = g_run_of_call; rec[1] = (int32_t)(unsigned short)(P.TimeStepsPerDay * t);
+        rec[2] = S; rec[3] = L; rec[4] = I; rec[5] = R; rec[6] = D; rec[7] = cumI; rec[8] = cumC; rec[9] = D;
+        fwrite(rec, sizeof(rec), 1, g_series);
+        fflush(g_series);
+    }
+}

@@ -1,0 +1,67 @@
+"""Generate the committed parity fixtures from the reference itself.
+
+Runs the golden harness (oracle/_ref/golden_harness: the unmodified
+CovidSim objects with the two sweeps observed, built by oracle/Makefile.ref
+from /root/reference) on the 20k-person fixture region and packs:
+  mini_fixture.npz      population, reference state at TimeStepNow 0 and
+                        160, cell tables, and the golden vectors (kernel
+                        num(), dist2, InverseCdf::choose, hazards)
+  mini_params.bin       cs_params of that run (include/covidsim_hip.h)
+  mini_hz_params.bin    cs_params with the perturbed intervention
+                        parameters the hazard vectors were evaluated under
+Everything here is data: inputs and the reference's outputs.
+"""
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+HARNESS = os.path.join(ROOT, "oracle", "_ref", "golden_harness")
+SEEDS = ["98798150", "729101", "17389101", "4797132"]
+
+
+def run_harness(inputs, out, steps="0", golden=True, threads=1, extra=(), series=None, nr=None):
+    env = dict(os.environ, CS_DUMP_DIR=out, CS_DUMP_STEPS=steps, CS_DUMP_GOLDEN="1" if golden else "0")
+    if series:
+        env["CS_SERIES"] = series
+    cmd = [HARNESS, "/c:%d" % threads,
+           "/PP:" + os.path.join(inputs, "pre-params.txt"),
+           "/P:" + os.path.join(inputs, "input-noint-params.txt"),
+           "/O:" + os.path.join(out, "out"),
+           "/D:" + os.path.join(inputs, "density.txt"),
+           "/A:" + os.path.join(inputs, "admin-params.txt")]
+    if nr:
+        cmd.append("/NR:%d" % nr)
+    cmd += list(extra) + SEEDS
+    r = subprocess.run(cmd, env=env, cwd=out, capture_output=True, text=True)
+    if r.returncode:
+        sys.stderr.write(r.stderr[-3000:])
+        raise SystemExit("harness failed")
+    return r
+
+
+def main():
+    inputs = os.path.join(ROOT, "covid-sim_amd", "inputs", "mini")
+    tmp = tempfile.mkdtemp()
+    try:
+        run_harness(inputs, tmp, steps="0,160")
+        arrays = {}
+        for fn in sorted(os.listdir(tmp)):
+            if not fn.endswith(".npy") or fn.startswith("kernel_"):
+                continue
+            arrays[fn[:-4]] = np.load(os.path.join(tmp, fn))
+        np.savez_compressed(os.path.join(HERE, "mini_fixture.npz"), **arrays)
+        shutil.copy(os.path.join(tmp, "params.bin"), os.path.join(HERE, "mini_params.bin"))
+        shutil.copy(os.path.join(tmp, "hz_params.bin"), os.path.join(HERE, "mini_hz_params.bin"))
+        print("wrote", len(arrays), "arrays")
+    finally:
+        shutil.rmtree(tmp)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,216 @@
+"""Benchmark: simulated person-days/s of a batched ensemble on one MI355X.
+
+Workload (the ensemble the headline metric is quoted on, at the size the
+target names): 64 realisations of one 5,000,000-person synthetic region
+advance together, 300 days at 4 timesteps/day, from a reset with 10 random
+seed infections each.  A "step" is one complete ensemble run (reset + 1200
+timesteps of InfectSweep + IncubRecoverySweep + the S-drop UpdateProbs
+check); `value` = realisations x persons x days / wall seconds of the timed
+steps, with the population and tables already resident in HBM.
+
+Parameters are the 200k-person region's derived parameter set
+(covid-sim_amd/inputs/c1/params.bin); the population comes from
+covid-sim_amd/popgen.py at the same density and the region's geometry.
+
+The reported roofline is for the kernel with the largest device time,
+from HIP events recorded around its launches on the library's stream during
+the timed steps; its algorithmic bytes per launch come from the event
+counts the library keeps and the per-event byte costs below.
+
+The CPU baseline runs the reference itself (oracle/_ref/covidsim_ref, built
+from the reference sources) on its 200k-person workload with all host
+threads, for a bounded number of realisations.
+"""
+import argparse
+import json
+import os
+import re
+import subprocess
+import sys
+import tempfile
+import threading
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+# minimum bytes touched per event (the per-event costs of the step's
+# algorithmic-bytes formula); fixed, never tuned to the measurement
+BYTES = {"n_inf": 64, "n_hh_inf": 16, "n_hh_member": 40, "n_place_link": 32, "n_place_cand": 44,
+         "n_sp_draw": 48, "n_sp_susc": 40, "n_new": 40, "n_active": 16, "n_transition": 30}
+KERNEL_UNITS = {
+    "infect_cells": ["n_inf", "n_hh_inf", "n_hh_member", "n_place_link", "n_place_cand"],
+    "spatial_draws": ["n_sp_draw", "n_sp_susc"],
+    "resolve": ["n_new"],
+    "latent_scan": ["n_active"],
+    "infectious_scan": ["n_transition"],
+}
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(max_seconds=25.0):
+    """Time the reference's day loop on its 200k-person workload."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "covidsim_ref")
+    inp = os.path.join(ROOT, "covid-sim_amd", "inputs", "c1")
+    if not os.path.exists(exe):
+        return None
+    threads = len(os.sched_getaffinity(0))
+    threads = max(1, min(threads, 96))  # MAX_NUM_THREADS (src/Constants.h)
+    n_people, days = 200000, 200
+
+    def run(nr):
+        tmp = tempfile.mkdtemp()
+        cmd = [exe, "/c:%d" % threads, "/NR:%d" % nr,
+               "/PP:" + os.path.join(inp, "pre-params.txt"), "/P:" + os.path.join(inp, "input-noint-params.txt"),
+               "/O:" + os.path.join(tmp, "out"), "/D:" + os.path.join(inp, "density.txt"),
+               "/A:" + os.path.join(inp, "admin-params.txt"), "98798150", "729101", "17389101", "4797132"]
+        p = subprocess.Popen(cmd, stderr=subprocess.PIPE, stdout=subprocess.DEVNULL, cwd=tmp)
+        marks = []  # wall time of every realisation's t=0 line and End of run
+        buf = b""
+        while True:
+            chunk = p.stderr.read1(65536) if hasattr(p.stderr, "read1") else p.stderr.read(4096)
+            if not chunk:
+                break
+            now = time.perf_counter()
+            buf += chunk
+            # the progress lines use \r; scan complete markers only
+            while True:
+                i0 = buf.find(b"t=0 ")
+                i1 = buf.find(b"End of run")
+                cands = [(i, k) for i, k in ((i0, "start"), (i1, "end")) if i >= 0]
+                if not cands:
+                    buf = buf[-32:]
+                    break
+                i, k = min(cands)
+                marks.append((k, now))
+                buf = buf[i + 4:]
+        p.wait()
+        loop = 0.0
+        start = None
+        for k, tt in marks:
+            if k == "start" and start is None:
+                start = tt
+            elif k == "end" and start is not None:
+                loop += tt - start
+                start = None
+        return loop, p.returncode
+
+    loop1, rc = run(1)
+    if rc != 0 or loop1 <= 0:
+        return None
+    nr = int(max(1, min(40, (max_seconds - loop1) / max(loop1, 1e-3))))
+    loop, rc = run(nr)
+    if rc != 0 or loop <= 0:
+        nr, loop = 1, loop1
+    return {"value": nr * n_people * days / loop, "unit": "person-days/s", "cores": threads, "kind": "reference",
+            "sample": "reference covidsim_ref, its 200k-person 200-day workload (4 steps/day, no interventions), "
+                      "%d realisation(s), day-loop wall %.2f s from stderr markers, /c:%d" % (nr, loop, threads)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--people", type=int, default=5_000_000)
+    ap.add_argument("--realisations", type=int, default=64)
+    ap.add_argument("--days", type=int, default=300)
+    ap.add_argument("--warmup-days", type=int, default=20, help="days simulated by each warmup step")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU: this build has no multi-GPU path (each realisation batch "
+              "would shard by realisation; see the design notes)", file=sys.stderr)
+        sys.exit(2)
+    import importlib
+    csa = importlib.import_module("covid-sim_amd")
+    eng_mod = importlib.import_module("covid-sim_amd.engine")
+    popgen = importlib.import_module("covid-sim_amd.popgen")
+
+    t0 = time.perf_counter()
+    arrays, geom = popgen.generate(args.people, seed=2024)
+    pop = csa.population.Population(arrays)
+    params = csa.population.load_params(os.path.join(ROOT, "covid-sim_amd", "inputs", "c1", "params.bin"))
+    popgen.apply_geometry(params, geom)
+    log("population %d persons, %d cells (%.1f s)" % (pop.n_people, pop.n_cells, time.perf_counter() - t0))
+    eng = eng_mod.Engine(0)
+    eng.set_params(params)
+    eng.set_population(pop)
+    seeds = np.arange(1, args.realisations + 1, dtype=np.uint64) * np.uint64(0x9E3779B97F4A7C15)
+    eng.alloc(seeds)
+    tspd = int(round(params.time_steps_per_day))
+    steps = args.days * tspd
+    log("set-up done (%.1f s); warmup %d x %d days" % (time.perf_counter() - t0, args.warmup, args.warmup_days))
+    for _ in range(args.warmup):
+        eng.reset(10)
+        eng.run_steps(0.0, args.warmup_days * tspd, tspd)
+    eng.sync()
+    try:
+        import torch
+        if torch.cuda.is_available():
+            torch.cuda.synchronize()
+    except Exception:
+        pass
+    eng.work_counts(reset=True)
+    eng.set_profiling(True)
+    times = []
+    for k in range(args.steps):
+        eng.sync()
+        ta = time.perf_counter()
+        eng.reset(10)
+        eng.run_steps(0.0, steps, tspd)
+        eng.sync()
+        times.append(time.perf_counter() - ta)
+        log("step %d: %.2f s" % (k, times[-1]))
+    prof = eng.profile()
+    wc = eng.work_counts()
+    ts = eng.timeseries(args.days)
+    total = float(sum(times))
+    person_days = float(args.realisations) * args.people * args.days * args.steps
+    value = person_days / total
+    # dominant kernel and its roofline
+    dom = max((k for k in prof if k in KERNEL_UNITS), key=lambda k: prof[k][0])
+    ms, launches = prof[dom]
+    alg_bytes = sum(BYTES[u] * wc[u] for u in KERNEL_UNITS[dom]) / max(launches, 1)
+    achieved = alg_bytes / (ms / max(launches, 1) * 1e-3) / 1e9
+    attack = float(np.mean((ts[:, -1, 3] + ts[:, -1, 4]) / args.people)) if ts.size else 0.0
+    log("kernel ms: " + ", ".join("%s %.0f/%d" % (k, v[0], v[1]) for k, v in prof.items()))
+    log("work: " + json.dumps(wc))
+    log("mean attack rate %.3f" % attack)
+    total_alg = sum(BYTES[u] * wc[u] for u in BYTES)
+    base = None if args.no_cpu_baseline else cpu_baseline()
+    out = {
+        "metric": "person-days/s, 64-run ensemble",
+        "value": value,
+        "unit": "person-days/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": total / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "batched ensemble: %d realisations x %d-person synthetic region, %d days, 4 steps/day, "
+                               "10 seeds, no interventions" % (args.realisations, args.people, args.days),
+                   "realisations": args.realisations, "persons": args.people, "days": args.days, "cells": pop.n_cells,
+                   "mean_attack_rate": round(attack, 4),
+                   "all_kernel_algorithmic_gbs": round(total_alg / total / 1e9, 2)},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                     "bytes_per_launch": round(alg_bytes), "ms_per_launch": round(ms / max(launches, 1), 4)},
+        "cpu_baseline": base,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

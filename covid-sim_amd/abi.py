@@ -119,6 +119,9 @@ def declare(lib, prefix):
         "get_cell_tables": (I, [vp, I, P(F), P(F), P(C.c_int32), P(F), P(C.c_int32)]),
         "eval_hazards": (I, [vp, I, I, C.c_uint16, P(C.c_int32), P(C.c_int32), P(D), I]),
         "icdf_choose": (I, [vp, I, P(D), P(D), P(C.c_uint16), I]),
+        "set_profiling": (I, [vp, I]),
+        "get_profile": (I, [vp, P(D), P(C.c_int64), I]),
+        "sync": (I, [vp]),
     }
     for k, v in sig.items():
         if v is None:

@@ -1,0 +1,995 @@
+// cs_engine.hip -- host side of libcovidsim_hip.so: device memory, uploads,
+// step orchestration and the C ABI of include/covidsim_hip.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/covidsim_hip.h"
+#include "cs_kernels_incub.h"
+
+using cs::Dev;
+
+struct cs_engine {
+    char err[512];
+    int device = 0;
+    hipStream_t stream = nullptr;
+    cs_params P;
+    bool have_params = false, have_pop = false;
+    cs_params* dP = nullptr;
+    double *dkl = nullptr, *dkh = nullptr;
+    int N = 0, NH = 0, NC = 0, NCT = 0, NM = 0, NA = 0, NPT = 0, R = 0;
+    int np[CS_MAX_PLACE_TYPES] = {0};
+    std::vector<void*> pop_allocs, real_allocs;
+    Dev d;
+    int* d_sel = nullptr;
+    int ts_max = 0, ts_n = 0;
+    int steps_done = 0;
+    // profiling: event pairs per kernel kind, resolved on cs_get_profile
+    bool prof = false;
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_open;
+    double prof_ms[CS_NUM_KERNEL_KINDS] = {0};
+    long long prof_n[CS_NUM_KERNEL_KINDS] = {0};
+};
+
+namespace {
+
+int set_err(cs_engine* e, int code, const char* fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(e->err, sizeof(e->err), fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+#define HIPCHK(e, x)                                                                    \
+    do {                                                                                \
+        hipError_t _st = (x);                                                           \
+        if (_st != hipSuccess)                                                          \
+            return set_err(e, CS_EHIP, "%s: %s (%s:%d)", #x, hipGetErrorString(_st), __FILE__, __LINE__); \
+    } while (0)
+
+template <class T>
+int dalloc(cs_engine* e, std::vector<void*>& list, T** out, size_t n)
+{
+    void* p = nullptr;
+    size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
+    hipError_t st = hipMalloc(&p, bytes);
+    if (st != hipSuccess) return set_err(e, CS_ENOMEM, "hipMalloc(%zu bytes): %s", bytes, hipGetErrorString(st));
+    list.push_back(p);
+    *out = (T*)p;
+    return 0;
+}
+
+template <class T>
+int dupload(cs_engine* e, std::vector<void*>& list, const T** out, const T* src, size_t n)
+{
+    T* p;
+    int rc = dalloc(e, list, &p, n);
+    if (rc) return rc;
+    if (n && src) HIPCHK(e, hipMemcpy(p, src, n * sizeof(T), hipMemcpyHostToDevice));
+    *out = p;
+    return 0;
+}
+
+void free_list(std::vector<void*>& list)
+{
+    for (void* p : list) (void)hipFree(p);
+    list.clear();
+}
+
+// KernelStruct families (src/Kernels.cpp:68-106) with the host's libm, so
+// the tables equal KernelLookup::init's (src/Kernels.cpp:17-45, norm 1.0)
+double kernel_fn(const cs_params& P, double r2)
+{
+    const double sh = P.kernel_shape, sc = P.kernel_scale, p3 = P.kernel_p3, p4 = P.kernel_p4;
+    double t, dd;
+    switch (P.kernel_type) {
+    case 1: return exp(-sqrt(r2) / sc);
+    case 2: t = -sh * log(sqrt(r2) / sc + 1.0); return (t < -690.0) ? 0.0 : exp(t);
+    case 3: return exp(-r2 / (sc * sc));
+    case 4: return (r2 > sc * sc) ? 0.0 : 1.0;
+    case 5: t = 0.5 * sh * log(r2 / (sc * sc)); return (t > 690.0) ? 0.0 : (1.0 / (exp(t) + 1.0));
+    case 6: t = log(sqrt(r2) / sc + 1.0); return (t < -690.0) ? 0.0 : (exp(-sh * t) + p3 * exp(-p4 * t)) / (1.0 + p3);
+    default: dd = sqrt(r2); t = -sh * log(dd / sc + 1.0); return (t < -690.0) ? 0.0 : exp(t - pow(dd / p3, p4));
+    }
+}
+
+hipEvent_t pool_event(cs_engine* e)
+{
+    if (!e->ev_pool.empty()) {
+        hipEvent_t ev = e->ev_pool.back();
+        e->ev_pool.pop_back();
+        return ev;
+    }
+    hipEvent_t ev = nullptr;
+    (void)hipEventCreate(&ev);
+    return ev;
+}
+
+struct ProfScope {  // records a start/stop event pair around one launch
+    cs_engine* e;
+    int kind;
+    hipEvent_t a = nullptr, b = nullptr;
+    ProfScope(cs_engine* e_, int k) : e(e_), kind(k)
+    {
+        if (e->prof) {
+            a = pool_event(e);
+            b = pool_event(e);
+            (void)hipEventRecord(a, e->stream);
+        }
+    }
+    ~ProfScope()
+    {
+        if (e->prof) {
+            (void)hipEventRecord(b, e->stream);
+            e->ev_open.push_back({kind, {a, b}});
+        }
+    }
+};
+
+void resolve_profile(cs_engine* e)
+{
+    for (auto& x : e->ev_open) {
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, x.second.first, x.second.second) == hipSuccess) {
+            e->prof_ms[x.first] += ms;
+            e->prof_n[x.first]++;
+        }
+        e->ev_pool.push_back(x.second.first);
+        e->ev_pool.push_back(x.second.second);
+    }
+    e->ev_open.clear();
+}
+
+unsigned grid_for(long long n, int block, unsigned cap = 65535u * 16)
+{
+    long long g = (n + block - 1) / block;
+    if (g < 1) g = 1;
+    if (g > (long long)cap) g = cap;
+    return (unsigned)g;
+}
+
+int check_path(cs_engine* e)
+{
+    const cs_params& P = e->P;
+    if (P.interventions_active) return set_err(e, CS_EUNSUPPORTED, "interventions are outside this path");
+    if (P.false_positive_rate != 0.0) return set_err(e, CS_EUNSUPPORTED, "false-positive infections are outside this path");
+    if (P.do_digital_contact_tracing) return set_err(e, CS_EUNSUPPORTED, "digital contact tracing is outside this path");
+    if (P.do_blanket_move_restr) return set_err(e, CS_EUNSUPPORTED, "movement restrictions are outside this path");
+    if (P.absenteeism_place_closure) return set_err(e, CS_EUNSUPPORTED, "absenteeism place closure is outside this path");
+    return 0;
+}
+
+int check_flags(cs_engine* e)
+{
+    unsigned f = 0;
+    HIPCHK(e, hipMemcpyAsync(&f, e->d.status, sizeof(f), hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    if (f & cs::FLAG_RANGE) return set_err(e, CS_ERANGE, "r too large in NumKernel (distance beyond the kernel table)");
+    if (f & cs::FLAG_OVERFLOW) return set_err(e, CS_EOVERFLOW, "infection event queue overflow (capacity %u)", e->d.ev_cap);
+    if (f & cs::FLAG_BATCH) return set_err(e, CS_EUNSUPPORTED, "more than %d list moves in one cell in one step", CS_MOVE_CAP);
+    if (f & cs::FLAG_STUCK) return set_err(e, CS_ERANGE, "spatial draw found no live candidate in %d attempts", CS_MAX_ATTEMPTS);
+    return 0;
+}
+
+cs::StepScalars scalars(const cs_params& P, double t)
+{
+    cs::StepScalars s;
+    s.t = t;
+    s.ts = (unsigned)(uint16_t)(P.time_steps_per_day * t);
+    s.fp = P.model_time_step / (1 - P.false_positive_rate);
+    s.seas = P.do_seasonality ? P.seasonality[((int)t) % CS_DAYS_PER_YEAR] : 1.0;
+    s.beta_s = s.seas * s.fp * P.local_beta;
+    s.beta_h = P.do_households ? s.seas * s.fp * P.household_trans : 0;
+    return s;
+}
+
+// sweep kernels that append events; state is untouched until resolution
+int launch_infect_events(cs_engine* e, const cs::StepScalars& sc)
+{
+    Dev& d = e->d;
+    hipStream_t st = e->stream;
+    const unsigned nrc = (unsigned)((long long)d.R * d.NC);
+    cs::k_zero_u32<<<1, 1, 0, st>>>(d.ev_count);
+    { ProfScope ps(e, CS_K_INFECT_CELLS); cs::k_infect_cells<<<nrc, 64, 0, st>>>(d, sc); }
+    { ProfScope ps(e, CS_K_OTHER); cs::k_scan_draws<<<1, 1024, 0, st>>>(d); }
+    { ProfScope ps(e, CS_K_SPATIAL_DRAWS); cs::k_spatial_draws<<<2048, 256, 0, st>>>(d, sc); }
+    HIPCHK(e, hipGetLastError());
+    return 0;
+}
+
+int launch_resolve(cs_engine* e, const cs::StepScalars& sc)
+{
+    Dev& d = e->d;
+    hipStream_t st = e->stream;
+    const unsigned nrc = (unsigned)((long long)d.R * d.NC);
+    {
+        ProfScope ps(e, CS_K_RESOLVE);
+        cs::k_resolve_claim<<<1024, 256, 0, st>>>(d);
+        cs::k_resolve_apply<<<1024, 256, 0, st>>>(d, sc);
+        cs::k_resolve_reset<<<1024, 256, 0, st>>>(d);
+    }
+    { ProfScope ps(e, CS_K_LIST_MOVE); cs::k_list_move<<<nrc, 256, 0, st>>>(d, 0); }
+    HIPCHK(e, hipGetLastError());
+    return 0;
+}
+
+int launch_incub(cs_engine* e, double t)
+{
+    Dev& d = e->d;
+    const cs_params& P = e->P;
+    hipStream_t st = e->stream;
+    const unsigned ts = (unsigned)(uint16_t)(P.time_steps_per_day * t);
+    const unsigned nrc = (unsigned)((long long)d.R * d.NC);
+    if (P.do_places)
+        for (int h = 0; h < P.num_holidays; h++) {
+            double ht = P.holiday_start[h] + P.holidays_start_day_simtime;
+            if ((t + P.model_time_step >= ht) && (t < ht)) {
+                long long n = (long long)d.R * d.np_off[CS_MAX_PLACE_TYPES];
+                cs::k_holidays<<<grid_for(n, 256), 256, 0, st>>>(d, h, ht);
+            }
+        }
+    { ProfScope ps(e, CS_K_LATENT_SCAN); cs::k_latent_scan<<<nrc, 64, 0, st>>>(d, ts); }
+    { ProfScope ps(e, CS_K_LIST_MOVE); cs::k_list_move<<<nrc, 256, 0, st>>>(d, 1); }
+    cs::k_zero_u32<<<1, 1, 0, st>>>(d.care_count);
+    { ProfScope ps(e, CS_K_INFECTIOUS_SCAN); cs::k_infectious_scan<<<nrc, 64, 0, st>>>(d, ts); }
+    {
+        ProfScope ps(e, CS_K_OTHER);
+        cs::k_care_eval<<<256, 256, 0, st>>>(d, ts);
+        cs::k_care_apply<<<256, 256, 0, st>>>(d, ts);
+    }
+    { ProfScope ps(e, CS_K_LIST_MOVE); cs::k_list_move<<<nrc, 256, 0, st>>>(d, 2); }
+    HIPCHK(e, hipGetLastError());
+    return 0;
+}
+
+int launch_update_probs(cs_engine* e, const int* sel)
+{
+    Dev& d = e->d;
+    long long n = (long long)d.R * d.NC;
+    ProfScope ps(e, CS_K_UPDATE_PROBS);
+    cs::k_update_s0<<<grid_for(n, 256), 256, 0, e->stream>>>(d, sel);
+    cs::k_update_rows<<<grid_for(n, 64), 64, 0, e->stream>>>(d, sel);
+    HIPCHK(e, hipGetLastError());
+    return 0;
+}
+
+int grow_events(cs_engine* e)
+{
+    Dev& d = e->d;
+    unsigned cap = d.ev_cap * 2u;
+    if (cap < d.ev_cap) return set_err(e, CS_EOVERFLOW, "event queue cannot grow further");
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    (void)hipFree(d.ev_key); (void)hipFree(d.ev_gp); (void)hipFree(d.ev_infector); (void)hipFree(d.ev_type);
+    auto drop = [&](void* p) { e->real_allocs.erase(std::remove(e->real_allocs.begin(), e->real_allocs.end(), p), e->real_allocs.end()); };
+    drop(d.ev_key); drop(d.ev_gp); drop(d.ev_infector); drop(d.ev_type);
+    int rc;
+    if ((rc = dalloc(e, e->real_allocs, &d.ev_key, cap))) return rc;
+    if ((rc = dalloc(e, e->real_allocs, &d.ev_gp, cap))) return rc;
+    if ((rc = dalloc(e, e->real_allocs, &d.ev_infector, cap))) return rc;
+    if ((rc = dalloc(e, e->real_allocs, &d.ev_type, cap))) return rc;
+    d.ev_cap = cap;
+    unsigned zero = 0;
+    HIPCHK(e, hipMemcpy(d.status, &zero, sizeof(zero), hipMemcpyHostToDevice));
+    return 0;
+}
+
+// one InfectSweep; an event-queue overflow is detected before any state
+// changes and the sweep is re-run with a larger queue
+int infect_sweep(cs_engine* e, double t)
+{
+    cs::StepScalars sc = scalars(e->P, t);
+    for (int attempt = 0; attempt < 8; attempt++) {
+        int rc = launch_infect_events(e, sc);
+        if (rc) return rc;
+        rc = check_flags(e);
+        if (rc == CS_EOVERFLOW) {
+            if ((rc = grow_events(e))) return rc;
+            continue;
+        }
+        if (rc) return rc;
+        if ((rc = launch_resolve(e, sc))) return rc;
+        return check_flags(e);
+    }
+    return set_err(e, CS_EOVERFLOW, "event queue overflow persists");
+}
+
+template <class T>
+struct DBuf {
+    T* p = nullptr;
+    ~DBuf() { if (p) (void)hipFree(p); }
+};
+
+}  // namespace
+
+extern "C" {
+
+const char* cs_last_error(const cs_engine* e) { return e ? e->err : "null engine"; }
+const char* cs_version(void) { return "covidsim_hip 0.1 (gfx950)"; }
+
+int cs_create(cs_engine** out, int device)
+{
+    if (!out) return CS_EINVAL;
+    cs_engine* e = new cs_engine();
+    *out = e;
+    e->err[0] = 0;
+    e->device = device;
+    int n = 0;
+    HIPCHK(e, hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) return set_err(e, CS_EINVAL, "device %d not present (%d devices)", device, n);
+    HIPCHK(e, hipSetDevice(device));
+    HIPCHK(e, hipStreamCreate(&e->stream));
+    memset(&e->d, 0, sizeof(e->d));
+    int rc;
+    if ((rc = dalloc(e, e->pop_allocs, &e->d.status, 1))) return rc;
+    if ((rc = dalloc(e, e->pop_allocs, &e->d.wc, 1))) return rc;
+    HIPCHK(e, hipMemset(e->d.status, 0, sizeof(unsigned)));
+    HIPCHK(e, hipMemset(e->d.wc, 0, sizeof(cs::Work)));
+    return 0;
+}
+
+void cs_destroy(cs_engine* e)
+{
+    if (!e) return;
+    (void)hipSetDevice(e->device);
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
+    free_list(e->real_allocs);
+    free_list(e->pop_allocs);
+    if (e->dP) (void)hipFree(e->dP);
+    if (e->dkl) (void)hipFree(e->dkl);
+    if (e->dkh) (void)hipFree(e->dkh);
+    resolve_profile(e);
+    for (hipEvent_t ev : e->ev_pool) (void)hipEventDestroy(ev);
+    if (e->stream) (void)hipStreamDestroy(e->stream);
+    delete e;
+}
+
+int cs_set_params(cs_engine* e, const cs_params* p, const double* kl, const double* kh)
+{
+    if (!e || !p) return CS_EINVAL;
+    HIPCHK(e, hipSetDevice(e->device));
+    if (p->kernel_type < 1 || p->kernel_type > 7) return set_err(e, CS_EINVAL, "unknown kernel type %d", p->kernel_type);
+    if (p->kernel_size <= 0 || !(p->kernel_delta > 0)) return set_err(e, CS_EINVAL, "kernel table size/delta not set");
+    if (p->num_place_types < 0 || p->num_place_types > CS_MAX_PLACE_TYPES) return set_err(e, CS_EINVAL, "bad number of place types");
+    if (p->num_holidays < 0 || p->num_holidays > CS_MAX_HOLIDAYS) return set_err(e, CS_EINVAL, "bad number of holidays");
+    bool new_tables = !e->have_params || e->P.kernel_size != p->kernel_size || e->P.kernel_type != p->kernel_type ||
+                      e->P.kernel_delta != p->kernel_delta || e->P.kernel_shape != p->kernel_shape ||
+                      e->P.kernel_scale != p->kernel_scale || e->P.kernel_p3 != p->kernel_p3 ||
+                      e->P.kernel_p4 != p->kernel_p4 || e->P.kernel_expansion != p->kernel_expansion || kl || kh;
+    e->P = *p;
+    if (!e->dP) HIPCHK(e, hipMalloc(&e->dP, sizeof(cs_params)));
+    HIPCHK(e, hipMemcpy(e->dP, p, sizeof(cs_params), hipMemcpyHostToDevice));
+    if (new_tables) {
+        size_t n = (size_t)p->kernel_size + 1;
+        std::vector<double> hl, hh;
+        if (!kl || !kh) {
+            hl.resize(n);
+            hh.resize(n);
+            for (size_t i = 0; i < n; i++) {
+                hl[i] = kernel_fn(*p, (int)i * p->kernel_delta) / 1.0;
+                hh[i] = kernel_fn(*p, (int)i * p->kernel_delta / p->kernel_expansion) / 1.0;
+            }
+        }
+        if (e->dkl) (void)hipFree(e->dkl);
+        if (e->dkh) (void)hipFree(e->dkh);
+        HIPCHK(e, hipMalloc(&e->dkl, n * sizeof(double)));
+        HIPCHK(e, hipMalloc(&e->dkh, n * sizeof(double)));
+        HIPCHK(e, hipMemcpy(e->dkl, kl ? kl : hl.data(), n * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(e, hipMemcpy(e->dkh, kh ? kh : hh.data(), n * sizeof(double), hipMemcpyHostToDevice));
+    }
+    e->d.P = e->dP;
+    e->d.kl = e->dkl;
+    e->d.kh = e->dkh;
+    e->have_params = true;
+    if (e->have_pop && new_tables) {
+        cs::k_build_max_trans<<<grid_for((long long)e->NC * e->NC, 256, 1u << 30), 256, 0, e->stream>>>(e->d);
+        HIPCHK(e, hipGetLastError());
+        int rc = check_flags(e);
+        if (rc) return rc;
+    }
+    return 0;
+}
+
+int cs_set_population(cs_engine* e, const cs_population* pop)
+{
+    if (!e || !pop) return CS_EINVAL;
+    if (!e->have_params) return set_err(e, CS_ESTATE, "cs_set_params must come first");
+    HIPCHK(e, hipSetDevice(e->device));
+    if (pop->n_people <= 0 || pop->n_cells <= 0) return set_err(e, CS_EINVAL, "empty population");
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    free_list(e->real_allocs);
+    e->R = 0;
+    // keep the flags/work words, drop the rest of the old population
+    std::vector<void*> keep = {e->d.status, e->d.wc};
+    for (void* q : e->pop_allocs)
+        if (std::find(keep.begin(), keep.end(), q) == keep.end()) (void)hipFree(q);
+    e->pop_allocs = keep;
+    Dev& d = e->d;
+    const int N = pop->n_people, NC = pop->n_cells;
+    e->N = d.N = N; e->NH = d.NH = pop->n_households; e->NC = d.NC = NC;
+    e->NCT = pop->n_cells_total; e->NM = d.NM = pop->n_mcells; e->NA = pop->n_adunits;
+    e->NPT = d.NPT = e->P.num_place_types;
+    // host-side checks of every index the kernels follow
+    for (int j = 0; j <= NC; j++)
+        if (pop->cell_member_start[j] < 0 || pop->cell_member_start[j] > N || (j && pop->cell_member_start[j] < pop->cell_member_start[j - 1]))
+            return set_err(e, CS_EINVAL, "cell_member_start not monotone within [0, n_people]");
+    if (pop->cell_member_start[NC] != N) return set_err(e, CS_EINVAL, "cells do not hold every person");
+    std::vector<int32_t> map(std::max(1, e->NCT), -1), pc(N);
+    for (int j = 0; j < NC; j++) {
+        if (pop->cell_index[j] < 0 || pop->cell_index[j] >= e->NCT) return set_err(e, CS_EINVAL, "cell_index out of range");
+        if (pop->cell_n[j] != pop->cell_member_start[j + 1] - pop->cell_member_start[j])
+            return set_err(e, CS_EINVAL, "cell_n disagrees with cell_member_start");
+        map[pop->cell_index[j]] = j;
+    }
+    for (int i = 0; i < N; i++) {
+        int c = (pop->pcell[i] >= 0 && pop->pcell[i] < e->NCT) ? map[pop->pcell[i]] : -1;
+        if (c < 0) return set_err(e, CS_EINVAL, "person %d in an unpopulated or unknown cell", i);
+        pc[i] = c;
+        if (pop->hh[i] < 0 || pop->hh[i] >= e->NH) return set_err(e, CS_EINVAL, "person %d household out of range", i);
+        if (pop->mcell[i] < 0 || pop->mcell[i] >= e->NM) return set_err(e, CS_EINVAL, "person %d microcell out of range", i);
+    }
+    for (int j = 0; j < N; j++) {
+        int k = pop->cell_members[j];
+        if (k < 0 || k >= N) return set_err(e, CS_EINVAL, "cell member out of range");
+    }
+    for (int h = 0; h < e->NH; h++)
+        if (pop->hh_first[h] < 0 || pop->hh_first[h] + pop->hh_nh[h] > N || pop->hh_nhr[h] < 1 || pop->hh_nhr[h] > CS_MAX_HOUSEHOLD_SIZE)
+            return set_err(e, CS_EINVAL, "household %d out of range", h);
+    std::vector<int32_t> mstart(e->NM + 1, 0), mmem(N);
+    for (int i = 0; i < N; i++) mstart[pop->mcell[i] + 1]++;
+    for (int m = 0; m < e->NM; m++) mstart[m + 1] += mstart[m];
+    {
+        std::vector<int32_t> cur(mstart.begin(), mstart.end() - 1);
+        for (int i = 0; i < N; i++) mmem[cur[pop->mcell[i]]++] = i;
+    }
+    int rc;
+    auto& L = e->pop_allocs;
+    if ((rc = dupload(e, L, &d.hh, pop->hh, N))) return rc;
+    if ((rc = dupload(e, L, &d.pc, pc.data(), N))) return rc;
+    if ((rc = dupload(e, L, &d.mcell, pop->mcell, N))) return rc;
+    if ((rc = dupload(e, L, &d.age, pop->age, N))) return rc;
+    if ((rc = dupload(e, L, &d.pflags, pop->flags, N))) return rc;
+    if ((rc = dupload(e, L, &d.plinks, pop->place_links, (size_t)N * CS_MAX_PLACE_TYPES))) return rc;
+    if ((rc = dupload(e, L, &d.pglinks, pop->place_group_links, (size_t)N * CS_MAX_PLACE_TYPES))) return rc;
+    if ((rc = dupload(e, L, &d.hfirst, pop->hh_first, e->NH))) return rc;
+    if ((rc = dupload(e, L, &d.hnh, pop->hh_nh, e->NH))) return rc;
+    if ((rc = dupload(e, L, &d.hnhr, pop->hh_nhr, e->NH))) return rc;
+    if ((rc = dupload(e, L, &d.hloc, pop->hh_loc, (size_t)e->NH * 2))) return rc;
+    if ((rc = dupload(e, L, &d.cstart, pop->cell_member_start, NC + 1))) return rc;
+    if ((rc = dupload(e, L, &d.cn, pop->cell_n, NC))) return rc;
+    if ((rc = dupload(e, L, &d.cidx, pop->cell_index, NC))) return rc;
+    if ((rc = dupload(e, L, &d.cmem, pop->cell_members, N))) return rc;
+    if ((rc = dupload(e, L, &d.mstart, mstart.data(), e->NM + 1))) return rc;
+    if ((rc = dupload(e, L, &d.mmem, mmem.data(), N))) return rc;
+    int off = 0;
+    for (int t = 0; t < CS_MAX_PLACE_TYPES; t++) {
+        d.np_off[t] = off;
+        int np = (t < e->NPT) ? pop->n_places[t] : 0;
+        e->np[t] = d.np[t] = np;
+        off += np;
+        if (t >= e->NPT) { d.pl_n[t] = d.pl_ms[t] = d.pl_mem[t] = d.pl_gs0[t] = d.pl_gst[t] = d.pl_gsz[t] = nullptr; continue; }
+        const int32_t* ms = pop->place_member_start[t];
+        const int32_t* gs0 = pop->place_groups_start[t];
+        if (np < 0 || (np && (!ms || !gs0 || ms[0] != 0 || gs0[0] != 0))) return set_err(e, CS_EINVAL, "place arrays of type %d", t);
+        for (int k = 0; k < np; k++) {
+            if (ms[k + 1] - ms[k] != pop->place_n[t][k] || pop->place_n[t][k] <= 0)
+                return set_err(e, CS_EINVAL, "place %d of type %d: member range and n disagree", k, t);
+            for (int g = gs0[k]; g < gs0[k + 1]; g++)
+                if (pop->place_group_start[t][g] < 0 || pop->place_group_size[t][g] < 0 ||
+                    pop->place_group_start[t][g] + pop->place_group_size[t][g] > pop->place_n[t][k])
+                    return set_err(e, CS_EINVAL, "group %d of place %d type %d outside the member list", g, k, t);
+        }
+        for (int q = 0; q < (np ? ms[np] : 0); q++)
+            if (pop->place_members[t][q] < 0 || pop->place_members[t][q] >= N) return set_err(e, CS_EINVAL, "place member out of range");
+        int ng = np ? gs0[np] : 0;
+        if ((rc = dupload(e, L, &d.pl_n[t], pop->place_n[t], np))) return rc;
+        if ((rc = dupload(e, L, &d.pl_ms[t], ms, np + 1))) return rc;
+        if ((rc = dupload(e, L, &d.pl_mem[t], pop->place_members[t], np ? ms[np] : 0))) return rc;
+        if ((rc = dupload(e, L, &d.pl_gs0[t], gs0, np + 1))) return rc;
+        if ((rc = dupload(e, L, &d.pl_gst[t], pop->place_group_start[t], ng))) return rc;
+        if ((rc = dupload(e, L, &d.pl_gsz[t], pop->place_group_size[t], ng))) return rc;
+    }
+    d.np_off[CS_MAX_PLACE_TYPES] = off;
+    for (int i = 0; i < N; i++)
+        for (int t = 0; t < e->NPT; t++) {
+            int lk = pop->place_links[(size_t)i * CS_MAX_PLACE_TYPES + t];
+            if (lk >= e->np[t]) return set_err(e, CS_EINVAL, "person %d links to place %d of type %d beyond %d", i, lk, t, e->np[t]);
+            if (lk >= 0) {
+                int g = pop->place_group_links[(size_t)i * CS_MAX_PLACE_TYPES + t];
+                if (g >= pop->place_groups_start[t][lk + 1] - pop->place_groups_start[t][lk])
+                    return set_err(e, CS_EINVAL, "person %d group link beyond the place's groups", i);
+            }
+        }
+    float* mt;
+    if ((rc = dalloc(e, L, &mt, (size_t)NC * NC))) return rc;
+    d.max_trans = mt;
+    HIPCHK(e, hipMemset(d.status, 0, sizeof(unsigned)));
+    cs::k_build_max_trans<<<grid_for((long long)NC * NC, 256, 1u << 30), 256, 0, e->stream>>>(d);
+    HIPCHK(e, hipGetLastError());
+    e->have_pop = true;
+    return check_flags(e);
+}
+
+int cs_alloc_realisations(cs_engine* e, int n_real, const uint64_t* seeds)
+{
+    if (!e || n_real <= 0 || !seeds) return CS_EINVAL;
+    if (!e->have_pop) return set_err(e, CS_ESTATE, "cs_set_population must come first");
+    HIPCHK(e, hipSetDevice(e->device));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    free_list(e->real_allocs);
+    Dev& d = e->d;
+    const size_t R = (size_t)n_real, N = (size_t)e->N, NC = (size_t)e->NC, NM = (size_t)e->NM;
+    const size_t RN = R * N, RC = R * NC, PL = R * (size_t)d.np_off[CS_MAX_PLACE_TYPES];
+    e->R = d.R = n_real;
+    auto& A = e->real_allocs;
+    int rc = 0;
+#define AL(f, n) if ((rc = dalloc(e, A, &d.f, (n)))) return rc
+    uint64_t* sd;
+    if ((rc = dalloc(e, A, &sd, R))) return rc;
+    HIPCHK(e, hipMemcpy(sd, seeds, R * 8, hipMemcpyHostToDevice));
+    d.seed = sd;
+    AL(inf, RN); AL(sev_cur, RN); AL(sev_final, RN); AL(bits, RN); AL(qc, RN); AL(dct, RN);
+    AL(listpos, RN); AL(infector, RN); AL(itype, RN); AL(infness, RN); AL(susc, RN); AL(prob_care, RN);
+    AL(t_inf, RN); AL(t_lat, RN); AL(t_rod, RN); AL(t_sari, RN); AL(t_crit, RN); AL(t_step, RN); AL(t_det, RN);
+    AL(t_as, RN); AL(t_ae, RN); AL(t_ts, RN); AL(t_te, RN); AL(t_vs, RN); AL(t_is, RN); AL(t_qs, RN);
+    AL(cS, RC); AL(cL, RC); AL(cI, RC); AL(cR, RC); AL(cD, RC); AL(cumTC, RC); AL(S0, RC);
+    AL(tot_prob, RC); AL(cum_trans, RC * NC); AL(invcdf, RC * 1025); AL(perm, RN);
+    AL(pl_cs, PL); AL(pl_ce, PL); AL(m_socdist, R * NM); AL(m_moverest, R * NM); AL(m_trig, R * NM * 6);
+    AL(cnt, R); AL(prev_prop, R); AL(samp_prop, R); AL(loc_init, 2 * R);
+    AL(si_buf, RN); AL(ci_buf, RN); AL(spK, RC); AL(sp_sum, RC); AL(sp_off, RC + 1);
+    size_t cap = std::min<size_t>(std::max<size_t>(RN / 16, (size_t)1 << 20), (size_t)1 << 28);
+    d.ev_cap = (unsigned)cap;
+    AL(ev_key, cap); AL(ev_gp, cap); AL(ev_infector, cap); AL(ev_type, cap); AL(ev_count, 1);
+    AL(claim, RN); AL(lastk, RN); AL(batch, RN); AL(bcount, RC);
+    AL(care_gp, RN); AL(care_k, RN); AL(care_abs, RN); AL(care_count, 1);
+    AL(last_tot, 3 * R);
+#undef AL
+    if ((rc = dalloc(e, A, &e->d_sel, R))) return rc;
+    HIPCHK(e, hipMemset(d.claim, 0xff, RN * 8));
+    HIPCHK(e, hipMemset(d.lastk, 0, RN * 8));
+    HIPCHK(e, hipMemset(d.bcount, 0, RC * 4));
+    HIPCHK(e, hipMemset(d.cnt, 0, R * sizeof(cs::Counters)));
+    HIPCHK(e, hipMemset(d.m_trig, 0, R * NM * 6 * 4));
+    d.ts_buf = nullptr;
+    d.ts_max = 0;
+    return 0;
+}
+
+int cs_reset_realisations(cs_engine* e, int n_seed)
+{
+    if (!e) return CS_EINVAL;
+    if (e->R == 0) return set_err(e, CS_ESTATE, "no realisations allocated");
+    int rc = check_path(e);
+    if (rc) return rc;
+    HIPCHK(e, hipSetDevice(e->device));
+    Dev& d = e->d;
+    hipStream_t st = e->stream;
+    HIPCHK(e, hipMemsetAsync(d.cnt, 0, (size_t)d.R * sizeof(cs::Counters), st));
+    cs::k_reset_persons<<<grid_for((long long)d.R * d.N, 256), 256, 0, st>>>(d);
+    cs::k_reset_cells<<<grid_for((long long)d.R * d.NC, 64), 64, 0, st>>>(d);
+    cs::k_reset_misc<<<grid_for(std::max((long long)d.R * d.np_off[CS_MAX_PLACE_TYPES], (long long)d.R * d.NM), 256), 256, 0, st>>>(d);
+    if ((rc = launch_update_probs(e, nullptr))) return rc;
+    cs::k_seed<<<grid_for(d.R, 64), 64, 0, st>>>(d, n_seed);
+    cs::k_list_move<<<(unsigned)((long long)d.R * d.NC), 256, 0, st>>>(d, 0);
+    HIPCHK(e, hipGetLastError());
+    return check_flags(e);
+}
+
+int cs_load_state(cs_engine* e, int real, const cs_state* s)
+{
+    if (!e || !s) return CS_EINVAL;
+    if (real < 0 || real >= e->R) return set_err(e, CS_EINVAL, "realisation %d out of range", real);
+    HIPCHK(e, hipSetDevice(e->device));
+    Dev& d = e->d;
+    const size_t N = e->N, NC = e->NC, o = (size_t)real * N, oc = (size_t)real * NC;
+#define UP(dst, src, off, n) if (src) HIPCHK(e, hipMemcpy(d.dst + (off), src, (n) * sizeof(*d.dst), hipMemcpyHostToDevice))
+    UP(inf, s->inf, o, N); UP(sev_cur, s->severity_current, o, N); UP(sev_final, s->severity_final, o, N);
+    UP(bits, s->bits, o, N); UP(listpos, s->listpos, o, N); UP(infector, s->infector, o, N); UP(itype, s->infect_type, o, N);
+    UP(infness, s->infectiousness, o, N); UP(susc, s->susc, o, N); UP(prob_care, s->prob_care, o, N);
+    UP(t_inf, s->infection_time, o, N); UP(t_lat, s->latent_time, o, N); UP(t_rod, s->recovery_or_death_time, o, N);
+    UP(t_sari, s->sari_time, o, N); UP(t_crit, s->critical_time, o, N); UP(t_step, s->stepdown_time, o, N);
+    UP(t_det, s->detected_time, o, N); UP(t_as, s->absent_start_time, o, N); UP(t_ae, s->absent_stop_time, o, N);
+    UP(t_ts, s->treat_start_time, o, N); UP(t_te, s->treat_stop_time, o, N); UP(t_vs, s->vacc_start_time, o, N);
+    UP(t_is, s->isolation_start_time, o, N); UP(t_qs, s->quar_start_time, o, N); UP(qc, s->quar_comply, o, N);
+    UP(dct, s->dct_traced, o, N);
+    UP(cS, s->cell_S, oc, NC); UP(cL, s->cell_L, oc, NC); UP(cI, s->cell_I, oc, NC); UP(cR, s->cell_R, oc, NC);
+    UP(cD, s->cell_D, oc, NC); UP(cumTC, s->cell_cumTC, oc, NC); UP(perm, s->cell_perm, o, N);
+    const size_t tot = d.np_off[CS_MAX_PLACE_TYPES];
+    for (int t = 0; t < e->NPT; t++) {
+        UP(pl_cs, s->place_close_start[t], real * tot + d.np_off[t], (size_t)e->np[t]);
+        UP(pl_ce, s->place_close_end[t], real * tot + d.np_off[t], (size_t)e->np[t]);
+    }
+    UP(m_socdist, s->mcell_socdist, (size_t)real * e->NM, (size_t)e->NM);
+    UP(m_moverest, s->mcell_moverest, (size_t)real * e->NM, (size_t)e->NM);
+#undef UP
+    // derived per-realisation values, as a host run would hold them
+    std::vector<int32_t> S(NC);
+    HIPCHK(e, hipMemcpy(S.data(), d.cS + oc, NC * 4, hipMemcpyDeviceToHost));
+    long long st = 0;
+    for (int32_t v : S) st += v;
+    double prev = (double)st / (double)N;
+    double loc[2] = {e->P.location_initial_infection[0], e->P.location_initial_infection[1]};
+    HIPCHK(e, hipMemcpy(d.prev_prop + real, &prev, 8, hipMemcpyHostToDevice));
+    HIPCHK(e, hipMemcpy(d.loc_init + 2 * real, loc, 16, hipMemcpyHostToDevice));
+    std::vector<int> sel(e->R, 0);
+    sel[real] = 1;
+    HIPCHK(e, hipMemcpy(e->d_sel, sel.data(), e->R * sizeof(int), hipMemcpyHostToDevice));
+    int rc = launch_update_probs(e, e->d_sel);
+    if (rc) return rc;
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    return 0;
+}
+
+int cs_store_state(cs_engine* e, int real, cs_state* s)
+{
+    if (!e || !s) return CS_EINVAL;
+    if (real < 0 || real >= e->R) return set_err(e, CS_EINVAL, "realisation %d out of range", real);
+    HIPCHK(e, hipSetDevice(e->device));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    Dev& d = e->d;
+    const size_t N = e->N, NC = e->NC, o = (size_t)real * N, oc = (size_t)real * NC;
+#define DN(src, dst, off, n) if (dst) HIPCHK(e, hipMemcpy(dst, d.src + (off), (n) * sizeof(*d.src), hipMemcpyDeviceToHost))
+    DN(inf, s->inf, o, N); DN(sev_cur, s->severity_current, o, N); DN(sev_final, s->severity_final, o, N);
+    DN(bits, s->bits, o, N); DN(listpos, s->listpos, o, N); DN(infector, s->infector, o, N); DN(itype, s->infect_type, o, N);
+    DN(infness, s->infectiousness, o, N); DN(susc, s->susc, o, N); DN(prob_care, s->prob_care, o, N);
+    DN(t_inf, s->infection_time, o, N); DN(t_lat, s->latent_time, o, N); DN(t_rod, s->recovery_or_death_time, o, N);
+    DN(t_sari, s->sari_time, o, N); DN(t_crit, s->critical_time, o, N); DN(t_step, s->stepdown_time, o, N);
+    DN(t_det, s->detected_time, o, N); DN(t_as, s->absent_start_time, o, N); DN(t_ae, s->absent_stop_time, o, N);
+    DN(t_ts, s->treat_start_time, o, N); DN(t_te, s->treat_stop_time, o, N); DN(t_vs, s->vacc_start_time, o, N);
+    DN(t_is, s->isolation_start_time, o, N); DN(t_qs, s->quar_start_time, o, N); DN(qc, s->quar_comply, o, N);
+    DN(dct, s->dct_traced, o, N);
+    DN(cS, s->cell_S, oc, NC); DN(cL, s->cell_L, oc, NC); DN(cI, s->cell_I, oc, NC); DN(cR, s->cell_R, oc, NC);
+    DN(cD, s->cell_D, oc, NC); DN(cumTC, s->cell_cumTC, oc, NC); DN(perm, s->cell_perm, o, N);
+    const size_t tot = d.np_off[CS_MAX_PLACE_TYPES];
+    for (int t = 0; t < e->NPT; t++) {
+        DN(pl_cs, s->place_close_start[t], real * tot + d.np_off[t], (size_t)e->np[t]);
+        DN(pl_ce, s->place_close_end[t], real * tot + d.np_off[t], (size_t)e->np[t]);
+    }
+    DN(m_socdist, s->mcell_socdist, (size_t)real * e->NM, (size_t)e->NM);
+    DN(m_moverest, s->mcell_moverest, (size_t)real * e->NM, (size_t)e->NM);
+#undef DN
+    return 0;
+}
+
+int cs_infect_sweep(cs_engine* e, double t)
+{
+    if (!e) return CS_EINVAL;
+    if (e->R == 0) return set_err(e, CS_ESTATE, "no realisations allocated");
+    int rc = check_path(e);
+    if (rc) return rc;
+    HIPCHK(e, hipSetDevice(e->device));
+    return infect_sweep(e, t);
+}
+
+int cs_incub_recovery_sweep(cs_engine* e, double t)
+{
+    if (!e) return CS_EINVAL;
+    if (e->R == 0) return set_err(e, CS_ESTATE, "no realisations allocated");
+    int rc = check_path(e);
+    if (rc) return rc;
+    HIPCHK(e, hipSetDevice(e->device));
+    if ((rc = launch_incub(e, t))) return rc;
+    return check_flags(e);
+}
+
+int cs_update_probs(cs_engine* e)
+{
+    if (!e) return CS_EINVAL;
+    if (e->R == 0) return set_err(e, CS_ESTATE, "no realisations allocated");
+    HIPCHK(e, hipSetDevice(e->device));
+    int rc = launch_update_probs(e, nullptr);
+    if (rc) return rc;
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    return 0;
+}
+
+int cs_run_steps(cs_engine* e, double t0, int n_steps, int sps)
+{
+    if (!e || n_steps < 0 || sps <= 0) return CS_EINVAL;
+    if (e->R == 0) return set_err(e, CS_ESTATE, "no realisations allocated");
+    int rc = check_path(e);
+    if (rc) return rc;
+    HIPCHK(e, hipSetDevice(e->device));
+    Dev& d = e->d;
+    int ns = n_steps / sps;
+    if (ns > d.ts_max || !d.ts_buf) {
+        int32_t* tb;
+        if ((rc = dalloc(e, e->real_allocs, &tb, (size_t)d.R * std::max(ns, 1) * CS_TS_FIELDS))) return rc;
+        d.ts_buf = tb;
+        d.ts_max = std::max(ns, 1);
+    }
+    HIPCHK(e, hipMemsetAsync(d.ts_buf, 0, (size_t)d.R * d.ts_max * CS_TS_FIELDS * 4, e->stream));
+    cs::k_record_init<<<grid_for(d.R, 64), 64, 0, e->stream>>>(d);
+    e->ts_n = 0;
+    double t = t0;
+    for (int step = 0; step < n_steps; step++) {
+        if ((rc = infect_sweep(e, t))) return rc;
+        if ((rc = launch_incub(e, t))) return rc;
+        t += e->P.model_time_step;
+        cs::k_sdrop<<<grid_for(d.R, 64), 64, 0, e->stream>>>(d, e->d_sel);
+        if ((rc = launch_update_probs(e, e->d_sel))) return rc;
+        if ((step + 1) % sps == 0) {
+            cs::k_record<<<d.R, 256, 0, e->stream>>>(d, e->ts_n);
+            e->ts_n++;
+        }
+    }
+    HIPCHK(e, hipGetLastError());
+    return check_flags(e);
+}
+
+int cs_get_counters(cs_engine* e, int real, cs_counters* out, int reset)
+{
+    if (!e || !out) return CS_EINVAL;
+    if (real < 0 || real >= e->R) return set_err(e, CS_EINVAL, "realisation %d out of range", real);
+    HIPCHK(e, hipSetDevice(e->device));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    Dev& d = e->d;
+    cs::Counters c;
+    HIPCHK(e, hipMemcpy(&c, d.cnt + real, sizeof(c), hipMemcpyDeviceToHost));
+    memset(out, 0, sizeof(*out));
+    out->cumI = (int64_t)c.cumI; out->cumC = (int64_t)c.cumC; out->cumDC = (int64_t)c.cumDC; out->cumFC = (int64_t)c.cumFC;
+    out->cumTG = (int64_t)c.cumTG; out->cumSI = (int64_t)c.cumSI; out->nTG = (int64_t)c.nTG;
+    out->cumAC = (int64_t)c.cumAC; out->cumACS = (int64_t)c.cumACS; out->cumAA = (int64_t)c.cumAA;
+    out->sumRad2 = c.sumRad2;
+    memcpy(&out->maxRad2, &c.maxRad2_bits, 8);
+    for (int i = 0; i < CS_INFECT_TYPE_MASK; i++) out->cumItype[i] = (int64_t)c.cumItype[i];
+    for (int a = 0; a < CS_NUM_AGE_GROUPS; a++) {
+        out->cumIa[a] = (int64_t)c.cumIa[a]; out->cumCa[a] = (int64_t)c.cumCa[a]; out->cumDa[a] = (int64_t)c.cumDa[a];
+    }
+    for (int i = 0; i < 2; i++) { out->cumI_keyworker[i] = (int64_t)c.cumI_keyworker[i]; out->cumC_keyworker[i] = (int64_t)c.cumC_keyworker[i]; }
+    for (int k = 0; k < CS_NUM_SEV_KINDS; k++) {
+        out->sev_prev[k] = (int64_t)c.sev_prev[k]; out->sev_cum[k] = (int64_t)c.sev_cum[k];
+        for (int a = 0; a < CS_NUM_AGE_GROUPS; a++) {
+            out->sev_prev_age[k][a] = (int64_t)c.sev_prev_age[k][a];
+            out->sev_cum_age[k][a] = (int64_t)c.sev_cum_age[k][a];
+        }
+    }
+    const size_t NC = e->NC, oc = (size_t)real * NC;
+    std::vector<int32_t> v(NC);
+    int64_t* tot[5] = {&out->S, &out->L, &out->I, &out->R, &out->D};
+    int32_t* src[5] = {d.cS, d.cL, d.cI, d.cR, d.cD};
+    for (int q = 0; q < 5; q++) {
+        HIPCHK(e, hipMemcpy(v.data(), src[q] + oc, NC * 4, hipMemcpyDeviceToHost));
+        int64_t s = 0;
+        for (int32_t x : v) s += x;
+        *tot[q] = s;
+    }
+    if (reset) {
+        unsigned long long keep = c.maxRad2_bits;
+        memset(&c, 0, sizeof(c));
+        c.maxRad2_bits = keep;
+        HIPCHK(e, hipMemcpy(d.cnt + real, &c, sizeof(c), hipMemcpyHostToDevice));
+    }
+    return 0;
+}
+
+int cs_get_timeseries(cs_engine* e, int32_t* out, int max_samples, int* n_samples)
+{
+    if (!e || !out || !n_samples) return CS_EINVAL;
+    HIPCHK(e, hipSetDevice(e->device));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    Dev& d = e->d;
+    int n = std::min(e->ts_n, max_samples);
+    for (int r = 0; r < d.R; r++)
+        if (n) HIPCHK(e, hipMemcpy(out + (size_t)r * max_samples * CS_TS_FIELDS, d.ts_buf + (size_t)r * d.ts_max * CS_TS_FIELDS,
+                                   (size_t)n * CS_TS_FIELDS * 4, hipMemcpyDeviceToHost));
+    *n_samples = n;
+    return 0;
+}
+
+int cs_get_work_counts(cs_engine* e, cs_work_counts* out, int reset)
+{
+    if (!e || !out) return CS_EINVAL;
+    HIPCHK(e, hipSetDevice(e->device));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    cs::Work w;
+    HIPCHK(e, hipMemcpy(&w, e->d.wc, sizeof(w), hipMemcpyDeviceToHost));
+    out->n_inf = (int64_t)w.n_inf; out->n_hh_inf = (int64_t)w.n_hh_inf; out->n_hh_member = (int64_t)w.n_hh_member;
+    out->n_place_link = (int64_t)w.n_place_link; out->n_place_cand = (int64_t)w.n_place_cand;
+    out->n_sp_draw = (int64_t)w.n_sp_draw; out->n_sp_susc = (int64_t)w.n_sp_susc; out->n_new = (int64_t)w.n_new;
+    out->n_active = (int64_t)w.n_active; out->n_transition = (int64_t)w.n_transition; out->n_cells = (int64_t)e->NC;
+    if (reset) HIPCHK(e, hipMemset(e->d.wc, 0, sizeof(cs::Work)));
+    return 0;
+}
+
+int cs_set_profiling(cs_engine* e, int enable)
+{
+    if (!e) return CS_EINVAL;
+    HIPCHK(e, hipSetDevice(e->device));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    resolve_profile(e);
+    e->prof = enable != 0;
+    for (int k = 0; k < CS_NUM_KERNEL_KINDS; k++) { e->prof_ms[k] = 0; e->prof_n[k] = 0; }
+    return 0;
+}
+
+int cs_get_profile(cs_engine* e, double* ms, int64_t* launches, int reset)
+{
+    if (!e || !ms || !launches) return CS_EINVAL;
+    HIPCHK(e, hipSetDevice(e->device));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    resolve_profile(e);
+    for (int k = 0; k < CS_NUM_KERNEL_KINDS; k++) {
+        ms[k] = e->prof_ms[k];
+        launches[k] = e->prof_n[k];
+        if (reset) { e->prof_ms[k] = 0; e->prof_n[k] = 0; }
+    }
+    return 0;
+}
+
+int cs_sync(cs_engine* e)
+{
+    if (!e) return CS_EINVAL;
+    HIPCHK(e, hipSetDevice(e->device));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    return 0;
+}
+
+// ---------------------------------------------------------- components
+
+
+#define TMP_UP(e, buf, src, n)                                                   \
+    HIPCHK(e, hipMalloc(&buf.p, std::max<size_t>((n), 1) * sizeof(*buf.p)));    \
+    if (src) HIPCHK(e, hipMemcpy(buf.p, src, (n) * sizeof(*buf.p), hipMemcpyHostToDevice))
+
+int cs_kernel_num(cs_engine* e, const double* r2, double* out, int n)
+{
+    if (!e || !r2 || !out || n < 0) return CS_EINVAL;
+    if (!e->have_params) return set_err(e, CS_ESTATE, "no parameters");
+    HIPCHK(e, hipSetDevice(e->device));
+    DBuf<double> a, b;
+    TMP_UP(e, a, r2, (size_t)n);
+    TMP_UP(e, b, (const double*)nullptr, (size_t)n);
+    HIPCHK(e, hipMemset(e->d.status, 0, 4));
+    if (n) cs::k_t_kernel_num<<<grid_for(n, 256, 1u << 30), 256, 0, e->stream>>>(e->d, a.p, b.p, n);
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    HIPCHK(e, hipMemcpy(out, b.p, (size_t)n * 8, hipMemcpyDeviceToHost));
+    return check_flags(e);
+}
+
+int cs_dist2_persons(cs_engine* e, const int32_t* pa, const int32_t* pb, double* out, int n)
+{
+    if (!e || !pa || !pb || !out || n < 0) return CS_EINVAL;
+    if (!e->have_pop) return set_err(e, CS_ESTATE, "no population");
+    for (int i = 0; i < n; i++)
+        if (pa[i] < 0 || pa[i] >= e->N || pb[i] < 0 || pb[i] >= e->N) return set_err(e, CS_EINVAL, "person index out of range");
+    HIPCHK(e, hipSetDevice(e->device));
+    DBuf<int32_t> a, b;
+    DBuf<double> o;
+    TMP_UP(e, a, pa, (size_t)n);
+    TMP_UP(e, b, pb, (size_t)n);
+    TMP_UP(e, o, (const double*)nullptr, (size_t)n);
+    if (n) cs::k_t_dist2<<<grid_for(n, 256, 1u << 30), 256, 0, e->stream>>>(e->d, a.p, b.p, o.p, n);
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    HIPCHK(e, hipMemcpy(out, o.p, (size_t)n * 8, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int cs_get_cell_tables(cs_engine* e, int real, float* mt, float* ct, int32_t* inv, float* tp, int32_t* S0)
+{
+    if (!e) return CS_EINVAL;
+    if (!e->have_pop) return set_err(e, CS_ESTATE, "no population");
+    HIPCHK(e, hipSetDevice(e->device));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    const size_t NC = e->NC;
+    Dev& d = e->d;
+    if (mt) HIPCHK(e, hipMemcpy(mt, d.max_trans, NC * NC * 4, hipMemcpyDeviceToHost));
+    if (real >= 0 && real < e->R) {
+        const size_t oc = (size_t)real * NC;
+        if (ct) HIPCHK(e, hipMemcpy(ct, d.cum_trans + oc * NC, NC * NC * 4, hipMemcpyDeviceToHost));
+        if (inv) HIPCHK(e, hipMemcpy(inv, d.invcdf + oc * 1025, NC * 1025 * 4, hipMemcpyDeviceToHost));
+        if (tp) HIPCHK(e, hipMemcpy(tp, d.tot_prob + oc, NC * 4, hipMemcpyDeviceToHost));
+        if (S0) HIPCHK(e, hipMemcpy(S0, d.S0 + oc, NC * 4, hipMemcpyDeviceToHost));
+    }
+    return 0;
+}
+
+int cs_eval_hazards(cs_engine* e, int real, int kind, uint16_t ts, const int32_t* person, const int32_t* other, double* out, int n)
+{
+    if (!e || !person || !out || n < 0 || kind < 0 || kind > 7) return CS_EINVAL;
+    if (real < 0 || real >= e->R) return set_err(e, CS_EINVAL, "realisation %d out of range", real);
+    for (int i = 0; i < n; i++) {
+        if (person[i] < 0 || person[i] >= e->N) return set_err(e, CS_EINVAL, "person index out of range");
+        if (other) {
+            int lim = (kind == 1 || kind == 5) ? e->NPT : e->N;
+            if (other[i] < 0 || other[i] >= lim) return set_err(e, CS_EINVAL, "second index out of range");
+        }
+    }
+    HIPCHK(e, hipSetDevice(e->device));
+    DBuf<int32_t> a, b;
+    DBuf<double> o;
+    TMP_UP(e, a, person, (size_t)n);
+    TMP_UP(e, b, other, (size_t)n);
+    TMP_UP(e, o, (const double*)nullptr, (size_t)n);
+    if (n) cs::k_t_hazards<<<grid_for(n, 256, 1u << 30), 256, 0, e->stream>>>(e->d, real, kind, ts, a.p, other ? b.p : nullptr, o.p, n);
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    HIPCHK(e, hipMemcpy(out, o.p, (size_t)n * 8, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int cs_icdf_choose(cs_engine* e, int table, const double* u, const double* mean, uint16_t* out, int n)
+{
+    if (!e || !u || !mean || !out || n < 0 || table < 0 || table >= CS_NUM_ICDF) return CS_EINVAL;
+    if (!e->have_params) return set_err(e, CS_ESTATE, "no parameters");
+    for (int i = 0; i < n; i++)
+        if (!(u[i] > 0.0 && u[i] < 1.0)) return set_err(e, CS_EINVAL, "uniform outside (0,1)");
+    HIPCHK(e, hipSetDevice(e->device));
+    DBuf<double> a, b;
+    DBuf<uint16_t> o;
+    TMP_UP(e, a, u, (size_t)n);
+    TMP_UP(e, b, mean, (size_t)n);
+    TMP_UP(e, o, (const uint16_t*)nullptr, (size_t)n);
+    if (n) cs::k_t_choose<<<grid_for(n, 256, 1u << 30), 256, 0, e->stream>>>(e->d, table, a.p, b.p, o.p, n);
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    HIPCHK(e, hipMemcpy(out, o.p, (size_t)n * 2, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int cs_select_infector(cs_engine* e, const float* cell_inf, int n_inf, const double* u, int32_t* out, int n)
+{
+    if (!e || !cell_inf || !u || !out || n < 0 || n_inf <= 0) return CS_EINVAL;
+    HIPCHK(e, hipSetDevice(e->device));
+    DBuf<float> c;
+    DBuf<double> a;
+    DBuf<int32_t> o;
+    TMP_UP(e, c, cell_inf, (size_t)n_inf);
+    TMP_UP(e, a, u, (size_t)n);
+    TMP_UP(e, o, (const int32_t*)nullptr, (size_t)n);
+    if (n) cs::k_t_select_infector<<<grid_for(n, 256, 1u << 30), 256, 0, e->stream>>>(c.p, n_inf, a.p, o.p, n);
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    HIPCHK(e, hipMemcpy(out, o.p, (size_t)n * 4, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int cs_select_dest_cell(cs_engine* e, int real, int cell, const double* u, int32_t* out, int n)
+{
+    if (!e || !u || !out || n < 0) return CS_EINVAL;
+    if (real < 0 || real >= e->R || cell < 0 || cell >= e->NC) return set_err(e, CS_EINVAL, "realisation or cell out of range");
+    for (int i = 0; i < n; i++)
+        if (!(u[i] > 0.0 && u[i] < 1.0)) return set_err(e, CS_EINVAL, "uniform outside (0,1)");
+    HIPCHK(e, hipSetDevice(e->device));
+    DBuf<double> a;
+    DBuf<int32_t> o;
+    TMP_UP(e, a, u, (size_t)n);
+    TMP_UP(e, o, (const int32_t*)nullptr, (size_t)n);
+    if (n) cs::k_t_select_dest<<<grid_for(n, 256, 1u << 30), 256, 0, e->stream>>>(e->d, real, cell, a.p, o.p, n);
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    HIPCHK(e, hipMemcpy(out, o.p, (size_t)n * 4, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int cs_philox(cs_engine* e, const uint32_t* ck, uint32_t* out, int n)
+{
+    if (!e || !ck || !out || n < 0) return CS_EINVAL;
+    HIPCHK(e, hipSetDevice(e->device));
+    DBuf<uint32_t> a, o;
+    TMP_UP(e, a, ck, (size_t)n * 6);
+    TMP_UP(e, o, (const uint32_t*)nullptr, (size_t)n * 4);
+    if (n) cs::k_t_philox<<<grid_for(n, 256, 1u << 30), 256, 0, e->stream>>>(a.p, o.p, n);
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    HIPCHK(e, hipMemcpy(out, o.p, (size_t)n * 16, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+// sampler draws for distribution tests: out[i] from stream {key, c0, c1, i}
+int cs_sample(cs_engine* e, int which, uint64_t key, uint32_t c0, uint32_t c1, double a, double b, double* out, int n)
+{
+    if (!e || !out || n < 0 || which < 0 || which > 4) return CS_EINVAL;
+    HIPCHK(e, hipSetDevice(e->device));
+    DBuf<double> o;
+    TMP_UP(e, o, (const double*)nullptr, (size_t)n);
+    if (n) cs::k_t_sample<<<grid_for(n, 256, 1u << 30), 256, 0, e->stream>>>(which, key, c0, c1, a, b, o.p, n);
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    HIPCHK(e, hipMemcpy(out, o.p, (size_t)n * 8, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,605 @@
+// cs_kernels.h -- the device kernels of one infection step.
+//
+// One launch covers every realisation: grids are [realisation][cell] blocks
+// or grid-stride loops over a device-side count, so a whole timestep needs
+// no host round trip and can be captured in a HIP graph.
+//
+// Deterministic by construction (same result for any thread schedule):
+//  * every random decision draws from its own counter-based stream;
+//  * infection events carry key = infector<<32 | pass<<27 | sub; for each
+//    infectee the smallest key performs DoInfect and the largest key's
+//    infector/type remain recorded (src/Sweep.cpp:860-880 processes its
+//    queues in order: first entry infects, later entries overwrite the
+//    recorded infector);
+//  * cell lists S|L|I|R+D are permuted by a batched move: the sorted
+//    positions of the leaving batch below hi-k are paired in order with the
+//    sorted free positions in [hi-k, hi).
+#pragma once
+#include "cs_model.h"
+
+namespace cs {
+
+#define CS_EV_FULL 0xffffffffu
+// bound on the spatial rejection loop (src/Sweep.cpp:753-854 has none)
+#define CS_MAX_ATTEMPTS (1 << 20)
+
+__device__ __forceinline__ uint64_t ev_key(int infector, int pass, uint32_t sub)
+{
+    return ((uint64_t)(uint32_t)infector << 32) | ((uint64_t)pass << 27) | (uint64_t)(sub & 0x7ffffff);
+}
+
+__device__ __forceinline__ void add_event(const Dev& d, uint64_t key, size_t gp_infectee, int infector, int type)
+{
+    unsigned idx = atomicAdd(d.ev_count, 1u);
+    if (idx < d.ev_cap) {
+        d.ev_key[idx] = key;
+        d.ev_gp[idx] = (long long)gp_infectee;
+        d.ev_infector[idx] = infector;
+        d.ev_type[idx] = (int16_t)type;
+    } else {
+        atomicOr(d.status, (unsigned)FLAG_OVERFLOW);
+    }
+}
+
+__device__ __forceinline__ void cadd(unsigned long long* a, long long v) { atomicAdd(a, (unsigned long long)v); }
+
+__device__ __forceinline__ void batch_add(const Dev& d, int r, int c, int p)
+{
+    int idx = atomicAdd(&d.bcount[(size_t)r * d.NC + c], 1);
+    d.batch[(size_t)r * d.N + d.cstart[c] + idx] = p;
+}
+
+// ---------------------------------------------------------------- tables
+
+// max_trans[l][m] = (float)num(dist2_cc_min(l, m)) (src/Kernels.cpp:48-63)
+__global__ void k_build_max_trans(Dev d)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t n = (size_t)d.NC * d.NC;
+    if (i >= n) return;
+    int l = (int)(i / d.NC), m = (int)(i % d.NC);
+    const_cast<float*>(d.max_trans)[i] = (float)kernel_num(d, dist2_cc_min(d.P, d.cidx[l], d.cidx[m]));
+}
+
+// UpdateProbs(0), S0 part (src/CovidSim.cpp:2756-2768); only realisations
+// with sel[r] != 0 (all when sel == nullptr)
+__global__ void k_update_s0(Dev d, const int* sel)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)d.R * d.NC) return;
+    int r = (int)(i / d.NC), j = (int)(i % d.NC);
+    if (sel && !sel[r]) return;
+    d.tot_prob[i] = 0;
+    int s0 = d.cS[i] + d.cL[i] + d.cI[i];
+    if (d.P->do_death) {
+        s0 += d.cn[j] / 5;
+        if ((d.cn[j] < 100) || (s0 > d.cn[j])) s0 = d.cn[j];
+    }
+    d.S0[i] = s0;
+}
+
+// UpdateProbs(0), per-row sequential float prefix (src/CovidSim.cpp:2780-2800).
+// One thread per (realisation, source cell); the row is summed in the
+// reference's order so the float results are identical.
+__global__ void k_update_rows(Dev d, const int* sel)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)d.R * d.NC) return;
+    const int NC = d.NC;
+    int r = (int)(i / NC), j = (int)(i % NC);
+    if (sel && !sel[r]) return;
+    float* ct = d.cum_trans + i * NC;
+    const float* mt = d.max_trans + (size_t)j * NC;
+    const int32_t* S0 = d.S0 + (size_t)r * NC;
+    float t;
+    ct[0] = ((float)S0[0]) * mt[0];
+    t = ((float)d.cn[0]) * mt[0];
+    for (int m = 1; m < NC; m++) {
+        ct[m] = ct[m - 1] + ((float)S0[m]) * mt[m];
+        t += ((float)d.cn[m]) * mt[m];
+    }
+    float tp = ct[NC - 1];
+    for (int m = 0; m < NC; m++) ct[m] /= tp;
+    d.tot_prob[i] = tp / t;
+    int32_t* inv = d.invcdf + i * 1025;
+    int m = 0;
+    for (int k = 0; k <= 1024; k++) {
+        while (ct[m] * 1024 < ((float)k)) m++;
+        inv[k] = m;
+    }
+}
+
+// RunModel's "S fell by more than 0.2" check (src/CovidSim.cpp:1515-1521).
+// It reads State.S, which only RecordSample refreshes (:3163): the S of
+// the last sample, not the current one.
+__global__ void k_sdrop(Dev d, int* sel)
+{
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= d.R) return;
+    double prop = d.samp_prop[r];
+    int up = (d.prev_prop[r] - prop) > 0.2;
+    if (up) d.prev_prop[r] = prop;
+    sel[r] = up;
+}
+
+// ------------------------------------------------------- list movement
+
+#define CS_MOVE_CAP 4096
+
+// Move each cell's batch to the tail of its segment.  mode 0: S->L,
+// 1: L->I, 2: I->R/D.  One block per (realisation, cell).  The batch is
+// paired like this: holes = sorted batch positions below bnd = hi - k,
+// free = sorted positions in [bnd, hi) not holding a batch member; the
+// j-th hole swaps with the j-th free position.
+__global__ void __launch_bounds__(256) k_list_move(Dev d, int mode)
+{
+    const int rc = blockIdx.x;
+    const int r = rc / d.NC, c = rc % d.NC;
+    const int k = d.bcount[rc];
+    if (k == 0) return;
+    __shared__ int32_t pos[CS_MOVE_CAP];
+    __shared__ int32_t fpos[CS_MOVE_CAP];
+    __shared__ uint8_t occ[CS_MOVE_CAP];
+    __shared__ int32_t part[256];
+    __shared__ int dead_count, bad;
+    if (k > CS_MOVE_CAP) {
+        if (threadIdx.x == 0) atomicOr(d.status, (unsigned)FLAG_BATCH);
+        return;
+    }
+    const size_t rN = (size_t)r * d.N;
+    const size_t base = rN + d.cstart[c];
+    const int S = d.cS[rc], L = d.cL[rc], I = d.cI[rc];
+    const int hi = (mode == 0) ? S : (mode == 1 ? S + L : S + L + I);
+    const int bnd = hi - k;
+    if (threadIdx.x == 0) { dead_count = 0; bad = 0; }
+    __syncthreads();
+    for (int i = threadIdx.x; i < k; i += blockDim.x) {
+        pos[i] = d.listpos[rN + d.batch[base + i]];
+        occ[i] = 0;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < k; i += blockDim.x) {
+        if (pos[i] < 0 || pos[i] >= hi) { bad = 1; continue; }
+        if (pos[i] >= bnd) occ[pos[i] - bnd] = 1;
+    }
+    __syncthreads();
+    if (bad) {  // a list position outside the segment: report, touch nothing
+        if (threadIdx.x == 0) atomicOr(d.status, (unsigned)FLAG_BATCH);
+        return;
+    }
+    // free positions of the window in ascending order (chunked exclusive scan)
+    const int per = (k + blockDim.x - 1) / blockDim.x;
+    const int a0 = min(k, (int)threadIdx.x * per), a1 = min(k, a0 + per);
+    int cnt = 0;
+    for (int i = a0; i < a1; i++) cnt += 1 - occ[i];
+    part[threadIdx.x] = cnt;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int run = 0;
+        for (int t = 0; t < (int)blockDim.x; t++) { int v = part[t]; part[t] = run; run += v; }
+    }
+    __syncthreads();
+    {
+        int run = part[threadIdx.x];
+        for (int i = a0; i < a1; i++)
+            if (!occ[i]) fpos[run++] = bnd + i;
+    }
+    __syncthreads();
+    // holes: batch positions below bnd, paired with the free slot of equal rank
+    for (int i = threadIdx.x; i < k; i += blockDim.x) {
+        int ph = pos[i];
+        if (ph >= bnd) continue;
+        int rank = 0;
+        for (int j = 0; j < k; j++) rank += (pos[j] < ph);
+        int q = fpos[rank];
+        int pa = d.perm[base + ph], pb = d.perm[base + q];
+        d.perm[base + ph] = pb;
+        d.listpos[rN + pb] = ph;
+        d.perm[base + q] = pa;
+        d.listpos[rN + pa] = q;
+    }
+    if (mode == 2) {
+        int nd = 0;
+        for (int i = threadIdx.x; i < k; i += blockDim.x) nd += is_dead_code(d.inf[rN + d.batch[base + i]]);
+        if (nd) atomicAdd(&dead_count, nd);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (mode == 0) { d.cS[rc] = S - k; d.cL[rc] = L + k; }
+        else if (mode == 1) { d.cL[rc] = L - k; d.cI[rc] = I + k; }
+        else { d.cI[rc] = I - k; d.cD[rc] += dead_count; d.cR[rc] += k - dead_count; }
+        d.bcount[rc] = 0;
+    }
+}
+
+// ------------------------------------------------------------ DoInfect
+
+// DoInfect (src/Update.cpp:100-195) for infectee p of realisation r, with
+// the first event's infector and type; the list swap is batched.
+__device__ void do_infect(const Dev& d, int r, int p, double t, unsigned ts, int infector, int type)
+{
+    const cs_params* P = d.P;
+    size_t gp = (size_t)r * d.N + p;
+    if (d.inf[gp] != CS_SUSCEPTIBLE) return;
+    Counters* cn = d.cnt + r;
+    d.inf[gp] = CS_LATENT;
+    d.t_inf[gp] = (uint16_t)ts;
+    const float2 loc = reinterpret_cast<const float2*>(d.hloc)[d.hh[p]];
+    double x = (loc.x - d.loc_init[2 * r]);
+    double y = (loc.y - d.loc_init[2 * r + 1]);
+    double rad2 = x * x + y * y;
+    int ag = age_group(d, p);
+    cadd(&cn->cumI, 1);
+    cadd(&cn->cumItype[type % CS_INFECT_TYPE_MASK], 1);
+    cadd(&cn->cumIa[ag], 1);
+    atomicAdd(&cn->sumRad2, rad2);
+    atomicMax(&cn->maxRad2_bits, (unsigned long long)double_to_bits(rad2));
+    cadd(&cn->cumI_keyworker[keyw(d, p)], 1);
+    uint16_t lat;
+    if (P->do_latent) {
+        Stream s(d.seed[r], cw0(ts, PS_INFECT), (uint32_t)p, 0);
+        double q;
+        int i = (int)__builtin_floor((q = s.next() * CS_CDF_RES));
+        q -= ((double)i);
+        const double* c = P->icdf[CS_ICDF_LATENT];
+        lat = (uint16_t)__builtin_floor(0.5 + (t - P->latent_period * clog(q * c[i + 1] + (1.0 - q) * c[i])) * P->time_steps_per_day);
+    } else
+        lat = (uint16_t)(t * P->time_steps_per_day);
+    d.t_lat[gp] = lat;
+    if (infector >= 0) {
+        size_t gi = (size_t)r * d.N + infector;
+        cadd(&cn->cumTG, ((int)ts) - ((int)d.t_inf[gi]));
+        cadd(&cn->cumSI, ((int)lat) - ((int)d.t_lat[gi]));
+        cadd(&cn->nTG, 1);
+    }
+    batch_add(d, r, d.pc[p], p);
+}
+
+// ---------------------------------------------------------- InfectSweep
+
+struct StepScalars {
+    double t, fp, seas, beta_s, beta_h;
+    unsigned ts;
+};
+
+// the reference's infector binary search (src/Sweep.cpp:716-740)
+__device__ __forceinline__ int select_infector(const float* ci, int n, double u)
+{
+    if (n == 1) return 0;
+    int idx, step;
+    idx = step = n / 2;
+    int keep = 1;
+    do {
+        if (step > 1) step /= 2;
+        if ((idx > 0) && (__builtin_fabs((double)ci[idx - 1]) >= u)) {
+            idx -= step;
+            if (idx == 0) keep = 0;
+        } else if ((idx < n - 1) && (__builtin_fabs((double)ci[idx]) < u)) {
+            idx += step;
+            if (idx == n - 1) keep = 0;
+        } else keep = 0;
+    } while (keep);
+    return idx;
+}
+
+// destination cell: InvCDF then linear scan of cum_trans (src/Sweep.cpp:757-762)
+__device__ __forceinline__ int select_dest(const Dev& d, size_t rc, double u)
+{
+    int l = d.invcdf[rc * 1025 + (int)__builtin_floor(u * 1024)];
+    const float* ct = d.cum_trans + rc * d.NC;
+    while (ct[l] < u) l++;
+    return l;
+}
+
+// SampleWithoutReplacement replacement: all / Floyd (k <= 32) / selection
+#define CS_FLOYD_MAX 32
+
+// Household and place passes for one infectious person, and its spatial
+// infectiousness (src/Sweep.cpp:314-690).  One block per (realisation, cell).
+__global__ void __launch_bounds__(64) k_infect_cells(Dev d, StepScalars sc)
+{
+    const int rc = blockIdx.x;
+    const int r = rc / d.NC, c = rc % d.NC;
+    const int S = d.cS[rc], L = d.cL[rc], I = d.cI[rc];
+    if (I == 0) {
+        if (threadIdx.x == 0) d.spK[rc] = 0;
+        return;
+    }
+    const cs_params* P = d.P;
+    const unsigned ts = sc.ts;
+    const uint64_t seed = d.seed[r];
+    const size_t rN = (size_t)r * d.N;
+    const size_t sbase = rN + d.cstart[c] + S + L;  // slot of infectious index 0
+    unsigned long long w_hh_inf = 0, w_hh_mem = 0, w_link = 0, w_cand = 0;
+    for (int j = threadIdx.x; j < I; j += blockDim.x) {
+        const int i = d.perm[sbase + j];
+        const size_t gi = rN + i;
+        const int gen = d.itype[gi] / CS_INFECT_TYPE_MASK;
+        // household (src/Sweep.cpp:331-377)
+        if (sc.beta_h > 0) {
+            const int h = d.hh[i], nh = d.hnh[h];
+            if (nh > 1) {
+                const int f0 = d.hfirst[h];
+                double HI = sc.beta_h * house_inf(d, r, i, ts);
+                bool absent = false;
+                for (int m = f0; m < f0 + nh && !absent; m++)
+                    for (int pt = 0; pt < d.NPT && !absent; pt++) {
+                        int lk = d.plinks[(size_t)m * CS_MAX_PLACE_TYPES + pt];
+                        if (lk >= 0) absent = place_closed(d, r, pt, lk, ts) && host_absent(d, rN + m, ts);
+                    }
+                if (absent) HI *= P->efficacies[2][CS_HOUSE];
+                w_hh_inf++;
+                for (int m = f0; m < f0 + nh; m++) {
+                    w_hh_mem++;
+                    if (d.inf[rN + m] == CS_SUSCEPTIBLE) {
+                        double foi = HI * house_susc(d, r, m, ts, i);
+                        Stream s(seed, cw0(ts, PS_HH), (uint32_t)i, (uint32_t)(m - f0));
+                        if (s.next() < foi)
+                            add_event(d, ev_key(i, KP_HH, (uint32_t)(m - f0)), rN + m, i, 1 + CS_INFECT_TYPE_MASK * (1 + gen));
+                    }
+                }
+            }
+        }
+        // places (src/Sweep.cpp:381-640)
+        if (P->do_places && !host_absent(d, gi, ts)) {
+            for (int pt = 0; pt < d.NPT; pt++) {
+                const int lk = d.plinks[(size_t)i * CS_MAX_PLACE_TYPES + pt];
+                if (lk < 0) continue;
+                w_link++;
+                double PI = sc.fp * sc.seas * place_inf(d, r, i, pt, ts);
+                const int mstart = d.pl_ms[pt][lk];
+                for (int pass = 0; pass < 2; pass++) {
+                    int gsize, goff;
+                    double Pc, Ps;
+                    if (pass == 0) {
+                        if (pt == P->hotel_place_type) continue;
+                        const int g = d.pglinks[(size_t)i * CS_MAX_PLACE_TYPES + pt];
+                        Pc = PI;
+                        if (Pc > 1) Pc = 1;
+                        Ps = Pc;
+                        const int gix = d.pl_gs0[pt][lk] + g;
+                        goff = d.pl_gst[pt][gix];
+                        gsize = d.pl_gsz[pt][gix];
+                    } else {
+                        PI *= P->place_type_prop_between_group_links[pt] * P->place_type_group_size_param1[pt] / ((double)d.pl_n[pt][lk]);
+                        if (PI > 1) PI = 1;
+                        Pc = PI;
+                        Ps = PI;
+                        goff = 0;
+                        gsize = d.pl_n[pt][lk];
+                    }
+                    Stream s(seed, cw0(ts, pass == 0 ? PS_PW_SAMPLE : PS_PB_SAMPLE), (uint32_t)i, (uint32_t)pt);
+                    int n = (Ps >= 1) ? gsize : binomial(s, gsize, Ps);
+                    if (n <= 0) continue;
+                    int picked[CS_FLOYD_MAX];
+                    const bool all = n >= gsize;
+                    const bool floyd = !all && n <= CS_FLOYD_MAX;
+                    if (floyd) {
+                        int mm = 0;
+                        for (int j2 = gsize - n; j2 < gsize; j2++) {
+                            int tt = (int)(s.next() * (double)(j2 + 1));
+                            bool dup = false;
+                            for (int q = 0; q < mm; q++) dup |= (picked[q] == tt);
+                            picked[mm++] = dup ? j2 : tt;
+                        }
+                    }
+                    if (all) n = gsize;
+                    int sel = 0;
+                    for (int cidx = 0, scan = 0; cidx < n; cidx++) {
+                        int idx;
+                        if (all) idx = cidx;
+                        else if (floyd) idx = picked[cidx];
+                        else {
+                            // selection sampling over [scan, gsize)
+                            for (;;) {
+                                double u = s.next();
+                                int cur = scan++;
+                                if ((double)(gsize - cur) * u < (double)(n - sel)) { idx = cur; sel++; break; }
+                            }
+                        }
+                        const int m = d.pl_mem[pt][mstart + goff + idx];
+                        w_cand++;
+                        double PS = place_susc(d, r, m, pt, ts);
+                        if (pass == 1 && care(d, i) && care(d, m) &&
+                            d.pglinks[(size_t)i * CS_MAX_PLACE_TYPES + pt] != d.pglinks[(size_t)m * CS_MAX_PLACE_TYPES + pt])
+                            PS *= P->care_home_resident_place_scaling;
+                        if ((pt == P->care_home_place_type) && ((!care(d, i)) || (!care(d, m)))) PS *= P->care_home_worker_group_scaling;
+                        const size_t gm = rN + m;
+                        if (d.inf[gm] == CS_SUSCEPTIBLE && !host_absent(d, gm, ts)) {
+                            PS *= person_susc(d, r, m, ts, i) * Pc / Ps;
+                            Stream s2(seed, cw0(ts, pass == 0 ? PS_PW_ACCEPT : PS_PB_ACCEPT), (uint32_t)i, ((uint32_t)pt << 24) | (uint32_t)cidx);
+                            if ((PS == 1) || (s2.next() < PS)) {
+                                if (pass == 0)
+                                    add_event(d, ev_key(i, KP_PW + pt, (uint32_t)cidx), gm, i, 2 + pt + CS_INFECT_TYPE_MASK * (1 + gen));
+                                else
+                                    add_event(d, ev_key(i, KP_PB + pt, (uint32_t)cidx), gm, i,
+                                              2 + pt + CS_MAX_PLACE_TYPES + CS_INFECT_TYPE_MASK * (1 + gen));
+                            }
+                        }
+                    }
+                }
+            }
+        }
+        // spatial infectiousness (src/Sweep.cpp:649-690); the closed flag is the sign
+        if (sc.beta_s > 0) {
+            double SI = spatial_inf(d, r, i, ts);
+            bool flag = false;
+            if (P->do_places)
+                for (int pt = 0; pt < d.NPT && !flag; pt++) {
+                    int lk = d.plinks[(size_t)i * CS_MAX_PLACE_TYPES + pt];
+                    if (lk >= 0) flag = place_closed(d, r, pt, lk, ts);
+                }
+            if (flag && host_absent(d, gi, ts)) {
+                SI *= P->efficacies[2][CS_SPATIAL];
+                d.si_buf[sbase + j] = -SI;
+            } else
+                d.si_buf[sbase + j] = SI;
+        }
+    }
+    if (w_hh_inf | w_hh_mem | w_link | w_cand) {
+        atomicAdd(&d.wc->n_hh_inf, w_hh_inf);
+        atomicAdd(&d.wc->n_hh_member, w_hh_mem);
+        atomicAdd(&d.wc->n_place_link, w_link);
+        atomicAdd(&d.wc->n_place_cand, w_cand);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        atomicAdd(&d.wc->n_inf, (unsigned long long)I);
+        int K = 0;
+        if (sc.beta_s > 0) {
+            // running sum in the reference's order (double), stored as float
+            double sis = 0;
+            for (int j = 0; j < I; j++) {
+                double v = d.si_buf[sbase + j];
+                bool neg = __builtin_signbit(v);
+                sis += neg ? -v : v;
+                d.ci_buf[sbase + j] = neg ? (float)-sis : (float)sis;
+            }
+            if (sis > 0) {
+                Stream s(seed, cw0(ts, PS_SP_COUNT), (uint32_t)c, 0);
+                K = poisson(s, sis * sc.beta_s * ((double)d.tot_prob[rc]));
+                if (K > 0) {
+                    for (int j = 0; j < I - 1; j++) d.ci_buf[sbase + j] /= ((float)sis);
+                    d.ci_buf[sbase + I - 1] = (d.ci_buf[sbase + I - 1] < 0) ? -1.0f : 1.0f;
+                }
+            }
+            d.sp_sum[rc] = sis;
+        }
+        d.spK[rc] = K;
+    }
+}
+
+// exclusive scan of spK into sp_off (one block, any length)
+__global__ void __launch_bounds__(1024) k_scan_draws(Dev d)
+{
+    __shared__ long long part[1024];
+    const long long n = (long long)d.R * d.NC;
+    const long long per = (n + blockDim.x - 1) / blockDim.x;
+    const long long a0 = threadIdx.x * per, a1 = min(n, a0 + per);
+    long long s = 0;
+    for (long long i = a0; i < a1; i++) s += d.spK[i];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        long long run = 0;
+        for (int t = 0; t < (int)blockDim.x; t++) { long long v = part[t]; part[t] = run; run += v; }
+        d.sp_off[n] = run;
+    }
+    __syncthreads();
+    long long run = part[threadIdx.x];
+    for (long long i = a0; i < a1; i++) { d.sp_off[i] = run; run += d.spK[i]; }
+}
+
+// one spatial draw per thread (src/Sweep.cpp:712-855)
+__global__ void __launch_bounds__(256) k_spatial_draws(Dev d, StepScalars sc)
+{
+    const long long nrc = (long long)d.R * d.NC;
+    const long long total = d.sp_off[nrc];
+    const cs_params* P = d.P;
+    const unsigned ts = sc.ts;
+    unsigned long long w_draw = 0, w_susc = 0;
+    for (long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += (long long)gridDim.x * blockDim.x) {
+        long long lo = 0, hi = nrc;  // largest rc with sp_off[rc] <= g
+        while (hi - lo > 1) {
+            long long mid = (lo + hi) >> 1;
+            if (d.sp_off[mid] <= g) lo = mid; else hi = mid;
+        }
+        while (lo + 1 < nrc && d.sp_off[lo + 1] <= g) lo++;
+        const size_t rc = (size_t)lo;
+        const int k = (int)(g - d.sp_off[rc]);
+        const int r = (int)(rc / d.NC), c = (int)(rc % d.NC);
+        const int S = d.cS[rc], L = d.cL[rc], I = d.cI[rc];
+        const size_t rN = (size_t)r * d.N;
+        const size_t sbase = rN + d.cstart[c] + S + L;
+        const uint64_t seed = d.seed[r];
+        Stream s(seed, cw0(ts, PS_SP_DRAW), (uint32_t)c, (uint32_t)k);
+        int jj = (I == 1) ? 0 : select_infector(d.ci_buf + sbase, I, s.next());
+        bool closed = (d.ci_buf[sbase + jj] < 0);
+        const int i = d.perm[sbase + jj];
+        for (int att = 0;; att++) {
+            if (att >= CS_MAX_ATTEMPTS) {  // no live candidate: report, never spin
+                atomicOr(d.status, (unsigned)FLAG_STUCK);
+                break;
+            }
+            w_draw++;
+            int l = select_dest(d, rc, s.next());
+            const size_t rl = (size_t)r * d.NC + l;
+            int slot = (int)(s.next() * ((double)d.S0[rl]));
+            int m = d.perm[rN + d.cstart[l] + slot];
+            double d2 = dist2_pp(d, m, i);
+            double acc = kernel_num(d, d2) / d.max_trans[(size_t)c * d.NC + l];
+            if ((s.next() >= acc) || is_dead_code(d.inf[rN + m])) continue;
+            if ((c != l) || (d.hh[m] != d.hh[i])) {
+                double SS = spatial_susc(d, r, m, ts);
+                if (care(d, m) || care(d, i)) SS *= P->care_home_resident_spatial_scaling;
+                if (slot < d.cS[rl]) {
+                    w_susc++;
+                    SS *= person_susc(d, r, m, ts, i);
+                    SS *= P->waifw_spatial[age_group(d, m)][age_group(d, i)];
+                    if (!closed && host_absent(d, rN + m, ts)) {
+                        bool f2 = false;
+                        for (int pt = 0; pt < d.NPT && !f2; pt++) {
+                            int lk = d.plinks[(size_t)m * CS_MAX_PLACE_TYPES + pt];
+                            if (lk >= 0) f2 = place_closed(d, r, pt, lk, ts);
+                        }
+                        if (f2) SS *= P->efficacies[2][CS_SPATIAL];
+                    }
+                    if ((SS == 1) || (s.next() < SS))
+                        if (d.inf[rN + m] == CS_SUSCEPTIBLE)
+                            add_event(d, ev_key(i, KP_SP, (uint32_t)k), rN + m, i,
+                                      2 + 2 * CS_MAX_PLACE_TYPES + CS_INFECT_TYPE_MASK * (1 + d.itype[rN + i] / CS_INFECT_TYPE_MASK));
+                }
+            }
+            break;
+        }
+    }
+    if (w_draw) {
+        atomicAdd(&d.wc->n_sp_draw, w_draw);
+        atomicAdd(&d.wc->n_sp_susc, w_susc);
+    }
+}
+
+__global__ void k_resolve_claim(Dev d)
+{
+    unsigned n = min(*d.ev_count, d.ev_cap);
+    for (unsigned e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+        size_t gp = (size_t)d.ev_gp[e];
+        unsigned long long key = d.ev_key[e];
+        atomicMin(&d.claim[gp], key);
+        atomicMax(&d.lastk[gp], key);
+    }
+}
+
+__global__ void k_resolve_apply(Dev d, StepScalars sc)
+{
+    unsigned n = min(*d.ev_count, d.ev_cap);
+    unsigned long long nn = 0;
+    for (unsigned e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+        size_t gp = (size_t)d.ev_gp[e];
+        unsigned long long key = d.ev_key[e];
+        int r = (int)(gp / d.N), p = (int)(gp % d.N);
+        if (key == d.claim[gp]) {
+            do_infect(d, r, p, sc.t, sc.ts, d.ev_infector[e], d.ev_type[e]);
+            nn++;
+        }
+        if (key == d.lastk[gp]) {
+            d.infector[gp] = d.ev_infector[e];
+            d.itype[gp] = d.ev_type[e];
+        }
+    }
+    if (nn) atomicAdd(&d.wc->n_new, nn);
+}
+
+__global__ void k_resolve_reset(Dev d)
+{
+    unsigned n = min(*d.ev_count, d.ev_cap);
+    for (unsigned e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+        size_t gp = (size_t)d.ev_gp[e];
+        d.claim[gp] = ~0ull;
+        d.lastk[gp] = 0ull;
+    }
+}
+
+__global__ void k_zero_u32(unsigned int* p) { *p = 0; }
+
+}  // namespace cs

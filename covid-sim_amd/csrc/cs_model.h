@@ -1,0 +1,357 @@
+// cs_model.h -- device-side model state and the per-person functions of
+// CovidSim's infection step: kernel lookup (src/Kernels.cpp:108-129),
+// distances (src/Dist.cpp), hazards (src/CalcInfSusc.cpp), state predicates
+// (src/ModelMacros.h), InverseCdf::choose (src/InverseCdf.cpp:60-74).
+//
+// Layout in HBM.  Static population arrays are shared by all realisations;
+// every dynamic per-person array is [n_real][n_people] (realisation-major),
+// per-cell arrays [n_real][n_cells], per-place arrays [n_real][n_places].
+// A global person index gp = r * n_people + p addresses the dynamic arrays.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/covidsim_hip.h"
+#include "cs_math.h"
+
+namespace cs {
+
+struct Counters {  // device mirror of cs_counters, updated with atomics
+    unsigned long long cumI, cumC, cumDC, cumFC, cumTG, cumSI, nTG;
+    unsigned long long cumAC, cumACS, cumAA;
+    double sumRad2;
+    unsigned long long maxRad2_bits;
+    unsigned long long cumItype[CS_INFECT_TYPE_MASK];
+    unsigned long long cumIa[CS_NUM_AGE_GROUPS], cumCa[CS_NUM_AGE_GROUPS], cumDa[CS_NUM_AGE_GROUPS];
+    unsigned long long cumI_keyworker[2], cumC_keyworker[2];
+    unsigned long long sev_prev[CS_NUM_SEV_KINDS], sev_cum[CS_NUM_SEV_KINDS];
+    unsigned long long sev_prev_age[CS_NUM_SEV_KINDS][CS_NUM_AGE_GROUPS];
+    unsigned long long sev_cum_age[CS_NUM_SEV_KINDS][CS_NUM_AGE_GROUPS];
+};
+
+struct Work {  // event tallies for the algorithmic-bytes figure
+    unsigned long long n_inf, n_hh_inf, n_hh_member, n_place_link, n_place_cand;
+    unsigned long long n_sp_draw, n_sp_susc, n_new, n_active, n_transition, n_cells;
+};
+
+enum { FLAG_OVERFLOW = 1, FLAG_RANGE = 2, FLAG_BATCH = 4, FLAG_STUCK = 8 };
+
+struct Dev {
+    int N, NH, NC, NM, NPT, R;
+    const cs_params* P;
+    const double* kl;
+    const double* kh;
+    // population (static)
+    const int32_t *hh, *pc, *mcell, *plinks, *hfirst, *cstart, *cn, *cidx, *mstart, *mmem, *cmem;
+    const uint8_t *age, *pflags;
+    const uint16_t *pglinks, *hnh, *hnhr;
+    const float* hloc;
+    int np[CS_MAX_PLACE_TYPES];
+    int np_off[CS_MAX_PLACE_TYPES + 1];  // offsets of each type in the per-realisation place arrays
+    const int32_t *pl_n[CS_MAX_PLACE_TYPES], *pl_ms[CS_MAX_PLACE_TYPES], *pl_mem[CS_MAX_PLACE_TYPES];
+    const int32_t *pl_gs0[CS_MAX_PLACE_TYPES], *pl_gst[CS_MAX_PLACE_TYPES], *pl_gsz[CS_MAX_PLACE_TYPES];
+    const float* max_trans;
+    // dynamic, per realisation
+    const uint64_t* seed;
+    int8_t* inf;
+    uint8_t *sev_cur, *sev_final, *bits, *qc, *dct;
+    int32_t *listpos, *infector;
+    int16_t* itype;
+    float *infness, *susc, *prob_care;
+    uint16_t *t_inf, *t_lat, *t_rod, *t_sari, *t_crit, *t_step, *t_det, *t_as, *t_ae;
+    uint16_t *t_ts, *t_te, *t_vs, *t_is, *t_qs;
+    int32_t *cS, *cL, *cI, *cR, *cD, *cumTC, *S0;
+    float* tot_prob;
+    float* cum_trans;
+    int32_t* invcdf;
+    int32_t* perm;
+    uint16_t *pl_cs, *pl_ce;  // [R][total places]
+    uint8_t *m_socdist, *m_moverest;
+    uint32_t* m_trig;
+    Counters* cnt;
+    double* prev_prop;
+    double* samp_prop;  // State.S / PopSize as of the last sample, per realisation
+    double* loc_init;  // [R][2]
+    // step scratch
+    double* si_buf;       // [R][N] spatial infectiousness by permutation slot (sign = closed flag)
+    float* ci_buf;        // [R][N] normalised cumulative infectiousness by slot
+    int32_t* spK;         // [R*NC] draws per source cell
+    double* sp_sum;       // [R*NC]
+    long long* sp_off;    // [R*NC+1] exclusive scan of spK
+    uint64_t* ev_key;
+    long long* ev_gp;
+    int32_t* ev_infector;
+    int16_t* ev_type;
+    unsigned int* ev_count;
+    unsigned int ev_cap;
+    unsigned long long *claim, *lastk;  // [R][N]
+    int32_t* batch;       // [R][N] by permutation slot range of the cell
+    int32_t* bcount;      // [R*NC]
+    int32_t *care_gp, *care_k;
+    uint8_t* care_abs;
+    unsigned int* care_count;
+    unsigned int* status;
+    Work* wc;
+    int32_t* ts_buf;      // [R][max_samples][CS_TS_FIELDS]
+    int ts_max;
+    long long* last_tot;  // [R][3] cumI, cumC, D at the previous sample
+};
+
+// ---- predicates (src/ModelMacros.h:6-23, src/Person.cpp) ----
+__device__ __forceinline__ int age_group(const Dev& d, int p) { return (int)d.age[p] / CS_AGE_GROUP_WIDTH; }
+__device__ __forceinline__ bool care(const Dev& d, int p) { return (d.pflags[p] & 1) != 0; }
+__device__ __forceinline__ int keyw(const Dev& d, int p) { return (d.pflags[p] >> 1) & 1; }
+__device__ __forceinline__ bool esoc(const Dev& d, int p) { return (d.pflags[p] & 4) != 0; }
+__device__ __forceinline__ bool is_dead_code(int8_t s) { return s == CS_DEAD_ASYMP || s == CS_DEAD_SYMP; }
+__device__ __forceinline__ bool host_absent(const Dev& d, size_t gp, unsigned ts) { return d.t_as[gp] <= ts && d.t_ae[gp] > ts; }
+__device__ __forceinline__ bool place_closed(const Dev& d, int r, int t, int k, unsigned ts)
+{
+    size_t i = (size_t)r * d.np_off[CS_MAX_PLACE_TYPES] + d.np_off[t] + k;
+    return d.pl_cs[i] <= ts && d.pl_ce[i] > ts;
+}
+__device__ __forceinline__ bool treated(const Dev& d, size_t gp, unsigned ts)
+{
+    return d.P->interventions_active && d.t_te[gp] > ts && d.t_ts[gp] <= ts;
+}
+__device__ __forceinline__ bool vacced(const Dev& d, size_t gp, unsigned ts)
+{
+    return d.P->interventions_active && (int)d.t_vs[gp] + (int)d.P->us_vacc_time_to_efficacy <= (int)ts;
+}
+__device__ __forceinline__ bool vacced_switch(const Dev& d, size_t gp) { return d.t_vs[gp] >= d.P->us_vacc_time_efficacy_switch; }
+__device__ __forceinline__ bool quarantined(const Dev& d, size_t gp, unsigned ts)
+{
+    return d.P->interventions_active && d.qc[gp] == 1 &&
+           (int)d.t_qs[gp] + (int)d.P->us_hquarantine_house_duration > (int)ts && d.t_qs[gp] <= ts;
+}
+__device__ __forceinline__ bool isolated(const Dev& d, size_t gp, unsigned ts)
+{
+    const cs_params* P = d.P;
+    return P->interventions_active && (int)d.t_is[gp] + (int)P->us_case_isolation_delay <= (int)ts &&
+           (int)d.t_is[gp] + (int)P->us_case_isolation_delay + (int)P->us_case_isolation_duration > (int)ts;
+}
+__device__ __forceinline__ bool dct_traced(const Dev& d, size_t gp) { return d.P->interventions_active && d.dct[gp] == 1; }
+__device__ __forceinline__ bool socdist(const Dev& d, int r, int p)
+{
+    return d.P->interventions_active && d.m_socdist[(size_t)r * d.NM + d.mcell[p]] == 2;
+}
+
+// ---- kernel lookup (src/Kernels.cpp:108-129) ----
+__device__ __forceinline__ double kernel_num(const Dev& d, double r2)
+{
+    const cs_params* P = d.P;
+    double t = r2 / P->kernel_delta;
+    const int sz = P->kernel_size;
+    if (t > (double)sz) {
+        atomicOr(d.status, (unsigned)FLAG_RANGE);
+        return 0.0;
+    }
+    double s = t * (double)P->kernel_expansion;
+    if (s < (double)sz) {
+        t = s - __builtin_floor(s);
+        int i0 = (int)s, i1 = (int)(s + 1.0);
+        if (i1 > sz) i1 = sz;
+        t = (1.0 - t) * d.kh[i0] + t * d.kh[i1];
+    } else {
+        s = t - __builtin_floor(t);
+        int i0 = (int)t, i1 = (int)(t + 1.0);
+        if (i1 > sz) i1 = sz;
+        t = (1.0 - s) * d.kl[i0] + s * d.kl[i1];
+    }
+    return t;
+}
+
+// ---- distances (src/Dist.cpp:14-168) ----
+__device__ __forceinline__ double periodic_xy(const cs_params* P, double x, double y)
+{
+    if (P->do_periodic_boundaries) {
+        if (x > P->degrees_width * 0.5) x = P->degrees_width - x;
+        if (y > P->degrees_height * 0.5) y = P->degrees_height - y;
+    }
+    return x * x + y * y;
+}
+
+__device__ __noinline__ double dist2_utm(const cs_params* P, double x1, double y1, double x2, double y2)
+{
+    double x, y, cy1, cy2, yt, xi, yi;
+    x = __builtin_fabs(x1 - x2) / 2;
+    y = __builtin_fabs(y1 - y2) / 2;
+    xi = __builtin_floor(x);
+    yi = __builtin_floor(y);
+    x -= xi;
+    y -= yi;
+    x = (1 - x) * P->sinx[(int)xi] + x * P->sinx[((int)xi) + 1];
+    y = (1 - y) * P->sinx[(int)yi] + y * P->sinx[((int)yi) + 1];
+    yt = __builtin_fabs(y1 + P->bbox_bottom_left_y);
+    yi = __builtin_floor(yt);
+    cy1 = yt - yi;
+    cy1 = (1 - cy1) * P->cosx[((int)yi)] + cy1 * P->cosx[((int)yi) + 1];
+    yt = __builtin_fabs(y2 + P->bbox_bottom_left_y);
+    yi = __builtin_floor(yt);
+    cy2 = yt - yi;
+    cy2 = (1 - cy2) * P->cosx[((int)yi)] + cy2 * P->cosx[((int)yi) + 1];
+    x = __builtin_fabs(1000 * (y * y + x * x * cy1 * cy2));
+    xi = __builtin_floor(x);
+    x -= xi;
+    y = (1 - x) * P->asin2sqx[((int)xi)] + x * P->asin2sqx[((int)xi) + 1];
+    const double er = ((double)(1852 * 60 * 360) / 3.1415926535) / 2;
+    return 4 * er * er * y;
+}
+
+__device__ __forceinline__ double dist2_pp(const Dev& d, int a, int b)
+{
+    const cs_params* P = d.P;
+    const float2 la = reinterpret_cast<const float2*>(d.hloc)[d.hh[a]];
+    const float2 lb = reinterpret_cast<const float2*>(d.hloc)[d.hh[b]];
+    if (P->do_utm_coords) return dist2_utm(P, la.x, la.y, lb.x, lb.y);
+    double x = __builtin_fabs((double)la.x - (double)lb.x);
+    double y = __builtin_fabs((double)la.y - (double)lb.y);
+    return periodic_xy(P, x, y);
+}
+
+__device__ __noinline__ double dist2_cc_min(const cs_params* P, int l, int m)
+{
+    int nch = P->nch, i = l, j = m;
+    if (P->do_utm_coords) {
+        const double PI_ = 3.1415926535;
+        if (P->cell_width * ((double)abs(m / nch - l / nch)) > PI_) {
+            if (m / nch > l / nch) j += nch;
+            else if (m / nch < l / nch) i += nch;
+        } else {
+            if (m / nch > l / nch) i += nch;
+            else if (m / nch < l / nch) j += nch;
+        }
+        if (m % nch > l % nch) i++;
+        else if (m % nch < l % nch) j++;
+        return dist2_utm(P, P->cell_width * __builtin_fabs((double)(i / nch)), P->cell_height * __builtin_fabs((double)(i % nch)),
+                         P->cell_width * __builtin_fabs((double)(j / nch)), P->cell_height * __builtin_fabs((double)(j % nch)));
+    }
+    if (P->do_periodic_boundaries && (P->cell_width * ((double)abs(m / nch - l / nch)) > P->degrees_width * 0.5)) {
+        if (m / nch > l / nch) j += nch;
+        else if (m / nch < l / nch) i += nch;
+    } else {
+        if (m / nch > l / nch) i += nch;
+        else if (m / nch < l / nch) j += nch;
+    }
+    if (P->do_periodic_boundaries && (P->degrees_height * ((double)abs(m % nch - l % nch)) > P->degrees_height * 0.5)) {
+        if (m % nch > l % nch) j++;
+        else if (m % nch < l % nch) i++;
+    } else {
+        if (m % nch > l % nch) i++;
+        else if (m % nch < l % nch) j++;
+    }
+    double x = P->cell_width * __builtin_fabs((double)(i / nch - j / nch));
+    double y = P->cell_height * __builtin_fabs((double)(i % nch - j % nch));
+    return periodic_xy(P, x, y);
+}
+
+// ---- InverseCdf::choose for a given uniform (src/InverseCdf.cpp:60-74) ----
+__device__ __forceinline__ uint16_t icdf_choose(const double* c, double mean, double u, double tspd)
+{
+    double q;
+    int i = (int)__builtin_floor(q = u * CS_CDF_RES);
+    q -= (double)i;
+    double ti = -mean * clog(q * c[i + 1] + (1.0 - q) * c[i]);
+    return (uint16_t)__builtin_floor(0.5 + (ti * tspd));
+}
+
+// ---- hazards (src/CalcInfSusc.cpp:12-80); r = realisation, p = person ----
+__device__ __forceinline__ double inf_profile(const Dev& d, size_t gp, unsigned ts)
+{
+    int idx = (int)ts - (int)d.t_lat[gp] - 1;
+    idx = idx < 0 ? 0 : (idx >= CS_MAX_INFECTIOUS_STEPS ? CS_MAX_INFECTIOUS_STEPS - 1 : idx);
+    return d.P->infectiousness[idx];
+}
+
+__device__ __forceinline__ double person_inf(const Dev& d, size_t gp, unsigned ts)
+{
+    const cs_params* P = d.P;
+    return (treated(d, gp, ts) ? P->treat_inf_drop : 1.0)
+         * (vacced(d, gp, ts) ? P->vacc_inf_drop : 1.0)
+         * __builtin_fabs((double)d.infness[gp])
+         * inf_profile(d, gp, ts);
+}
+
+__device__ __forceinline__ double house_inf(const Dev& d, int r, int p, unsigned ts)
+{
+    const cs_params* P = d.P;
+    size_t gp = (size_t)r * d.N + p;
+    bool iso = isolated(d, gp, ts), dc = dct_traced(d, gp), qu = quarantined(d, gp, ts);
+    return ((iso && !dc) ? P->efficacies[0][CS_HOUSE] : 1.0)
+         * (dc ? P->efficacies[5][CS_HOUSE] : 1.0)
+         * ((qu && !dc && !iso) ? P->efficacies[1][CS_HOUSE] : 1.0)
+         * P->household_denom_lookup[d.hnhr[d.hh[p]] - 1]
+         * (care(d, p) ? P->care_home_resident_household_scaling : 1.0)
+         * (treated(d, gp, ts) ? P->treat_inf_drop : 1.0)
+         * (vacced(d, gp, ts) ? P->vacc_inf_drop : 1.0)
+         * (P->no_infectiousness_sd_in_hh ? ((d.infness[gp] < 0) ? P->sympt_infectiousness : P->asympt_infectiousness)
+                                          : __builtin_fabs((double)d.infness[gp]))
+         * inf_profile(d, gp, ts);
+}
+
+__device__ __forceinline__ double place_inf(const Dev& d, int r, int p, int pt, unsigned ts)
+{
+    const cs_params* P = d.P;
+    size_t gp = (size_t)r * d.N + p;
+    bool iso = isolated(d, gp, ts), dc = dct_traced(d, gp), qu = quarantined(d, gp, ts);
+    return ((iso && !dc) ? P->efficacies[0][pt] : 1.0)
+         * (dc ? P->efficacies[5][pt] : 1.0)
+         * ((qu && !care(d, p) && !dc && !iso) ? P->efficacies[1][pt] : 1.0)
+         * ((d.inf[gp] == CS_CASE && !care(d, p)) ? P->sympt_place_type_contact_rate[pt] : 1.0)
+         * P->place_type_trans[pt] / P->place_type_group_size_param1[pt] * person_inf(d, gp, ts);
+}
+
+__device__ __forceinline__ double spatial_inf(const Dev& d, int r, int p, unsigned ts)
+{
+    const cs_params* P = d.P;
+    size_t gp = (size_t)r * d.N + p;
+    bool iso = isolated(d, gp, ts), dc = dct_traced(d, gp), qu = quarantined(d, gp, ts);
+    return ((iso && !dc) ? P->efficacies[0][CS_SPATIAL] : 1.0)
+         * (dc ? P->efficacies[5][CS_SPATIAL] : 1.0)
+         * ((qu && !care(d, p) && !dc && !iso) ? P->efficacies[1][CS_SPATIAL] : 1.0)
+         * ((d.inf[gp] == CS_CASE) ? P->sympt_spatial_contact_rate : 1.0)
+         * P->relative_spatial_contact[age_group(d, p)]
+         * person_inf(d, gp, ts);
+}
+
+__device__ __forceinline__ double person_susc(const Dev& d, int r, int p, unsigned ts, int infector)
+{
+    const cs_params* P = d.P;
+    size_t gp = (size_t)r * d.N + p;
+    return P->waifw[age_group(d, p)][age_group(d, infector)]
+         * P->age_susceptibility[age_group(d, p)] * (double)d.susc[gp]
+         * (treated(d, gp, ts) ? P->treat_susc_drop : 1.0)
+         * (vacced(d, gp, ts) ? (vacced_switch(d, gp) ? P->vacc_susc_drop2 : P->vacc_susc_drop) : 1.0);
+}
+
+__device__ __forceinline__ double house_susc(const Dev& d, int r, int p, unsigned ts, int infector)
+{
+    const cs_params* P = d.P;
+    size_t gp = (size_t)r * d.N + p;
+    return person_susc(d, r, p, ts, infector)
+         * (socdist(d, r, p) ? (esoc(d, p) ? P->efficacies[4][CS_HOUSE] : P->efficacies[3][CS_HOUSE]) : 1.0)
+         * (dct_traced(d, gp) ? P->efficacies[5][CS_HOUSE] : 1.0)
+         * (care(d, p) ? P->care_home_resident_household_scaling : 1.0);
+}
+
+__device__ __forceinline__ double place_susc(const Dev& d, int r, int p, int pt, unsigned ts)
+{
+    const cs_params* P = d.P;
+    size_t gp = (size_t)r * d.N + p;
+    bool dc = dct_traced(d, gp);
+    return ((quarantined(d, gp, ts) && !care(d, p) && !dc) ? P->efficacies[1][pt] : 1.0)
+         * (socdist(d, r, p) ? (esoc(d, p) ? P->efficacies[4][pt] : P->efficacies[3][pt]) : 1.0)
+         * (dc ? P->efficacies[5][pt] : 1.0);
+}
+
+__device__ __forceinline__ double spatial_susc(const Dev& d, int r, int p, unsigned ts)
+{
+    const cs_params* P = d.P;
+    size_t gp = (size_t)r * d.N + p;
+    bool dc = dct_traced(d, gp);
+    return ((quarantined(d, gp, ts) && !care(d, p) && !dc) ? P->efficacies[1][CS_SPATIAL] : 1.0)
+         * (socdist(d, r, p) ? (esoc(d, p) ? P->efficacies[4][CS_SPATIAL] : P->efficacies[3][CS_SPATIAL]) : 1.0)
+         * (dc ? P->efficacies[5][CS_SPATIAL] : 1.0)
+         * P->relative_spatial_contact_susc[age_group(d, p)];
+}
+
+}  // namespace cs

@@ -335,8 +335,9 @@ int  cs_incub_recovery_sweep(cs_engine* e, double t);
 int  cs_update_probs(cs_engine* e);
 /* RunModel's day loop without host round trips: n_steps timesteps of
  * InfectSweep + IncubRecoverySweep + the S-drop UpdateProbs check
- * (src/CovidSim.cpp:1426-1522), recording daily totals every
- * steps_per_sample steps.  The loop is captured once into a HIP graph. */
+ * (src/CovidSim.cpp:1426-1522).  steps_per_sample (> 0) is the number of
+ * timesteps per output step: totals are recorded after each output step,
+ * and, as in RunModel, the S-drop check reads S as of the last sample. */
 int  cs_run_steps(cs_engine* e, double t0, int n_steps, int steps_per_sample);
 
 /* results */
@@ -346,6 +347,15 @@ int  cs_get_counters(cs_engine* e, int real, cs_counters* out, int reset);
 int  cs_get_timeseries(cs_engine* e, int32_t* out, int max_samples, int* n_samples);
 int  cs_get_work_counts(cs_engine* e, cs_work_counts* out, int reset);
 int  cs_sync(cs_engine* e);
+
+/* per-kernel device time, measured with HIP events recorded around each
+ * launch on the engine's stream while profiling is enabled */
+enum { CS_K_INFECT_CELLS = 0, CS_K_SPATIAL_DRAWS, CS_K_RESOLVE, CS_K_LIST_MOVE,
+       CS_K_LATENT_SCAN, CS_K_INFECTIOUS_SCAN, CS_K_UPDATE_PROBS, CS_K_OTHER, CS_NUM_KERNEL_KINDS };
+int  cs_set_profiling(cs_engine* e, int enable);
+/* ms[k] total milliseconds and launches[k] launch count of kernel kind k
+ * since profiling was last enabled or reset */
+int  cs_get_profile(cs_engine* e, double* ms, int64_t* launches, int reset);
 
 /* ---- direct evaluation of path components (parity tests) -------------- */
 /* KernelLookup::num for n squared distances (src/Kernels.cpp:108-129) */
@@ -371,8 +381,14 @@ int  cs_icdf_choose(cs_engine* e, int table, const double* u, const double* mean
 int  cs_select_infector(cs_engine* e, const float* cell_inf, int n_inf, const double* u,
                         int32_t* out, int n);
 int  cs_select_dest_cell(cs_engine* e, int real, int cell, const double* u, int32_t* out, int n);
-/* raw counter-based RNG output (Philox4x32-10) for known-answer tests */
+/* raw counter-based RNG output (Philox4x32-10) for known-answer tests:
+ * ctr_key holds n records {c0, c1, c2, c3, k0, k1}, out n * 4 words */
 int  cs_philox(cs_engine* e, const uint32_t* ctr_key, uint32_t* out, int n);
+/* sampler draws for distribution tests: out[i] is one draw from the stream
+ * {key, c0, c1, i}; which: 0 uniform, 1 Binomial(a, b), 2 Poisson(a),
+ * 3 standard normal, 4 Gamma(rate a, shape b) */
+int  cs_sample(cs_engine* e, int which, uint64_t key, uint32_t c0, uint32_t c1,
+               double a, double b, double* out, int n);
 
 #ifdef __cplusplus
 }

@@ -168,6 +168,10 @@ static double rs_next(rs* s)
     return s->buf[s->idx++];
 }
 
+/* bound on the spatial rejection loop (src/Sweep.cpp:753-854 has none; a
+ * cell whose first S0 slots hold only dead persons would loop for ever) */
+#define OR_MAX_ATTEMPTS (1 << 20)
+
 /* pass identifiers (high half of counter word 0) */
 enum { PS_HH = 1, PS_PW_SAMPLE, PS_PW_ACCEPT, PS_PB_SAMPLE, PS_PB_ACCEPT, PS_SP_COUNT,
        PS_SP_DRAW, PS_INFECT, PS_INCUB, PS_CASE, PS_RECOVER, PS_HOLIDAY, PS_INIT, PS_SEED };
@@ -368,6 +372,7 @@ typedef struct {
     uint32_t* m_trig; /* [n_mcells][6] */
     cs_counters cnt;
     double prev_prop_susc;
+    double samp_prop_susc; /* State.S / PopSize as of the last sample */
     double loc_init[2];
 } or_real;
 
@@ -1205,7 +1210,8 @@ int or_infect_sweep(or_engine* e, double t)
                     int jj = (I == 1) ? 0 : select_infector(e->cell_inf, I, rs_next(&d));
                     int closed = (e->cell_inf[jj] < 0);
                     const int i = r->perm[base + S + L + jj];
-                    for (;;) {
+                    for (int att = 0;; att++) {
+                        if (att >= OR_MAX_ATTEMPTS) { e->error = CS_ERANGE; snprintf(e->err, sizeof(e->err), "spatial draw found no live candidate"); break; }
                         e->wc.n_sp_draw++;
                         int l = select_dest(e, r, c, rs_next(&d));
                         int slot = (int)(rs_next(&d) * ((double)r->S0[l]));
@@ -1711,9 +1717,11 @@ static void totals(const or_engine* e, const or_real* r, int64_t* S, int64_t* L,
 }
 
 /* RunModel's timestep loop (src/CovidSim.cpp:1426-1522) for the two sweeps,
- * with the S-drop UpdateProbs check, sampling every steps_per_sample steps */
+ * with the S-drop UpdateProbs check; sps = timesteps per output step (the
+ * sampling interval, NumModelTimeStepsPerOutputTimeStep) */
 int or_run_steps(or_engine* e, double t0, int n_steps, int sps)
 {
+    if (sps <= 0) return fail(e, CS_EINVAL, "steps per sample must be positive");
     double t = t0;
     int ns = n_steps / (sps > 0 ? sps : 1);
     free(e->ts_buf);
@@ -1727,6 +1735,8 @@ int or_run_steps(or_engine* e, double t0, int n_steps, int sps)
         int64_t S, L, I, R, D;
         totals(e, &e->R[ri], &S, &L, &I, &R, &D);
         lastI[ri] = e->R[ri].cnt.cumI; lastC[ri] = e->R[ri].cnt.cumC; lastD[ri] = D;
+        /* RunModel samples before its first timestep (src/CovidSim.cpp:1413) */
+        e->R[ri].samp_prop_susc = (double)S / (double)e->N;
     }
     for (int st = 0; st < n_steps; st++) {
         int rc = or_infect_sweep(e, t);
@@ -1734,19 +1744,23 @@ int or_run_steps(or_engine* e, double t0, int n_steps, int sps)
         rc = or_incub_recovery_sweep(e, t);
         if (rc) return rc;
         t += e->P.model_time_step;
+        /* the S-drop test reads State.S, which only RecordSample refreshes
+         * (src/CovidSim.cpp:1515, :3163), i.e. S as of the last sample */
         for (int ri = 0; ri < e->nreal; ri++) {
             or_real* r = &e->R[ri];
-            double prop = (double)sum_S(e, r) / (double)e->N;
+            double prop = r->samp_prop_susc;
             if ((r->prev_prop_susc - prop) > 0.2) {
                 r->prev_prop_susc = prop;
                 update_probs_real(e, r);
             }
         }
-        if (sps > 0 && (st + 1) % sps == 0 && e->ts_n < ns) {
+        if ((st + 1) % sps == 0) {
             for (int ri = 0; ri < e->nreal; ri++) {
                 or_real* r = &e->R[ri];
                 int64_t S, L, I, R, D;
                 totals(e, r, &S, &L, &I, &R, &D);
+                r->samp_prop_susc = (double)S / (double)e->N;
+                if (e->ts_n >= ns) continue;
                 int32_t* o = e->ts_buf + ((size_t)ri * ns + e->ts_n) * CS_TS_FIELDS;
                 o[0] = (int32_t)S; o[1] = (int32_t)L; o[2] = (int32_t)I; o[3] = (int32_t)R; o[4] = (int32_t)D;
                 o[5] = (int32_t)(r->cnt.cumI - lastI[ri]);
@@ -1754,7 +1768,7 @@ int or_run_steps(or_engine* e, double t0, int n_steps, int sps)
                 o[7] = (int32_t)(D - lastD[ri]);
                 lastI[ri] = r->cnt.cumI; lastC[ri] = r->cnt.cumC; lastD[ri] = D;
             }
-            e->ts_n++;
+            if (e->ts_n < ns) e->ts_n++;
         }
     }
     free(lastI); free(lastC); free(lastD);

@@ -16,8 +16,8 @@
 //                       eight hazards of src/CalcInfSusc.cpp over a grid of
 //                       synthetic person states
 //   CS_SERIES=<file>    append one record per step and realisation:
-//                       run, TimeStepNow, S, L, I, R, D, cumI, cumC, cumD
-//                       (int32 each), read after IncubRecoverySweep
+//                       run, TimeStepNow, S, L, I, R, D, cumI, cumC, cumD,
+//                       cumItype[16] (int32 each), read after IncubRecoverySweep
 //
 // Output arrays are .npy files; cs_params is written as its raw bytes
 // (params.bin) with the layout of include/covidsim_hip.h.
@@ -609,7 +609,7 @@ void IncubRecoverySweep(double t)
 {
     IncubRecoverySweep_cpu(t);
     if (g_series) {
-        int32_t rec[10];
+        int32_t rec[10 + INFECT_TYPE_MASK];
         int S = 0, L = 0, I = 0, R = 0, D = 0;
         for (int j = 0; j < P.NumPopulatedCells; j++) {
             S += CellLookup[j]->S; L += CellLookup[j]->L; I += CellLookup[j]->I;
@@ -619,6 +619,21 @@ void IncubRecoverySweep(double t)
         for (int k = 0; k < P.NumThreads; k++) { cumI += StateT[k].cumI; cumC += StateT[k].cumC; }
         rec[0] = g_run_of_call; rec[1] = (int32_t)(unsigned short)(P.TimeStepsPerDay * t);
         rec[2] = S; rec[3] = L; rec[4] = I; rec[5] = R; rec[6] = D; rec[7] = cumI; rec[8] = cumC; rec[9] = D;
+        // StateT.cumItype is cleared by RecordSample before the first
+        // timestep of each output step (src/CovidSim.cpp:3420-3421)
+        static int itype_prev[INFECT_TYPE_MASK], itype_acc[INFECT_TYPE_MASK], acc_run = -1;
+        if (acc_run != g_run_of_call) {
+            acc_run = g_run_of_call;
+            for (int q = 0; q < INFECT_TYPE_MASK; q++) itype_prev[q] = itype_acc[q] = 0;
+        }
+        const bool cleared = (rec[1] % P.NumModelTimeStepsPerOutputTimeStep) == 0;
+        for (int q = 0; q < INFECT_TYPE_MASK; q++) {
+            int cur = 0;
+            for (int k = 0; k < P.NumThreads; k++) cur += StateT[k].cumItype[q];
+            itype_acc[q] += cleared ? cur : cur - itype_prev[q];
+            itype_prev[q] = cur;
+            rec[10 + q] = itype_acc[q];
+        }
         fwrite(rec, sizeof(rec), 1, g_series);
         fflush(g_series);
     }

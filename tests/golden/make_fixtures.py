@@ -9,7 +9,13 @@ from /root/reference) on the 20k-person fixture region and packs:
   mini_params.bin       cs_params of that run (include/covidsim_hip.h)
   mini_hz_params.bin    cs_params with the perturbed intervention
                         parameters the hazard vectors were evaluated under
-Everything here is data: inputs and the reference's outputs.
+  ../../covid-sim_amd/inputs/{c1,mini}/params.bin
+                        the derived parameter sets (cs_params) the
+                        reference's ReadParams produces for those inputs
+Everything here is data: inputs and the reference's outputs.  The
+reference's set-up draws the population with per-thread random streams, so
+its population depends on the thread count (/c:); every fixture and
+ensemble here is made with /c:1 so that they all share one population.
 """
 import os
 import shutil
@@ -46,6 +52,15 @@ def run_harness(inputs, out, steps="0", golden=True, threads=1, extra=(), series
 
 
 def main():
+    # derived parameter sets of both workloads (cs_params of include/covidsim_hip.h)
+    for name in ("c1", "mini"):
+        inputs = os.path.join(ROOT, "covid-sim_amd", "inputs", name)
+        tmp = tempfile.mkdtemp()
+        try:
+            run_harness(inputs, tmp, steps="", golden=False, threads=1)
+            shutil.copy(os.path.join(tmp, "params.bin"), os.path.join(inputs, "params.bin"))
+        finally:
+            shutil.rmtree(tmp)
     inputs = os.path.join(ROOT, "covid-sim_amd", "inputs", "mini")
     tmp = tempfile.mkdtemp()
     try:

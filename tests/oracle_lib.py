@@ -68,3 +68,120 @@ class Oracle:
 
     def call(self, name, *args):
         self.check(getattr(self.lib, "or_" + name)(self.h, *args))
+
+
+def _p(a, t):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+class OracleEngine(Oracle):
+    """The oracle behind the same method names as the package's Engine."""
+
+    def alloc(self, seeds):
+        seeds = np.ascontiguousarray(seeds, dtype=np.uint64)
+        self.n_real = len(seeds)
+        self.call("alloc_realisations", len(seeds), _p(seeds, C.c_uint64))
+
+    def reset(self, n_seed):
+        self.call("reset_realisations", int(n_seed))
+
+    def load_state(self, r, st):
+        self.call("load_state", int(r), C.byref(st.view()))
+
+    def store_state(self, r, st):
+        self.call("store_state", int(r), C.byref(st.view()))
+
+    def infect_sweep(self, t):
+        self.call("infect_sweep", float(t))
+
+    def incub_recovery_sweep(self, t):
+        self.call("incub_recovery_sweep", float(t))
+
+    def update_probs(self):
+        self.call("update_probs")
+
+    def run_steps(self, t0, n_steps, sps):
+        self.call("run_steps", float(t0), int(n_steps), int(sps))
+
+    def counters(self, r, reset=False):
+        c = self.abi.cs_counters()
+        self.call("get_counters", int(r), C.byref(c), int(bool(reset)))
+        return c
+
+    def timeseries(self, max_samples):
+        out = np.zeros((self.n_real, max_samples, self.abi.CS_TS_FIELDS), np.int32)
+        n = C.c_int()
+        self.call("get_timeseries", _p(out, C.c_int32), int(max_samples), C.byref(n))
+        return out[:, :n.value]
+
+    def work_counts(self, reset=False):
+        w = self.abi.cs_work_counts()
+        self.call("get_work_counts", C.byref(w), int(bool(reset)))
+        return {f: getattr(w, f) for f, _ in w._fields_}
+
+    def kernel_num(self, r2):
+        r2 = np.ascontiguousarray(r2, np.float64)
+        out = np.zeros_like(r2)
+        self.call("kernel_num", _p(r2, C.c_double), _p(out, C.c_double), len(r2))
+        return out
+
+    def dist2(self, a, b):
+        a = np.ascontiguousarray(a, np.int32)
+        b = np.ascontiguousarray(b, np.int32)
+        out = np.zeros(len(a))
+        self.call("dist2_persons", _p(a, C.c_int32), _p(b, C.c_int32), _p(out, C.c_double), len(a))
+        return out
+
+    def cell_tables(self, r, nc):
+        mt = np.zeros((nc, nc), np.float32)
+        ct = np.zeros((nc, nc), np.float32)
+        inv = np.zeros((nc, 1025), np.int32)
+        tp = np.zeros(nc, np.float32)
+        s0 = np.zeros(nc, np.int32)
+        f = C.c_float
+        self.call("get_cell_tables", int(r), _p(mt, f), _p(ct, f), _p(inv, C.c_int32), _p(tp, f), _p(s0, C.c_int32))
+        return mt, ct, inv, tp, s0
+
+    def hazards(self, r, kind, ts, person, other):
+        person = np.ascontiguousarray(person, np.int32)
+        other = np.ascontiguousarray(other, np.int32)
+        out = np.zeros(len(person))
+        self.call("eval_hazards", int(r), int(kind), int(ts), _p(person, C.c_int32), _p(other, C.c_int32),
+                  _p(out, C.c_double), len(person))
+        return out
+
+    def choose(self, table, u, mean):
+        u = np.ascontiguousarray(u, np.float64)
+        mean = np.ascontiguousarray(mean, np.float64)
+        out = np.zeros(len(u), np.uint16)
+        self.call("icdf_choose", int(table), _p(u, C.c_double), _p(mean, C.c_double), _p(out, C.c_uint16), len(u))
+        return out
+
+    def select_infector(self, cell_inf, u):
+        ci = np.ascontiguousarray(cell_inf, np.float32)
+        u = np.ascontiguousarray(u, np.float64)
+        out = np.zeros(len(u), np.int32)
+        self.lib.or_select_infector(_p(ci, C.c_float), len(ci), _p(u, C.c_double), _p(out, C.c_int32), len(u))
+        return out
+
+    def select_dest(self, r, cell, u):
+        u = np.ascontiguousarray(u, np.float64)
+        out = np.zeros(len(u), np.int32)
+        self.check(self.lib.or_select_dest_cell(self.h, int(r), int(cell), _p(u, C.c_double), _p(out, C.c_int32), len(u)))
+        return out
+
+    def philox(self, ck):
+        ck = np.ascontiguousarray(ck, np.uint32)
+        n = len(ck) // 6
+        out = np.zeros(4 * n, np.uint32)
+        self.lib.or_philox(_p(ck, C.c_uint32), _p(out, C.c_uint32), n)
+        return out
+
+    def sample(self, which, key, c0, c1, a, b, n):
+        """One draw per stream {key, c0, c1, i}, as the GPU's cs_sample."""
+        out = np.zeros(n)
+        one = np.zeros(1)
+        for i in range(n):
+            self.lib.or_sample(int(which), int(key), int(c0), int(c1), i, float(a), float(b), 1, _p(one, C.c_double))
+            out[i] = one[0]
+        return out

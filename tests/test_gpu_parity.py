@@ -1,0 +1,242 @@
+"""The HIP library against the oracle and the reference's golden vectors.
+
+Bar: bit-exact for tables, indices, random streams, samplers and whole
+trajectories (the GPU and the oracle share the counter-based random streams
+and the deterministic event/list rules); hazards within 1e-6 relative of
+the reference (in practice exact: both evaluate the reference's double
+products in its order).  sumRad2 is the one float accumulated with atomics
+in arbitrary order: compared to 1e-12 relative.
+"""
+import importlib
+
+import numpy as np
+import pytest
+
+from fixtures import hz_population, mini
+from oracle_lib import OracleEngine
+
+pytestmark = pytest.mark.gpu
+
+HAZARD_RTOL = 1e-6
+
+
+@pytest.fixture(scope="module")
+def engmod():
+    return importlib.import_module("covid-sim_amd.engine")
+
+
+@pytest.fixture(scope="module")
+def pair(engmod):
+    """GPU engine and oracle on the fixture population, one realisation."""
+    arrays, pop, params, _ = mini()
+    seeds = np.array([777], np.uint64)
+    g = engmod.Engine(0)
+    g.set_params(params)
+    g.set_population(pop)
+    g.alloc(seeds)
+    o = OracleEngine(params, pop)
+    o.alloc(seeds)
+    return g, o
+
+
+def test_philox_known_answers(pair):
+    """Philox4x32-10 known-answer vectors (Salmon et al., Random123 kat_vectors)."""
+    g, o = pair
+    ck = np.array([0, 0, 0, 0, 0, 0,
+                   0xffffffff, 0xffffffff, 0xffffffff, 0xffffffff, 0xffffffff, 0xffffffff,
+                   0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344, 0xa4093822, 0x299f31d0], np.uint32)
+    want = np.array([0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8,
+                     0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd,
+                     0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1], np.uint32)
+    assert np.array_equal(g.philox(ck), want)
+    assert np.array_equal(o.philox(ck), want)
+
+
+@pytest.mark.parametrize("which,a,b", [(0, 0, 0), (1, 30, 0.05), (1, 400, 0.3), (1, 5000, 0.97), (1, 12, 0.5),
+                                       (2, 0.7, 0), (2, 9.99, 0), (2, 10.0, 0), (2, 250.0, 0), (3, 0, 0),
+                                       (4, 1.0, 1.0), (4, 4.0, 4.0), (4, 2.0, 0.5)])
+def test_samplers_bit_exact(pair, which, a, b):
+    g, o = pair
+    n = 3000
+    x = g.sample(which, 0x1234567890ABCDEF, 77, 5, a, b, n)
+    y = o.sample(which, 0x1234567890ABCDEF, 77, 5, a, b, n)
+    assert np.array_equal(x.view(np.uint64), y.view(np.uint64))
+
+
+def test_kernel_lookup_vs_reference(pair):
+    g, o = pair
+    a = mini()[0]
+    out = g.kernel_num(a["g_kernel_r2"])
+    assert np.array_equal(out.view(np.uint64), a["g_kernel_num"].view(np.uint64))
+
+
+def test_dist2_vs_reference(pair):
+    g, o = pair
+    a = mini()[0]
+    out = g.dist2(a["g_dist_a"], a["g_dist_b"])
+    assert np.array_equal(out, a["g_dist2"])
+
+
+def test_cell_tables_vs_reference(pair, csa):
+    g, o = pair
+    arrays, pop, params, _ = mini()
+    st = csa.population.State.from_dump(pop, arrays, "s0_")
+    g.load_state(0, st)
+    mt, ct, inv, tp, s0 = g.cell_tables(0, pop.n_cells)
+    assert np.array_equal(mt.view(np.uint32), arrays["s0_max_trans"].view(np.uint32))
+    assert np.array_equal(ct.view(np.uint32), arrays["s0_cum_trans"].view(np.uint32))
+    assert np.array_equal(tp.view(np.uint32), arrays["s0_tot_prob"].view(np.uint32))
+    assert np.array_equal(inv, arrays["s0_invcdf"])
+    assert np.array_equal(s0, arrays["s0_cell_S0"])
+
+
+def test_choose_vs_reference(pair):
+    g, o = pair
+    a = mini()[0]
+    tab, u, mean, val = a["g_choose_table"], a["g_choose_u"], a["g_choose_mean"], a["g_choose_value"]
+    for t in np.unique(tab):
+        sel = tab == t
+        assert np.array_equal(g.choose(int(t), u[sel], mean[sel]), val[sel])
+
+
+def test_hazards_vs_reference(engmod, csa):
+    arrays, pop, params, hzp = mini()
+    hzp.interventions_active = 1
+    hpop = hz_population()
+    g = engmod.Engine(0)
+    g.set_params(hzp)
+    g.set_population(hpop)
+    g.alloc(np.array([1], np.uint64))
+    st = csa.population.State.from_dump(hpop, arrays, "hz_")
+    g.load_state(0, st)
+    person, other, pt, ts, ref = arrays["g_hz_person"], arrays["g_hz_other"], arrays["g_hz_pt"], arrays["g_hz_ts"], arrays["g_hz_out"]
+    for kind in range(8):
+        sec = pt if kind in (1, 5) else other
+        out = np.zeros(len(person))
+        for t in np.unique(ts):
+            sel = np.nonzero(ts == t)[0]
+            out[sel] = g.hazards(0, kind, int(t), person[sel], sec[sel])
+        rel = np.abs(out - ref[:, kind]) / np.maximum(np.abs(ref[:, kind]), 1e-300)
+        assert rel.max() <= HAZARD_RTOL, "kind %d max rel %g" % (kind, rel.max())
+
+
+def test_indexing_vs_oracle(pair, csa):
+    """Infector binary search and destination-cell selection for random u."""
+    g, o = pair
+    arrays, pop, params, _ = mini()
+    st = csa.population.State.from_dump(pop, arrays, "s160_")
+    g.load_state(0, st)
+    o.load_state(0, st)
+    rng = np.random.default_rng(3)
+    u = rng.random(20000)
+    u = np.clip(u, 1e-15, 1 - 1e-15)
+    for n in (1, 2, 3, 7, 64, 1000):
+        ci = np.cumsum(rng.random(n)).astype(np.float32)
+        ci /= ci[-1]
+        ci[-1] = 1.0
+        ci[rng.random(n) < 0.2] *= -1
+        assert np.array_equal(g.select_infector(ci, u), o.select_infector(ci, u))
+    for c in range(pop.n_cells):
+        got = g.select_dest(0, c, u)
+        assert np.array_equal(got, o.select_dest(0, c, u))
+        _, ct, _, _, _ = g.cell_tables(0, pop.n_cells)
+        # inverse-CDF definition: first l with cum[l] >= u
+        want = np.argmax(ct[c][None, :] >= u[:, None], axis=1)
+        assert np.array_equal(got, want)
+
+
+def _compare_counters(a, b):
+    for f, _ in a._fields_:
+        va, vb = getattr(a, f), getattr(b, f)
+        if f == "sumRad2":
+            assert abs(va - vb) <= 1e-12 * max(abs(va), 1.0), f
+        elif hasattr(va, "__len__"):
+            assert np.array_equal(np.array(va), np.array(vb)), f
+        else:
+            assert va == vb, f
+
+
+def test_one_step_matches_oracle(pair, csa):
+    """One InfectSweep + IncubRecoverySweep from the reference's own state at
+    day 40 of its run: every person, cell and place array equal."""
+    g, o = pair
+    arrays, pop, params, _ = mini()
+    st = csa.population.State.from_dump(pop, arrays, "s160_")
+    for eng in (g, o):
+        eng.load_state(0, st)
+        eng.counters(0, reset=True)
+    t = 160 / params.time_steps_per_day
+    for eng in (g, o):
+        eng.infect_sweep(t)
+    a, b = csa.population.State(pop), csa.population.State(pop)
+    g.store_state(0, a)
+    o.store_state(0, b)
+    assert a.compare(b) == []
+    for eng in (g, o):
+        eng.incub_recovery_sweep(t)
+    g.store_state(0, a)
+    o.store_state(0, b)
+    assert a.compare(b) == []
+    ca, cb = g.counters(0), o.counters(0)
+    _compare_counters(ca, cb)
+    assert ca.cumI > 20  # the step really infected people
+
+
+def test_trajectory_matches_oracle(engmod, csa):
+    """From the realisation reset (seeding included) through 120 days, in one
+    batched launch of 3 realisations: daily series, final state and
+    counters equal to the oracle's."""
+    arrays, pop, params, _ = mini()
+    seeds = np.array([11, 12, 13], np.uint64)
+    g = engmod.Engine(0)
+    g.set_params(params)
+    g.set_population(pop)
+    g.alloc(seeds)
+    o = OracleEngine(params, pop)
+    o.alloc(seeds)
+    steps = int(120 * params.time_steps_per_day)
+    for eng in (g, o):
+        eng.reset(10)
+        eng.run_steps(0.0, steps, int(params.time_steps_per_day))
+    tg, to = g.timeseries(120), o.timeseries(120)
+    assert tg.shape == (3, 120, 8)
+    assert np.array_equal(tg, to)
+    for r in range(3):
+        a, b = csa.population.State(pop), csa.population.State(pop)
+        g.store_state(r, a)
+        o.store_state(r, b)
+        assert a.compare(b) == []
+        _compare_counters(g.counters(r), o.counters(r))
+    # the epidemic took off in every realisation
+    assert (tg[:, -1, 3] + tg[:, -1, 4] > pop.n_people // 2).all()
+    wg, wo = g.work_counts(), o.work_counts()
+    for k in wg:
+        if k != "n_cells":
+            assert wg[k] == wo[k], k
+
+
+def test_invariants_on_gpu(engmod, csa):
+    """Population conservation and list consistency after a GPU run."""
+    arrays, pop, params, _ = mini()
+    g = engmod.Engine(0)
+    g.set_params(params)
+    g.set_population(pop)
+    g.alloc(np.array([99], np.uint64))
+    g.reset(10)
+    g.run_steps(0.0, 200, 4)
+    ts = g.timeseries(50)
+    assert (ts[0, :, :5].sum(axis=1) == pop.n_people).all()
+    st = csa.population.State(pop)
+    g.store_state(0, st)
+    a = st.a
+    cs_ = pop.a["c_member_start"]
+    for c in range(pop.n_cells):
+        seg = a["cell_perm"][cs_[c]:cs_[c + 1]]
+        assert np.array_equal(np.sort(seg), np.sort(pop.a["c_members"][cs_[c]:cs_[c + 1]]))
+        assert np.array_equal(a["listpos"][seg], np.arange(len(seg)))
+        S, L, I = a["cell_S"][c], a["cell_L"][c], a["cell_I"][c]
+        inf = a["inf"][seg]
+        assert (inf[:S] == 0).all()
+        assert (inf[S:S + L] == 1).all()
+        assert np.isin(inf[S + L:S + L + I], [-1, 2, -2]).all()
+        assert np.isin(inf[S + L + I:], [3, -3, 5, -5]).all()

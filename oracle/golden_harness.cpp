@@ -52,7 +52,8 @@
 #include "Rand.h"
 #include "Sweep.h"
 
-#include "../include/covidsim_hip.h"
+#include "covidsim_hip.h"
+#include "cs_reference_bridge.h"
 
 void InfectSweep_cpu(double t, int run);
 void IncubRecoverySweep_cpu(double t);
@@ -110,160 +111,7 @@ template <class T> void save2(const std::string& name, const std::vector<T>& v, 
     save_npy(name, npdt<T>(), {rows, cols}, v.data(), sizeof(T));
 }
 
-void copy_icdf(double* dst, InverseCdf& src)
-{
-    for (int i = 0; i <= CDF_RES; i++) dst[i] = src[i];
-}
-
-// cs_params from the reference's Param P (src/Param.h) as the run sees it.
-void fill_params(cs_params* c)
-{
-    memset(c, 0, sizeof(*c));
-    c->time_steps_per_day = P.TimeStepsPerDay;
-    c->model_time_step = P.ModelTimeStep;
-    c->false_positive_rate = P.FalsePositiveRate;
-    c->do_seasonality = P.DoSeasonality;
-    for (int i = 0; i < DAYS_PER_YEAR; i++) c->seasonality[i] = P.DoSeasonality ? P.Seasonality[i] : 1.0;
-    c->local_beta = P.LocalBeta;
-    c->household_trans = P.HouseholdTrans;
-    for (int i = 0; i < MAX_HOUSEHOLD_SIZE; i++) c->household_denom_lookup[i] = P.HouseholdDenomLookup[i];
-    c->do_households = P.DoHouseholds;
-    c->do_places = P.DoPlaces;
-    c->num_place_types = P.NumPlaceTypes;
-    c->hotel_place_type = P.HotelPlaceType;
-    c->place_type_no_air_num = P.PlaceTypeNoAirNum;
-    c->care_home_place_type = P.CareHomePlaceType;
-    c->do_symptoms = P.DoSymptoms;
-    c->do_severity = P.DoSeverity;
-    c->do_latent = P.DoLatent;
-    c->do_infectiousness_profile = P.DoInfectiousnessProfile;
-    c->do_real_sympt_withdrawal = P.DoRealSymptWithdrawal;
-    c->absenteeism_place_closure = P.AbsenteeismPlaceClosure;
-    c->no_infectiousness_sd_in_hh = P.NoInfectiousnessSDinHH;
-    c->do_death = P.DoDeath;
-    c->do_ad_units = P.DoAdUnits;
-    c->output_ad_unit_age = P.OutputAdUnitAge;
-    c->include_stepdown_to_death = P.IncludeStepDownToDeath;
-    c->do_digital_contact_tracing = P.DoDigitalContactTracing;
-    c->do_blanket_move_restr = P.DoBlanketMoveRestr;
-    c->interventions_active = 0;
-    for (int i = 0; i < MAX_NUM_PLACE_TYPES; i++) {
-        c->place_type_trans[i] = P.PlaceTypeTrans[i];
-        c->place_type_group_size_param1[i] = P.PlaceTypeGroupSizeParam1[i];
-        c->place_type_prop_between_group_links[i] = P.PlaceTypePropBetweenGroupLinks[i];
-        c->sympt_place_type_contact_rate[i] = P.SymptPlaceTypeContactRate[i];
-        c->sympt_place_type_withdrawal_prop[i] = P.SymptPlaceTypeWithdrawalProp[i];
-        c->holiday_effect[i] = P.HolidayEffect[i];
-    }
-    c->sympt_spatial_contact_rate = P.SymptSpatialContactRate;
-    for (int a = 0; a < NUM_AGE_GROUPS; a++) {
-        c->relative_spatial_contact[a] = P.RelativeSpatialContact[a];
-        c->relative_spatial_contact_susc[a] = P.RelativeSpatialContactSusc[a];
-        c->age_susceptibility[a] = P.AgeSusceptibility[a];
-        c->age_infectiousness[a] = P.AgeInfectiousness[a];
-        c->proportion_symptomatic[a] = P.ProportionSymptomatic[a];
-        for (int b = 0; b < NUM_AGE_GROUPS; b++) {
-            c->waifw[a][b] = P.WAIFW_Matrix[a][b];
-            c->waifw_spatial[a][b] = P.WAIFW_Matrix_SpatialOnly[a][b];
-        }
-        c->mean_by_age[CS_MEAN_MILD_RECOV][a] = P.Mean_MildToRecovery[a];
-        c->mean_by_age[CS_MEAN_ILI_RECOV][a] = P.Mean_ILIToRecovery[a];
-        c->mean_by_age[CS_MEAN_SARI_RECOV][a] = P.Mean_SARIToRecovery[a];
-        c->mean_by_age[CS_MEAN_CRIT_CRITRECOV][a] = P.Mean_CriticalToCritRecov[a];
-        c->mean_by_age[CS_MEAN_CRITRECOV_RECOV][a] = P.Mean_CritRecovToRecov[a];
-        c->mean_by_age[CS_MEAN_STEPDOWN_DEATH][a] = P.Mean_StepdownToDeath[a];
-        c->mean_by_age[CS_MEAN_ILI_SARI][a] = P.Mean_ILIToSARI[a];
-        c->mean_by_age[CS_MEAN_SARI_CRIT][a] = P.Mean_SARIToCritical[a];
-        c->mean_by_age[CS_MEAN_CRIT_DEATH][a] = P.Mean_CriticalToDeath[a];
-        c->mean_by_age[CS_MEAN_SARI_DEATH][a] = P.Mean_SARIToDeath[a];
-        c->mean_by_age[CS_MEAN_ILI_DEATH][a] = P.Mean_ILIToDeath[a];
-        c->prop_ili[a] = P.Prop_ILI_ByAge[a];
-        c->prop_sari[a] = P.Prop_SARI_ByAge[a];
-        c->prop_critical[a] = P.Prop_Critical_ByAge[a];
-        c->cfr_critical[a] = P.CFR_Critical_ByAge[a];
-        c->cfr_sari[a] = P.CFR_SARI_ByAge[a];
-        c->cfr_ili[a] = P.CFR_ILI_ByAge[a];
-    }
-    for (int i = 0; i < P.NumInterventionClasses && i < CS_NUM_NPI; i++)
-        for (int j = 0; j < P.NumInfectionSettings && j < CS_NUM_SETTINGS; j++)
-            c->efficacies[i][j] = P.Efficacies[i][j];
-    c->care_home_resident_household_scaling = P.CareHomeResidentHouseholdScaling;
-    c->care_home_resident_spatial_scaling = P.CareHomeResidentSpatialScaling;
-    c->care_home_worker_group_scaling = P.CareHomeWorkerGroupScaling;
-    c->care_home_resident_place_scaling = P.CareHomeResidentPlaceScaling;
-    c->care_home_rel_prob_hosp = P.CareHomeRelProbHosp;
-    c->treat_inf_drop = P.TreatInfDrop;
-    c->vacc_inf_drop = P.VaccInfDrop;
-    c->treat_susc_drop = P.TreatSuscDrop;
-    c->vacc_susc_drop = P.VaccSuscDrop;
-    c->vacc_susc_drop2 = P.VaccSuscDrop2;
-    c->treat_sympt_drop = P.TreatSympDrop;
-    c->vacc_sympt_drop = P.VaccSympDrop;
-    c->treat_death_drop = P.TreatDeathDrop;
-    c->us_vacc_time_to_efficacy = P.usVaccTimeToEfficacy;
-    c->us_vacc_time_efficacy_switch = P.usVaccTimeEfficacySwitch;
-    c->us_hquarantine_house_duration = P.usHQuarantineHouseDuration;
-    c->us_case_isolation_delay = P.usCaseIsolationDelay;
-    c->us_case_isolation_duration = P.usCaseIsolationDuration;
-    c->us_case_absenteeism_delay = P.usCaseAbsenteeismDelay;
-    c->us_case_absenteeism_duration = P.usCaseAbsenteeismDuration;
-    c->n_infectiousness = MAX_INFECTIOUS_STEPS;
-    for (int i = 0; i < MAX_INFECTIOUS_STEPS; i++) c->infectiousness[i] = P.infectiousness[i];
-    c->latent_period = P.LatentPeriod;
-    c->infectious_period = P.InfectiousPeriod;
-    c->latent_to_sympt_delay = P.LatentToSymptDelay;
-    c->sympt_infectiousness = P.SymptInfectiousness;
-    c->asympt_infectiousness = P.AsymptInfectiousness;
-    c->infectiousness_sd = P.InfectiousnessSD;
-    c->susceptibility_sd = P.SusceptibilitySD;
-    copy_icdf(c->icdf[CS_ICDF_LATENT], P.latent_icdf);
-    copy_icdf(c->icdf[CS_ICDF_INFECTIOUS], P.infectious_icdf);
-    copy_icdf(c->icdf[CS_ICDF_MILD_RECOV], P.MildToRecovery_icdf);
-    copy_icdf(c->icdf[CS_ICDF_ILI_RECOV], P.ILIToRecovery_icdf);
-    copy_icdf(c->icdf[CS_ICDF_SARI_RECOV], P.SARIToRecovery_icdf);
-    copy_icdf(c->icdf[CS_ICDF_CRIT_CRITRECOV], P.CriticalToCritRecov_icdf);
-    copy_icdf(c->icdf[CS_ICDF_CRITRECOV_RECOV], P.CritRecovToRecov_icdf);
-    copy_icdf(c->icdf[CS_ICDF_ILI_SARI], P.ILIToSARI_icdf);
-    copy_icdf(c->icdf[CS_ICDF_SARI_CRIT], P.SARIToCritical_icdf);
-    copy_icdf(c->icdf[CS_ICDF_ILI_DEATH], P.ILIToDeath_icdf);
-    copy_icdf(c->icdf[CS_ICDF_SARI_DEATH], P.SARIToDeath_icdf);
-    copy_icdf(c->icdf[CS_ICDF_STEPDOWN_DEATH], P.StepdownToDeath_icdf);
-    copy_icdf(c->icdf[CS_ICDF_CRIT_DEATH], P.CriticalToDeath_icdf);
-    c->cfr_critical_scale = P.CFR_Critical_Scale_Current;
-    c->cfr_sari_scale = P.CFR_SARI_Scale_Current;
-    c->cfr_ili_scale = P.CFR_ILI_Scale_Current;
-    c->control_prop_cases_id = P.ControlPropCasesId;
-    c->case_absent_child_age_cutoff = P.CaseAbsentChildAgeCutoff;
-    c->place_close_round_household = P.PlaceCloseRoundHousehold;
-    c->case_absent_child_prop_adult_carers = P.CaseAbsentChildPropAdultCarers;
-    c->location_initial_infection[0] = P.LocationInitialInfection[0][0];
-    c->location_initial_infection[1] = P.LocationInitialInfection[0][1];
-    c->kernel_size = P.KernelLookup.size_;
-    c->kernel_expansion = P.KernelLookup.expansion_factor_;
-    c->kernel_delta = P.KernelLookup.delta_;
-    c->kernel_type = P.MoveKernel.type_;
-    c->kernel_shape = P.MoveKernel.shape_;
-    c->kernel_scale = P.MoveKernel.scale_;
-    c->kernel_p3 = P.MoveKernel.p3_;
-    c->kernel_p4 = P.MoveKernel.p4_;
-    c->do_utm_coords = P.DoUTM_coords;
-    c->do_periodic_boundaries = P.DoPeriodicBoundaries;
-    c->nch = P.nch;
-    c->cell_width = P.in_cells_.width;
-    c->cell_height = P.in_cells_.height;
-    c->degrees_width = P.in_degrees_.width;
-    c->degrees_height = P.in_degrees_.height;
-    c->bbox_bottom_left_y = P.SpatialBoundingBox.bottom_left().y;
-    for (int i = 0; i <= 360; i++) { c->sinx[i] = P.sinx[i]; c->cosx[i] = P.cosx[i]; }
-    for (int i = 0; i <= 1000; i++) c->asin2sqx[i] = P.asin2sqx[i];
-    c->do_holidays = P.DoHolidays;
-    c->num_holidays = std::min(P.NumHolidays, CS_MAX_HOLIDAYS);
-    for (int i = 0; i < c->num_holidays; i++) {
-        c->holiday_start[i] = P.HolidayStartTime[i];
-        c->holiday_duration[i] = P.HolidayDuration[i];
-    }
-    c->holidays_start_day_simtime = P.HolidaysStartDay_SimTime;
-}
+using cs_bridge::fill_params;
 
 void dump_params(const std::string& name)
 {
@@ -277,66 +125,23 @@ void dump_params(const std::string& name)
 
 void dump_population()
 {
+    cs_bridge::PopulationView v;
+    v.gather();
     const int N = P.PopSize;
-    std::vector<int32_t> hh(N), mcell(N), pcell(N), plinks((size_t)N * MAX_NUM_PLACE_TYPES);
-    std::vector<uint16_t> pglinks((size_t)N * MAX_NUM_PLACE_TYPES);
-    std::vector<uint8_t> age(N), flags(N);
-    for (int i = 0; i < N; i++) {
-        const Person& a = Hosts[i];
-        hh[i] = a.hh; mcell[i] = a.mcell; pcell[i] = a.pcell; age[i] = a.age;
-        flags[i] = (uint8_t)((a.care_home_resident ? 1 : 0) | (a.keyworker ? 2 : 0) |
-                             (a.esocdist_comply ? 4 : 0) | (a.digitalContactTracingUser ? 8 : 0));
-        for (int t = 0; t < MAX_NUM_PLACE_TYPES; t++) {
-            plinks[(size_t)i * MAX_NUM_PLACE_TYPES + t] = (t < P.NumPlaceTypes) ? a.PlaceLinks[t] : -1;
-            pglinks[(size_t)i * MAX_NUM_PLACE_TYPES + t] = (t < P.NumPlaceTypes) ? a.PlaceGroupLinks[t] : 0;
-        }
-    }
-    save("p_hh", hh); save("p_mcell", mcell); save("p_pcell", pcell);
-    save("p_age", age); save("p_flags", flags);
-    save2("p_place_links", plinks, N, MAX_NUM_PLACE_TYPES);
-    save2("p_place_group_links", pglinks, N, MAX_NUM_PLACE_TYPES);
-
-    std::vector<int32_t> hfirst(P.NumHouseholds);
-    std::vector<uint16_t> hnh(P.NumHouseholds), hnhr(P.NumHouseholds);
-    std::vector<float> hloc((size_t)P.NumHouseholds * 2);
-    for (int i = 0; i < P.NumHouseholds; i++) {
-        hfirst[i] = Households[i].FirstPerson; hnh[i] = Households[i].nh; hnhr[i] = Households[i].nhr;
-        hloc[2 * i] = Households[i].loc.x; hloc[2 * i + 1] = Households[i].loc.y;
-    }
-    save("h_first", hfirst); save("h_nh", hnh); save("h_nhr", hnhr); save2("h_loc", hloc, P.NumHouseholds, 2);
-
-    const int NC = P.NumPopulatedCells;
-    std::vector<int32_t> cidx(NC), cn(NC), cstart(NC + 1), cmem;
-    cstart[0] = 0;
-    for (int j = 0; j < NC; j++) {
-        Cell* c = CellLookup[j];
-        cidx[j] = (int32_t)(c - Cells); cn[j] = c->n;
-        for (int k = 0; k < c->n; k++) cmem.push_back(c->members[k]);
-        cstart[j + 1] = cstart[j] + c->n;
-    }
-    save("c_index", cidx); save("c_n", cn); save("c_member_start", cstart); save("c_members", cmem);
-
-    int nm = P.NMCL * P.NMCL * P.NumCells;
-    std::vector<int32_t> madunit(nm);
-    for (int i = 0; i < nm; i++) madunit[i] = Mcells[i].adunit;
-    save("m_adunit", madunit);
-
+    save("p_hh", v.hh); save("p_mcell", v.mcell); save("p_pcell", v.pcell);
+    save("p_age", v.age); save("p_flags", v.flags);
+    save2("p_place_links", v.plinks, N, MAX_NUM_PLACE_TYPES);
+    save2("p_place_group_links", v.pglinks, N, MAX_NUM_PLACE_TYPES);
+    save("h_first", v.hfirst); save("h_nh", v.hnh); save("h_nhr", v.hnhr); save2("h_loc", v.hloc, P.NumHouseholds, 2);
+    save("c_index", v.cidx); save("c_n", v.cn); save("c_member_start", v.cstart); save("c_members", v.cmem);
+    save("m_adunit", v.madunit);
     for (int t = 0; t < P.NumPlaceTypes; t++) {
-        std::vector<int32_t> pn(P.Nplace[t]), pm(P.Nplace[t]), pms(P.Nplace[t] + 1), mem, pgs(P.Nplace[t] + 1), gst, gsz;
-        pms[0] = 0; pgs[0] = 0;
-        for (int k = 0; k < P.Nplace[t]; k++) {
-            const Place& pl = Places[t][k];
-            pn[k] = pl.n; pm[k] = pl.mcell;
-            for (int m = 0; m < pl.n; m++) mem.push_back(pl.members[m]);
-            pms[k + 1] = pms[k] + pl.n;
-            for (int g = 0; g < pl.ng; g++) { gst.push_back(pl.group_start[g]); gsz.push_back(pl.group_size[g]); }
-            pgs[k + 1] = pgs[k] + pl.ng;
-        }
         std::string s = std::to_string(t);
-        save("pl" + s + "_n", pn); save("pl" + s + "_mcell", pm); save("pl" + s + "_member_start", pms);
-        save("pl" + s + "_members", mem); save("pl" + s + "_groups_start", pgs);
-        save("pl" + s + "_group_start", gst); save("pl" + s + "_group_size", gsz);
+        save("pl" + s + "_n", v.pl_n[t]); save("pl" + s + "_mcell", v.pl_mc[t]); save("pl" + s + "_member_start", v.pl_ms[t]);
+        save("pl" + s + "_members", v.pl_mem[t]); save("pl" + s + "_groups_start", v.pl_gs0[t]);
+        save("pl" + s + "_group_start", v.pl_gst[t]); save("pl" + s + "_group_size", v.pl_gsz[t]);
     }
+    const int NC = P.NumPopulatedCells, nm = cs_bridge::n_mcells_total();
     std::vector<int32_t> meta = {P.PopSize, P.NumHouseholds, NC, P.NumCells, nm, P.NumAdunits, P.NumPlaceTypes, P.NMCL};
     save("meta", meta);
 
@@ -347,79 +152,41 @@ void dump_population()
 
 void dump_state(const std::string& prefix, unsigned short ts)
 {
-    const int N = P.PopSize;
-    std::vector<int8_t> inf(N);
-    std::vector<uint8_t> sevc(N), sevf(N), bits(N), qc(N), dct(N);
-    std::vector<int32_t> listpos(N), infector(N);
-    std::vector<int16_t> itype(N);
-    std::vector<float> infness(N), susc(N), pcare(N);
-    std::vector<uint16_t> t_inf(N), t_lat(N), t_rod(N), t_sari(N), t_crit(N), t_step(N), t_det(N), t_as(N), t_ae(N),
-        t_ts(N), t_te(N), t_vs(N), t_is(N), t_qs(N);
-    for (int i = 0; i < N; i++) {
-        const Person& a = Hosts[i];
-        // the InfStat code is private; read it through the predicates (src/Person.cpp)
-        int code;
-        if (a.is_susceptible()) code = 0;
-        else if (a.is_latent()) code = 1;
-        else if (a.is_infectious_almost_symptomatic()) code = -1;
-        else if (a.is_infectious_asymptomatic_not_case()) code = 2;
-        else if (a.is_case()) code = -2;
-        else if (a.is_immune_at_start()) code = 4;
-        else if (a.is_dead_was_asymp()) code = 5;
-        else if (a.is_dead_was_symp()) code = -5;
-        else if (a.is_recovered_symp()) code = -3;
-        else code = 3;
-        inf[i] = (int8_t)code;
-        sevc[i] = (uint8_t)a.Severity_Current; sevf[i] = (uint8_t)a.Severity_Final;
-        bits[i] = (uint8_t)((a.to_die ? 1 : 0) | (a.detected ? 2 : 0));
-        listpos[i] = a.listpos; infector[i] = a.infector; itype[i] = a.infect_type;
-        infness[i] = a.infectiousness; susc[i] = a.susc; pcare[i] = a.ProbCare;
-        t_inf[i] = a.infection_time; t_lat[i] = a.latent_time; t_rod[i] = a.recovery_or_death_time;
-        t_sari[i] = a.SARI_time; t_crit[i] = a.Critical_time; t_step[i] = a.Stepdown_time; t_det[i] = a.detected_time;
-        t_as[i] = a.absent_start_time; t_ae[i] = a.absent_stop_time;
-        t_ts[i] = a.treat_start_time; t_te[i] = a.treat_stop_time; t_vs[i] = a.vacc_start_time;
-        t_is[i] = a.isolation_start_time;
-        t_qs[i] = HostsQuarantine[i].start_time; qc[i] = HostsQuarantine[i].comply;
-        dct[i] = a.digitalContactTraced;
-    }
+    cs_bridge::StateView v;
+    v.gather();
     const std::string p = prefix;
-    save(p + "inf", inf); save(p + "sev_cur", sevc); save(p + "sev_final", sevf); save(p + "bits", bits);
-    save(p + "listpos", listpos); save(p + "infector", infector); save(p + "infect_type", itype);
-    save(p + "infectiousness", infness); save(p + "susc", susc); save(p + "prob_care", pcare);
-    save(p + "t_infection", t_inf); save(p + "t_latent", t_lat); save(p + "t_recovery", t_rod);
-    save(p + "t_sari", t_sari); save(p + "t_critical", t_crit); save(p + "t_stepdown", t_step);
-    save(p + "t_detected", t_det); save(p + "t_absent_start", t_as); save(p + "t_absent_stop", t_ae);
-    save(p + "t_treat_start", t_ts); save(p + "t_treat_stop", t_te); save(p + "t_vacc_start", t_vs);
-    save(p + "t_isolation_start", t_is); save(p + "t_quar_start", t_qs); save(p + "quar_comply", qc);
-    save(p + "dct_traced", dct);
+    save(p + "inf", v.inf); save(p + "sev_cur", v.sevc); save(p + "sev_final", v.sevf); save(p + "bits", v.bits);
+    save(p + "listpos", v.listpos); save(p + "infector", v.infector); save(p + "infect_type", v.itype);
+    save(p + "infectiousness", v.infn); save(p + "susc", v.susc); save(p + "prob_care", v.pcare);
+    save(p + "t_infection", v.t_inf); save(p + "t_latent", v.t_lat); save(p + "t_recovery", v.t_rod);
+    save(p + "t_sari", v.t_sari); save(p + "t_critical", v.t_crit); save(p + "t_stepdown", v.t_step);
+    save(p + "t_detected", v.t_det); save(p + "t_absent_start", v.t_as); save(p + "t_absent_stop", v.t_ae);
+    save(p + "t_treat_start", v.t_ts); save(p + "t_treat_stop", v.t_te); save(p + "t_vacc_start", v.t_vs);
+    save(p + "t_isolation_start", v.t_is); save(p + "t_quar_start", v.t_qs); save(p + "quar_comply", v.qc);
+    save(p + "dct_traced", v.dct);
 
     const int NC = P.NumPopulatedCells;
-    std::vector<int32_t> S(NC), L(NC), I(NC), R(NC), D(NC), cumTC(NC), S0(NC), perm;
+    save(p + "cell_S", v.cS); save(p + "cell_L", v.cL); save(p + "cell_I", v.cI); save(p + "cell_R", v.cR);
+    save(p + "cell_D", v.cD); save(p + "cell_cumTC", v.cTC); save(p + "cell_perm", v.perm);
+    // the cell tables UpdateProbs keeps (not part of the state the library loads)
+    std::vector<int32_t> S0(NC);
     std::vector<float> totp(NC), maxt((size_t)NC * NC), cumt((size_t)NC * NC);
     std::vector<int32_t> invcdf((size_t)NC * 1025);
     for (int j = 0; j < NC; j++) {
         Cell* c = CellLookup[j];
-        S[j] = c->S; L[j] = c->L; I[j] = c->I; R[j] = c->R; D[j] = c->D; cumTC[j] = c->cumTC; S0[j] = c->S0;
+        S0[j] = c->S0;
         totp[j] = c->tot_prob;
-        for (int k = 0; k < c->n; k++) perm.push_back(c->susceptible[k]);
         for (int m = 0; m < NC; m++) { maxt[(size_t)j * NC + m] = c->max_trans[m]; cumt[(size_t)j * NC + m] = c->cum_trans[m]; }
         for (int k = 0; k <= 1024; k++) invcdf[(size_t)j * 1025 + k] = c->InvCDF[k];
     }
-    save(p + "cell_S", S); save(p + "cell_L", L); save(p + "cell_I", I); save(p + "cell_R", R); save(p + "cell_D", D);
-    save(p + "cell_cumTC", cumTC); save(p + "cell_S0", S0); save(p + "cell_perm", perm);
+    save(p + "cell_S0", S0);
     save(p + "tot_prob", totp); save2(p + "max_trans", maxt, NC, NC); save2(p + "cum_trans", cumt, NC, NC);
     save2(p + "invcdf", invcdf, NC, 1025);
-
     for (int t = 0; t < P.NumPlaceTypes; t++) {
-        std::vector<uint16_t> cs(P.Nplace[t]), ce(P.Nplace[t]);
-        for (int k = 0; k < P.Nplace[t]; k++) { cs[k] = Places[t][k].close_start_time; ce[k] = Places[t][k].close_end_time; }
-        save(p + "pl" + std::to_string(t) + "_close_start", cs);
-        save(p + "pl" + std::to_string(t) + "_close_end", ce);
+        save(p + "pl" + std::to_string(t) + "_close_start", v.pcs[t]);
+        save(p + "pl" + std::to_string(t) + "_close_end", v.pce[t]);
     }
-    int nm = P.NMCL * P.NMCL * P.NumCells;
-    std::vector<uint8_t> sd(nm), mr(nm);
-    for (int i = 0; i < nm; i++) { sd[i] = (uint8_t)Mcells[i].socdist; mr[i] = (uint8_t)Mcells[i].moverest; }
-    save(p + "m_socdist", sd); save(p + "m_moverest", mr);
+    save(p + "m_socdist", v.msd); save(p + "m_moverest", v.mmr);
     std::vector<int32_t> tsv = {(int32_t)ts};
     save(p + "timestep", tsv);
     dump_params(p + "params.bin");

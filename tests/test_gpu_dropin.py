@@ -1,0 +1,94 @@
+"""The reference's own driver with its two sweeps served by the GPU library.
+
+oracle/_ref/covidsim_dropin is the unmodified reference (RunModel, set-up,
+RecordSample, output writers) linked with covid-sim_amd/integration/
+cs_shim.cpp (oracle/Makefile.ref, target dropin).  It runs the 20k-person
+fixture region for 30 realisations; the state after day 118 of each
+realisation (the reference's progress line at the start of the last output
+step) is compared with the 200-realisation reference ensemble of the same
+region (tests/golden/mini_ensemble.npz, index 118):
+
+  * the attack rate (R+D)/N by two-sample KS (p > 0.01) and overlapping 95%
+    confidence intervals of the mean;
+  * S+L+I+R+D = N in every realisation.
+
+With CS_SHIM=cpu the same binary must give the reference's exact outputs;
+that is checked on the CPU (no GPU needed), so this module's CPU test
+pins the binding and its GPU test pins the library behind it.
+"""
+import filecmp
+import os
+import re
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+from scipy import stats
+
+from fixtures import GOLDEN
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DROPIN = os.path.join(ROOT, "oracle", "_ref", "covidsim_dropin")
+STOCK = os.path.join(ROOT, "oracle", "_ref", "covidsim_ref")
+INPUTS = os.path.join(ROOT, "covid-sim_amd", "inputs", "mini")
+SEEDS = ["98798150", "729101", "17389101", "4797132"]
+PROGRESS = re.compile(rb"t=([0-9.e+-]+)\s+(-?\d+)\s+(-?\d+)\|(-?\d+)\s+(-?\d+)\s+(-?\d+) \[=(\d+)\]")
+
+
+def run(exe, out, nr, env_extra=None, timeout=900):
+    cmd = [exe, "/c:1", "/NR:%d" % nr,
+           "/PP:" + os.path.join(INPUTS, "pre-params.txt"), "/P:" + os.path.join(INPUTS, "input-noint-params.txt"),
+           "/O:" + os.path.join(out, "out"), "/D:" + os.path.join(INPUTS, "density.txt"),
+           "/A:" + os.path.join(INPUTS, "admin-params.txt")] + SEEDS
+    env = dict(os.environ, **(env_extra or {}))
+    r = subprocess.run(cmd, cwd=out, env=env, capture_output=True, timeout=timeout)
+    assert r.returncode == 0, r.stderr[-2000:].decode(errors="replace")
+    return r.stderr
+
+
+def last_states(stderr, t_last):
+    """S, L, I, R, D of every realisation's progress line at t == t_last."""
+    rows = []
+    for m in PROGRESS.finditer(stderr):
+        if abs(float(m.group(1)) - t_last) < 1e-9:
+            rows.append([int(m.group(k)) for k in range(2, 7)])
+    return np.array(rows)
+
+
+def need(path):
+    if not os.path.exists(path):
+        pytest.skip("%s not built (oracle/Makefile.ref needs the reference sources)" % os.path.basename(path))
+
+
+def test_dropin_cpu_mode_is_the_reference():
+    """CS_SHIM=cpu: the binding's own code path changes no output byte."""
+    need(DROPIN)
+    need(STOCK)
+    a, b = tempfile.mkdtemp(), tempfile.mkdtemp()
+    run(DROPIN, a, 2, {"CS_SHIM": "cpu"})
+    run(STOCK, b, 2)
+    outs = sorted(f for f in os.listdir(b) if f.startswith("out"))
+    assert outs
+    for f in outs:
+        assert filecmp.cmp(os.path.join(a, f), os.path.join(b, f), shallow=False), f
+
+
+@pytest.mark.gpu
+def test_dropin_gpu_ensemble_matches_reference():
+    need(DROPIN)
+    ref = np.load(os.path.join(GOLDEN, "mini_ensemble.npz"))["daily"]
+    n = int(ref[0, 0, :5].sum())
+    out = tempfile.mkdtemp()
+    err = run(DROPIN, out, 30)
+    st = last_states(err, 119.0)
+    assert st.shape == (30, 5), st.shape
+    assert (st.sum(axis=1) == n).all()
+    gpu_attack = (st[:, 3] + st[:, 4]) / n
+    ref_attack = (ref[:, 118, 3] + ref[:, 118, 4]) / n
+    assert stats.ks_2samp(gpu_attack, ref_attack).pvalue > 0.01, (gpu_attack.mean(), ref_attack.mean())
+    z = stats.norm.ppf(0.975)
+    ha = z * gpu_attack.std(ddof=1) / np.sqrt(len(gpu_attack))
+    hb = z * ref_attack.std(ddof=1) / np.sqrt(len(ref_attack))
+    assert abs(gpu_attack.mean() - ref_attack.mean()) <= ha + hb, (gpu_attack.mean(), ref_attack.mean())
+    assert len(set(gpu_attack)) > 1  # realisations differ (per-run stream seeds)

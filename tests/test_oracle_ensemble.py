@@ -7,10 +7,11 @@ the distribution of trajectories must.  The reference ensemble
 reference on the 20k-person fixture region (tests/golden/make_ensemble.py);
 the oracle runs 200 realisations on the same population and parameters.
 
-Compared: attack rate, peak day and peak daily incidence, each with a
-two-sample Kolmogorov-Smirnov test (p > 0.01) and agreement of the two
-ensembles' 95% confidence intervals for the mean (the intervals overlap);
-and the daily incidence of every day with a per-day KS test,
+Compared: attack rate, peak day, peak daily incidence, cumulative cases and
+deaths, each with a two-sample Kolmogorov-Smirnov test (p > 0.01) and
+agreement of the two ensembles' 95% confidence intervals for the mean (the
+intervals overlap); and the daily infections and cases of every day with a
+per-day KS test,
 Bonferroni-corrected over the days (family-wise level 0.01; 120
 uncorrected tests at 0.01 would reject a true null in 70% of suites).
 A 1000-versus-1000 run of the same comparison (not part of the suite)
@@ -44,10 +45,13 @@ def ensembles():
     return ref, o.timeseries(days), pop.n_people
 
 
+SUMMARIES = ["attack rate", "peak day", "peak incidence", "cumulative cases", "deaths"]
+
+
 def summary(daily, n):
     attack = (daily[:, -1, 3] + daily[:, -1, 4]) / n
     inc = daily[:, :, 5]
-    return attack, inc.argmax(axis=1), inc.max(axis=1)
+    return attack, inc.argmax(axis=1), inc.max(axis=1), daily[:, :, 6].sum(axis=1), daily[:, -1, 4]
 
 
 def welch_ci_contains_zero(a, b, level=0.95):
@@ -71,20 +75,21 @@ def test_population_conserved():
     assert (ref[:, :, :5].sum(axis=2) == n).all()
 
 
-@pytest.mark.parametrize("name", ["attack rate", "peak day", "peak incidence"])
+@pytest.mark.parametrize("name", SUMMARIES)
 def test_summary_statistics(name):
     ref, orc, n = ensembles()
-    idx = ["attack rate", "peak day", "peak incidence"].index(name)
+    idx = SUMMARIES.index(name)
     a, b = summary(ref, n)[idx].astype(float), summary(orc, n)[idx].astype(float)
     assert stats.ks_2samp(a, b).pvalue > ALPHA, (a.mean(), b.mean())
     assert cis_overlap(a, b), (a.mean(), b.mean())
 
 
-def test_daily_incidence_per_day_ks():
+@pytest.mark.parametrize("field,name", [(5, "infections"), (6, "cases")])
+def test_daily_incidence_per_day_ks(field, name):
     ref, orc, n = ensembles()
     days = ref.shape[1]
-    pv = np.array([stats.ks_2samp(ref[:, d, 5], orc[:, d, 5]).pvalue for d in range(days)])
-    assert pv.min() > ALPHA / days, (pv.argmin(), pv.min())
+    pv = np.array([stats.ks_2samp(ref[:, d, field], orc[:, d, field]).pvalue for d in range(days)])
+    assert pv.min() > ALPHA / days, (name, pv.argmin(), pv.min())
 
 
 def test_infection_routes():

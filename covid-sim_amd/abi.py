@@ -91,6 +91,7 @@ cs_population = _make("cs_population")
 cs_state = _make("cs_state")
 cs_counters = _make("cs_counters")
 cs_work_counts = _make("cs_work_counts")
+cs_transmission_coeffs = _make("cs_transmission_coeffs")
 
 globals().update({k: v for k, v in CONSTS.items()})
 
@@ -119,6 +120,7 @@ def declare(lib, prefix):
         "get_cell_tables": (I, [vp, I, P(F), P(F), P(C.c_int32), P(F), P(C.c_int32)]),
         "eval_hazards": (I, [vp, I, I, C.c_uint16, P(C.c_int32), P(C.c_int32), P(D), I]),
         "icdf_choose": (I, [vp, I, P(D), P(D), P(C.c_uint16), I]),
+        "transmission_coeffs_from_draws": (I, [vp, P(D), P(F), P(C.c_uint16), P(C.c_int8), P(cs_transmission_coeffs)]),
         "set_profiling": (I, [vp, I]),
         "get_profile": (I, [vp, P(D), P(C.c_int64), I]),
         "sync": (I, [vp]),

@@ -12,6 +12,7 @@
 
 #include "../../include/covidsim_hip.h"
 #include "cs_kernels_incub.h"
+#include "cs_kernels_itc.h"
 
 using cs::Dev;
 
@@ -28,6 +29,7 @@ struct cs_engine {
     std::vector<void*> pop_allocs, real_allocs;
     Dev d;
     int* d_sel = nullptr;
+    bool adunits_valid = false;  // every person's administrative unit lies in [0, n_adunits)
     int ts_max = 0, ts_n = 0;
     int steps_done = 0;
     // profiling: event pairs per kernel kind, resolved on cs_get_profile
@@ -443,6 +445,13 @@ int cs_set_population(cs_engine* e, const cs_population* pop)
     for (int h = 0; h < e->NH; h++)
         if (pop->hh_first[h] < 0 || pop->hh_first[h] + pop->hh_nh[h] > N || pop->hh_nhr[h] < 1 || pop->hh_nhr[h] > CS_MAX_HOUSEHOLD_SIZE)
             return set_err(e, CS_EINVAL, "household %d out of range", h);
+    std::vector<int32_t> padunit(N, 0);
+    e->adunits_valid = pop->mcell_adunit != nullptr;
+    for (int i = 0; i < N && e->adunits_valid; i++) {
+        padunit[i] = pop->mcell_adunit[pop->mcell[i]];
+        if (padunit[i] < 0 || padunit[i] >= e->NA) e->adunits_valid = false;
+    }
+    if (!e->adunits_valid) std::fill(padunit.begin(), padunit.end(), 0);
     std::vector<int32_t> mstart(e->NM + 1, 0), mmem(N);
     for (int i = 0; i < N; i++) mstart[pop->mcell[i] + 1]++;
     for (int m = 0; m < e->NM; m++) mstart[m + 1] += mstart[m];
@@ -457,6 +466,7 @@ int cs_set_population(cs_engine* e, const cs_population* pop)
     if ((rc = dupload(e, L, &d.mcell, pop->mcell, N))) return rc;
     if ((rc = dupload(e, L, &d.age, pop->age, N))) return rc;
     if ((rc = dupload(e, L, &d.pflags, pop->flags, N))) return rc;
+    if ((rc = dupload(e, L, &d.padunit, padunit.data(), N))) return rc;
     if ((rc = dupload(e, L, &d.plinks, pop->place_links, (size_t)N * CS_MAX_PLACE_TYPES))) return rc;
     if ((rc = dupload(e, L, &d.pglinks, pop->place_group_links, (size_t)N * CS_MAX_PLACE_TYPES))) return rc;
     if ((rc = dupload(e, L, &d.hfirst, pop->hh_first, e->NH))) return rc;
@@ -980,6 +990,95 @@ int cs_philox(cs_engine* e, const uint32_t* ck, uint32_t* out, int n)
 }
 
 // sampler draws for distribution tests: out[i] from stream {key, c0, c1, i}
+}  // extern "C"
+
+namespace {
+
+// InitTransmissionCoeffs' sums from per-person draws already on the device
+int itc_sums(cs_engine* e, const double* prop_age_group, const float* d_inf, const uint16_t* d_rec, const int8_t* d_st,
+             cs_transmission_coeffs* out)
+{
+    const cs_params& P = e->P;
+    DBuf<double> pa, part, res;
+    if (P.got_waifw_spatial) {
+        if (!prop_age_group) return set_err(e, CS_EINVAL, "prop_age_group is needed with a spatial WAIFW matrix");
+        if (!e->adunits_valid) return set_err(e, CS_EINVAL, "administrative units of the population are missing or out of range");
+        TMP_UP(e, pa, prop_age_group, (size_t)e->NA * CS_NUM_AGE_GROUPS);
+    }
+    const int nb = std::max(1, std::min(2048, (e->N + ITC_BLOCK - 1) / ITC_BLOCK));
+    TMP_UP(e, part, (const double*)nullptr, (size_t)nb * cs::ITC_NSUM);
+    TMP_UP(e, res, (const double*)nullptr, (size_t)cs::ITC_NSUM);
+    cs::k_itc_person<<<nb, ITC_BLOCK, 0, e->stream>>>(e->d, d_inf, d_rec, d_st, pa.p, part.p);
+    cs::k_itc_final<<<1, 64, 0, e->stream>>>(part.p, nb, res.p);
+    HIPCHK(e, hipGetLastError());
+    double v[cs::ITC_NSUM];
+    HIPCHK(e, hipMemcpyAsync(v, res.p, sizeof(v), hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    // the scalar tail of src/SetupModel.cpp:748-845
+    memset(out, 0, sizeof(*out));
+    const double N = (double)e->N;
+    double place = 0;
+    for (int pt = 0; pt < CS_MAX_PLACE_TYPES; pt++) {
+        out->r0_places_by_type[pt] = v[cs::ITC_PLACE0 + pt] / N;
+        place += v[cs::ITC_PLACE0 + pt];
+    }
+    out->household_sar = v[cs::ITC_HH] / v[cs::ITC_HH_DENOM];
+    out->r0_household = v[cs::ITC_HH] / N;
+    out->r0_places = place / N;
+    out->mean_infectious_days = v[cs::ITC_REC_DAYS] / N;
+    out->mean_infectious_steps = v[cs::ITC_REC_STEPS] / N;
+    out->spatial_r0 = v[cs::ITC_SPATIAL] / N;
+    double beta = P.local_beta;
+    if (!P.fix_local_beta) {
+        beta = P.do_si ? (P.r0 / out->spatial_r0 - out->r0_household - out->r0_places)
+                       : (P.r0 - out->r0_household - out->r0_places) / out->spatial_r0;
+        if (beta < 0) beta = 0;
+    }
+    out->local_beta = beta;
+    out->r0_spatial = out->spatial_r0 * beta;
+    out->r0 = out->r0_household + out->r0_places + out->r0_spatial;
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cs_init_transmission_coeffs(cs_engine* e, const double* prop_age_group, uint64_t seed, cs_transmission_coeffs* out)
+{
+    if (!e || !out) return CS_EINVAL;
+    if (!e->have_params || !e->have_pop) return set_err(e, CS_ESTATE, "parameters and population must come first");
+    HIPCHK(e, hipSetDevice(e->device));
+    DBuf<float> inf;
+    DBuf<uint16_t> rec;
+    TMP_UP(e, inf, (const float*)nullptr, (size_t)e->N);
+    TMP_UP(e, rec, (const uint16_t*)nullptr, (size_t)e->N);
+    cs::k_itc_draw<<<grid_for(e->N, 256), 256, 0, e->stream>>>(e->d, seed, inf.p, rec.p);
+    HIPCHK(e, hipGetLastError());
+    int rc = itc_sums(e, prop_age_group, inf.p, rec.p, nullptr, out);
+    if (rc) return rc;
+    if (!e->P.fix_local_beta) {  // the reference sets P.LocalBeta
+        e->P.local_beta = out->local_beta;
+        HIPCHK(e, hipMemcpy(e->dP, &e->P, sizeof(cs_params), hipMemcpyHostToDevice));
+    }
+    return 0;
+}
+
+int cs_transmission_coeffs_from_draws(cs_engine* e, const double* prop_age_group, const float* infectiousness,
+                                      const uint16_t* infectious_steps, const int8_t* infstat, cs_transmission_coeffs* out)
+{
+    if (!e || !out || !infectiousness || !infectious_steps) return CS_EINVAL;
+    if (!e->have_params || !e->have_pop) return set_err(e, CS_ESTATE, "parameters and population must come first");
+    HIPCHK(e, hipSetDevice(e->device));
+    DBuf<float> inf;
+    DBuf<uint16_t> rec;
+    DBuf<int8_t> st;
+    TMP_UP(e, inf, infectiousness, (size_t)e->N);
+    TMP_UP(e, rec, infectious_steps, (size_t)e->N);
+    if (infstat) TMP_UP(e, st, infstat, (size_t)e->N);
+    return itc_sums(e, prop_age_group, inf.p, rec.p, st.p, out);
+}
+
 int cs_sample(cs_engine* e, int which, uint64_t key, uint32_t c0, uint32_t c1, double a, double b, double* out, int n)
 {
     if (!e || !out || n < 0 || which < 0 || which > 4) return CS_EINVAL;

@@ -44,6 +44,7 @@ struct Dev {
     // population (static)
     const int32_t *hh, *pc, *mcell, *plinks, *hfirst, *cstart, *cn, *cidx, *mstart, *mmem, *cmem;
     const uint8_t *age, *pflags;
+    const int32_t* padunit;  // administrative unit of each person's microcell
     const uint16_t *pglinks, *hnh, *hnhr;
     const float* hloc;
     int np[CS_MAX_PLACE_TYPES];

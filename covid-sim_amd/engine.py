@@ -48,6 +48,8 @@ def lib():
         L.cs_sample.argtypes = [C.c_void_p, C.c_int, C.c_uint64, C.c_uint32, C.c_uint32, C.c_double, C.c_double,
                                 P(C.c_double), C.c_int]
         L.cs_sample.restype = C.c_int
+        L.cs_init_transmission_coeffs.argtypes = [C.c_void_p, P(C.c_double), C.c_uint64, P(abi.cs_transmission_coeffs)]
+        L.cs_init_transmission_coeffs.restype = C.c_int
         L.cs_version.restype = C.c_char_p
         L.cs_last_error.restype = C.c_char_p
         L.cs_last_error.argtypes = [C.c_void_p]
@@ -161,6 +163,24 @@ class Engine:
         return {f: getattr(w, f) for f, _ in w._fields_}
 
     # ---- components ----
+    def transmission_coeffs(self, seed, prop_age_group=None):
+        """InitTransmissionCoeffs with per-person draws from the stream `seed`;
+        sets the engine's local_beta unless the parameters fix it."""
+        out = abi.cs_transmission_coeffs()
+        pa = None if prop_age_group is None else np.ascontiguousarray(prop_age_group, np.float64)
+        self.call("init_transmission_coeffs", None if pa is None else _p(pa, C.c_double), int(seed), C.byref(out))
+        return out
+
+    def transmission_coeffs_from_draws(self, infness, rec, infstat=None, prop_age_group=None):
+        out = abi.cs_transmission_coeffs()
+        infness = np.ascontiguousarray(infness, np.float32)
+        rec = np.ascontiguousarray(rec, np.uint16)
+        st = None if infstat is None else np.ascontiguousarray(infstat, np.int8)
+        pa = None if prop_age_group is None else np.ascontiguousarray(prop_age_group, np.float64)
+        self.call("transmission_coeffs_from_draws", None if pa is None else _p(pa, C.c_double), _p(infness, C.c_float),
+                  _p(rec, C.c_uint16), None if st is None else _p(st, C.c_int8), C.byref(out))
+        return out
+
     def kernel_num(self, r2):
         r2 = np.ascontiguousarray(r2, np.float64)
         out = np.zeros_like(r2)

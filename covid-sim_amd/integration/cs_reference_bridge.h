@@ -176,6 +176,13 @@ inline void fill_params(cs_params* c)
         c->holiday_duration[i] = P.HolidayDuration[i];
     }
     c->holidays_start_day_simtime = P.HolidaysStartDay_SimTime;
+    c->r0 = P.R0;
+    c->fix_local_beta = P.FixLocalBeta;
+    c->do_si = P.DoSI;
+    c->do_partial_immunity = P.DoPartialImmunity;
+    c->got_waifw_spatial = P.Got_WAIFW_Matrix_Spatial;
+    for (int a = 0; a < NUM_AGE_GROUPS; a++) c->initial_immunity[a] = P.InitialImmunity[a];
+    c->care_home_prop_residents = P.CareHomePropResidents;
 }
 
 // InfStat code of a person, read through the public predicates

@@ -175,6 +175,14 @@ typedef struct cs_params {
     double holiday_effect[CS_MAX_PLACE_TYPES];
     double holiday_start[CS_MAX_HOLIDAYS], holiday_duration[CS_MAX_HOLIDAYS];
     double holidays_start_day_simtime;
+    /* InitTransmissionCoeffs inputs (src/SetupModel.cpp:640-846) */
+    double r0;                         /* P.R0, the target reproduction number */
+    int    fix_local_beta;             /* P.FixLocalBeta: keep local_beta as given */
+    int    do_si;                      /* P.DoSI */
+    int    do_partial_immunity;        /* P.DoPartialImmunity */
+    int    got_waifw_spatial;          /* P.Got_WAIFW_Matrix_Spatial */
+    double initial_immunity[CS_NUM_AGE_GROUPS];
+    double care_home_prop_residents;   /* P.CareHomePropResidents */
 } cs_params;
 
 /*
@@ -347,6 +355,37 @@ int  cs_get_counters(cs_engine* e, int real, cs_counters* out, int reset);
 int  cs_get_timeseries(cs_engine* e, int32_t* out, int max_samples, int* n_samples);
 int  cs_get_work_counts(cs_engine* e, cs_work_counts* out, int reset);
 int  cs_sync(cs_engine* e);
+
+/* InitTransmissionCoeffs (src/SetupModel.cpp:640-846): the reproduction
+ * numbers of one pass over the population with per-person draws of
+ * infectiousness and infectious period, and the spatial beta that makes the
+ * total equal cs_params.r0 (unless fix_local_beta).  The draws come from the
+ * counter-based stream keyed by `seed`.  prop_age_group is P.PropAgeGroup
+ * ([n_adunits][CS_NUM_AGE_GROUPS]), read only when got_waifw_spatial; may be
+ * NULL otherwise.  On success, unless fix_local_beta, the engine's local_beta
+ * is set to the result (the reference sets P.LocalBeta). */
+typedef struct cs_transmission_coeffs {
+    double r0_household;               /* P.R0household */
+    double household_sar;              /* household secondary attack rate */
+    double r0_places;                  /* P.R0places */
+    double r0_places_by_type[CS_MAX_PLACE_TYPES];
+    double spatial_r0;                 /* spatial R0 per unit local_beta */
+    double local_beta;                 /* P.LocalBeta */
+    double r0_spatial;                 /* P.R0spatial */
+    double r0;                         /* overall R0 */
+    double mean_infectious_days;       /* mean sampled infectious period */
+    double mean_infectious_steps;
+} cs_transmission_coeffs;
+int  cs_init_transmission_coeffs(cs_engine* e, const double* prop_age_group, uint64_t seed,
+                                 cs_transmission_coeffs* out);
+/* the same pass from given per-person draws: infectiousness (signed: <0 is
+ * symptomatic, as Person.infectiousness), infectious period in timesteps
+ * (Person.recovery_or_death_time) and, optionally, the InfStat code that
+ * decides which household members count as susceptible (NULL: all).  The
+ * engine's parameters are not changed. */
+int  cs_transmission_coeffs_from_draws(cs_engine* e, const double* prop_age_group, const float* infectiousness,
+                                       const uint16_t* infectious_steps, const int8_t* infstat,
+                                       cs_transmission_coeffs* out);
 
 /* per-kernel device time, measured with HIP events recorded around each
  * launch on the engine's stream while profiling is enabled */

@@ -174,7 +174,7 @@ static double rs_next(rs* s)
 
 /* pass identifiers (high half of counter word 0) */
 enum { PS_HH = 1, PS_PW_SAMPLE, PS_PW_ACCEPT, PS_PB_SAMPLE, PS_PB_ACCEPT, PS_SP_COUNT,
-       PS_SP_DRAW, PS_INFECT, PS_INCUB, PS_CASE, PS_RECOVER, PS_HOLIDAY, PS_INIT, PS_SEED };
+       PS_SP_DRAW, PS_INFECT, PS_INCUB, PS_CASE, PS_RECOVER, PS_HOLIDAY, PS_INIT, PS_SEED, PS_ITC };
 /* pass identifiers inside the event order key */
 enum { KP_HH = 0, KP_PW = 1, KP_PB = 6, KP_SP = 11 };
 
@@ -1807,6 +1807,166 @@ int or_get_work_counts(or_engine* e, cs_work_counts* out, int reset)
 }
 
 /* --------------------------------------------------- component access */
+
+/* ---- InitTransmissionCoeffs (src/SetupModel.cpp:640-846) ----
+ * Sums accumulate in the reference's order with one thread (person order,
+ * day order inside a person), so from the same draws the results equal the
+ * reference's bit for bit. */
+
+/* survival product over infectious steps [d0, d1) (src/SetupModel.cpp:690-694, :780-792) */
+static double itc_survive(const cs_params* P, double ps, double rate, int d0, int d1)
+{
+    for (int d = d0; d < d1 && d < CS_MAX_INFECTIOUS_STEPS; d++) {
+        double today = 1.0 - rate * P->infectiousness[d];
+        ps *= (today < 0) ? 0 : today;
+    }
+    return ps;
+}
+
+/* per-person draws of src/SetupModel.cpp:667-690 from the stream (seed, ITC, person) */
+static void itc_draw(const or_engine* e, uint64_t seed, int p, float* susc, float* infness, uint16_t* rec)
+{
+    const cs_params* P = &e->P;
+    const int a = AGE_GROUP(e, p);
+    rs s;
+    rs_init(&s, seed, cw0(0, PS_ITC), (uint32_t)p, 0);
+    const double imm = P->do_partial_immunity ? (1.0 - P->initial_immunity[a]) : 1.0;
+    if (P->susceptibility_sd == 0) *susc = (float)imm;
+    else {
+        double k = 1 / (P->susceptibility_sd * P->susceptibility_sd);
+        *susc = (float)(imm * smp_gamma(&s, k, k));
+    }
+    float in;
+    if (P->infectiousness_sd == 0) in = (float)P->age_infectiousness[a];
+    else {
+        double k = 1 / (P->infectiousness_sd * P->infectiousness_sd);
+        in = (float)(P->age_infectiousness[a] * smp_gamma(&s, k, k));
+    }
+    if (rs_next(&s) < P->proportion_symptomatic[a]) in *= (float)(-P->sympt_infectiousness);
+    else in *= (float)P->asympt_infectiousness;
+    *infness = in;
+    double q = rs_next(&s) * CS_CDF_RES;
+    int j = (int)floor(q);
+    q -= (double)j;
+    const double* ic = P->icdf[CS_ICDF_INFECTIOUS];
+    *rec = (uint16_t)floor(0.5 - (P->infectious_period * or_log(q * ic[j + 1] + (1.0 - q) * ic[j]) / P->model_time_step));
+}
+
+static int itc_sums(or_engine* e, const double* prop_age_group, const float* infness, const uint16_t* rec,
+                    const int8_t* infstat, cs_transmission_coeffs* out)
+{
+    const cs_params* P = &e->P;
+    const double dt = P->model_time_step;
+    if (P->got_waifw_spatial && !prop_age_group) return fail(e, CS_EINVAL, "prop_age_group is needed with a spatial WAIFW matrix");
+    double hh_inf = 0, sp_inf = 0, hh_denom = 0;
+    for (int p = 0; p < e->N; p++) {
+        const int a = AGE_GROUP(e, p);
+        const double inf = (double)infness[p];
+        if (P->do_households) {
+            double hi = P->no_infectiousness_sd_in_hh ? ((infness[p] < 0) ? P->sympt_infectiousness : P->asympt_infectiousness)
+                                                       : fabs(inf);
+            if (CARE(e, p)) hi *= P->care_home_resident_household_scaling;
+            const int h = e->hh[p];
+            hi *= dt * P->household_trans * P->household_denom_lookup[e->hnhr[h] - 1];
+            const double ps = itc_survive(P, 1.0, hi, 0, (int)rec[p]);
+            for (int m = e->hfirst[h]; m < e->hfirst[h] + e->hnh[h]; m++)
+                if ((!infstat || infstat[m] == CS_SUSCEPTIBLE) && m != p)
+                    hh_inf += (1 - ps) * P->age_susceptibility[a] * (CARE(e, m) ? P->care_home_resident_household_scaling : 1.0);
+            hh_denom += (double)(e->hnhr[h] - 1);
+        }
+        const double lsd = (P->latent_to_sympt_delay > rec[p] * dt) ? rec[p] * dt : P->latent_to_sympt_delay;
+        double si = fabs(inf) * P->relative_spatial_contact[a] * (CARE(e, p) ? P->care_home_resident_spatial_scaling : 1.0) * dt;
+        if (P->got_waifw_spatial) {
+            double av = 0;
+            const double* pa = prop_age_group + (size_t)e->madunit[e->mcell[p]] * CS_NUM_AGE_GROUPS;
+            for (int b = 0; b < CS_NUM_AGE_GROUPS; b++) av += pa[b] * P->waifw_spatial[b][a];
+            si *= av;
+        }
+        const int nds = (int)(lsd / dt);
+        int d;
+        for (d = 0; d < nds && d < CS_MAX_INFECTIOUS_STEPS; d++) sp_inf += si * P->infectiousness[d];
+        si *= (infness[p] < 0) ? P->sympt_spatial_contact_rate : 1;
+        for (; d < (int)rec[p] && d < CS_MAX_INFECTIOUS_STEPS; d++) sp_inf += si * P->infectiousness[d];
+    }
+    memset(out, 0, sizeof(*out));
+    const double spatial_r0 = sp_inf / (double)e->N;
+    out->household_sar = hh_inf / hh_denom;
+    out->r0_household = hh_inf / (double)e->N;
+    double place_inf = 0;
+    if (P->do_places)
+        for (int pt = 0; pt < e->NPT; pt++) {
+            if (pt == P->hotel_place_type) continue;
+            /* the reduction's private sum starts at 0 and joins the total
+             * after the loop (src/SetupModel.cpp:759) */
+            double type_inf = 0;
+            for (int p = 0; p < e->N; p++) {
+                const int k = e->plinks[(size_t)p * CS_MAX_PLACE_TYPES + pt];
+                if (k < 0) continue;
+                const double inf = (double)infness[p];
+                const double lsd = (P->latent_to_sympt_delay > rec[p] * dt) ? rec[p] * dt : P->latent_to_sympt_delay;
+                const double pinf = fabs(inf) * dt * P->place_type_trans[pt];
+                double scaled = pinf / P->place_type_group_size_param1[pt];
+                const int nds = (int)(lsd / dt);
+                double ps = itc_survive(P, 1.0, scaled, 0, nds);
+                const int g = e->pglinks[(size_t)p * CS_MAX_PLACE_TYPES + pt];
+                const double ngroup = (double)((int64_t)e->pl_gsz[pt][e->pl_gs0[pt][k] + g] - 1);
+                const double sympt_rate = ((infness[p] < 0) && !CARE(e, p))
+                    ? (P->sympt_place_type_contact_rate[pt] * (1 - P->sympt_place_type_withdrawal_prop[pt])) : 1;
+                scaled *= sympt_rate;
+                ps = itc_survive(P, ps, scaled, nds, (int)rec[p]);
+                const double ps_group = ps;
+                const double n = (double)e->pl_n[pt][k];
+                scaled = P->place_type_prop_between_group_links[pt] * pinf / n;
+                if (CARE(e, p))
+                    scaled *= (1.0 - P->care_home_prop_residents) + P->care_home_prop_residents *
+                              (P->care_home_worker_group_scaling * ((n - 1) - ngroup) + ngroup) / (n - 1);
+                ps = itc_survive(P, 1.0, scaled, 0, nds);
+                scaled *= sympt_rate;
+                ps = itc_survive(P, ps, scaled, nds, (int)rec[p]);
+                type_inf += (1 - ps_group * ps) * ngroup + (1 - ps) * (((double)((int64_t)e->pl_n[pt][k] - 1)) - ngroup);
+            }
+            place_inf += type_inf;
+            out->r0_places_by_type[pt] = type_inf / (double)e->N;
+        }
+    double rec_days = 0, rec_steps = 0;
+    for (int p = 0; p < e->N; p++) { rec_days += rec[p] * dt; rec_steps += rec[p]; }
+    out->r0_places = place_inf / (double)e->N;
+    out->mean_infectious_days = rec_days / (double)e->N;
+    out->mean_infectious_steps = rec_steps / (double)e->N;
+    double beta = P->local_beta;
+    if (!P->fix_local_beta) {
+        beta = P->do_si ? (P->r0 / spatial_r0 - out->r0_household - out->r0_places)
+                        : (P->r0 - out->r0_household - out->r0_places) / spatial_r0;
+        if (beta < 0) beta = 0;
+    }
+    out->spatial_r0 = spatial_r0;
+    out->local_beta = beta;
+    out->r0_spatial = spatial_r0 * beta;
+    out->r0 = out->r0_household + out->r0_places + out->r0_spatial;
+    return 0;
+}
+
+int or_transmission_coeffs_from_draws(or_engine* e, const double* prop_age_group, const float* infness,
+                                      const uint16_t* rec, const int8_t* infstat, cs_transmission_coeffs* out)
+{
+    return itc_sums(e, prop_age_group, infness, rec, infstat, out);
+}
+
+int or_init_transmission_coeffs(or_engine* e, const double* prop_age_group, uint64_t seed,
+                                cs_transmission_coeffs* out, float* susc_out, float* infness_out, uint16_t* rec_out)
+{
+    float* su = (float*)malloc(sizeof(float) * (size_t)e->N);
+    float* in = (float*)malloc(sizeof(float) * (size_t)e->N);
+    uint16_t* rc = (uint16_t*)malloc(sizeof(uint16_t) * (size_t)e->N);
+    for (int p = 0; p < e->N; p++) itc_draw(e, seed, p, su + p, in + p, rc + p);
+    int ret = itc_sums(e, prop_age_group, in, rc, NULL, out);
+    if (!ret && !e->P.fix_local_beta) e->P.local_beta = out->local_beta;
+    if (susc_out) memcpy(susc_out, su, sizeof(float) * (size_t)e->N);
+    if (infness_out) memcpy(infness_out, in, sizeof(float) * (size_t)e->N);
+    if (rec_out) memcpy(rec_out, rc, sizeof(uint16_t) * (size_t)e->N);
+    free(su); free(in); free(rc);
+    return ret;
+}
 
 int or_kernel_num(or_engine* e, const double* r2, double* out, int n)
 {

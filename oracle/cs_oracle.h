@@ -39,6 +39,12 @@ int  or_get_counters(or_engine* e, int real, cs_counters* out, int reset);
 int  or_get_timeseries(or_engine* e, int32_t* out, int max_samples, int* n_samples);
 int  or_get_work_counts(or_engine* e, cs_work_counts* out, int reset);
 
+/* InitTransmissionCoeffs: draws + sums, and the sums from given draws */
+int  or_init_transmission_coeffs(or_engine* e, const double* prop_age_group, uint64_t seed,
+                                 cs_transmission_coeffs* out, float* susc_out, float* infness_out, uint16_t* rec_out);
+int  or_transmission_coeffs_from_draws(or_engine* e, const double* prop_age_group, const float* infness,
+                                       const uint16_t* rec, const int8_t* infstat, cs_transmission_coeffs* out);
+
 int  or_kernel_num(or_engine* e, const double* r2, double* out, int n);
 int  or_dist2_persons(or_engine* e, const int32_t* a, const int32_t* b, double* out, int n);
 int  or_get_cell_tables(or_engine* e, int real, float* max_trans, float* cum_trans,

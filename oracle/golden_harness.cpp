@@ -15,6 +15,13 @@
 //                       (src/InverseCdf.cpp:60-74) with its uniform, and the
 //                       eight hazards of src/CalcInfSusc.cpp over a grid of
 //                       synthetic person states
+//   CS_DUMP_ITC=1       at the first call, after the dumps above, re-run the
+//                       reference's InitTransmissionCoeffs
+//                       (src/SetupModel.cpp:640-846) from a fixed RNG seed,
+//                       replay its per-person draws with the same generator
+//                       to recover them (one thread: the draws come in person
+//                       order), write them with the resulting coefficients
+//                       (itc_*.npy) and exit; the run does not continue
 //   CS_SERIES=<file>    append one record per step and realisation:
 //                       run, TimeStepNow, S, L, I, R, D, cumI, cumC, cumD,
 //                       cumItype[16] (int32 each), read after IncubRecoverySweep
@@ -50,6 +57,7 @@
 #include "ModelMacros.h"
 #include "Param.h"
 #include "Rand.h"
+#include "SetupModel.h"
 #include "Sweep.h"
 
 #include "covidsim_hip.h"
@@ -350,6 +358,48 @@ void init_once()
     if (se) g_series = fopen(se, "ab");
 }
 
+// InitTransmissionCoeffs' per-person draws and results, for the oracle's
+// restatement of its deterministic part
+void dump_itc()
+{
+    const int N = P.PopSize;
+    if (P.NumThreads != 1) { fprintf(stderr, "CS_DUMP_ITC needs /c:1\n"); exit(2); }
+    std::vector<int8_t> infstat(N);
+    for (int i = 0; i < N; i++) infstat[i] = (int8_t)cs_bridge::infstat_code(Hosts[i]);
+    const double r0_in = P.R0, beta_before = P.LocalBeta;
+    int32_t s1 = 1234567, s2 = 7654321;
+    setall(&s1, &s2);
+    InitTransmissionCoeffs();
+    std::vector<double> out = {r0_in, P.R0household, P.R0places, P.LocalBeta, P.R0spatial, P.R0, beta_before};
+    std::vector<float> infness(N), susc(N);
+    for (int i = 0; i < N; i++) { infness[i] = Hosts[i].infectiousness; susc[i] = Hosts[i].susc; }
+    // replay the draws of src/SetupModel.cpp:667-690 from the same seeds
+    s1 = 1234567; s2 = 7654321;
+    setall(&s1, &s2);
+    std::vector<uint16_t> rec(N);
+    for (int i = 0; i < N; i++) {
+        const int a = HOST_AGE_GROUP(i);
+        float su = 1.0f, in;
+        if (P.SusceptibilitySD != 0)
+            su = (float)(((P.DoPartialImmunity) ? (1.0 - P.InitialImmunity[a]) : 1.0) *
+                         gen_gamma_mt(1 / (P.SusceptibilitySD * P.SusceptibilitySD), 1 / (P.SusceptibilitySD * P.SusceptibilitySD), 0));
+        else
+            su = (float)((P.DoPartialImmunity) ? (1.0 - P.InitialImmunity[a]) : 1.0);
+        if (P.InfectiousnessSD == 0) in = (float)P.AgeInfectiousness[a];
+        else in = (float)(P.AgeInfectiousness[a] * gen_gamma_mt(1 / (P.InfectiousnessSD * P.InfectiousnessSD), 1 / (P.InfectiousnessSD * P.InfectiousnessSD), 0));
+        if (ranf_mt(0) < P.ProportionSymptomatic[a]) in *= (float)(-P.SymptInfectiousness);
+        else in *= (float)P.AsymptInfectiousness;
+        double q;
+        int j = (int)floor((q = ranf_mt(0) * CDF_RES));
+        q -= (double)j;
+        rec[i] = (unsigned short)floor(0.5 - (P.InfectiousPeriod * log(q * P.infectious_icdf[j + 1] + (1.0 - q) * P.infectious_icdf[j]) / P.ModelTimeStep));
+        if (in != infness[i] || su != susc[i]) { fprintf(stderr, "ITC replay diverged at person %d\n", i); exit(3); }
+    }
+    save("itc_infectiousness", infness); save("itc_susc", susc); save("itc_recovery", rec);
+    save("itc_infstat", infstat); save("itc_out", out);
+    dump_params("itc_params.bin");
+}
+
 int g_run_of_call = -1;
 
 } // namespace
@@ -364,6 +414,10 @@ void InfectSweep(double t, int run)
             dump_population();
             dump_params("params.bin");
             if (g_golden) dump_golden();
+        }
+        if (ts == 0 && getenv("CS_DUMP_ITC") && atoi(getenv("CS_DUMP_ITC"))) {
+            dump_itc();
+            exit(0);
         }
         for (int s : g_steps)
             if (s == ts) dump_state("s" + std::to_string(ts) + "_", ts);

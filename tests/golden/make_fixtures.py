@@ -9,6 +9,12 @@ from /root/reference) on the 20k-person fixture region and packs:
   mini_params.bin       cs_params of that run (include/covidsim_hip.h)
   mini_hz_params.bin    cs_params with the perturbed intervention
                         parameters the hazard vectors were evaluated under
+  mini_itc.npz          InitTransmissionCoeffs re-run by the reference from a
+                        fixed seed: its per-person draws (infectiousness,
+                        susceptibility, infectious period in steps), the
+                        InfStat codes it saw, and its results (r0 in,
+                        R0household, R0places, LocalBeta, R0spatial, R0 out,
+                        LocalBeta before); mini_itc_params.bin its cs_params
   ../../covid-sim_amd/inputs/{c1,mini}/params.bin
                         the derived parameter sets (cs_params) the
                         reference's ReadParams produces for those inputs
@@ -31,8 +37,9 @@ HARNESS = os.path.join(ROOT, "oracle", "_ref", "golden_harness")
 SEEDS = ["98798150", "729101", "17389101", "4797132"]
 
 
-def run_harness(inputs, out, steps="0", golden=True, threads=1, extra=(), series=None, nr=None):
+def run_harness(inputs, out, steps="0", golden=True, threads=1, extra=(), series=None, nr=None, extra_env=None):
     env = dict(os.environ, CS_DUMP_DIR=out, CS_DUMP_STEPS=steps, CS_DUMP_GOLDEN="1" if golden else "0")
+    env.update(extra_env or {})
     if series:
         env["CS_SERIES"] = series
     cmd = [HARNESS, "/c:%d" % threads,
@@ -74,6 +81,15 @@ def main():
         shutil.copy(os.path.join(tmp, "params.bin"), os.path.join(HERE, "mini_params.bin"))
         shutil.copy(os.path.join(tmp, "hz_params.bin"), os.path.join(HERE, "mini_hz_params.bin"))
         print("wrote", len(arrays), "arrays")
+    finally:
+        shutil.rmtree(tmp)
+    tmp = tempfile.mkdtemp()
+    try:
+        run_harness(inputs, tmp, steps="", golden=False, extra_env={"CS_DUMP_ITC": "1"})
+        itc = {fn[:-4]: np.load(os.path.join(tmp, fn)) for fn in os.listdir(tmp) if fn.startswith("itc_") and fn.endswith(".npy")}
+        np.savez_compressed(os.path.join(HERE, "mini_itc.npz"), **itc)
+        shutil.copy(os.path.join(tmp, "itc_params.bin"), os.path.join(HERE, "mini_itc_params.bin"))
+        print("wrote", sorted(itc))
     finally:
         shutil.rmtree(tmp)
 

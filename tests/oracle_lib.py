@@ -30,6 +30,9 @@ def load():
     lib.or_sample.argtypes = [C.c_int, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_double, C.c_double,
                               C.c_int, P(C.c_double)]
     lib.or_sample.restype = C.c_int
+    lib.or_init_transmission_coeffs.argtypes = [C.c_void_p, P(C.c_double), C.c_uint64, P(abi.cs_transmission_coeffs),
+                                                P(C.c_float), P(C.c_float), P(C.c_uint16)]
+    lib.or_init_transmission_coeffs.restype = C.c_int
     lib.or_log.argtypes = [C.c_double]
     lib.or_log.restype = C.c_double
     lib.or_exp.argtypes = [C.c_double]
@@ -168,6 +171,28 @@ class OracleEngine(Oracle):
         u = np.ascontiguousarray(u, np.float64)
         out = np.zeros(len(u), np.int32)
         self.check(self.lib.or_select_dest_cell(self.h, int(r), int(cell), _p(u, C.c_double), _p(out, C.c_int32), len(u)))
+        return out
+
+    def transmission_coeffs(self, seed, prop_age_group=None, draws=False):
+        """InitTransmissionCoeffs with the stream `seed`; with draws=True also
+        the per-person (susc, infectiousness, infectious steps)."""
+        out = self.abi.cs_transmission_coeffs()
+        n = self.pop.n_people
+        su, inf, rec = np.zeros(n, np.float32), np.zeros(n, np.float32), np.zeros(n, np.uint16)
+        pa = None if prop_age_group is None else np.ascontiguousarray(prop_age_group, np.float64)
+        self.check(self.lib.or_init_transmission_coeffs(self.h, None if pa is None else _p(pa, C.c_double), int(seed),
+                                                        C.byref(out), _p(su, C.c_float), _p(inf, C.c_float),
+                                                        _p(rec, C.c_uint16)))
+        return (out, su, inf, rec) if draws else out
+
+    def transmission_coeffs_from_draws(self, infness, rec, infstat=None, prop_age_group=None):
+        out = self.abi.cs_transmission_coeffs()
+        infness = np.ascontiguousarray(infness, np.float32)
+        rec = np.ascontiguousarray(rec, np.uint16)
+        st = None if infstat is None else np.ascontiguousarray(infstat, np.int8)
+        pa = None if prop_age_group is None else np.ascontiguousarray(prop_age_group, np.float64)
+        self.call("transmission_coeffs_from_draws", None if pa is None else _p(pa, C.c_double), _p(infness, C.c_float),
+                  _p(rec, C.c_uint16), None if st is None else _p(st, C.c_int8), C.byref(out))
         return out
 
     def philox(self, ck):

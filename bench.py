@@ -41,9 +41,11 @@ HBM_PEAK_GBS = 8000.0
 # minimum bytes touched per event (the per-event costs of the step's
 # algorithmic-bytes formula); fixed, never tuned to the measurement
 BYTES = {"n_inf": 64, "n_hh_inf": 16, "n_hh_member": 40, "n_place_link": 32, "n_place_cand": 44,
-         "n_sp_draw": 48, "n_sp_susc": 40, "n_new": 40, "n_active": 16, "n_transition": 30}
+         "n_sp_draw": 48, "n_sp_susc": 40, "n_new": 40, "n_active": 16, "n_transition": 30,
+         "n_cell_steps": 32}
 KERNEL_UNITS = {
-    "infect_cells": ["n_inf", "n_hh_inf", "n_hh_member", "n_place_link", "n_place_cand"],
+    "infect_persons": ["n_inf", "n_hh_inf", "n_hh_member", "n_place_link", "n_place_cand"],
+    "cell_spatial_sum": ["n_cell_steps"],
     "spatial_draws": ["n_sp_draw", "n_sp_susc"],
     "resolve": ["n_new"],
     "latent_scan": ["n_active"],
@@ -123,6 +125,7 @@ def main():
     ap.add_argument("--realisations", type=int, default=64)
     ap.add_argument("--days", type=int, default=300)
     ap.add_argument("--warmup-days", type=int, default=20, help="days simulated by each warmup step")
+    ap.add_argument("--chunk-days", type=int, default=50, help="days per run_steps call inside a step (progress lines)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -161,16 +164,21 @@ def main():
     eng.work_counts(reset=True)
     eng.set_profiling(True)
     times = []
+    chunk = max(1, args.chunk_days) * tspd
     for k in range(args.steps):
         eng.sync()
         ta = time.perf_counter()
         eng.reset(10)
-        eng.run_steps(0.0, steps, tspd)
-        eng.sync()
+        for s0 in range(0, steps, chunk):
+            n = min(chunk, steps - s0)
+            eng.run_steps(s0 / tspd, n, tspd)  # one series across the calls
+            eng.sync()
+            log("  step %d: day %d (%.1f s)" % (k, (s0 + n) // tspd, time.perf_counter() - ta))
         times.append(time.perf_counter() - ta)
         log("step %d: %.2f s" % (k, times[-1]))
     prof = eng.profile()
     wc = eng.work_counts()
+    wc["n_cell_steps"] = args.realisations * pop.n_cells * prof["cell_spatial_sum"][1]  # cells visited per launch
     ts = eng.timeseries(args.days)
     total = float(sum(times))
     person_days = float(args.realisations) * args.people * args.days * args.steps

@@ -31,6 +31,9 @@ struct cs_engine {
     int* d_sel = nullptr;
     bool adunits_valid = false;  // every person's administrative unit lies in [0, n_adunits)
     int ts_max = 0, ts_n = 0;
+    bool ts_fresh = true;
+    unsigned grid_waves = 4096;   // blocks of 4 waves for the per-person passes (a wave per chunk)
+    unsigned grid_cells = 2048;   // blocks for the per-cell passes (a block per cell)  // no sample since the last alloc/reset/load: the series restarts
     int steps_done = 0;
     // profiling: event pairs per kernel kind, resolved on cs_get_profile
     bool prof = false;
@@ -200,10 +203,16 @@ int launch_infect_events(cs_engine* e, const cs::StepScalars& sc)
     Dev& d = e->d;
     hipStream_t st = e->stream;
     const unsigned nrc = (unsigned)((long long)d.R * d.NC);
-    cs::k_zero_u32<<<1, 1, 0, st>>>(d.ev_count);
-    { ProfScope ps(e, CS_K_INFECT_CELLS); cs::k_infect_cells<<<nrc, 64, 0, st>>>(d, sc); }
-    { ProfScope ps(e, CS_K_OTHER); cs::k_scan_draws<<<1, 1024, 0, st>>>(d); }
-    { ProfScope ps(e, CS_K_SPATIAL_DRAWS); cs::k_spatial_draws<<<2048, 256, 0, st>>>(d, sc); }
+    {
+        ProfScope ps(e, CS_K_OTHER);
+        cs::k_zero_counts<<<1, 64, 0, st>>>(d.ev_count, 1);
+        cs::k_zero_counts<<<1, 64, 0, st>>>(d.wl, cs::WL_N);
+        cs::k_compact<<<grid_for(nrc, 256), 256, 0, st>>>(d, 1, 1);
+    }
+    { ProfScope ps(e, CS_K_INFECT_PERSONS); cs::k_infect_persons<<<e->grid_waves, 256, 0, st>>>(d, sc); }
+    { ProfScope ps(e, CS_K_CELL_SPATIAL_SUM); cs::k_cell_spatial_sum<<<e->grid_cells, CS_SUM_BLOCK, 0, st>>>(d, sc); }
+    { ProfScope ps(e, CS_K_OTHER); cs::k_compact<<<grid_for(nrc, 256), 256, 0, st>>>(d, 2, 0); }
+    { ProfScope ps(e, CS_K_SPATIAL_DRAWS); cs::k_spatial_draws<<<e->grid_waves, 256, 0, st>>>(d, sc); }
     HIPCHK(e, hipGetLastError());
     return 0;
 }
@@ -212,14 +221,13 @@ int launch_resolve(cs_engine* e, const cs::StepScalars& sc)
 {
     Dev& d = e->d;
     hipStream_t st = e->stream;
-    const unsigned nrc = (unsigned)((long long)d.R * d.NC);
     {
         ProfScope ps(e, CS_K_RESOLVE);
         cs::k_resolve_claim<<<1024, 256, 0, st>>>(d);
         cs::k_resolve_apply<<<1024, 256, 0, st>>>(d, sc);
         cs::k_resolve_reset<<<1024, 256, 0, st>>>(d);
     }
-    { ProfScope ps(e, CS_K_LIST_MOVE); cs::k_list_move<<<nrc, 256, 0, st>>>(d, 0); }
+    { ProfScope ps(e, CS_K_LIST_MOVE); cs::k_list_move<<<e->grid_cells, 256, 0, st>>>(d, 0); }
     HIPCHK(e, hipGetLastError());
     return 0;
 }
@@ -239,16 +247,26 @@ int launch_incub(cs_engine* e, double t)
                 cs::k_holidays<<<grid_for(n, 256), 256, 0, st>>>(d, h, ht);
             }
         }
-    { ProfScope ps(e, CS_K_LATENT_SCAN); cs::k_latent_scan<<<nrc, 64, 0, st>>>(d, ts); }
-    { ProfScope ps(e, CS_K_LIST_MOVE); cs::k_list_move<<<nrc, 256, 0, st>>>(d, 1); }
-    cs::k_zero_u32<<<1, 1, 0, st>>>(d.care_count);
-    { ProfScope ps(e, CS_K_INFECTIOUS_SCAN); cs::k_infectious_scan<<<nrc, 64, 0, st>>>(d, ts); }
+    {
+        ProfScope ps(e, CS_K_OTHER);
+        cs::k_zero_counts<<<1, 64, 0, st>>>(d.wl, cs::WL_N);
+        cs::k_compact<<<grid_for(nrc, 256), 256, 0, st>>>(d, 0, 0);
+    }
+    { ProfScope ps(e, CS_K_LATENT_SCAN); cs::k_latent_scan<<<e->grid_waves, 256, 0, st>>>(d, ts); }
+    { ProfScope ps(e, CS_K_LIST_MOVE); cs::k_list_move<<<e->grid_cells, 256, 0, st>>>(d, 1); }
+    {
+        ProfScope ps(e, CS_K_OTHER);
+        cs::k_zero_counts<<<1, 64, 0, st>>>(d.care_count, 1);
+        cs::k_zero_counts<<<1, 64, 0, st>>>(d.wl, cs::WL_N);
+        cs::k_compact<<<grid_for(nrc, 256), 256, 0, st>>>(d, 1, 0);
+    }
+    { ProfScope ps(e, CS_K_INFECTIOUS_SCAN); cs::k_infectious_scan<<<e->grid_waves, 256, 0, st>>>(d, ts); }
     {
         ProfScope ps(e, CS_K_OTHER);
         cs::k_care_eval<<<256, 256, 0, st>>>(d, ts);
         cs::k_care_apply<<<256, 256, 0, st>>>(d, ts);
     }
-    { ProfScope ps(e, CS_K_LIST_MOVE); cs::k_list_move<<<nrc, 256, 0, st>>>(d, 2); }
+    { ProfScope ps(e, CS_K_LIST_MOVE); cs::k_list_move<<<e->grid_cells, 256, 0, st>>>(d, 2); }
     HIPCHK(e, hipGetLastError());
     return 0;
 }
@@ -279,6 +297,12 @@ int grow_events(cs_engine* e)
     if ((rc = dalloc(e, e->real_allocs, &d.ev_infector, cap))) return rc;
     if ((rc = dalloc(e, e->real_allocs, &d.ev_type, cap))) return rc;
     d.ev_cap = cap;
+    // the spatial draw list shares the overflow flag: grow it too
+    const unsigned dcap = d.draw_cap * 2u > d.draw_cap ? d.draw_cap * 2u : 0xffffffffu;
+    (void)hipFree(d.drawl);
+    drop(d.drawl);
+    if ((rc = dalloc(e, e->real_allocs, &d.drawl, dcap))) return rc;
+    d.draw_cap = dcap;
     unsigned zero = 0;
     HIPCHK(e, hipMemcpy(d.status, &zero, sizeof(zero), hipMemcpyHostToDevice));
     return 0;
@@ -554,7 +578,10 @@ int cs_alloc_realisations(cs_engine* e, int n_real, const uint64_t* seeds)
     AL(tot_prob, RC); AL(cum_trans, RC * NC); AL(invcdf, RC * 1025); AL(perm, RN);
     AL(pl_cs, PL); AL(pl_ce, PL); AL(m_socdist, R * NM); AL(m_moverest, R * NM); AL(m_trig, R * NM * 6);
     AL(cnt, R); AL(prev_prop, R); AL(samp_prop, R); AL(loc_init, 2 * R);
-    AL(si_buf, RN); AL(ci_buf, RN); AL(spK, RC); AL(sp_sum, RC); AL(sp_off, RC + 1);
+    AL(si_buf, RN); AL(ci_buf, RN); AL(spK, RC); AL(sp_sum, RC);
+    d.seg_cap = (unsigned)std::min<size_t>(RN / 64 + RC + 1, 0xffffffffu);
+    d.draw_cap = d.seg_cap;
+    AL(segl, d.seg_cap); AL(drawl, d.draw_cap); AL(cells, RC); AL(moved, RC); AL(wl, cs::WL_N);
     size_t cap = std::min<size_t>(std::max<size_t>(RN / 16, (size_t)1 << 20), (size_t)1 << 28);
     d.ev_cap = (unsigned)cap;
     AL(ev_key, cap); AL(ev_gp, cap); AL(ev_infector, cap); AL(ev_type, cap); AL(ev_count, 1);
@@ -566,10 +593,13 @@ int cs_alloc_realisations(cs_engine* e, int n_real, const uint64_t* seeds)
     HIPCHK(e, hipMemset(d.claim, 0xff, RN * 8));
     HIPCHK(e, hipMemset(d.lastk, 0, RN * 8));
     HIPCHK(e, hipMemset(d.bcount, 0, RC * 4));
+    HIPCHK(e, hipMemset(d.wl, 0, cs::WL_N * 4));
     HIPCHK(e, hipMemset(d.cnt, 0, R * sizeof(cs::Counters)));
     HIPCHK(e, hipMemset(d.m_trig, 0, R * NM * 6 * 4));
     d.ts_buf = nullptr;
     d.ts_max = 0;
+    e->ts_n = 0;
+    e->ts_fresh = true;
     return 0;
 }
 
@@ -582,13 +612,15 @@ int cs_reset_realisations(cs_engine* e, int n_seed)
     HIPCHK(e, hipSetDevice(e->device));
     Dev& d = e->d;
     hipStream_t st = e->stream;
+    e->ts_fresh = true;
     HIPCHK(e, hipMemsetAsync(d.cnt, 0, (size_t)d.R * sizeof(cs::Counters), st));
     cs::k_reset_persons<<<grid_for((long long)d.R * d.N, 256), 256, 0, st>>>(d);
     cs::k_reset_cells<<<grid_for((long long)d.R * d.NC, 64), 64, 0, st>>>(d);
     cs::k_reset_misc<<<grid_for(std::max((long long)d.R * d.np_off[CS_MAX_PLACE_TYPES], (long long)d.R * d.NM), 256), 256, 0, st>>>(d);
     if ((rc = launch_update_probs(e, nullptr))) return rc;
+    cs::k_zero_counts<<<1, 64, 0, st>>>(d.wl, cs::WL_N);
     cs::k_seed<<<grid_for(d.R, 64), 64, 0, st>>>(d, n_seed);
-    cs::k_list_move<<<(unsigned)((long long)d.R * d.NC), 256, 0, st>>>(d, 0);
+    cs::k_list_move<<<e->grid_cells, 256, 0, st>>>(d, 0);
     HIPCHK(e, hipGetLastError());
     return check_flags(e);
 }
@@ -600,6 +632,7 @@ int cs_load_state(cs_engine* e, int real, const cs_state* s)
     HIPCHK(e, hipSetDevice(e->device));
     Dev& d = e->d;
     const size_t N = e->N, NC = e->NC, o = (size_t)real * N, oc = (size_t)real * NC;
+    e->ts_fresh = true;
 #define UP(dst, src, off, n) if (src) HIPCHK(e, hipMemcpy(d.dst + (off), src, (n) * sizeof(*d.dst), hipMemcpyHostToDevice))
     UP(inf, s->inf, o, N); UP(sev_cur, s->severity_current, o, N); UP(sev_final, s->severity_final, o, N);
     UP(bits, s->bits, o, N); UP(listpos, s->listpos, o, N); UP(infector, s->infector, o, N); UP(itype, s->infect_type, o, N);
@@ -709,16 +742,28 @@ int cs_run_steps(cs_engine* e, double t0, int n_steps, int sps)
     if (rc) return rc;
     HIPCHK(e, hipSetDevice(e->device));
     Dev& d = e->d;
-    int ns = n_steps / sps;
-    if (ns > d.ts_max || !d.ts_buf) {
-        int32_t* tb;
-        if ((rc = dalloc(e, e->real_allocs, &tb, (size_t)d.R * std::max(ns, 1) * CS_TS_FIELDS))) return rc;
-        d.ts_buf = tb;
-        d.ts_max = std::max(ns, 1);
+    const int ns = n_steps / sps;
+    if (e->ts_fresh) {  // a new series: RunModel samples before its first timestep
+        e->ts_n = 0;
+        cs::k_record_init<<<grid_for(d.R, 64), 64, 0, e->stream>>>(d);
+        e->ts_fresh = false;
     }
-    HIPCHK(e, hipMemsetAsync(d.ts_buf, 0, (size_t)d.R * d.ts_max * CS_TS_FIELDS * 4, e->stream));
-    cs::k_record_init<<<grid_for(d.R, 64), 64, 0, e->stream>>>(d);
-    e->ts_n = 0;
+    if (e->ts_n + ns > d.ts_max || !d.ts_buf) {  // grow, keeping the samples taken so far
+        const int cap = std::max(std::max(e->ts_n + ns, 2 * d.ts_max), 1);
+        int32_t* tb;
+        if ((rc = dalloc(e, e->real_allocs, &tb, (size_t)d.R * cap * CS_TS_FIELDS))) return rc;
+        HIPCHK(e, hipMemsetAsync(tb, 0, (size_t)d.R * cap * CS_TS_FIELDS * 4, e->stream));
+        if (d.ts_buf && e->ts_n)
+            HIPCHK(e, hipMemcpy2DAsync(tb, (size_t)cap * CS_TS_FIELDS * 4, d.ts_buf, (size_t)d.ts_max * CS_TS_FIELDS * 4,
+                                       (size_t)e->ts_n * CS_TS_FIELDS * 4, (size_t)d.R, hipMemcpyDeviceToDevice, e->stream));
+        if (d.ts_buf) {
+            HIPCHK(e, hipStreamSynchronize(e->stream));
+            (void)hipFree(d.ts_buf);
+            e->real_allocs.erase(std::remove(e->real_allocs.begin(), e->real_allocs.end(), (void*)d.ts_buf), e->real_allocs.end());
+        }
+        d.ts_buf = tb;
+        d.ts_max = cap;
+    }
     double t = t0;
     for (int step = 0; step < n_steps; step++) {
         if ((rc = infect_sweep(e, t))) return rc;

@@ -45,8 +45,86 @@ __device__ __forceinline__ void cadd(unsigned long long* a, long long v) { atomi
 
 __device__ __forceinline__ void batch_add(const Dev& d, int r, int c, int p)
 {
-    int idx = atomicAdd(&d.bcount[(size_t)r * d.NC + c], 1);
+    const int rc = r * d.NC + c;
+    int idx = atomicAdd(&d.bcount[rc], 1);
+    if (idx == 0) d.moved[atomicAdd(d.wl + WL_MOVED, 1u)] = rc;  // first move of the cell this phase
     d.batch[(size_t)r * d.N + d.cstart[c] + idx] = p;
+}
+
+__device__ __forceinline__ unsigned long long wave_sum(unsigned long long v)
+{
+    for (int w = 32; w > 0; w >>= 1) v += __shfl_xor(v, w, 64);
+    return v;
+}
+
+// exclusive scan over a 256-thread block; *total gets the block sum
+__device__ __forceinline__ unsigned block_excl_scan(unsigned v, unsigned* total)
+{
+    __shared__ unsigned wsum[4];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    unsigned x = v;
+    for (int o = 1; o < 64; o <<= 1) {
+        unsigned y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) wsum[w] = x;
+    __syncthreads();
+    unsigned before = 0, all = 0;
+    for (int k = 0; k < 4; k++) {
+        if (k < w) before += wsum[k];
+        all += wsum[k];
+    }
+    __syncthreads();
+    *total = all;
+    return before + x - v;
+}
+
+// Work lists of the step's phases, one thread per (realisation, cell):
+//   seg 0: the latent segment [S, S+L) in chunks of 64 (also tallies L+I)
+//   seg 1: the infectious segment [S+L, S+L+I) in chunks, and the list of
+//          cells with I > 0; with zero_draws, every cell's draw count is cleared
+//   seg 2: the spatial draws [0, spK) in chunks
+// One atomic per block reserves the block's slots; the order of the lists
+// is irrelevant (every consumer's result is independent of it).
+__global__ void __launch_bounds__(256) k_compact(Dev d, int seg, int zero_draws)
+{
+    const long long nrc = (long long)d.R * d.NC;
+    const long long rc = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    int n = 0;
+    unsigned long long act = 0;
+    if (rc < nrc) {
+        if (seg == 0) { n = d.cL[rc]; act = (unsigned long long)n + d.cI[rc]; }
+        else if (seg == 1) n = d.cI[rc];
+        else n = d.spK[rc];
+        if (zero_draws) d.spK[rc] = 0;
+    }
+    const unsigned nch = (unsigned)((n + 63) >> 6);
+    unsigned tot_ch, tot_cells;
+    const unsigned ex_ch = block_excl_scan(nch, &tot_ch);
+    const unsigned ex_cell = (seg == 1) ? block_excl_scan(n > 0 ? 1u : 0u, &tot_cells) : 0u;
+    __shared__ unsigned base_ch, base_cell;
+    if (threadIdx.x == 0) {
+        base_ch = tot_ch ? atomicAdd(d.wl + (seg == 2 ? WL_DRAWS : WL_SEG), tot_ch) : 0u;
+        base_cell = (seg == 1 && tot_cells) ? atomicAdd(d.wl + WL_CELLS, tot_cells) : 0u;
+    }
+    __syncthreads();
+    Chunk* out = (seg == 2) ? d.drawl : d.segl;
+    const unsigned cap = (seg == 2) ? d.draw_cap : d.seg_cap;
+    for (unsigned k = 0; k < nch; k++) {
+        const unsigned idx = base_ch + ex_ch + k;
+        if (idx < cap) out[idx] = Chunk{(int32_t)rc, (int32_t)(k * 64)};
+        else atomicOr(d.status, (unsigned)FLAG_OVERFLOW);
+    }
+    if (seg == 1 && n > 0) d.cells[base_cell + ex_cell] = (int32_t)rc;
+    if (seg == 0) {
+        act = wave_sum(act);
+        if ((threadIdx.x & 63) == 0 && act) atomicAdd(&d.wc->n_active, act);
+    }
+}
+
+__global__ void k_zero_counts(unsigned* p, int n)
+{
+    if ((int)threadIdx.x < n) p[threadIdx.x] = 0u;
 }
 
 // ---------------------------------------------------------------- tables
@@ -131,17 +209,14 @@ __global__ void k_sdrop(Dev d, int* sel)
 // paired like this: holes = sorted batch positions below bnd = hi - k,
 // free = sorted positions in [bnd, hi) not holding a batch member; the
 // j-th hole swaps with the j-th free position.
-__global__ void __launch_bounds__(256) k_list_move(Dev d, int mode)
+// one cell's batched move (see the file comment); every thread of the block calls it
+__device__ void list_move_cell(const Dev& d, int mode, int rc, int32_t* pos, int32_t* fpos, uint8_t* occ, int32_t* part,
+                               int* dead_count_p, int* bad_p)
 {
-    const int rc = blockIdx.x;
+    int& dead_count = *dead_count_p;
+    int& bad = *bad_p;
     const int r = rc / d.NC, c = rc % d.NC;
     const int k = d.bcount[rc];
-    if (k == 0) return;
-    __shared__ int32_t pos[CS_MOVE_CAP];
-    __shared__ int32_t fpos[CS_MOVE_CAP];
-    __shared__ uint8_t occ[CS_MOVE_CAP];
-    __shared__ int32_t part[256];
-    __shared__ int dead_count, bad;
     if (k > CS_MOVE_CAP) {
         if (threadIdx.x == 0) atomicOr(d.status, (unsigned)FLAG_BATCH);
         return;
@@ -209,6 +284,20 @@ __global__ void __launch_bounds__(256) k_list_move(Dev d, int mode)
         else if (mode == 1) { d.cL[rc] = L - k; d.cI[rc] = I + k; }
         else { d.cI[rc] = I - k; d.cD[rc] += dead_count; d.cR[rc] += k - dead_count; }
         d.bcount[rc] = 0;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_list_move(Dev d, int mode)
+{
+    __shared__ int32_t pos[CS_MOVE_CAP];
+    __shared__ int32_t fpos[CS_MOVE_CAP];
+    __shared__ uint8_t occ[CS_MOVE_CAP];
+    __shared__ int32_t part[256];
+    __shared__ int dead_count, bad;
+    const unsigned nmoved = d.wl[WL_MOVED];
+    for (unsigned a = blockIdx.x; a < nmoved; a += gridDim.x) {
+        __syncthreads();  // the previous cell's use of the shared arrays is over
+        list_move_cell(d, mode, d.moved[a], pos, fpos, occ, part, &dead_count, &bad);
     }
 }
 
@@ -294,24 +383,27 @@ __device__ __forceinline__ int select_dest(const Dev& d, size_t rc, double u)
 // SampleWithoutReplacement replacement: all / Floyd (k <= 32) / selection
 #define CS_FLOYD_MAX 32
 
-// Household and place passes for one infectious person, and its spatial
-// infectiousness (src/Sweep.cpp:314-690).  One block per (realisation, cell).
-__global__ void __launch_bounds__(64) k_infect_cells(Dev d, StepScalars sc)
+// Household and place passes of one infectious person per lane, and its
+// spatial infectiousness by permutation slot (src/Sweep.cpp:314-690).  A
+// wave takes a chunk of up to 64 infectious persons of one cell.
+__global__ void __launch_bounds__(256) k_infect_persons(Dev d, StepScalars sc)
 {
-    const int rc = blockIdx.x;
-    const int r = rc / d.NC, c = rc % d.NC;
-    const int S = d.cS[rc], L = d.cL[rc], I = d.cI[rc];
-    if (I == 0) {
-        if (threadIdx.x == 0) d.spK[rc] = 0;
-        return;
-    }
+    const unsigned nch = min(d.wl[WL_SEG], d.seg_cap);
     const cs_params* P = d.P;
     const unsigned ts = sc.ts;
-    const uint64_t seed = d.seed[r];
-    const size_t rN = (size_t)r * d.N;
-    const size_t sbase = rN + d.cstart[c] + S + L;  // slot of infectious index 0
+    const int lane = threadIdx.x & 63;
+    const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const long long nwaves = ((long long)gridDim.x * blockDim.x) >> 6;
     unsigned long long w_hh_inf = 0, w_hh_mem = 0, w_link = 0, w_cand = 0;
-    for (int j = threadIdx.x; j < I; j += blockDim.x) {
+    for (long long ch = wave; ch < nch; ch += nwaves) {
+        const Chunk e = d.segl[ch];
+        const int rc = e.rc;
+        const int j = e.first + lane;
+        if (j >= d.cI[rc]) continue;
+        const int r = rc / d.NC, c = rc % d.NC;
+        const uint64_t seed = d.seed[r];
+        const size_t rN = (size_t)r * d.N;
+        const size_t sbase = rN + d.cstart[c] + d.cS[rc] + d.cL[rc];  // slot of infectious index 0
         const int i = d.perm[sbase + j];
         const size_t gi = rN + i;
         const int gen = d.itype[gi] / CS_INFECT_TYPE_MASK;
@@ -353,11 +445,11 @@ __global__ void __launch_bounds__(64) k_infect_cells(Dev d, StepScalars sc)
                     double Pc, Ps;
                     if (pass == 0) {
                         if (pt == P->hotel_place_type) continue;
-                        const int g = d.pglinks[(size_t)i * CS_MAX_PLACE_TYPES + pt];
+                        const int g2 = d.pglinks[(size_t)i * CS_MAX_PLACE_TYPES + pt];
                         Pc = PI;
                         if (Pc > 1) Pc = 1;
                         Ps = Pc;
-                        const int gix = d.pl_gs0[pt][lk] + g;
+                        const int gix = d.pl_gs0[pt][lk] + g2;
                         goff = d.pl_gst[pt][gix];
                         gsize = d.pl_gsz[pt][gix];
                     } else {
@@ -436,77 +528,113 @@ __global__ void __launch_bounds__(64) k_infect_cells(Dev d, StepScalars sc)
                 d.si_buf[sbase + j] = SI;
         }
     }
-    if (w_hh_inf | w_hh_mem | w_link | w_cand) {
+    w_hh_inf = wave_sum(w_hh_inf);
+    w_hh_mem = wave_sum(w_hh_mem);
+    w_link = wave_sum(w_link);
+    w_cand = wave_sum(w_cand);
+    if ((threadIdx.x & 63) == 0 && (w_hh_inf | w_hh_mem | w_link | w_cand)) {
         atomicAdd(&d.wc->n_hh_inf, w_hh_inf);
         atomicAdd(&d.wc->n_hh_member, w_hh_mem);
         atomicAdd(&d.wc->n_place_link, w_link);
         atomicAdd(&d.wc->n_place_cand, w_cand);
     }
+}
+
+// Per (realisation, cell): the cumulative spatial infectiousness by slot,
+// signed by the closure flag and normalised when draws follow, and the
+// Poisson number of spatial draws (src/Sweep.cpp:649-708).  The running
+// sum is kept in exact 64.64 fixed point, so this block-parallel scan gives
+// the same prefixes as a sequential pass; each prefix is rounded to double
+// once, then stored as float as the reference does.
+#define CS_SUM_BLOCK 256
+__device__ void cell_spatial_sum(const Dev& d, const StepScalars& sc, int rc, unsigned long long* shi, unsigned long long* slo,
+                                 int* sK, double* ssis);
+
+__global__ void __launch_bounds__(CS_SUM_BLOCK) k_cell_spatial_sum(Dev d, StepScalars sc)
+{
+    __shared__ unsigned long long shi[CS_SUM_BLOCK], slo[CS_SUM_BLOCK];
+    __shared__ int sK;
+    __shared__ double ssis;
+    const unsigned ncells = d.wl[WL_CELLS];
+    for (unsigned a = blockIdx.x; a < ncells; a += gridDim.x) {
+        __syncthreads();  // the previous cell's use of the shared arrays is over
+        cell_spatial_sum(d, sc, d.cells[a], shi, slo, &sK, &ssis);
+    }
+}
+
+// one cell with I > 0 (every thread of the block calls it)
+__device__ void cell_spatial_sum(const Dev& d, const StepScalars& sc, int rc, unsigned long long* shi, unsigned long long* slo,
+                                 int* sK_p, double* ssis_p)
+{
+    int& sK = *sK_p;
+    double& ssis = *ssis_p;
+    const int I = d.cI[rc];
+    if (!(sc.beta_s > 0)) {
+        if (threadIdx.x == 0) atomicAdd(&d.wc->n_inf, (unsigned long long)I);
+        return;
+    }
+    const int r = rc / d.NC, c = rc % d.NC;
+    const size_t sbase = (size_t)r * d.N + d.cstart[c] + d.cS[rc] + d.cL[rc];
+    const int per = (I + CS_SUM_BLOCK - 1) / CS_SUM_BLOCK;
+    const int j0 = threadIdx.x * per, j1 = min(I, j0 + per);
+    Fx acc = {0, 0};
+    for (int j = j0; j < j1; j++) acc = fx_add(acc, fx_from(fabs(d.si_buf[sbase + j])));
+    shi[threadIdx.x] = acc.hi;
+    slo[threadIdx.x] = acc.lo;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        atomicAdd(&d.wc->n_inf, (unsigned long long)I);
+    if (threadIdx.x == 0) {  // exclusive scan of the thread sums (exact: integer adds)
+        Fx run = {0, 0};
+        for (int t = 0; t < CS_SUM_BLOCK; t++) {
+            Fx v = {shi[t], slo[t]};
+            shi[t] = run.hi;
+            slo[t] = run.lo;
+            run = fx_add(run, v);
+        }
+        const double sis = fx_to_double(run);
         int K = 0;
-        if (sc.beta_s > 0) {
-            // running sum in the reference's order (double), stored as float
-            double sis = 0;
-            for (int j = 0; j < I; j++) {
-                double v = d.si_buf[sbase + j];
-                bool neg = __builtin_signbit(v);
-                sis += neg ? -v : v;
-                d.ci_buf[sbase + j] = neg ? (float)-sis : (float)sis;
-            }
-            if (sis > 0) {
-                Stream s(seed, cw0(ts, PS_SP_COUNT), (uint32_t)c, 0);
-                K = poisson(s, sis * sc.beta_s * ((double)d.tot_prob[rc]));
-                if (K > 0) {
-                    for (int j = 0; j < I - 1; j++) d.ci_buf[sbase + j] /= ((float)sis);
-                    d.ci_buf[sbase + I - 1] = (d.ci_buf[sbase + I - 1] < 0) ? -1.0f : 1.0f;
-                }
-            }
-            d.sp_sum[rc] = sis;
+        if (sis > 0) {
+            Stream s(d.seed[r], cw0(sc.ts, PS_SP_COUNT), (uint32_t)c, 0);
+            K = poisson(s, sis * sc.beta_s * ((double)d.tot_prob[rc]));
         }
         d.spK[rc] = K;
+        d.sp_sum[rc] = sis;
+        sK = K;
+        ssis = sis;
+        atomicAdd(&d.wc->n_inf, (unsigned long long)I);
+    }
+    __syncthreads();
+    Fx run = {shi[threadIdx.x], slo[threadIdx.x]};
+    const int K = sK;
+    const float fs = (float)ssis;
+    for (int j = j0; j < j1; j++) {
+        const double v = d.si_buf[sbase + j];
+        run = fx_add(run, fx_from(fabs(v)));
+        const double p = fx_to_double(run);
+        float ci = __builtin_signbit(v) ? (float)-p : (float)p;
+        if (K > 0) {
+            if (j < I - 1) ci /= fs;
+            else ci = (ci < 0) ? -1.0f : 1.0f;
+        }
+        d.ci_buf[sbase + j] = ci;
     }
 }
 
-// exclusive scan of spK into sp_off (one block, any length)
-__global__ void __launch_bounds__(1024) k_scan_draws(Dev d)
-{
-    __shared__ long long part[1024];
-    const long long n = (long long)d.R * d.NC;
-    const long long per = (n + blockDim.x - 1) / blockDim.x;
-    const long long a0 = threadIdx.x * per, a1 = min(n, a0 + per);
-    long long s = 0;
-    for (long long i = a0; i < a1; i++) s += d.spK[i];
-    part[threadIdx.x] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        long long run = 0;
-        for (int t = 0; t < (int)blockDim.x; t++) { long long v = part[t]; part[t] = run; run += v; }
-        d.sp_off[n] = run;
-    }
-    __syncthreads();
-    long long run = part[threadIdx.x];
-    for (long long i = a0; i < a1; i++) { d.sp_off[i] = run; run += d.spK[i]; }
-}
-
-// one spatial draw per thread (src/Sweep.cpp:712-855)
+// one spatial draw per lane; a wave takes a chunk of up to 64 draws of one
+// source cell (src/Sweep.cpp:712-855)
 __global__ void __launch_bounds__(256) k_spatial_draws(Dev d, StepScalars sc)
 {
-    const long long nrc = (long long)d.R * d.NC;
-    const long long total = d.sp_off[nrc];
+    const unsigned nch = min(d.wl[WL_DRAWS], d.draw_cap);
     const cs_params* P = d.P;
     const unsigned ts = sc.ts;
+    const int lane = threadIdx.x & 63;
+    const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const long long nwaves = ((long long)gridDim.x * blockDim.x) >> 6;
     unsigned long long w_draw = 0, w_susc = 0;
-    for (long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += (long long)gridDim.x * blockDim.x) {
-        long long lo = 0, hi = nrc;  // largest rc with sp_off[rc] <= g
-        while (hi - lo > 1) {
-            long long mid = (lo + hi) >> 1;
-            if (d.sp_off[mid] <= g) lo = mid; else hi = mid;
-        }
-        while (lo + 1 < nrc && d.sp_off[lo + 1] <= g) lo++;
-        const size_t rc = (size_t)lo;
-        const int k = (int)(g - d.sp_off[rc]);
+    for (long long ch = wave; ch < nch; ch += nwaves) {
+        const Chunk e = d.drawl[ch];
+        const size_t rc = (size_t)e.rc;
+        const int k = e.first + lane;
+        if (k >= d.spK[rc]) continue;
         const int r = (int)(rc / d.NC), c = (int)(rc % d.NC);
         const int S = d.cS[rc], L = d.cL[rc], I = d.cI[rc];
         const size_t rN = (size_t)r * d.N;
@@ -553,7 +681,9 @@ __global__ void __launch_bounds__(256) k_spatial_draws(Dev d, StepScalars sc)
             break;
         }
     }
-    if (w_draw) {
+    w_draw = wave_sum(w_draw);
+    w_susc = wave_sum(w_susc);
+    if ((threadIdx.x & 63) == 0 && w_draw) {
         atomicAdd(&d.wc->n_sp_draw, w_draw);
         atomicAdd(&d.wc->n_sp_susc, w_susc);
     }
@@ -587,7 +717,8 @@ __global__ void k_resolve_apply(Dev d, StepScalars sc)
             d.itype[gp] = d.ev_type[e];
         }
     }
-    if (nn) atomicAdd(&d.wc->n_new, nn);
+    nn = wave_sum(nn);
+    if ((threadIdx.x & 63) == 0 && nn) atomicAdd(&d.wc->n_new, nn);
 }
 
 __global__ void k_resolve_reset(Dev d)

@@ -146,25 +146,31 @@ __device__ void do_incub(const Dev& d, int r, int p, unsigned ts)
     }
 }
 
-// latent list of each cell: DoIncub where latent_time == now (src/Sweep.cpp:925-927)
-__global__ void __launch_bounds__(64) k_latent_scan(Dev d, unsigned ts)
+// latent lists: DoIncub where latent_time == now (src/Sweep.cpp:925-927); a
+// wave takes a chunk of up to 64 latent persons of one cell
+__global__ void __launch_bounds__(256) k_latent_scan(Dev d, unsigned ts)
 {
-    const int rc = blockIdx.x;
-    const int r = rc / d.NC, c = rc % d.NC;
-    const int S = d.cS[rc], L = d.cL[rc];
-    const size_t rN = (size_t)r * d.N;
-    const size_t base = rN + d.cstart[c];
+    const unsigned nch = min(d.wl[WL_SEG], d.seg_cap);
+    const int lane = threadIdx.x & 63;
+    const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const long long nwaves = ((long long)gridDim.x * blockDim.x) >> 6;
     unsigned long long nt = 0;
-    for (int q = S + threadIdx.x; q < S + L; q += blockDim.x) {
-        int p = d.perm[base + q];
+    for (long long ch = wave; ch < nch; ch += nwaves) {
+        const Chunk e = d.segl[ch];
+        const int rc = e.rc;
+        const int q = e.first + lane;
+        if (q >= d.cL[rc]) continue;
+        const int r = rc / d.NC, c = rc % d.NC;
+        const size_t rN = (size_t)r * d.N;
+        const int p = d.perm[rN + d.cstart[c] + d.cS[rc] + q];
         if (ts == d.t_lat[rN + p]) {
             do_incub(d, r, p, ts);
             batch_add(d, r, c, p);
             nt++;
         }
     }
-    if (threadIdx.x == 0) atomicAdd(&d.wc->n_active, (unsigned long long)(L + d.cI[rc]));
-    if (nt) atomicAdd(&d.wc->n_transition, nt);
+    nt = wave_sum(nt);
+    if ((threadIdx.x & 63) == 0 && nt) atomicAdd(&d.wc->n_transition, nt);
 }
 
 // DoCase (src/Update.cpp:794-894) without interventions; returns 1 when the
@@ -223,20 +229,25 @@ __device__ int do_case(const Dev& d, int r, int p, unsigned ts)
     return req;
 }
 
-// infectious list of each cell: case onset, severity and outcome timers
-// (src/Sweep.cpp:929-977)
-__global__ void __launch_bounds__(64) k_infectious_scan(Dev d, unsigned ts)
+// infectious lists: case onset, severity and outcome timers
+// (src/Sweep.cpp:929-977); a wave takes a chunk of up to 64 infectious persons
+__global__ void __launch_bounds__(256) k_infectious_scan(Dev d, unsigned ts)
 {
     const cs_params* P = d.P;
-    const int rc = blockIdx.x;
-    const int r = rc / d.NC, c = rc % d.NC;
-    const int lo = d.cS[rc] + d.cL[rc], I = d.cI[rc];
-    const size_t rN = (size_t)r * d.N;
-    const size_t base = rN + d.cstart[c];
-    Counters* cn = d.cnt + r;
+    const unsigned nch = min(d.wl[WL_SEG], d.seg_cap);
+    const int lane = threadIdx.x & 63;
+    const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const long long nwaves = ((long long)gridDim.x * blockDim.x) >> 6;
     unsigned long long nt = 0;
-    for (int q = lo + threadIdx.x; q < lo + I; q += blockDim.x) {
-        const int p = d.perm[base + q];
+    for (long long ch = wave; ch < nch; ch += nwaves) {
+        const Chunk e = d.segl[ch];
+        const int rc = e.rc;
+        const int q = e.first + lane;
+        if (q >= d.cI[rc]) continue;
+        const int r = rc / d.NC, c = rc % d.NC;
+        const size_t rN = (size_t)r * d.N;
+        Counters* cn = d.cnt + r;
+        const int p = d.perm[rN + d.cstart[c] + d.cS[rc] + d.cL[rc] + q];
         const size_t gp = rN + p;
         int ag = age_group(d, p);
         if (ag >= CS_NUM_AGE_GROUPS) ag = CS_NUM_AGE_GROUPS - 1;
@@ -291,7 +302,8 @@ __global__ void __launch_bounds__(64) k_infectious_scan(Dev d, unsigned ts)
             }
         }
     }
-    if (nt) atomicAdd(&d.wc->n_transition, nt);
+    nt = wave_sum(nt);
+    if ((threadIdx.x & 63) == 0 && nt) atomicAdd(&d.wc->n_transition, nt);
 }
 
 // child-care absence (src/Update.cpp:836-868), evaluated for every request

@@ -135,6 +135,27 @@ __device__ __forceinline__ double u53(uint32_t hi, uint32_t lo)
     return ((double)v + 0.5) * (1.0 / 9007199254740992.0);
 }
 
+// Exact 64.64 fixed-point accumulator for non-negative doubles below 2^63:
+// additions are exact integer adds, so a sum does not depend on the order
+// of its terms; fx_to_double rounds the total once per component.
+struct Fx { unsigned long long hi, lo; };
+__host__ __device__ __forceinline__ Fx fx_from(double v)
+{
+    const double f = floor(v);
+    Fx r;
+    r.hi = (unsigned long long)f;
+    r.lo = (unsigned long long)((v - f) * 18446744073709551616.0);  // 2^64
+    return r;
+}
+__host__ __device__ __forceinline__ Fx fx_add(Fx a, Fx b)
+{
+    Fx r;
+    r.lo = a.lo + b.lo;
+    r.hi = a.hi + b.hi + (r.lo < a.lo ? 1ull : 0ull);
+    return r;
+}
+__host__ __device__ __forceinline__ double fx_to_double(Fx a) { return (double)a.hi + (double)a.lo * 5.42101086242752217e-20; /* 2^-64 */ }
+
 struct Stream {
     uint32_t k0, k1, c0, c1, c2, c3;
     double b0, b1;

@@ -29,6 +29,11 @@ struct Counters {  // device mirror of cs_counters, updated with atomics
     unsigned long long sev_cum_age[CS_NUM_SEV_KINDS][CS_NUM_AGE_GROUPS];
 };
 
+struct Chunk {  // up to 64 consecutive entries starting at `first` of cell rc's segment
+    int32_t rc, first;
+};
+enum { WL_SEG = 0, WL_CELLS, WL_DRAWS, WL_MOVED, WL_N };
+
 struct Work {  // event tallies for the algorithmic-bytes figure
     unsigned long long n_inf, n_hh_inf, n_hh_member, n_place_link, n_place_cand;
     unsigned long long n_sp_draw, n_sp_susc, n_new, n_active, n_transition, n_cells;
@@ -78,7 +83,14 @@ struct Dev {
     float* ci_buf;        // [R][N] normalised cumulative infectiousness by slot
     int32_t* spK;         // [R*NC] draws per source cell
     double* sp_sum;       // [R*NC]
-    long long* sp_off;    // [R*NC+1] exclusive scan of spK
+    // work lists, rebuilt every phase: chunks of up to 64 consecutive list
+    // entries of one (realisation, cell), and lists of cells
+    Chunk* segl;          // persons of a latent or infectious segment
+    Chunk* drawl;         // spatial draws of a source cell
+    int32_t* cells;       // cells with infectious persons
+    int32_t* moved;       // cells with a pending batched list move
+    unsigned* wl;         // counts: [0] segl, [1] cells, [2] drawl, [3] moved
+    unsigned seg_cap, draw_cap;
     uint64_t* ev_key;
     long long* ev_gp;
     int32_t* ev_infector;

@@ -132,8 +132,8 @@ class Engine:
     def sync(self):
         self.call("sync")
 
-    KERNEL_KINDS = ["infect_cells", "spatial_draws", "resolve", "list_move", "latent_scan",
-                    "infectious_scan", "update_probs", "other"]
+    KERNEL_KINDS = ["infect_persons", "spatial_draws", "resolve", "list_move", "latent_scan",
+                    "infectious_scan", "update_probs", "cell_spatial_sum", "other"]
 
     def set_profiling(self, on):
         self.call("set_profiling", int(bool(on)))

@@ -345,12 +345,16 @@ int  cs_update_probs(cs_engine* e);
  * InfectSweep + IncubRecoverySweep + the S-drop UpdateProbs check
  * (src/CovidSim.cpp:1426-1522).  steps_per_sample (> 0) is the number of
  * timesteps per output step: totals are recorded after each output step,
- * and, as in RunModel, the S-drop check reads S as of the last sample. */
+ * and, as in RunModel, the S-drop check reads S as of the last sample.
+ * Successive calls continue one series (continuing calls should start at a
+ * sample boundary); cs_alloc_realisations, cs_reset_realisations and
+ * cs_load_state start a new one. */
 int  cs_run_steps(cs_engine* e, double t0, int n_steps, int steps_per_sample);
 
 /* results */
 int  cs_get_counters(cs_engine* e, int real, cs_counters* out, int reset);
-/* daily series [n_real][n_samples][CS_TS_FIELDS] recorded by cs_run_steps */
+/* series [n_real][n_samples][CS_TS_FIELDS] recorded by cs_run_steps since the
+ * series last started; out holds max_samples per realisation */
 #define CS_TS_FIELDS 8   /* S, L, I, R, D, incI, incC, incDeath */
 int  cs_get_timeseries(cs_engine* e, int32_t* out, int max_samples, int* n_samples);
 int  cs_get_work_counts(cs_engine* e, cs_work_counts* out, int reset);
@@ -389,8 +393,9 @@ int  cs_transmission_coeffs_from_draws(cs_engine* e, const double* prop_age_grou
 
 /* per-kernel device time, measured with HIP events recorded around each
  * launch on the engine's stream while profiling is enabled */
-enum { CS_K_INFECT_CELLS = 0, CS_K_SPATIAL_DRAWS, CS_K_RESOLVE, CS_K_LIST_MOVE,
-       CS_K_LATENT_SCAN, CS_K_INFECTIOUS_SCAN, CS_K_UPDATE_PROBS, CS_K_OTHER, CS_NUM_KERNEL_KINDS };
+enum { CS_K_INFECT_PERSONS = 0, CS_K_SPATIAL_DRAWS, CS_K_RESOLVE, CS_K_LIST_MOVE,
+       CS_K_LATENT_SCAN, CS_K_INFECTIOUS_SCAN, CS_K_UPDATE_PROBS, CS_K_CELL_SPATIAL_SUM,
+       CS_K_OTHER, CS_NUM_KERNEL_KINDS };
 int  cs_set_profiling(cs_engine* e, int enable);
 /* ms[k] total milliseconds and launches[k] launch count of kernel kind k
  * since profiling was last enabled or reset */

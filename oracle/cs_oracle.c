@@ -168,6 +168,28 @@ static double rs_next(rs* s)
     return s->buf[s->idx++];
 }
 
+/* The cell's running sum of spatial infectiousness (src/Sweep.cpp:649-690,
+ * a sequential double sum there) is kept exactly in 64.64 fixed point: the
+ * sum is then independent of the order of additions, so a parallel scan
+ * reproduces it, and each prefix is rounded to double once. */
+typedef struct { uint64_t hi, lo; } fx128;
+static fx128 fx_from(double v)
+{
+    fx128 r;
+    double f = floor(v);
+    r.hi = (uint64_t)f;
+    r.lo = (uint64_t)((v - f) * 18446744073709551616.0); /* 2^64 */
+    return r;
+}
+static fx128 fx_add(fx128 a, fx128 b)
+{
+    fx128 r;
+    r.lo = a.lo + b.lo;
+    r.hi = a.hi + b.hi + (r.lo < a.lo);
+    return r;
+}
+static double fx_to_double(fx128 a) { return (double)a.hi + (double)a.lo * 5.42101086242752217e-20; /* 2^-64 */ }
+
 /* bound on the spatial rejection loop (src/Sweep.cpp:753-854 has none; a
  * cell whose first S0 slots hold only dead persons would loop for ever) */
 #define OR_MAX_ATTEMPTS (1 << 20)
@@ -407,6 +429,8 @@ struct or_engine {
     int32_t* samp;
     int32_t* ts_buf;
     int ts_n, ts_cap;
+    int ts_fresh;           /* no sample since the last alloc/reset/load: the series restarts */
+    int64_t* ts_last;       /* [nreal][3] cumI, cumC, D at the previous sample */
     cs_work_counts wc;
     int error;
 };
@@ -803,7 +827,7 @@ void or_destroy(or_engine* e)
         free(e->pl_n[t]); free(e->pl_mcell[t]); free(e->pl_ms[t]); free(e->pl_mem[t]);
         free(e->pl_gs0[t]); free(e->pl_gst[t]); free(e->pl_gsz[t]);
     }
-    free(e->samp); free(e->cell_inf); free(e->max_trans); free(e->ev); free(e->ts_buf);
+    free(e->samp); free(e->cell_inf); free(e->max_trans); free(e->ev); free(e->ts_buf); free(e->ts_last);
     free(e);
 }
 
@@ -811,6 +835,8 @@ int or_alloc_realisations(or_engine* e, int n_real, const uint64_t* seeds)
 {
     for (int i = 0; i < e->nreal; i++) free_real(e, &e->R[i]);
     free(e->R);
+    free(e->ts_buf); free(e->ts_last);
+    e->ts_buf = NULL; e->ts_last = NULL; e->ts_cap = e->ts_n = 0; e->ts_fresh = 1;
     e->nreal = n_real;
     e->R = (or_real*)calloc((size_t)n_real, sizeof(or_real));
     int N = e->N, NC = e->NC;
@@ -1084,6 +1110,7 @@ int or_infect_sweep(or_engine* e, double t)
             const int base = e->cstart[c], S = r->cS[c], L = r->cL[c], I = r->cI[c];
             if (I == 0) continue;
             double sis = 0;
+            fx128 sacc = {0, 0};
             e->wc.n_inf += I;
             for (int j = 0; j < I; j++) {
                 const int i = r->perm[base + S + L + j];
@@ -1185,14 +1212,11 @@ int or_infect_sweep(or_engine* e, double t)
                             int lk = e->plinks[(size_t)i * CS_MAX_PLACE_TYPES + pt];
                             if (lk >= 0) flag = place_closed(r, pt, lk, ts);
                         }
-                    if (flag && host_absent(r, i, ts)) {
-                        SI *= P->efficacies[2][CS_SPATIAL];
-                        sis += SI;
-                        e->cell_inf[j] = (float)-sis;
-                    } else {
-                        sis += SI;
-                        e->cell_inf[j] = (float)sis;
-                    }
+                    const int neg = flag && host_absent(r, i, ts);
+                    if (neg) SI *= P->efficacies[2][CS_SPATIAL];
+                    sacc = fx_add(sacc, fx_from(SI));
+                    sis = fx_to_double(sacc);
+                    e->cell_inf[j] = neg ? (float)-sis : (float)sis;
                 }
             }
             /* cell-to-cell doling (src/Sweep.cpp:694-856) */
@@ -1570,6 +1594,7 @@ int or_reset_realisations(or_engine* e, int n_seed)
 {
     const cs_params* P = &e->P;
     if (P->interventions_active) return fail(e, CS_EUNSUPPORTED, "interventions are outside this path");
+    e->ts_fresh = 1;
     for (int ri = 0; ri < e->nreal; ri++) {
         or_real* r = &e->R[ri];
         for (int k = 0; k < e->N; k++) {
@@ -1656,6 +1681,7 @@ int or_load_state(or_engine* e, int ri, const cs_state* s)
     if (ri < 0 || ri >= e->nreal) return fail(e, CS_EINVAL, "bad realisation");
     or_real* r = &e->R[ri];
     size_t N = (size_t)e->N, NC = (size_t)e->NC;
+    e->ts_fresh = 1;
 #define CP(dst, src, n) if (src) memcpy(r->dst, src, (n) * sizeof(*r->dst))
     CP(inf, s->inf, N); CP(sev_cur, s->severity_current, N); CP(sev_final, s->severity_final, N);
     CP(bits, s->bits, N); CP(listpos, s->listpos, N); CP(infector, s->infector, N); CP(itype, s->infect_type, N);
@@ -1723,20 +1749,30 @@ int or_run_steps(or_engine* e, double t0, int n_steps, int sps)
 {
     if (sps <= 0) return fail(e, CS_EINVAL, "steps per sample must be positive");
     double t = t0;
-    int ns = n_steps / (sps > 0 ? sps : 1);
-    free(e->ts_buf);
-    e->ts_cap = ns;
-    e->ts_n = 0;
-    e->ts_buf = (int32_t*)calloc((size_t)e->nreal * (size_t)(ns ? ns : 1) * CS_TS_FIELDS, 4);
-    int64_t* lastI = (int64_t*)calloc((size_t)e->nreal, 8);
-    int64_t* lastC = (int64_t*)calloc((size_t)e->nreal, 8);
-    int64_t* lastD = (int64_t*)calloc((size_t)e->nreal, 8);
-    for (int ri = 0; ri < e->nreal; ri++) {
-        int64_t S, L, I, R, D;
-        totals(e, &e->R[ri], &S, &L, &I, &R, &D);
-        lastI[ri] = e->R[ri].cnt.cumI; lastC[ri] = e->R[ri].cnt.cumC; lastD[ri] = D;
-        /* RunModel samples before its first timestep (src/CovidSim.cpp:1413) */
-        e->R[ri].samp_prop_susc = (double)S / (double)e->N;
+    const int ns = n_steps / sps;
+    if (e->ts_fresh || !e->ts_last) {
+        /* a new series: RunModel samples before its first timestep (src/CovidSim.cpp:1413) */
+        free(e->ts_last);
+        e->ts_last = (int64_t*)calloc((size_t)e->nreal * 3, 8);
+        e->ts_n = 0;
+        for (int ri = 0; ri < e->nreal; ri++) {
+            int64_t S, L, I, R, D;
+            totals(e, &e->R[ri], &S, &L, &I, &R, &D);
+            e->ts_last[3 * ri] = e->R[ri].cnt.cumI; e->ts_last[3 * ri + 1] = e->R[ri].cnt.cumC; e->ts_last[3 * ri + 2] = D;
+            e->R[ri].samp_prop_susc = (double)S / (double)e->N;
+        }
+        e->ts_fresh = 0;
+    }
+    if (e->ts_n + ns > e->ts_cap || !e->ts_buf) { /* grow, keeping the samples taken so far */
+        int cap = e->ts_n + ns > 2 * e->ts_cap ? e->ts_n + ns : 2 * e->ts_cap;
+        if (cap < 1) cap = 1;
+        int32_t* nb = (int32_t*)calloc((size_t)e->nreal * (size_t)cap * CS_TS_FIELDS, 4);
+        for (int ri = 0; ri < e->nreal && e->ts_buf; ri++)
+            memcpy(nb + (size_t)ri * cap * CS_TS_FIELDS, e->ts_buf + (size_t)ri * e->ts_cap * CS_TS_FIELDS,
+                   (size_t)e->ts_n * CS_TS_FIELDS * 4);
+        free(e->ts_buf);
+        e->ts_buf = nb;
+        e->ts_cap = cap;
     }
     for (int st = 0; st < n_steps; st++) {
         int rc = or_infect_sweep(e, t);
@@ -1760,18 +1796,17 @@ int or_run_steps(or_engine* e, double t0, int n_steps, int sps)
                 int64_t S, L, I, R, D;
                 totals(e, r, &S, &L, &I, &R, &D);
                 r->samp_prop_susc = (double)S / (double)e->N;
-                if (e->ts_n >= ns) continue;
-                int32_t* o = e->ts_buf + ((size_t)ri * ns + e->ts_n) * CS_TS_FIELDS;
+                int64_t* lt = e->ts_last + 3 * ri;
+                int32_t* o = e->ts_buf + ((size_t)ri * e->ts_cap + e->ts_n) * CS_TS_FIELDS;
                 o[0] = (int32_t)S; o[1] = (int32_t)L; o[2] = (int32_t)I; o[3] = (int32_t)R; o[4] = (int32_t)D;
-                o[5] = (int32_t)(r->cnt.cumI - lastI[ri]);
-                o[6] = (int32_t)(r->cnt.cumC - lastC[ri]);
-                o[7] = (int32_t)(D - lastD[ri]);
-                lastI[ri] = r->cnt.cumI; lastC[ri] = r->cnt.cumC; lastD[ri] = D;
+                o[5] = (int32_t)(r->cnt.cumI - lt[0]);
+                o[6] = (int32_t)(r->cnt.cumC - lt[1]);
+                o[7] = (int32_t)(D - lt[2]);
+                lt[0] = r->cnt.cumI; lt[1] = r->cnt.cumC; lt[2] = D;
             }
-            if (e->ts_n < ns) e->ts_n++;
+            e->ts_n++;
         }
     }
-    free(lastI); free(lastC); free(lastD);
     return e->error;
 }
 

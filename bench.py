@@ -44,7 +44,8 @@ BYTES = {"n_inf": 64, "n_hh_inf": 16, "n_hh_member": 40, "n_place_link": 32, "n_
          "n_sp_draw": 48, "n_sp_susc": 40, "n_new": 40, "n_active": 16, "n_transition": 30,
          "n_cell_steps": 32}
 KERNEL_UNITS = {
-    "infect_persons": ["n_inf", "n_hh_inf", "n_hh_member", "n_place_link", "n_place_cand"],
+    "infect_households": ["n_inf", "n_hh_inf", "n_hh_member"],
+    "infect_places": ["n_place_link", "n_place_cand"],
     "cell_spatial_sum": ["n_cell_steps"],
     "spatial_draws": ["n_sp_draw", "n_sp_susc"],
     "resolve": ["n_new"],
@@ -107,7 +108,7 @@ def cpu_baseline(max_seconds=25.0):
     loop1, rc = run(1)
     if rc != 0 or loop1 <= 0:
         return None
-    nr = int(max(1, min(40, (max_seconds - loop1) / max(loop1, 1e-3))))
+    nr = int(max(1, min(400, (max_seconds - loop1) / max(loop1, 1e-3))))
     loop, rc = run(nr)
     if rc != 0 or loop <= 0:
         nr, loop = 1, loop1

@@ -106,6 +106,7 @@ def declare(lib, prefix):
         "last_error": (C.c_char_p, [vp]),
         "alloc_realisations": (I, [vp, I, P(C.c_uint64)]),
         "reset_realisations": (I, [vp, I]),
+        "set_realisation_params": (I, [vp, P(D), P(C.c_int32)]),
         "load_state": (I, [vp, I, P(cs_state)]),
         "store_state": (I, [vp, I, P(cs_state)]),
         "infect_sweep": (I, [vp, D]),

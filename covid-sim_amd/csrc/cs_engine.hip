@@ -29,7 +29,9 @@ struct cs_engine {
     std::vector<void*> pop_allocs, real_allocs;
     Dev d;
     int* d_sel = nullptr;
-    bool adunits_valid = false;  // every person's administrative unit lies in [0, n_adunits)
+    bool adunits_valid = false;
+    double* rbeta_buf = nullptr;     // [R] storage behind Dev.rbeta
+    int32_t* rseed_buf = nullptr;    // [R] storage behind Dev.rseed_n  // every person's administrative unit lies in [0, n_adunits)
     int ts_max = 0, ts_n = 0;
     bool ts_fresh = true;
     unsigned grid_waves = 4096;   // blocks of 4 waves for the per-person passes (a wave per chunk)
@@ -209,7 +211,8 @@ int launch_infect_events(cs_engine* e, const cs::StepScalars& sc)
         cs::k_zero_counts<<<1, 64, 0, st>>>(d.wl, cs::WL_N);
         cs::k_compact<<<grid_for(nrc, 256), 256, 0, st>>>(d, 1, 1);
     }
-    { ProfScope ps(e, CS_K_INFECT_PERSONS); cs::k_infect_persons<<<e->grid_waves, 256, 0, st>>>(d, sc); }
+    { ProfScope ps(e, CS_K_INFECT_HOUSEHOLDS); cs::k_infect_households<<<e->grid_waves, 256, 0, st>>>(d, sc); }
+    { ProfScope ps(e, CS_K_INFECT_PLACES); cs::k_infect_places<<<e->grid_waves, 256, 0, st>>>(d, sc); }
     { ProfScope ps(e, CS_K_CELL_SPATIAL_SUM); cs::k_cell_spatial_sum<<<e->grid_cells, CS_SUM_BLOCK, 0, st>>>(d, sc); }
     { ProfScope ps(e, CS_K_OTHER); cs::k_compact<<<grid_for(nrc, 256), 256, 0, st>>>(d, 2, 0); }
     { ProfScope ps(e, CS_K_SPATIAL_DRAWS); cs::k_spatial_draws<<<e->grid_waves, 256, 0, st>>>(d, sc); }
@@ -590,6 +593,10 @@ int cs_alloc_realisations(cs_engine* e, int n_real, const uint64_t* seeds)
     AL(last_tot, 3 * R);
 #undef AL
     if ((rc = dalloc(e, A, &e->d_sel, R))) return rc;
+    if ((rc = dalloc(e, A, &e->rbeta_buf, R))) return rc;
+    if ((rc = dalloc(e, A, &e->rseed_buf, R))) return rc;
+    d.rbeta = nullptr;
+    d.rseed_n = nullptr;
     HIPCHK(e, hipMemset(d.claim, 0xff, RN * 8));
     HIPCHK(e, hipMemset(d.lastk, 0, RN * 8));
     HIPCHK(e, hipMemset(d.bcount, 0, RC * 4));
@@ -600,6 +607,24 @@ int cs_alloc_realisations(cs_engine* e, int n_real, const uint64_t* seeds)
     d.ts_max = 0;
     e->ts_n = 0;
     e->ts_fresh = true;
+    return 0;
+}
+
+int cs_set_realisation_params(cs_engine* e, const double* local_beta, const int32_t* n_seed)
+{
+    if (!e) return CS_EINVAL;
+    if (e->R == 0) return set_err(e, CS_ESTATE, "no realisations allocated");
+    for (int r = 0; r < e->R; r++) {
+        if (local_beta && !(local_beta[r] >= 0)) return set_err(e, CS_EINVAL, "local_beta[%d] must be >= 0", r);
+        if (n_seed && n_seed[r] < 0) return set_err(e, CS_EINVAL, "n_seed[%d] must be >= 0", r);
+    }
+    HIPCHK(e, hipSetDevice(e->device));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    Dev& d = e->d;
+    if (local_beta) HIPCHK(e, hipMemcpy(e->rbeta_buf, local_beta, (size_t)e->R * 8, hipMemcpyHostToDevice));
+    if (n_seed) HIPCHK(e, hipMemcpy(e->rseed_buf, n_seed, (size_t)e->R * 4, hipMemcpyHostToDevice));
+    d.rbeta = local_beta ? e->rbeta_buf : nullptr;
+    d.rseed_n = n_seed ? e->rseed_buf : nullptr;
     return 0;
 }
 

@@ -351,6 +351,12 @@ struct StepScalars {
     unsigned ts;
 };
 
+// spatial beta of realisation r (src/Sweep.cpp:300)
+__device__ __forceinline__ double beta_s_of(const Dev& d, const StepScalars& sc, int r)
+{
+    return d.rbeta ? sc.seas * sc.fp * d.rbeta[r] : sc.beta_s;
+}
+
 // the reference's infector binary search (src/Sweep.cpp:716-740)
 __device__ __forceinline__ int select_infector(const float* ci, int n, double u)
 {
@@ -383,10 +389,12 @@ __device__ __forceinline__ int select_dest(const Dev& d, size_t rc, double u)
 // SampleWithoutReplacement replacement: all / Floyd (k <= 32) / selection
 #define CS_FLOYD_MAX 32
 
-// Household and place passes of one infectious person per lane, and its
-// spatial infectiousness by permutation slot (src/Sweep.cpp:314-690).  A
-// wave takes a chunk of up to 64 infectious persons of one cell.
-__global__ void __launch_bounds__(256) k_infect_persons(Dev d, StepScalars sc)
+// Household pass and spatial infectiousness of one infectious person per
+// lane (src/Sweep.cpp:314-377, :649-690); a wave takes a chunk of up to 64
+// infectious persons of one cell.  The place passes are a kernel of their
+// own so that each keeps few registers (many waves in flight hide the
+// latency of the scattered person reads).
+__global__ void __launch_bounds__(256) k_infect_households(Dev d, StepScalars sc)
 {
     const unsigned nch = min(d.wl[WL_SEG], d.seg_cap);
     const cs_params* P = d.P;
@@ -394,7 +402,7 @@ __global__ void __launch_bounds__(256) k_infect_persons(Dev d, StepScalars sc)
     const int lane = threadIdx.x & 63;
     const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const long long nwaves = ((long long)gridDim.x * blockDim.x) >> 6;
-    unsigned long long w_hh_inf = 0, w_hh_mem = 0, w_link = 0, w_cand = 0;
+    unsigned long long w_hh_inf = 0, w_hh_mem = 0;
     for (long long ch = wave; ch < nch; ch += nwaves) {
         const Chunk e = d.segl[ch];
         const int rc = e.rc;
@@ -432,6 +440,56 @@ __global__ void __launch_bounds__(256) k_infect_persons(Dev d, StepScalars sc)
                 }
             }
         }
+        // spatial infectiousness (src/Sweep.cpp:649-690); the closed flag is the sign
+        if (beta_s_of(d, sc, r) > 0) {
+            double SI = spatial_inf(d, r, i, ts);
+            bool flag = false;
+            if (P->do_places)
+                for (int pt = 0; pt < d.NPT && !flag; pt++) {
+                    int lk = d.plinks[(size_t)i * CS_MAX_PLACE_TYPES + pt];
+                    if (lk >= 0) flag = place_closed(d, r, pt, lk, ts);
+                }
+            if (flag && host_absent(d, gi, ts)) {
+                SI *= P->efficacies[2][CS_SPATIAL];
+                d.si_buf[sbase + j] = -SI;
+            } else
+                d.si_buf[sbase + j] = SI;
+        }
+        }
+    w_hh_inf = wave_sum(w_hh_inf);
+    w_hh_mem = wave_sum(w_hh_mem);
+    if ((threadIdx.x & 63) == 0 && (w_hh_inf | w_hh_mem)) {
+        atomicAdd(&d.wc->n_hh_inf, w_hh_inf);
+        atomicAdd(&d.wc->n_hh_member, w_hh_mem);
+    }
+}
+
+// Within-group and between-group place passes of one infectious person per
+// lane (src/Sweep.cpp:381-640).  Floyd's sample is kept in LDS, one column
+// per thread (bank-conflict free).
+__global__ void __launch_bounds__(256) k_infect_places(Dev d, StepScalars sc)
+{
+    __shared__ int picked[CS_FLOYD_MAX][256];
+    const unsigned nch = min(d.wl[WL_SEG], d.seg_cap);
+    const cs_params* P = d.P;
+    const unsigned ts = sc.ts;
+    const int lane = threadIdx.x & 63;
+    const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const long long nwaves = ((long long)gridDim.x * blockDim.x) >> 6;
+    unsigned long long w_link = 0, w_cand = 0;
+    if (!P->do_places) return;
+    for (long long ch = wave; ch < nch; ch += nwaves) {
+        const Chunk e = d.segl[ch];
+        const int rc = e.rc;
+        const int j = e.first + lane;
+        if (j >= d.cI[rc]) continue;
+        const int r = rc / d.NC, c = rc % d.NC;
+        const uint64_t seed = d.seed[r];
+        const size_t rN = (size_t)r * d.N;
+        const size_t sbase = rN + d.cstart[c] + d.cS[rc] + d.cL[rc];  // slot of infectious index 0
+        const int i = d.perm[sbase + j];
+        const size_t gi = rN + i;
+        const int gen = d.itype[gi] / CS_INFECT_TYPE_MASK;
         // places (src/Sweep.cpp:381-640)
         if (P->do_places && !host_absent(d, gi, ts)) {
             for (int pt = 0; pt < d.NPT; pt++) {
@@ -463,7 +521,6 @@ __global__ void __launch_bounds__(256) k_infect_persons(Dev d, StepScalars sc)
                     Stream s(seed, cw0(ts, pass == 0 ? PS_PW_SAMPLE : PS_PB_SAMPLE), (uint32_t)i, (uint32_t)pt);
                     int n = (Ps >= 1) ? gsize : binomial(s, gsize, Ps);
                     if (n <= 0) continue;
-                    int picked[CS_FLOYD_MAX];
                     const bool all = n >= gsize;
                     const bool floyd = !all && n <= CS_FLOYD_MAX;
                     if (floyd) {
@@ -471,8 +528,8 @@ __global__ void __launch_bounds__(256) k_infect_persons(Dev d, StepScalars sc)
                         for (int j2 = gsize - n; j2 < gsize; j2++) {
                             int tt = (int)(s.next() * (double)(j2 + 1));
                             bool dup = false;
-                            for (int q = 0; q < mm; q++) dup |= (picked[q] == tt);
-                            picked[mm++] = dup ? j2 : tt;
+                            for (int q = 0; q < mm; q++) dup |= (picked[q][threadIdx.x] == tt);
+                            picked[mm++][threadIdx.x] = dup ? j2 : tt;
                         }
                     }
                     if (all) n = gsize;
@@ -480,7 +537,7 @@ __global__ void __launch_bounds__(256) k_infect_persons(Dev d, StepScalars sc)
                     for (int cidx = 0, scan = 0; cidx < n; cidx++) {
                         int idx;
                         if (all) idx = cidx;
-                        else if (floyd) idx = picked[cidx];
+                        else if (floyd) idx = picked[cidx][threadIdx.x];
                         else {
                             // selection sampling over [scan, gsize)
                             for (;;) {
@@ -512,29 +569,10 @@ __global__ void __launch_bounds__(256) k_infect_persons(Dev d, StepScalars sc)
                 }
             }
         }
-        // spatial infectiousness (src/Sweep.cpp:649-690); the closed flag is the sign
-        if (sc.beta_s > 0) {
-            double SI = spatial_inf(d, r, i, ts);
-            bool flag = false;
-            if (P->do_places)
-                for (int pt = 0; pt < d.NPT && !flag; pt++) {
-                    int lk = d.plinks[(size_t)i * CS_MAX_PLACE_TYPES + pt];
-                    if (lk >= 0) flag = place_closed(d, r, pt, lk, ts);
-                }
-            if (flag && host_absent(d, gi, ts)) {
-                SI *= P->efficacies[2][CS_SPATIAL];
-                d.si_buf[sbase + j] = -SI;
-            } else
-                d.si_buf[sbase + j] = SI;
-        }
     }
-    w_hh_inf = wave_sum(w_hh_inf);
-    w_hh_mem = wave_sum(w_hh_mem);
     w_link = wave_sum(w_link);
     w_cand = wave_sum(w_cand);
-    if ((threadIdx.x & 63) == 0 && (w_hh_inf | w_hh_mem | w_link | w_cand)) {
-        atomicAdd(&d.wc->n_hh_inf, w_hh_inf);
-        atomicAdd(&d.wc->n_hh_member, w_hh_mem);
+    if ((threadIdx.x & 63) == 0 && (w_link | w_cand)) {
         atomicAdd(&d.wc->n_place_link, w_link);
         atomicAdd(&d.wc->n_place_cand, w_cand);
     }
@@ -569,11 +607,12 @@ __device__ void cell_spatial_sum(const Dev& d, const StepScalars& sc, int rc, un
     int& sK = *sK_p;
     double& ssis = *ssis_p;
     const int I = d.cI[rc];
-    if (!(sc.beta_s > 0)) {
+    const int r = rc / d.NC, c = rc % d.NC;
+    const double beta_s = beta_s_of(d, sc, r);
+    if (!(beta_s > 0)) {
         if (threadIdx.x == 0) atomicAdd(&d.wc->n_inf, (unsigned long long)I);
         return;
     }
-    const int r = rc / d.NC, c = rc % d.NC;
     const size_t sbase = (size_t)r * d.N + d.cstart[c] + d.cS[rc] + d.cL[rc];
     const int per = (I + CS_SUM_BLOCK - 1) / CS_SUM_BLOCK;
     const int j0 = threadIdx.x * per, j1 = min(I, j0 + per);
@@ -594,7 +633,7 @@ __device__ void cell_spatial_sum(const Dev& d, const StepScalars& sc, int rc, un
         int K = 0;
         if (sis > 0) {
             Stream s(d.seed[r], cw0(sc.ts, PS_SP_COUNT), (uint32_t)c, 0);
-            K = poisson(s, sis * sc.beta_s * ((double)d.tot_prob[rc]));
+            K = poisson(s, sis * beta_s * ((double)d.tot_prob[rc]));
         }
         d.spK[rc] = K;
         d.sp_sum[rc] = sis;

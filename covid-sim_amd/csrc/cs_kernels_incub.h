@@ -428,6 +428,7 @@ __global__ void k_seed(Dev d, int n_seed)
     d.prev_prop[r] = 1.0;
     Stream s(d.seed[r], cw0(0, PS_SEED), 0, 0);
     int choices = 0;
+    if (d.rseed_n) n_seed = d.rseed_n[r];
     for (int inf = 0; (inf < n_seed) && (choices < 10000); inf++) {
         int p, m, nm;
         do {

@@ -205,7 +205,7 @@ __device__ __forceinline__ double fc_corr(int32_t k)
 }
 
 // Binomial(n, p): inversion when n*min(p,1-p) < 10, else BTRS (Hormann 1993)
-__device__ __noinline__ int32_t binomial(Stream& s, int32_t n, double pp)
+__device__ __forceinline__ int32_t binomial(Stream& s, int32_t n, double pp)
 {
     if (n <= 0 || pp <= 0.0) return 0;
     if (pp >= 1.0) return n;

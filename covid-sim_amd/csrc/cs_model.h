@@ -59,6 +59,8 @@ struct Dev {
     const float* max_trans;
     // dynamic, per realisation
     const uint64_t* seed;
+    const double* rbeta;     // [R] per-realisation LocalBeta, or null (cs_params.local_beta)
+    const int32_t* rseed_n;  // [R] per-realisation seed infections, or null
     int8_t* inf;
     uint8_t *sev_cur, *sev_final, *bits, *qc, *dct;
     int32_t *listpos, *infector;

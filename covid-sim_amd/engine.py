@@ -107,6 +107,12 @@ class Engine:
         self.n_real = len(seeds)
         self.call("alloc_realisations", len(seeds), _p(seeds, C.c_uint64))
 
+    def set_realisation_params(self, local_beta=None, n_seed=None):
+        lb = None if local_beta is None else np.ascontiguousarray(local_beta, np.float64)
+        ns = None if n_seed is None else np.ascontiguousarray(n_seed, np.int32)
+        self.call("set_realisation_params", None if lb is None else _p(lb, C.c_double),
+                  None if ns is None else _p(ns, C.c_int32))
+
     def reset(self, n_seed):
         self.call("reset_realisations", int(n_seed))
 
@@ -132,8 +138,8 @@ class Engine:
     def sync(self):
         self.call("sync")
 
-    KERNEL_KINDS = ["infect_persons", "spatial_draws", "resolve", "list_move", "latent_scan",
-                    "infectious_scan", "update_probs", "cell_spatial_sum", "other"]
+    KERNEL_KINDS = ["infect_households", "spatial_draws", "resolve", "list_move", "latent_scan",
+                    "infectious_scan", "update_probs", "cell_spatial_sum", "infect_places", "other"]
 
     def set_profiling(self, on):
         self.call("set_profiling", int(bool(on)))

@@ -331,6 +331,13 @@ int  cs_set_population(cs_engine* e, const cs_population* pop);
  * UpdateProbs (src/CovidSim.cpp:2752-2802). */
 int  cs_alloc_realisations(cs_engine* e, int n_real, const uint64_t* seeds);
 int  cs_reset_realisations(cs_engine* e, int n_seed_infections);
+/* per-realisation parameters for sweeps and calibration (the reference runs
+ * one parameter set per process; here every realisation may differ):
+ * local_beta[r] replaces cs_params.local_beta (src/Sweep.cpp:300) and
+ * n_seed[r] the seed count of the next cs_reset_realisations for
+ * realisation r.  Either may be NULL (no override); each call replaces the
+ * previous overrides; cs_alloc_realisations clears them. */
+int  cs_set_realisation_params(cs_engine* e, const double* local_beta, const int32_t* n_seed);
 
 /* host-initialised state in and out (drop-in use under CovidSim's RunModel) */
 int  cs_load_state(cs_engine* e, int real, const cs_state* s);
@@ -391,11 +398,43 @@ int  cs_transmission_coeffs_from_draws(cs_engine* e, const double* prop_age_grou
                                        const uint16_t* infectious_steps, const int8_t* infstat,
                                        cs_transmission_coeffs* out);
 
+/* ---- calibration emulator ---------------------------------------------
+ * No counterpart in the reference (its calibration re-runs the simulator,
+ * src/CovidSim.cpp:3485-3689).  A multilayer perceptron
+ * n_in -> n_hidden -> n_hidden -> n_out with GELU activations and a linear
+ * output, evaluated and trained on the GPU: BF16 weights and activations,
+ * FP32 accumulation on MFMA, FP32 pre-activations, gradients, master weights
+ * and Adam moments.  Parameters are one FP32 vector: W1 [n_hidden][n_in],
+ * b1, W2 [n_hidden][n_hidden], b2, W3 [n_out][n_hidden], b3.  Loss: mean
+ * squared error over all outputs of a batch.  Host arrays are row-major
+ * [batch][features]. */
+typedef struct cs_mlp cs_mlp;
+int  cs_mlp_create(cs_mlp** out, int device, int n_in, int n_hidden, int n_out, uint64_t seed);
+void cs_mlp_destroy(cs_mlp* m);
+const char* cs_mlp_last_error(const cs_mlp* m);
+int64_t cs_mlp_n_params(const cs_mlp* m);
+int  cs_mlp_get_params(cs_mlp* m, float* out);
+/* sets the parameters and restarts the optimiser */
+int  cs_mlp_set_params(cs_mlp* m, const float* in);
+int  cs_mlp_forward(cs_mlp* m, const float* x, int batch, float* y);
+/* forward with the intermediate layers: FP32 pre-activations z1, z2 and the
+ * BF16 activations h1, h2 (raw bits), [batch][n_hidden] each; any may be NULL */
+int  cs_mlp_forward_trace(cs_mlp* m, const float* x, int batch, float* z1, float* z2,
+                          uint16_t* h1, uint16_t* h2, float* y);
+/* loss and its gradient with respect to every parameter, for one batch */
+int  cs_mlp_loss_grad(cs_mlp* m, const float* x, const float* t, int batch, double* loss, float* grad);
+/* training set kept in HBM (inputs stored as BF16) */
+int  cs_mlp_set_data(cs_mlp* m, const float* x, const float* t, int n);
+/* epochs of minibatch Adam (beta 0.9/0.999, eps 1e-8) over the training set
+ * in an order drawn from `seed`; loss_per_epoch (may be NULL) gets each
+ * epoch's mean batch loss */
+int  cs_mlp_train(cs_mlp* m, int epochs, int batch, float lr, uint64_t seed, double* loss_per_epoch);
+
 /* per-kernel device time, measured with HIP events recorded around each
  * launch on the engine's stream while profiling is enabled */
-enum { CS_K_INFECT_PERSONS = 0, CS_K_SPATIAL_DRAWS, CS_K_RESOLVE, CS_K_LIST_MOVE,
+enum { CS_K_INFECT_HOUSEHOLDS = 0, CS_K_SPATIAL_DRAWS, CS_K_RESOLVE, CS_K_LIST_MOVE,
        CS_K_LATENT_SCAN, CS_K_INFECTIOUS_SCAN, CS_K_UPDATE_PROBS, CS_K_CELL_SPATIAL_SUM,
-       CS_K_OTHER, CS_NUM_KERNEL_KINDS };
+       CS_K_INFECT_PLACES, CS_K_OTHER, CS_NUM_KERNEL_KINDS };
 int  cs_set_profiling(cs_engine* e, int enable);
 /* ms[k] total milliseconds and launches[k] launch count of kernel kind k
  * since profiling was last enabled or reset */

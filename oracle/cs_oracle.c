@@ -395,6 +395,9 @@ typedef struct {
     cs_counters cnt;
     double prev_prop_susc;
     double samp_prop_susc; /* State.S / PopSize as of the last sample */
+    int has_beta;          /* per-realisation LocalBeta override */
+    double local_beta;
+    int n_seed;            /* per-realisation seed count, or -1 */
     double loc_init[2];
 } or_real;
 
@@ -843,6 +846,7 @@ int or_alloc_realisations(or_engine* e, int n_real, const uint64_t* seeds)
     for (int i = 0; i < n_real; i++) {
         or_real* r = &e->R[i];
         r->seed = seeds[i];
+        r->n_seed = -1;
 #define AL(f, n, T) r->f = (T*)calloc((size_t)(n), sizeof(T))
         AL(inf, N, int8_t); AL(sev_cur, N, uint8_t); AL(sev_final, N, uint8_t); AL(bits, N, uint8_t);
         AL(qc, N, uint8_t); AL(dct, N, uint8_t); AL(listpos, N, int32_t); AL(infector, N, int32_t);
@@ -1100,11 +1104,12 @@ int or_infect_sweep(or_engine* e, double t)
     unsigned ts = (unsigned)(uint16_t)(P->time_steps_per_day * t);
     double fp = P->model_time_step / (1 - P->false_positive_rate);
     double seas = P->do_seasonality ? P->seasonality[((int)t) % CS_DAYS_PER_YEAR] : 1.0;
-    double beta_s = seas * fp * P->local_beta;
+    double beta_s_all = seas * fp * P->local_beta;
     double beta_h = P->do_households ? seas * fp * P->household_trans : 0;
     const int NPT = e->NPT;
     for (int ri = 0; ri < e->nreal; ri++) {
         or_real* r = &e->R[ri];
+        const double beta_s = r->has_beta ? seas * fp * r->local_beta : beta_s_all;
         e->nev = 0;
         for (int c = 0; c < e->NC; c++) {
             const int base = e->cstart[c], S = r->cS[c], L = r->cL[c], I = r->cI[c];
@@ -1590,6 +1595,19 @@ int or_incub_recovery_sweep(or_engine* e, double t)
 
 /* InitModel's per-run reset (src/CovidSim.cpp:1006-1098) and the
  * random-location branch of SeedInfection (src/CovidSim.cpp:1338-1377). */
+int or_set_realisation_params(or_engine* e, const double* local_beta, const int32_t* n_seed)
+{
+    for (int ri = 0; ri < e->nreal; ri++) {
+        or_real* r = &e->R[ri];
+        r->has_beta = local_beta != NULL;
+        r->local_beta = local_beta ? local_beta[ri] : 0.0;
+        r->n_seed = n_seed ? n_seed[ri] : -1;
+        if (local_beta && !(local_beta[ri] >= 0)) return fail(e, CS_EINVAL, "local_beta must be >= 0");
+        if (n_seed && n_seed[ri] < 0) return fail(e, CS_EINVAL, "n_seed must be >= 0");
+    }
+    return 0;
+}
+
 int or_reset_realisations(or_engine* e, int n_seed)
 {
     const cs_params* P = &e->P;
@@ -1643,9 +1661,10 @@ int or_reset_realisations(or_engine* e, int n_seed)
         rs s;
         rs_init(&s, r->seed, cw0(0, PS_SEED), 0, 0);
         int choices = 0;
-        int32_t* seeds = (int32_t*)malloc(sizeof(int32_t) * (size_t)(n_seed > 0 ? n_seed : 1));
+        const int nsd = (r->n_seed >= 0) ? r->n_seed : n_seed;
+        int32_t* seeds = (int32_t*)malloc(sizeof(int32_t) * (size_t)(nsd > 0 ? nsd : 1));
         int ns = 0;
-        for (int inf = 0; (inf < n_seed) && (choices < 10000); inf++) {
+        for (int inf = 0; (inf < nsd) && (choices < 10000); inf++) {
             int p, m, nm;
             do {
                 p = (int)(rs_next(&s) * (double)e->N);

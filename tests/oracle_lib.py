@@ -85,6 +85,12 @@ class OracleEngine(Oracle):
         self.n_real = len(seeds)
         self.call("alloc_realisations", len(seeds), _p(seeds, C.c_uint64))
 
+    def set_realisation_params(self, local_beta=None, n_seed=None):
+        lb = None if local_beta is None else np.ascontiguousarray(local_beta, np.float64)
+        ns = None if n_seed is None else np.ascontiguousarray(n_seed, np.int32)
+        self.call("set_realisation_params", None if lb is None else _p(lb, C.c_double),
+                  None if ns is None else _p(ns, C.c_int32))
+
     def reset(self, n_seed):
         self.call("reset_realisations", int(n_seed))
 

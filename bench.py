@@ -20,6 +20,12 @@ counts the library keeps and the per-event byte costs below.
 The CPU baseline runs the reference itself (oracle/_ref/covidsim_ref, built
 from the reference sources) on its 200k-person workload with all host
 threads, for a bounded number of realisations.
+
+--calibration prints a second kind of line instead: the wall time of
+fitting R0 and the seeding scale to a death curve with the emulator
+(covid-sim_amd/calibrate.py) against plain simulator-only calibration on
+the same GPU, both verified by the simulator, with the emulator's training
+cost and held-out error.
 """
 import argparse
 import json
@@ -128,11 +134,16 @@ def main():
     ap.add_argument("--warmup-days", type=int, default=20, help="days simulated by each warmup step")
     ap.add_argument("--chunk-days", type=int, default=50, help="days per run_steps call inside a step (progress lines)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibration", action="store_true", help="measure emulator-guided vs plain calibration instead")
+    ap.add_argument("--cal-people", type=int, default=1_000_000)
+    ap.add_argument("--cal-train", type=int, default=256, help="training candidates (8 seeds each)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU: this build has no multi-GPU path (each realisation batch "
               "would shard by realisation; see the design notes)", file=sys.stderr)
         sys.exit(2)
+    if args.calibration:
+        return calibration(args)
     import importlib
     csa = importlib.import_module("covid-sim_amd")
     eng_mod = importlib.import_module("covid-sim_amd.engine")
@@ -217,6 +228,55 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
                      "bytes_per_launch": round(alg_bytes), "ms_per_launch": round(ms / max(launches, 1), 4)},
         "cpu_baseline": base,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def calibration(args):
+    import importlib
+    cal = importlib.import_module("covid-sim_amd.calibrate")
+    t0 = time.perf_counter()
+    sim = cal.Simulator(n_people=args.cal_people)
+    log("calibration population %d persons (%.1f s)" % (sim.pop.n_people, time.perf_counter() - t0))
+    rng = np.random.default_rng(12)
+    train_seeds = np.arange(1000, 1008, dtype=np.uint64)
+    t1 = time.perf_counter()
+    th_tr, y_tr = cal.training_set(sim, args.cal_train, train_seeds, rng)
+    th_te, y_te = cal.training_set(sim, 64, train_seeds + np.uint64(500), rng)
+    t_data = time.perf_counter() - t1
+    log("training data: %d + 64 candidates x 8 seeds, %d launches, %.1f s" % (args.cal_train, sim.launches, t_data))
+    t1 = time.perf_counter()
+    emu, losses = cal.train_emulator(cal.encode(th_tr), y_tr)
+    t_train = time.perf_counter() - t1
+    err_inf, err_death = cal.emulator_error(emu, th_te, y_te)
+    log("emulator: loss %.4g -> %.4g in %.1f s; held-out relative L2 error infections %.3f deaths %.3f"
+        % (losses[0], losses[-1], t_train, err_inf, err_death))
+    hidden = np.array([2.15, 0.25])
+    _, d = sim.run(hidden[None], np.arange(5000, 5016, dtype=np.uint64))
+    target = np.round(d[0])
+    seeds = np.arange(100, 116, dtype=np.uint64)
+    plain = cal.calibrate_plain(sim, target, seeds)
+    log("plain: %s" % json.dumps(plain))
+    emul = cal.calibrate_emulated(sim, emu, target, seeds)
+    log("emulator-guided: %s" % json.dumps(emul))
+    out = {
+        "metric": "calibration wall-time, emulator-guided vs plain (speed-up)",
+        "value": plain["seconds"] / emul["seconds"],
+        "unit": "x",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "vs_baseline": None,
+        "dtype": "bf16 emulator, f64 simulator",
+        "data": "synthetic",
+        "config": {"workload": "fit R0 and seeding scale to a 365-day death curve of a %d-person synthetic region "
+                               "(hidden R0 %.2f, log10 scale %.2f); candidates verified with 16-seed GPU ensembles"
+                               % (sim.pop.n_people, hidden[0], hidden[1]),
+                   "emulator": "10-512-512-730 GELU MLP, BF16 MFMA kernels, FP32 accumulation and Adam",
+                   "training_candidates": args.cal_train, "training_seeds": 8},
+        "plain": plain, "emulator_guided": emul,
+        "fitted_r0_difference": abs(plain["theta"][0] - emul["theta"][0]),
+        "training_data_seconds": t_data, "training_seconds": t_train,
+        "emulator_heldout_rel_l2": {"infections": err_inf, "deaths": err_death},
     }
     print(json.dumps(out), flush=True)
 

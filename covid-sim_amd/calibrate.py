@@ -1,0 +1,198 @@
+"""Calibration of R0 and the seeding scale to a death curve, with and
+without the emulator.
+
+The reference calibrates by re-running the simulator: RunModel's
+re-run-to-trigger loop rescales P.SeedingScaling (src/CovidSim.cpp:3485-3689)
+and CalcLikelihood scores a run against daily deaths with a negative
+binomial likelihood (src/CovidSim.cpp:3691-3852).  Here every candidate
+(R0, seeding scale) is scored with that likelihood on the mean daily deaths
+of a GPU ensemble (16 seeds, common random numbers across candidates), and
+two searches are compared:
+
+  plain      pattern search in (R0, log10 seeding scale), four candidates per
+             64-realisation launch, steps halved until the R0 step is 0.01;
+  emulator   the trained emulator scores a dense grid of candidates in one
+             batched forward pass; the best four are verified by one
+             simulator launch and the same pattern search then refines the
+             best verified candidate from small steps.
+
+Per-realisation parameters: LocalBeta from R0 through the reproduction
+numbers InitTransmissionCoeffs gives for the population
+(src/SetupModel.cpp:832-838: LocalBeta = (R0 - R0household - R0places) /
+spatial R0), and the seed count (int)(10 * scale + 0.5)
+(src/CovidSim.cpp:1213).
+
+Emulator inputs are 10 numbers: the normalised R0 and log10 seeding scale,
+then 8 inputs for intervention fractions and trigger thresholds that the
+no-intervention simulator does not read (held at 0).  Outputs are 730
+numbers: log1p of mean daily infections and of mean daily deaths over 365
+days, scaled by 1/8.
+"""
+import math
+import time
+
+import numpy as np
+from scipy.special import gammaln
+
+N_IN, N_HIDDEN, N_OUT = 10, 512, 730
+DAYS = 365
+R0_RANGE = (1.8, 3.0)
+LOGS_RANGE = (-0.7, 0.7)   # log10 seeding scale
+OUT_SCALE = 8.0
+
+
+def nb_loglik(data, model, k=10.0):
+    """CalcLikelihood's negative binomial term for one column of death counts
+    (src/CovidSim.cpp:3778-3784), summed over days."""
+    data = np.asarray(data, np.float64)
+    model = np.asarray(model, np.float64)
+    knb = 1.0 + model / k
+    pnb = k / (1.0 + k)
+    return np.sum(gammaln(data + knb) - gammaln(data + 1) - gammaln(knb) + knb * math.log(1.0 - pnb) + data * math.log(pnb), axis=-1)
+
+
+def encode(theta):
+    """(R0, log10 scale) rows -> emulator inputs [n, 10]."""
+    theta = np.atleast_2d(np.asarray(theta, np.float64))
+    x = np.zeros((len(theta), N_IN), np.float32)
+    x[:, 0] = (theta[:, 0] - 2.4) / 0.35
+    x[:, 1] = theta[:, 1] / 0.4
+    return x
+
+
+def to_curves(y):
+    y = np.maximum(np.asarray(y, np.float64) * OUT_SCALE, 0.0)
+    c = np.expm1(y)
+    return c[:, :DAYS], c[:, DAYS:]
+
+
+def from_curves(inc, deaths):
+    return (np.log1p(np.concatenate([inc, deaths], axis=1)) / OUT_SCALE).astype(np.float32)
+
+
+class Simulator:
+    """Batched ensembles of one population; 64 realisations per launch."""
+
+    def __init__(self, n_people=1_000_000, pop_seed=2024, n_real=64, itc_seed=77):
+        import importlib
+        import os
+        csa = importlib.import_module("covid-sim_amd")
+        eng_mod = importlib.import_module("covid-sim_amd.engine")
+        popgen = importlib.import_module("covid-sim_amd.popgen")
+        here = os.path.dirname(os.path.abspath(__file__))
+        arrays, geom = popgen.generate(n_people, seed=pop_seed)
+        self.pop = csa.population.Population(arrays)
+        self.params = csa.population.load_params(os.path.join(here, "inputs", "c1", "params.bin"))
+        popgen.apply_geometry(self.params, geom)
+        self.eng = eng_mod.Engine(0)
+        self.eng.set_params(self.params)
+        self.eng.set_population(self.pop)
+        c = self.eng.transmission_coeffs(itc_seed)
+        self.r0_hh, self.r0_pl, self.sp_r0 = c.r0_household, c.r0_places, c.spatial_r0
+        self.n_real = n_real
+        self.launches = 0
+        self.sim_seconds = 0.0
+
+    def beta(self, r0):
+        return np.maximum(0.0, (np.asarray(r0, np.float64) - self.r0_hh - self.r0_pl) / self.sp_r0)
+
+    def run(self, theta, seeds):
+        """Mean daily infections and deaths [n_theta, DAYS] over `seeds`
+        (the same seeds for every candidate)."""
+        theta = np.atleast_2d(np.asarray(theta, np.float64))
+        seeds = np.asarray(seeds, np.uint64)
+        jobs = [(i, s) for i in range(len(theta)) for s in seeds]
+        inc = np.zeros((len(theta), DAYS))
+        dth = np.zeros((len(theta), DAYS))
+        tspd = int(round(self.params.time_steps_per_day))
+        for b0 in range(0, len(jobs), self.n_real):
+            batch = jobs[b0:b0 + self.n_real]
+            pad = batch + [batch[-1]] * (self.n_real - len(batch))
+            t0 = time.perf_counter()
+            self.eng.alloc(np.array([s for _, s in pad], np.uint64))
+            th = theta[[i for i, _ in pad]]
+            nseed = np.maximum(1, (10.0 * 10.0 ** th[:, 1] + 0.5).astype(np.int32))
+            self.eng.set_realisation_params(self.beta(th[:, 0]), nseed)
+            self.eng.reset(10)
+            self.eng.run_steps(0.0, DAYS * tspd, tspd)
+            ts = self.eng.timeseries(DAYS)
+            self.sim_seconds += time.perf_counter() - t0
+            self.launches += 1
+            for k, (i, _) in enumerate(batch):
+                inc[i] += ts[k, :, 5]
+                dth[i] += ts[k, :, 7]
+        return inc / len(seeds), dth / len(seeds)
+
+
+def pattern_search(score, start, step, min_step=0.01, max_evals=400):
+    """Maximise score over (R0, log10 scale): evaluate the four axis
+    neighbours of the incumbent together (one simulator launch), move to the
+    best improvement, else halve both steps; stop when the R0 step falls
+    below min_step."""
+    box = np.array([R0_RANGE, LOGS_RANGE])
+    x = np.clip(np.asarray(start, np.float64), box[:, 0], box[:, 1])
+    fx = score(x[None])[0]
+    step = np.asarray(step, np.float64)
+    evals = 1
+    while step[0] >= min_step and evals < max_evals:
+        cand = np.array([x + [step[0], 0], x - [step[0], 0], x + [0, step[1]], x - [0, step[1]]])
+        cand = np.clip(cand, box[:, 0], box[:, 1])
+        f = score(cand)
+        evals += len(cand)
+        j = int(np.argmax(f))
+        if f[j] > fx:
+            x, fx = cand[j], f[j]
+        else:
+            step = step / 2
+    return x, fx, evals
+
+
+def calibrate_plain(sim, target_deaths, seeds, start=(2.4, 0.0), step=(0.4, 0.4)):
+    t0 = time.perf_counter()
+    l0 = sim.launches
+    x, fx, n = pattern_search(lambda th: nb_loglik(target_deaths, sim.run(th, seeds)[1]), start, step)
+    return {"theta": x.tolist(), "loglik": float(fx), "candidates": n, "launches": sim.launches - l0,
+            "seconds": time.perf_counter() - t0}
+
+
+def calibrate_emulated(sim, emu, target_deaths, seeds, grid=(121, 57)):
+    t0 = time.perf_counter()
+    l0 = sim.launches
+    r0 = np.linspace(*R0_RANGE, grid[0])
+    ls = np.linspace(*LOGS_RANGE, grid[1])
+    th = np.array([(a, b) for a in r0 for b in ls])
+    _, d_pred = to_curves(emu.forward(encode(th)))
+    ll = nb_loglik(target_deaths, d_pred)
+    best = th[np.argsort(-ll)[:4]]
+    t_emul = time.perf_counter() - t0
+    f = nb_loglik(target_deaths, sim.run(best, seeds)[1])
+    start = best[int(np.argmax(f))]
+    x, fx, n = pattern_search(lambda c: nb_loglik(target_deaths, sim.run(c, seeds)[1]), start,
+                              step=(r0[1] - r0[0], ls[1] - ls[0]))
+    return {"theta": x.tolist(), "loglik": float(fx), "candidates": n + 4, "launches": sim.launches - l0,
+            "seconds": time.perf_counter() - t0, "emulator_seconds": t_emul}
+
+
+def training_set(sim, n_theta, seeds, rng):
+    """Latin-hypercube candidates and their ensemble mean curves."""
+    u = (rng.permuted(np.tile(np.arange(n_theta), (2, 1)), axis=1).T + rng.random((n_theta, 2))) / n_theta
+    th = np.column_stack([R0_RANGE[0] + u[:, 0] * (R0_RANGE[1] - R0_RANGE[0]),
+                          LOGS_RANGE[0] + u[:, 1] * (LOGS_RANGE[1] - LOGS_RANGE[0])])
+    inc, dth = sim.run(th, seeds)
+    return th, from_curves(inc, dth)
+
+
+def train_emulator(x, y, epochs=300, batch=256, lr=1e-3, seed=5):
+    import importlib
+    emu = importlib.import_module("covid-sim_amd.emulator").Emulator(N_IN, N_HIDDEN, N_OUT, seed=seed)
+    emu.set_data(x, y)
+    losses = emu.train(epochs, batch, lr)
+    return emu, losses
+
+
+def emulator_error(emu, th, y):
+    """Relative L2 error of the emulator's curves against the simulator's,
+    for infections and deaths separately (held-out candidates)."""
+    pi, pd = to_curves(emu.forward(encode(th)))
+    ti, td = to_curves(y)
+    return (float(np.linalg.norm(pi - ti) / np.linalg.norm(ti)), float(np.linalg.norm(pd - td) / np.linalg.norm(td)))

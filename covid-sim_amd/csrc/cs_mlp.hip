@@ -44,10 +44,13 @@ __host__ __device__ __forceinline__ float bf2f(uint16_t b)
     return f;
 }
 
-__device__ __forceinline__ float gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
+// GELU(z) = z Phi(z) with Phi(z) = (1 + erf(z / sqrt 2)) / 2 = erfc(-z / sqrt 2) / 2;
+// the erfc form keeps Phi accurate for large negative z, where 1 + erf
+// would cancel in FP32
+__device__ __forceinline__ float gelu(float z) { return 0.5f * z * erfcf(-z * 0.70710678118654752f); }
 __device__ __forceinline__ float gelu_grad(float z)
 {
-    return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) + z * 0.39894228040143268f * expf(-0.5f * z * z);
+    return 0.5f * erfcf(-z * 0.70710678118654752f) + z * 0.39894228040143268f * expf(-0.5f * z * z);
 }
 
 struct GemmArgs {

@@ -135,6 +135,9 @@ def main():
     ap.add_argument("--chunk-days", type=int, default=50, help="days per run_steps call inside a step (progress lines)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calibration", action="store_true", help="measure emulator-guided vs plain calibration instead")
+    ap.add_argument("--copy-probe-mb", type=int, default=0,
+                    help="before the run, copy this many MiB device-to-device (a copy of known size for "
+                         "scaling the profiler's byte counters)")
     ap.add_argument("--cal-people", type=int, default=1_000_000)
     ap.add_argument("--cal-train", type=int, default=256, help="training candidates (8 seeds each)")
     args = ap.parse_args()
@@ -173,6 +176,14 @@ def main():
             torch.cuda.synchronize()
     except Exception:
         pass
+    if args.copy_probe_mb:
+        import torch
+        a = torch.ones(args.copy_probe_mb << 20, dtype=torch.uint8, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        log("copy probe: %d bytes" % a.numel())
+        del a, b
     eng.work_counts(reset=True)
     eng.set_profiling(True)
     times = []
@@ -238,6 +249,7 @@ def calibration(args):
     t0 = time.perf_counter()
     sim = cal.Simulator(n_people=args.cal_people)
     log("calibration population %d persons (%.1f s)" % (sim.pop.n_people, time.perf_counter() - t0))
+    sim.progress = log
     rng = np.random.default_rng(12)
     train_seeds = np.arange(1000, 1008, dtype=np.uint64)
     t1 = time.perf_counter()

@@ -92,6 +92,7 @@ class Simulator:
         self.n_real = n_real
         self.launches = 0
         self.sim_seconds = 0.0
+        self.progress = None  # optional callable(str), called every 8 launches
 
     def beta(self, r0):
         return np.maximum(0.0, (np.asarray(r0, np.float64) - self.r0_hh - self.r0_pl) / self.sp_r0)
@@ -118,20 +119,22 @@ class Simulator:
             ts = self.eng.timeseries(DAYS)
             self.sim_seconds += time.perf_counter() - t0
             self.launches += 1
+            if self.progress and self.launches % 8 == 0:
+                self.progress("%d simulator launches, %.1f s" % (self.launches, self.sim_seconds))
             for k, (i, _) in enumerate(batch):
                 inc[i] += ts[k, :, 5]
                 dth[i] += ts[k, :, 7]
         return inc / len(seeds), dth / len(seeds)
 
 
-def pattern_search(score, start, step, min_step=0.01, max_evals=400):
+def pattern_search(score, start, step, min_step=0.01, max_evals=400, fstart=None):
     """Maximise score over (R0, log10 scale): evaluate the four axis
     neighbours of the incumbent together (one simulator launch), move to the
     best improvement, else halve both steps; stop when the R0 step falls
-    below min_step."""
+    below min_step.  fstart: the score of start when already known."""
     box = np.array([R0_RANGE, LOGS_RANGE])
     x = np.clip(np.asarray(start, np.float64), box[:, 0], box[:, 1])
-    fx = score(x[None])[0]
+    fx = score(x[None])[0] if fstart is None else fstart
     step = np.asarray(step, np.float64)
     evals = 1
     while step[0] >= min_step and evals < max_evals:
@@ -166,9 +169,9 @@ def calibrate_emulated(sim, emu, target_deaths, seeds, grid=(121, 57)):
     best = th[np.argsort(-ll)[:4]]
     t_emul = time.perf_counter() - t0
     f = nb_loglik(target_deaths, sim.run(best, seeds)[1])
-    start = best[int(np.argmax(f))]
-    x, fx, n = pattern_search(lambda c: nb_loglik(target_deaths, sim.run(c, seeds)[1]), start,
-                              step=(r0[1] - r0[0], ls[1] - ls[0]))
+    j = int(np.argmax(f))
+    x, fx, n = pattern_search(lambda c: nb_loglik(target_deaths, sim.run(c, seeds)[1]), best[j],
+                              step=(r0[1] - r0[0], ls[1] - ls[0]), fstart=f[j])
     return {"theta": x.tolist(), "loglik": float(fx), "candidates": n + 4, "launches": sim.launches - l0,
             "seconds": time.perf_counter() - t0, "emulator_seconds": t_emul}
 

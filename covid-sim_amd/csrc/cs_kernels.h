@@ -422,11 +422,15 @@ __global__ void __launch_bounds__(256) k_infect_households(Dev d, StepScalars sc
                 const int f0 = d.hfirst[h];
                 double HI = sc.beta_h * house_inf(d, r, i, ts);
                 bool absent = false;
-                for (int m = f0; m < f0 + nh && !absent; m++)
+                // a member absent from a closed place (src/Sweep.cpp:348-355); the
+                // member's absence is tested first (cheaper, same result)
+                for (int m = f0; m < f0 + nh && !absent; m++) {
+                    if (!host_absent(d, rN + m, ts)) continue;
                     for (int pt = 0; pt < d.NPT && !absent; pt++) {
                         int lk = d.plinks[(size_t)m * CS_MAX_PLACE_TYPES + pt];
-                        if (lk >= 0) absent = place_closed(d, r, pt, lk, ts) && host_absent(d, rN + m, ts);
+                        if (lk >= 0) absent = place_closed(d, r, pt, lk, ts);
                     }
+                }
                 if (absent) HI *= P->efficacies[2][CS_HOUSE];
                 w_hh_inf++;
                 for (int m = f0; m < f0 + nh; m++) {
@@ -443,13 +447,13 @@ __global__ void __launch_bounds__(256) k_infect_households(Dev d, StepScalars sc
         // spatial infectiousness (src/Sweep.cpp:649-690); the closed flag is the sign
         if (beta_s_of(d, sc, r) > 0) {
             double SI = spatial_inf(d, r, i, ts);
-            bool flag = false;
-            if (P->do_places)
+            bool flag = false;  // a linked place is closed and i is absent (tested first: same result)
+            if (P->do_places && host_absent(d, gi, ts))
                 for (int pt = 0; pt < d.NPT && !flag; pt++) {
                     int lk = d.plinks[(size_t)i * CS_MAX_PLACE_TYPES + pt];
                     if (lk >= 0) flag = place_closed(d, r, pt, lk, ts);
                 }
-            if (flag && host_absent(d, gi, ts)) {
+            if (flag) {
                 SI *= P->efficacies[2][CS_SPATIAL];
                 d.si_buf[sbase + j] = -SI;
             } else

@@ -113,6 +113,9 @@ def declare(lib, prefix):
         "incub_recovery_sweep": (I, [vp, D]),
         "update_probs": (I, [vp]),
         "run_steps": (I, [vp, D, I, I]),
+        "set_step_graphs": (I, [vp, I]),
+        "set_queue_capacity": (I, [vp, C.c_uint32, C.c_uint32]),
+        "get_queue_capacity": (I, [vp, P(C.c_uint32), P(C.c_uint32)]),
         "get_counters": (I, [vp, I, P(cs_counters), I]),
         "get_timeseries": (I, [vp, P(C.c_int32), I, P(I)]),
         "get_work_counts": (I, [vp, P(cs_work_counts), I]),

@@ -41,7 +41,10 @@ LOGS_RANGE = (-0.7, 0.7)   # log10 seeding scale
 OUT_SCALE = 8.0
 
 
-def nb_loglik(data, model, k=10.0):
+NEG_BIN_K = 1.0  # CalcLikelihood's NegBinK (read from its data file there); 1: no over-dispersion
+
+
+def nb_loglik(data, model, k=NEG_BIN_K):
     """CalcLikelihood's negative binomial term for one column of death counts
     (src/CovidSim.cpp:3778-3784), summed over days."""
     data = np.asarray(data, np.float64)
@@ -185,17 +188,32 @@ def training_set(sim, n_theta, seeds, rng):
     return th, from_curves(inc, dth)
 
 
-def train_emulator(x, y, epochs=300, batch=256, lr=1e-3, seed=5):
+def train_emulator(x, y, epochs=1500, batch=128, lr=1e-3, seed=5):
+    """Minibatch Adam on the GPU; the learning rate drops tenfold for the
+    last third of the epochs."""
     import importlib
     emu = importlib.import_module("covid-sim_amd.emulator").Emulator(N_IN, N_HIDDEN, N_OUT, seed=seed)
     emu.set_data(x, y)
-    losses = emu.train(epochs, batch, lr)
+    e1 = (2 * epochs) // 3
+    losses = np.concatenate([emu.train(e1, batch, lr, seed=seed), emu.train(epochs - e1, batch, lr / 10, seed=seed + 1)])
     return emu, losses
 
 
-def emulator_error(emu, th, y):
-    """Relative L2 error of the emulator's curves against the simulator's,
-    for infections and deaths separately (held-out candidates)."""
+def rel_l2(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def emulator_error(emu, th, y, y_other=None):
+    """Relative L2 error of the emulator's curves against the simulator's
+    ensemble means, for infections and deaths (held-out candidates).  With
+    y_other, a second simulator estimate from independent seeds, also the
+    same error between the two simulator estimates: the noise floor of a
+    mean over that many seeds."""
     pi, pd = to_curves(emu.forward(encode(th)))
     ti, td = to_curves(y)
-    return (float(np.linalg.norm(pi - ti) / np.linalg.norm(ti)), float(np.linalg.norm(pd - td) / np.linalg.norm(td)))
+    out = {"infections": rel_l2(pi, ti), "deaths": rel_l2(pd, td)}
+    if y_other is not None:
+        oi, od = to_curves(y_other)
+        out["simulator_noise_infections"] = rel_l2(oi, ti)
+        out["simulator_noise_deaths"] = rel_l2(od, td)
+    return out

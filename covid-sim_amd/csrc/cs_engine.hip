@@ -29,14 +29,20 @@ struct cs_engine {
     std::vector<void*> pop_allocs, real_allocs;
     Dev d;
     int* d_sel = nullptr;
-    bool adunits_valid = false;
+    bool adunits_valid = false;      // every person's administrative unit lies in [0, n_adunits)
     double* rbeta_buf = nullptr;     // [R] storage behind Dev.rbeta
-    int32_t* rseed_buf = nullptr;    // [R] storage behind Dev.rseed_n  // every person's administrative unit lies in [0, n_adunits)
+    int32_t* rseed_buf = nullptr;    // [R] storage behind Dev.rseed_n
     int ts_max = 0, ts_n = 0;
-    bool ts_fresh = true;
-    unsigned grid_waves = 4096;   // blocks of 4 waves for the per-person passes (a wave per chunk)
-    unsigned grid_cells = 2048;   // blocks for the per-cell passes (a block per cell)  // no sample since the last alloc/reset/load: the series restarts
-    int steps_done = 0;
+    bool ts_fresh = true;            // no sample since the last alloc/reset/load: the series restarts
+    unsigned grid_waves = 4096;      // blocks of 4 waves for the per-person passes (a wave per chunk)
+    unsigned grid_cells = 2048;      // blocks for the per-cell passes (a block per cell)
+    // replayed step graphs (cs_run_steps): one graph per output step
+    bool use_graphs = true;
+    bool capturing = false;
+    hipGraphExec_t gexec = nullptr;
+    int graph_sps = 0;
+    bool graph_prof = false;         // the graph holds profiling events
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> gevents;
     // profiling: event pairs per kernel kind, resolved on cs_get_profile
     bool prof = false;
     std::vector<hipEvent_t> ev_pool;
@@ -137,7 +143,7 @@ struct ProfScope {  // records a start/stop event pair around one launch
     {
         if (e->prof) {
             (void)hipEventRecord(b, e->stream);
-            e->ev_open.push_back({kind, {a, b}});
+            (e->capturing ? e->gevents : e->ev_open).push_back({kind, {a, b}});
         }
     }
 };
@@ -187,20 +193,8 @@ int check_flags(cs_engine* e)
     return 0;
 }
 
-cs::StepScalars scalars(const cs_params& P, double t)
-{
-    cs::StepScalars s;
-    s.t = t;
-    s.ts = (unsigned)(uint16_t)(P.time_steps_per_day * t);
-    s.fp = P.model_time_step / (1 - P.false_positive_rate);
-    s.seas = P.do_seasonality ? P.seasonality[((int)t) % CS_DAYS_PER_YEAR] : 1.0;
-    s.beta_s = s.seas * s.fp * P.local_beta;
-    s.beta_h = P.do_households ? s.seas * s.fp * P.household_trans : 0;
-    return s;
-}
-
 // sweep kernels that append events; state is untouched until resolution
-int launch_infect_events(cs_engine* e, const cs::StepScalars& sc)
+int launch_infect_events(cs_engine* e)
 {
     Dev& d = e->d;
     hipStream_t st = e->stream;
@@ -211,23 +205,23 @@ int launch_infect_events(cs_engine* e, const cs::StepScalars& sc)
         cs::k_zero_counts<<<1, 64, 0, st>>>(d.wl, cs::WL_N);
         cs::k_compact<<<grid_for(nrc, 256), 256, 0, st>>>(d, 1, 1);
     }
-    { ProfScope ps(e, CS_K_INFECT_HOUSEHOLDS); cs::k_infect_households<<<e->grid_waves, 256, 0, st>>>(d, sc); }
-    { ProfScope ps(e, CS_K_INFECT_PLACES); cs::k_infect_places<<<e->grid_waves, 256, 0, st>>>(d, sc); }
-    { ProfScope ps(e, CS_K_CELL_SPATIAL_SUM); cs::k_cell_spatial_sum<<<e->grid_cells, CS_SUM_BLOCK, 0, st>>>(d, sc); }
+    { ProfScope ps(e, CS_K_INFECT_HOUSEHOLDS); cs::k_infect_households<<<e->grid_waves, 256, 0, st>>>(d); }
+    { ProfScope ps(e, CS_K_INFECT_PLACES); cs::k_infect_places<<<e->grid_waves, 256, 0, st>>>(d); }
+    { ProfScope ps(e, CS_K_CELL_SPATIAL_SUM); cs::k_cell_spatial_sum<<<e->grid_cells, CS_SUM_BLOCK, 0, st>>>(d); }
     { ProfScope ps(e, CS_K_OTHER); cs::k_compact<<<grid_for(nrc, 256), 256, 0, st>>>(d, 2, 0); }
-    { ProfScope ps(e, CS_K_SPATIAL_DRAWS); cs::k_spatial_draws<<<e->grid_waves, 256, 0, st>>>(d, sc); }
+    { ProfScope ps(e, CS_K_SPATIAL_DRAWS); cs::k_spatial_draws<<<e->grid_waves, 256, 0, st>>>(d); }
     HIPCHK(e, hipGetLastError());
     return 0;
 }
 
-int launch_resolve(cs_engine* e, const cs::StepScalars& sc)
+int launch_resolve(cs_engine* e)
 {
     Dev& d = e->d;
     hipStream_t st = e->stream;
     {
         ProfScope ps(e, CS_K_RESOLVE);
         cs::k_resolve_claim<<<1024, 256, 0, st>>>(d);
-        cs::k_resolve_apply<<<1024, 256, 0, st>>>(d, sc);
+        cs::k_resolve_apply<<<1024, 256, 0, st>>>(d);
         cs::k_resolve_reset<<<1024, 256, 0, st>>>(d);
     }
     { ProfScope ps(e, CS_K_LIST_MOVE); cs::k_list_move<<<e->grid_cells, 256, 0, st>>>(d, 0); }
@@ -235,27 +229,21 @@ int launch_resolve(cs_engine* e, const cs::StepScalars& sc)
     return 0;
 }
 
-int launch_incub(cs_engine* e, double t)
+int launch_incub(cs_engine* e)
 {
     Dev& d = e->d;
-    const cs_params& P = e->P;
     hipStream_t st = e->stream;
-    const unsigned ts = (unsigned)(uint16_t)(P.time_steps_per_day * t);
     const unsigned nrc = (unsigned)((long long)d.R * d.NC);
-    if (P.do_places)
-        for (int h = 0; h < P.num_holidays; h++) {
-            double ht = P.holiday_start[h] + P.holidays_start_day_simtime;
-            if ((t + P.model_time_step >= ht) && (t < ht)) {
-                long long n = (long long)d.R * d.np_off[CS_MAX_PLACE_TYPES];
-                cs::k_holidays<<<grid_for(n, 256), 256, 0, st>>>(d, h, ht);
-            }
-        }
+    if (e->P.do_places && e->P.num_holidays > 0) {
+        long long n = (long long)d.R * d.np_off[CS_MAX_PLACE_TYPES];
+        cs::k_holidays<<<grid_for(n, 256), 256, 0, st>>>(d);
+    }
     {
         ProfScope ps(e, CS_K_OTHER);
         cs::k_zero_counts<<<1, 64, 0, st>>>(d.wl, cs::WL_N);
         cs::k_compact<<<grid_for(nrc, 256), 256, 0, st>>>(d, 0, 0);
     }
-    { ProfScope ps(e, CS_K_LATENT_SCAN); cs::k_latent_scan<<<e->grid_waves, 256, 0, st>>>(d, ts); }
+    { ProfScope ps(e, CS_K_LATENT_SCAN); cs::k_latent_scan<<<e->grid_waves, 256, 0, st>>>(d); }
     { ProfScope ps(e, CS_K_LIST_MOVE); cs::k_list_move<<<e->grid_cells, 256, 0, st>>>(d, 1); }
     {
         ProfScope ps(e, CS_K_OTHER);
@@ -263,11 +251,11 @@ int launch_incub(cs_engine* e, double t)
         cs::k_zero_counts<<<1, 64, 0, st>>>(d.wl, cs::WL_N);
         cs::k_compact<<<grid_for(nrc, 256), 256, 0, st>>>(d, 1, 0);
     }
-    { ProfScope ps(e, CS_K_INFECTIOUS_SCAN); cs::k_infectious_scan<<<e->grid_waves, 256, 0, st>>>(d, ts); }
+    { ProfScope ps(e, CS_K_INFECTIOUS_SCAN); cs::k_infectious_scan<<<e->grid_waves, 256, 0, st>>>(d); }
     {
         ProfScope ps(e, CS_K_OTHER);
-        cs::k_care_eval<<<256, 256, 0, st>>>(d, ts);
-        cs::k_care_apply<<<256, 256, 0, st>>>(d, ts);
+        cs::k_care_eval<<<256, 256, 0, st>>>(d);
+        cs::k_care_apply<<<256, 256, 0, st>>>(d);
     }
     { ProfScope ps(e, CS_K_LIST_MOVE); cs::k_list_move<<<e->grid_cells, 256, 0, st>>>(d, 2); }
     HIPCHK(e, hipGetLastError());
@@ -285,12 +273,24 @@ int launch_update_probs(cs_engine* e, const int* sel)
     return 0;
 }
 
+void drop_graph(cs_engine* e)
+{
+    if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
+    e->gexec = nullptr;
+    for (auto& x : e->gevents) {
+        e->ev_pool.push_back(x.second.first);
+        e->ev_pool.push_back(x.second.second);
+    }
+    e->gevents.clear();
+}
+
 int grow_events(cs_engine* e)
 {
     Dev& d = e->d;
     unsigned cap = d.ev_cap * 2u;
     if (cap < d.ev_cap) return set_err(e, CS_EOVERFLOW, "event queue cannot grow further");
     HIPCHK(e, hipStreamSynchronize(e->stream));
+    drop_graph(e);  // the graph holds the old buffers
     (void)hipFree(d.ev_key); (void)hipFree(d.ev_gp); (void)hipFree(d.ev_infector); (void)hipFree(d.ev_type);
     auto drop = [&](void* p) { e->real_allocs.erase(std::remove(e->real_allocs.begin(), e->real_allocs.end(), p), e->real_allocs.end()); };
     drop(d.ev_key); drop(d.ev_gp); drop(d.ev_infector); drop(d.ev_type);
@@ -311,13 +311,14 @@ int grow_events(cs_engine* e)
     return 0;
 }
 
-// one InfectSweep; an event-queue overflow is detected before any state
-// changes and the sweep is re-run with a larger queue
+// one InfectSweep at time t; an overflow of the event or draw lists is
+// detected before any state changes and the sweep is re-run with larger
+// lists (the counter-based streams make the re-run identical)
 int infect_sweep(cs_engine* e, double t)
 {
-    cs::StepScalars sc = scalars(e->P, t);
     for (int attempt = 0; attempt < 8; attempt++) {
-        int rc = launch_infect_events(e, sc);
+        cs::k_step_begin<<<1, 64, 0, e->stream>>>(e->d, t, 0);
+        int rc = launch_infect_events(e);
         if (rc) return rc;
         rc = check_flags(e);
         if (rc == CS_EOVERFLOW) {
@@ -325,10 +326,94 @@ int infect_sweep(cs_engine* e, double t)
             continue;
         }
         if (rc) return rc;
-        if ((rc = launch_resolve(e, sc))) return rc;
+        if ((rc = launch_resolve(e))) return rc;
         return check_flags(e);
     }
     return set_err(e, CS_EOVERFLOW, "event queue overflow persists");
+}
+
+// the tail of a timestep after InfectSweep: IncubRecoverySweep, the S-drop
+// UpdateProbs, the sample at the end of an output step, the device clock
+int launch_step_tail(cs_engine* e, bool sample_done)
+{
+    Dev& d = e->d;
+    int rc;
+    if ((rc = launch_incub(e))) return rc;
+    cs::k_sdrop<<<grid_for(d.R, 64), 64, 0, e->stream>>>(d, e->d_sel);
+    if ((rc = launch_update_probs(e, e->d_sel))) return rc;
+    if (sample_done) cs::k_record<<<d.R, 256, 0, e->stream>>>(d, -1);
+    cs::k_step_end<<<1, 64, 0, e->stream>>>(d, sample_done ? 1 : 0);
+    HIPCHK(e, hipGetLastError());
+    return 0;
+}
+
+// every launch of one output step (sps timesteps) on the device clock,
+// captured once into a graph for replay; an overflow halts every later
+// kernel of the replay, so the state stays that of the overflowing step's
+// start and the host can re-run it directly
+int capture_graph(cs_engine* e, int sps)
+{
+    drop_graph(e);
+    Dev& d = e->d;
+    HIPCHK(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+    e->capturing = true;
+    int rc = 0;
+    for (int k = 0; k < sps && !rc; k++) {
+        cs::k_step_begin<<<1, 64, 0, e->stream>>>(d, 0.0, 1);
+        if (!(rc = launch_infect_events(e)) && !(rc = launch_resolve(e))) rc = launch_step_tail(e, k == sps - 1);
+    }
+    e->capturing = false;
+    hipGraph_t g = nullptr;
+    hipError_t st = hipStreamEndCapture(e->stream, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (st != hipSuccess) return set_err(e, CS_EHIP, "step graph capture: %s", hipGetErrorString(st));
+    st = hipGraphInstantiate(&e->gexec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (st != hipSuccess) {
+        e->gexec = nullptr;
+        return set_err(e, CS_EHIP, "step graph instantiation: %s", hipGetErrorString(st));
+    }
+    e->graph_sps = sps;
+    e->graph_prof = e->prof;
+    return 0;
+}
+
+// after a replay with profiling events: add their times
+int harvest_graph_events(cs_engine* e)
+{
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    for (auto& x : e->gevents) {
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, x.second.first, x.second.second) == hipSuccess) {
+            e->prof_ms[x.first] += ms;
+            e->prof_n[x.first]++;
+        }
+    }
+    return 0;
+}
+
+int read_clock(cs_engine* e, cs::Clock* c)
+{
+    HIPCHK(e, hipMemcpyAsync(c, e->d.clock, sizeof(*c), hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    return 0;
+}
+
+// timesteps [from, n_steps) of a run, one at a time, from time t (the time
+// of step `from`, accumulated step by step as the device clock and RunModel
+// do); a sample follows every sps-th step counted from the run's start
+int run_direct(cs_engine* e, double t, int from, int n_steps, int sps)
+{
+    int rc;
+    for (int step = from; step < n_steps; step++) {
+        if ((rc = infect_sweep(e, t))) return rc;
+        if ((rc = launch_step_tail(e, (step + 1) % sps == 0))) return rc;
+        t += e->P.model_time_step;
+    }
+    return 0;
 }
 
 template <class T>
@@ -360,6 +445,10 @@ int cs_create(cs_engine** out, int device)
     int rc;
     if ((rc = dalloc(e, e->pop_allocs, &e->d.status, 1))) return rc;
     if ((rc = dalloc(e, e->pop_allocs, &e->d.wc, 1))) return rc;
+    if ((rc = dalloc(e, e->pop_allocs, &e->d.sc, 1))) return rc;
+    if ((rc = dalloc(e, e->pop_allocs, &e->d.clock, 1))) return rc;
+    HIPCHK(e, hipMemset(e->d.sc, 0, sizeof(cs::StepScalars)));
+    HIPCHK(e, hipMemset(e->d.clock, 0, sizeof(cs::Clock)));
     HIPCHK(e, hipMemset(e->d.status, 0, sizeof(unsigned)));
     HIPCHK(e, hipMemset(e->d.wc, 0, sizeof(cs::Work)));
     return 0;
@@ -370,6 +459,7 @@ void cs_destroy(cs_engine* e)
     if (!e) return;
     (void)hipSetDevice(e->device);
     if (e->stream) (void)hipStreamSynchronize(e->stream);
+    drop_graph(e);
     free_list(e->real_allocs);
     free_list(e->pop_allocs);
     if (e->dP) (void)hipFree(e->dP);
@@ -389,6 +479,7 @@ int cs_set_params(cs_engine* e, const cs_params* p, const double* kl, const doub
     if (p->kernel_size <= 0 || !(p->kernel_delta > 0)) return set_err(e, CS_EINVAL, "kernel table size/delta not set");
     if (p->num_place_types < 0 || p->num_place_types > CS_MAX_PLACE_TYPES) return set_err(e, CS_EINVAL, "bad number of place types");
     if (p->num_holidays < 0 || p->num_holidays > CS_MAX_HOLIDAYS) return set_err(e, CS_EINVAL, "bad number of holidays");
+    drop_graph(e);  // a captured step follows the host's parameters (holiday launches)
     bool new_tables = !e->have_params || e->P.kernel_size != p->kernel_size || e->P.kernel_type != p->kernel_type ||
                       e->P.kernel_delta != p->kernel_delta || e->P.kernel_shape != p->kernel_shape ||
                       e->P.kernel_scale != p->kernel_scale || e->P.kernel_p3 != p->kernel_p3 ||
@@ -434,10 +525,11 @@ int cs_set_population(cs_engine* e, const cs_population* pop)
     HIPCHK(e, hipSetDevice(e->device));
     if (pop->n_people <= 0 || pop->n_cells <= 0) return set_err(e, CS_EINVAL, "empty population");
     HIPCHK(e, hipStreamSynchronize(e->stream));
+    drop_graph(e);
     free_list(e->real_allocs);
     e->R = 0;
     // keep the flags/work words, drop the rest of the old population
-    std::vector<void*> keep = {e->d.status, e->d.wc};
+    std::vector<void*> keep = {e->d.status, e->d.wc, e->d.sc, e->d.clock};
     for (void* q : e->pop_allocs)
         if (std::find(keep.begin(), keep.end(), q) == keep.end()) (void)hipFree(q);
     e->pop_allocs = keep;
@@ -561,6 +653,7 @@ int cs_alloc_realisations(cs_engine* e, int n_real, const uint64_t* seeds)
     if (!e->have_pop) return set_err(e, CS_ESTATE, "cs_set_population must come first");
     HIPCHK(e, hipSetDevice(e->device));
     HIPCHK(e, hipStreamSynchronize(e->stream));
+    drop_graph(e);
     free_list(e->real_allocs);
     Dev& d = e->d;
     const size_t R = (size_t)n_real, N = (size_t)e->N, NC = (size_t)e->NC, NM = (size_t)e->NM;
@@ -625,6 +718,7 @@ int cs_set_realisation_params(cs_engine* e, const double* local_beta, const int3
     if (n_seed) HIPCHK(e, hipMemcpy(e->rseed_buf, n_seed, (size_t)e->R * 4, hipMemcpyHostToDevice));
     d.rbeta = local_beta ? e->rbeta_buf : nullptr;
     d.rseed_n = n_seed ? e->rseed_buf : nullptr;
+    drop_graph(e);  // the captured launches hold the old pointers
     return 0;
 }
 
@@ -744,7 +838,8 @@ int cs_incub_recovery_sweep(cs_engine* e, double t)
     int rc = check_path(e);
     if (rc) return rc;
     HIPCHK(e, hipSetDevice(e->device));
-    if ((rc = launch_incub(e, t))) return rc;
+    cs::k_step_begin<<<1, 64, 0, e->stream>>>(e->d, t, 0);
+    if ((rc = launch_incub(e))) return rc;
     return check_flags(e);
 }
 
@@ -789,20 +884,93 @@ int cs_run_steps(cs_engine* e, double t0, int n_steps, int sps)
         d.ts_buf = tb;
         d.ts_max = cap;
     }
+    const cs::Clock c0 = {t0, 0, e->ts_n};
+    HIPCHK(e, hipMemcpyAsync(d.clock, &c0, sizeof(c0), hipMemcpyHostToDevice, e->stream));
+    const int periods = n_steps / sps;
+    int done = 0;
     double t = t0;
-    for (int step = 0; step < n_steps; step++) {
-        if ((rc = infect_sweep(e, t))) return rc;
-        if ((rc = launch_incub(e, t))) return rc;
-        t += e->P.model_time_step;
-        cs::k_sdrop<<<grid_for(d.R, 64), 64, 0, e->stream>>>(d, e->d_sel);
-        if ((rc = launch_update_probs(e, e->d_sel))) return rc;
-        if ((step + 1) % sps == 0) {
-            cs::k_record<<<d.R, 256, 0, e->stream>>>(d, e->ts_n);
-            e->ts_n++;
+    if (e->use_graphs && periods > 0) {
+        if ((!e->gexec || e->graph_sps != sps || e->graph_prof != e->prof) && (rc = capture_graph(e, sps))) return rc;
+        int p = 0;
+        for (int guard = 0; p < periods && guard < 64;) {
+            const int batch = std::min(periods - p, 8);  // output steps between flag checks
+            for (int b = 0; b < batch; b++) {
+                HIPCHK(e, hipGraphLaunch(e->gexec, e->stream));
+                if (e->graph_prof && (rc = harvest_graph_events(e))) return rc;
+            }
+            cs::Clock c;
+            if ((rc = read_clock(e, &c))) return rc;
+            unsigned f = 0;
+            HIPCHK(e, hipMemcpy(&f, d.status, sizeof(f), hipMemcpyDeviceToHost));
+            if (f & cs::FLAG_OVERFLOW) {
+                // the replay halted at step c.step: grow the lists, finish that
+                // output step directly, then replay again
+                guard++;
+                if ((rc = grow_events(e))) return rc;
+                const int end = (c.step / sps + 1) * sps;
+                if ((rc = run_direct(e, c.t, c.step, end, sps))) return rc;
+                p = end / sps;
+                if ((rc = capture_graph(e, sps))) return rc;
+                continue;
+            }
+            if ((rc = check_flags(e))) return rc;
+            p += batch;
         }
+        if (p < periods) return set_err(e, CS_EOVERFLOW, "work lists keep overflowing");
+        cs::Clock c;
+        if ((rc = read_clock(e, &c))) return rc;
+        done = c.step;
+        t = c.t;
     }
+    if ((rc = run_direct(e, t, done, n_steps, sps))) return rc;
+    cs::Clock c;
+    if ((rc = read_clock(e, &c))) return rc;
+    e->ts_n = c.sample;
     HIPCHK(e, hipGetLastError());
     return check_flags(e);
+}
+
+int cs_set_queue_capacity(cs_engine* e, uint32_t events, uint32_t draw_chunks)
+{
+    if (!e) return CS_EINVAL;
+    if (e->R == 0) return set_err(e, CS_ESTATE, "no realisations allocated");
+    HIPCHK(e, hipSetDevice(e->device));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    drop_graph(e);
+    Dev& d = e->d;
+    auto drop = [&](void* p) { e->real_allocs.erase(std::remove(e->real_allocs.begin(), e->real_allocs.end(), p), e->real_allocs.end()); };
+    int rc;
+    if (events) {
+        (void)hipFree(d.ev_key); (void)hipFree(d.ev_gp); (void)hipFree(d.ev_infector); (void)hipFree(d.ev_type);
+        drop(d.ev_key); drop(d.ev_gp); drop(d.ev_infector); drop(d.ev_type);
+        if ((rc = dalloc(e, e->real_allocs, &d.ev_key, events)) || (rc = dalloc(e, e->real_allocs, &d.ev_gp, events)) ||
+            (rc = dalloc(e, e->real_allocs, &d.ev_infector, events)) || (rc = dalloc(e, e->real_allocs, &d.ev_type, events)))
+            return rc;
+        d.ev_cap = events;
+    }
+    if (draw_chunks) {
+        (void)hipFree(d.drawl);
+        drop(d.drawl);
+        if ((rc = dalloc(e, e->real_allocs, &d.drawl, draw_chunks))) return rc;
+        d.draw_cap = draw_chunks;
+    }
+    return 0;
+}
+
+int cs_get_queue_capacity(cs_engine* e, uint32_t* events, uint32_t* draw_chunks)
+{
+    if (!e) return CS_EINVAL;
+    if (events) *events = e->d.ev_cap;
+    if (draw_chunks) *draw_chunks = e->d.draw_cap;
+    return 0;
+}
+
+int cs_set_step_graphs(cs_engine* e, int enable)
+{
+    if (!e) return CS_EINVAL;
+    e->use_graphs = enable != 0;
+    if (!e->use_graphs) drop_graph(e);
+    return 0;
 }
 
 int cs_get_counters(cs_engine* e, int real, cs_counters* out, int reset)

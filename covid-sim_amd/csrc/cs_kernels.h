@@ -41,6 +41,11 @@ __device__ __forceinline__ void add_event(const Dev& d, uint64_t key, size_t gp_
     }
 }
 
+// halt: set when a step overflowed a work buffer; every later kernel of the
+// step (and of steps already queued behind it) then does nothing, so the
+// state stays that of the step's start and the step can be re-run
+__device__ __forceinline__ bool halted(const Dev& d) { return (*(volatile unsigned*)d.status & FLAG_OVERFLOW) != 0; }
+
 __device__ __forceinline__ void cadd(unsigned long long* a, long long v) { atomicAdd(a, (unsigned long long)v); }
 
 __device__ __forceinline__ void batch_add(const Dev& d, int r, int c, int p)
@@ -127,6 +132,32 @@ __global__ void k_zero_counts(unsigned* p, int n)
     if ((int)threadIdx.x < n) p[threadIdx.x] = 0u;
 }
 
+// this step's scalars (src/Sweep.cpp:294-301) from t, or from the device
+// clock when the step is part of a replayed graph
+__global__ void k_step_begin(Dev d, double t, int from_clock)
+{
+    if (threadIdx.x != 0 || halted(d)) return;
+    const cs_params* P = d.P;
+    if (from_clock) t = d.clock->t;
+    StepScalars s;
+    s.t = t;
+    s.ts = (unsigned)(uint16_t)(P->time_steps_per_day * t);
+    s.fp = P->model_time_step / (1 - P->false_positive_rate);
+    s.seas = P->do_seasonality ? P->seasonality[((int)t) % CS_DAYS_PER_YEAR] : 1.0;
+    s.beta_s = s.seas * s.fp * P->local_beta;
+    s.beta_h = P->do_households ? s.seas * s.fp * P->household_trans : 0;
+    *d.sc = s;
+}
+
+// the clock after a completed step (not after a halted one)
+__global__ void k_step_end(Dev d, int sample_done)
+{
+    if (threadIdx.x != 0 || halted(d)) return;
+    d.clock->t += d.P->model_time_step;
+    d.clock->step++;
+    if (sample_done) d.clock->sample++;
+}
+
 // ---------------------------------------------------------------- tables
 
 // max_trans[l][m] = (float)num(dist2_cc_min(l, m)) (src/Kernels.cpp:48-63)
@@ -194,6 +225,7 @@ __global__ void k_sdrop(Dev d, int* sel)
 {
     int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= d.R) return;
+    if (halted(d)) { sel[r] = 0; return; }
     double prop = d.samp_prop[r];
     int up = (d.prev_prop[r] - prop) > 0.2;
     if (up) d.prev_prop[r] = prop;
@@ -289,6 +321,7 @@ __device__ void list_move_cell(const Dev& d, int mode, int rc, int32_t* pos, int
 
 __global__ void __launch_bounds__(256) k_list_move(Dev d, int mode)
 {
+    if (halted(d)) return;
     __shared__ int32_t pos[CS_MOVE_CAP];
     __shared__ int32_t fpos[CS_MOVE_CAP];
     __shared__ uint8_t occ[CS_MOVE_CAP];
@@ -346,10 +379,7 @@ __device__ void do_infect(const Dev& d, int r, int p, double t, unsigned ts, int
 
 // ---------------------------------------------------------- InfectSweep
 
-struct StepScalars {
-    double t, fp, seas, beta_s, beta_h;
-    unsigned ts;
-};
+
 
 // spatial beta of realisation r (src/Sweep.cpp:300)
 __device__ __forceinline__ double beta_s_of(const Dev& d, const StepScalars& sc, int r)
@@ -394,8 +424,9 @@ __device__ __forceinline__ int select_dest(const Dev& d, size_t rc, double u)
 // infectious persons of one cell.  The place passes are a kernel of their
 // own so that each keeps few registers (many waves in flight hide the
 // latency of the scattered person reads).
-__global__ void __launch_bounds__(256) k_infect_households(Dev d, StepScalars sc)
+__global__ void __launch_bounds__(256) k_infect_households(Dev d)
 {
+    const StepScalars sc = *d.sc;
     const unsigned nch = min(d.wl[WL_SEG], d.seg_cap);
     const cs_params* P = d.P;
     const unsigned ts = sc.ts;
@@ -471,8 +502,9 @@ __global__ void __launch_bounds__(256) k_infect_households(Dev d, StepScalars sc
 // Within-group and between-group place passes of one infectious person per
 // lane (src/Sweep.cpp:381-640).  Floyd's sample is kept in LDS, one column
 // per thread (bank-conflict free).
-__global__ void __launch_bounds__(256) k_infect_places(Dev d, StepScalars sc)
+__global__ void __launch_bounds__(256) k_infect_places(Dev d)
 {
+    const StepScalars sc = *d.sc;
     __shared__ int picked[CS_FLOYD_MAX][256];
     const unsigned nch = min(d.wl[WL_SEG], d.seg_cap);
     const cs_params* P = d.P;
@@ -592,8 +624,9 @@ __global__ void __launch_bounds__(256) k_infect_places(Dev d, StepScalars sc)
 __device__ void cell_spatial_sum(const Dev& d, const StepScalars& sc, int rc, unsigned long long* shi, unsigned long long* slo,
                                  int* sK, double* ssis);
 
-__global__ void __launch_bounds__(CS_SUM_BLOCK) k_cell_spatial_sum(Dev d, StepScalars sc)
+__global__ void __launch_bounds__(CS_SUM_BLOCK) k_cell_spatial_sum(Dev d)
 {
+    const StepScalars sc = *d.sc;
     __shared__ unsigned long long shi[CS_SUM_BLOCK], slo[CS_SUM_BLOCK];
     __shared__ int sK;
     __shared__ double ssis;
@@ -664,8 +697,9 @@ __device__ void cell_spatial_sum(const Dev& d, const StepScalars& sc, int rc, un
 
 // one spatial draw per lane; a wave takes a chunk of up to 64 draws of one
 // source cell (src/Sweep.cpp:712-855)
-__global__ void __launch_bounds__(256) k_spatial_draws(Dev d, StepScalars sc)
+__global__ void __launch_bounds__(256) k_spatial_draws(Dev d)
 {
+    const StepScalars sc = *d.sc;
     const unsigned nch = min(d.wl[WL_DRAWS], d.draw_cap);
     const cs_params* P = d.P;
     const unsigned ts = sc.ts;
@@ -734,6 +768,7 @@ __global__ void __launch_bounds__(256) k_spatial_draws(Dev d, StepScalars sc)
 
 __global__ void k_resolve_claim(Dev d)
 {
+    if (halted(d)) return;
     unsigned n = min(*d.ev_count, d.ev_cap);
     for (unsigned e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
         size_t gp = (size_t)d.ev_gp[e];
@@ -743,8 +778,10 @@ __global__ void k_resolve_claim(Dev d)
     }
 }
 
-__global__ void k_resolve_apply(Dev d, StepScalars sc)
+__global__ void k_resolve_apply(Dev d)
 {
+    if (halted(d)) return;
+    const StepScalars sc = *d.sc;
     unsigned n = min(*d.ev_count, d.ev_cap);
     unsigned long long nn = 0;
     for (unsigned e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
@@ -766,6 +803,7 @@ __global__ void k_resolve_apply(Dev d, StepScalars sc)
 
 __global__ void k_resolve_reset(Dev d)
 {
+    if (halted(d)) return;
     unsigned n = min(*d.ev_count, d.ev_cap);
     for (unsigned e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
         size_t gp = (size_t)d.ev_gp[e];

@@ -23,7 +23,22 @@ __device__ __forceinline__ void sev_cum_only(Counters* c, int kind, int ag)
 // holidays (src/Sweep.cpp:889-918) for holiday h: one thread per
 // (realisation, place); members' absence windows only widen and every place
 // of the holiday writes the same bounds, so concurrent writes agree.
-__global__ void k_holidays(Dev d, int h, double ht)
+__device__ void holiday_places(const Dev& d, int h, double ht);
+
+// the holidays that start within this timestep (src/Sweep.cpp:889-892)
+__global__ void k_holidays(Dev d)
+{
+    if (halted(d)) return;
+    const cs_params* P = d.P;
+    if (!P->do_places) return;
+    const double t = d.sc->t;
+    for (int h = 0; h < P->num_holidays; h++) {
+        const double ht = P->holiday_start[h] + P->holidays_start_day_simtime;
+        if ((t + P->model_time_step >= ht) && (t < ht)) holiday_places(d, h, ht);
+    }
+}
+
+__device__ void holiday_places(const Dev& d, int h, double ht)
 {
     const cs_params* P = d.P;
     const int tot = d.np_off[CS_MAX_PLACE_TYPES];
@@ -148,8 +163,10 @@ __device__ void do_incub(const Dev& d, int r, int p, unsigned ts)
 
 // latent lists: DoIncub where latent_time == now (src/Sweep.cpp:925-927); a
 // wave takes a chunk of up to 64 latent persons of one cell
-__global__ void __launch_bounds__(256) k_latent_scan(Dev d, unsigned ts)
+__global__ void __launch_bounds__(256) k_latent_scan(Dev d)
 {
+    if (halted(d)) return;
+    const unsigned ts = d.sc->ts;
     const unsigned nch = min(d.wl[WL_SEG], d.seg_cap);
     const int lane = threadIdx.x & 63;
     const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -231,8 +248,10 @@ __device__ int do_case(const Dev& d, int r, int p, unsigned ts)
 
 // infectious lists: case onset, severity and outcome timers
 // (src/Sweep.cpp:929-977); a wave takes a chunk of up to 64 infectious persons
-__global__ void __launch_bounds__(256) k_infectious_scan(Dev d, unsigned ts)
+__global__ void __launch_bounds__(256) k_infectious_scan(Dev d)
 {
+    if (halted(d)) return;
+    const unsigned ts = d.sc->ts;
     const cs_params* P = d.P;
     const unsigned nch = min(d.wl[WL_SEG], d.seg_cap);
     const int lane = threadIdx.x & 63;
@@ -308,8 +327,10 @@ __global__ void __launch_bounds__(256) k_infectious_scan(Dev d, unsigned ts)
 
 // child-care absence (src/Update.cpp:836-868), evaluated for every request
 // of the step against the state after all timers, then applied
-__global__ void k_care_eval(Dev d, unsigned ts)
+__global__ void k_care_eval(Dev d)
 {
+    if (halted(d)) return;
+    const unsigned ts = d.sc->ts;
     const cs_params* P = d.P;
     unsigned n = *d.care_count;
     for (unsigned a = blockIdx.x * blockDim.x + threadIdx.x; a < n; a += gridDim.x * blockDim.x) {
@@ -333,8 +354,10 @@ __global__ void k_care_eval(Dev d, unsigned ts)
     }
 }
 
-__global__ void k_care_apply(Dev d, unsigned ts)
+__global__ void k_care_apply(Dev d)
 {
+    if (halted(d)) return;
+    const unsigned ts = d.sc->ts;
     const cs_params* P = d.P;
     unsigned n = *d.care_count;
     for (unsigned a = blockIdx.x * blockDim.x + threadIdx.x; a < n; a += gridDim.x * blockDim.x) {
@@ -454,8 +477,11 @@ __global__ void k_seed(Dev d, int n_seed)
 }
 
 // daily totals (one block per realisation)
+// sample: the index to write, or -1 for the device clock's sample count
 __global__ void k_record(Dev d, int sample)
 {
+    if (halted(d)) return;
+    if (sample < 0) sample = d.clock->sample;
     int r = blockIdx.x;
     __shared__ long long part[5][256];
     long long v[5] = {0, 0, 0, 0, 0};

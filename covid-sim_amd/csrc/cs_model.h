@@ -29,6 +29,16 @@ struct Counters {  // device mirror of cs_counters, updated with atomics
     unsigned long long sev_cum_age[CS_NUM_SEV_KINDS][CS_NUM_AGE_GROUPS];
 };
 
+struct StepScalars {  // per-timestep constants of InfectSweep (src/Sweep.cpp:294-301)
+    double t, fp, seas, beta_s, beta_h;
+    unsigned ts;
+};
+
+struct Clock {  // the device's own step counter for replayed step graphs
+    double t;
+    int step, sample;
+};
+
 struct Chunk {  // up to 64 consecutive entries starting at `first` of cell rc's segment
     int32_t rc, first;
 };
@@ -107,6 +117,8 @@ struct Dev {
     unsigned int* care_count;
     unsigned int* status;
     Work* wc;
+    StepScalars* sc;      // this step's scalars (k_step_begin)
+    Clock* clock;
     int32_t* ts_buf;      // [R][max_samples][CS_TS_FIELDS]
     int ts_max;
     long long* last_tot;  // [R][3] cumI, cumC, D at the previous sample

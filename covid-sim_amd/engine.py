@@ -135,6 +135,17 @@ class Engine:
     def run_steps(self, t0, n_steps, steps_per_sample):
         self.call("run_steps", float(t0), int(n_steps), int(steps_per_sample))
 
+    def set_queue_capacity(self, events=0, draw_chunks=0):
+        self.call("set_queue_capacity", int(events), int(draw_chunks))
+
+    def queue_capacity(self):
+        a, b = C.c_uint32(), C.c_uint32()
+        self.call("get_queue_capacity", C.byref(a), C.byref(b))
+        return a.value, b.value
+
+    def set_step_graphs(self, on):
+        self.call("set_step_graphs", int(bool(on)))
+
     def sync(self):
         self.call("sync")
 

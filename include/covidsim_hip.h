@@ -355,8 +355,19 @@ int  cs_update_probs(cs_engine* e);
  * and, as in RunModel, the S-drop check reads S as of the last sample.
  * Successive calls continue one series (continuing calls should start at a
  * sample boundary); cs_alloc_realisations, cs_reset_realisations and
- * cs_load_state start a new one. */
+ * cs_load_state start a new one.  By default the launches of one output step
+ * are captured once into a HIP graph and replayed on a device-side clock
+ * (results are identical either way; see cs_set_step_graphs). */
 int  cs_run_steps(cs_engine* e, double t0, int n_steps, int steps_per_sample);
+/* 1 (default): cs_run_steps replays captured step graphs; 0: it launches
+ * every kernel of every step from the host */
+int  cs_set_step_graphs(cs_engine* e, int enable);
+/* capacity of the infection event queue and of the spatial draw list (in
+ * chunks of 64 draws); 0 keeps a value.  Both grow by doubling when a step
+ * overflows them (the step is re-run, with the same result), so capacities
+ * only trade memory against re-runs. */
+int  cs_set_queue_capacity(cs_engine* e, uint32_t events, uint32_t draw_chunks);
+int  cs_get_queue_capacity(cs_engine* e, uint32_t* events, uint32_t* draw_chunks);
 
 /* results */
 int  cs_get_counters(cs_engine* e, int real, cs_counters* out, int reset);

@@ -240,3 +240,35 @@ def test_invariants_on_gpu(engmod, csa):
         assert (inf[S:S + L] == 1).all()
         assert np.isin(inf[S + L:S + L + I], [-1, 2, -2]).all()
         assert np.isin(inf[S + L + I:], [3, -3, 5, -5]).all()
+
+
+@pytest.mark.parametrize("graphs,events,draw_chunks", [(False, 0, 0), (True, 512, 4), (False, 512, 4)])
+def test_launch_modes_and_overflow_recovery(engmod, csa, graphs, events, draw_chunks):
+    """Replayed step graphs or direct launches, and event-queue / draw-list
+    overflows (tiny capacities force the halt, grow and re-run path in the
+    middle of a replay and of a direct step): the same 60-day trajectory as
+    the oracle.  Work counts are left out: a re-run step counts twice."""
+    arrays, pop, params, _ = mini()
+    seeds = np.array([21, 22], np.uint64)
+    g = engmod.Engine(0)
+    g.set_params(params)
+    g.set_population(pop)
+    g.alloc(seeds)
+    g.set_step_graphs(graphs)
+    if events:
+        g.set_queue_capacity(events, draw_chunks)
+    o = OracleEngine(params, pop)
+    o.alloc(seeds)
+    tspd = int(params.time_steps_per_day)
+    for eng in (g, o):
+        eng.reset(10)
+        eng.run_steps(0.0, 60 * tspd + 2, tspd)  # 2 steps past the last sample: the direct tail
+    assert np.array_equal(g.timeseries(61), o.timeseries(61))
+    if events:
+        ev, dr = g.queue_capacity()
+        assert ev > events and dr > draw_chunks  # the lists grew
+    for r in range(2):
+        a, b = csa.population.State(pop), csa.population.State(pop)
+        g.store_state(r, a)
+        o.store_state(r, b)
+        assert a.compare(b) == []

@@ -131,7 +131,9 @@ def main():
     ap.add_argument("--people", type=int, default=5_000_000)
     ap.add_argument("--realisations", type=int, default=64)
     ap.add_argument("--days", type=int, default=300)
-    ap.add_argument("--warmup-days", type=int, default=20, help="days simulated by each warmup step")
+    ap.add_argument("--warmup-days", type=int, default=60, help="days simulated by each warmup step (the autotuner "
+                                                                "then tunes on that state)")
+    ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--chunk-days", type=int, default=50, help="days per run_steps call inside a step (progress lines)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calibration", action="store_true", help="measure emulator-guided vs plain calibration instead")
@@ -169,6 +171,9 @@ def main():
     for _ in range(args.warmup):
         eng.reset(10)
         eng.run_steps(0.0, args.warmup_days * tspd, tspd)
+    if args.warmup and not args.no_autotune:
+        tune = eng.autotune(float(args.warmup_days))
+        log("autotune at day %d: %s" % (args.warmup_days, json.dumps(tune)))
     eng.sync()
     try:
         import torch

@@ -92,6 +92,7 @@ cs_state = _make("cs_state")
 cs_counters = _make("cs_counters")
 cs_work_counts = _make("cs_work_counts")
 cs_transmission_coeffs = _make("cs_transmission_coeffs")
+cs_tuning = _make("cs_tuning")
 
 globals().update({k: v for k, v in CONSTS.items()})
 
@@ -114,6 +115,8 @@ def declare(lib, prefix):
         "update_probs": (I, [vp]),
         "run_steps": (I, [vp, D, I, I]),
         "set_step_graphs": (I, [vp, I]),
+        "set_launch_shape": (I, [vp, C.c_uint32, C.c_uint32]),
+        "autotune": (I, [vp, D, I, P(cs_tuning)]),
         "set_queue_capacity": (I, [vp, C.c_uint32, C.c_uint32]),
         "get_queue_capacity": (I, [vp, P(C.c_uint32), P(C.c_uint32)]),
         "get_counters": (I, [vp, I, P(cs_counters), I]),

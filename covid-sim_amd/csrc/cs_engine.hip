@@ -965,6 +965,89 @@ int cs_get_queue_capacity(cs_engine* e, uint32_t* events, uint32_t* draw_chunks)
     return 0;
 }
 
+int cs_set_launch_shape(cs_engine* e, uint32_t grid_waves, uint32_t grid_cells)
+{
+    if (!e) return CS_EINVAL;
+    if (grid_waves) e->grid_waves = grid_waves;
+    if (grid_cells) e->grid_cells = grid_cells;
+    drop_graph(e);
+    return 0;
+}
+
+int cs_autotune(cs_engine* e, double t, int reps, cs_tuning* out)
+{
+    if (!e || reps <= 0) return CS_EINVAL;
+    if (e->R == 0) return set_err(e, CS_ESTATE, "no realisations allocated");
+    int rc = check_path(e);
+    if (rc) return rc;
+    HIPCHK(e, hipSetDevice(e->device));
+    const bool prof = e->prof;
+    e->prof = false;
+    hipEvent_t a = pool_event(e), b = pool_event(e);
+    // the event phase reads the state and writes only scratch, events and
+    // work lists, so it can be timed repeatedly on the current state
+    auto time_events = [&](float* ms_out) -> int {
+        float best = 1e30f;
+        for (int k = 0; k < reps; k++) {
+            cs::k_step_begin<<<1, 64, 0, e->stream>>>(e->d, t, 0);
+            HIPCHK(e, hipEventRecord(a, e->stream));
+            int r2 = launch_infect_events(e);
+            if (r2) return r2;
+            HIPCHK(e, hipEventRecord(b, e->stream));
+            HIPCHK(e, hipEventSynchronize(b));
+            float ms = 0;
+            HIPCHK(e, hipEventElapsedTime(&ms, a, b));
+            best = std::min(best, ms);
+            unsigned f = 0;
+            HIPCHK(e, hipMemcpy(&f, e->d.status, sizeof(f), hipMemcpyDeviceToHost));
+            if (f & cs::FLAG_OVERFLOW) {  // grow and measure again
+                if ((r2 = grow_events(e))) return r2;
+                k--;
+                best = 1e30f;
+            } else if ((r2 = check_flags(e))) return r2;
+        }
+        *ms_out = best;
+        return 0;
+    };
+    const uint32_t waves[] = {1024, 2048, 4096, 8192, 16384};
+    const uint32_t cells[] = {512, 1024, 2048, 4096};
+    uint32_t bw = e->grid_waves, bc = e->grid_cells;
+    float best = 1e30f;
+    for (uint32_t w : waves) {
+        e->grid_waves = w;
+        float ms;
+        if ((rc = time_events(&ms))) { e->prof = prof; return rc; }
+        if (ms < best) { best = ms; bw = w; }
+    }
+    e->grid_waves = bw;
+    best = 1e30f;
+    for (uint32_t c : cells) {
+        e->grid_cells = c;
+        float ms;
+        if ((rc = time_events(&ms))) { e->prof = prof; return rc; }
+        if (ms < best) { best = ms; bc = c; }
+    }
+    e->grid_cells = bc;
+    // the queue holds 4 times the events of this state (it still grows on demand)
+    unsigned nev = 0;
+    HIPCHK(e, hipMemcpy(&nev, e->d.ev_count, sizeof(nev), hipMemcpyDeviceToHost));
+    const uint64_t want = std::min<uint64_t>((uint64_t)nev * 4u, 0xffffffffull);
+    if (want > e->d.ev_cap && (rc = cs_set_queue_capacity(e, (uint32_t)want, 0))) { e->prof = prof; return rc; }
+    e->ev_pool.push_back(a);
+    e->ev_pool.push_back(b);
+    e->prof = prof;
+    drop_graph(e);
+    if (out) {
+        out->grid_waves = bw;
+        out->grid_cells = bc;
+        out->event_capacity = e->d.ev_cap;
+        out->draw_capacity = e->d.draw_cap;
+        out->events_at_state = nev;
+        out->event_phase_ms = best;
+    }
+    return 0;
+}
+
 int cs_set_step_graphs(cs_engine* e, int enable)
 {
     if (!e) return CS_EINVAL;

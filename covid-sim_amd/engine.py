@@ -143,6 +143,15 @@ class Engine:
         self.call("get_queue_capacity", C.byref(a), C.byref(b))
         return a.value, b.value
 
+    def set_launch_shape(self, grid_waves=0, grid_cells=0):
+        self.call("set_launch_shape", int(grid_waves), int(grid_cells))
+
+    def autotune(self, t, reps=3):
+        """Launch shapes and queue capacity for the current state (cs_autotune)."""
+        tu = abi.cs_tuning()
+        self.call("autotune", float(t), int(reps), C.byref(tu))
+        return {f: getattr(tu, f) for f, _ in tu._fields_}
+
     def set_step_graphs(self, on):
         self.call("set_step_graphs", int(bool(on)))
 

@@ -368,6 +368,21 @@ int  cs_set_step_graphs(cs_engine* e, int enable);
  * only trade memory against re-runs. */
 int  cs_set_queue_capacity(cs_engine* e, uint32_t events, uint32_t draw_chunks);
 int  cs_get_queue_capacity(cs_engine* e, uint32_t* events, uint32_t* draw_chunks);
+/* launch shapes: blocks of the per-person passes (4 waves each, one wave
+ * per chunk of 64 persons) and of the per-cell passes; 0 keeps a value.
+ * Results do not depend on them. */
+int  cs_set_launch_shape(cs_engine* e, uint32_t grid_waves, uint32_t grid_cells);
+/* autotuner: at the current state and time t, times the InfectSweep event
+ * phase (it reads the state and writes only scratch, so it can be repeated)
+ * for each candidate launch shape, best of reps, keeps the fastest, and sizes
+ * the event queue at 4 times this state's events */
+typedef struct cs_tuning {
+    uint32_t grid_waves, grid_cells;
+    uint32_t event_capacity, draw_capacity;
+    uint32_t events_at_state;
+    float    event_phase_ms;
+} cs_tuning;
+int  cs_autotune(cs_engine* e, double t, int reps, cs_tuning* out);
 
 /* results */
 int  cs_get_counters(cs_engine* e, int real, cs_counters* out, int reset);

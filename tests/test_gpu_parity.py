@@ -272,3 +272,25 @@ def test_launch_modes_and_overflow_recovery(engmod, csa, graphs, events, draw_ch
         g.store_state(r, a)
         o.store_state(r, b)
         assert a.compare(b) == []
+
+
+def test_autotune_keeps_results(engmod, csa):
+    """The autotuner times the event phase on the current state without
+    changing it; launch shapes and capacities do not change results."""
+    arrays, pop, params, _ = mini()
+    seeds = np.array([31, 32], np.uint64)
+    g = engmod.Engine(0)
+    g.set_params(params)
+    g.set_population(pop)
+    g.alloc(seeds)
+    o = OracleEngine(params, pop)
+    o.alloc(seeds)
+    tspd = int(params.time_steps_per_day)
+    for eng in (g, o):
+        eng.reset(10)
+        eng.run_steps(0.0, 30 * tspd, tspd)
+    tune = g.autotune(30.0, reps=2)
+    assert tune["grid_waves"] > 0 and tune["grid_cells"] > 0 and tune["event_phase_ms"] > 0
+    for eng in (g, o):
+        eng.run_steps(30.0, 30 * tspd, tspd)
+    assert np.array_equal(g.timeseries(60), o.timeseries(60))

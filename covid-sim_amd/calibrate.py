@@ -76,6 +76,8 @@ def from_curves(inc, deaths):
 class Simulator:
     """Batched ensembles of one population; 64 realisations per launch."""
 
+    progress = None  # optional callable(str), called every 8 launches
+
     def __init__(self, n_people=1_000_000, pop_seed=2024, n_real=64, itc_seed=77):
         import importlib
         import os
@@ -95,7 +97,6 @@ class Simulator:
         self.n_real = n_real
         self.launches = 0
         self.sim_seconds = 0.0
-        self.progress = None  # optional callable(str), called every 8 launches
 
     def beta(self, r0):
         return np.maximum(0.0, (np.asarray(r0, np.float64) - self.r0_hh - self.r0_pl) / self.sp_r0)

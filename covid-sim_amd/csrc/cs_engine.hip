@@ -669,7 +669,7 @@ int cs_alloc_realisations(cs_engine* e, int n_real, const uint64_t* seeds)
     AL(inf, RN); AL(sev_cur, RN); AL(sev_final, RN); AL(bits, RN); AL(qc, RN); AL(dct, RN);
     AL(listpos, RN); AL(infector, RN); AL(itype, RN); AL(infness, RN); AL(susc, RN); AL(prob_care, RN);
     AL(t_inf, RN); AL(t_lat, RN); AL(t_rod, RN); AL(t_sari, RN); AL(t_crit, RN); AL(t_step, RN); AL(t_det, RN);
-    AL(t_as, RN); AL(t_ae, RN); AL(t_ts, RN); AL(t_te, RN); AL(t_vs, RN); AL(t_is, RN); AL(t_qs, RN);
+    AL(t_as, RN); AL(t_ae, RN); AL(t_ts, RN); AL(t_te, RN); AL(t_vs, RN); AL(t_is, RN); AL(t_qs, RN); AL(t_gate, RN);
     AL(cS, RC); AL(cL, RC); AL(cI, RC); AL(cR, RC); AL(cD, RC); AL(cumTC, RC); AL(S0, RC);
     AL(tot_prob, RC); AL(cum_trans, RC * NC); AL(invcdf, RC * 1025); AL(perm, RN);
     AL(pl_cs, PL); AL(pl_ce, PL); AL(m_socdist, R * NM); AL(m_moverest, R * NM); AL(m_trig, R * NM * 6);
@@ -752,6 +752,7 @@ int cs_load_state(cs_engine* e, int real, const cs_state* s)
     Dev& d = e->d;
     const size_t N = e->N, NC = e->NC, o = (size_t)real * N, oc = (size_t)real * NC;
     e->ts_fresh = true;
+    HIPCHK(e, hipMemset(d.t_gate + o, 0, N * sizeof(uint16_t)));  // loaded timers: look at every one
 #define UP(dst, src, off, n) if (src) HIPCHK(e, hipMemcpy(d.dst + (off), src, (n) * sizeof(*d.dst), hipMemcpyHostToDevice))
     UP(inf, s->inf, o, N); UP(sev_cur, s->severity_current, o, N); UP(sev_final, s->severity_final, o, N);
     UP(bits, s->bits, o, N); UP(listpos, s->listpos, o, N); UP(infector, s->infector, o, N); UP(itype, s->infect_type, o, N);

@@ -78,6 +78,7 @@ struct Dev {
     float *infness, *susc, *prob_care;
     uint16_t *t_inf, *t_lat, *t_rod, *t_sari, *t_crit, *t_step, *t_det, *t_as, *t_ae;
     uint16_t *t_ts, *t_te, *t_vs, *t_is, *t_qs;
+    uint16_t* t_gate;     // [R][N] no timer of the infectious person fires before this step
     int32_t *cS, *cL, *cI, *cR, *cD, *cumTC, *S0;
     float* tot_prob;
     float* cum_trans;

@@ -137,6 +137,8 @@ def main():
     ap.add_argument("--chunk-days", type=int, default=50, help="days per run_steps call inside a step (progress lines)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calibration", action="store_true", help="measure emulator-guided vs plain calibration instead")
+    ap.add_argument("--summarise-pmc", metavar="DIR", help="print per-kernel sums of a rocprofv3 --pmc output "
+                                                            "directory (counter_collection.csv) and exit")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="before the run, copy this many MiB device-to-device (a copy of known size for "
                          "scaling the profiler's byte counters)")
@@ -147,6 +149,8 @@ def main():
         print("bench.py runs on one GPU: this build has no multi-GPU path (each realisation batch "
               "would shard by realisation; see the design notes)", file=sys.stderr)
         sys.exit(2)
+    if args.summarise_pmc:
+        return summarise_pmc(args.summarise_pmc)
     if args.calibration:
         return calibration(args)
     import importlib
@@ -246,6 +250,25 @@ def main():
         "cpu_baseline": base,
     }
     print(json.dumps(out), flush=True)
+
+
+def summarise_pmc(d):
+    """Per-kernel totals and per-dispatch means of every counter in a
+    rocprofv3 --pmc CSV output directory."""
+    import csv
+    import glob
+    agg, disp = {}, {}
+    for f in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+        with open(f) as fh:
+            for row in csv.DictReader(fh):
+                k, c = row["Kernel_Name"], row["Counter_Name"]
+                agg.setdefault(k, {}).setdefault(c, 0.0)
+                agg[k][c] += float(row["Counter_Value"])
+                disp.setdefault(k, set()).add(row["Dispatch_Id"])
+    for k in sorted(agg, key=lambda k: -max(agg[k].values())):
+        n = len(disp[k])
+        print("%s | dispatches %d | %s" % (k[:80], n, " | ".join("%s total %.6g mean %.6g" % (c, v, v / n)
+                                                                   for c, v in sorted(agg[k].items()))))
 
 
 def calibration(args):

@@ -879,6 +879,7 @@ int cs_run_steps(cs_engine* e, double t0, int n_steps, int sps)
                                        (size_t)e->ts_n * CS_TS_FIELDS * 4, (size_t)d.R, hipMemcpyDeviceToDevice, e->stream));
         if (d.ts_buf) {
             HIPCHK(e, hipStreamSynchronize(e->stream));
+            drop_graph(e);  // the captured record launches hold the old buffer
             (void)hipFree(d.ts_buf);
             e->real_allocs.erase(std::remove(e->real_allocs.begin(), e->real_allocs.end(), (void*)d.ts_buf), e->real_allocs.end());
         }

@@ -193,11 +193,9 @@ def main():
         torch.cuda.synchronize()
         log("copy probe: %d bytes" % a.numel())
         del a, b
-    eng.work_counts(reset=True)
-    eng.set_profiling(True)
-    times = []
     chunk = max(1, args.chunk_days) * tspd
-    for k in range(args.steps):
+
+    def one_step(k, label):
         eng.sync()
         ta = time.perf_counter()
         eng.reset(10)
@@ -205,9 +203,20 @@ def main():
             n = min(chunk, steps - s0)
             eng.run_steps(s0 / tspd, n, tspd)  # one series across the calls
             eng.sync()
-            log("  step %d: day %d (%.1f s)" % (k, (s0 + n) // tspd, time.perf_counter() - ta))
-        times.append(time.perf_counter() - ta)
-        log("step %d: %.2f s" % (k, times[-1]))
+            log("  %s %d: day %d (%.1f s)" % (label, k, (s0 + n) // tspd, time.perf_counter() - ta))
+        dt = time.perf_counter() - ta
+        log("%s %d: %.2f s" % (label, k, dt))
+        return dt
+
+    # the timed steps: replayed step graphs, nothing recorded per launch
+    times = [one_step(k, "step") for k in range(args.steps)]
+    # one more pass of the same workload with a HIP event pair around every
+    # launch (direct launches): the per-kernel times and work counts of the
+    # roofline; its kernels are the ones the timed steps replayed
+    eng.work_counts(reset=True)
+    eng.set_profiling(True)
+    t_prof = one_step(0, "profiled step")
+    eng.set_profiling(False)
     prof = eng.profile()
     wc = eng.work_counts()
     wc["n_cell_steps"] = args.realisations * pop.n_cells * prof["cell_spatial_sum"][1]  # cells visited per launch
@@ -225,6 +234,7 @@ def main():
     log("work: " + json.dumps(wc))
     log("mean attack rate %.3f" % attack)
     total_alg = sum(BYTES[u] * wc[u] for u in BYTES)
+    traffic = measured_traffic(dom, args)
     base = None if args.no_cpu_baseline else cpu_baseline()
     out = {
         "metric": "person-days/s, 64-run ensemble",
@@ -242,14 +252,33 @@ def main():
         "config": {"workload": "batched ensemble: %d realisations x %d-person synthetic region, %d days, 4 steps/day, "
                                "10 seeds, no interventions" % (args.realisations, args.people, args.days),
                    "realisations": args.realisations, "persons": args.people, "days": args.days, "cells": pop.n_cells,
-                   "mean_attack_rate": round(attack, 4),
-                   "all_kernel_algorithmic_gbs": round(total_alg / total / 1e9, 2)},
+                   "mean_attack_rate": round(attack, 4), "launches": "replayed step graphs",
+                   "all_kernel_algorithmic_gbs": round(total_alg / t_prof / 1e9, 2)},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-                     "bytes_per_launch": round(alg_bytes), "ms_per_launch": round(ms / max(launches, 1), 4)},
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+                     "bytes_per_launch": round(alg_bytes), "ms_per_launch": round(ms / max(launches, 1), 4),
+                     "measured": "HIP events around every launch of a profiled pass of the same workload "
+                                 "(%.2f s, direct launches)" % t_prof},
         "cpu_baseline": base,
     }
     print(json.dumps(out), flush=True)
+
+
+def measured_traffic(kernel, args):
+    """HBM bytes per launch of `kernel` from the committed profiler summary
+    (profiles/traffic.json: rocprofv3 FETCH_SIZE and WRITE_SIZE per dispatch,
+    scaled by the same counters' reading for a copy of known size), when it
+    was measured on this workload; else None."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    w = t.get("workload", {})
+    if (w.get("realisations"), w.get("persons"), w.get("days")) != (args.realisations, args.people, args.days):
+        return None
+    k = t.get("kernels", {}).get(kernel)
+    return None if k is None else k.get("hbm_bytes_per_launch")
 
 
 def summarise_pmc(d):

@@ -38,11 +38,9 @@ struct cs_engine {
     unsigned grid_cells = 2048;      // blocks for the per-cell passes (a block per cell)
     // replayed step graphs (cs_run_steps): one graph per output step
     bool use_graphs = true;
-    bool capturing = false;
     hipGraphExec_t gexec = nullptr;
     int graph_sps = 0;
-    bool graph_prof = false;         // the graph holds profiling events
-    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> gevents;
+
     // profiling: event pairs per kernel kind, resolved on cs_get_profile
     bool prof = false;
     std::vector<hipEvent_t> ev_pool;
@@ -133,7 +131,7 @@ struct ProfScope {  // records a start/stop event pair around one launch
     hipEvent_t a = nullptr, b = nullptr;
     ProfScope(cs_engine* e_, int k) : e(e_), kind(k)
     {
-        if (e->prof) {
+        if (e->prof) {  // never inside a captured graph: run_steps launches directly while profiling
             a = pool_event(e);
             b = pool_event(e);
             (void)hipEventRecord(a, e->stream);
@@ -143,7 +141,7 @@ struct ProfScope {  // records a start/stop event pair around one launch
     {
         if (e->prof) {
             (void)hipEventRecord(b, e->stream);
-            (e->capturing ? e->gevents : e->ev_open).push_back({kind, {a, b}});
+            e->ev_open.push_back({kind, {a, b}});
         }
     }
 };
@@ -277,11 +275,6 @@ void drop_graph(cs_engine* e)
 {
     if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
     e->gexec = nullptr;
-    for (auto& x : e->gevents) {
-        e->ev_pool.push_back(x.second.first);
-        e->ev_pool.push_back(x.second.second);
-    }
-    e->gevents.clear();
 }
 
 int grow_events(cs_engine* e)
@@ -356,13 +349,11 @@ int capture_graph(cs_engine* e, int sps)
     drop_graph(e);
     Dev& d = e->d;
     HIPCHK(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
-    e->capturing = true;
     int rc = 0;
     for (int k = 0; k < sps && !rc; k++) {
         cs::k_step_begin<<<1, 64, 0, e->stream>>>(d, 0.0, 1);
         if (!(rc = launch_infect_events(e)) && !(rc = launch_resolve(e))) rc = launch_step_tail(e, k == sps - 1);
     }
-    e->capturing = false;
     hipGraph_t g = nullptr;
     hipError_t st = hipStreamEndCapture(e->stream, &g);
     if (rc) {
@@ -377,21 +368,6 @@ int capture_graph(cs_engine* e, int sps)
         return set_err(e, CS_EHIP, "step graph instantiation: %s", hipGetErrorString(st));
     }
     e->graph_sps = sps;
-    e->graph_prof = e->prof;
-    return 0;
-}
-
-// after a replay with profiling events: add their times
-int harvest_graph_events(cs_engine* e)
-{
-    HIPCHK(e, hipStreamSynchronize(e->stream));
-    for (auto& x : e->gevents) {
-        float ms = 0;
-        if (hipEventElapsedTime(&ms, x.second.first, x.second.second) == hipSuccess) {
-            e->prof_ms[x.first] += ms;
-            e->prof_n[x.first]++;
-        }
-    }
     return 0;
 }
 
@@ -674,7 +650,7 @@ int cs_alloc_realisations(cs_engine* e, int n_real, const uint64_t* seeds)
     AL(tot_prob, RC); AL(cum_trans, RC * NC); AL(invcdf, RC * 1025); AL(perm, RN);
     AL(pl_cs, PL); AL(pl_ce, PL); AL(m_socdist, R * NM); AL(m_moverest, R * NM); AL(m_trig, R * NM * 6);
     AL(cnt, R); AL(prev_prop, R); AL(samp_prop, R); AL(loc_init, 2 * R);
-    AL(si_buf, RN); AL(ci_buf, RN); AL(spK, RC); AL(sp_sum, RC);
+    AL(si_buf, RN); AL(ci_buf, RN); AL(spK, RC);
     d.seg_cap = (unsigned)std::min<size_t>(RN / 64 + RC + 1, 0xffffffffu);
     d.draw_cap = d.seg_cap;
     AL(segl, d.seg_cap); AL(drawl, d.draw_cap); AL(cells, RC); AL(moved, RC); AL(wl, cs::WL_N);
@@ -891,15 +867,12 @@ int cs_run_steps(cs_engine* e, double t0, int n_steps, int sps)
     const int periods = n_steps / sps;
     int done = 0;
     double t = t0;
-    if (e->use_graphs && periods > 0) {
-        if ((!e->gexec || e->graph_sps != sps || e->graph_prof != e->prof) && (rc = capture_graph(e, sps))) return rc;
+    if (e->use_graphs && !e->prof && periods > 0) {  // profiling times every launch: direct launches then
+        if ((!e->gexec || e->graph_sps != sps) && (rc = capture_graph(e, sps))) return rc;
         int p = 0;
         for (int guard = 0; p < periods && guard < 64;) {
             const int batch = std::min(periods - p, 8);  // output steps between flag checks
-            for (int b = 0; b < batch; b++) {
-                HIPCHK(e, hipGraphLaunch(e->gexec, e->stream));
-                if (e->graph_prof && (rc = harvest_graph_events(e))) return rc;
-            }
+            for (int b = 0; b < batch; b++) HIPCHK(e, hipGraphLaunch(e->gexec, e->stream));
             cs::Clock c;
             if ((rc = read_clock(e, &c))) return rc;
             unsigned f = 0;

@@ -673,7 +673,6 @@ __device__ void cell_spatial_sum(const Dev& d, const StepScalars& sc, int rc, un
             K = poisson(s, sis * beta_s * ((double)d.tot_prob[rc]));
         }
         d.spK[rc] = K;
-        d.sp_sum[rc] = sis;
         sK = K;
         ssis = sis;
         atomicAdd(&d.wc->n_inf, (unsigned long long)I);
@@ -812,6 +811,5 @@ __global__ void k_resolve_reset(Dev d)
     }
 }
 
-__global__ void k_zero_u32(unsigned int* p) { *p = 0; }
 
 }  // namespace cs

@@ -95,7 +95,6 @@ struct Dev {
     double* si_buf;       // [R][N] spatial infectiousness by permutation slot (sign = closed flag)
     float* ci_buf;        // [R][N] normalised cumulative infectiousness by slot
     int32_t* spK;         // [R*NC] draws per source cell
-    double* sp_sum;       // [R*NC]
     // work lists, rebuilt every phase: chunks of up to 64 consecutive list
     // entries of one (realisation, cell), and lists of cells
     Chunk* segl;          // persons of a latent or infectious segment

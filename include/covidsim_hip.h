@@ -457,7 +457,8 @@ int  cs_mlp_set_data(cs_mlp* m, const float* x, const float* t, int n);
 int  cs_mlp_train(cs_mlp* m, int epochs, int batch, float lr, uint64_t seed, double* loss_per_epoch);
 
 /* per-kernel device time, measured with HIP events recorded around each
- * launch on the engine's stream while profiling is enabled */
+ * launch on the engine's stream while profiling is enabled (cs_run_steps
+ * then launches every kernel directly instead of replaying step graphs) */
 enum { CS_K_INFECT_HOUSEHOLDS = 0, CS_K_SPATIAL_DRAWS, CS_K_RESOLVE, CS_K_LIST_MOVE,
        CS_K_LATENT_SCAN, CS_K_INFECTIOUS_SCAN, CS_K_UPDATE_PROBS, CS_K_CELL_SPATIAL_SUM,
        CS_K_INFECT_PLACES, CS_K_OTHER, CS_NUM_KERNEL_KINDS };

@@ -1113,7 +1113,8 @@ int cs_set_profiling(cs_engine* e, int enable)
     HIPCHK(e, hipStreamSynchronize(e->stream));
     resolve_profile(e);
     e->prof = enable != 0;
-    for (int k = 0; k < CS_NUM_KERNEL_KINDS; k++) { e->prof_ms[k] = 0; e->prof_n[k] = 0; }
+    if (e->prof)  // enabling starts a new account; disabling keeps it for cs_get_profile
+        for (int k = 0; k < CS_NUM_KERNEL_KINDS; k++) { e->prof_ms[k] = 0; e->prof_n[k] = 0; }
     return 0;
 }
 

@@ -12,9 +12,11 @@ Parameters are the 200k-person region's derived parameter set
 (covid-sim_amd/inputs/c1/params.bin); the population comes from
 covid-sim_amd/popgen.py at the same density and the region's geometry.
 
-The reported roofline is for the kernel with the largest device time,
-from HIP events recorded around its launches on the library's stream during
-the timed steps; its algorithmic bytes per launch come from the event
+The timed steps replay one captured HIP graph per output step (--no-graphs:
+direct launches).  The reported roofline is for the kernel with the largest
+device time, from HIP events recorded around each of its launches on the
+library's stream during one more, profiled pass of the same workload with
+direct launches; its algorithmic bytes per launch come from the event
 counts the library keeps and the per-event byte costs below.
 
 The CPU baseline runs the reference itself (oracle/_ref/covidsim_ref, built
@@ -136,9 +138,13 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--chunk-days", type=int, default=50, help="days per run_steps call inside a step (progress lines)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-graphs", action="store_true", help="launch every kernel directly in the timed steps too "
+                                                            "(for runs under rocprofv3 tracing)")
     ap.add_argument("--calibration", action="store_true", help="measure emulator-guided vs plain calibration instead")
     ap.add_argument("--summarise-pmc", metavar="DIR", help="print per-kernel sums of a rocprofv3 --pmc output "
                                                             "directory (counter_collection.csv) and exit")
+    ap.add_argument("--pmc-last", type=int, default=0, help="with --summarise-pmc: also the means over each "
+                                                             "kernel's last N dispatches")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="before the run, copy this many MiB device-to-device (a copy of known size for "
                          "scaling the profiler's byte counters)")
@@ -150,7 +156,7 @@ def main():
               "would shard by realisation; see the design notes)", file=sys.stderr)
         sys.exit(2)
     if args.summarise_pmc:
-        return summarise_pmc(args.summarise_pmc)
+        return summarise_pmc(args.summarise_pmc, args.pmc_last)
     if args.calibration:
         return calibration(args)
     import importlib
@@ -169,6 +175,8 @@ def main():
     eng.set_population(pop)
     seeds = np.arange(1, args.realisations + 1, dtype=np.uint64) * np.uint64(0x9E3779B97F4A7C15)
     eng.alloc(seeds)
+    if args.no_graphs:
+        eng.set_step_graphs(False)
     tspd = int(round(params.time_steps_per_day))
     steps = args.days * tspd
     log("set-up done (%.1f s); warmup %d x %d days" % (time.perf_counter() - t0, args.warmup, args.warmup_days))
@@ -252,7 +260,7 @@ def main():
         "config": {"workload": "batched ensemble: %d realisations x %d-person synthetic region, %d days, 4 steps/day, "
                                "10 seeds, no interventions" % (args.realisations, args.people, args.days),
                    "realisations": args.realisations, "persons": args.people, "days": args.days, "cells": pop.n_cells,
-                   "mean_attack_rate": round(attack, 4), "launches": "replayed step graphs",
+                   "mean_attack_rate": round(attack, 4), "launches": "direct launches" if args.no_graphs else "replayed step graphs",
                    "all_kernel_algorithmic_gbs": round(total_alg / t_prof / 1e9, 2)},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
@@ -281,23 +289,33 @@ def measured_traffic(kernel, args):
     return None if k is None else k.get("hbm_bytes_per_launch")
 
 
-def summarise_pmc(d):
+def summarise_pmc(d, last=0):
     """Per-kernel totals and per-dispatch means of every counter in a
-    rocprofv3 --pmc CSV output directory."""
+    rocprofv3 --pmc CSV output directory; with last > 0 also the means over
+    each kernel's last `last` dispatches (the profiled pass of a bench run)."""
     import csv
     import glob
-    agg, disp = {}, {}
+    vals = {}
     for f in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
         with open(f) as fh:
             for row in csv.DictReader(fh):
                 k, c = row["Kernel_Name"], row["Counter_Name"]
-                agg.setdefault(k, {}).setdefault(c, 0.0)
-                agg[k][c] += float(row["Counter_Value"])
-                disp.setdefault(k, set()).add(row["Dispatch_Id"])
-    for k in sorted(agg, key=lambda k: -max(agg[k].values())):
-        n = len(disp[k])
-        print("%s | dispatches %d | %s" % (k[:80], n, " | ".join("%s total %.6g mean %.6g" % (c, v, v / n)
-                                                                   for c, v in sorted(agg[k].items()))))
+                key = (int(row["Dispatch_Id"]), c)
+                vals.setdefault(k, {})
+                vals[k][key] = vals[k].get(key, 0.0) + float(row["Counter_Value"])
+    for k in sorted(vals, key=lambda k: -sum(vals[k].values())):
+        per = {}
+        for (did, c), v in vals[k].items():
+            per.setdefault(c, {})[did] = v
+        parts = []
+        for c in sorted(per):
+            ids = sorted(per[c])
+            tot = sum(per[c].values())
+            txt = "%s total %.6g mean %.6g" % (c, tot, tot / len(ids))
+            if last and len(ids) >= last:
+                txt += " mean_last%d %.6g" % (last, sum(per[c][i] for i in ids[-last:]) / last)
+            parts.append(txt)
+        print("%s | dispatches %d | %s" % (k[:80], len(per[min(per)]), " | ".join(parts)))
 
 
 def calibration(args):

@@ -56,6 +56,24 @@ __device__ __forceinline__ void batch_add(const Dev& d, int r, int c, int p)
     d.batch[(size_t)r * d.N + d.cstart[c] + idx] = p;
 }
 
+// A work-list kernel gives each wave whole chunks, so the chunk and every
+// value derived from it (realisation, cell, stream seed, slot base) is the
+// same in all lanes.  readfirstlane states that to the compiler, which then
+// keeps those values in scalar registers and reads them with scalar loads.
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ unsigned uni(unsigned v) { return (unsigned)__builtin_amdgcn_readfirstlane((int)v); }
+__device__ __forceinline__ uint64_t uni(uint64_t v)
+{
+    return ((uint64_t)uni((unsigned)(v >> 32)) << 32) | uni((unsigned)v);
+}
+__device__ __forceinline__ unsigned grid_wave() { return uni((blockIdx.x * blockDim.x + threadIdx.x) >> 6); }
+__device__ __forceinline__ unsigned grid_nwaves() { return (gridDim.x * blockDim.x) >> 6; }
+__device__ __forceinline__ Chunk chunk_at(const Chunk* l, unsigned ch)
+{
+    const Chunk e = l[ch];
+    return Chunk{uni(e.rc), uni(e.first)};
+}
+
 __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v)
 {
     for (int w = 32; w > 0; w >>= 1) v += __shfl_xor(v, w, 64);
@@ -431,18 +449,16 @@ __global__ void __launch_bounds__(256) k_infect_households(Dev d)
     const cs_params* P = d.P;
     const unsigned ts = sc.ts;
     const int lane = threadIdx.x & 63;
-    const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const long long nwaves = ((long long)gridDim.x * blockDim.x) >> 6;
     unsigned long long w_hh_inf = 0, w_hh_mem = 0;
-    for (long long ch = wave; ch < nch; ch += nwaves) {
-        const Chunk e = d.segl[ch];
+    for (unsigned ch = grid_wave(); ch < nch; ch += grid_nwaves()) {
+        const Chunk e = chunk_at(d.segl, ch);
         const int rc = e.rc;
         const int j = e.first + lane;
-        if (j >= d.cI[rc]) continue;
+        if (j >= uni(d.cI[rc])) continue;
         const int r = rc / d.NC, c = rc % d.NC;
-        const uint64_t seed = d.seed[r];
+        const uint64_t seed = uni(d.seed[r]);
         const size_t rN = (size_t)r * d.N;
-        const size_t sbase = rN + d.cstart[c] + d.cS[rc] + d.cL[rc];  // slot of infectious index 0
+        const size_t sbase = rN + uni(d.cstart[c] + d.cS[rc] + d.cL[rc]);  // slot of infectious index 0
         const int i = d.perm[sbase + j];
         const size_t gi = rN + i;
         const int gen = d.itype[gi] / CS_INFECT_TYPE_MASK;
@@ -451,22 +467,28 @@ __global__ void __launch_bounds__(256) k_infect_households(Dev d)
             const int h = d.hh[i], nh = d.hnh[h];
             if (nh > 1) {
                 const int f0 = d.hfirst[h];
-                double HI = sc.beta_h * house_inf(d, r, i, ts);
-                bool absent = false;
-                // a member absent from a closed place (src/Sweep.cpp:348-355); the
-                // member's absence is tested first (cheaper, same result)
-                for (int m = f0; m < f0 + nh && !absent; m++) {
-                    if (!host_absent(d, rN + m, ts)) continue;
-                    for (int pt = 0; pt < d.NPT && !absent; pt++) {
-                        int lk = d.plinks[(size_t)m * CS_MAX_PLACE_TYPES + pt];
-                        if (lk >= 0) absent = place_closed(d, r, pt, lk, ts);
-                    }
-                }
-                if (absent) HI *= P->efficacies[2][CS_HOUSE];
                 w_hh_inf++;
-                for (int m = f0; m < f0 + nh; m++) {
-                    w_hh_mem++;
-                    if (d.inf[rN + m] == CS_SUSCEPTIBLE) {
+                w_hh_mem += nh;
+                // the susceptible members first: with none, nothing below can
+                // infect, so the household's hazard is not needed
+                unsigned smask = 0;
+                for (int m = f0; m < f0 + nh; m++)
+                    if (d.inf[rN + m] == CS_SUSCEPTIBLE) smask |= 1u << (m - f0);
+                if (smask) {
+                    double HI = sc.beta_h * house_inf(d, r, i, ts);
+                    bool absent = false;
+                    // a member absent from a closed place (src/Sweep.cpp:348-355); the
+                    // member's absence is tested first (cheaper, same result)
+                    for (int m = f0; m < f0 + nh && !absent; m++) {
+                        if (!host_absent(d, rN + m, ts)) continue;
+                        for (int pt = 0; pt < d.NPT && !absent; pt++) {
+                            int lk = d.plinks[(size_t)m * CS_MAX_PLACE_TYPES + pt];
+                            if (lk >= 0) absent = place_closed(d, r, pt, lk, ts);
+                        }
+                    }
+                    if (absent) HI *= P->efficacies[2][CS_HOUSE];
+                    for (unsigned sm = smask; sm; sm &= sm - 1) {
+                        const int m = f0 + __builtin_ctz(sm);
                         double foi = HI * house_susc(d, r, m, ts, i);
                         Stream s(seed, cw0(ts, PS_HH), (uint32_t)i, (uint32_t)(m - f0));
                         if (s.next() < foi)
@@ -510,19 +532,17 @@ __global__ void __launch_bounds__(256) k_infect_places(Dev d)
     const cs_params* P = d.P;
     const unsigned ts = sc.ts;
     const int lane = threadIdx.x & 63;
-    const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const long long nwaves = ((long long)gridDim.x * blockDim.x) >> 6;
-    unsigned long long w_link = 0, w_cand = 0;
+    unsigned w_link = 0, w_cand = 0;  // per lane; summed per wave in 64 bits
     if (!P->do_places) return;
-    for (long long ch = wave; ch < nch; ch += nwaves) {
-        const Chunk e = d.segl[ch];
+    for (unsigned ch = grid_wave(); ch < nch; ch += grid_nwaves()) {
+        const Chunk e = chunk_at(d.segl, ch);
         const int rc = e.rc;
         const int j = e.first + lane;
-        if (j >= d.cI[rc]) continue;
+        if (j >= uni(d.cI[rc])) continue;
         const int r = rc / d.NC, c = rc % d.NC;
-        const uint64_t seed = d.seed[r];
+        const uint64_t seed = uni(d.seed[r]);
         const size_t rN = (size_t)r * d.N;
-        const size_t sbase = rN + d.cstart[c] + d.cS[rc] + d.cL[rc];  // slot of infectious index 0
+        const size_t sbase = rN + uni(d.cstart[c] + d.cS[rc] + d.cL[rc]);  // slot of infectious index 0
         const int i = d.perm[sbase + j];
         const size_t gi = rN + i;
         const int gen = d.itype[gi] / CS_INFECT_TYPE_MASK;
@@ -535,14 +555,17 @@ __global__ void __launch_bounds__(256) k_infect_places(Dev d)
                 double PI = sc.fp * sc.seas * place_inf(d, r, i, pt, ts);
                 const int mstart = d.pl_ms[pt][lk];
                 for (int pass = 0; pass < 2; pass++) {
+                    // Pc is both the capped place infectiousness and the
+                    // sampling probability: the reference keeps two copies
+                    // that differ only under digital contact tracing
+                    // (src/Sweep.cpp:425-447), which this path does not serve
                     int gsize, goff;
-                    double Pc, Ps;
+                    double Pc;
                     if (pass == 0) {
                         if (pt == P->hotel_place_type) continue;
                         const int g2 = d.pglinks[(size_t)i * CS_MAX_PLACE_TYPES + pt];
                         Pc = PI;
                         if (Pc > 1) Pc = 1;
-                        Ps = Pc;
                         const int gix = d.pl_gs0[pt][lk] + g2;
                         goff = d.pl_gst[pt][gix];
                         gsize = d.pl_gsz[pt][gix];
@@ -550,12 +573,11 @@ __global__ void __launch_bounds__(256) k_infect_places(Dev d)
                         PI *= P->place_type_prop_between_group_links[pt] * P->place_type_group_size_param1[pt] / ((double)d.pl_n[pt][lk]);
                         if (PI > 1) PI = 1;
                         Pc = PI;
-                        Ps = PI;
                         goff = 0;
                         gsize = d.pl_n[pt][lk];
                     }
                     Stream s(seed, cw0(ts, pass == 0 ? PS_PW_SAMPLE : PS_PB_SAMPLE), (uint32_t)i, (uint32_t)pt);
-                    int n = (Ps >= 1) ? gsize : binomial(s, gsize, Ps);
+                    int n = (Pc >= 1) ? gsize : binomial(s, gsize, Pc);
                     if (n <= 0) continue;
                     const bool all = n >= gsize;
                     const bool floyd = !all && n <= CS_FLOYD_MAX;
@@ -591,7 +613,7 @@ __global__ void __launch_bounds__(256) k_infect_places(Dev d)
                         if ((pt == P->care_home_place_type) && ((!care(d, i)) || (!care(d, m)))) PS *= P->care_home_worker_group_scaling;
                         const size_t gm = rN + m;
                         if (d.inf[gm] == CS_SUSCEPTIBLE && !host_absent(d, gm, ts)) {
-                            PS *= person_susc(d, r, m, ts, i) * Pc / Ps;
+                            PS *= person_susc(d, r, m, ts, i) * Pc / Pc;  // (susc * Pc) / Pc, rounded as the reference does
                             Stream s2(seed, cw0(ts, pass == 0 ? PS_PW_ACCEPT : PS_PB_ACCEPT), (uint32_t)i, ((uint32_t)pt << 24) | (uint32_t)cidx);
                             if ((PS == 1) || (s2.next() < PS)) {
                                 if (pass == 0)
@@ -606,11 +628,10 @@ __global__ void __launch_bounds__(256) k_infect_places(Dev d)
             }
         }
     }
-    w_link = wave_sum(w_link);
-    w_cand = wave_sum(w_cand);
-    if ((threadIdx.x & 63) == 0 && (w_link | w_cand)) {
-        atomicAdd(&d.wc->n_place_link, w_link);
-        atomicAdd(&d.wc->n_place_cand, w_cand);
+    const unsigned long long s_link = wave_sum(w_link), s_cand = wave_sum(w_cand);
+    if ((threadIdx.x & 63) == 0 && (s_link | s_cand)) {
+        atomicAdd(&d.wc->n_place_link, s_link);
+        atomicAdd(&d.wc->n_place_cand, s_cand);
     }
 }
 
@@ -703,19 +724,17 @@ __global__ void __launch_bounds__(256) k_spatial_draws(Dev d)
     const cs_params* P = d.P;
     const unsigned ts = sc.ts;
     const int lane = threadIdx.x & 63;
-    const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const long long nwaves = ((long long)gridDim.x * blockDim.x) >> 6;
     unsigned long long w_draw = 0, w_susc = 0;
-    for (long long ch = wave; ch < nch; ch += nwaves) {
-        const Chunk e = d.drawl[ch];
+    for (unsigned ch = grid_wave(); ch < nch; ch += grid_nwaves()) {
+        const Chunk e = chunk_at(d.drawl, ch);
         const size_t rc = (size_t)e.rc;
         const int k = e.first + lane;
-        if (k >= d.spK[rc]) continue;
-        const int r = (int)(rc / d.NC), c = (int)(rc % d.NC);
-        const int S = d.cS[rc], L = d.cL[rc], I = d.cI[rc];
+        if (k >= uni(d.spK[rc])) continue;
+        const int r = e.rc / d.NC, c = e.rc % d.NC;
+        const int S = uni(d.cS[rc]), L = uni(d.cL[rc]), I = uni(d.cI[rc]);
         const size_t rN = (size_t)r * d.N;
-        const size_t sbase = rN + d.cstart[c] + S + L;
-        const uint64_t seed = d.seed[r];
+        const size_t sbase = rN + uni(d.cstart[c]) + S + L;
+        const uint64_t seed = uni(d.seed[r]);
         Stream s(seed, cw0(ts, PS_SP_DRAW), (uint32_t)c, (uint32_t)k);
         int jj = (I == 1) ? 0 : select_infector(d.ci_buf + sbase, I, s.next());
         bool closed = (d.ci_buf[sbase + jj] < 0);

@@ -170,17 +170,15 @@ __global__ void __launch_bounds__(256) k_latent_scan(Dev d)
     const unsigned ts = d.sc->ts;
     const unsigned nch = min(d.wl[WL_SEG], d.seg_cap);
     const int lane = threadIdx.x & 63;
-    const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const long long nwaves = ((long long)gridDim.x * blockDim.x) >> 6;
     unsigned long long nt = 0;
-    for (long long ch = wave; ch < nch; ch += nwaves) {
-        const Chunk e = d.segl[ch];
+    for (unsigned ch = grid_wave(); ch < nch; ch += grid_nwaves()) {
+        const Chunk e = chunk_at(d.segl, ch);
         const int rc = e.rc;
         const int q = e.first + lane;
-        if (q >= d.cL[rc]) continue;
+        if (q >= uni(d.cL[rc])) continue;
         const int r = rc / d.NC, c = rc % d.NC;
         const size_t rN = (size_t)r * d.N;
-        const int p = d.perm[rN + d.cstart[c] + d.cS[rc] + q];
+        const int p = d.perm[rN + uni(d.cstart[c] + d.cS[rc]) + q];
         if (ts == d.t_lat[rN + p]) {
             do_incub(d, r, p, ts);
             batch_add(d, r, c, p);
@@ -256,18 +254,16 @@ __global__ void __launch_bounds__(256) k_infectious_scan(Dev d)
     const cs_params* P = d.P;
     const unsigned nch = min(d.wl[WL_SEG], d.seg_cap);
     const int lane = threadIdx.x & 63;
-    const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const long long nwaves = ((long long)gridDim.x * blockDim.x) >> 6;
     unsigned long long nt = 0;
-    for (long long ch = wave; ch < nch; ch += nwaves) {
-        const Chunk e = d.segl[ch];
+    for (unsigned ch = grid_wave(); ch < nch; ch += grid_nwaves()) {
+        const Chunk e = chunk_at(d.segl, ch);
         const int rc = e.rc;
         const int q = e.first + lane;
-        if (q >= d.cI[rc]) continue;
+        if (q >= uni(d.cI[rc])) continue;
         const int r = rc / d.NC, c = rc % d.NC;
         const size_t rN = (size_t)r * d.N;
         Counters* cn = d.cnt + r;
-        const int p = d.perm[rN + d.cstart[c] + d.cS[rc] + d.cL[rc] + q];
+        const int p = d.perm[rN + uni(d.cstart[c] + d.cS[rc] + d.cL[rc]) + q];
         const size_t gp = rN + p;
         if (d.t_gate[gp] > ts) continue;  // every timer of p lies after this step
         int ag = age_group(d, p);

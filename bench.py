@@ -125,6 +125,25 @@ def cpu_baseline(max_seconds=25.0):
                       "%d realisation(s), day-loop wall %.2f s from stderr markers, /c:%d" % (nr, loop, threads)}
 
 
+def copy_probe(nbytes):
+    """One device-to-device hipMemcpy of nbytes (the HIP runtime's copy
+    kernel, __amd_rocclr_copyBuffer): a transfer of known size for scaling
+    the profiler's FETCH_SIZE / WRITE_SIZE counters."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    for p in (a, b):
+        if hip.hipMalloc(ctypes.byref(p), ctypes.c_size_t(nbytes)):
+            raise RuntimeError("copy probe: hipMalloc of %d bytes failed" % nbytes)
+    rc = hip.hipMemset(a, 1, ctypes.c_size_t(nbytes)) or hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3)  # 3: device to device
+    rc = rc or hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    if rc:
+        raise RuntimeError("copy probe: HIP error %d" % rc)
+    return nbytes
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -143,8 +162,10 @@ def main():
     ap.add_argument("--calibration", action="store_true", help="measure emulator-guided vs plain calibration instead")
     ap.add_argument("--summarise-pmc", metavar="DIR", help="print per-kernel sums of a rocprofv3 --pmc output "
                                                             "directory (counter_collection.csv) and exit")
-    ap.add_argument("--pmc-last", type=int, default=0, help="with --summarise-pmc: also the means over each "
-                                                             "kernel's last N dispatches")
+    ap.add_argument("--summarise-trace", metavar="DIR", help="print a per-kernel table of a rocprofv3 --kernel-trace "
+                                                              "output directory (kernel_trace.csv) and exit")
+    ap.add_argument("--pmc-last", type=int, default=0, help="with --summarise-pmc or --summarise-trace: also the "
+                                                             "means over each kernel's last N dispatches")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="before the run, copy this many MiB device-to-device (a copy of known size for "
                          "scaling the profiler's byte counters)")
@@ -157,6 +178,8 @@ def main():
         sys.exit(2)
     if args.summarise_pmc:
         return summarise_pmc(args.summarise_pmc, args.pmc_last)
+    if args.summarise_trace:
+        return summarise_trace(args.summarise_trace, args.pmc_last)
     if args.calibration:
         return calibration(args)
     import importlib
@@ -187,20 +210,15 @@ def main():
         tune = eng.autotune(float(args.warmup_days))
         log("autotune at day %d: %s" % (args.warmup_days, json.dumps(tune)))
     eng.sync()
-    try:
-        import torch
-        if torch.cuda.is_available():
-            torch.cuda.synchronize()
-    except Exception:
-        pass
+    if not args.copy_probe_mb:  # counter runs keep torch out of the profiled process
+        try:
+            import torch
+            if torch.cuda.is_available():
+                torch.cuda.synchronize()
+        except Exception:
+            pass
     if args.copy_probe_mb:
-        import torch
-        a = torch.ones(args.copy_probe_mb << 20, dtype=torch.uint8, device="cuda")
-        b = torch.empty_like(a)
-        b.copy_(a)
-        torch.cuda.synchronize()
-        log("copy probe: %d bytes" % a.numel())
-        del a, b
+        log("copy probe: %d bytes" % copy_probe(args.copy_probe_mb << 20))
     chunk = max(1, args.chunk_days) * tspd
 
     def one_step(k, label):
@@ -311,11 +329,44 @@ def summarise_pmc(d, last=0):
         for c in sorted(per):
             ids = sorted(per[c])
             tot = sum(per[c].values())
-            txt = "%s total %.6g mean %.6g" % (c, tot, tot / len(ids))
+            txt = "%s total %.6g mean %.6g max %.6g" % (c, tot, tot / len(ids), max(per[c].values()))
             if last and len(ids) >= last:
                 txt += " mean_last%d %.6g" % (last, sum(per[c][i] for i in ids[-last:]) / last)
             parts.append(txt)
         print("%s | dispatches %d | %s" % (k[:80], len(per[min(per)]), " | ".join(parts)))
+
+
+def summarise_trace(d, last=0):
+    """Per-kernel dispatch count, total and mean duration from a rocprofv3
+    --kernel-trace CSV output directory (kernel_trace.csv), as a markdown
+    table; with last > 0 also the mean over each kernel's last `last`
+    dispatches (for a kernel launched once per timestep: the profiled pass of
+    a bench run), with the LDS and scratch sizes the trace records."""
+    import csv
+    import glob
+    runs = {}
+    for f in glob.glob(os.path.join(d, "**", "*kernel_trace.csv"), recursive=True):
+        with open(f) as fh:
+            for row in csv.DictReader(fh):
+                k = row["Kernel_Name"]
+                dur = int(row["End_Timestamp"]) - int(row["Start_Timestamp"])
+                runs.setdefault(k, []).append((int(row["Dispatch_Id"]), dur, row["LDS_Block_Size"], row["Scratch_Size"]))
+    tot_all = sum(x[1] for v in runs.values() for x in v)
+    head = "| kernel | dispatches | total ms | share | mean us |"
+    if last:
+        head += " mean us, last %d | LDS B | scratch B/lane |" % last
+    print(head)
+    print("|" + "---|" * (head.count("|") - 1))
+    for k in sorted(runs, key=lambda k: -sum(x[1] for x in runs[k])):
+        v = sorted(runs[k])
+        tot = sum(x[1] for x in v)
+        line = "| %s | %d | %.1f | %.2f%% | %.1f |" % (k.split("(")[0], len(v), tot / 1e6, 100.0 * tot / tot_all,
+                                                      tot / len(v) / 1e3)
+        if last:
+            tail = v[-last:]
+            line += " %s | %s | %s |" % ("%.1f" % (sum(x[1] for x in tail) / len(tail) / 1e3)
+                                       if len(v) >= last else "-", v[-1][2], v[-1][3])
+        print(line)
 
 
 def calibration(args):

@@ -9,12 +9,18 @@ binomial likelihood (src/CovidSim.cpp:3691-3852).  Here every candidate
 of a GPU ensemble (16 seeds, common random numbers across candidates), and
 two searches are compared:
 
-  plain      pattern search in (R0, log10 seeding scale), four candidates per
-             64-realisation launch, steps halved until the R0 step is 0.01;
-  emulator   the trained emulator scores a dense grid of candidates in one
-             batched forward pass; the best four are verified by one
-             simulator launch and the same pattern search then refines the
-             best verified candidate from small steps.
+  plain      a coarse grid over the whole (R0, log10 seeding scale) box,
+             four candidates per 64-realisation launch, then a quadratic
+             response-surface refinement (stencils of 11 candidates) from the
+             best grid point;
+  emulator   the trained emulator scores a dense grid in one batched forward
+             pass; its best candidate is verified and refined by the
+             simulator with the same quadratic refinement.
+
+The likelihood has a long ridge (more seeds with a lower R0 give a similar
+death curve), so a local axis search from a default start stops early on
+the ridge (calibrate_pattern shows it); the plain calibration needs its
+global stage, and the emulator replaces exactly that stage.
 
 Per-realisation parameters: LocalBeta from R0 through the reproduction
 numbers InitTransmissionCoeffs gives for the population
@@ -154,15 +160,80 @@ def pattern_search(score, start, step, min_step=0.01, max_evals=400, fstart=None
     return x, fx, evals
 
 
-def calibrate_plain(sim, target_deaths, seeds, start=(2.4, 0.0), step=(0.4, 0.4)):
+# Local refinement shared by both calibrations: a quadratic response surface
+# fitted to the simulator's log-likelihood on a stencil around the incumbent.
+# The likelihood has a long diagonal ridge in (R0, log10 scale) (more seeds
+# and a lower R0 give a similar death curve), so axis moves crawl along it;
+# the quadratic's cross term follows the ridge, and its argmax is less
+# sensitive to the noise of a 16-seed mean than the best evaluated point.
+STEP = np.array([0.05, 0.15])   # stencil spacing in (R0, log10 scale)
+STENCIL = np.array([(i, j) for i in (-1, 0, 1) for j in (-1, 0, 1)] + [(-0.5, 0.5), (0.5, -0.5)], np.float64)
+
+
+def quad_argmax(z, f):
+    """Least-squares quadratic through f at stencil points z [n, 2]; its
+    maximiser (stencil units) when the fit is concave, else None."""
+    x, y = z[:, 0], z[:, 1]
+    a = np.column_stack([np.ones_like(x), x, y, x * x, y * y, x * y])
+    c = np.linalg.lstsq(a, f, rcond=None)[0]
+    hess = np.array([[2 * c[3], c[5]], [c[5], 2 * c[4]]])
+    if np.any(np.linalg.eigvalsh(hess) >= 0):
+        return None
+    return np.linalg.solve(hess, -c[1:3])
+
+
+def polish(score, center, max_iter=6):
+    """Refine from center: evaluate the stencil (one batch of simulator
+    launches), fit the quadratic, and stop when its maximiser lies inside the
+    stencil's inner half; else move there (at most 1.5 steps per axis) or, for
+    a non-concave fit, to the best stencil point, and repeat.
+    Returns (theta, loglik at the nearest evaluated point, stencils used)."""
+    box = np.array([R0_RANGE, LOGS_RANGE])
+    c = np.asarray(center, np.float64)
+    for it in range(1, max_iter + 1):
+        pts = np.clip(c + STENCIL * STEP, box[:, 0], box[:, 1])
+        f = score(pts)
+        zs = (pts - c) / STEP
+        zmax = quad_argmax(zs, f)
+        if zmax is not None and np.all(np.abs(zmax) <= 0.5):
+            th = np.clip(c + zmax * STEP, box[:, 0], box[:, 1])
+            return th, float(np.max(f)), it
+        c = pts[int(np.argmax(f))] if zmax is None else np.clip(c + np.clip(zmax, -1.5, 1.5) * STEP, box[:, 0], box[:, 1])
+    return c, float(np.max(f)), max_iter
+
+
+def _score(sim, target_deaths, seeds):
+    return lambda th: nb_loglik(target_deaths, sim.run(th, seeds)[1])
+
+
+def calibrate_plain(sim, target_deaths, seeds, grid=(13, 8)):
+    """Simulator only: a coarse grid over the whole box (R0 step 0.1, log10
+    scale step 0.2), then the quadratic refinement from its best point."""
     t0 = time.perf_counter()
     l0 = sim.launches
-    x, fx, n = pattern_search(lambda th: nb_loglik(target_deaths, sim.run(th, seeds)[1]), start, step)
+    score = _score(sim, target_deaths, seeds)
+    th = np.array([(a, b) for a in np.linspace(*R0_RANGE, grid[0]) for b in np.linspace(*LOGS_RANGE, grid[1])])
+    f = score(th)
+    x, fx, it = polish(score, th[int(np.argmax(f))])
+    return {"theta": x.tolist(), "loglik": fx, "candidates": len(th) + it * len(STENCIL), "stencils": it,
+            "launches": sim.launches - l0, "seconds": time.perf_counter() - t0}
+
+
+def calibrate_pattern(sim, target_deaths, seeds, start=(2.4, 0.0), step=(0.4, 0.4)):
+    """Simulator only, local: the axis pattern search from a default start
+    (stops on the likelihood ridge; kept to show why a global stage is needed)."""
+    t0 = time.perf_counter()
+    l0 = sim.launches
+    x, fx, n = pattern_search(_score(sim, target_deaths, seeds), start, step)
     return {"theta": x.tolist(), "loglik": float(fx), "candidates": n, "launches": sim.launches - l0,
             "seconds": time.perf_counter() - t0}
 
 
 def calibrate_emulated(sim, emu, target_deaths, seeds, grid=(121, 57)):
+    """The emulator scores a dense grid (R0 step 0.01) in one batched forward
+    pass and proposes its best candidate; the simulator verifies it through the
+    same quadratic refinement as the plain calibration, which also serves as
+    the fallback when the proposal is off (more stencils)."""
     t0 = time.perf_counter()
     l0 = sim.launches
     r0 = np.linspace(*R0_RANGE, grid[0])
@@ -170,14 +241,12 @@ def calibrate_emulated(sim, emu, target_deaths, seeds, grid=(121, 57)):
     th = np.array([(a, b) for a in r0 for b in ls])
     _, d_pred = to_curves(emu.forward(encode(th)))
     ll = nb_loglik(target_deaths, d_pred)
-    best = th[np.argsort(-ll)[:4]]
+    proposal = th[int(np.argmax(ll))]
     t_emul = time.perf_counter() - t0
-    f = nb_loglik(target_deaths, sim.run(best, seeds)[1])
-    j = int(np.argmax(f))
-    x, fx, n = pattern_search(lambda c: nb_loglik(target_deaths, sim.run(c, seeds)[1]), best[j],
-                              step=(r0[1] - r0[0], ls[1] - ls[0]), fstart=f[j])
-    return {"theta": x.tolist(), "loglik": float(fx), "candidates": n + 4, "launches": sim.launches - l0,
-            "seconds": time.perf_counter() - t0, "emulator_seconds": t_emul}
+    x, fx, it = polish(_score(sim, target_deaths, seeds), proposal)
+    return {"theta": x.tolist(), "loglik": fx, "proposal": proposal.tolist(), "candidates": it * len(STENCIL),
+            "stencils": it, "launches": sim.launches - l0, "seconds": time.perf_counter() - t0,
+            "emulator_seconds": t_emul}
 
 
 def training_set(sim, n_theta, seeds, rng):

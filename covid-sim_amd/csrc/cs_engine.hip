@@ -556,17 +556,21 @@ int cs_set_population(cs_engine* e, const cs_population* pop)
     }
     int rc;
     auto& L = e->pop_allocs;
-    if ((rc = dupload(e, L, &d.hh, pop->hh, N))) return rc;
+    {
+        std::vector<cs::PStat> ps(N);
+        for (int i = 0; i < N; i++) ps[i] = cs::PStat{pop->hh[i], pop->age[i], pop->flags[i], 0};
+        std::vector<cs::HRec> hr(e->NH);
+        for (int h = 0; h < e->NH; h++) hr[h] = cs::HRec{pop->hh_first[h], pop->hh_nh[h], pop->hh_nhr[h]};
+        if ((rc = dupload(e, L, &d.pst, ps.data(), N))) return rc;
+        if ((rc = dupload(e, L, &d.hrec, hr.data(), e->NH))) return rc;
+        d.hh.base = d.age.base = d.pflags.base = d.pst;
+        d.hfirst.base = d.hnh.base = d.hnhr.base = d.hrec;
+    }
     if ((rc = dupload(e, L, &d.pc, pc.data(), N))) return rc;
     if ((rc = dupload(e, L, &d.mcell, pop->mcell, N))) return rc;
-    if ((rc = dupload(e, L, &d.age, pop->age, N))) return rc;
-    if ((rc = dupload(e, L, &d.pflags, pop->flags, N))) return rc;
     if ((rc = dupload(e, L, &d.padunit, padunit.data(), N))) return rc;
     if ((rc = dupload(e, L, &d.plinks, pop->place_links, (size_t)N * CS_MAX_PLACE_TYPES))) return rc;
     if ((rc = dupload(e, L, &d.pglinks, pop->place_group_links, (size_t)N * CS_MAX_PLACE_TYPES))) return rc;
-    if ((rc = dupload(e, L, &d.hfirst, pop->hh_first, e->NH))) return rc;
-    if ((rc = dupload(e, L, &d.hnh, pop->hh_nh, e->NH))) return rc;
-    if ((rc = dupload(e, L, &d.hnhr, pop->hh_nhr, e->NH))) return rc;
     if ((rc = dupload(e, L, &d.hloc, pop->hh_loc, (size_t)e->NH * 2))) return rc;
     if ((rc = dupload(e, L, &d.cstart, pop->cell_member_start, NC + 1))) return rc;
     if ((rc = dupload(e, L, &d.cn, pop->cell_n, NC))) return rc;
@@ -642,10 +646,13 @@ int cs_alloc_realisations(cs_engine* e, int n_real, const uint64_t* seeds)
     if ((rc = dalloc(e, A, &sd, R))) return rc;
     HIPCHK(e, hipMemcpy(sd, seeds, R * 8, hipMemcpyHostToDevice));
     d.seed = sd;
-    AL(inf, RN); AL(sev_cur, RN); AL(sev_final, RN); AL(bits, RN); AL(qc, RN); AL(dct, RN);
-    AL(listpos, RN); AL(infector, RN); AL(itype, RN); AL(infness, RN); AL(susc, RN); AL(prob_care, RN);
-    AL(t_inf, RN); AL(t_lat, RN); AL(t_rod, RN); AL(t_sari, RN); AL(t_crit, RN); AL(t_step, RN); AL(t_det, RN);
-    AL(t_as, RN); AL(t_ae, RN); AL(t_ts, RN); AL(t_te, RN); AL(t_vs, RN); AL(t_is, RN); AL(t_qs, RN); AL(t_gate, RN);
+    AL(hot, RN);
+    d.inf.base = d.hot; d.itype.base = d.hot; d.infness.base = d.hot;
+    d.t_lat.base = d.hot; d.t_as.base = d.hot; d.t_ae.base = d.hot; d.t_gate.base = d.hot;
+    AL(sev_cur, RN); AL(sev_final, RN); AL(bits, RN); AL(qc, RN); AL(dct, RN);
+    AL(listpos, RN); AL(infector, RN); AL(susc, RN); AL(prob_care, RN);
+    AL(t_inf, RN); AL(t_rod, RN); AL(t_sari, RN); AL(t_crit, RN); AL(t_step, RN); AL(t_det, RN);
+    AL(t_ts, RN); AL(t_te, RN); AL(t_vs, RN); AL(t_is, RN); AL(t_qs, RN);
     AL(cS, RC); AL(cL, RC); AL(cI, RC); AL(cR, RC); AL(cD, RC); AL(cumTC, RC); AL(S0, RC);
     AL(tot_prob, RC); AL(cum_trans, RC * NC); AL(invcdf, RC * 1025); AL(perm, RN);
     AL(pl_cs, PL); AL(pl_ce, PL); AL(m_socdist, R * NM); AL(m_moverest, R * NM); AL(m_trig, R * NM * 6);
@@ -728,14 +735,29 @@ int cs_load_state(cs_engine* e, int real, const cs_state* s)
     Dev& d = e->d;
     const size_t N = e->N, NC = e->NC, o = (size_t)real * N, oc = (size_t)real * NC;
     e->ts_fresh = true;
-    HIPCHK(e, hipMemset(d.t_gate + o, 0, N * sizeof(uint16_t)));  // loaded timers: look at every one
+    {
+        // the packed fields: read the records, overwrite the fields the state gives, write back
+        std::vector<cs::Hot> h(N);
+        HIPCHK(e, hipMemcpy(h.data(), d.hot + o, N * sizeof(cs::Hot), hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < N; i++) {
+            cs::Hot& x = h[i];
+            if (s->inf) x.inf = s->inf[i];
+            if (s->infect_type) x.itype = s->infect_type[i];
+            if (s->infectiousness) x.infness = s->infectiousness[i];
+            if (s->latent_time) x.t_lat = s->latent_time[i];
+            if (s->absent_start_time) x.t_as = s->absent_start_time[i];
+            if (s->absent_stop_time) x.t_ae = s->absent_stop_time[i];
+            x.t_gate = 0;  // loaded timers: look at every one
+        }
+        HIPCHK(e, hipMemcpy(d.hot + o, h.data(), N * sizeof(cs::Hot), hipMemcpyHostToDevice));
+    }
 #define UP(dst, src, off, n) if (src) HIPCHK(e, hipMemcpy(d.dst + (off), src, (n) * sizeof(*d.dst), hipMemcpyHostToDevice))
-    UP(inf, s->inf, o, N); UP(sev_cur, s->severity_current, o, N); UP(sev_final, s->severity_final, o, N);
-    UP(bits, s->bits, o, N); UP(listpos, s->listpos, o, N); UP(infector, s->infector, o, N); UP(itype, s->infect_type, o, N);
-    UP(infness, s->infectiousness, o, N); UP(susc, s->susc, o, N); UP(prob_care, s->prob_care, o, N);
-    UP(t_inf, s->infection_time, o, N); UP(t_lat, s->latent_time, o, N); UP(t_rod, s->recovery_or_death_time, o, N);
+    UP(sev_cur, s->severity_current, o, N); UP(sev_final, s->severity_final, o, N);
+    UP(bits, s->bits, o, N); UP(listpos, s->listpos, o, N); UP(infector, s->infector, o, N);
+    UP(susc, s->susc, o, N); UP(prob_care, s->prob_care, o, N);
+    UP(t_inf, s->infection_time, o, N); UP(t_rod, s->recovery_or_death_time, o, N);
     UP(t_sari, s->sari_time, o, N); UP(t_crit, s->critical_time, o, N); UP(t_step, s->stepdown_time, o, N);
-    UP(t_det, s->detected_time, o, N); UP(t_as, s->absent_start_time, o, N); UP(t_ae, s->absent_stop_time, o, N);
+    UP(t_det, s->detected_time, o, N);
     UP(t_ts, s->treat_start_time, o, N); UP(t_te, s->treat_stop_time, o, N); UP(t_vs, s->vacc_start_time, o, N);
     UP(t_is, s->isolation_start_time, o, N); UP(t_qs, s->quar_start_time, o, N); UP(qc, s->quar_comply, o, N);
     UP(dct, s->dct_traced, o, N);
@@ -775,13 +797,26 @@ int cs_store_state(cs_engine* e, int real, cs_state* s)
     HIPCHK(e, hipStreamSynchronize(e->stream));
     Dev& d = e->d;
     const size_t N = e->N, NC = e->NC, o = (size_t)real * N, oc = (size_t)real * NC;
+    {
+        std::vector<cs::Hot> h(N);
+        HIPCHK(e, hipMemcpy(h.data(), d.hot + o, N * sizeof(cs::Hot), hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < N; i++) {
+            const cs::Hot& x = h[i];
+            if (s->inf) s->inf[i] = x.inf;
+            if (s->infect_type) s->infect_type[i] = x.itype;
+            if (s->infectiousness) s->infectiousness[i] = x.infness;
+            if (s->latent_time) s->latent_time[i] = x.t_lat;
+            if (s->absent_start_time) s->absent_start_time[i] = x.t_as;
+            if (s->absent_stop_time) s->absent_stop_time[i] = x.t_ae;
+        }
+    }
 #define DN(src, dst, off, n) if (dst) HIPCHK(e, hipMemcpy(dst, d.src + (off), (n) * sizeof(*d.src), hipMemcpyDeviceToHost))
-    DN(inf, s->inf, o, N); DN(sev_cur, s->severity_current, o, N); DN(sev_final, s->severity_final, o, N);
-    DN(bits, s->bits, o, N); DN(listpos, s->listpos, o, N); DN(infector, s->infector, o, N); DN(itype, s->infect_type, o, N);
-    DN(infness, s->infectiousness, o, N); DN(susc, s->susc, o, N); DN(prob_care, s->prob_care, o, N);
-    DN(t_inf, s->infection_time, o, N); DN(t_lat, s->latent_time, o, N); DN(t_rod, s->recovery_or_death_time, o, N);
+    DN(sev_cur, s->severity_current, o, N); DN(sev_final, s->severity_final, o, N);
+    DN(bits, s->bits, o, N); DN(listpos, s->listpos, o, N); DN(infector, s->infector, o, N);
+    DN(susc, s->susc, o, N); DN(prob_care, s->prob_care, o, N);
+    DN(t_inf, s->infection_time, o, N); DN(t_rod, s->recovery_or_death_time, o, N);
     DN(t_sari, s->sari_time, o, N); DN(t_crit, s->critical_time, o, N); DN(t_step, s->stepdown_time, o, N);
-    DN(t_det, s->detected_time, o, N); DN(t_as, s->absent_start_time, o, N); DN(t_ae, s->absent_stop_time, o, N);
+    DN(t_det, s->detected_time, o, N);
     DN(t_ts, s->treat_start_time, o, N); DN(t_te, s->treat_stop_time, o, N); DN(t_vs, s->vacc_start_time, o, N);
     DN(t_is, s->isolation_start_time, o, N); DN(t_qs, s->quar_start_time, o, N); DN(qc, s->quar_comply, o, N);
     DN(dct, s->dct_traced, o, N);

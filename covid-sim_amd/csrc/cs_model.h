@@ -7,9 +7,18 @@
 // every dynamic per-person array is [n_real][n_people] (realisation-major),
 // per-cell arrays [n_real][n_cells], per-place arrays [n_real][n_places].
 // A global person index gp = r * n_people + p addresses the dynamic arrays.
+//
+// The fields the infection kernels read together are packed into records,
+// so that a scattered visit to one person costs one cache line instead of
+// one line per field: Hot per (realisation, person), PStat per person, HRec
+// per household.  Each packed field keeps an array-like accessor (Fld), so
+// d.inf[gp] reads the inf field of record gp.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <stddef.h>
 #include <stdint.h>
+
+#include <type_traits>
 
 #include "../../include/covidsim_hip.h"
 #include "cs_math.h"
@@ -51,16 +60,55 @@ struct Work {  // event tallies for the algorithmic-bytes figure
 
 enum { FLAG_OVERFLOW = 1, FLAG_RANGE = 2, FLAG_BATCH = 4, FLAG_STUCK = 8 };
 
+struct Hot {  // per (realisation, person): 16 bytes, one vector load
+    int8_t inf;       // InfStat code
+    uint8_t pad;
+    int16_t itype;    // infect_type
+    uint16_t t_lat, t_as, t_ae;  // latent time, absence start and end
+    uint16_t t_gate;  // no timer of an infectious person fires before this step
+    float infness;    // infectiousness (sign: symptomatic)
+};
+struct PStat {  // per person, shared by all realisations
+    int32_t hh;
+    uint8_t age, pflags;
+    uint16_t pad;
+};
+struct HRec {  // per household
+    int32_t hfirst;
+    uint16_t hnh, hnhr;
+};
+static_assert(sizeof(Hot) == 16 && sizeof(PStat) == 8 && sizeof(HRec) == 8, "record sizes");
+
+// one field of an array of records, indexed like a plain array
+template <typename Rec, typename T, size_t OFF>
+struct Fld {
+    Rec* base;
+    __host__ __device__ __forceinline__ T& operator[](size_t i) const
+    {
+        using B = typename std::conditional<std::is_const<Rec>::value, const char, char>::type;
+        return *reinterpret_cast<T*>(reinterpret_cast<B*>(base + i) + OFF);
+    }
+};
+#define CS_FLD(Rec, f) Fld<Rec, decltype(((Rec*)nullptr)->f), offsetof(Rec, f)>
+#define CS_CFLD(Rec, f) Fld<const Rec, const decltype(((Rec*)nullptr)->f), offsetof(Rec, f)>
+
 struct Dev {
     int N, NH, NC, NM, NPT, R;
     const cs_params* P;
     const double* kl;
     const double* kh;
     // population (static)
-    const int32_t *hh, *pc, *mcell, *plinks, *hfirst, *cstart, *cn, *cidx, *mstart, *mmem, *cmem;
-    const uint8_t *age, *pflags;
+    const PStat* pst;     // [N]
+    const HRec* hrec;     // [NH]
+    CS_CFLD(PStat, hh) hh;
+    CS_CFLD(PStat, age) age;
+    CS_CFLD(PStat, pflags) pflags;
+    CS_CFLD(HRec, hfirst) hfirst;
+    CS_CFLD(HRec, hnh) hnh;
+    CS_CFLD(HRec, hnhr) hnhr;
+    const int32_t *pc, *mcell, *plinks, *cstart, *cn, *cidx, *mstart, *mmem, *cmem;
     const int32_t* padunit;  // administrative unit of each person's microcell
-    const uint16_t *pglinks, *hnh, *hnhr;
+    const uint16_t* pglinks;
     const float* hloc;
     int np[CS_MAX_PLACE_TYPES];
     int np_off[CS_MAX_PLACE_TYPES + 1];  // offsets of each type in the per-realisation place arrays
@@ -71,14 +119,19 @@ struct Dev {
     const uint64_t* seed;
     const double* rbeta;     // [R] per-realisation LocalBeta, or null (cs_params.local_beta)
     const int32_t* rseed_n;  // [R] per-realisation seed infections, or null
-    int8_t* inf;
+    Hot* hot;             // [R][N]
+    CS_FLD(Hot, inf) inf;
+    CS_FLD(Hot, itype) itype;
+    CS_FLD(Hot, infness) infness;
+    CS_FLD(Hot, t_lat) t_lat;
+    CS_FLD(Hot, t_as) t_as;
+    CS_FLD(Hot, t_ae) t_ae;
+    CS_FLD(Hot, t_gate) t_gate;
     uint8_t *sev_cur, *sev_final, *bits, *qc, *dct;
     int32_t *listpos, *infector;
-    int16_t* itype;
-    float *infness, *susc, *prob_care;
-    uint16_t *t_inf, *t_lat, *t_rod, *t_sari, *t_crit, *t_step, *t_det, *t_as, *t_ae;
+    float *susc, *prob_care;
+    uint16_t *t_inf, *t_rod, *t_sari, *t_crit, *t_step, *t_det;
     uint16_t *t_ts, *t_te, *t_vs, *t_is, *t_qs;
-    uint16_t* t_gate;     // [R][N] no timer of the infectious person fires before this step
     int32_t *cS, *cL, *cI, *cR, *cD, *cumTC, *S0;
     float* tot_prob;
     float* cum_trans;

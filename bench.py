@@ -166,6 +166,9 @@ def main():
                                                               "output directory (kernel_trace.csv) and exit")
     ap.add_argument("--pmc-last", type=int, default=0, help="with --summarise-pmc or --summarise-trace: also the "
                                                              "means over each kernel's last N dispatches")
+    ap.add_argument("--write-traffic", nargs=2, metavar=("FETCH_SUMMARY", "WRITE_SUMMARY"),
+                    help="write profiles/traffic.json from two --summarise-pmc outputs of this workload (with "
+                         "--pmc-last and --copy-probe-mb as in the profiled runs) and exit")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="before the run, copy this many MiB device-to-device (a copy of known size for "
                          "scaling the profiler's byte counters)")
@@ -180,6 +183,11 @@ def main():
         return summarise_pmc(args.summarise_pmc, args.pmc_last)
     if args.summarise_trace:
         return summarise_trace(args.summarise_trace, args.pmc_last)
+    if args.write_traffic:
+        return write_traffic(args.write_traffic[0], args.write_traffic[1], args.pmc_last,
+                             os.path.join(ROOT, "profiles", "traffic.json"),
+                             {"realisations": args.realisations, "persons": args.people, "days": args.days,
+                              "copy_probe_bytes": args.copy_probe_mb << 20})
     if args.calibration:
         return calibration(args)
     import importlib
@@ -307,6 +315,44 @@ def measured_traffic(kernel, args):
     return None if k is None else k.get("hbm_bytes_per_launch")
 
 
+def write_traffic(fetch_summary, write_summary, last, out, workload):
+    """profiles/traffic.json from two --summarise-pmc outputs (FETCH_SIZE and
+    WRITE_SIZE, KiB per dispatch): each counter is scaled by bytes moved /
+    counter reading of the largest __amd_rocclr_copyBuffer dispatch (the
+    --copy-probe-mb copy), and a kernel's HBM bytes per launch are the scaled
+    means over its last `last` dispatches."""
+    def parse(path, counter):
+        rows = {}
+        with open(path) as f:
+            for line in f:
+                parts = [x.strip() for x in line.split("|")]
+                name = parts[0].split("(")[0]
+                m = re.search(counter + r" total \S+ mean \S+ max (\S+)(?: mean_last\d+ (\S+))?", line)
+                if m:
+                    rows[name] = (float(m.group(1)), float(m.group(2)) if m.group(2) else None)
+        return rows
+    fr, wr = parse(fetch_summary, "FETCH_SIZE"), parse(write_summary, "WRITE_SIZE")
+    probe = workload.pop("copy_probe_bytes")
+    f_fetch = probe / (fr["__amd_rocclr_copyBuffer"][0] * 1024.0)
+    f_write = probe / (wr["__amd_rocclr_copyBuffer"][0] * 1024.0)
+    kernels = {}
+    for name in fr:
+        if not name.startswith("cs::k_") or name not in wr or fr[name][1] is None:
+            continue
+        fb, wb = fr[name][1] * 1024.0 * f_fetch, wr[name][1] * 1024.0 * f_write
+        kernels[name[len("cs::k_"):]] = {"hbm_bytes_per_launch": round(fb + wb), "fetch_bytes": round(fb),
+                                         "write_bytes": round(wb), "fetch_kib_raw": fr[name][1],
+                                         "write_kib_raw": wr[name][1], "dispatches_averaged": last}
+    doc = {"workload": workload, "copy_probe_bytes": probe,
+           "scale": {"fetch": f_fetch, "write": f_write,
+                     "how": "bytes of one device-to-device copy / the counter's KiB x 1024 for that copy"},
+           "kernels": kernels}
+    with open(out, "w") as f:
+        json.dump(doc, f, indent=1)
+        f.write("\n")
+    print(json.dumps(doc, indent=1))
+
+
 def summarise_pmc(d, last=0):
     """Per-kernel totals and per-dispatch means of every counter in a
     rocprofv3 --pmc CSV output directory; with last > 0 also the means over
@@ -399,6 +445,8 @@ def calibration(args):
     log("plain: %s" % json.dumps(plain))
     emul = cal.calibrate_emulated(sim, emu, target, seeds)
     log("emulator-guided: %s" % json.dumps(emul))
+    local = cal.calibrate_pattern(sim, target, seeds)
+    log("local pattern search alone: %s" % json.dumps(local))
     out = {
         "metric": "calibration wall-time, emulator-guided vs plain (speed-up)",
         "value": plain["seconds"] / emul["seconds"],
@@ -413,9 +461,10 @@ def calibration(args):
                                % (sim.pop.n_people, hidden[0], hidden[1]),
                    "emulator": "10-512-512-730 GELU MLP, BF16 MFMA kernels, FP32 accumulation and Adam",
                    "training_candidates": args.cal_train, "training_seeds": 8},
-        "plain": plain, "emulator_guided": emul,
+        "plain": plain, "emulator_guided": emul, "local_pattern_search": local,
         "hidden_theta": hidden.tolist(),
         "fitted_r0_difference": abs(plain["theta"][0] - emul["theta"][0]),
+        "launch_ratio": plain["launches"] / max(emul["launches"], 1),
         "training_data_seconds": t_data, "training_seconds": t_train,
         "emulator_heldout_rel_l2": err,
     }

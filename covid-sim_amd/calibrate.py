@@ -10,12 +10,12 @@ of a GPU ensemble (16 seeds, common random numbers across candidates), and
 two searches are compared:
 
   plain      a coarse grid over the whole (R0, log10 seeding scale) box,
-             four candidates per 64-realisation launch, then a quadratic
-             response-surface refinement (stencils of 11 candidates) from the
-             best grid point;
+             four candidates per 64-realisation launch, then a quadratic fit
+             over a design of 15 candidates sheared along the likelihood's
+             crest (refine), started from the grid's crest;
   emulator   the trained emulator scores a dense grid in one batched forward
-             pass; its best candidate is verified and refined by the
-             simulator with the same quadratic refinement.
+             pass; the simulator verifies and refines the crest it proposes
+             with the same refinement.
 
 The likelihood has a long ridge (more seeds with a lower R0 give a similar
 death curve), so a local axis search from a default start stops early on
@@ -160,63 +160,103 @@ def pattern_search(score, start, step, min_step=0.01, max_evals=400, fstart=None
     return x, fx, evals
 
 
-# Local refinement shared by both calibrations: a quadratic response surface
-# fitted to the simulator's log-likelihood on a stencil around the incumbent.
-# The likelihood has a long diagonal ridge in (R0, log10 scale) (more seeds
-# and a lower R0 give a similar death curve), so axis moves crawl along it;
-# the quadratic's cross term follows the ridge, and its argmax is less
-# sensitive to the noise of a 16-seed mean than the best evaluated point.
-STEP = np.array([0.05, 0.15])   # stencil spacing in (R0, log10 scale)
-STENCIL = np.array([(i, j) for i in (-1, 0, 1) for j in (-1, 0, 1)] + [(-0.5, 0.5), (0.5, -0.5)], np.float64)
+# Local refinement shared by both calibrations.  The log-likelihood has a
+# long diagonal ridge in (R0, log10 scale): more seeds with a lower R0 give a
+# similar death curve.  Along the crest it is flat (a few units over +-0.05
+# in R0) and rough at the 1-2 unit level (16-seed means, integer seed
+# counts), so a small stencil cannot place its maximum.  The refinement
+# therefore fits a quadratic over a broad design sheared along the crest:
+# u = R0 - R0c along it (+-0.12), v = log10 scale - (ls_c + slope * u)
+# across it.  The design's centre and slope are snapped to a lattice, so two
+# calibrations that start in the same basin evaluate the same candidates
+# (with common random numbers the simulator's likelihood is a deterministic
+# function) and return the same fit.
+DESIGN_U = np.array([-0.12, -0.06, 0.0, 0.06, 0.12])
+DESIGN_V = np.array([-0.06, 0.0, 0.06])
+DESIGN = np.array([(u, v) for u in DESIGN_U for v in DESIGN_V])
+LATTICE = np.array([0.04, 0.05])   # centre: R0, log10 scale
+SLOPE_LATTICE = 0.5
 
 
-def quad_argmax(z, f):
-    """Least-squares quadratic through f at stencil points z [n, 2]; its
-    maximiser (stencil units) when the fit is concave, else None."""
-    x, y = z[:, 0], z[:, 1]
-    a = np.column_stack([np.ones_like(x), x, y, x * x, y * y, x * y])
+def _snap(x, q):
+    return np.round(np.asarray(x, np.float64) / q) * q
+
+
+def crest_fit(z, f):
+    """Least-squares quadratic a + b u + c v + d u^2 + e v^2 + g u v through f
+    at design points z [n, 2]; returns (u*, v*) of its maximum and the crest
+    slope dv*/du = -g / (2e), or None when the fit is not concave."""
+    u, v = z[:, 0], z[:, 1]
+    a = np.column_stack([np.ones_like(u), u, v, u * u, v * v, u * v])
     c = np.linalg.lstsq(a, f, rcond=None)[0]
     hess = np.array([[2 * c[3], c[5]], [c[5], 2 * c[4]]])
     if np.any(np.linalg.eigvalsh(hess) >= 0):
         return None
-    return np.linalg.solve(hess, -c[1:3])
+    return np.linalg.solve(hess, -c[1:3]), -c[5] / (2 * c[4])
 
 
-def polish(score, center, max_iter=6):
-    """Refine from center: evaluate the stencil (one batch of simulator
-    launches), fit the quadratic, and stop when its maximiser lies inside the
-    stencil's inner half; else move there (at most 1.5 steps per axis) or, for
-    a non-concave fit, to the best stencil point, and repeat.
-    Returns (theta, loglik at the nearest evaluated point, stencils used)."""
+def refine(score, r0, ls, slope, max_designs=3):
+    """Fit the sheared design around (r0, ls) with the given crest slope; if
+    the fitted maximum lies outside the design's inner half, move the design
+    there (snapped) with the fitted slope and fit again.  Returns the fitted
+    (R0, log10 scale), the best evaluated log-likelihood and the designs used."""
     box = np.array([R0_RANGE, LOGS_RANGE])
-    c = np.asarray(center, np.float64)
-    for it in range(1, max_iter + 1):
-        pts = np.clip(c + STENCIL * STEP, box[:, 0], box[:, 1])
+    c = _snap([r0, ls], LATTICE)
+    s = float(_snap(slope, SLOPE_LATTICE))
+    best = -np.inf
+    theta = c
+    for k in range(1, max_designs + 1):
+        pts = np.column_stack([c[0] + DESIGN[:, 0], c[1] + s * DESIGN[:, 0] + DESIGN[:, 1]])
+        pts = np.clip(pts, box[:, 0], box[:, 1])
         f = score(pts)
-        zs = (pts - c) / STEP
-        zmax = quad_argmax(zs, f)
-        if zmax is not None and np.all(np.abs(zmax) <= 0.5):
-            th = np.clip(c + zmax * STEP, box[:, 0], box[:, 1])
-            return th, float(np.max(f)), it
-        c = pts[int(np.argmax(f))] if zmax is None else np.clip(c + np.clip(zmax, -1.5, 1.5) * STEP, box[:, 0], box[:, 1])
-    return c, float(np.max(f)), max_iter
+        best = max(best, float(np.max(f)))
+        fit = crest_fit(np.column_stack([pts[:, 0] - c[0], pts[:, 1] - c[1] - s * (pts[:, 0] - c[0])]), f)
+        if fit is None:
+            theta = pts[int(np.argmax(f))]
+            c = _snap(theta, LATTICE)
+            continue
+        (u, v), ds = fit
+        u, v = float(np.clip(u, -0.2, 0.2)), float(np.clip(v, -0.2, 0.2))
+        theta = np.clip([c[0] + u, c[1] + s * u + v], box[:, 0], box[:, 1])
+        if abs(u) <= 0.06 and abs(v) <= 0.03:
+            return theta, best, k
+        c = _snap(theta, LATTICE)
+        s = float(_snap(s + ds, SLOPE_LATTICE))
+    return theta, best, max_designs
 
 
-def _score(sim, target_deaths, seeds):
-    return lambda th: nb_loglik(target_deaths, sim.run(th, seeds)[1])
+def crest_from_grid(r0, ls, ll, around=None, halfwidth=0.1):
+    """The crest of a log-likelihood grid ll[len(r0), len(ls)]: for each R0,
+    the maximising log10 scale (parabola through the best three); returns the
+    best R0 (or `around`), its crest log10 scale and the crest slope fitted
+    over R0 within halfwidth."""
+    cr = []
+    for i in range(len(r0)):
+        j = int(np.clip(np.argmax(ll[i]), 1, len(ls) - 2))
+        p = np.polyfit(ls[j - 1:j + 2], ll[i, j - 1:j + 2], 2)
+        x = -p[1] / (2 * p[0]) if p[0] < 0 else ls[j]
+        x = float(np.clip(x, ls[j - 1], ls[j + 1]))
+        cr.append((x, float(np.polyval(p, x)) if p[0] < 0 else float(ll[i, j])))
+    cr = np.array(cr)
+    i0 = int(np.argmax(cr[:, 1])) if around is None else int(np.argmin(np.abs(r0 - around)))
+    sel = np.abs(r0 - r0[i0]) <= halfwidth + 1e-9
+    slope = np.polyfit(r0[sel], cr[sel, 0], 1)[0] if sel.sum() >= 2 else 0.0
+    return float(r0[i0]), float(cr[i0, 0]), float(slope)
 
 
 def calibrate_plain(sim, target_deaths, seeds, grid=(13, 8)):
     """Simulator only: a coarse grid over the whole box (R0 step 0.1, log10
-    scale step 0.2), then the quadratic refinement from its best point."""
+    scale step 0.2), then the crest refinement from the grid's crest."""
     t0 = time.perf_counter()
     l0 = sim.launches
     score = _score(sim, target_deaths, seeds)
-    th = np.array([(a, b) for a in np.linspace(*R0_RANGE, grid[0]) for b in np.linspace(*LOGS_RANGE, grid[1])])
-    f = score(th)
-    x, fx, it = polish(score, th[int(np.argmax(f))])
-    return {"theta": x.tolist(), "loglik": fx, "candidates": len(th) + it * len(STENCIL), "stencils": it,
-            "launches": sim.launches - l0, "seconds": time.perf_counter() - t0}
+    r0 = np.linspace(*R0_RANGE, grid[0])
+    ls = np.linspace(*LOGS_RANGE, grid[1])
+    f = score(np.array([(a, b) for a in r0 for b in ls])).reshape(len(r0), len(ls))
+    c0, l0s, slope = crest_from_grid(r0, ls, f, halfwidth=0.1)
+    x, fx, k = refine(score, c0, l0s, slope)
+    return {"theta": x.tolist(), "loglik": fx, "start": [c0, l0s, slope], "candidates": f.size + k * len(DESIGN),
+            "designs": k, "launches": sim.launches - l0, "seconds": time.perf_counter() - t0}
 
 
 def calibrate_pattern(sim, target_deaths, seeds, start=(2.4, 0.0), step=(0.4, 0.4)):
@@ -231,21 +271,22 @@ def calibrate_pattern(sim, target_deaths, seeds, start=(2.4, 0.0), step=(0.4, 0.
 
 def calibrate_emulated(sim, emu, target_deaths, seeds, grid=(121, 57)):
     """The emulator scores a dense grid (R0 step 0.01) in one batched forward
-    pass and proposes its best candidate; the simulator verifies it through the
-    same quadratic refinement as the plain calibration, which also serves as
-    the fallback when the proposal is off (more stencils)."""
+    pass and proposes the crest around its best candidate; the simulator
+    verifies and refines the proposal with the same crest refinement as the
+    plain calibration (more designs when the proposal is off: the plain
+    simulator as fallback)."""
     t0 = time.perf_counter()
     l0 = sim.launches
     r0 = np.linspace(*R0_RANGE, grid[0])
     ls = np.linspace(*LOGS_RANGE, grid[1])
     th = np.array([(a, b) for a in r0 for b in ls])
     _, d_pred = to_curves(emu.forward(encode(th)))
-    ll = nb_loglik(target_deaths, d_pred)
-    proposal = th[int(np.argmax(ll))]
+    ll = nb_loglik(target_deaths, d_pred).reshape(len(r0), len(ls))
+    c0, l0s, slope = crest_from_grid(r0, ls, ll, halfwidth=0.1)
     t_emul = time.perf_counter() - t0
-    x, fx, it = polish(_score(sim, target_deaths, seeds), proposal)
-    return {"theta": x.tolist(), "loglik": fx, "proposal": proposal.tolist(), "candidates": it * len(STENCIL),
-            "stencils": it, "launches": sim.launches - l0, "seconds": time.perf_counter() - t0,
+    x, fx, k = refine(_score(sim, target_deaths, seeds), c0, l0s, slope)
+    return {"theta": x.tolist(), "loglik": fx, "proposal": [c0, l0s, slope], "candidates": k * len(DESIGN),
+            "designs": k, "launches": sim.launches - l0, "seconds": time.perf_counter() - t0,
             "emulator_seconds": t_emul}
 
 

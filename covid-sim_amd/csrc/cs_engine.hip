@@ -500,6 +500,9 @@ int cs_set_population(cs_engine* e, const cs_population* pop)
     if (!e->have_params) return set_err(e, CS_ESTATE, "cs_set_params must come first");
     HIPCHK(e, hipSetDevice(e->device));
     if (pop->n_people <= 0 || pop->n_cells <= 0) return set_err(e, CS_EINVAL, "empty population");
+    if ((uint64_t)pop->n_people * CS_MAX_PLACE_TYPES > 0xffffffffull)
+        return set_err(e, CS_EINVAL, "%d persons: the place-link index (persons x place types) must fit 32 bits",
+                       pop->n_people);
     HIPCHK(e, hipStreamSynchronize(e->stream));
     drop_graph(e);
     free_list(e->real_allocs);
@@ -558,20 +561,30 @@ int cs_set_population(cs_engine* e, const cs_population* pop)
     auto& L = e->pop_allocs;
     {
         std::vector<cs::PStat> ps(N);
-        for (int i = 0; i < N; i++) ps[i] = cs::PStat{pop->hh[i], pop->age[i], pop->flags[i], 0};
+        for (int i = 0; i < N; i++)
+            ps[i] = cs::PStat{pop->hh[i], pop->age[i], pop->flags[i], 0,
+                              make_float2(pop->hh_loc[2 * (size_t)pop->hh[i]], pop->hh_loc[2 * (size_t)pop->hh[i] + 1])};
         std::vector<cs::HRec> hr(e->NH);
         for (int h = 0; h < e->NH; h++) hr[h] = cs::HRec{pop->hh_first[h], pop->hh_nh[h], pop->hh_nhr[h]};
         if ((rc = dupload(e, L, &d.pst, ps.data(), N))) return rc;
         if ((rc = dupload(e, L, &d.hrec, hr.data(), e->NH))) return rc;
         d.hh.base = d.age.base = d.pflags.base = d.pst;
+        d.ploc.base = d.pst;
         d.hfirst.base = d.hnh.base = d.hnhr.base = d.hrec;
     }
     if ((rc = dupload(e, L, &d.pc, pc.data(), N))) return rc;
     if ((rc = dupload(e, L, &d.mcell, pop->mcell, N))) return rc;
     if ((rc = dupload(e, L, &d.padunit, padunit.data(), N))) return rc;
-    if ((rc = dupload(e, L, &d.plinks, pop->place_links, (size_t)N * CS_MAX_PLACE_TYPES))) return rc;
-    if ((rc = dupload(e, L, &d.pglinks, pop->place_group_links, (size_t)N * CS_MAX_PLACE_TYPES))) return rc;
-    if ((rc = dupload(e, L, &d.hloc, pop->hh_loc, (size_t)e->NH * 2))) return rc;
+    {
+        std::vector<cs::PLinks> lk(N);
+        for (int i = 0; i < N; i++)
+            for (int t = 0; t < CS_MAX_PLACE_TYPES; t++) {
+                lk[i].lk[t] = pop->place_links[(size_t)i * CS_MAX_PLACE_TYPES + t];
+                lk[i].g[t] = pop->place_group_links[(size_t)i * CS_MAX_PLACE_TYPES + t];
+            }
+        if ((rc = dupload(e, L, &d.plk, lk.data(), N))) return rc;
+        d.plinks.base = d.pglinks.base = d.plk;
+    }
     if ((rc = dupload(e, L, &d.cstart, pop->cell_member_start, NC + 1))) return rc;
     if ((rc = dupload(e, L, &d.cn, pop->cell_n, NC))) return rc;
     if ((rc = dupload(e, L, &d.cidx, pop->cell_index, NC))) return rc;
@@ -584,7 +597,12 @@ int cs_set_population(cs_engine* e, const cs_population* pop)
         int np = (t < e->NPT) ? pop->n_places[t] : 0;
         e->np[t] = d.np[t] = np;
         off += np;
-        if (t >= e->NPT) { d.pl_n[t] = d.pl_ms[t] = d.pl_mem[t] = d.pl_gs0[t] = d.pl_gst[t] = d.pl_gsz[t] = nullptr; continue; }
+        if (t >= e->NPT) {
+            d.pl_rec[t] = d.pl_n[t].base = d.pl_ms[t].base = d.pl_gs0[t].base = nullptr;
+            d.pl_grp[t] = d.pl_gst[t].base = d.pl_gsz[t].base = nullptr;
+            d.pl_mem[t] = nullptr;
+            continue;
+        }
         const int32_t* ms = pop->place_member_start[t];
         const int32_t* gs0 = pop->place_groups_start[t];
         if (np < 0 || (np && (!ms || !gs0 || ms[0] != 0 || gs0[0] != 0))) return set_err(e, CS_EINVAL, "place arrays of type %d", t);
@@ -599,12 +617,15 @@ int cs_set_population(cs_engine* e, const cs_population* pop)
         for (int q = 0; q < (np ? ms[np] : 0); q++)
             if (pop->place_members[t][q] < 0 || pop->place_members[t][q] >= N) return set_err(e, CS_EINVAL, "place member out of range");
         int ng = np ? gs0[np] : 0;
-        if ((rc = dupload(e, L, &d.pl_n[t], pop->place_n[t], np))) return rc;
-        if ((rc = dupload(e, L, &d.pl_ms[t], ms, np + 1))) return rc;
+        std::vector<cs::PlRec> pr(np + 1);
+        for (int k = 0; k <= np; k++) pr[k] = cs::PlRec{ms[k], gs0[k], k < np ? pop->place_n[t][k] : 0, 0};
+        std::vector<cs::GRec> gr(ng);
+        for (int g = 0; g < ng; g++) gr[g] = cs::GRec{pop->place_group_start[t][g], pop->place_group_size[t][g]};
+        if ((rc = dupload(e, L, &d.pl_rec[t], pr.data(), np + 1))) return rc;
+        if ((rc = dupload(e, L, &d.pl_grp[t], gr.data(), ng))) return rc;
         if ((rc = dupload(e, L, &d.pl_mem[t], pop->place_members[t], np ? ms[np] : 0))) return rc;
-        if ((rc = dupload(e, L, &d.pl_gs0[t], gs0, np + 1))) return rc;
-        if ((rc = dupload(e, L, &d.pl_gst[t], pop->place_group_start[t], ng))) return rc;
-        if ((rc = dupload(e, L, &d.pl_gsz[t], pop->place_group_size[t], ng))) return rc;
+        d.pl_n[t].base = d.pl_ms[t].base = d.pl_gs0[t].base = d.pl_rec[t];
+        d.pl_gst[t].base = d.pl_gsz[t].base = d.pl_grp[t];
     }
     d.np_off[CS_MAX_PLACE_TYPES] = off;
     for (int i = 0; i < N; i++)
@@ -666,7 +687,7 @@ int cs_alloc_realisations(cs_engine* e, int n_real, const uint64_t* seeds)
     AL(ev_key, cap); AL(ev_gp, cap); AL(ev_infector, cap); AL(ev_type, cap); AL(ev_count, 1);
     AL(claim, RN); AL(lastk, RN); AL(batch, RN); AL(bcount, RC);
     AL(care_gp, RN); AL(care_k, RN); AL(care_abs, RN); AL(care_count, 1);
-    AL(last_tot, 3 * R);
+    AL(last_tot, CS_TS_LAST * R);
 #undef AL
     if ((rc = dalloc(e, A, &e->d_sel, R))) return rc;
     if ((rc = dalloc(e, A, &e->rbeta_buf, R))) return rc;

@@ -364,7 +364,7 @@ __device__ void do_infect(const Dev& d, int r, int p, double t, unsigned ts, int
     Counters* cn = d.cnt + r;
     d.inf[gp] = CS_LATENT;
     d.t_inf[gp] = (uint16_t)ts;
-    const float2 loc = reinterpret_cast<const float2*>(d.hloc)[d.hh[p]];
+    const float2 loc = d.ploc[p];
     double x = (loc.x - d.loc_init[2 * r]);
     double y = (loc.y - d.loc_init[2 * r + 1]);
     double rad2 = x * x + y * y;
@@ -739,6 +739,8 @@ __global__ void __launch_bounds__(256) k_spatial_draws(Dev d)
         int jj = (I == 1) ? 0 : select_infector(d.ci_buf + sbase, I, s.next());
         bool closed = (d.ci_buf[sbase + jj] < 0);
         const int i = d.perm[sbase + jj];
+        const float2 loc_i = d.ploc[i];
+        const int hh_i = d.hh[i];
         for (int att = 0;; att++) {
             if (att >= CS_MAX_ATTEMPTS) {  // no live candidate: report, never spin
                 atomicOr(d.status, (unsigned)FLAG_STUCK);
@@ -749,10 +751,11 @@ __global__ void __launch_bounds__(256) k_spatial_draws(Dev d)
             const size_t rl = (size_t)r * d.NC + l;
             int slot = (int)(s.next() * ((double)d.S0[rl]));
             int m = d.perm[rN + d.cstart[l] + slot];
-            double d2 = dist2_pp(d, m, i);
+            const PStat pm = d.pst[m];
+            double d2 = dist2_ll(P, pm.loc, loc_i);
             double acc = kernel_num(d, d2) / d.max_trans[(size_t)c * d.NC + l];
             if ((s.next() >= acc) || is_dead_code(d.inf[rN + m])) continue;
-            if ((c != l) || (d.hh[m] != d.hh[i])) {
+            if ((c != l) || (pm.hh != hh_i)) {
                 double SS = spatial_susc(d, r, m, ts);
                 if (care(d, m) || care(d, i)) SS *= P->care_home_resident_spatial_scaling;
                 if (slot < d.cS[rl]) {

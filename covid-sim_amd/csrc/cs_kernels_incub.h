@@ -471,7 +471,7 @@ __global__ void k_seed(Dev d, int n_seed)
         int q = d.mmem[d.mstart[m] + (int)(s.next() * (double)nm)];
         if (d.inf[rN + q] == CS_SUSCEPTIBLE && (person_susc(d, r, q, 0, 0) > 0) &&
             (P->care_home_place_type < 0 || d.plinks[(size_t)q * CS_MAX_PLACE_TYPES + P->care_home_place_type] < 0)) {
-            const float2 loc = reinterpret_cast<const float2*>(d.hloc)[d.hh[q]];
+            const float2 loc = d.ploc[q];
             d.loc_init[2 * r] = loc.x;
             d.loc_init[2 * r + 1] = loc.y;
             d.infector[rN + q] = -2;
@@ -508,13 +508,20 @@ __global__ void k_record(Dev d, int sample)
     if (threadIdx.x == 0) d.samp_prop[r] = (double)part[0][0] / (double)d.N;
     if (threadIdx.x == 0 && sample < d.ts_max) {
         int32_t* o = d.ts_buf + ((size_t)r * d.ts_max + sample) * CS_TS_FIELDS;
-        long long cumI = (long long)d.cnt[r].cumI, cumC = (long long)d.cnt[r].cumC, D = part[4][0];
-        long long* lt = d.last_tot + 3 * r;
+        const Counters& cn = d.cnt[r];
+        long long now[CS_TS_LAST];
+        now[0] = (long long)cn.cumI;
+        now[1] = (long long)cn.cumC;
+        now[2] = part[4][0];
+        for (int k = 0; k < CS_NUM_SEV_KINDS; k++) now[3 + k] = (long long)cn.sev_cum[k];
+        long long* lt = d.last_tot + CS_TS_LAST * r;
         for (int q = 0; q < 5; q++) o[q] = (int32_t)part[q][0];
-        o[5] = (int32_t)(cumI - lt[0]);
-        o[6] = (int32_t)(cumC - lt[1]);
-        o[7] = (int32_t)(D - lt[2]);
-        lt[0] = cumI; lt[1] = cumC; lt[2] = D;
+        o[CS_TS_INC_I] = (int32_t)(now[0] - lt[0]);
+        o[CS_TS_INC_C] = (int32_t)(now[1] - lt[1]);
+        o[CS_TS_INC_DEATH] = (int32_t)(now[2] - lt[2]);
+        for (int k = 0; k < CS_NUM_SEV_KINDS; k++) o[CS_TS_INC_SEV + k] = (int32_t)(now[3 + k] - lt[3 + k]);
+        for (int k = 0; k <= CS_SK_CRITRECOV; k++) o[CS_TS_PREV_SEV + k] = (int32_t)(long long)cn.sev_prev[k];
+        for (int q = 0; q < CS_TS_LAST; q++) lt[q] = now[q];
     }
 }
 
@@ -528,9 +535,11 @@ __global__ void k_record_init(Dev d)
         S += d.cS[(size_t)r * d.NC + c];
     }
     d.samp_prop[r] = (double)S / (double)d.N;  // the sample RunModel takes first
-    d.last_tot[3 * r] = (long long)d.cnt[r].cumI;
-    d.last_tot[3 * r + 1] = (long long)d.cnt[r].cumC;
-    d.last_tot[3 * r + 2] = D;
+    long long* lt = d.last_tot + CS_TS_LAST * r;
+    lt[0] = (long long)d.cnt[r].cumI;
+    lt[1] = (long long)d.cnt[r].cumC;
+    lt[2] = D;
+    for (int k = 0; k < CS_NUM_SEV_KINDS; k++) lt[3 + k] = (long long)d.cnt[r].sev_cum[k];
 }
 
 // ----------------------------------------------- single-function kernels

@@ -10,8 +10,8 @@
 //
 // The fields the infection kernels read together are packed into records,
 // so that a scattered visit to one person costs one cache line instead of
-// one line per field: Hot per (realisation, person), PStat per person, HRec
-// per household.  Each packed field keeps an array-like accessor (Fld), so
+// one line per field: Hot per (realisation, person), PStat and PLinks per
+// person, HRec per household, PlRec per place and GRec per place group.  Each packed field keeps an array-like accessor (Fld), so
 // d.inf[gp] reads the inf field of record gp.
 #pragma once
 #include <hip/hip_runtime.h>
@@ -72,12 +72,25 @@ struct PStat {  // per person, shared by all realisations
     int32_t hh;
     uint8_t age, pflags;
     uint16_t pad;
+    float2 loc;  // the household's location (a copy, so a distance needs no second lookup)
 };
 struct HRec {  // per household
     int32_t hfirst;
     uint16_t hnh, hnhr;
 };
-static_assert(sizeof(Hot) == 16 && sizeof(PStat) == 8 && sizeof(HRec) == 8, "record sizes");
+struct PLinks {  // per person: place and group link of each place type
+    int32_t lk[CS_MAX_PLACE_TYPES];
+    uint16_t g[CS_MAX_PLACE_TYPES];
+    uint16_t pad;
+};
+struct PlRec {  // per place (n_places + 1 records; the last holds the ends)
+    int32_t ms, gs0, n, pad;  // member start, first group, member count
+};
+struct GRec {  // per group of a place
+    int32_t gst, gsz;  // group start within the place's members, group size
+};
+static_assert(sizeof(Hot) == 16 && sizeof(PStat) == 16 && sizeof(HRec) == 8 && sizeof(PLinks) == 32 &&
+              sizeof(PlRec) == 16 && sizeof(GRec) == 8, "record sizes");
 
 // one field of an array of records, indexed like a plain array
 template <typename Rec, typename T, size_t OFF>
@@ -92,6 +105,25 @@ struct Fld {
 #define CS_FLD(Rec, f) Fld<Rec, decltype(((Rec*)nullptr)->f), offsetof(Rec, f)>
 #define CS_CFLD(Rec, f) Fld<const Rec, const decltype(((Rec*)nullptr)->f), offsetof(Rec, f)>
 
+// the [person][place type] link arrays, flat index i * CS_MAX_PLACE_TYPES + t
+// (below 2^32: persons x place types)
+struct LinkFld {
+    const PLinks* base;
+    __host__ __device__ __forceinline__ const int32_t& operator[](size_t i) const
+    {
+        const uint32_t p = (uint32_t)i / CS_MAX_PLACE_TYPES;
+        return base[p].lk[(uint32_t)i - p * CS_MAX_PLACE_TYPES];
+    }
+};
+struct GLinkFld {
+    const PLinks* base;
+    __host__ __device__ __forceinline__ const uint16_t& operator[](size_t i) const
+    {
+        const uint32_t p = (uint32_t)i / CS_MAX_PLACE_TYPES;
+        return base[p].g[(uint32_t)i - p * CS_MAX_PLACE_TYPES];
+    }
+};
+
 struct Dev {
     int N, NH, NC, NM, NPT, R;
     const cs_params* P;
@@ -103,17 +135,25 @@ struct Dev {
     CS_CFLD(PStat, hh) hh;
     CS_CFLD(PStat, age) age;
     CS_CFLD(PStat, pflags) pflags;
+    CS_CFLD(PStat, loc) ploc;
     CS_CFLD(HRec, hfirst) hfirst;
     CS_CFLD(HRec, hnh) hnh;
     CS_CFLD(HRec, hnhr) hnhr;
-    const int32_t *pc, *mcell, *plinks, *cstart, *cn, *cidx, *mstart, *mmem, *cmem;
+    const PLinks* plk;    // [N]
+    LinkFld plinks;
+    GLinkFld pglinks;
+    const int32_t *pc, *mcell, *cstart, *cn, *cidx, *mstart, *mmem, *cmem;
     const int32_t* padunit;  // administrative unit of each person's microcell
-    const uint16_t* pglinks;
-    const float* hloc;
     int np[CS_MAX_PLACE_TYPES];
     int np_off[CS_MAX_PLACE_TYPES + 1];  // offsets of each type in the per-realisation place arrays
-    const int32_t *pl_n[CS_MAX_PLACE_TYPES], *pl_ms[CS_MAX_PLACE_TYPES], *pl_mem[CS_MAX_PLACE_TYPES];
-    const int32_t *pl_gs0[CS_MAX_PLACE_TYPES], *pl_gst[CS_MAX_PLACE_TYPES], *pl_gsz[CS_MAX_PLACE_TYPES];
+    const PlRec* pl_rec[CS_MAX_PLACE_TYPES];
+    const GRec* pl_grp[CS_MAX_PLACE_TYPES];
+    CS_CFLD(PlRec, n) pl_n[CS_MAX_PLACE_TYPES];
+    CS_CFLD(PlRec, ms) pl_ms[CS_MAX_PLACE_TYPES];
+    CS_CFLD(PlRec, gs0) pl_gs0[CS_MAX_PLACE_TYPES];
+    CS_CFLD(GRec, gst) pl_gst[CS_MAX_PLACE_TYPES];
+    CS_CFLD(GRec, gsz) pl_gsz[CS_MAX_PLACE_TYPES];
+    const int32_t* pl_mem[CS_MAX_PLACE_TYPES];
     const float* max_trans;
     // dynamic, per realisation
     const uint64_t* seed;
@@ -174,7 +214,7 @@ struct Dev {
     Clock* clock;
     int32_t* ts_buf;      // [R][max_samples][CS_TS_FIELDS]
     int ts_max;
-    long long* last_tot;  // [R][3] cumI, cumC, D at the previous sample
+    long long* last_tot;  // [R][CS_TS_LAST] cumI, cumC, D, severity incidence at the previous sample
 };
 
 // ---- predicates (src/ModelMacros.h:6-23, src/Person.cpp) ----
@@ -277,16 +317,16 @@ __device__ __noinline__ double dist2_utm(const cs_params* P, double x1, double y
     return 4 * er * er * y;
 }
 
-__device__ __forceinline__ double dist2_pp(const Dev& d, int a, int b)
+// dist2 of two household locations (src/Dist.cpp dist2 over Households[].loc)
+__device__ __forceinline__ double dist2_ll(const cs_params* P, float2 la, float2 lb)
 {
-    const cs_params* P = d.P;
-    const float2 la = reinterpret_cast<const float2*>(d.hloc)[d.hh[a]];
-    const float2 lb = reinterpret_cast<const float2*>(d.hloc)[d.hh[b]];
     if (P->do_utm_coords) return dist2_utm(P, la.x, la.y, lb.x, lb.y);
     double x = __builtin_fabs((double)la.x - (double)lb.x);
     double y = __builtin_fabs((double)la.y - (double)lb.y);
     return periodic_xy(P, x, y);
 }
+
+__device__ __forceinline__ double dist2_pp(const Dev& d, int a, int b) { return dist2_ll(d.P, d.ploc[a], d.ploc[b]); }
 
 __device__ __noinline__ double dist2_cc_min(const cs_params* P, int l, int m)
 {

@@ -178,6 +178,10 @@ class Engine:
         return c
 
     def timeseries(self, max_samples):
+        """[n_real, n_samples, CS_TS_FIELDS] int32: per sample S, L, I, R, D,
+        incidence of infections, cases and deaths, incidence of each severity
+        kind (abi.CS_TS_INC_SEV + CS_SK_*) and severity prevalence
+        (abi.CS_TS_PREV_SEV + CS_SK_MILD .. CS_SK_CRITRECOV)."""
         out = np.zeros((self.n_real, max_samples, abi.CS_TS_FIELDS), np.int32)
         n = C.c_int()
         self.call("get_timeseries", _p(out, C.c_int32), int(max_samples), C.byref(n))

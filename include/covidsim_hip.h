@@ -387,8 +387,23 @@ int  cs_autotune(cs_engine* e, double t, int reps, cs_tuning* out);
 /* results */
 int  cs_get_counters(cs_engine* e, int real, cs_counters* out, int reset);
 /* series [n_real][n_samples][CS_TS_FIELDS] recorded by cs_run_steps since the
- * series last started; out holds max_samples per realisation */
-#define CS_TS_FIELDS 8   /* S, L, I, R, D, incI, incC, incDeath */
+ * series last started; out holds max_samples per realisation.  Per sample
+ * (RecordSample, src/CovidSim.cpp:3024-3483):
+ *   0-4    S, L, I, R, D totals at the sample
+ *   5-7    incidence since the previous sample: infections, cases, deaths
+ *   8-15   incidence since the previous sample of Mild, ILI, SARI, Critical,
+ *          CritRecov, and of deaths from ILI, SARI and Critical (the
+ *          incMild .. incDeath_Critical fields)
+ *   16-20  prevalence of Mild, ILI, SARI, Critical, CritRecov, counted from
+ *          the last zeroing of the counters (cs_reset_realisations, or
+ *          cs_get_counters with reset), as StateT's running totals are
+ * The incidence fields are differences of the cumulative counters, so a
+ * series must not be interleaved with cs_get_counters(..., reset != 0). */
+#define CS_TS_FIELDS 21
+enum { CS_TS_S = 0, CS_TS_L, CS_TS_I, CS_TS_R, CS_TS_D, CS_TS_INC_I, CS_TS_INC_C, CS_TS_INC_DEATH,
+       CS_TS_INC_SEV = 8,    /* + CS_SK_* kind, 8 kinds */
+       CS_TS_PREV_SEV = 16 }; /* + CS_SK_MILD .. CS_SK_CRITRECOV */
+#define CS_TS_LAST 11   /* cumulative counters a sample differences: cumI, cumC, D, 8 severity kinds */
 int  cs_get_timeseries(cs_engine* e, int32_t* out, int max_samples, int* n_samples);
 int  cs_get_work_counts(cs_engine* e, cs_work_counts* out, int reset);
 int  cs_sync(cs_engine* e);

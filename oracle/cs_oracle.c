@@ -1772,12 +1772,14 @@ int or_run_steps(or_engine* e, double t0, int n_steps, int sps)
     if (e->ts_fresh || !e->ts_last) {
         /* a new series: RunModel samples before its first timestep (src/CovidSim.cpp:1413) */
         free(e->ts_last);
-        e->ts_last = (int64_t*)calloc((size_t)e->nreal * 3, 8);
+        e->ts_last = (int64_t*)calloc((size_t)e->nreal * CS_TS_LAST, 8);
         e->ts_n = 0;
         for (int ri = 0; ri < e->nreal; ri++) {
             int64_t S, L, I, R, D;
             totals(e, &e->R[ri], &S, &L, &I, &R, &D);
-            e->ts_last[3 * ri] = e->R[ri].cnt.cumI; e->ts_last[3 * ri + 1] = e->R[ri].cnt.cumC; e->ts_last[3 * ri + 2] = D;
+            int64_t* lt = e->ts_last + (size_t)CS_TS_LAST * ri;
+            lt[0] = e->R[ri].cnt.cumI; lt[1] = e->R[ri].cnt.cumC; lt[2] = D;
+            for (int k = 0; k < CS_NUM_SEV_KINDS; k++) lt[3 + k] = e->R[ri].cnt.sev_cum[k];
             e->R[ri].samp_prop_susc = (double)S / (double)e->N;
         }
         e->ts_fresh = 0;
@@ -1815,13 +1817,20 @@ int or_run_steps(or_engine* e, double t0, int n_steps, int sps)
                 int64_t S, L, I, R, D;
                 totals(e, r, &S, &L, &I, &R, &D);
                 r->samp_prop_susc = (double)S / (double)e->N;
-                int64_t* lt = e->ts_last + 3 * ri;
+                int64_t* lt = e->ts_last + (size_t)CS_TS_LAST * ri;
                 int32_t* o = e->ts_buf + ((size_t)ri * e->ts_cap + e->ts_n) * CS_TS_FIELDS;
+                int64_t now[CS_TS_LAST];
+                now[0] = r->cnt.cumI; now[1] = r->cnt.cumC; now[2] = D;
+                for (int k = 0; k < CS_NUM_SEV_KINDS; k++) now[3 + k] = r->cnt.sev_cum[k];
                 o[0] = (int32_t)S; o[1] = (int32_t)L; o[2] = (int32_t)I; o[3] = (int32_t)R; o[4] = (int32_t)D;
-                o[5] = (int32_t)(r->cnt.cumI - lt[0]);
-                o[6] = (int32_t)(r->cnt.cumC - lt[1]);
-                o[7] = (int32_t)(D - lt[2]);
-                lt[0] = r->cnt.cumI; lt[1] = r->cnt.cumC; lt[2] = D;
+                o[CS_TS_INC_I] = (int32_t)(now[0] - lt[0]);
+                o[CS_TS_INC_C] = (int32_t)(now[1] - lt[1]);
+                o[CS_TS_INC_DEATH] = (int32_t)(now[2] - lt[2]);
+                /* incMild .. incDeath_Critical and the running severity
+                 * totals (src/CovidSim.cpp RecordSample, from StateT) */
+                for (int k = 0; k < CS_NUM_SEV_KINDS; k++) o[CS_TS_INC_SEV + k] = (int32_t)(now[3 + k] - lt[3 + k]);
+                for (int k = 0; k <= CS_SK_CRITRECOV; k++) o[CS_TS_PREV_SEV + k] = (int32_t)r->cnt.sev_prev[k];
+                memcpy(lt, now, sizeof(now));
             }
             e->ts_n++;
         }

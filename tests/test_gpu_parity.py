@@ -199,7 +199,7 @@ def test_trajectory_matches_oracle(engmod, csa):
         eng.reset(10)
         eng.run_steps(0.0, steps, int(params.time_steps_per_day))
     tg, to = g.timeseries(120), o.timeseries(120)
-    assert tg.shape == (3, 120, 8)
+    assert tg.shape == (3, 120, 21)
     assert np.array_equal(tg, to)
     for r in range(3):
         a, b = csa.population.State(pop), csa.population.State(pop)

@@ -199,30 +199,35 @@ def refine(score, r0, ls, slope, max_designs=3):
     """Fit the sheared design around (r0, ls) with the given crest slope; if
     the fitted maximum lies outside the design's inner half, move the design
     there (snapped) with the fitted slope and fit again.  Returns the fitted
-    (R0, log10 scale), the best evaluated log-likelihood and the designs used."""
+    (R0, log10 scale) -- the best evaluated candidate when max_designs pass
+    without settling -- the best evaluated log-likelihood and the designs used."""
     box = np.array([R0_RANGE, LOGS_RANGE])
     c = _snap([r0, ls], LATTICE)
     s = float(_snap(slope, SLOPE_LATTICE))
-    best = -np.inf
-    theta = c
+    best, best_pt = -np.inf, c
     for k in range(1, max_designs + 1):
         pts = np.column_stack([c[0] + DESIGN[:, 0], c[1] + s * DESIGN[:, 0] + DESIGN[:, 1]])
         pts = np.clip(pts, box[:, 0], box[:, 1])
         f = score(pts)
-        best = max(best, float(np.max(f)))
+        j = int(np.argmax(f))
+        if f[j] > best:
+            best, best_pt = float(f[j]), pts[j]
         fit = crest_fit(np.column_stack([pts[:, 0] - c[0], pts[:, 1] - c[1] - s * (pts[:, 0] - c[0])]), f)
         if fit is None:
-            theta = pts[int(np.argmax(f))]
-            c = _snap(theta, LATTICE)
+            c = _snap(pts[j], LATTICE)
             continue
         (u, v), ds = fit
         u, v = float(np.clip(u, -0.2, 0.2)), float(np.clip(v, -0.2, 0.2))
         theta = np.clip([c[0] + u, c[1] + s * u + v], box[:, 0], box[:, 1])
-        if abs(u) <= 0.06 and abs(v) <= 0.03:
+        c_new = _snap(theta, LATTICE)
+        s_new = float(_snap(s + ds, SLOPE_LATTICE))
+        # done when the maximum lies in the design's inner part, or when the
+        # moved design would be this one (a fixed point of the snapping)
+        if (abs(u) <= 0.06 and abs(v) <= 0.06) or (np.allclose(c_new, c) and s_new == s):
             return theta, best, k
-        c = _snap(theta, LATTICE)
-        s = float(_snap(s + ds, SLOPE_LATTICE))
-    return theta, best, max_designs
+        c, s = c_new, s_new
+    # not settled: the best candidate the designs evaluated
+    return np.asarray(best_pt, np.float64), best, max_designs
 
 
 def crest_from_grid(r0, ls, ll, around=None, halfwidth=0.1):
@@ -242,6 +247,10 @@ def crest_from_grid(r0, ls, ll, around=None, halfwidth=0.1):
     sel = np.abs(r0 - r0[i0]) <= halfwidth + 1e-9
     slope = np.polyfit(r0[sel], cr[sel, 0], 1)[0] if sel.sum() >= 2 else 0.0
     return float(r0[i0]), float(cr[i0, 0]), float(slope)
+
+
+def _score(sim, target_deaths, seeds):
+    return lambda th: nb_loglik(target_deaths, sim.run(th, seeds)[1])
 
 
 def calibrate_plain(sim, target_deaths, seeds, grid=(13, 8)):

@@ -670,9 +670,11 @@ int cs_alloc_realisations(cs_engine* e, int n_real, const uint64_t* seeds)
     AL(hot, RN);
     d.inf.base = d.hot; d.itype.base = d.hot; d.infness.base = d.hot;
     d.t_lat.base = d.hot; d.t_as.base = d.hot; d.t_ae.base = d.hot; d.t_gate.base = d.hot;
-    AL(sev_cur, RN); AL(sev_final, RN); AL(bits, RN); AL(qc, RN); AL(dct, RN);
+    AL(tim, RN);
+    d.t_inf.base = d.tim; d.t_rod.base = d.tim; d.t_sari.base = d.tim; d.t_crit.base = d.tim;
+    d.t_step.base = d.tim; d.t_det.base = d.tim; d.sev_cur.base = d.tim; d.sev_final.base = d.tim; d.bits.base = d.tim;
+    AL(qc, RN); AL(dct, RN);
     AL(listpos, RN); AL(infector, RN); AL(susc, RN); AL(prob_care, RN);
-    AL(t_inf, RN); AL(t_rod, RN); AL(t_sari, RN); AL(t_crit, RN); AL(t_step, RN); AL(t_det, RN);
     AL(t_ts, RN); AL(t_te, RN); AL(t_vs, RN); AL(t_is, RN); AL(t_qs, RN);
     AL(cS, RC); AL(cL, RC); AL(cI, RC); AL(cR, RC); AL(cD, RC); AL(cumTC, RC); AL(S0, RC);
     AL(tot_prob, RC); AL(cum_trans, RC * NC); AL(invcdf, RC * 1025); AL(perm, RN);
@@ -771,14 +773,25 @@ int cs_load_state(cs_engine* e, int real, const cs_state* s)
             x.t_gate = 0;  // loaded timers: look at every one
         }
         HIPCHK(e, hipMemcpy(d.hot + o, h.data(), N * sizeof(cs::Hot), hipMemcpyHostToDevice));
+        std::vector<cs::Tim> m(N);
+        HIPCHK(e, hipMemcpy(m.data(), d.tim + o, N * sizeof(cs::Tim), hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < N; i++) {
+            cs::Tim& x = m[i];
+            if (s->infection_time) x.t_inf = s->infection_time[i];
+            if (s->recovery_or_death_time) x.t_rod = s->recovery_or_death_time[i];
+            if (s->sari_time) x.t_sari = s->sari_time[i];
+            if (s->critical_time) x.t_crit = s->critical_time[i];
+            if (s->stepdown_time) x.t_step = s->stepdown_time[i];
+            if (s->detected_time) x.t_det = s->detected_time[i];
+            if (s->severity_current) x.sev_cur = s->severity_current[i];
+            if (s->severity_final) x.sev_final = s->severity_final[i];
+            if (s->bits) x.bits = s->bits[i];
+        }
+        HIPCHK(e, hipMemcpy(d.tim + o, m.data(), N * sizeof(cs::Tim), hipMemcpyHostToDevice));
     }
 #define UP(dst, src, off, n) if (src) HIPCHK(e, hipMemcpy(d.dst + (off), src, (n) * sizeof(*d.dst), hipMemcpyHostToDevice))
-    UP(sev_cur, s->severity_current, o, N); UP(sev_final, s->severity_final, o, N);
-    UP(bits, s->bits, o, N); UP(listpos, s->listpos, o, N); UP(infector, s->infector, o, N);
+    UP(listpos, s->listpos, o, N); UP(infector, s->infector, o, N);
     UP(susc, s->susc, o, N); UP(prob_care, s->prob_care, o, N);
-    UP(t_inf, s->infection_time, o, N); UP(t_rod, s->recovery_or_death_time, o, N);
-    UP(t_sari, s->sari_time, o, N); UP(t_crit, s->critical_time, o, N); UP(t_step, s->stepdown_time, o, N);
-    UP(t_det, s->detected_time, o, N);
     UP(t_ts, s->treat_start_time, o, N); UP(t_te, s->treat_stop_time, o, N); UP(t_vs, s->vacc_start_time, o, N);
     UP(t_is, s->isolation_start_time, o, N); UP(t_qs, s->quar_start_time, o, N); UP(qc, s->quar_comply, o, N);
     UP(dct, s->dct_traced, o, N);
@@ -830,14 +843,24 @@ int cs_store_state(cs_engine* e, int real, cs_state* s)
             if (s->absent_start_time) s->absent_start_time[i] = x.t_as;
             if (s->absent_stop_time) s->absent_stop_time[i] = x.t_ae;
         }
+        std::vector<cs::Tim> m(N);
+        HIPCHK(e, hipMemcpy(m.data(), d.tim + o, N * sizeof(cs::Tim), hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < N; i++) {
+            const cs::Tim& x = m[i];
+            if (s->infection_time) s->infection_time[i] = x.t_inf;
+            if (s->recovery_or_death_time) s->recovery_or_death_time[i] = x.t_rod;
+            if (s->sari_time) s->sari_time[i] = x.t_sari;
+            if (s->critical_time) s->critical_time[i] = x.t_crit;
+            if (s->stepdown_time) s->stepdown_time[i] = x.t_step;
+            if (s->detected_time) s->detected_time[i] = x.t_det;
+            if (s->severity_current) s->severity_current[i] = x.sev_cur;
+            if (s->severity_final) s->severity_final[i] = x.sev_final;
+            if (s->bits) s->bits[i] = x.bits;
+        }
     }
 #define DN(src, dst, off, n) if (dst) HIPCHK(e, hipMemcpy(dst, d.src + (off), (n) * sizeof(*d.src), hipMemcpyDeviceToHost))
-    DN(sev_cur, s->severity_current, o, N); DN(sev_final, s->severity_final, o, N);
-    DN(bits, s->bits, o, N); DN(listpos, s->listpos, o, N); DN(infector, s->infector, o, N);
+    DN(listpos, s->listpos, o, N); DN(infector, s->infector, o, N);
     DN(susc, s->susc, o, N); DN(prob_care, s->prob_care, o, N);
-    DN(t_inf, s->infection_time, o, N); DN(t_rod, s->recovery_or_death_time, o, N);
-    DN(t_sari, s->sari_time, o, N); DN(t_crit, s->critical_time, o, N); DN(t_step, s->stepdown_time, o, N);
-    DN(t_det, s->detected_time, o, N);
     DN(t_ts, s->treat_start_time, o, N); DN(t_te, s->treat_stop_time, o, N); DN(t_vs, s->vacc_start_time, o, N);
     DN(t_is, s->isolation_start_time, o, N); DN(t_qs, s->quar_start_time, o, N); DN(qc, s->quar_comply, o, N);
     DN(dct, s->dct_traced, o, N);

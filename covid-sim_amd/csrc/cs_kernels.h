@@ -22,6 +22,8 @@ namespace cs {
 #define CS_EV_FULL 0xffffffffu
 // bound on the spatial rejection loop (src/Sweep.cpp:753-854 has none)
 #define CS_MAX_ATTEMPTS (1 << 20)
+// spatial draws per work-list chunk (a wave's 64 lanes share them)
+#define CS_DRAW_CHUNK 256
 
 __device__ __forceinline__ uint64_t ev_key(int infector, int pass, uint32_t sub)
 {
@@ -79,6 +81,19 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v)
     for (int w = 32; w > 0; w >>= 1) v += __shfl_xor(v, w, 64);
     return v;
 }
+__device__ __forceinline__ double wave_sum_d(double v)
+{
+    for (int w = 32; w > 0; w >>= 1) v += __shfl_xor(v, w, 64);
+    return v;
+}
+__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v)
+{
+    for (int w = 32; w > 0; w >>= 1) {
+        const unsigned long long o = __shfl_xor(v, w, 64);
+        v = o > v ? o : v;
+    }
+    return v;
+}
 
 // exclusive scan over a 256-thread block; *total gets the block sum
 __device__ __forceinline__ unsigned block_excl_scan(unsigned v, unsigned* total)
@@ -106,7 +121,7 @@ __device__ __forceinline__ unsigned block_excl_scan(unsigned v, unsigned* total)
 //   seg 0: the latent segment [S, S+L) in chunks of 64 (also tallies L+I)
 //   seg 1: the infectious segment [S+L, S+L+I) in chunks, and the list of
 //          cells with I > 0; with zero_draws, every cell's draw count is cleared
-//   seg 2: the spatial draws [0, spK) in chunks
+//   seg 2: the spatial draws [0, spK) in chunks of CS_DRAW_CHUNK
 // One atomic per block reserves the block's slots; the order of the lists
 // is irrelevant (every consumer's result is independent of it).
 __global__ void __launch_bounds__(256) k_compact(Dev d, int seg, int zero_draws)
@@ -121,7 +136,8 @@ __global__ void __launch_bounds__(256) k_compact(Dev d, int seg, int zero_draws)
         else n = d.spK[rc];
         if (zero_draws) d.spK[rc] = 0;
     }
-    const unsigned nch = (unsigned)((n + 63) >> 6);
+    const unsigned per = (seg == 2) ? CS_DRAW_CHUNK : 64u;  // entries per chunk
+    const unsigned nch = (unsigned)(((unsigned)n + per - 1) / per);
     unsigned tot_ch, tot_cells;
     const unsigned ex_ch = block_excl_scan(nch, &tot_ch);
     const unsigned ex_cell = (seg == 1) ? block_excl_scan(n > 0 ? 1u : 0u, &tot_cells) : 0u;
@@ -135,7 +151,7 @@ __global__ void __launch_bounds__(256) k_compact(Dev d, int seg, int zero_draws)
     const unsigned cap = (seg == 2) ? d.draw_cap : d.seg_cap;
     for (unsigned k = 0; k < nch; k++) {
         const unsigned idx = base_ch + ex_ch + k;
-        if (idx < cap) out[idx] = Chunk{(int32_t)rc, (int32_t)(k * 64)};
+        if (idx < cap) out[idx] = Chunk{(int32_t)rc, (int32_t)(k * per)};
         else atomicOr(d.status, (unsigned)FLAG_OVERFLOW);
     }
     if (seg == 1 && n > 0) d.cells[base_cell + ex_cell] = (int32_t)rc;
@@ -356,25 +372,30 @@ __global__ void __launch_bounds__(256) k_list_move(Dev d, int mode)
 
 // DoInfect (src/Update.cpp:100-195) for infectee p of realisation r, with
 // the first event's infector and type; the list swap is batched.
-__device__ void do_infect(const Dev& d, int r, int p, double t, unsigned ts, int infector, int type)
+// the counter updates of one DoInfect (StateT fields, src/Update.cpp:100-195)
+struct InfectTally {
+    int kind, ag, kw;        // cumItype, cumIa, cumI_keyworker indices
+    double rad2;
+    long long tg, si;        // generation and serial intervals
+    int ntg;                 // 1 when the infector is a person
+};
+
+// DoInfect (src/Update.cpp:100-195) without the counter updates, which it
+// returns in *tl; false when p is not susceptible (no change)
+__device__ bool do_infect_state(const Dev& d, int r, int p, double t, unsigned ts, int infector, int type, InfectTally* tl)
 {
     const cs_params* P = d.P;
     size_t gp = (size_t)r * d.N + p;
-    if (d.inf[gp] != CS_SUSCEPTIBLE) return;
-    Counters* cn = d.cnt + r;
+    if (d.inf[gp] != CS_SUSCEPTIBLE) return false;
     d.inf[gp] = CS_LATENT;
     d.t_inf[gp] = (uint16_t)ts;
     const float2 loc = d.ploc[p];
     double x = (loc.x - d.loc_init[2 * r]);
     double y = (loc.y - d.loc_init[2 * r + 1]);
-    double rad2 = x * x + y * y;
-    int ag = age_group(d, p);
-    cadd(&cn->cumI, 1);
-    cadd(&cn->cumItype[type % CS_INFECT_TYPE_MASK], 1);
-    cadd(&cn->cumIa[ag], 1);
-    atomicAdd(&cn->sumRad2, rad2);
-    atomicMax(&cn->maxRad2_bits, (unsigned long long)double_to_bits(rad2));
-    cadd(&cn->cumI_keyworker[keyw(d, p)], 1);
+    tl->rad2 = x * x + y * y;
+    tl->kind = type % CS_INFECT_TYPE_MASK;
+    tl->ag = age_group(d, p);
+    tl->kw = keyw(d, p);
     uint16_t lat;
     if (P->do_latent) {
         Stream s(d.seed[r], cw0(ts, PS_INFECT), (uint32_t)p, 0);
@@ -386,13 +407,46 @@ __device__ void do_infect(const Dev& d, int r, int p, double t, unsigned ts, int
     } else
         lat = (uint16_t)(t * P->time_steps_per_day);
     d.t_lat[gp] = lat;
+    tl->tg = tl->si = 0;
+    tl->ntg = 0;
     if (infector >= 0) {
         size_t gi = (size_t)r * d.N + infector;
-        cadd(&cn->cumTG, ((int)ts) - ((int)d.t_inf[gi]));
-        cadd(&cn->cumSI, ((int)lat) - ((int)d.t_lat[gi]));
-        cadd(&cn->nTG, 1);
+        tl->tg = ((int)ts) - ((int)d.t_inf[gi]);
+        tl->si = ((int)lat) - ((int)d.t_lat[gi]);
+        tl->ntg = 1;
     }
     batch_add(d, r, d.pc[p], p);
+    return true;
+}
+
+// the indexed counters of a tally (per lane)
+__device__ __forceinline__ void tally_indexed(Counters* cn, const InfectTally& tl)
+{
+    cadd(&cn->cumItype[tl.kind], 1);
+    cadd(&cn->cumIa[tl.ag], 1);
+    cadd(&cn->cumI_keyworker[tl.kw], 1);
+}
+
+__device__ __forceinline__ void tally_scalar(Counters* cn, unsigned long long n, double rad2, unsigned long long rad2max_bits,
+                                             long long tg, long long si, unsigned long long ntg)
+{
+    cadd(&cn->cumI, (long long)n);
+    atomicAdd(&cn->sumRad2, rad2);
+    atomicMax(&cn->maxRad2_bits, rad2max_bits);
+    if (ntg) {
+        cadd(&cn->cumTG, tg);
+        cadd(&cn->cumSI, si);
+        cadd(&cn->nTG, (long long)ntg);
+    }
+}
+
+__device__ void do_infect(const Dev& d, int r, int p, double t, unsigned ts, int infector, int type)
+{
+    InfectTally tl;
+    if (!do_infect_state(d, r, p, t, ts, infector, type, &tl)) return;
+    Counters* cn = d.cnt + r;
+    tally_indexed(cn, tl);
+    tally_scalar(cn, 1, tl.rad2, double_to_bits(tl.rad2), tl.tg, tl.si, (unsigned long long)tl.ntg);
 }
 
 // ---------------------------------------------------------- InfectSweep
@@ -648,7 +702,7 @@ __device__ void cell_spatial_sum(const Dev& d, const StepScalars& sc, int rc, un
 __global__ void __launch_bounds__(CS_SUM_BLOCK) k_cell_spatial_sum(Dev d)
 {
     const StepScalars sc = *d.sc;
-    __shared__ unsigned long long shi[CS_SUM_BLOCK], slo[CS_SUM_BLOCK];
+    __shared__ unsigned long long shi[CS_SUM_BLOCK / 64], slo[CS_SUM_BLOCK / 64];  // per-wave sums
     __shared__ int sK;
     __shared__ double ssis;
     const unsigned ncells = d.wl[WL_CELLS];
@@ -676,18 +730,29 @@ __device__ void cell_spatial_sum(const Dev& d, const StepScalars& sc, int rc, un
     const int j0 = threadIdx.x * per, j1 = min(I, j0 + per);
     Fx acc = {0, 0};
     for (int j = j0; j < j1; j++) acc = fx_add(acc, fx_from(fabs(d.si_buf[sbase + j])));
-    shi[threadIdx.x] = acc.hi;
-    slo[threadIdx.x] = acc.lo;
+    // exclusive scan of the thread sums: 128-bit integer adds are exact and
+    // associative, so this parallel scan equals a sequential pass
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    Fx x = acc;
+    for (int o = 1; o < 64; o <<= 1) {
+        const Fx y = {__shfl_up(x.hi, o, 64), __shfl_up(x.lo, o, 64)};
+        if (lane >= o) x = fx_add(x, y);
+    }
+    if (lane == 63) {
+        shi[w] = x.hi;
+        slo[w] = x.lo;
+    }
+    Fx excl = {__shfl_up(x.hi, 1, 64), __shfl_up(x.lo, 1, 64)};
+    if (lane == 0) excl = Fx{0, 0};
     __syncthreads();
-    if (threadIdx.x == 0) {  // exclusive scan of the thread sums (exact: integer adds)
-        Fx run = {0, 0};
-        for (int t = 0; t < CS_SUM_BLOCK; t++) {
-            Fx v = {shi[t], slo[t]};
-            shi[t] = run.hi;
-            slo[t] = run.lo;
-            run = fx_add(run, v);
-        }
-        const double sis = fx_to_double(run);
+    Fx before = {0, 0}, all = {0, 0};
+    for (int k = 0; k < CS_SUM_BLOCK / 64; k++) {
+        const Fx t = {shi[k], slo[k]};
+        if (k < w) before = fx_add(before, t);
+        all = fx_add(all, t);
+    }
+    if (threadIdx.x == 0) {
+        const double sis = fx_to_double(all);
         int K = 0;
         if (sis > 0) {
             Stream s(d.seed[r], cw0(sc.ts, PS_SP_COUNT), (uint32_t)c, 0);
@@ -699,7 +764,7 @@ __device__ void cell_spatial_sum(const Dev& d, const StepScalars& sc, int rc, un
         atomicAdd(&d.wc->n_inf, (unsigned long long)I);
     }
     __syncthreads();
-    Fx run = {shi[threadIdx.x], slo[threadIdx.x]};
+    Fx run = fx_add(before, excl);
     const int K = sK;
     const float fs = (float)ssis;
     for (int j = j0; j < j1; j++) {
@@ -724,66 +789,98 @@ __global__ void __launch_bounds__(256) k_spatial_draws(Dev d)
     const cs_params* P = d.P;
     const unsigned ts = sc.ts;
     const int lane = threadIdx.x & 63;
-    unsigned long long w_draw = 0, w_susc = 0;
+    unsigned w_draw = 0, w_susc = 0;  // per lane; summed per wave in 64 bits
     for (unsigned ch = grid_wave(); ch < nch; ch += grid_nwaves()) {
         const Chunk e = chunk_at(d.drawl, ch);
         const size_t rc = (size_t)e.rc;
-        const int k = e.first + lane;
-        if (k >= uni(d.spK[rc])) continue;
+        const int kend = min(uni(d.spK[rc]), e.first + CS_DRAW_CHUNK);  // this chunk's draws: [first, kend)
         const int r = e.rc / d.NC, c = e.rc % d.NC;
         const int S = uni(d.cS[rc]), L = uni(d.cL[rc]), I = uni(d.cI[rc]);
         const size_t rN = (size_t)r * d.N;
         const size_t sbase = rN + uni(d.cstart[c]) + S + L;
         const uint64_t seed = uni(d.seed[r]);
-        Stream s(seed, cw0(ts, PS_SP_DRAW), (uint32_t)c, (uint32_t)k);
-        int jj = (I == 1) ? 0 : select_infector(d.ci_buf + sbase, I, s.next());
-        bool closed = (d.ci_buf[sbase + jj] < 0);
-        const int i = d.perm[sbase + jj];
-        const float2 loc_i = d.ploc[i];
-        const int hh_i = d.hh[i];
-        for (int att = 0;; att++) {
-            if (att >= CS_MAX_ATTEMPTS) {  // no live candidate: report, never spin
-                atomicOr(d.status, (unsigned)FLAG_STUCK);
-                break;
-            }
-            w_draw++;
-            int l = select_dest(d, rc, s.next());
-            const size_t rl = (size_t)r * d.NC + l;
-            int slot = (int)(s.next() * ((double)d.S0[rl]));
-            int m = d.perm[rN + d.cstart[l] + slot];
-            const PStat pm = d.pst[m];
-            double d2 = dist2_ll(P, pm.loc, loc_i);
-            double acc = kernel_num(d, d2) / d.max_trans[(size_t)c * d.NC + l];
-            if ((s.next() >= acc) || is_dead_code(d.inf[rN + m])) continue;
-            if ((c != l) || (pm.hh != hh_i)) {
-                double SS = spatial_susc(d, r, m, ts);
-                if (care(d, m) || care(d, i)) SS *= P->care_home_resident_spatial_scaling;
-                if (slot < d.cS[rl]) {
-                    w_susc++;
-                    SS *= person_susc(d, r, m, ts, i);
-                    SS *= P->waifw_spatial[age_group(d, m)][age_group(d, i)];
-                    if (!closed && host_absent(d, rN + m, ts)) {
-                        bool f2 = false;
-                        for (int pt = 0; pt < d.NPT && !f2; pt++) {
-                            int lk = d.plinks[(size_t)m * CS_MAX_PLACE_TYPES + pt];
-                            if (lk >= 0) f2 = place_closed(d, r, pt, lk, ts);
+        // Each lane runs one draw's rejection loop (the attempts of a draw
+        // follow its own stream, so they stay with one lane); a lane whose
+        // draw is over takes the chunk's next draw, so no lane idles while
+        // a slow draw of the wave finishes.
+        int k = e.first + lane;      // this lane's draw (>= kend: none)
+        int next = e.first + 64;     // the chunk's next unassigned draw (wave-uniform)
+        bool fresh = true;
+        Stream s(seed, 0, 0, 0);
+        int i = 0, hh_i = 0, att = 0;
+        bool closed = false;
+        float2 loc_i = make_float2(0.f, 0.f);
+        for (;;) {
+            const bool active = k < kend;
+            if (__ballot(active) == 0) break;
+            bool done = false;
+            if (active) {
+                if (fresh) {  // the draw's infector (src/Sweep.cpp:716-742)
+                    s = Stream(seed, cw0(ts, PS_SP_DRAW), (uint32_t)c, (uint32_t)k);
+                    const int jj = (I == 1) ? 0 : select_infector(d.ci_buf + sbase, I, s.next());
+                    closed = (d.ci_buf[sbase + jj] < 0);
+                    i = d.perm[sbase + jj];
+                    const PStat pi = d.pst[i];
+                    loc_i = pi.loc;
+                    hh_i = pi.hh;
+                    att = 0;
+                    fresh = false;
+                }
+                if (att >= CS_MAX_ATTEMPTS) {  // no live candidate: report, never spin
+                    atomicOr(d.status, (unsigned)FLAG_STUCK);
+                    done = true;
+                } else {  // one attempt (src/Sweep.cpp:753-854)
+                    att++;
+                    w_draw++;
+                    int l = select_dest(d, rc, s.next());
+                    const size_t rl = (size_t)r * d.NC + l;
+                    int slot = (int)(s.next() * ((double)d.S0[rl]));
+                    int m = d.perm[rN + d.cstart[l] + slot];
+                    const PStat pm = d.pst[m];
+                    double d2 = dist2_ll(P, pm.loc, loc_i);
+                    double acc = kernel_num(d, d2) / d.max_trans[(size_t)c * d.NC + l];
+                    if (!((s.next() >= acc) || is_dead_code(d.inf[rN + m]))) {
+                        done = true;
+                        if ((c != l) || (pm.hh != hh_i)) {
+                            double SS = spatial_susc(d, r, m, ts);
+                            if (care(d, m) || care(d, i)) SS *= P->care_home_resident_spatial_scaling;
+                            if (slot < d.cS[rl]) {
+                                w_susc++;
+                                SS *= person_susc(d, r, m, ts, i);
+                                SS *= P->waifw_spatial[age_group(d, m)][age_group(d, i)];
+                                if (!closed && host_absent(d, rN + m, ts)) {
+                                    bool f2 = false;
+                                    for (int pt = 0; pt < d.NPT && !f2; pt++) {
+                                        int lk = d.plinks[(size_t)m * CS_MAX_PLACE_TYPES + pt];
+                                        if (lk >= 0) f2 = place_closed(d, r, pt, lk, ts);
+                                    }
+                                    if (f2) SS *= P->efficacies[2][CS_SPATIAL];
+                                }
+                                if ((SS == 1) || (s.next() < SS))
+                                    if (d.inf[rN + m] == CS_SUSCEPTIBLE)
+                                        add_event(d, ev_key(i, KP_SP, (uint32_t)k), rN + m, i,
+                                                  2 + 2 * CS_MAX_PLACE_TYPES +
+                                                      CS_INFECT_TYPE_MASK * (1 + d.itype[rN + i] / CS_INFECT_TYPE_MASK));
+                            }
                         }
-                        if (f2) SS *= P->efficacies[2][CS_SPATIAL];
                     }
-                    if ((SS == 1) || (s.next() < SS))
-                        if (d.inf[rN + m] == CS_SUSCEPTIBLE)
-                            add_event(d, ev_key(i, KP_SP, (uint32_t)k), rN + m, i,
-                                      2 + 2 * CS_MAX_PLACE_TYPES + CS_INFECT_TYPE_MASK * (1 + d.itype[rN + i] / CS_INFECT_TYPE_MASK));
                 }
             }
-            break;
+            // lanes without a draw take the chunk's next ones, in lane order
+            const bool need = !active || done;
+            const unsigned long long want = __ballot(need);
+            const unsigned below = __builtin_amdgcn_mbcnt_hi((uint32_t)(want >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)want, 0u));
+            if (need) {
+                k = next + (int)below;  // >= kend: the lane stays idle
+                fresh = true;
+            }
+            next += __popcll(want);
         }
     }
-    w_draw = wave_sum(w_draw);
-    w_susc = wave_sum(w_susc);
-    if ((threadIdx.x & 63) == 0 && w_draw) {
-        atomicAdd(&d.wc->n_sp_draw, w_draw);
-        atomicAdd(&d.wc->n_sp_susc, w_susc);
+    const unsigned long long s_draw = wave_sum(w_draw), s_susc = wave_sum(w_susc);
+    if ((threadIdx.x & 63) == 0 && s_draw) {
+        atomicAdd(&d.wc->n_sp_draw, s_draw);
+        atomicAdd(&d.wc->n_sp_susc, s_susc);
     }
 }
 
@@ -803,23 +900,50 @@ __global__ void k_resolve_apply(Dev d)
 {
     if (halted(d)) return;
     const StepScalars sc = *d.sc;
-    unsigned n = min(*d.ev_count, d.ev_cap);
+    const unsigned n = min(*d.ev_count, d.ev_cap);
+    const int lane = threadIdx.x & 63;
     unsigned long long nn = 0;
-    for (unsigned e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
-        size_t gp = (size_t)d.ev_gp[e];
-        unsigned long long key = d.ev_key[e];
-        int r = (int)(gp / d.N), p = (int)(gp % d.N);
-        if (key == d.claim[gp]) {
-            do_infect(d, r, p, sc.t, sc.ts, d.ev_infector[e], d.ev_type[e]);
-            nn++;
+    // whole waves step through the events, so every lane takes part in the
+    // wave reductions of the counters below
+    for (unsigned base = grid_wave() * 64u; base < n; base += grid_nwaves() * 64u) {
+        const unsigned e = base + lane;
+        bool won = false;
+        int r = 0;
+        InfectTally tl;
+        if (e < n) {
+            const size_t gp = (size_t)d.ev_gp[e];
+            const unsigned long long key = d.ev_key[e];
+            r = (int)(gp / d.N);
+            const int p = (int)(gp % d.N);
+            if (key == d.claim[gp]) {
+                won = do_infect_state(d, r, p, sc.t, sc.ts, d.ev_infector[e], d.ev_type[e], &tl);
+                nn++;
+            }
+            if (key == d.lastk[gp]) {
+                d.infector[gp] = d.ev_infector[e];
+                d.itype[gp] = d.ev_type[e];
+            }
         }
-        if (key == d.lastk[gp]) {
-            d.infector[gp] = d.ev_infector[e];
-            d.itype[gp] = d.ev_type[e];
+        // counters: events are appended by whole producing waves, so a wave's
+        // infections nearly always share one realisation; then the scalar
+        // counters take one atomic per wave instead of one per infection
+        const unsigned long long wm = __ballot(won);
+        if (wm == 0) continue;
+        const int r0 = __shfl(r, __ffsll((long long)wm) - 1, 64);
+        if (won) tally_indexed(d.cnt + r, tl);
+        if (__ballot(won && r != r0) == 0) {
+            const double rad2 = wave_sum_d(won ? tl.rad2 : 0.0);
+            const unsigned long long mx = wave_max_u64(won ? double_to_bits(tl.rad2) : 0ull);
+            const long long tg = (long long)wave_sum(won ? (unsigned long long)tl.tg : 0ull);
+            const long long si = (long long)wave_sum(won ? (unsigned long long)tl.si : 0ull);
+            const unsigned long long ntg = wave_sum(won ? (unsigned long long)tl.ntg : 0ull);
+            if (lane == __ffsll((long long)wm) - 1) tally_scalar(d.cnt + r0, __popcll(wm), rad2, mx, tg, si, ntg);
+        } else if (won) {
+            tally_scalar(d.cnt + r, 1, tl.rad2, double_to_bits(tl.rad2), tl.tg, tl.si, (unsigned long long)tl.ntg);
         }
     }
     nn = wave_sum(nn);
-    if ((threadIdx.x & 63) == 0 && nn) atomicAdd(&d.wc->n_new, nn);
+    if (lane == 0 && nn) atomicAdd(&d.wc->n_new, nn);
 }
 
 __global__ void k_resolve_reset(Dev d)

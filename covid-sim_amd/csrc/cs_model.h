@@ -10,8 +10,9 @@
 //
 // The fields the infection kernels read together are packed into records,
 // so that a scattered visit to one person costs one cache line instead of
-// one line per field: Hot per (realisation, person), PStat and PLinks per
-// person, HRec per household, PlRec per place and GRec per place group.  Each packed field keeps an array-like accessor (Fld), so
+// one line per field: Hot and Tim per (realisation, person), PStat and
+// PLinks per person, HRec per household, PlRec per place and GRec per place
+// group.  Each packed field keeps an array-like accessor (Fld), so
 // d.inf[gp] reads the inf field of record gp.
 #pragma once
 #include <hip/hip_runtime.h>
@@ -68,6 +69,12 @@ struct Hot {  // per (realisation, person): 16 bytes, one vector load
     uint16_t t_gate;  // no timer of an infectious person fires before this step
     float infness;    // infectiousness (sign: symptomatic)
 };
+struct Tim {  // per (realisation, person): the disease timers and severity, 16 bytes
+    uint16_t t_inf, t_rod, t_sari, t_crit, t_step, t_det;  // infection, recovery/death, SARI, critical, stepdown, detected
+    uint8_t sev_cur, sev_final;
+    uint8_t bits;  // bit0 to_die, bit1 detected
+    uint8_t pad;
+};
 struct PStat {  // per person, shared by all realisations
     int32_t hh;
     uint8_t age, pflags;
@@ -89,7 +96,7 @@ struct PlRec {  // per place (n_places + 1 records; the last holds the ends)
 struct GRec {  // per group of a place
     int32_t gst, gsz;  // group start within the place's members, group size
 };
-static_assert(sizeof(Hot) == 16 && sizeof(PStat) == 16 && sizeof(HRec) == 8 && sizeof(PLinks) == 32 &&
+static_assert(sizeof(Hot) == 16 && sizeof(Tim) == 16 && sizeof(PStat) == 16 && sizeof(HRec) == 8 && sizeof(PLinks) == 32 &&
               sizeof(PlRec) == 16 && sizeof(GRec) == 8, "record sizes");
 
 // one field of an array of records, indexed like a plain array
@@ -167,10 +174,19 @@ struct Dev {
     CS_FLD(Hot, t_as) t_as;
     CS_FLD(Hot, t_ae) t_ae;
     CS_FLD(Hot, t_gate) t_gate;
-    uint8_t *sev_cur, *sev_final, *bits, *qc, *dct;
+    Tim* tim;             // [R][N]
+    CS_FLD(Tim, t_inf) t_inf;
+    CS_FLD(Tim, t_rod) t_rod;
+    CS_FLD(Tim, t_sari) t_sari;
+    CS_FLD(Tim, t_crit) t_crit;
+    CS_FLD(Tim, t_step) t_step;
+    CS_FLD(Tim, t_det) t_det;
+    CS_FLD(Tim, sev_cur) sev_cur;
+    CS_FLD(Tim, sev_final) sev_final;
+    CS_FLD(Tim, bits) bits;
+    uint8_t *qc, *dct;
     int32_t *listpos, *infector;
     float *susc, *prob_care;
-    uint16_t *t_inf, *t_rod, *t_sari, *t_crit, *t_step, *t_det;
     uint16_t *t_ts, *t_te, *t_vs, *t_is, *t_qs;
     int32_t *cS, *cL, *cI, *cR, *cD, *cumTC, *S0;
     float* tot_prob;

@@ -4,12 +4,13 @@ CPU: the likelihood is CalcLikelihood's negative binomial term
 (src/CovidSim.cpp:3778-3784), checked against scipy's negative binomial
 pmf with the same parameters; the pattern search finds the maximum of a
 smooth test function; the crest refinement recovers the maximum and slope
-of a sheared quadratic ridge from starts anywhere in its basin, and returns
-identical fits from starts that snap to the same design.
+of a sheared quadratic ridge from starts anywhere in its basin, and starts
+across the basin agree on a rough ridge; both calibration drivers run end to
+end on a stand-in simulator and agree on R0.
 GPU: on the 20k-person fixture region, an emulator trained on simulator
-ensembles guides calibration to a simulator-verified likelihood within 2
-log-likelihood units of plain calibration's with fewer simulator launches
--- what the emulator gains over the method without it.
+ensembles guides calibration to a fit whose simulator likelihood is within
+2 units of plain calibration's, with at most a third of its simulator
+launches -- what the emulator gains over the method without it.
 """
 import importlib
 
@@ -67,15 +68,16 @@ def test_refine_converges_on_the_crest_from_any_start_in_the_basin(start):
     assert k <= 3
 
 
-def test_refine_is_deterministic_on_its_lattice():
-    """Starts in the same basin end on the same snapped design, so they
-    evaluate the same final candidates and return the same fit, even on a
-    rough objective."""
-    rough = lambda th: ridge(th) + 2.0 * np.sin(97.0 * th[:, 0]) * np.cos(41.0 * th[:, 1])
-    a = cal.refine(rough, 2.161, 0.17, -3.1)
-    b = cal.refine(rough, 2.170, 0.19, -3.4)
-    assert np.array_equal(a[0], b[0]), (a, b)
-    assert abs(a[0][0] - 2.16) < 0.01
+@pytest.mark.parametrize("amp,tol", [(1.0, 0.01), (2.0, 0.01), (4.0, 0.02)])
+def test_refine_is_robust_to_rough_objectives(amp, tol):
+    """On a ridge with roughness of `amp` log-likelihood units (a 16-seed
+    mean's is about 2), starts across the basin agree on R0 within tol: the
+    broad design averages the roughness out."""
+    rough = lambda th: ridge(th) + amp * np.sin(97.0 * th[:, 0]) * np.cos(41.0 * th[:, 1])
+    fits = [cal.refine(rough, *st)[0][0] for st in
+            [(2.161, 0.17, -3.1), (2.170, 0.19, -3.4), (2.12, 0.3, -3.0), (2.2, 0.05, -3.75)]]
+    assert max(fits) - min(fits) < tol, fits
+    assert abs(np.mean(fits) - 2.16) < tol, fits
 
 
 def test_crest_from_grid():
@@ -85,6 +87,44 @@ def test_crest_from_grid():
     c, l, slope = cal.crest_from_grid(r0, ls, ridge(th).reshape(len(r0), len(ls)))
     assert abs(c - 2.2) < 1e-9  # the grid row nearest the maximum
     assert abs(l - (0.17 - 3.25 * 0.04)) < 0.05 and abs(slope + 3.25) < 0.5
+
+
+class CurveSim:
+    """Stand-in simulator: a deterministic death curve whose timing trades
+    R0 against the seeding scale (the likelihood ridge of the real one)."""
+
+    launches = 0
+
+    def run(self, theta, seeds):
+        theta = np.atleast_2d(theta)
+        self.launches += -(-len(theta) * len(seeds) // 64)
+        t = np.arange(cal.DAYS)[None, :]
+        r = (theta[:, :1] - 1.0) / 5.0                  # growth rate per day
+        peak = 40.0 + (np.log(1e4) - theta[:, 1:] * np.log(10.0) * 1.0) / r
+        width = 1.2 / r
+        deaths = 6000.0 * (theta[:, :1] - 1.0) / 1.15 * np.exp(-0.5 * ((t - peak) / width) ** 2) / (width * 2.5066)
+        return deaths * 0.5, deaths
+
+
+class ExactEmulator:
+    def forward(self, x):
+        th = np.column_stack([x[:, 0] * 0.35 + 2.4, x[:, 1] * 0.4])
+        inc, d = CurveSim().run(th, [0])
+        return cal.from_curves(inc, d)
+
+
+def test_calibrations_agree_on_a_stand_in_simulator():
+    """Both drivers run end to end on the stand-in: the plain grid and the
+    emulator's dense grid start the same crest refinement, which returns
+    the same fit; the emulator-guided one uses far fewer launches."""
+    sim = CurveSim()
+    target = np.round(sim.run(np.array([[2.15, 0.25]]), [0])[1][0])
+    seeds = np.arange(16)
+    plain = cal.calibrate_plain(sim, target, seeds)
+    emul = cal.calibrate_emulated(sim, ExactEmulator(), target, seeds)
+    assert abs(plain["theta"][0] - 2.15) < 0.02 and abs(emul["theta"][0] - 2.15) < 0.02, (plain, emul)
+    assert abs(plain["theta"][0] - emul["theta"][0]) < 0.02
+    assert plain["launches"] >= 5 * emul["launches"], (plain["launches"], emul["launches"])
 
 
 def test_encoding_round_trip():
@@ -116,16 +156,17 @@ def test_emulator_guided_calibration_uses_fewer_launches():
 
     sim = MiniSim()
     rng = np.random.default_rng(1)
-    th, y = cal.training_set(sim, 128, np.arange(1000, 1004, dtype=np.uint64), rng)
-    emu, losses = cal.train_emulator(cal.encode(th), y, epochs=200, batch=64)
+    th, y = cal.training_set(sim, 256, np.arange(1000, 1004, dtype=np.uint64), rng)
+    emu, losses = cal.train_emulator(cal.encode(th), y, epochs=600, batch=64)
     assert losses[-1] < 0.1 * losses[0]
     _, d = sim.run(np.array([[2.2, 0.1]]), np.arange(5000, 5016, dtype=np.uint64))
     target = np.round(d[0])
     seeds = np.arange(100, 116, dtype=np.uint64)
     plain = cal.calibrate_plain(sim, target, seeds)
     emul = cal.calibrate_emulated(sim, emu, target, seeds)
-    # deaths of a 20k-person region leave a ridge of near-equal likelihood
-    # along R0 x seeding scale, so the two searches are compared by the
-    # simulator-verified likelihood they reach, not by position on the ridge
-    assert emul["loglik"] >= plain["loglik"] - 2.0, (plain, emul)
-    assert emul["launches"] < plain["launches"], (plain, emul)
+    # deaths of a 20k-person region leave a flat, rough ridge of likelihood
+    # along R0 x seeding scale, so the two fits are compared by the
+    # simulator's likelihood at each (same seeds, one launch)
+    f = cal.nb_loglik(target, sim.run(np.array([plain["theta"], emul["theta"]]), seeds)[1])
+    assert f[1] >= f[0] - 2.0, (plain, emul, f)
+    assert emul["launches"] * 3 <= plain["launches"], (plain, emul)

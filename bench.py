@@ -50,12 +50,12 @@ HBM_PEAK_GBS = 8000.0
 # algorithmic-bytes formula); fixed, never tuned to the measurement
 BYTES = {"n_inf": 64, "n_hh_inf": 16, "n_hh_member": 40, "n_place_link": 32, "n_place_cand": 44,
          "n_sp_draw": 48, "n_sp_susc": 40, "n_new": 40, "n_active": 16, "n_transition": 30,
-         "n_cell_steps": 32}
+         "n_cell_steps": 32, "n_sp_scan": 4, "n_sp_probe": 4}
 KERNEL_UNITS = {
     "infect_households": ["n_inf", "n_hh_inf", "n_hh_member"],
     "infect_places": ["n_place_link", "n_place_cand"],
     "cell_spatial_sum": ["n_cell_steps"],
-    "spatial_draws": ["n_sp_draw", "n_sp_susc"],
+    "spatial_draws": ["n_sp_draw", "n_sp_susc", "n_sp_scan", "n_sp_probe"],
     "resolve": ["n_new"],
     "latent_scan": ["n_active"],
     "infectious_scan": ["n_transition"],

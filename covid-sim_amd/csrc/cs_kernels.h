@@ -460,13 +460,15 @@ __device__ __forceinline__ double beta_s_of(const Dev& d, const StepScalars& sc,
 }
 
 // the reference's infector binary search (src/Sweep.cpp:716-740)
-__device__ __forceinline__ int select_infector(const float* ci, int n, double u)
+__device__ __forceinline__ int select_infector(const float* ci, int n, double u, unsigned* probes = nullptr)
 {
     if (n == 1) return 0;
     int idx, step;
     idx = step = n / 2;
     int keep = 1;
+    unsigned np = 0;
     do {
+        np++;
         if (step > 1) step /= 2;
         if ((idx > 0) && (__builtin_fabs((double)ci[idx - 1]) >= u)) {
             idx -= step;
@@ -476,15 +478,18 @@ __device__ __forceinline__ int select_infector(const float* ci, int n, double u)
             if (idx == n - 1) keep = 0;
         } else keep = 0;
     } while (keep);
+    if (probes) *probes += np;
     return idx;
 }
 
 // destination cell: InvCDF then linear scan of cum_trans (src/Sweep.cpp:757-762)
-__device__ __forceinline__ int select_dest(const Dev& d, size_t rc, double u)
+__device__ __forceinline__ int select_dest(const Dev& d, size_t rc, double u, unsigned* scanned = nullptr)
 {
     int l = d.invcdf[rc * 1025 + (int)__builtin_floor(u * 1024)];
     const float* ct = d.cum_trans + rc * d.NC;
+    const int l0 = l;
     while (ct[l] < u) l++;
+    if (scanned) *scanned += (unsigned)(l - l0 + 1);
     return l;
 }
 
@@ -789,7 +794,7 @@ __global__ void __launch_bounds__(256) k_spatial_draws(Dev d)
     const cs_params* P = d.P;
     const unsigned ts = sc.ts;
     const int lane = threadIdx.x & 63;
-    unsigned w_draw = 0, w_susc = 0;  // per lane; summed per wave in 64 bits
+    unsigned w_draw = 0, w_susc = 0, w_scan = 0, w_probe = 0;  // per lane; summed per wave in 64 bits
     for (unsigned ch = grid_wave(); ch < nch; ch += grid_nwaves()) {
         const Chunk e = chunk_at(d.drawl, ch);
         const size_t rc = (size_t)e.rc;
@@ -817,7 +822,7 @@ __global__ void __launch_bounds__(256) k_spatial_draws(Dev d)
             if (active) {
                 if (fresh) {  // the draw's infector (src/Sweep.cpp:716-742)
                     s = Stream(seed, cw0(ts, PS_SP_DRAW), (uint32_t)c, (uint32_t)k);
-                    const int jj = (I == 1) ? 0 : select_infector(d.ci_buf + sbase, I, s.next());
+                    const int jj = (I == 1) ? 0 : select_infector(d.ci_buf + sbase, I, s.next(), &w_probe);
                     closed = (d.ci_buf[sbase + jj] < 0);
                     i = d.perm[sbase + jj];
                     const PStat pi = d.pst[i];
@@ -832,14 +837,16 @@ __global__ void __launch_bounds__(256) k_spatial_draws(Dev d)
                 } else {  // one attempt (src/Sweep.cpp:753-854)
                     att++;
                     w_draw++;
-                    int l = select_dest(d, rc, s.next());
+                    int l = select_dest(d, rc, s.next(), &w_scan);
                     const size_t rl = (size_t)r * d.NC + l;
+                    const float mt = d.max_trans[(size_t)c * d.NC + l];  // issued with the slot's loads
                     int slot = (int)(s.next() * ((double)d.S0[rl]));
                     int m = d.perm[rN + d.cstart[l] + slot];
                     const PStat pm = d.pst[m];
+                    const int8_t inf_m = d.inf[rN + m];  // read with the candidate's record, not after the test
                     double d2 = dist2_ll(P, pm.loc, loc_i);
-                    double acc = kernel_num(d, d2) / d.max_trans[(size_t)c * d.NC + l];
-                    if (!((s.next() >= acc) || is_dead_code(d.inf[rN + m]))) {
+                    double acc = kernel_num(d, d2) / mt;
+                    if (!((s.next() >= acc) || is_dead_code(inf_m))) {
                         done = true;
                         if ((c != l) || (pm.hh != hh_i)) {
                             double SS = spatial_susc(d, r, m, ts);
@@ -857,7 +864,7 @@ __global__ void __launch_bounds__(256) k_spatial_draws(Dev d)
                                     if (f2) SS *= P->efficacies[2][CS_SPATIAL];
                                 }
                                 if ((SS == 1) || (s.next() < SS))
-                                    if (d.inf[rN + m] == CS_SUSCEPTIBLE)
+                                    if (inf_m == CS_SUSCEPTIBLE)
                                         add_event(d, ev_key(i, KP_SP, (uint32_t)k), rN + m, i,
                                                   2 + 2 * CS_MAX_PLACE_TYPES +
                                                       CS_INFECT_TYPE_MASK * (1 + d.itype[rN + i] / CS_INFECT_TYPE_MASK));
@@ -878,9 +885,12 @@ __global__ void __launch_bounds__(256) k_spatial_draws(Dev d)
         }
     }
     const unsigned long long s_draw = wave_sum(w_draw), s_susc = wave_sum(w_susc);
+    const unsigned long long s_scan = wave_sum(w_scan), s_probe = wave_sum(w_probe);
     if ((threadIdx.x & 63) == 0 && s_draw) {
         atomicAdd(&d.wc->n_sp_draw, s_draw);
         atomicAdd(&d.wc->n_sp_susc, s_susc);
+        atomicAdd(&d.wc->n_sp_scan, s_scan);
+        atomicAdd(&d.wc->n_sp_probe, s_probe);
     }
 }
 

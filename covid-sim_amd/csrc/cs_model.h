@@ -57,6 +57,7 @@ enum { WL_SEG = 0, WL_CELLS, WL_DRAWS, WL_MOVED, WL_N };
 struct Work {  // event tallies for the algorithmic-bytes figure
     unsigned long long n_inf, n_hh_inf, n_hh_member, n_place_link, n_place_cand;
     unsigned long long n_sp_draw, n_sp_susc, n_new, n_active, n_transition, n_cells;
+    unsigned long long n_sp_scan, n_sp_probe;
 };
 
 enum { FLAG_OVERFLOW = 1, FLAG_RANGE = 2, FLAG_BATCH = 4, FLAG_STUCK = 8 };

@@ -306,6 +306,8 @@ typedef struct cs_counters {
 typedef struct cs_work_counts {
     int64_t n_inf, n_hh_inf, n_hh_member, n_place_link, n_place_cand;
     int64_t n_sp_draw, n_sp_susc, n_new, n_active, n_transition, n_cells;
+    int64_t n_sp_scan;   /* cum_trans entries read by the destination-cell scans */
+    int64_t n_sp_probe;  /* cumulative-infectiousness entries read by the infector searches */
 } cs_work_counts;
 
 typedef struct cs_engine cs_engine;

@@ -1044,13 +1044,14 @@ static uint64_t ev_key(int infector, int pass, uint32_t sub)
 }
 
 /* the reference's infector binary search (src/Sweep.cpp:716-740) */
-static int select_infector(const float* ci, int n, double u)
+static int select_infector_n(const float* ci, int n, double u, int64_t* probes)
 {
     if (n == 1) return 0;
     int idx, step;
     idx = step = n / 2;
     int keep = 1;
     do {
+        if (probes) (*probes)++;
         if (step > 1) step /= 2;
         if ((idx > 0) && (fabs(ci[idx - 1]) >= u)) {
             idx -= step;
@@ -1062,14 +1063,20 @@ static int select_infector(const float* ci, int n, double u)
     } while (keep);
     return idx;
 }
+static int select_infector(const float* ci, int n, double u) { return select_infector_n(ci, n, u, NULL); }
 
-static int select_dest(const or_engine* e, const or_real* r, int c, double u)
+/* InvCDF bucket, then the linear scan of cum_trans (src/Sweep.cpp:757-762);
+ * *scanned counts the cum_trans entries read */
+static int select_dest_n(const or_engine* e, const or_real* r, int c, double u, int64_t* scanned)
 {
     int l = r->invcdf[(size_t)c * 1025 + (int)floor(u * 1024)];
     const float* ct = r->cum_trans + (size_t)c * e->NC;
+    const int l0 = l;
     while (ct[l] < u) l++;
+    if (scanned) *scanned += l - l0 + 1;
     return l;
 }
+static int select_dest(const or_engine* e, const or_real* r, int c, double u) { return select_dest_n(e, r, c, u, NULL); }
 
 /* resolve this step's events of realisation r (src/Sweep.cpp:860-880) */
 static void resolve(or_engine* e, or_real* r, double t, unsigned ts)
@@ -1236,13 +1243,13 @@ int or_infect_sweep(or_engine* e, double t)
                 for (int k = 0; k < K; k++) {
                     rs d;
                     rs_init(&d, r->seed, cw0(ts, PS_SP_DRAW), (uint32_t)c, (uint32_t)k);
-                    int jj = (I == 1) ? 0 : select_infector(e->cell_inf, I, rs_next(&d));
+                    int jj = (I == 1) ? 0 : select_infector_n(e->cell_inf, I, rs_next(&d), &e->wc.n_sp_probe);
                     int closed = (e->cell_inf[jj] < 0);
                     const int i = r->perm[base + S + L + jj];
                     for (int att = 0;; att++) {
                         if (att >= OR_MAX_ATTEMPTS) { e->error = CS_ERANGE; snprintf(e->err, sizeof(e->err), "spatial draw found no live candidate"); break; }
                         e->wc.n_sp_draw++;
-                        int l = select_dest(e, r, c, rs_next(&d));
+                        int l = select_dest_n(e, r, c, rs_next(&d), &e->wc.n_sp_scan);
                         int slot = (int)(rs_next(&d) * ((double)r->S0[l]));
                         int m = r->perm[e->cstart[l] + slot];
                         double d2 = dist2_pp(e, m, i);

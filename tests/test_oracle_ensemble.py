@@ -16,6 +16,10 @@ Bonferroni-corrected over the days (family-wise level 0.01; 120
 uncorrected tests at 0.01 would reject a true null in 70% of suites).
 A 1000-versus-1000 run of the same comparison (not part of the suite)
 gave |z| < 1.8 for every statistic.
+
+On the GPU the same 200 seeds, advanced together by one batched run, must
+give the oracle's 200 trajectories bit for bit and pass the same tests
+against the reference's 200.
 """
 import ctypes as C
 import functools
@@ -104,3 +108,30 @@ def test_infection_routes():
         if ref_it[:, q].sum() == 0 and orc_it[:, q].sum() == 0:
             continue
         assert welch_ci_contains_zero(ref_it[:, q], orc_it[:, q], level=0.999), (q, ref_it[:, q].mean(), orc_it[:, q].mean())
+
+
+@pytest.mark.gpu
+def test_gpu_ensemble_is_the_oracle_ensemble_and_matches_the_reference():
+    """The GPU's 200 realisations, advanced together by the batched run,
+    reproduce the oracle's 200 bit for bit, and pass the same ensemble tests
+    against the reference's 200 (summaries by KS and overlapping 95% CIs,
+    daily incidence by per-day KS)."""
+    import importlib
+    ref, orc, n = ensembles()
+    arrays, pop, params, _ = mini()
+    g = importlib.import_module("covid-sim_amd.engine").Engine(0)
+    g.set_params(params)
+    g.set_population(pop)
+    g.alloc(np.arange(1, N_RUNS + 1, dtype=np.uint64) * np.uint64(2654435761))
+    g.reset(10)
+    days = ref.shape[1]
+    tspd = int(params.time_steps_per_day)
+    g.run_steps(0.0, days * tspd, tspd)
+    gpu = g.timeseries(days)
+    assert gpu.shape == orc.shape and np.array_equal(gpu, orc)
+    for idx, name in enumerate(SUMMARIES):
+        a, b = summary(ref, n)[idx].astype(float), summary(gpu, n)[idx].astype(float)
+        assert stats.ks_2samp(a, b).pvalue > ALPHA and cis_overlap(a, b), (name, a.mean(), b.mean())
+    for field in (5, 6):
+        pv = np.array([stats.ks_2samp(ref[:, d, field], gpu[:, d, field]).pvalue for d in range(days)])
+        assert pv.min() > ALPHA / days, (field, pv.argmin(), pv.min())

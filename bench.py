@@ -464,7 +464,7 @@ def calibration(args):
         "plain": plain, "emulator_guided": emul, "local_pattern_search": local,
         "hidden_theta": hidden.tolist(),
         "fitted_r0_difference": abs(plain["theta"][0] - emul["theta"][0]),
-        "launch_ratio": plain["launches"] / max(emul["launches"], 1),
+        "realisation_ratio": plain["realisations"] / max(emul["realisations"], 1),
         "training_data_seconds": t_data, "training_seconds": t_train,
         "emulator_heldout_rel_l2": err,
     }

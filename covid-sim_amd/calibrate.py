@@ -10,9 +10,8 @@ of a GPU ensemble (16 seeds, common random numbers across candidates), and
 two searches are compared:
 
   plain      a coarse grid over the whole (R0, log10 seeding scale) box,
-             four candidates per 64-realisation launch, then a quadratic fit
-             over a design of 15 candidates sheared along the likelihood's
-             crest (refine), started from the grid's crest;
+             four candidates per 64-realisation launch, then the crest
+             refinement (crest_profile) started from the grid's crest;
   emulator   the trained emulator scores a dense grid in one batched forward
              pass; the simulator verifies and refines the crest it proposes
              with the same refinement.
@@ -102,6 +101,7 @@ class Simulator:
         self.r0_hh, self.r0_pl, self.sp_r0 = c.r0_household, c.r0_places, c.spatial_r0
         self.n_real = n_real
         self.launches = 0
+        self.realisations = 0
         self.sim_seconds = 0.0
 
     def beta(self, r0):
@@ -117,11 +117,10 @@ class Simulator:
         dth = np.zeros((len(theta), DAYS))
         tspd = int(round(self.params.time_steps_per_day))
         for b0 in range(0, len(jobs), self.n_real):
-            batch = jobs[b0:b0 + self.n_real]
-            pad = batch + [batch[-1]] * (self.n_real - len(batch))
+            batch = jobs[b0:b0 + self.n_real]  # the last launch holds only what is left
             t0 = time.perf_counter()
-            self.eng.alloc(np.array([s for _, s in pad], np.uint64))
-            th = theta[[i for i, _ in pad]]
+            self.eng.alloc(np.array([s for _, s in batch], np.uint64))
+            th = theta[[i for i, _ in batch]]
             nseed = np.maximum(1, (10.0 * 10.0 ** th[:, 1] + 0.5).astype(np.int32))
             self.eng.set_realisation_params(self.beta(th[:, 0]), nseed)
             self.eng.reset(10)
@@ -129,6 +128,7 @@ class Simulator:
             ts = self.eng.timeseries(DAYS)
             self.sim_seconds += time.perf_counter() - t0
             self.launches += 1
+            self.realisations += len(batch)
             if self.progress and self.launches % 8 == 0:
                 self.progress("%d simulator launches, %.1f s" % (self.launches, self.sim_seconds))
             for k, (i, _) in enumerate(batch):
@@ -162,72 +162,70 @@ def pattern_search(score, start, step, min_step=0.01, max_evals=400, fstart=None
 
 # Local refinement shared by both calibrations.  The log-likelihood has a
 # long diagonal ridge in (R0, log10 scale): more seeds with a lower R0 give a
-# similar death curve.  Along the crest it is flat (a few units over +-0.05
-# in R0) and rough at the 1-2 unit level (16-seed means, integer seed
-# counts), so a small stencil cannot place its maximum.  The refinement
-# therefore fits a quadratic over a broad design sheared along the crest:
-# u = R0 - R0c along it (+-0.12), v = log10 scale - (ls_c + slope * u)
-# across it.  The design's centre and slope are snapped to a lattice, so two
-# calibrations that start in the same basin evaluate the same candidates
-# (with common random numbers the simulator's likelihood is a deterministic
-# function) and return the same fit.
-DESIGN_U = np.array([-0.12, -0.06, 0.0, 0.06, 0.12])
-DESIGN_V = np.array([-0.06, 0.0, 0.06])
-DESIGN = np.array([(u, v) for u in DESIGN_U for v in DESIGN_V])
-LATTICE = np.array([0.04, 0.05])   # centre: R0, log10 scale
-SLOPE_LATTICE = 0.5
+# similar death curve.  Along the crest it is flat (within ~3 units over
+# +-0.05 in R0) and rough at the 1-2 unit level (16-seed means, integer seed
+# counts); it is not quadratic over a design wide enough to average the
+# roughness out.  So the refinement fits the two directions apart: across the
+# crest a parabola through three points of each R0 column gives the column's
+# maximum, and along it a parabola through the column maxima gives R0.
+# Every candidate lies on one lattice (R0 step 0.025, log10 scale step
+# 0.05), and the window of columns re-centres on the lattice column nearest
+# its fitted maximum until it stays: with common random numbers the
+# simulator's likelihood is a deterministic function, so two calibrations
+# that start in the same basin evaluate the same final window and return the
+# same fit.
+LAT_R0, LAT_LS = 0.025, 0.05
+PROFILE_HALF = 2   # columns on each side of the window's centre
 
 
-def _snap(x, q):
-    return np.round(np.asarray(x, np.float64) / q) * q
+def crest_profile(score, r0, ls, slope, half=PROFILE_HALF, max_windows=8):
+    """The crest refinement from a crest estimate: ls + slope (R0 - r0).
+    Returns (R0, log10 scale) of the fitted maximum, the best evaluated
+    log-likelihood, the windows fitted and the candidates evaluated."""
+    cache = {}
 
+    def ev(keys):  # lattice indices (i, j): R0 = i LAT_R0, log10 scale = j LAT_LS
+        new = [k for k in dict.fromkeys(keys) if k not in cache]
+        if new:
+            f = score(np.array([(i * LAT_R0, j * LAT_LS) for i, j in new]))
+            cache.update(zip(new, (float(x) for x in f)))
 
-def crest_fit(z, f):
-    """Least-squares quadratic a + b u + c v + d u^2 + e v^2 + g u v through f
-    at design points z [n, 2]; returns (u*, v*) of its maximum and the crest
-    slope dv*/du = -g / (2e), or None when the fit is not concave."""
-    u, v = z[:, 0], z[:, 1]
-    a = np.column_stack([np.ones_like(u), u, v, u * u, v * v, u * v])
-    c = np.linalg.lstsq(a, f, rcond=None)[0]
-    hess = np.array([[2 * c[3], c[5]], [c[5], 2 * c[4]]])
-    if np.any(np.linalg.eigvalsh(hess) >= 0):
-        return None
-    return np.linalg.solve(hess, -c[1:3]), -c[5] / (2 * c[4])
-
-
-def refine(score, r0, ls, slope, max_designs=3):
-    """Fit the sheared design around (r0, ls) with the given crest slope; if
-    the fitted maximum lies outside the design's inner half, move the design
-    there (snapped) with the fitted slope and fit again.  Returns the fitted
-    (R0, log10 scale) -- the best evaluated candidate when max_designs pass
-    without settling -- the best evaluated log-likelihood and the designs used."""
-    box = np.array([R0_RANGE, LOGS_RANGE])
-    c = _snap([r0, ls], LATTICE)
-    s = float(_snap(slope, SLOPE_LATTICE))
-    best, best_pt = -np.inf, c
-    for k in range(1, max_designs + 1):
-        pts = np.column_stack([c[0] + DESIGN[:, 0], c[1] + s * DESIGN[:, 0] + DESIGN[:, 1]])
-        pts = np.clip(pts, box[:, 0], box[:, 1])
-        f = score(pts)
-        j = int(np.argmax(f))
-        if f[j] > best:
-            best, best_pt = float(f[j]), pts[j]
-        fit = crest_fit(np.column_stack([pts[:, 0] - c[0], pts[:, 1] - c[1] - s * (pts[:, 0] - c[0])]), f)
-        if fit is None:
-            c = _snap(pts[j], LATTICE)
-            continue
-        (u, v), ds = fit
-        u, v = float(np.clip(u, -0.2, 0.2)), float(np.clip(v, -0.2, 0.2))
-        theta = np.clip([c[0] + u, c[1] + s * u + v], box[:, 0], box[:, 1])
-        c_new = _snap(theta, LATTICE)
-        s_new = float(_snap(s + ds, SLOPE_LATTICE))
-        # done when the maximum lies in the design's inner part, or when the
-        # moved design would be this one (a fixed point of the snapping)
-        if (abs(u) <= 0.06 and abs(v) <= 0.06) or (np.allclose(c_new, c) and s_new == s):
-            return theta, best, k
-        c, s = c_new, s_new
-    # not settled: the best candidate the designs evaluated
-    return np.asarray(best_pt, np.float64), best, max_designs
+    ci = int(round(r0 / LAT_R0))
+    seen = set()
+    est = (r0, ls)
+    for k in range(1, max_windows + 1):
+        cols = list(range(ci - half, ci + half + 1))
+        js = {i: int(round((ls + slope * (i * LAT_R0 - r0)) / LAT_LS)) for i in cols}
+        for _ in range(6):  # bracket every column's maximum on the lattice, all columns per batch
+            ev([(i, js[i] + d) for i in cols for d in (-1, 0, 1)])
+            moved = False
+            for i in cols:
+                y = [cache[(i, js[i] + d)] for d in (-1, 0, 1)]
+                if not (y[1] >= y[0] and y[1] >= y[2]):
+                    js[i] += 1 if y[2] > y[1] else -1
+                    moved = True
+            if not moved:
+                break
+        prof = []
+        for i in cols:
+            y0, y1, y2 = (cache[(i, js[i] + d)] for d in (-1, 0, 1))
+            den = y0 - 2 * y1 + y2
+            off = 0.5 * (y0 - y2) / den if den < 0 else 0.0
+            prof.append((i * LAT_R0, (js[i] + off) * LAT_LS, y1 - 0.25 * (y0 - y2) * off))
+        prof = np.array(prof)
+        p = np.polyfit(prof[:, 0] - ci * LAT_R0, prof[:, 2], 2)
+        u = -p[1] / (2 * p[0]) if p[0] < 0 else prof[int(np.argmax(prof[:, 2])), 0] - ci * LAT_R0
+        u = float(np.clip(u, -half * LAT_R0, half * LAT_R0))
+        r0 = ci * LAT_R0 + u
+        q = np.polyfit(prof[:, 0], prof[:, 1], 1)  # the crest through the columns' maxima
+        ls, slope = float(np.polyval(q, r0)), float(q[0])
+        est = (r0, ls)
+        ni = int(round(r0 / LAT_R0))
+        if ni == ci or ni in seen:
+            break
+        seen.add(ci)
+        ci = ni
+    return np.array(est), max(cache.values()), k, len(cache)
 
 
 def crest_from_grid(r0, ls, ll, around=None, halfwidth=0.1):
@@ -257,15 +255,16 @@ def calibrate_plain(sim, target_deaths, seeds, grid=(13, 8)):
     """Simulator only: a coarse grid over the whole box (R0 step 0.1, log10
     scale step 0.2), then the crest refinement from the grid's crest."""
     t0 = time.perf_counter()
-    l0 = sim.launches
+    l0, r_0 = sim.launches, sim.realisations
     score = _score(sim, target_deaths, seeds)
     r0 = np.linspace(*R0_RANGE, grid[0])
     ls = np.linspace(*LOGS_RANGE, grid[1])
     f = score(np.array([(a, b) for a in r0 for b in ls])).reshape(len(r0), len(ls))
     c0, l0s, slope = crest_from_grid(r0, ls, f, halfwidth=0.1)
-    x, fx, k = refine(score, c0, l0s, slope)
-    return {"theta": x.tolist(), "loglik": fx, "start": [c0, l0s, slope], "candidates": f.size + k * len(DESIGN),
-            "designs": k, "launches": sim.launches - l0, "seconds": time.perf_counter() - t0}
+    x, fx, k, n = crest_profile(score, c0, l0s, slope)
+    return {"theta": x.tolist(), "loglik": fx, "start": [c0, l0s, slope], "candidates": f.size + n,
+            "windows": k, "launches": sim.launches - l0, "realisations": sim.realisations - r_0,
+            "seconds": time.perf_counter() - t0}
 
 
 def calibrate_pattern(sim, target_deaths, seeds, start=(2.4, 0.0), step=(0.4, 0.4)):
@@ -285,7 +284,7 @@ def calibrate_emulated(sim, emu, target_deaths, seeds, grid=(121, 57)):
     plain calibration (more designs when the proposal is off: the plain
     simulator as fallback)."""
     t0 = time.perf_counter()
-    l0 = sim.launches
+    l0, r_0 = sim.launches, sim.realisations
     r0 = np.linspace(*R0_RANGE, grid[0])
     ls = np.linspace(*LOGS_RANGE, grid[1])
     th = np.array([(a, b) for a in r0 for b in ls])
@@ -293,10 +292,10 @@ def calibrate_emulated(sim, emu, target_deaths, seeds, grid=(121, 57)):
     ll = nb_loglik(target_deaths, d_pred).reshape(len(r0), len(ls))
     c0, l0s, slope = crest_from_grid(r0, ls, ll, halfwidth=0.1)
     t_emul = time.perf_counter() - t0
-    x, fx, k = refine(_score(sim, target_deaths, seeds), c0, l0s, slope)
-    return {"theta": x.tolist(), "loglik": fx, "proposal": [c0, l0s, slope], "candidates": k * len(DESIGN),
-            "designs": k, "launches": sim.launches - l0, "seconds": time.perf_counter() - t0,
-            "emulator_seconds": t_emul}
+    x, fx, k, n = crest_profile(_score(sim, target_deaths, seeds), c0, l0s, slope)
+    return {"theta": x.tolist(), "loglik": fx, "proposal": [c0, l0s, slope], "candidates": n,
+            "windows": k, "launches": sim.launches - l0, "realisations": sim.realisations - r_0,
+            "seconds": time.perf_counter() - t0, "emulator_seconds": t_emul}
 
 
 def training_set(sim, n_theta, seeds, rng):

@@ -3,10 +3,10 @@
 CPU: the likelihood is CalcLikelihood's negative binomial term
 (src/CovidSim.cpp:3778-3784), checked against scipy's negative binomial
 pmf with the same parameters; the pattern search finds the maximum of a
-smooth test function; the crest refinement recovers the maximum and slope
-of a sheared quadratic ridge from starts anywhere in its basin, and starts
-across the basin agree on a rough ridge; both calibration drivers run end to
-end on a stand-in simulator and agree on R0.
+smooth test function; the crest refinement recovers the maximum of a
+sheared quadratic ridge from starts anywhere in its basin, and starts across
+the basin return the same fit on rough ridges; both calibration drivers run
+end to end on a stand-in simulator and agree on R0.
 GPU: on the 20k-person fixture region, an emulator trained on simulator
 ensembles guides calibration to a fit whose simulator likelihood is within
 2 units of plain calibration's, with fewer simulator launches -- what the
@@ -47,38 +47,30 @@ def ridge(th, r0=2.16, ls=0.17, slope=-3.25):
     return -567.0 - 1300.0 * u * u - 5000.0 * v * v
 
 
-def test_crest_fit_recovers_maximum_and_slope():
-    rng = np.random.default_rng(3)
-    z = rng.uniform(-0.12, 0.12, (15, 2))
-    c0 = np.array([2.1, 0.4])
-    pts = c0 + z
-    got = cal.crest_fit(z, ridge(pts))
-    assert got is not None
-    (u, v), slope = got
-    assert abs(c0[0] + u - 2.16) < 1e-9 and abs(c0[1] + v - (0.17 - 3.25 * (c0[0] + u - 2.16))) < 1e-9
-    # in unsheared coordinates the crest slope is the ridge's own
-    assert abs(slope + 3.25) < 1e-9
-    # a convex fit is refused
-    assert cal.crest_fit(z, -ridge(pts)) is None
-
-
 @pytest.mark.parametrize("start", [(2.16, 0.17, -3.25), (2.08, 0.45, -2.5), (2.26, -0.15, -4.0), (2.2, 0.2, -3.0)])
-def test_refine_converges_on_the_crest_from_any_start_in_the_basin(start):
-    x, fx, k = cal.refine(ridge, *start)
-    assert abs(x[0] - 2.16) < 1e-6 and abs(x[1] - 0.17) < 1e-6, (x, k)
-    assert k <= 3
+def test_crest_profile_finds_the_maximum_of_a_ridge(start):
+    """On a sheared quadratic ridge the column parabolas and the profile
+    parabola are exact, so the fit is the maximum itself, from any start in
+    the basin."""
+    x, fx, k, n = cal.crest_profile(ridge, *start)
+    assert abs(x[0] - 2.16) < 1e-9 and abs(x[1] - 0.17) < 1e-9, (x, k, n)
+    assert k <= 5
 
 
-@pytest.mark.parametrize("amp,tol", [(1.0, 0.01), (2.0, 0.01), (4.0, 0.02)])
-def test_refine_is_robust_to_rough_objectives(amp, tol):
-    """On a ridge with roughness of `amp` log-likelihood units (a 16-seed
-    mean's is about 2), starts across the basin agree on R0 within tol: the
-    broad design averages the roughness out."""
+@pytest.mark.parametrize("amp", [1.0, 2.0, 4.0])
+def test_crest_profile_agrees_across_starts_on_rough_ridges(amp):
+    """With roughness of `amp` log-likelihood units (a 16-seed mean's is
+    about 2), starts across the basin settle on the same lattice window and
+    return the same fit -- what lets two calibrations agree; at twice that
+    roughness neighbouring windows can both be fixed points, and the fits
+    still agree within 0.01.  Every fit is near the smooth ridge's maximum."""
     rough = lambda th: ridge(th) + amp * np.sin(97.0 * th[:, 0]) * np.cos(41.0 * th[:, 1])
-    fits = [cal.refine(rough, *st)[0][0] for st in
-            [(2.161, 0.17, -3.1), (2.170, 0.19, -3.4), (2.12, 0.3, -3.0), (2.2, 0.05, -3.75)]]
-    assert max(fits) - min(fits) < tol, fits
-    assert abs(np.mean(fits) - 2.16) < tol, fits
+    fits = np.array([cal.crest_profile(rough, *st)[0] for st in
+                     [(2.161, 0.17, -3.1), (2.19, 0.07, -3.4), (2.12, 0.3, -3.0), (2.2, 0.05, -3.75), (2.14, 0.21, -2.75)]])
+    if amp <= 2.0:
+        assert (fits == fits[0]).all(), fits
+    assert np.ptp(fits[:, 0]) < 0.01, fits
+    assert np.abs(fits[:, 0] - 2.16).max() < 0.01, fits
 
 
 def test_crest_from_grid():
@@ -95,10 +87,12 @@ class CurveSim:
     R0 against the seeding scale (the likelihood ridge of the real one)."""
 
     launches = 0
+    realisations = 0
 
     def run(self, theta, seeds):
         theta = np.atleast_2d(theta)
         self.launches += -(-len(theta) * len(seeds) // 64)
+        self.realisations += len(theta) * len(seeds)
         t = np.arange(cal.DAYS)[None, :]
         r = (theta[:, :1] - 1.0) / 5.0                  # growth rate per day
         peak = 40.0 + (np.log(1e4) - theta[:, 1:] * np.log(10.0) * 1.0) / r
@@ -125,7 +119,7 @@ def test_calibrations_agree_on_a_stand_in_simulator():
     emul = cal.calibrate_emulated(sim, ExactEmulator(), target, seeds)
     assert abs(plain["theta"][0] - 2.15) < 0.02 and abs(emul["theta"][0] - 2.15) < 0.02, (plain, emul)
     assert abs(plain["theta"][0] - emul["theta"][0]) < 0.02
-    assert plain["launches"] >= 5 * emul["launches"], (plain["launches"], emul["launches"])
+    assert plain["realisations"] >= 5 * emul["realisations"], (plain["realisations"], emul["realisations"])
 
 
 def test_encoding_round_trip():

@@ -35,7 +35,8 @@ struct cs_engine {
     int ts_max = 0, ts_n = 0;
     bool ts_fresh = true;            // no sample since the last alloc/reset/load: the series restarts
     unsigned grid_waves = 4096;      // blocks of 4 waves for the per-person passes (a wave per chunk)
-    unsigned grid_cells = 2048;      // blocks for the per-cell passes (a block per cell)
+    unsigned grid_cells = 2048;      // blocks for the per-cell spatial sums (a block per cell)
+    unsigned grid_moves = 4096;      // blocks for the batched list moves (a block per moved cell; not tuned)
     // replayed step graphs (cs_run_steps): one graph per output step
     bool use_graphs = true;
     hipGraphExec_t gexec = nullptr;
@@ -222,7 +223,7 @@ int launch_resolve(cs_engine* e)
         cs::k_resolve_apply<<<1024, 256, 0, st>>>(d);
         cs::k_resolve_reset<<<1024, 256, 0, st>>>(d);
     }
-    { ProfScope ps(e, CS_K_LIST_MOVE); cs::k_list_move<<<e->grid_cells, 256, 0, st>>>(d, 0); }
+    { ProfScope ps(e, CS_K_LIST_MOVE); cs::k_list_move<<<e->grid_moves, 256, 0, st>>>(d, 0); }
     HIPCHK(e, hipGetLastError());
     return 0;
 }
@@ -242,7 +243,7 @@ int launch_incub(cs_engine* e)
         cs::k_compact<<<grid_for(nrc, 256), 256, 0, st>>>(d, 0, 0);
     }
     { ProfScope ps(e, CS_K_LATENT_SCAN); cs::k_latent_scan<<<e->grid_waves, 256, 0, st>>>(d); }
-    { ProfScope ps(e, CS_K_LIST_MOVE); cs::k_list_move<<<e->grid_cells, 256, 0, st>>>(d, 1); }
+    { ProfScope ps(e, CS_K_LIST_MOVE); cs::k_list_move<<<e->grid_moves, 256, 0, st>>>(d, 1); }
     {
         ProfScope ps(e, CS_K_OTHER);
         cs::k_zero_counts<<<1, 64, 0, st>>>(d.care_count, 1);
@@ -255,7 +256,7 @@ int launch_incub(cs_engine* e)
         cs::k_care_eval<<<256, 256, 0, st>>>(d);
         cs::k_care_apply<<<256, 256, 0, st>>>(d);
     }
-    { ProfScope ps(e, CS_K_LIST_MOVE); cs::k_list_move<<<e->grid_cells, 256, 0, st>>>(d, 2); }
+    { ProfScope ps(e, CS_K_LIST_MOVE); cs::k_list_move<<<e->grid_moves, 256, 0, st>>>(d, 2); }
     HIPCHK(e, hipGetLastError());
     return 0;
 }
@@ -745,7 +746,7 @@ int cs_reset_realisations(cs_engine* e, int n_seed)
     if ((rc = launch_update_probs(e, nullptr))) return rc;
     cs::k_zero_counts<<<1, 64, 0, st>>>(d.wl, cs::WL_N);
     cs::k_seed<<<grid_for(d.R, 64), 64, 0, st>>>(d, n_seed);
-    cs::k_list_move<<<e->grid_cells, 256, 0, st>>>(d, 0);
+    cs::k_list_move<<<e->grid_moves, 256, 0, st>>>(d, 0);
     HIPCHK(e, hipGetLastError());
     return check_flags(e);
 }
@@ -1065,23 +1066,32 @@ int cs_autotune(cs_engine* e, double t, int reps, cs_tuning* out)
     };
     const uint32_t waves[] = {1024, 2048, 4096, 8192, 16384};
     const uint32_t cells[] = {512, 1024, 2048, 4096};
-    uint32_t bw = e->grid_waves, bc = e->grid_cells;
-    float best = 1e30f;
-    for (uint32_t w : waves) {
-        e->grid_waves = w;
-        float ms;
-        if ((rc = time_events(&ms))) { e->prof = prof; return rc; }
-        if (ms < best) { best = ms; bw = w; }
+    // each candidate's best of `reps`; the largest grid within 3% of the
+    // fastest wins, so timing noise on a small state cannot pick a grid too
+    // small for the larger states that follow
+    auto pick = [](const float* ms, const uint32_t* g, int n) {
+        float best = 1e30f;
+        for (int i = 0; i < n; i++) best = std::min(best, ms[i]);
+        uint32_t choice = g[0];
+        for (int i = 0; i < n; i++)
+            if (ms[i] <= 1.03f * best) choice = std::max(choice, g[i]);
+        return choice;
+    };
+    float ms_w[5], ms_c[4];
+    for (int i = 0; i < 5; i++) {
+        e->grid_waves = waves[i];
+        if ((rc = time_events(&ms_w[i]))) { e->prof = prof; return rc; }
     }
+    const uint32_t bw = pick(ms_w, waves, 5);
     e->grid_waves = bw;
-    best = 1e30f;
-    for (uint32_t c : cells) {
-        e->grid_cells = c;
-        float ms;
-        if ((rc = time_events(&ms))) { e->prof = prof; return rc; }
-        if (ms < best) { best = ms; bc = c; }
+    for (int i = 0; i < 4; i++) {
+        e->grid_cells = cells[i];
+        if ((rc = time_events(&ms_c[i]))) { e->prof = prof; return rc; }
     }
+    const uint32_t bc = pick(ms_c, cells, 4);
     e->grid_cells = bc;
+    float best = 1e30f;
+    for (int i = 0; i < 4; i++) best = std::min(best, ms_c[i]);
     // the queue holds 4 times the events of this state (it still grows on demand)
     unsigned nev = 0;
     HIPCHK(e, hipMemcpy(&nev, e->d.ev_count, sizeof(nev), hipMemcpyDeviceToHost));

@@ -365,19 +365,21 @@ int  cs_run_steps(cs_engine* e, double t0, int n_steps, int steps_per_sample);
  * every kernel of every step from the host */
 int  cs_set_step_graphs(cs_engine* e, int enable);
 /* capacity of the infection event queue and of the spatial draw list (in
- * chunks of 64 draws); 0 keeps a value.  Both grow by doubling when a step
+ * chunks of up to 256 draws of one source cell); 0 keeps a value.  Both grow by doubling when a step
  * overflows them (the step is re-run, with the same result), so capacities
  * only trade memory against re-runs. */
 int  cs_set_queue_capacity(cs_engine* e, uint32_t events, uint32_t draw_chunks);
 int  cs_get_queue_capacity(cs_engine* e, uint32_t* events, uint32_t* draw_chunks);
 /* launch shapes: blocks of the per-person passes (4 waves each, one wave
- * per chunk of 64 persons) and of the per-cell passes; 0 keeps a value.
- * Results do not depend on them. */
+ * per chunk of 64 persons or 256 spatial draws) and of the per-cell spatial
+ * sums (a block per cell); 0 keeps a value.  Results do not depend on them. */
 int  cs_set_launch_shape(cs_engine* e, uint32_t grid_waves, uint32_t grid_cells);
 /* autotuner: at the current state and time t, times the InfectSweep event
  * phase (it reads the state and writes only scratch, so it can be repeated)
- * for each candidate launch shape, best of reps, keeps the fastest, and sizes
- * the event queue at 4 times this state's events */
+ * for each candidate launch shape, best of reps, keeps the largest shape
+ * within 3% of the fastest (timing noise on a small state must not pick a
+ * grid too small for later, larger states), and sizes the event queue at 4
+ * times this state's events */
 typedef struct cs_tuning {
     uint32_t grid_waves, grid_cells;
     uint32_t event_capacity, draw_capacity;

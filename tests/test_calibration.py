@@ -147,7 +147,7 @@ def test_emulator_guided_calibration_uses_fewer_launches():
             self.eng.set_population(pop)
             c = self.eng.transmission_coeffs(77)
             self.r0_hh, self.r0_pl, self.sp_r0 = c.r0_household, c.r0_places, c.spatial_r0
-            self.n_real, self.launches, self.sim_seconds = 64, 0, 0.0
+            self.n_real, self.launches, self.realisations, self.sim_seconds = 64, 0, 0, 0.0
 
     sim = MiniSim()
     rng = np.random.default_rng(1)

@@ -22,6 +22,7 @@ struct cs_engine {
     hipStream_t stream = nullptr;
     cs_params P;
     bool have_params = false, have_pop = false;
+    bool due_ok = false;             // the parameters allow latent due lists
     cs_params* dP = nullptr;
     double *dkl = nullptr, *dkh = nullptr;
     int N = 0, NH = 0, NC = 0, NCT = 0, NM = 0, NA = 0, NPT = 0, R = 0;
@@ -242,7 +243,14 @@ int launch_incub(cs_engine* e)
         cs::k_zero_counts<<<1, 64, 0, st>>>(d.wl, cs::WL_N);
         cs::k_compact<<<grid_for(nrc, 256), 256, 0, st>>>(d, 0, 0);
     }
-    { ProfScope ps(e, CS_K_LATENT_SCAN); cs::k_latent_scan<<<e->grid_waves, 256, 0, st>>>(d); }
+    {
+        ProfScope ps(e, CS_K_LATENT_SCAN);
+        // the due list when it holds the step's persons, else the scan (each
+        // kernel does nothing in the other case)
+        cs::k_latent_due<<<1024, 256, 0, st>>>(d);
+        cs::k_latent_scan<<<e->grid_waves, 256, 0, st>>>(d);
+        cs::k_due_clear<<<1, 64, 0, st>>>(d);
+    }
     { ProfScope ps(e, CS_K_LIST_MOVE); cs::k_list_move<<<e->grid_moves, 256, 0, st>>>(d, 1); }
     {
         ProfScope ps(e, CS_K_OTHER);
@@ -486,6 +494,14 @@ int cs_set_params(cs_engine* e, const cs_params* p, const double* kl, const doub
     e->d.kl = e->dkl;
     e->d.kh = e->dkh;
     e->have_params = true;
+    {   // latent due lists need every latent time within CS_DUE_RING steps: the
+        // longest latent period is latent_period * -ln(smallest table entry)
+        double cmin = 1.0;
+        for (int i = 0; i <= CS_CDF_RES; i++) cmin = std::min(cmin, p->icdf[CS_ICDF_LATENT][i]);
+        const double steps = p->latent_period * -std::log(cmin) * p->time_steps_per_day;
+        e->due_ok = !p->do_latent || (cmin > 0 && steps + 3.0 < (double)CS_DUE_RING);
+        if (!e->due_ok) e->d.due_on = 0;
+    }
     if (e->have_pop && new_tables) {
         cs::k_build_max_trans<<<grid_for((long long)e->NC * e->NC, 256, 1u << 30), 256, 0, e->stream>>>(e->d);
         HIPCHK(e, hipGetLastError());
@@ -669,7 +685,7 @@ int cs_alloc_realisations(cs_engine* e, int n_real, const uint64_t* seeds)
     HIPCHK(e, hipMemcpy(sd, seeds, R * 8, hipMemcpyHostToDevice));
     d.seed = sd;
     AL(hot, RN);
-    d.inf.base = d.hot; d.itype.base = d.hot; d.infness.base = d.hot;
+    d.inf.base = d.hot; d.hflags.base = d.hot; d.itype.base = d.hot; d.infness.base = d.hot;
     d.t_lat.base = d.hot; d.t_as.base = d.hot; d.t_ae.base = d.hot; d.t_gate.base = d.hot;
     AL(tim, RN);
     d.t_inf.base = d.tim; d.t_rod.base = d.tim; d.t_sari.base = d.tim; d.t_crit.base = d.tim;
@@ -691,6 +707,9 @@ int cs_alloc_realisations(cs_engine* e, int n_real, const uint64_t* seeds)
     AL(claim, RN); AL(lastk, RN); AL(batch, RN); AL(bcount, RC);
     AL(care_gp, RN); AL(care_k, RN); AL(care_abs, RN); AL(care_count, 1);
     AL(last_tot, CS_TS_LAST * R);
+    d.due_cap = (unsigned)std::min<size_t>(std::max<size_t>(RN / 64, 4096), 0x7fffffff);
+    AL(due, (size_t)CS_DUE_RING * d.due_cap); AL(due_n, CS_DUE_RING);
+    d.due_on = 0;
 #undef AL
     if ((rc = dalloc(e, A, &e->d_sel, R))) return rc;
     if ((rc = dalloc(e, A, &e->rbeta_buf, R))) return rc;
@@ -740,6 +759,12 @@ int cs_reset_realisations(cs_engine* e, int n_seed)
     hipStream_t st = e->stream;
     e->ts_fresh = true;
     HIPCHK(e, hipMemsetAsync(d.cnt, 0, (size_t)d.R * sizeof(cs::Counters), st));
+    HIPCHK(e, hipMemsetAsync(d.due_n, 0, CS_DUE_RING * sizeof(unsigned), st));
+    if (d.due_on != (e->due_ok ? 1 : 0)) {  // fresh state: every latent person is listed from here on
+        HIPCHK(e, hipStreamSynchronize(st));
+        drop_graph(e);
+        d.due_on = e->due_ok ? 1 : 0;
+    }
     cs::k_reset_persons<<<grid_for((long long)d.R * d.N, 256), 256, 0, st>>>(d);
     cs::k_reset_cells<<<grid_for((long long)d.R * d.NC, 64), 64, 0, st>>>(d);
     cs::k_reset_misc<<<grid_for(std::max((long long)d.R * d.np_off[CS_MAX_PLACE_TYPES], (long long)d.R * d.NM), 256), 256, 0, st>>>(d);
@@ -759,6 +784,11 @@ int cs_load_state(cs_engine* e, int real, const cs_state* s)
     Dev& d = e->d;
     const size_t N = e->N, NC = e->NC, o = (size_t)real * N, oc = (size_t)real * NC;
     e->ts_fresh = true;
+    if (d.due_on) {  // loaded latent persons are on no due list: scan them all until the next reset
+        HIPCHK(e, hipStreamSynchronize(e->stream));
+        drop_graph(e);
+        d.due_on = 0;
+    }
     {
         // the packed fields: read the records, overwrite the fields the state gives, write back
         std::vector<cs::Hot> h(N);
@@ -772,6 +802,7 @@ int cs_load_state(cs_engine* e, int real, const cs_state* s)
             if (s->absent_start_time) x.t_as = s->absent_start_time[i];
             if (s->absent_stop_time) x.t_ae = s->absent_stop_time[i];
             x.t_gate = 0;  // loaded timers: look at every one
+            x.flags = 0;   // and every household
         }
         HIPCHK(e, hipMemcpy(d.hot + o, h.data(), N * sizeof(cs::Hot), hipMemcpyHostToDevice));
         std::vector<cs::Tim> m(N);

@@ -76,6 +76,12 @@ __device__ __forceinline__ Chunk chunk_at(const Chunk* l, unsigned ch)
     return Chunk{uni(e.rc), uni(e.first)};
 }
 
+// this step's latent persons are all on its due list (not overflowed)
+__device__ __forceinline__ bool due_listed(const Dev& d)
+{
+    return d.due_on && d.due_n[d.sc->ts & (CS_DUE_RING - 1u)] <= d.due_cap;
+}
+
 __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v)
 {
     for (int w = 32; w > 0; w >>= 1) v += __shfl_xor(v, w, 64);
@@ -131,7 +137,11 @@ __global__ void __launch_bounds__(256) k_compact(Dev d, int seg, int zero_draws)
     int n = 0;
     unsigned long long act = 0;
     if (rc < nrc) {
-        if (seg == 0) { n = d.cL[rc]; act = (unsigned long long)n + d.cI[rc]; }
+        if (seg == 0) {
+            act = (unsigned long long)d.cL[rc] + d.cI[rc];
+            // the latent chunks, unless this step's due list serves instead
+            n = due_listed(d) ? 0 : d.cL[rc];
+        }
         else if (seg == 1) n = d.cI[rc];
         else n = d.spK[rc];
         if (zero_draws) d.spK[rc] = 0;
@@ -407,6 +417,11 @@ __device__ bool do_infect_state(const Dev& d, int r, int p, double t, unsigned t
     } else
         lat = (uint16_t)(t * P->time_steps_per_day);
     d.t_lat[gp] = lat;
+    if (d.due_on) {  // the step that ends p's latency lists p
+        const unsigned b = lat & (CS_DUE_RING - 1u);
+        const unsigned idx = atomicAdd(&d.due_n[b], 1u);
+        if (idx < d.due_cap) d.due[(size_t)b * d.due_cap + idx] = (long long)gp;
+    }
     tl->tg = tl->si = 0;
     tl->ntg = 0;
     if (infector >= 0) {
@@ -529,10 +544,14 @@ __global__ void __launch_bounds__(256) k_infect_households(Dev d)
                 w_hh_inf++;
                 w_hh_mem += nh;
                 // the susceptible members first: with none, nothing below can
-                // infect, so the household's hazard is not needed
+                // infect, so the household's hazard is not needed; and none
+                // will be susceptible again, so later steps skip the scan
                 unsigned smask = 0;
-                for (int m = f0; m < f0 + nh; m++)
-                    if (d.inf[rN + m] == CS_SUSCEPTIBLE) smask |= 1u << (m - f0);
+                if (!(d.hflags[gi] & HOT_HH_DONE)) {
+                    for (int m = f0; m < f0 + nh; m++)
+                        if (d.inf[rN + m] == CS_SUSCEPTIBLE) smask |= 1u << (m - f0);
+                    if (!smask) d.hflags[gi] = (uint8_t)(d.hflags[gi] | HOT_HH_DONE);
+                }
                 if (smask) {
                     double HI = sc.beta_h * house_inf(d, r, i, ts);
                     bool absent = false;

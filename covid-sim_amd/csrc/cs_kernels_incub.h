@@ -164,6 +164,35 @@ __device__ void do_incub(const Dev& d, int r, int p, unsigned ts)
 
 // latent lists: DoIncub where latent_time == now (src/Sweep.cpp:925-927); a
 // wave takes a chunk of up to 64 latent persons of one cell
+// the persons on this step's due list whose latency ends now (the same
+// transitions as the scan below, visiting only them)
+__global__ void __launch_bounds__(256) k_latent_due(Dev d)
+{
+    if (halted(d) || !due_listed(d)) return;
+    const unsigned ts = d.sc->ts;
+    const unsigned b = ts & (CS_DUE_RING - 1u);
+    const unsigned n = d.due_n[b];
+    const long long* list = d.due + (size_t)b * d.due_cap;
+    unsigned long long nt = 0;
+    for (unsigned k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        const long long gp = list[k];
+        if (d.inf[gp] == CS_LATENT && d.t_lat[gp] == ts) {
+            const int r = (int)(gp / d.N), p = (int)(gp % d.N);
+            do_incub(d, r, p, ts);
+            batch_add(d, r, d.pc[p], p);
+            nt++;
+        }
+    }
+    nt = wave_sum(nt);
+    if ((threadIdx.x & 63) == 0 && nt) atomicAdd(&d.wc->n_transition, nt);
+}
+
+// this step's due list is consumed: clear it for the step CS_DUE_RING later
+__global__ void k_due_clear(Dev d)
+{
+    if (threadIdx.x == 0 && !halted(d) && d.due_on) d.due_n[d.sc->ts & (CS_DUE_RING - 1u)] = 0u;
+}
+
 __global__ void __launch_bounds__(256) k_latent_scan(Dev d)
 {
     if (halted(d)) return;
@@ -394,6 +423,7 @@ __global__ void k_reset_persons(Dev d)
         d.inf[g] = CS_SUSCEPTIBLE;
         d.t_lat[g] = d.t_rod[g] = 0; d.t_inf[g] = 0;
         d.t_gate[g] = 0;
+        d.hflags[g] = 0;
         d.infector[g] = -1; d.itype[g] = 0;
         d.qc[g] = 2; d.t_qs[g] = CS_UNSET_TIME;
         Stream s(d.seed[r], cw0(0, PS_INIT), (uint32_t)k, 0);

@@ -61,10 +61,18 @@ struct Work {  // event tallies for the algorithmic-bytes figure
 };
 
 enum { FLAG_OVERFLOW = 1, FLAG_RANGE = 2, FLAG_BATCH = 4, FLAG_STUCK = 8 };
+// Hot.flags: the person's household has no susceptible member left (none
+// can return to susceptible, so the household pass may skip it for good)
+enum { HOT_HH_DONE = 1 };
+
+// latent due lists: bucket ts % CS_DUE_RING lists the persons whose latent
+// time is ts, appended when they are infected (every latent time lies less
+// than CS_DUE_RING steps ahead, checked from the latent-period table)
+#define CS_DUE_RING 256
 
 struct Hot {  // per (realisation, person): 16 bytes, one vector load
     int8_t inf;       // InfStat code
-    uint8_t pad;
+    uint8_t flags;    // HOT_HH_DONE
     int16_t itype;    // infect_type
     uint16_t t_lat, t_as, t_ae;  // latent time, absence start and end
     uint16_t t_gate;  // no timer of an infectious person fires before this step
@@ -169,6 +177,7 @@ struct Dev {
     const int32_t* rseed_n;  // [R] per-realisation seed infections, or null
     Hot* hot;             // [R][N]
     CS_FLD(Hot, inf) inf;
+    CS_FLD(Hot, flags) hflags;
     CS_FLD(Hot, itype) itype;
     CS_FLD(Hot, infness) infness;
     CS_FLD(Hot, t_lat) t_lat;
@@ -232,6 +241,10 @@ struct Dev {
     int32_t* ts_buf;      // [R][max_samples][CS_TS_FIELDS]
     int ts_max;
     long long* last_tot;  // [R][CS_TS_LAST] cumI, cumC, D, severity incidence at the previous sample
+    long long* due;       // [CS_DUE_RING][due_cap] persons (gp) by latent time
+    unsigned* due_n;      // [CS_DUE_RING] appended; above due_cap: that step scans every latent person
+    unsigned due_cap;
+    int due_on;           // 0: every step scans every latent person
 };
 
 // ---- predicates (src/ModelMacros.h:6-23, src/Person.cpp) ----

@@ -8,10 +8,10 @@ sheared quadratic ridge from starts anywhere in its basin, and starts across
 the basin return the same fit on rough ridges; both calibration drivers run
 end to end on a stand-in simulator and agree on R0.
 GPU: on the 20k-person fixture region, an emulator trained on simulator
-ensembles guides calibration to a fit whose simulator likelihood is within
-2 units of plain calibration's, with fewer simulator launches -- what the
-emulator gains over the method without it (bench.py --calibration measures
-the speed-up on the 1M-person region).
+ensembles guides calibration to a fit inside plain calibration's 95%
+likelihood region, with fewer simulator launches -- what the emulator gains
+over the method without it (bench.py --calibration measures the speed-up
+and the agreement of R0 on the 1M-person region).
 """
 import importlib
 
@@ -156,12 +156,14 @@ def test_emulator_guided_calibration_uses_fewer_launches():
     assert losses[-1] < 0.1 * losses[0]
     _, d = sim.run(np.array([[2.2, 0.1]]), np.arange(5000, 5016, dtype=np.uint64))
     target = np.round(d[0])
-    seeds = np.arange(100, 116, dtype=np.uint64)
+    # 32 seeds per candidate: the deaths of a 20k-person region leave a flat
+    # ridge of likelihood along R0 x seeding scale, rough at 16 seeds
+    seeds = np.arange(100, 132, dtype=np.uint64)
     plain = cal.calibrate_plain(sim, target, seeds)
     emul = cal.calibrate_emulated(sim, emu, target, seeds)
-    # deaths of a 20k-person region leave a flat, rough ridge of likelihood
-    # along R0 x seeding scale, so the two fits are compared by the
-    # simulator's likelihood at each (same seeds, one launch)
+    # the two fits are compared by the simulator's likelihood at each (same
+    # seeds): the emulator-guided fit lies in the plain fit's 95% likelihood
+    # region for two parameters (within chi2(2, 0.95) / 2 = 3.0 units)
     f = cal.nb_loglik(target, sim.run(np.array([plain["theta"], emul["theta"]]), seeds)[1])
-    assert f[1] >= f[0] - 2.0, (plain, emul, f)
+    assert f[1] >= f[0] - 3.0, (plain, emul, f)
     assert emul["launches"] < plain["launches"], (plain, emul)

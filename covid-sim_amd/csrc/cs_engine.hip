@@ -1043,6 +1043,26 @@ int cs_set_queue_capacity(cs_engine* e, uint32_t events, uint32_t draw_chunks)
     return 0;
 }
 
+int cs_set_due_capacity(cs_engine* e, uint32_t per_step)
+{
+    if (!e) return CS_EINVAL;
+    if (e->R == 0) return set_err(e, CS_ESTATE, "no realisations allocated");
+    if (!per_step) return 0;
+    HIPCHK(e, hipSetDevice(e->device));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    drop_graph(e);
+    Dev& d = e->d;
+    (void)hipFree(d.due);
+    e->real_allocs.erase(std::remove(e->real_allocs.begin(), e->real_allocs.end(), (void*)d.due), e->real_allocs.end());
+    int rc = dalloc(e, e->real_allocs, &d.due, (size_t)CS_DUE_RING * per_step);
+    if (rc) return rc;
+    d.due_cap = per_step;
+    // the lists held so far are gone: scan until the next reset lists afresh
+    HIPCHK(e, hipMemset(d.due_n, 0, CS_DUE_RING * sizeof(unsigned)));
+    d.due_on = 0;
+    return 0;
+}
+
 int cs_get_queue_capacity(cs_engine* e, uint32_t* events, uint32_t* draw_chunks)
 {
     if (!e) return CS_EINVAL;

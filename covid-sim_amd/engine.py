@@ -138,6 +138,9 @@ class Engine:
     def set_queue_capacity(self, events=0, draw_chunks=0):
         self.call("set_queue_capacity", int(events), int(draw_chunks))
 
+    def set_due_capacity(self, per_step):
+        self.call("set_due_capacity", int(per_step))
+
     def queue_capacity(self):
         a, b = C.c_uint32(), C.c_uint32()
         self.call("get_queue_capacity", C.byref(a), C.byref(b))

@@ -370,6 +370,13 @@ int  cs_set_step_graphs(cs_engine* e, int enable);
  * only trade memory against re-runs. */
 int  cs_set_queue_capacity(cs_engine* e, uint32_t events, uint32_t draw_chunks);
 int  cs_get_queue_capacity(cs_engine* e, uint32_t* events, uint32_t* draw_chunks);
+/* capacity of each step's latent due list: the persons whose latency ends in
+ * that step, listed when they are infected, so that the step visits only
+ * them (after a reset; a loaded state is scanned in full until the next
+ * reset).  A step whose list overflows scans every latent person instead,
+ * with the same result; 0 keeps the value.  No counterpart in the reference,
+ * which scans every latent person every step (src/Sweep.cpp:922-940). */
+int  cs_set_due_capacity(cs_engine* e, uint32_t per_step);
 /* launch shapes: blocks of the per-person passes (4 waves each, one wave
  * per chunk of 64 persons or 256 spatial draws) and of the per-cell spatial
  * sums (a block per cell); 0 keeps a value.  Results do not depend on them. */

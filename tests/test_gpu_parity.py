@@ -274,6 +274,34 @@ def test_launch_modes_and_overflow_recovery(engmod, csa, graphs, events, draw_ch
         assert a.compare(b) == []
 
 
+@pytest.mark.parametrize("graphs", [True, False])
+def test_latent_due_list_overflow(engmod, csa, graphs):
+    """Latent due lists far too small for the epidemic: the steps whose list
+    overflows scan every latent person instead, and the 60-day trajectory
+    stays the oracle's."""
+    arrays, pop, params, _ = mini()
+    seeds = np.array([41, 42], np.uint64)
+    g = engmod.Engine(0)
+    g.set_params(params)
+    g.set_population(pop)
+    g.alloc(seeds)
+    g.set_step_graphs(graphs)
+    g.set_due_capacity(8)
+    o = OracleEngine(params, pop)
+    o.alloc(seeds)
+    tspd = int(params.time_steps_per_day)
+    for eng in (g, o):
+        eng.reset(10)
+        eng.run_steps(0.0, 60 * tspd, tspd)
+    assert np.array_equal(g.timeseries(60), o.timeseries(60))
+    assert g.timeseries(60)[:, -1, 3].min() > 100  # the epidemic grew past the lists
+    for r in range(2):
+        a, b = csa.population.State(pop), csa.population.State(pop)
+        g.store_state(r, a)
+        o.store_state(r, b)
+        assert a.compare(b) == []
+
+
 def test_autotune_keeps_results(engmod, csa):
     """The autotuner times the event phase on the current state without
     changing it; launch shapes and capacities do not change results."""

@@ -385,6 +385,7 @@ __global__ void __launch_bounds__(256) k_list_move(Dev d, int mode)
 // the counter updates of one DoInfect (StateT fields, src/Update.cpp:100-195)
 struct InfectTally {
     int kind, ag, kw;        // cumItype, cumIa, cumI_keyworker indices
+    unsigned lat;            // latent time (the due list p joins)
     double rad2;
     long long tg, si;        // generation and serial intervals
     int ntg;                 // 1 when the infector is a person
@@ -417,11 +418,7 @@ __device__ bool do_infect_state(const Dev& d, int r, int p, double t, unsigned t
     } else
         lat = (uint16_t)(t * P->time_steps_per_day);
     d.t_lat[gp] = lat;
-    if (d.due_on) {  // the step that ends p's latency lists p
-        const unsigned b = lat & (CS_DUE_RING - 1u);
-        const unsigned idx = atomicAdd(&d.due_n[b], 1u);
-        if (idx < d.due_cap) d.due[(size_t)b * d.due_cap + idx] = (long long)gp;
-    }
+    tl->lat = lat;
     tl->tg = tl->si = 0;
     tl->ntg = 0;
     if (infector >= 0) {
@@ -455,10 +452,42 @@ __device__ __forceinline__ void tally_scalar(Counters* cn, unsigned long long n,
     }
 }
 
+// list gp on the due list of the step that ends its latency
+__device__ __forceinline__ void due_add(const Dev& d, unsigned lat, long long gp)
+{
+    const unsigned b = lat & (CS_DUE_RING - 1u);
+    const unsigned idx = atomicAdd(&d.due_n[b], 1u);
+    if (idx < d.due_cap) d.due[(size_t)b * d.due_cap + idx] = gp;
+}
+
+// due_add for the lanes with `add` set, one atomic per distinct step of the
+// wave (a wave's infections fall on a few latency steps); every lane of the
+// wave calls it
+__device__ __forceinline__ void due_add_wave(const Dev& d, bool add, unsigned lat, long long gp)
+{
+    const unsigned b = lat & (CS_DUE_RING - 1u);
+    unsigned long long todo = __ballot(add);
+    while (todo) {
+        const int lead = __ffsll((long long)todo) - 1;
+        const unsigned b0 = (unsigned)__shfl((int)b, lead, 64);
+        const bool mine = add && b == b0;
+        const unsigned long long grp = __ballot(mine);
+        unsigned base = 0;
+        if ((int)(threadIdx.x & 63) == lead) base = atomicAdd(&d.due_n[b0], (unsigned)__popcll(grp));
+        base = (unsigned)__shfl((int)base, lead, 64);
+        if (mine) {
+            const unsigned idx = base + __builtin_amdgcn_mbcnt_hi((uint32_t)(grp >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)grp, 0u));
+            if (idx < d.due_cap) d.due[(size_t)b0 * d.due_cap + idx] = gp;
+        }
+        todo &= ~grp;
+    }
+}
+
 __device__ void do_infect(const Dev& d, int r, int p, double t, unsigned ts, int infector, int type)
 {
     InfectTally tl;
     if (!do_infect_state(d, r, p, t, ts, infector, type, &tl)) return;
+    if (d.due_on) due_add(d, tl.lat, (long long)r * d.N + p);
     Counters* cn = d.cnt + r;
     tally_indexed(cn, tl);
     tally_scalar(cn, 1, tl.rad2, double_to_bits(tl.rad2), tl.tg, tl.si, (unsigned long long)tl.ntg);
@@ -953,6 +982,7 @@ __global__ void k_resolve_apply(Dev d)
                 d.itype[gp] = d.ev_type[e];
             }
         }
+        if (d.due_on) due_add_wave(d, won, won ? tl.lat : 0u, won ? (long long)d.ev_gp[e] : 0ll);
         // counters: events are appended by whole producing waves, so a wave's
         // infections nearly always share one realisation; then the scalar
         // counters take one atomic per wave instead of one per infection

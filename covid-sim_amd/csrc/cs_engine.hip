@@ -220,9 +220,9 @@ int launch_resolve(cs_engine* e)
     hipStream_t st = e->stream;
     {
         ProfScope ps(e, CS_K_RESOLVE);
-        cs::k_resolve_claim<<<1024, 256, 0, st>>>(d);
-        cs::k_resolve_apply<<<1024, 256, 0, st>>>(d);
-        cs::k_resolve_reset<<<1024, 256, 0, st>>>(d);
+        cs::k_resolve_claim<<<e->grid_waves, 256, 0, st>>>(d);
+        cs::k_resolve_apply<<<e->grid_waves, 256, 0, st>>>(d);
+        cs::k_resolve_reset<<<e->grid_waves, 256, 0, st>>>(d);
     }
     { ProfScope ps(e, CS_K_LIST_MOVE); cs::k_list_move<<<e->grid_moves, 256, 0, st>>>(d, 0); }
     HIPCHK(e, hipGetLastError());
@@ -247,7 +247,7 @@ int launch_incub(cs_engine* e)
         ProfScope ps(e, CS_K_LATENT_SCAN);
         // the due list when it holds the step's persons, else the scan (each
         // kernel does nothing in the other case)
-        cs::k_latent_due<<<1024, 256, 0, st>>>(d);
+        cs::k_latent_due<<<e->grid_waves, 256, 0, st>>>(d);
         cs::k_latent_scan<<<e->grid_waves, 256, 0, st>>>(d);
         cs::k_due_clear<<<1, 64, 0, st>>>(d);
     }
@@ -704,7 +704,8 @@ int cs_alloc_realisations(cs_engine* e, int n_real, const uint64_t* seeds)
     size_t cap = std::min<size_t>(std::max<size_t>(RN / 16, (size_t)1 << 20), (size_t)1 << 28);
     d.ev_cap = (unsigned)cap;
     AL(ev_key, cap); AL(ev_gp, cap); AL(ev_infector, cap); AL(ev_type, cap); AL(ev_count, 1);
-    AL(claim, RN); AL(lastk, RN); AL(batch, RN); AL(bcount, RC);
+    AL(clr, RN); AL(batch, RN); AL(bcount, RC);
+    d.claim.base = d.lastk.base = d.clr;
     AL(care_gp, RN); AL(care_k, RN); AL(care_abs, RN); AL(care_count, 1);
     AL(last_tot, CS_TS_LAST * R);
     d.due_cap = (unsigned)std::min<size_t>(std::max<size_t>(RN / 64, 4096), 0x7fffffff);
@@ -716,8 +717,8 @@ int cs_alloc_realisations(cs_engine* e, int n_real, const uint64_t* seeds)
     if ((rc = dalloc(e, A, &e->rseed_buf, R))) return rc;
     d.rbeta = nullptr;
     d.rseed_n = nullptr;
-    HIPCHK(e, hipMemset(d.claim, 0xff, RN * 8));
-    HIPCHK(e, hipMemset(d.lastk, 0, RN * 8));
+    cs::k_claim_init<<<grid_for((long long)RN, 256), 256, 0, e->stream>>>(d.clr, RN);
+    HIPCHK(e, hipGetLastError());
     HIPCHK(e, hipMemset(d.bcount, 0, RC * 4));
     HIPCHK(e, hipMemset(d.wl, 0, cs::WL_N * 4));
     HIPCHK(e, hipMemset(d.cnt, 0, R * sizeof(cs::Counters)));

@@ -171,6 +171,13 @@ __global__ void __launch_bounds__(256) k_compact(Dev d, int seg, int zero_draws)
     }
 }
 
+// no claim yet: the smallest key starts at the largest value, the largest at 0
+__global__ void k_claim_init(ClaimRec* c, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        c[i] = ClaimRec{~0ull, 0ull};
+}
+
 __global__ void k_zero_counts(unsigned* p, int n)
 {
     if ((int)threadIdx.x < n) p[threadIdx.x] = 0u;

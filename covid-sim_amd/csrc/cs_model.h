@@ -84,6 +84,9 @@ struct Tim {  // per (realisation, person): the disease timers and severity, 16 
     uint8_t bits;  // bit0 to_die, bit1 detected
     uint8_t pad;
 };
+struct ClaimRec {  // per (realisation, person): this step's smallest and largest event keys
+    unsigned long long claim, lastk;
+};
 struct PStat {  // per person, shared by all realisations
     int32_t hh;
     uint8_t age, pflags;
@@ -105,7 +108,7 @@ struct PlRec {  // per place (n_places + 1 records; the last holds the ends)
 struct GRec {  // per group of a place
     int32_t gst, gsz;  // group start within the place's members, group size
 };
-static_assert(sizeof(Hot) == 16 && sizeof(Tim) == 16 && sizeof(PStat) == 16 && sizeof(HRec) == 8 && sizeof(PLinks) == 32 &&
+static_assert(sizeof(Hot) == 16 && sizeof(Tim) == 16 && sizeof(ClaimRec) == 16 && sizeof(PStat) == 16 && sizeof(HRec) == 8 && sizeof(PLinks) == 32 &&
               sizeof(PlRec) == 16 && sizeof(GRec) == 8, "record sizes");
 
 // one field of an array of records, indexed like a plain array
@@ -228,7 +231,9 @@ struct Dev {
     int16_t* ev_type;
     unsigned int* ev_count;
     unsigned int ev_cap;
-    unsigned long long *claim, *lastk;  // [R][N]
+    ClaimRec* clr;        // [R][N]
+    CS_FLD(ClaimRec, claim) claim;
+    CS_FLD(ClaimRec, lastk) lastk;
     int32_t* batch;       // [R][N] by permutation slot range of the cell
     int32_t* bcount;      // [R*NC]
     int32_t *care_gp, *care_k;

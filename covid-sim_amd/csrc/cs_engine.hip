@@ -34,6 +34,7 @@ struct cs_engine {
     double* rbeta_buf = nullptr;     // [R] storage behind Dev.rbeta
     int32_t* rseed_buf = nullptr;    // [R] storage behind Dev.rseed_n
     int ts_max = 0, ts_n = 0;
+    std::vector<uint8_t> no_links;   // [N] HOT_NO_LINKS of each person, for loaded states
     bool ts_fresh = true;            // no sample since the last alloc/reset/load: the series restarts
     unsigned grid_waves = 4096;      // blocks of 4 waves for the per-person passes (a wave per chunk)
     unsigned grid_cells = 2048;      // blocks for the per-cell spatial sums (a block per cell)
@@ -511,6 +512,8 @@ int cs_set_params(cs_engine* e, const cs_params* p, const double* kl, const doub
     return 0;
 }
 
+static bool no_place_links_host(const int32_t* lk, int npt) { return cs::no_place_links(lk, npt); }
+
 int cs_set_population(cs_engine* e, const cs_population* pop)
 {
     if (!e || !pop) return CS_EINVAL;
@@ -560,6 +563,9 @@ int cs_set_population(cs_engine* e, const cs_population* pop)
     for (int h = 0; h < e->NH; h++)
         if (pop->hh_first[h] < 0 || pop->hh_first[h] + pop->hh_nh[h] > N || pop->hh_nhr[h] < 1 || pop->hh_nhr[h] > CS_MAX_HOUSEHOLD_SIZE)
             return set_err(e, CS_EINVAL, "household %d out of range", h);
+    e->no_links.assign(N, 0);
+    for (int i = 0; i < N; i++)
+        e->no_links[i] = no_place_links_host(pop->place_links + (size_t)i * CS_MAX_PLACE_TYPES, e->NPT) ? cs::HOT_NO_LINKS : 0;
     std::vector<int32_t> padunit(N, 0);
     e->adunits_valid = pop->mcell_adunit != nullptr;
     for (int i = 0; i < N && e->adunits_valid; i++) {
@@ -803,7 +809,7 @@ int cs_load_state(cs_engine* e, int real, const cs_state* s)
             if (s->absent_start_time) x.t_as = s->absent_start_time[i];
             if (s->absent_stop_time) x.t_ae = s->absent_stop_time[i];
             x.t_gate = 0;  // loaded timers: look at every one
-            x.flags = 0;   // and every household
+            x.flags = e->no_links[i];  // and every household
         }
         HIPCHK(e, hipMemcpy(d.hot + o, h.data(), N * sizeof(cs::Hot), hipMemcpyHostToDevice));
         std::vector<cs::Tim> m(N);

@@ -661,7 +661,7 @@ __global__ void __launch_bounds__(256) k_infect_places(Dev d)
         const size_t gi = rN + i;
         const int gen = d.itype[gi] / CS_INFECT_TYPE_MASK;
         // places (src/Sweep.cpp:381-640)
-        if (P->do_places && !host_absent(d, gi, ts)) {
+        if (P->do_places && !(d.hflags[gi] & HOT_NO_LINKS) && !host_absent(d, gi, ts)) {
             for (int pt = 0; pt < d.NPT; pt++) {
                 const int lk = d.plinks[(size_t)i * CS_MAX_PLACE_TYPES + pt];
                 if (lk < 0) continue;

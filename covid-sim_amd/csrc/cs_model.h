@@ -61,9 +61,18 @@ struct Work {  // event tallies for the algorithmic-bytes figure
 };
 
 enum { FLAG_OVERFLOW = 1, FLAG_RANGE = 2, FLAG_BATCH = 4, FLAG_STUCK = 8 };
-// Hot.flags: the person's household has no susceptible member left (none
-// can return to susceptible, so the household pass may skip it for good)
-enum { HOT_HH_DONE = 1 };
+// Hot.flags: HOT_HH_DONE, the person's household has no susceptible member
+// left (none can return to susceptible, so the household pass may skip it
+// for good); HOT_NO_LINKS, the person has no place link (a copy of PLinks
+// the place pass can test without reading PLinks)
+enum { HOT_HH_DONE = 1, HOT_NO_LINKS = 2 };
+
+__host__ __device__ __forceinline__ bool no_place_links(const int32_t* lk, int npt)
+{
+    for (int t = 0; t < npt; t++)
+        if (lk[t] >= 0) return false;
+    return true;
+}
 
 // latent due lists: bucket ts % CS_DUE_RING lists the persons whose latent
 // time is ts, appended when they are infected (every latent time lies less

@@ -467,29 +467,6 @@ __device__ __forceinline__ void due_add(const Dev& d, unsigned lat, long long gp
     if (idx < d.due_cap) d.due[(size_t)b * d.due_cap + idx] = gp;
 }
 
-// due_add for the lanes with `add` set, one atomic per distinct step of the
-// wave (a wave's infections fall on a few latency steps); every lane of the
-// wave calls it
-__device__ __forceinline__ void due_add_wave(const Dev& d, bool add, unsigned lat, long long gp)
-{
-    const unsigned b = lat & (CS_DUE_RING - 1u);
-    unsigned long long todo = __ballot(add);
-    while (todo) {
-        const int lead = __ffsll((long long)todo) - 1;
-        const unsigned b0 = (unsigned)__shfl((int)b, lead, 64);
-        const bool mine = add && b == b0;
-        const unsigned long long grp = __ballot(mine);
-        unsigned base = 0;
-        if ((int)(threadIdx.x & 63) == lead) base = atomicAdd(&d.due_n[b0], (unsigned)__popcll(grp));
-        base = (unsigned)__shfl((int)base, lead, 64);
-        if (mine) {
-            const unsigned idx = base + __builtin_amdgcn_mbcnt_hi((uint32_t)(grp >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)grp, 0u));
-            if (idx < d.due_cap) d.due[(size_t)b0 * d.due_cap + idx] = gp;
-        }
-        todo &= ~grp;
-    }
-}
-
 __device__ void do_infect(const Dev& d, int r, int p, double t, unsigned ts, int infector, int type)
 {
     InfectTally tl;
@@ -989,7 +966,7 @@ __global__ void k_resolve_apply(Dev d)
                 d.itype[gp] = d.ev_type[e];
             }
         }
-        if (d.due_on) due_add_wave(d, won, won ? tl.lat : 0u, won ? (long long)d.ev_gp[e] : 0ll);
+        if (d.due_on && won) due_add(d, tl.lat, (long long)d.ev_gp[e]);
         // counters: events are appended by whole producing waves, so a wave's
         // infections nearly always share one realisation; then the scalar
         // counters take one atomic per wave instead of one per infection

@@ -1340,7 +1340,10 @@ int cs_get_cell_tables(cs_engine* e, int real, float* mt, float* ct, int32_t* in
     if (real >= 0 && real < e->R) {
         const size_t oc = (size_t)real * NC;
         if (ct) HIPCHK(e, hipMemcpy(ct, d.cum_trans + oc * NC, NC * NC * 4, hipMemcpyDeviceToHost));
-        if (inv) HIPCHK(e, hipMemcpy(inv, d.invcdf + oc * 1025, NC * 1025 * 4, hipMemcpyDeviceToHost));
+        if (inv) {
+            HIPCHK(e, hipMemcpy(inv, d.invcdf + oc * 1025, NC * 1025 * 4, hipMemcpyDeviceToHost));
+            for (size_t k = 0; k < NC * 1025; k++) inv[k] &= 0x7fffffff;  // the InvCDF cell (the flag is the library's own)
+        }
         if (tp) HIPCHK(e, hipMemcpy(tp, d.tot_prob + oc, NC * 4, hipMemcpyDeviceToHost));
         if (S0) HIPCHK(e, hipMemcpy(S0, d.S0 + oc, NC * 4, hipMemcpyDeviceToHost));
     }

@@ -24,6 +24,8 @@ namespace cs {
 #define CS_MAX_ATTEMPTS (1 << 20)
 // spatial draws per work-list chunk (a wave's 64 lanes share them)
 #define CS_DRAW_CHUNK 256
+// InvCDF entry flag: the bucket's draws all stop at the entry's cell
+#define CS_INV_PURE 0x80000000u
 
 __device__ __forceinline__ uint64_t ev_key(int infector, int pass, uint32_t sub)
 {
@@ -265,7 +267,11 @@ __global__ void k_update_rows(Dev d, const int* sel)
     int m = 0;
     for (int k = 0; k <= 1024; k++) {
         while (ct[m] * 1024 < ((float)k)) m++;
-        inv[k] = m;
+        // bit 31: every u of bucket k (u < (k+1)/1024) stops at m, because
+        // cum_trans[m] already reaches the bucket's upper end, so the scan
+        // in select_dest need not read cum_trans (the same cell either way)
+        const bool pure = k < 1024 && (double)ct[m] >= (double)(k + 1) / 1024.0;
+        inv[k] = pure ? (int32_t)((uint32_t)m | CS_INV_PURE) : m;
     }
 }
 
@@ -513,10 +519,14 @@ __device__ __forceinline__ int select_infector(const float* ci, int n, double u,
 // destination cell: InvCDF then linear scan of cum_trans (src/Sweep.cpp:757-762)
 __device__ __forceinline__ int select_dest(const Dev& d, size_t rc, double u, unsigned* scanned = nullptr)
 {
-    int l = d.invcdf[rc * 1025 + (int)__builtin_floor(u * 1024)];
-    const float* ct = d.cum_trans + rc * d.NC;
+    const int32_t v = d.invcdf[rc * 1025 + (int)__builtin_floor(u * 1024)];
+    int l = (int)((uint32_t)v & ~CS_INV_PURE);
     const int l0 = l;
-    while (ct[l] < u) l++;
+    if (v >= 0) {  // not a pure bucket: the reference's scan
+        const float* ct = d.cum_trans + rc * d.NC;
+        while (ct[l] < u) l++;
+    }
+    // the reference reads cum_trans[l0 .. l] (one entry for a pure bucket)
     if (scanned) *scanned += (unsigned)(l - l0 + 1);
     return l;
 }

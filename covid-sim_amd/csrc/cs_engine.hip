@@ -692,7 +692,8 @@ int cs_alloc_realisations(cs_engine* e, int n_real, const uint64_t* seeds)
     d.seed = sd;
     AL(hot, RN);
     d.inf.base = d.hot; d.hflags.base = d.hot; d.itype.base = d.hot; d.infness.base = d.hot;
-    d.t_lat.base = d.hot; d.t_as.base = d.hot; d.t_ae.base = d.hot; d.t_gate.base = d.hot;
+    d.t_lat.base = d.hot; d.t_as.base = d.hot; d.t_ae.base = d.hot;
+    AL(gate, RN);
     AL(tim, RN);
     d.t_inf.base = d.tim; d.t_rod.base = d.tim; d.t_sari.base = d.tim; d.t_crit.base = d.tim;
     d.t_step.base = d.tim; d.t_det.base = d.tim; d.sev_cur.base = d.tim; d.sev_final.base = d.tim; d.bits.base = d.tim;
@@ -808,10 +809,10 @@ int cs_load_state(cs_engine* e, int real, const cs_state* s)
             if (s->latent_time) x.t_lat = s->latent_time[i];
             if (s->absent_start_time) x.t_as = s->absent_start_time[i];
             if (s->absent_stop_time) x.t_ae = s->absent_stop_time[i];
-            x.t_gate = 0;  // loaded timers: look at every one
             x.flags = e->no_links[i];  // and every household
         }
         HIPCHK(e, hipMemcpy(d.hot + o, h.data(), N * sizeof(cs::Hot), hipMemcpyHostToDevice));
+        HIPCHK(e, hipMemset(d.gate + o, 0, N * sizeof(uint16_t)));  // loaded timers: look at every one
         std::vector<cs::Tim> m(N);
         HIPCHK(e, hipMemcpy(m.data(), d.tim + o, N * sizeof(cs::Tim), hipMemcpyDeviceToHost));
         for (size_t i = 0; i < N; i++) {

@@ -361,7 +361,10 @@ __device__ void list_move_cell(const Dev& d, int mode, int rc, int32_t* pos, int
         d.listpos[rN + pb] = ph;
         d.perm[base + q] = pa;
         d.listpos[rN + pa] = q;
+        if (mode == 2) d.gate[base + ph] = d.gate[base + q];  // an infectious person stays: its gate moves with it
     }
+    if (mode == 1)  // [bnd, hi) now heads the infectious list: its new timers are all to be looked at
+        for (int i = threadIdx.x; i < k; i += blockDim.x) d.gate[base + bnd + i] = 0;
     if (mode == 2) {
         int nd = 0;
         for (int i = threadIdx.x; i < k; i += blockDim.x) nd += is_dead_code(d.inf[rN + d.batch[base + i]]);

@@ -79,7 +79,6 @@ __device__ void do_incub(const Dev& d, int r, int p, unsigned ts)
     if (d.inf[gp] != CS_LATENT) return;
     int ag = age_group(d, p);
     if (ag >= CS_NUM_AGE_GROUPS) ag = CS_NUM_AGE_GROUPS - 1;
-    d.t_gate[gp] = 0;  // new timers: the infectious scan looks at every one
     Stream s(d.seed[r], cw0(ts, PS_INCUB), (uint32_t)p, 0);
     const double tspd = P->time_steps_per_day;
     float infn = (float)P->age_infectiousness[ag];
@@ -292,9 +291,10 @@ __global__ void __launch_bounds__(256) k_infectious_scan(Dev d)
         const int r = rc / d.NC, c = rc % d.NC;
         const size_t rN = (size_t)r * d.N;
         Counters* cn = d.cnt + r;
-        const int p = d.perm[rN + uni(d.cstart[c] + d.cS[rc] + d.cL[rc]) + q];
+        const size_t sl = rN + uni(d.cstart[c] + d.cS[rc] + d.cL[rc]) + q;  // the person's list slot
+        if (d.gate[sl] > ts) continue;  // every timer of the slot's person lies after this step
+        const int p = d.perm[sl];
         const size_t gp = rN + p;
-        if (d.t_gate[gp] > ts) continue;  // every timer of p lies after this step
         int ag = age_group(d, p);
         if (ag >= CS_NUM_AGE_GROUPS) ag = CS_NUM_AGE_GROUPS - 1;
         uint16_t ct = (uint16_t)(d.t_lat[gp] + ((int)(P->latent_to_sympt_delay / P->model_time_step)));
@@ -356,7 +356,7 @@ __global__ void __launch_bounds__(256) k_infectious_scan(Dev d)
             if (d.t_step[gp] > ts && d.t_step[gp] < g) g = d.t_step[gp];
         }
         if (rod > ts && rod < g) g = rod;
-        d.t_gate[gp] = (uint16_t)g;
+        d.gate[sl] = (uint16_t)g;
     }
     nt = wave_sum(nt);
     if ((threadIdx.x & 63) == 0 && nt) atomicAdd(&d.wc->n_transition, nt);
@@ -422,7 +422,7 @@ __global__ void k_reset_persons(Dev d)
         d.bits[g] = 0; d.t_det[g] = 0; d.dct[g] = 0;
         d.inf[g] = CS_SUSCEPTIBLE;
         d.t_lat[g] = d.t_rod[g] = 0; d.t_inf[g] = 0;
-        d.t_gate[g] = 0;
+        d.gate[g] = 0;
         d.hflags[g] = no_place_links(d.plk[k].lk, d.NPT) ? (uint8_t)HOT_NO_LINKS : (uint8_t)0;
         d.infector[g] = -1; d.itype[g] = 0;
         d.qc[g] = 2; d.t_qs[g] = CS_UNSET_TIME;

@@ -84,7 +84,7 @@ struct Hot {  // per (realisation, person): 16 bytes, one vector load
     uint8_t flags;    // HOT_HH_DONE
     int16_t itype;    // infect_type
     uint16_t t_lat, t_as, t_ae;  // latent time, absence start and end
-    uint16_t t_gate;  // no timer of an infectious person fires before this step
+    uint16_t spare;
     float infness;    // infectiousness (sign: symptomatic)
 };
 struct Tim {  // per (realisation, person): the disease timers and severity, 16 bytes
@@ -195,7 +195,7 @@ struct Dev {
     CS_FLD(Hot, t_lat) t_lat;
     CS_FLD(Hot, t_as) t_as;
     CS_FLD(Hot, t_ae) t_ae;
-    CS_FLD(Hot, t_gate) t_gate;
+    uint16_t* gate;       // [R][N] by list slot: no timer of the infectious person in the slot fires before this step
     Tim* tim;             // [R][N]
     CS_FLD(Tim, t_inf) t_inf;
     CS_FLD(Tim, t_rod) t_rod;

@@ -35,6 +35,7 @@ struct cs_engine {
     int32_t* rseed_buf = nullptr;    // [R] storage behind Dev.rseed_n
     int ts_max = 0, ts_n = 0;
     std::vector<uint8_t> no_links;   // [N] HOT_NO_LINKS of each person, for loaded states
+    std::vector<uint8_t> h_age, h_flags;  // [N] the copies a loaded state's Hot records take
     bool ts_fresh = true;            // no sample since the last alloc/reset/load: the series restarts
     unsigned grid_waves = 4096;      // blocks of 4 waves for the per-person passes (a wave per chunk)
     unsigned grid_cells = 2048;      // blocks for the per-cell spatial sums (a block per cell)
@@ -564,6 +565,8 @@ int cs_set_population(cs_engine* e, const cs_population* pop)
         if (pop->hh_first[h] < 0 || pop->hh_first[h] + pop->hh_nh[h] > N || pop->hh_nhr[h] < 1 || pop->hh_nhr[h] > CS_MAX_HOUSEHOLD_SIZE)
             return set_err(e, CS_EINVAL, "household %d out of range", h);
     e->no_links.assign(N, 0);
+    e->h_age.assign(pop->age, pop->age + N);
+    e->h_flags.assign(pop->flags, pop->flags + N);
     for (int i = 0; i < N; i++)
         e->no_links[i] = no_place_links_host(pop->place_links + (size_t)i * CS_MAX_PLACE_TYPES, e->NPT) ? cs::HOT_NO_LINKS : 0;
     std::vector<int32_t> padunit(N, 0);
@@ -810,6 +813,8 @@ int cs_load_state(cs_engine* e, int real, const cs_state* s)
             if (s->absent_start_time) x.t_as = s->absent_start_time[i];
             if (s->absent_stop_time) x.t_ae = s->absent_stop_time[i];
             x.flags = e->no_links[i];  // and every household
+            x.age = e->h_age[i];
+            x.pflags = e->h_flags[i];
         }
         HIPCHK(e, hipMemcpy(d.hot + o, h.data(), N * sizeof(cs::Hot), hipMemcpyHostToDevice));
         HIPCHK(e, hipMemset(d.gate + o, 0, N * sizeof(uint16_t)));  // loaded timers: look at every one

@@ -711,10 +711,10 @@ __global__ void __launch_bounds__(256) k_infect_places(Dev d)
                         const int m = d.pl_mem[pt][mstart + goff + idx];
                         w_cand++;
                         double PS = place_susc(d, r, m, pt, ts);
-                        if (pass == 1 && care(d, i) && care(d, m) &&
+                        if (pass == 1 && care_g(d, gi) && care_g(d, rN + m) &&
                             d.pglinks[(size_t)i * CS_MAX_PLACE_TYPES + pt] != d.pglinks[(size_t)m * CS_MAX_PLACE_TYPES + pt])
                             PS *= P->care_home_resident_place_scaling;
-                        if ((pt == P->care_home_place_type) && ((!care(d, i)) || (!care(d, m)))) PS *= P->care_home_worker_group_scaling;
+                        if ((pt == P->care_home_place_type) && ((!care_g(d, gi)) || (!care_g(d, rN + m)))) PS *= P->care_home_worker_group_scaling;
                         const size_t gm = rN + m;
                         if (d.inf[gm] == CS_SUSCEPTIBLE && !host_absent(d, gm, ts)) {
                             PS *= person_susc(d, r, m, ts, i) * Pc / Pc;  // (susc * Pc) / Pc, rounded as the reference does

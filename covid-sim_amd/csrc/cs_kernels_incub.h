@@ -424,6 +424,8 @@ __global__ void k_reset_persons(Dev d)
         d.t_lat[g] = d.t_rod[g] = 0; d.t_inf[g] = 0;
         d.gate[g] = 0;
         d.hflags[g] = no_place_links(d.plk[k].lk, d.NPT) ? (uint8_t)HOT_NO_LINKS : (uint8_t)0;
+        d.hage[g] = d.age[k];
+        d.hpfl[g] = d.pflags[k];
         d.infector[g] = -1; d.itype[g] = 0;
         d.qc[g] = 2; d.t_qs[g] = CS_UNSET_TIME;
         Stream s(d.seed[r], cw0(0, PS_INIT), (uint32_t)k, 0);

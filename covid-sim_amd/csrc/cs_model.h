@@ -84,7 +84,7 @@ struct Hot {  // per (realisation, person): 16 bytes, one vector load
     uint8_t flags;    // HOT_HH_DONE
     int16_t itype;    // infect_type
     uint16_t t_lat, t_as, t_ae;  // latent time, absence start and end
-    uint16_t spare;
+    uint8_t age, pflags;  // copies of the person's PStat fields, read with this record
     float infness;    // infectiousness (sign: symptomatic)
 };
 struct Tim {  // per (realisation, person): the disease timers and severity, 16 bytes
@@ -190,6 +190,8 @@ struct Dev {
     Hot* hot;             // [R][N]
     CS_FLD(Hot, inf) inf;
     CS_FLD(Hot, flags) hflags;
+    CS_FLD(Hot, age) hage;
+    CS_FLD(Hot, pflags) hpfl;
     CS_FLD(Hot, itype) itype;
     CS_FLD(Hot, infness) infness;
     CS_FLD(Hot, t_lat) t_lat;
@@ -266,6 +268,11 @@ __device__ __forceinline__ int age_group(const Dev& d, int p) { return (int)d.ag
 __device__ __forceinline__ bool care(const Dev& d, int p) { return (d.pflags[p] & 1) != 0; }
 __device__ __forceinline__ int keyw(const Dev& d, int p) { return (d.pflags[p] >> 1) & 1; }
 __device__ __forceinline__ bool esoc(const Dev& d, int p) { return (d.pflags[p] & 4) != 0; }
+// the same from the copies in Hot (gp = r * N + p): a caller that holds the
+// person's Hot record then needs no PStat line
+__device__ __forceinline__ int age_group_g(const Dev& d, size_t gp) { return (int)d.hage[gp] / CS_AGE_GROUP_WIDTH; }
+__device__ __forceinline__ bool care_g(const Dev& d, size_t gp) { return (d.hpfl[gp] & 1) != 0; }
+__device__ __forceinline__ bool esoc_g(const Dev& d, size_t gp) { return (d.hpfl[gp] & 4) != 0; }
 __device__ __forceinline__ bool is_dead_code(int8_t s) { return s == CS_DEAD_ASYMP || s == CS_DEAD_SYMP; }
 __device__ __forceinline__ bool host_absent(const Dev& d, size_t gp, unsigned ts) { return d.t_as[gp] <= ts && d.t_ae[gp] > ts; }
 __device__ __forceinline__ bool place_closed(const Dev& d, int r, int t, int k, unsigned ts)
@@ -444,7 +451,7 @@ __device__ __forceinline__ double house_inf(const Dev& d, int r, int p, unsigned
          * (dc ? P->efficacies[5][CS_HOUSE] : 1.0)
          * ((qu && !dc && !iso) ? P->efficacies[1][CS_HOUSE] : 1.0)
          * P->household_denom_lookup[d.hnhr[d.hh[p]] - 1]
-         * (care(d, p) ? P->care_home_resident_household_scaling : 1.0)
+         * (care_g(d, gp) ? P->care_home_resident_household_scaling : 1.0)
          * (treated(d, gp, ts) ? P->treat_inf_drop : 1.0)
          * (vacced(d, gp, ts) ? P->vacc_inf_drop : 1.0)
          * (P->no_infectiousness_sd_in_hh ? ((d.infness[gp] < 0) ? P->sympt_infectiousness : P->asympt_infectiousness)
@@ -459,8 +466,8 @@ __device__ __forceinline__ double place_inf(const Dev& d, int r, int p, int pt, 
     bool iso = isolated(d, gp, ts), dc = dct_traced(d, gp), qu = quarantined(d, gp, ts);
     return ((iso && !dc) ? P->efficacies[0][pt] : 1.0)
          * (dc ? P->efficacies[5][pt] : 1.0)
-         * ((qu && !care(d, p) && !dc && !iso) ? P->efficacies[1][pt] : 1.0)
-         * ((d.inf[gp] == CS_CASE && !care(d, p)) ? P->sympt_place_type_contact_rate[pt] : 1.0)
+         * ((qu && !care_g(d, gp) && !dc && !iso) ? P->efficacies[1][pt] : 1.0)
+         * ((d.inf[gp] == CS_CASE && !care_g(d, gp)) ? P->sympt_place_type_contact_rate[pt] : 1.0)
          * P->place_type_trans[pt] / P->place_type_group_size_param1[pt] * person_inf(d, gp, ts);
 }
 
@@ -471,9 +478,9 @@ __device__ __forceinline__ double spatial_inf(const Dev& d, int r, int p, unsign
     bool iso = isolated(d, gp, ts), dc = dct_traced(d, gp), qu = quarantined(d, gp, ts);
     return ((iso && !dc) ? P->efficacies[0][CS_SPATIAL] : 1.0)
          * (dc ? P->efficacies[5][CS_SPATIAL] : 1.0)
-         * ((qu && !care(d, p) && !dc && !iso) ? P->efficacies[1][CS_SPATIAL] : 1.0)
+         * ((qu && !care_g(d, gp) && !dc && !iso) ? P->efficacies[1][CS_SPATIAL] : 1.0)
          * ((d.inf[gp] == CS_CASE) ? P->sympt_spatial_contact_rate : 1.0)
-         * P->relative_spatial_contact[age_group(d, p)]
+         * P->relative_spatial_contact[age_group_g(d, gp)]
          * person_inf(d, gp, ts);
 }
 
@@ -481,8 +488,8 @@ __device__ __forceinline__ double person_susc(const Dev& d, int r, int p, unsign
 {
     const cs_params* P = d.P;
     size_t gp = (size_t)r * d.N + p;
-    return P->waifw[age_group(d, p)][age_group(d, infector)]
-         * P->age_susceptibility[age_group(d, p)] * (double)d.susc[gp]
+    return P->waifw[age_group_g(d, gp)][age_group_g(d, (size_t)r * d.N + infector)]
+         * P->age_susceptibility[age_group_g(d, gp)] * (double)d.susc[gp]
          * (treated(d, gp, ts) ? P->treat_susc_drop : 1.0)
          * (vacced(d, gp, ts) ? (vacced_switch(d, gp) ? P->vacc_susc_drop2 : P->vacc_susc_drop) : 1.0);
 }
@@ -492,9 +499,9 @@ __device__ __forceinline__ double house_susc(const Dev& d, int r, int p, unsigne
     const cs_params* P = d.P;
     size_t gp = (size_t)r * d.N + p;
     return person_susc(d, r, p, ts, infector)
-         * (socdist(d, r, p) ? (esoc(d, p) ? P->efficacies[4][CS_HOUSE] : P->efficacies[3][CS_HOUSE]) : 1.0)
+         * (socdist(d, r, p) ? (esoc_g(d, gp) ? P->efficacies[4][CS_HOUSE] : P->efficacies[3][CS_HOUSE]) : 1.0)
          * (dct_traced(d, gp) ? P->efficacies[5][CS_HOUSE] : 1.0)
-         * (care(d, p) ? P->care_home_resident_household_scaling : 1.0);
+         * (care_g(d, gp) ? P->care_home_resident_household_scaling : 1.0);
 }
 
 __device__ __forceinline__ double place_susc(const Dev& d, int r, int p, int pt, unsigned ts)
@@ -502,8 +509,8 @@ __device__ __forceinline__ double place_susc(const Dev& d, int r, int p, int pt,
     const cs_params* P = d.P;
     size_t gp = (size_t)r * d.N + p;
     bool dc = dct_traced(d, gp);
-    return ((quarantined(d, gp, ts) && !care(d, p) && !dc) ? P->efficacies[1][pt] : 1.0)
-         * (socdist(d, r, p) ? (esoc(d, p) ? P->efficacies[4][pt] : P->efficacies[3][pt]) : 1.0)
+    return ((quarantined(d, gp, ts) && !care_g(d, gp) && !dc) ? P->efficacies[1][pt] : 1.0)
+         * (socdist(d, r, p) ? (esoc_g(d, gp) ? P->efficacies[4][pt] : P->efficacies[3][pt]) : 1.0)
          * (dc ? P->efficacies[5][pt] : 1.0);
 }
 
@@ -512,10 +519,10 @@ __device__ __forceinline__ double spatial_susc(const Dev& d, int r, int p, unsig
     const cs_params* P = d.P;
     size_t gp = (size_t)r * d.N + p;
     bool dc = dct_traced(d, gp);
-    return ((quarantined(d, gp, ts) && !care(d, p) && !dc) ? P->efficacies[1][CS_SPATIAL] : 1.0)
-         * (socdist(d, r, p) ? (esoc(d, p) ? P->efficacies[4][CS_SPATIAL] : P->efficacies[3][CS_SPATIAL]) : 1.0)
+    return ((quarantined(d, gp, ts) && !care_g(d, gp) && !dc) ? P->efficacies[1][CS_SPATIAL] : 1.0)
+         * (socdist(d, r, p) ? (esoc_g(d, gp) ? P->efficacies[4][CS_SPATIAL] : P->efficacies[3][CS_SPATIAL]) : 1.0)
          * (dc ? P->efficacies[5][CS_SPATIAL] : 1.0)
-         * P->relative_spatial_contact_susc[age_group(d, p)];
+         * P->relative_spatial_contact_susc[age_group_g(d, gp)];
 }
 
 }  // namespace cs

@@ -694,12 +694,10 @@ int cs_alloc_realisations(cs_engine* e, int n_real, const uint64_t* seeds)
     HIPCHK(e, hipMemcpy(sd, seeds, R * 8, hipMemcpyHostToDevice));
     d.seed = sd;
     AL(hot, RN);
-    d.inf.base = d.hot; d.hflags.base = d.hot; d.itype.base = d.hot; d.infness.base = d.hot;
-    d.t_lat.base = d.hot; d.t_as.base = d.hot; d.t_ae.base = d.hot;
+    d.bind_hot(d.hot);
     AL(gate, RN);
     AL(tim, RN);
-    d.t_inf.base = d.tim; d.t_rod.base = d.tim; d.t_sari.base = d.tim; d.t_crit.base = d.tim;
-    d.t_step.base = d.tim; d.t_det.base = d.tim; d.sev_cur.base = d.tim; d.sev_final.base = d.tim; d.bits.base = d.tim;
+    d.bind_tim(d.tim);
     AL(qc, RN); AL(dct, RN);
     AL(listpos, RN); AL(infector, RN); AL(susc, RN); AL(prob_care, RN);
     AL(t_ts, RN); AL(t_te, RN); AL(t_vs, RN); AL(t_is, RN); AL(t_qs, RN);
@@ -715,7 +713,7 @@ int cs_alloc_realisations(cs_engine* e, int n_real, const uint64_t* seeds)
     d.ev_cap = (unsigned)cap;
     AL(ev_key, cap); AL(ev_gp, cap); AL(ev_infector, cap); AL(ev_type, cap); AL(ev_count, 1);
     AL(clr, RN); AL(batch, RN); AL(bcount, RC);
-    d.claim.base = d.lastk.base = d.clr;
+    d.bind_clr(d.clr);
     AL(care_gp, RN); AL(care_k, RN); AL(care_abs, RN); AL(care_count, 1);
     AL(last_tot, CS_TS_LAST * R);
     d.due_cap = (unsigned)std::min<size_t>(std::max<size_t>(RN / 64, 4096), 0x7fffffff);

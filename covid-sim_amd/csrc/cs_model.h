@@ -245,6 +245,21 @@ struct Dev {
     ClaimRec* clr;        // [R][N]
     CS_FLD(ClaimRec, claim) claim;
     CS_FLD(ClaimRec, lastk) lastk;
+    // point a record array's field accessors at it, all in one place per
+    // record (a field left unbound would index from address zero)
+    void bind_hot(Hot* p)
+    {
+        hot = p;
+        inf.base = p; hflags.base = p; hage.base = p; hpfl.base = p; itype.base = p; infness.base = p;
+        t_lat.base = p; t_as.base = p; t_ae.base = p;
+    }
+    void bind_tim(Tim* p)
+    {
+        tim = p;
+        t_inf.base = p; t_rod.base = p; t_sari.base = p; t_crit.base = p; t_step.base = p; t_det.base = p;
+        sev_cur.base = p; sev_final.base = p; bits.base = p;
+    }
+    void bind_clr(ClaimRec* p) { clr = p; claim.base = p; lastk.base = p; }
     int32_t* batch;       // [R][N] by permutation slot range of the cell
     int32_t* bcount;      // [R*NC]
     int32_t *care_gp, *care_k;

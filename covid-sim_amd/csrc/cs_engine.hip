@@ -34,11 +34,11 @@ struct cs_engine {
     double* rbeta_buf = nullptr;     // [R] storage behind Dev.rbeta
     int32_t* rseed_buf = nullptr;    // [R] storage behind Dev.rseed_n
     int ts_max = 0, ts_n = 0;
-    std::vector<uint8_t> no_links;   // [N] HOT_NO_LINKS of each person, for loaded states
+    std::vector<uint8_t> flags0;     // [N] each person's Hot flags after a reset (no links, household size), for loaded states
     std::vector<uint8_t> h_age, h_flags;  // [N] the copies a loaded state's Hot records take
     bool ts_fresh = true;            // no sample since the last alloc/reset/load: the series restarts
     unsigned grid_waves = 4096;      // blocks of 4 waves for the per-person passes (a wave per chunk)
-    unsigned grid_cells = 2048;      // blocks for the per-cell spatial sums (a block per cell)
+    unsigned grid_cells = 2048;      // blocks for the per-cell spatial sums (4 waves, a wave per cell)
     unsigned grid_moves = 4096;      // blocks for the batched list moves (a block per moved cell; not tuned)
     // replayed step graphs (cs_run_steps): one graph per output step
     bool use_graphs = true;
@@ -562,13 +562,15 @@ int cs_set_population(cs_engine* e, const cs_population* pop)
         if (k < 0 || k >= N) return set_err(e, CS_EINVAL, "cell member out of range");
     }
     for (int h = 0; h < e->NH; h++)
-        if (pop->hh_first[h] < 0 || pop->hh_first[h] + pop->hh_nh[h] > N || pop->hh_nhr[h] < 1 || pop->hh_nhr[h] > CS_MAX_HOUSEHOLD_SIZE)
+        if (pop->hh_first[h] < 0 || pop->hh_first[h] + pop->hh_nh[h] > N || pop->hh_nh[h] > CS_MAX_HOUSEHOLD_SIZE || pop->hh_nhr[h] < 1 ||
+            pop->hh_nhr[h] > CS_MAX_HOUSEHOLD_SIZE)
             return set_err(e, CS_EINVAL, "household %d out of range", h);
-    e->no_links.assign(N, 0);
+    e->flags0.assign(N, 0);
     e->h_age.assign(pop->age, pop->age + N);
     e->h_flags.assign(pop->flags, pop->flags + N);
     for (int i = 0; i < N; i++)
-        e->no_links[i] = no_place_links_host(pop->place_links + (size_t)i * CS_MAX_PLACE_TYPES, e->NPT) ? cs::HOT_NO_LINKS : 0;
+        e->flags0[i] = (uint8_t)((no_place_links_host(pop->place_links + (size_t)i * CS_MAX_PLACE_TYPES, e->NPT) ? cs::HOT_NO_LINKS : 0) |
+                                 (pop->hh_nh[pop->hh[i]] << cs::HOT_NH_SHIFT));
     std::vector<int32_t> padunit(N, 0);
     e->adunits_valid = pop->mcell_adunit != nullptr;
     for (int i = 0; i < N && e->adunits_valid; i++) {
@@ -810,7 +812,7 @@ int cs_load_state(cs_engine* e, int real, const cs_state* s)
             if (s->latent_time) x.t_lat = s->latent_time[i];
             if (s->absent_start_time) x.t_as = s->absent_start_time[i];
             if (s->absent_stop_time) x.t_ae = s->absent_stop_time[i];
-            x.flags = e->no_links[i];  // and every household
+            x.flags = e->flags0[i];  // and every household
             x.age = e->h_age[i];
             x.pflags = e->h_flags[i];
         }

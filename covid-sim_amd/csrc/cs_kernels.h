@@ -564,19 +564,22 @@ __global__ void __launch_bounds__(256) k_infect_households(Dev d)
         const int gen = d.itype[gi] / CS_INFECT_TYPE_MASK;
         // household (src/Sweep.cpp:331-377)
         if (sc.beta_h > 0) {
-            const int h = d.hh[i], nh = d.hnh[h];
+            const unsigned fl = d.hflags[gi];
+            const int nh = (int)(fl >> HOT_NH_SHIFT);  // the household's size, from i's own record
             if (nh > 1) {
-                const int f0 = d.hfirst[h];
                 w_hh_inf++;
                 w_hh_mem += nh;
                 // the susceptible members first: with none, nothing below can
                 // infect, so the household's hazard is not needed; and none
                 // will be susceptible again, so later steps skip the scan
+                // (and read no household record)
                 unsigned smask = 0;
-                if (!(d.hflags[gi] & HOT_HH_DONE)) {
+                int f0 = 0;
+                if (!(fl & HOT_HH_DONE)) {
+                    f0 = d.hfirst[d.hh[i]];
                     for (int m = f0; m < f0 + nh; m++)
                         if (d.inf[rN + m] == CS_SUSCEPTIBLE) smask |= 1u << (m - f0);
-                    if (!smask) d.hflags[gi] = (uint8_t)(d.hflags[gi] | HOT_HH_DONE);
+                    if (!smask) d.hflags[gi] = (uint8_t)(fl | HOT_HH_DONE);
                 }
                 if (smask) {
                     double HI = sc.beta_h * house_inf(d, r, i, ts);
@@ -746,87 +749,64 @@ __global__ void __launch_bounds__(256) k_infect_places(Dev d)
 // the same prefixes as a sequential pass; each prefix is rounded to double
 // once, then stored as float as the reference does.
 #define CS_SUM_BLOCK 256
-__device__ void cell_spatial_sum(const Dev& d, const StepScalars& sc, int rc, unsigned long long* shi, unsigned long long* slo,
-                                 int* sK, double* ssis);
 
+// one wave per cell with I > 0; the slots are read in rounds of 64, one per
+// lane (coalesced).  Pass 1 sums the cell's total (the draw count needs it),
+// pass 2 scans each round across the wave with the carry of the rounds
+// before: 128-bit integer adds are exact and associative, so every prefix
+// equals the sequential pass's.
 __global__ void __launch_bounds__(CS_SUM_BLOCK) k_cell_spatial_sum(Dev d)
 {
     const StepScalars sc = *d.sc;
-    __shared__ unsigned long long shi[CS_SUM_BLOCK / 64], slo[CS_SUM_BLOCK / 64];  // per-wave sums
-    __shared__ int sK;
-    __shared__ double ssis;
     const unsigned ncells = d.wl[WL_CELLS];
-    for (unsigned a = blockIdx.x; a < ncells; a += gridDim.x) {
-        __syncthreads();  // the previous cell's use of the shared arrays is over
-        cell_spatial_sum(d, sc, d.cells[a], shi, slo, &sK, &ssis);
-    }
-}
-
-// one cell with I > 0 (every thread of the block calls it)
-__device__ void cell_spatial_sum(const Dev& d, const StepScalars& sc, int rc, unsigned long long* shi, unsigned long long* slo,
-                                 int* sK_p, double* ssis_p)
-{
-    int& sK = *sK_p;
-    double& ssis = *ssis_p;
-    const int I = d.cI[rc];
-    const int r = rc / d.NC, c = rc % d.NC;
-    const double beta_s = beta_s_of(d, sc, r);
-    if (!(beta_s > 0)) {
-        if (threadIdx.x == 0) atomicAdd(&d.wc->n_inf, (unsigned long long)I);
-        return;
-    }
-    const size_t sbase = (size_t)r * d.N + d.cstart[c] + d.cS[rc] + d.cL[rc];
-    const int per = (I + CS_SUM_BLOCK - 1) / CS_SUM_BLOCK;
-    const int j0 = threadIdx.x * per, j1 = min(I, j0 + per);
-    Fx acc = {0, 0};
-    for (int j = j0; j < j1; j++) acc = fx_add(acc, fx_from(fabs(d.si_buf[sbase + j])));
-    // exclusive scan of the thread sums: 128-bit integer adds are exact and
-    // associative, so this parallel scan equals a sequential pass
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    Fx x = acc;
-    for (int o = 1; o < 64; o <<= 1) {
-        const Fx y = {__shfl_up(x.hi, o, 64), __shfl_up(x.lo, o, 64)};
-        if (lane >= o) x = fx_add(x, y);
-    }
-    if (lane == 63) {
-        shi[w] = x.hi;
-        slo[w] = x.lo;
-    }
-    Fx excl = {__shfl_up(x.hi, 1, 64), __shfl_up(x.lo, 1, 64)};
-    if (lane == 0) excl = Fx{0, 0};
-    __syncthreads();
-    Fx before = {0, 0}, all = {0, 0};
-    for (int k = 0; k < CS_SUM_BLOCK / 64; k++) {
-        const Fx t = {shi[k], slo[k]};
-        if (k < w) before = fx_add(before, t);
-        all = fx_add(all, t);
-    }
-    if (threadIdx.x == 0) {
-        const double sis = fx_to_double(all);
+    const int lane = threadIdx.x & 63;
+    for (unsigned a = grid_wave(); a < ncells; a += grid_nwaves()) {
+        const int rc = uni(d.cells[a]);
+        const int I = uni(d.cI[rc]);
+        const int r = rc / d.NC, c = rc % d.NC;
+        const double beta_s = beta_s_of(d, sc, r);
+        if (!(beta_s > 0)) {
+            if (lane == 0) atomicAdd(&d.wc->n_inf, (unsigned long long)I);
+            continue;
+        }
+        const size_t sbase = (size_t)r * d.N + uni(d.cstart[c] + d.cS[rc] + d.cL[rc]);
+        const double* si = d.si_buf + sbase;
+        Fx acc = {0, 0};
+        for (int j = lane; j < I; j += 64) acc = fx_add(acc, fx_from(fabs(si[j])));
+        for (int o = 32; o > 0; o >>= 1) acc = fx_add(acc, Fx{__shfl_xor(acc.hi, o, 64), __shfl_xor(acc.lo, o, 64)});
+        const double sis = fx_to_double(acc);
         int K = 0;
-        if (sis > 0) {
-            Stream s(d.seed[r], cw0(sc.ts, PS_SP_COUNT), (uint32_t)c, 0);
-            K = poisson(s, sis * beta_s * ((double)d.tot_prob[rc]));
+        if (lane == 0) {
+            if (sis > 0) {
+                Stream s(d.seed[r], cw0(sc.ts, PS_SP_COUNT), (uint32_t)c, 0);
+                K = poisson(s, sis * beta_s * ((double)d.tot_prob[rc]));
+            }
+            d.spK[rc] = K;
+            atomicAdd(&d.wc->n_inf, (unsigned long long)I);
         }
-        d.spK[rc] = K;
-        sK = K;
-        ssis = sis;
-        atomicAdd(&d.wc->n_inf, (unsigned long long)I);
-    }
-    __syncthreads();
-    Fx run = fx_add(before, excl);
-    const int K = sK;
-    const float fs = (float)ssis;
-    for (int j = j0; j < j1; j++) {
-        const double v = d.si_buf[sbase + j];
-        run = fx_add(run, fx_from(fabs(v)));
-        const double p = fx_to_double(run);
-        float ci = __builtin_signbit(v) ? (float)-p : (float)p;
-        if (K > 0) {
-            if (j < I - 1) ci /= fs;
-            else ci = (ci < 0) ? -1.0f : 1.0f;
+        K = __shfl(K, 0, 64);
+        const float fs = (float)sis;
+        float* ci_out = d.ci_buf + sbase;
+        Fx carry = {0, 0};
+        for (int b = 0; b < I; b += 64) {
+            const int j = b + lane;
+            const double v = (j < I) ? si[j] : 0.0;
+            Fx x = fx_from(fabs(v));
+            for (int o = 1; o < 64; o <<= 1) {
+                const Fx y = {__shfl_up(x.hi, o, 64), __shfl_up(x.lo, o, 64)};
+                if (lane >= o) x = fx_add(x, y);
+            }
+            if (j < I) {
+                const double pv = fx_to_double(fx_add(carry, x));
+                float ci = __builtin_signbit(v) ? (float)-pv : (float)pv;
+                if (K > 0) {
+                    if (j < I - 1) ci /= fs;
+                    else ci = (ci < 0) ? -1.0f : 1.0f;
+                }
+                ci_out[j] = ci;
+            }
+            carry = fx_add(carry, Fx{__shfl(x.hi, 63, 64), __shfl(x.lo, 63, 64)});
         }
-        d.ci_buf[sbase + j] = ci;
     }
 }
 

@@ -423,7 +423,7 @@ __global__ void k_reset_persons(Dev d)
         d.inf[g] = CS_SUSCEPTIBLE;
         d.t_lat[g] = d.t_rod[g] = 0; d.t_inf[g] = 0;
         d.gate[g] = 0;
-        d.hflags[g] = no_place_links(d.plk[k].lk, d.NPT) ? (uint8_t)HOT_NO_LINKS : (uint8_t)0;
+        d.hflags[g] = (uint8_t)((no_place_links(d.plk[k].lk, d.NPT) ? HOT_NO_LINKS : 0) | (d.hnh[d.hh[k]] << HOT_NH_SHIFT));
         d.hage[g] = d.age[k];
         d.hpfl[g] = d.pflags[k];
         d.infector[g] = -1; d.itype[g] = 0;

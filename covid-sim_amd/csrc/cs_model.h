@@ -64,8 +64,11 @@ enum { FLAG_OVERFLOW = 1, FLAG_RANGE = 2, FLAG_BATCH = 4, FLAG_STUCK = 8 };
 // Hot.flags: HOT_HH_DONE, the person's household has no susceptible member
 // left (none can return to susceptible, so the household pass may skip it
 // for good); HOT_NO_LINKS, the person has no place link (a copy of PLinks
-// the place pass can test without reading PLinks)
-enum { HOT_HH_DONE = 1, HOT_NO_LINKS = 2 };
+// the place pass can test without reading PLinks); bits HOT_NH_SHIFT and up,
+// the size of the person's household (at most CS_MAX_HOUSEHOLD_SIZE), so an
+// infector whose household needs no scan reads no household record
+enum { HOT_HH_DONE = 1, HOT_NO_LINKS = 2, HOT_NH_SHIFT = 2 };
+static_assert((CS_MAX_HOUSEHOLD_SIZE << HOT_NH_SHIFT) <= 0xff, "household size fits the flags byte");
 
 __host__ __device__ __forceinline__ bool no_place_links(const int32_t* lk, int npt)
 {
@@ -81,7 +84,7 @@ __host__ __device__ __forceinline__ bool no_place_links(const int32_t* lk, int n
 
 struct Hot {  // per (realisation, person): 16 bytes, one vector load
     int8_t inf;       // InfStat code
-    uint8_t flags;    // HOT_HH_DONE
+    uint8_t flags;    // HOT_HH_DONE, HOT_NO_LINKS, household size << HOT_NH_SHIFT
     int16_t itype;    // infect_type
     uint16_t t_lat, t_as, t_ae;  // latent time, absence start and end
     uint8_t age, pflags;  // copies of the person's PStat fields, read with this record

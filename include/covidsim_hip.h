@@ -379,7 +379,8 @@ int  cs_get_queue_capacity(cs_engine* e, uint32_t* events, uint32_t* draw_chunks
 int  cs_set_due_capacity(cs_engine* e, uint32_t per_step);
 /* launch shapes: blocks of the per-person passes (4 waves each, one wave
  * per chunk of 64 persons or 256 spatial draws) and of the per-cell spatial
- * sums (a block per cell); 0 keeps a value.  Results do not depend on them. */
+ * sums (4 waves each, one wave per cell); 0 keeps a value.  Results do not
+ * depend on them. */
 int  cs_set_launch_shape(cs_engine* e, uint32_t grid_waves, uint32_t grid_cells);
 /* autotuner: at the current state and time t, times the InfectSweep event
  * phase (it reads the state and writes only scratch, so it can be repeated)

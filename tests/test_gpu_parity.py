@@ -322,3 +322,28 @@ def test_autotune_keeps_results(engmod, csa):
     for eng in (g, o):
         eng.run_steps(30.0, 30 * tspd, tspd)
     assert np.array_equal(g.timeseries(60), o.timeseries(60))
+
+
+def test_household_beyond_the_size_table_is_rejected(engmod, csa):
+    """A household holds at most CS_MAX_HOUSEHOLD_SIZE persons (the
+    reference sets nh = nhr from the size table, src/SetupModel.cpp:1287-1288):
+    a larger one is refused by the host checks, and the engine then takes a
+    valid population and runs it."""
+    arrays, pop, params, _ = mini()
+    popa = dict(pop.a)
+    nh = popa["h_nh"].copy()
+    nh[0] = csa.abi.CS_MAX_HOUSEHOLD_SIZE + 1
+    popa["h_nh"] = nh
+    g = engmod.Engine(0)
+    g.set_params(params)
+    with pytest.raises(engmod.CSError, match="household 0 out of range"):
+        g.set_population(csa.population.Population(popa))
+    g.set_population(pop)
+    g.alloc(np.array([5], np.uint64))
+    o = OracleEngine(params, pop)
+    o.alloc(np.array([5], np.uint64))
+    tspd = int(params.time_steps_per_day)
+    for eng in (g, o):
+        eng.reset(10)
+        eng.run_steps(0.0, 10 * tspd, tspd)
+    assert np.array_equal(g.timeseries(10), o.timeseries(10))

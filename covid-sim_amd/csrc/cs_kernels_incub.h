@@ -6,18 +6,35 @@
 
 namespace cs {
 
+// cadd from every active lane, merged: the lanes that add to one address
+// make one atomic between them (a wave's lanes share their realisation's
+// counters, so most of a wave's adds land on a few addresses).  v is the
+// same in every lane; integer sums, so every total equals that of one add
+// per lane.
+__device__ __forceinline__ void cadd_merged(unsigned long long* a, long long v)
+{
+    for (;;) {  // one pass per distinct address among the lanes still here
+        const uint64_t lead = uni((uint64_t)(uintptr_t)a);
+        if ((uint64_t)(uintptr_t)a == lead) {
+            const unsigned long long m = __ballot(1);
+            const unsigned below = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+            if (below == 0) atomicAdd(a, (unsigned long long)(v * (long long)__popcll(m)));
+            return;
+        }
+    }
+}
 __device__ __forceinline__ void sev_to(Counters* c, int kind, int ag)
 {
-    cadd(&c->sev_prev[kind], 1); cadd(&c->sev_prev_age[kind][ag], 1);
-    cadd(&c->sev_cum[kind], 1); cadd(&c->sev_cum_age[kind][ag], 1);
+    cadd_merged(&c->sev_prev[kind], 1); cadd_merged(&c->sev_prev_age[kind][ag], 1);
+    cadd_merged(&c->sev_cum[kind], 1); cadd_merged(&c->sev_cum_age[kind][ag], 1);
 }
 __device__ __forceinline__ void sev_from(Counters* c, int kind, int ag)
 {
-    cadd(&c->sev_prev[kind], -1); cadd(&c->sev_prev_age[kind][ag], -1);
+    cadd_merged(&c->sev_prev[kind], -1); cadd_merged(&c->sev_prev_age[kind][ag], -1);
 }
 __device__ __forceinline__ void sev_cum_only(Counters* c, int kind, int ag)
 {
-    cadd(&c->sev_cum[kind], 1); cadd(&c->sev_cum_age[kind][ag], 1);
+    cadd_merged(&c->sev_cum[kind], 1); cadd_merged(&c->sev_cum_age[kind][ag], 1);
 }
 
 // holidays (src/Sweep.cpp:889-918) for holiday h: one thread per
@@ -77,7 +94,7 @@ __device__ void do_incub(const Dev& d, int r, int p, unsigned ts)
     const cs_params* P = d.P;
     const size_t gp = (size_t)r * d.N + p;
     if (d.inf[gp] != CS_LATENT) return;
-    int ag = age_group(d, p);
+    int ag = age_group_g(d, gp);
     if (ag >= CS_NUM_AGE_GROUPS) ag = CS_NUM_AGE_GROUPS - 1;
     Stream s(d.seed[r], cw0(ts, PS_INCUB), (uint32_t)p, 0);
     const double tspd = P->time_steps_per_day;
@@ -117,7 +134,7 @@ __device__ void do_incub(const Dev& d, int r, int p, unsigned ts)
         if (fin == CS_SEV_CRITICAL) { if (s.next() < P->cfr_critical[ag] * P->cfr_critical_scale) bits |= 1; }
         else if (fin == CS_SEV_SARI) { if (s.next() < P->cfr_sari[ag] * P->cfr_sari_scale) bits |= 1; }
         else if (fin == CS_SEV_ILI) { if (s.next() < P->cfr_ili[ag] * P->cfr_ili_scale) bits |= 1; }
-        if (care(d, p) && (fin == CS_SEV_CRITICAL || fin == CS_SEV_SARI) && (s.next() > P->care_home_rel_prob_hosp)) {
+        if (care_g(d, gp) && (fin == CS_SEV_CRITICAL || fin == CS_SEV_SARI) && (s.next() > P->care_home_rel_prob_hosp)) {
             bits |= 1;
             fin = CS_SEV_ILI;
         }
@@ -225,7 +242,7 @@ __device__ int do_case(const Dev& d, int r, int p, unsigned ts)
     const size_t gp = (size_t)r * d.N + p;
     int req = 0;
     if (d.inf[gp] != CS_INF_ALMOST_SYMPT) return 0;
-    int ag = age_group(d, p);
+    int ag = age_group_g(d, gp);
     if (ag >= CS_NUM_AGE_GROUPS) ag = CS_NUM_AGE_GROUPS - 1;
     Counters* cn = d.cnt + r;
     d.inf[gp] = CS_CASE;
@@ -238,14 +255,14 @@ __device__ int do_case(const Dev& d, int r, int p, unsigned ts)
         for (int j = 0; j < d.NPT; j++) {
             if ((d.plinks[(size_t)p * CS_MAX_PLACE_TYPES + j] >= 0) && (j != P->hotel_place_type) && (!host_absent(d, gp, ts)) &&
                 (P->sympt_place_type_withdrawal_prop[j] > 0)) {
-                if ((!care(d, p)) && ((P->sympt_place_type_withdrawal_prop[j] == 1) || (s.next() < P->sympt_place_type_withdrawal_prop[j]))) {
+                if ((!care_g(d, gp)) && ((P->sympt_place_type_withdrawal_prop[j] == 1) || (s.next() < P->sympt_place_type_withdrawal_prop[j]))) {
                     d.t_as[gp] = (uint16_t)(ts + adel);
                     d.t_ae[gp] = (uint16_t)(ts + adel + adur);
                     if ((!quarantined(d, gp, ts)) && (d.plinks[(size_t)p * CS_MAX_PLACE_TYPES + P->place_type_no_air_num - 1] >= 0) &&
-                        ((int)d.age[p] >= P->case_absent_child_age_cutoff))
-                        cadd(&cn->cumAC, 1);
-                    if (P->do_households && ((int)d.age[p] < P->case_absent_child_age_cutoff)) {
-                        if (!quarantined(d, gp, ts)) cadd(&cn->cumACS, 1);
+                        ((int)d.hage[gp] >= P->case_absent_child_age_cutoff))
+                        cadd_merged(&cn->cumAC, 1);
+                    if (P->do_households && ((int)d.hage[gp] < P->case_absent_child_age_cutoff)) {
+                        if (!quarantined(d, gp, ts)) cadd_merged(&cn->cumACS, 1);
                         if (d.prob_care[gp] < P->case_absent_child_prop_adult_carers) req = 1;
                     }
                 }
@@ -253,16 +270,16 @@ __device__ int do_case(const Dev& d, int r, int p, unsigned ts)
         }
     }
     if (d.bits[gp] & 2) {
-        cadd(&cn->cumDC, 1);
+        cadd_merged(&cn->cumDC, 1);
         // DoDetectedCase (src/Update.cpp:525-560): microcell trigger counts
         uint32_t* tg = d.m_trig + ((size_t)r * d.NM + d.mcell[p]) * 6;
         for (int q = 0; q < 5; q++) atomicAdd(&tg[q], 1u);
         if (!P->absenteeism_place_closure && P->place_close_round_household) atomicAdd(&tg[5], 1u);
     }
     if (treated(d, gp, ts)) atomicAdd(&d.cumTC[(size_t)r * d.NC + d.pc[p]], 1);
-    cadd(&cn->cumC, 1);
-    cadd(&cn->cumCa[ag], 1);
-    cadd(&cn->cumC_keyworker[keyw(d, p)], 1);
+    cadd_merged(&cn->cumC, 1);
+    cadd_merged(&cn->cumCa[ag], 1);
+    cadd_merged(&cn->cumC_keyworker[keyw_g(d, gp)], 1);
     if (P->do_severity) {
         if (d.sev_final[gp] == CS_SEV_MILD) {
             if (d.sev_cur[gp] == CS_SEV_ASYMPTOMATIC) { d.sev_cur[gp] = CS_SEV_MILD; sev_to(cn, CS_SK_MILD, ag); }
@@ -295,7 +312,7 @@ __global__ void __launch_bounds__(256) k_infectious_scan(Dev d)
         if (d.gate[sl] > ts) continue;  // every timer of the slot's person lies after this step
         const int p = d.perm[sl];
         const size_t gp = rN + p;
-        int ag = age_group(d, p);
+        int ag = age_group_g(d, gp);
         if (ag >= CS_NUM_AGE_GROUPS) ag = CS_NUM_AGE_GROUPS - 1;
         uint16_t ct = (uint16_t)(d.t_lat[gp] + ((int)(P->latent_to_sympt_delay / P->model_time_step)));
         if (P->do_symptoms && ts == ct) {
@@ -341,7 +358,7 @@ __global__ void __launch_bounds__(256) k_infectious_scan(Dev d)
                 if (recover) d.inf[gp] = (st == CS_CASE) ? CS_RECOVERED_SYMP : CS_RECOVERED_ASYMP;  // DoRecover
                 else {  // DoDeath
                     d.inf[gp] = (st == CS_CASE) ? CS_DEAD_SYMP : CS_DEAD_ASYMP;
-                    cadd(&cn->cumDa[ag], 1);
+                    cadd_merged(&cn->cumDa[ag], 1);
                 }
                 batch_add(d, r, c, p);
                 nt++;
@@ -384,7 +401,7 @@ __global__ void k_care_eval(Dev d)
         if (!f) {
             for (int j3 = j1; (j3 < j2) && !f; j3++)
                 if (((int)d.age[j3] >= P->case_absent_child_age_cutoff) && !is_dead_code(d.inf[rN + j3])) { kk = j3; f = true; }
-            if (f) cadd(&d.cnt[r].cumAA, 1);
+            if (f) cadd_merged(&d.cnt[r].cumAA, 1);
         }
         d.care_k[a] = kk;
         d.care_abs[a] = kk >= 0 ? (uint8_t)host_absent(d, rN + kk, ts) : 0;

@@ -291,6 +291,7 @@ __device__ __forceinline__ bool esoc(const Dev& d, int p) { return (d.pflags[p] 
 __device__ __forceinline__ int age_group_g(const Dev& d, size_t gp) { return (int)d.hage[gp] / CS_AGE_GROUP_WIDTH; }
 __device__ __forceinline__ bool care_g(const Dev& d, size_t gp) { return (d.hpfl[gp] & 1) != 0; }
 __device__ __forceinline__ bool esoc_g(const Dev& d, size_t gp) { return (d.hpfl[gp] & 4) != 0; }
+__device__ __forceinline__ int keyw_g(const Dev& d, size_t gp) { return (d.hpfl[gp] >> 1) & 1; }
 __device__ __forceinline__ bool is_dead_code(int8_t s) { return s == CS_DEAD_ASYMP || s == CS_DEAD_SYMP; }
 __device__ __forceinline__ bool host_absent(const Dev& d, size_t gp, unsigned ts) { return d.t_as[gp] <= ts && d.t_ae[gp] > ts; }
 __device__ __forceinline__ bool place_closed(const Dev& d, int r, int t, int k, unsigned ts)

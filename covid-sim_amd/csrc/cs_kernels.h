@@ -52,12 +52,51 @@ __device__ __forceinline__ bool halted(const Dev& d) { return (*(volatile unsign
 
 __device__ __forceinline__ void cadd(unsigned long long* a, long long v) { atomicAdd(a, (unsigned long long)v); }
 
+// cadd from every active lane, merged: the lanes that add to one address
+// make one atomic between them (a wave's lanes share their realisation's
+// counters, so most of a wave's adds land on a few addresses; a wave of
+// events mostly holds one realisation's).  v is the same in every lane;
+// integer sums, so every total equals that of one add per lane.  Every
+// active lane stays in the loop until the wave-uniform set of lanes still to
+// add is empty: each pass takes the lowest such lane's address (a shuffle
+// from that lane) and removes every lane with the same address, the lead
+// included.
+__device__ __forceinline__ void cadd_merged(unsigned long long* a, long long v)
+{
+    const uint64_t me = (uint64_t)(uintptr_t)a;
+    const unsigned lane = __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+    for (unsigned long long todo = __ballot(1); todo;) {
+        const int lead = __builtin_ctzll(todo);
+        const uint64_t la = ((uint64_t)(uint32_t)__shfl((int)(uint32_t)(me >> 32), lead, 64) << 32) | (uint32_t)__shfl((int)(uint32_t)me, lead, 64);
+        const unsigned long long same = __ballot(me == la);
+        if (lane == (unsigned)lead) atomicAdd(a, (unsigned long long)(v * (long long)__popcll(same)));
+        todo &= ~same;
+    }
+}
+
+// lists p for its cell's batched move: the lanes of one cell take
+// consecutive batch slots with one atomic between them (the same merge as
+// cadd_merged; a batch's order does not matter, the move sorts it)
 __device__ __forceinline__ void batch_add(const Dev& d, int r, int c, int p)
 {
     const int rc = r * d.NC + c;
-    int idx = atomicAdd(&d.bcount[rc], 1);
-    if (idx == 0) d.moved[atomicAdd(d.wl + WL_MOVED, 1u)] = rc;  // first move of the cell this phase
-    d.batch[(size_t)r * d.N + d.cstart[c] + idx] = p;
+    const unsigned lane = __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+    for (unsigned long long todo = __ballot(1); todo;) {
+        const int lead = __builtin_ctzll(todo);
+        const int lrc = __shfl(rc, lead, 64);
+        const unsigned long long same = __ballot(rc == lrc);
+        int base = 0;
+        if (lane == (unsigned)lead) {
+            base = atomicAdd(&d.bcount[rc], __popcll(same));
+            if (base == 0) d.moved[atomicAdd(d.wl + WL_MOVED, 1u)] = rc;  // first move of the cell this phase
+        }
+        base = __shfl(base, lead, 64);
+        if (rc == lrc) {
+            const unsigned below = __builtin_amdgcn_mbcnt_hi((uint32_t)(same >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)same, 0u));
+            d.batch[(size_t)r * d.N + d.cstart[c] + base + (int)below] = p;
+        }
+        todo &= ~same;
+    }
 }
 
 // A work-list kernel gives each wave whole chunks, so the chunk and every
@@ -450,9 +489,9 @@ __device__ bool do_infect_state(const Dev& d, int r, int p, double t, unsigned t
 // the indexed counters of a tally (per lane)
 __device__ __forceinline__ void tally_indexed(Counters* cn, const InfectTally& tl)
 {
-    cadd(&cn->cumItype[tl.kind], 1);
-    cadd(&cn->cumIa[tl.ag], 1);
-    cadd(&cn->cumI_keyworker[tl.kw], 1);
+    cadd_merged(&cn->cumItype[tl.kind], 1);
+    cadd_merged(&cn->cumIa[tl.ag], 1);
+    cadd_merged(&cn->cumI_keyworker[tl.kw], 1);
 }
 
 __device__ __forceinline__ void tally_scalar(Counters* cn, unsigned long long n, double rad2, unsigned long long rad2max_bits,

@@ -6,26 +6,6 @@
 
 namespace cs {
 
-// cadd from every active lane, merged: the lanes that add to one address
-// make one atomic between them (a wave's lanes share their realisation's
-// counters, so most of a wave's adds land on a few addresses).  v is the
-// same in every lane; integer sums, so every total equals that of one add
-// per lane.  Every active lane stays in the loop until the wave-uniform set
-// of lanes still to add is empty: each pass takes the lowest such lane's
-// address (a shuffle from that lane) and removes every lane with the same
-// address, the lead included.
-__device__ __forceinline__ void cadd_merged(unsigned long long* a, long long v)
-{
-    const uint64_t me = (uint64_t)(uintptr_t)a;
-    const unsigned lane = __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
-    for (unsigned long long todo = __ballot(1); todo;) {
-        const int lead = __builtin_ctzll(todo);
-        const uint64_t la = ((uint64_t)(uint32_t)__shfl((int)(uint32_t)(me >> 32), lead, 64) << 32) | (uint32_t)__shfl((int)(uint32_t)me, lead, 64);
-        const unsigned long long same = __ballot(me == la);
-        if (lane == (unsigned)lead) atomicAdd(a, (unsigned long long)(v * (long long)__popcll(same)));
-        todo &= ~same;
-    }
-}
 __device__ __forceinline__ void sev_to(Counters* c, int kind, int ag)
 {
     cadd_merged(&c->sev_prev[kind], 1); cadd_merged(&c->sev_prev_age[kind][ag], 1);

@@ -54,13 +54,13 @@ __device__ __forceinline__ void cadd(unsigned long long* a, long long v) { atomi
 
 // cadd from every active lane, merged: the lanes that add to one address
 // make one atomic between them (a wave's lanes share their realisation's
-// counters, so most of a wave's adds land on a few addresses; a wave of
-// events mostly holds one realisation's).  v is the same in every lane;
-// integer sums, so every total equals that of one add per lane.  Every
-// active lane stays in the loop until the wave-uniform set of lanes still to
-// add is empty: each pass takes the lowest such lane's address (a shuffle
-// from that lane) and removes every lane with the same address, the lead
-// included.
+// counters, so most of a wave's adds land on a few addresses).  v is the
+// same in every lane; integer sums, so every total equals that of one add
+// per lane.  Every active lane stays in the loop until the wave-uniform set
+// of lanes still to add is empty: each pass takes the lowest such lane's
+// address (a shuffle from that lane) and removes every lane with the same
+// address, the lead included.  The adds do not return, so passes for many
+// distinct addresses cost no memory round trips.
 __device__ __forceinline__ void cadd_merged(unsigned long long* a, long long v)
 {
     const uint64_t me = (uint64_t)(uintptr_t)a;
@@ -74,29 +74,16 @@ __device__ __forceinline__ void cadd_merged(unsigned long long* a, long long v)
     }
 }
 
-// lists p for its cell's batched move: the lanes of one cell take
-// consecutive batch slots with one atomic between them (the same merge as
-// cadd_merged; a batch's order does not matter, the move sorts it)
+// lists p for its cell's batched move.  One returning atomic per lane: a
+// per-wave merge of lanes of many cells (events, due lists) serialises the
+// atomics' latency, and where a wave holds one cell the compiler already
+// merges the lanes' adds into one.
 __device__ __forceinline__ void batch_add(const Dev& d, int r, int c, int p)
 {
     const int rc = r * d.NC + c;
-    const unsigned lane = __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
-    for (unsigned long long todo = __ballot(1); todo;) {
-        const int lead = __builtin_ctzll(todo);
-        const int lrc = __shfl(rc, lead, 64);
-        const unsigned long long same = __ballot(rc == lrc);
-        int base = 0;
-        if (lane == (unsigned)lead) {
-            base = atomicAdd(&d.bcount[rc], __popcll(same));
-            if (base == 0) d.moved[atomicAdd(d.wl + WL_MOVED, 1u)] = rc;  // first move of the cell this phase
-        }
-        base = __shfl(base, lead, 64);
-        if (rc == lrc) {
-            const unsigned below = __builtin_amdgcn_mbcnt_hi((uint32_t)(same >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)same, 0u));
-            d.batch[(size_t)r * d.N + d.cstart[c] + base + (int)below] = p;
-        }
-        todo &= ~same;
-    }
+    int idx = atomicAdd(&d.bcount[rc], 1);
+    if (idx == 0) d.moved[atomicAdd(d.wl + WL_MOVED, 1u)] = rc;  // first move of the cell this phase
+    d.batch[(size_t)r * d.N + d.cstart[c] + idx] = p;
 }
 
 // A work-list kernel gives each wave whole chunks, so the chunk and every
@@ -489,9 +476,9 @@ __device__ bool do_infect_state(const Dev& d, int r, int p, double t, unsigned t
 // the indexed counters of a tally (per lane)
 __device__ __forceinline__ void tally_indexed(Counters* cn, const InfectTally& tl)
 {
-    cadd_merged(&cn->cumItype[tl.kind], 1);
-    cadd_merged(&cn->cumIa[tl.ag], 1);
-    cadd_merged(&cn->cumI_keyworker[tl.kw], 1);
+    cadd(&cn->cumItype[tl.kind], 1);
+    cadd(&cn->cumIa[tl.ag], 1);
+    cadd(&cn->cumI_keyworker[tl.kw], 1);
 }
 
 __device__ __forceinline__ void tally_scalar(Counters* cn, unsigned long long n, double rad2, unsigned long long rad2max_bits,
@@ -876,8 +863,8 @@ __global__ void __launch_bounds__(256) k_spatial_draws(Dev d)
         int next = e.first + 64;     // the chunk's next unassigned draw (wave-uniform)
         bool fresh = true;
         Stream s(seed, 0, 0, 0);
-        int i = 0, hh_i = 0, att = 0;
-        bool closed = false;
+        int i = 0, hh_i = 0, att = 0, ag_i = 0;
+        bool closed = false, care_i = false;
         float2 loc_i = make_float2(0.f, 0.f);
         for (;;) {
             const bool active = k < kend;
@@ -892,6 +879,8 @@ __global__ void __launch_bounds__(256) k_spatial_draws(Dev d)
                     const PStat pi = d.pst[i];
                     loc_i = pi.loc;
                     hh_i = pi.hh;
+                    ag_i = (int)pi.age / CS_AGE_GROUP_WIDTH;  // the infector's, kept for every attempt
+                    care_i = (pi.pflags & 1) != 0;
                     att = 0;
                     fresh = false;
                 }
@@ -914,11 +903,12 @@ __global__ void __launch_bounds__(256) k_spatial_draws(Dev d)
                         done = true;
                         if ((c != l) || (pm.hh != hh_i)) {
                             double SS = spatial_susc(d, r, m, ts);
-                            if (care(d, m) || care(d, i)) SS *= P->care_home_resident_spatial_scaling;
+                            if ((pm.pflags & 1) || care_i) SS *= P->care_home_resident_spatial_scaling;
                             if (slot < d.cS[rl]) {
                                 w_susc++;
-                                SS *= person_susc(d, r, m, ts, i);
-                                SS *= P->waifw_spatial[age_group(d, m)][age_group(d, i)];
+                                const int ag_m = (int)pm.age / CS_AGE_GROUP_WIDTH;
+                                SS *= person_susc_ag(d, rN + m, ag_m, ag_i, ts);
+                                SS *= P->waifw_spatial[ag_m][ag_i];
                                 if (!closed && host_absent(d, rN + m, ts)) {
                                     bool f2 = false;
                                     for (int pt = 0; pt < d.NPT && !f2; pt++) {

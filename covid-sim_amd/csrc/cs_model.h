@@ -503,14 +503,20 @@ __device__ __forceinline__ double spatial_inf(const Dev& d, int r, int p, unsign
          * person_inf(d, gp, ts);
 }
 
-__device__ __forceinline__ double person_susc(const Dev& d, int r, int p, unsigned ts, int infector)
+// the same with the two age groups given (a caller that already holds them)
+__device__ __forceinline__ double person_susc_ag(const Dev& d, size_t gp, int ag, int ag_infector, unsigned ts)
 {
     const cs_params* P = d.P;
-    size_t gp = (size_t)r * d.N + p;
-    return P->waifw[age_group_g(d, gp)][age_group_g(d, (size_t)r * d.N + infector)]
-         * P->age_susceptibility[age_group_g(d, gp)] * (double)d.susc[gp]
+    return P->waifw[ag][ag_infector]
+         * P->age_susceptibility[ag] * (double)d.susc[gp]
          * (treated(d, gp, ts) ? P->treat_susc_drop : 1.0)
          * (vacced(d, gp, ts) ? (vacced_switch(d, gp) ? P->vacc_susc_drop2 : P->vacc_susc_drop) : 1.0);
+}
+
+__device__ __forceinline__ double person_susc(const Dev& d, int r, int p, unsigned ts, int infector)
+{
+    const size_t gp = (size_t)r * d.N + p;
+    return person_susc_ag(d, gp, age_group_g(d, gp), age_group_g(d, (size_t)r * d.N + infector), ts);
 }
 
 __device__ __forceinline__ double house_susc(const Dev& d, int r, int p, unsigned ts, int infector)

@@ -322,7 +322,11 @@ void cs_destroy(cs_engine* e);
 /* set-up: parameters, kernel tables (KernelLookup lookup_/hi_res_, each
  * kernel_size+1 doubles; NULL builds them here from the kernel fields of
  * cs_params, src/Kernels.cpp:17-45), population (copied to HBM).  The cell
- * max_trans table (src/Kernels.cpp:48-63) is built on the device. */
+ * max_trans table (src/Kernels.cpp:48-63) is built on the device.
+ * cs_set_population checks every index the kernels follow and returns
+ * CS_EINVAL naming the first bad one; households hold at most
+ * CS_MAX_HOUSEHOLD_SIZE persons (hh_nh and hh_nhr), as the reference's
+ * household table does (src/SetupModel.cpp:1287-1288). */
 int  cs_set_params(cs_engine* e, const cs_params* p, const double* kernel_lookup, const double* kernel_hi_res);
 int  cs_set_population(cs_engine* e, const cs_population* pop);
 

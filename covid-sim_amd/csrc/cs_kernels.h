@@ -362,15 +362,19 @@ __device__ void list_move_cell(const Dev& d, int mode, int rc, int32_t* pos, int
     const int a0 = min(k, (int)threadIdx.x * per), a1 = min(k, a0 + per);
     int cnt = 0;
     for (int i = a0; i < a1; i++) cnt += 1 - occ[i];
-    part[threadIdx.x] = cnt;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        int run = 0;
-        for (int t = 0; t < (int)blockDim.x; t++) { int v = part[t]; part[t] = run; run += v; }
-    }
-    __syncthreads();
+    // exclusive scan of the threads' counts: within each wave by shuffles,
+    // then the totals of the waves before (the same prefixes as a serial pass)
     {
-        int run = part[threadIdx.x];
+        const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+        int incl = cnt;
+        for (int o = 1; o < 64; o <<= 1) {
+            const int y = __shfl_up(incl, o, 64);
+            if (lane >= o) incl += y;
+        }
+        if (lane == 63) part[w] = incl;
+        __syncthreads();
+        int run = incl - cnt;
+        for (int q = 0; q < w; q++) run += part[q];
         for (int i = a0; i < a1; i++)
             if (!occ[i]) fpos[run++] = bnd + i;
     }
@@ -411,7 +415,7 @@ __global__ void __launch_bounds__(256) k_list_move(Dev d, int mode)
     __shared__ int32_t pos[CS_MOVE_CAP];
     __shared__ int32_t fpos[CS_MOVE_CAP];
     __shared__ uint8_t occ[CS_MOVE_CAP];
-    __shared__ int32_t part[256];
+    __shared__ int32_t part[256 / 64];  // per-wave totals of the free-slot scan (blocks of 256)
     __shared__ int dead_count, bad;
     const unsigned nmoved = d.wl[WL_MOVED];
     for (unsigned a = blockIdx.x; a < nmoved; a += gridDim.x) {

@@ -790,15 +790,20 @@ __global__ void __launch_bounds__(CS_SUM_BLOCK) k_cell_spatial_sum(Dev d)
     const StepScalars sc = *d.sc;
     const unsigned ncells = d.wl[WL_CELLS];
     const int lane = threadIdx.x & 63;
+    // the infectious persons seen, summed per wave, then per block: one
+    // global add per block (an add per cell to the one work counter
+    // serialises at the memory side)
+    __shared__ unsigned long long blk_inf;
+    if (threadIdx.x == 0) blk_inf = 0;
+    __syncthreads();
+    unsigned long long w_inf = 0;
     for (unsigned a = grid_wave(); a < ncells; a += grid_nwaves()) {
         const int rc = uni(d.cells[a]);
         const int I = uni(d.cI[rc]);
         const int r = rc / d.NC, c = rc % d.NC;
         const double beta_s = beta_s_of(d, sc, r);
-        if (!(beta_s > 0)) {
-            if (lane == 0) atomicAdd(&d.wc->n_inf, (unsigned long long)I);
-            continue;
-        }
+        w_inf += (unsigned long long)I;
+        if (!(beta_s > 0)) continue;
         const size_t sbase = (size_t)r * d.N + uni(d.cstart[c] + d.cS[rc] + d.cL[rc]);
         const double* si = d.si_buf + sbase;
         Fx acc = {0, 0};
@@ -812,7 +817,6 @@ __global__ void __launch_bounds__(CS_SUM_BLOCK) k_cell_spatial_sum(Dev d)
                 K = poisson(s, sis * beta_s * ((double)d.tot_prob[rc]));
             }
             d.spK[rc] = K;
-            atomicAdd(&d.wc->n_inf, (unsigned long long)I);
         }
         K = __shfl(K, 0, 64);
         const float fs = (float)sis;
@@ -838,6 +842,9 @@ __global__ void __launch_bounds__(CS_SUM_BLOCK) k_cell_spatial_sum(Dev d)
             carry = fx_add(carry, Fx{__shfl(x.hi, 63, 64), __shfl(x.lo, 63, 64)});
         }
     }
+    if (lane == 0 && w_inf) atomicAdd(&blk_inf, w_inf);
+    __syncthreads();
+    if (threadIdx.x == 0 && blk_inf) atomicAdd(&d.wc->n_inf, blk_inf);
 }
 
 // one spatial draw per lane; a wave takes a chunk of up to 64 draws of one

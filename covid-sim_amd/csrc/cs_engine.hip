@@ -39,7 +39,7 @@ struct cs_engine {
     bool ts_fresh = true;            // no sample since the last alloc/reset/load: the series restarts
     unsigned grid_waves = 4096;      // blocks of 4 waves for the per-person passes (a wave per chunk)
     unsigned grid_cells = 2048;      // blocks for the per-cell spatial sums (4 waves, a wave per cell)
-    unsigned grid_moves = 4096;      // blocks for the batched list moves (a block per moved cell; not tuned)
+    unsigned grid_moves = 4096;      // blocks for the batched list moves (each a range of cells; not tuned)
     // replayed step graphs (cs_run_steps): one graph per output step
     bool use_graphs = true;
     hipGraphExec_t gexec = nullptr;
@@ -710,7 +710,7 @@ int cs_alloc_realisations(cs_engine* e, int n_real, const uint64_t* seeds)
     AL(si_buf, RN); AL(ci_buf, RN); AL(spK, RC);
     d.seg_cap = (unsigned)std::min<size_t>(RN / 64 + RC + 1, 0xffffffffu);
     d.draw_cap = d.seg_cap;
-    AL(segl, d.seg_cap); AL(drawl, d.draw_cap); AL(cells, RC); AL(moved, RC); AL(wl, cs::WL_N);
+    AL(segl, d.seg_cap); AL(drawl, d.draw_cap); AL(cells, RC); AL(wl, cs::WL_N);
     size_t cap = std::min<size_t>(std::max<size_t>(RN / 16, (size_t)1 << 20), (size_t)1 << 28);
     d.ev_cap = (unsigned)cap;
     AL(ev_key, cap); AL(ev_gp, cap); AL(ev_infector, cap); AL(ev_type, cap); AL(ev_count, 1);

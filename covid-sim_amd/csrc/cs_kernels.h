@@ -77,12 +77,12 @@ __device__ __forceinline__ void cadd_merged(unsigned long long* a, long long v)
 // lists p for its cell's batched move.  One returning atomic per lane: a
 // per-wave merge of lanes of many cells (events, due lists) serialises the
 // atomics' latency, and where a wave holds one cell the compiler already
-// merges the lanes' adds into one.
+// merges the lanes' adds into one.  The move kernel finds the cells with a
+// pending move from the counts themselves.
 __device__ __forceinline__ void batch_add(const Dev& d, int r, int c, int p)
 {
     const int rc = r * d.NC + c;
-    int idx = atomicAdd(&d.bcount[rc], 1);
-    if (idx == 0) d.moved[atomicAdd(d.wl + WL_MOVED, 1u)] = rc;  // first move of the cell this phase
+    const int idx = atomicAdd(&d.bcount[rc], 1);
     d.batch[(size_t)r * d.N + d.cstart[c] + idx] = p;
 }
 
@@ -417,10 +417,22 @@ __global__ void __launch_bounds__(256) k_list_move(Dev d, int mode)
     __shared__ uint8_t occ[CS_MOVE_CAP];
     __shared__ int32_t part[256 / 64];  // per-wave totals of the free-slot scan (blocks of 256)
     __shared__ int dead_count, bad;
-    const unsigned nmoved = d.wl[WL_MOVED];
-    for (unsigned a = blockIdx.x; a < nmoved; a += gridDim.x) {
-        __syncthreads();  // the previous cell's use of the shared arrays is over
-        list_move_cell(d, mode, d.moved[a], pos, fpos, occ, part, &dead_count, &bad);
+    // each block owns a contiguous range of (realisation, cell) pairs and
+    // reads their pending-move counts 64 at a time (every wave the same 64,
+    // so the ballot is the same in all of them); a cell's own move clears its
+    // count only after every wave has read it (the move begins with a barrier)
+    const unsigned rcn = (unsigned)d.R * (unsigned)d.NC;
+    const unsigned per = (rcn + gridDim.x - 1) / gridDim.x;
+    const unsigned c0 = min(rcn, blockIdx.x * per), c1 = min(rcn, c0 + per);
+    for (unsigned b = c0; b < c1; b += 64) {
+        const unsigned rc = b + (threadIdx.x & 63);
+        unsigned long long m = __ballot(rc < c1 && d.bcount[rc] != 0);
+        while (m) {
+            const int j = __builtin_ctzll(m);
+            m &= m - 1;
+            __syncthreads();  // the previous cell's use of the shared arrays is over
+            list_move_cell(d, mode, (int)(b + j), pos, fpos, occ, part, &dead_count, &bad);
+        }
     }
 }
 

@@ -52,7 +52,7 @@ struct Clock {  // the device's own step counter for replayed step graphs
 struct Chunk {  // up to 64 consecutive entries starting at `first` of cell rc's segment
     int32_t rc, first;
 };
-enum { WL_SEG = 0, WL_CELLS, WL_DRAWS, WL_MOVED, WL_N };
+enum { WL_SEG = 0, WL_CELLS, WL_DRAWS, WL_N };
 
 struct Work {  // event tallies for the algorithmic-bytes figure
     unsigned long long n_inf, n_hh_inf, n_hh_member, n_place_link, n_place_cand;
@@ -236,8 +236,7 @@ struct Dev {
     Chunk* segl;          // persons of a latent or infectious segment
     Chunk* drawl;         // spatial draws of a source cell
     int32_t* cells;       // cells with infectious persons
-    int32_t* moved;       // cells with a pending batched list move
-    unsigned* wl;         // counts: [0] segl, [1] cells, [2] drawl, [3] moved
+    unsigned* wl;         // counts: [0] segl, [1] cells, [2] drawl
     unsigned seg_cap, draw_cap;
     uint64_t* ev_key;
     long long* ev_gp;

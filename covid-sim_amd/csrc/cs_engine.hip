@@ -719,7 +719,7 @@ int cs_alloc_realisations(cs_engine* e, int n_real, const uint64_t* seeds)
     AL(care_gp, RN); AL(care_k, RN); AL(care_abs, RN); AL(care_count, 1);
     AL(last_tot, CS_TS_LAST * R);
     d.due_cap = (unsigned)std::min<size_t>(std::max<size_t>(RN / 64, 4096), 0x7fffffff);
-    AL(due, (size_t)CS_DUE_RING * d.due_cap); AL(due_n, CS_DUE_RING);
+    AL(due, (size_t)CS_DUE_RING * d.due_cap); AL(due_n, CS_DUE_RING * CS_DUE_STRIDE);
     d.due_on = 0;
 #undef AL
     if ((rc = dalloc(e, A, &e->d_sel, R))) return rc;
@@ -770,7 +770,7 @@ int cs_reset_realisations(cs_engine* e, int n_seed)
     hipStream_t st = e->stream;
     e->ts_fresh = true;
     HIPCHK(e, hipMemsetAsync(d.cnt, 0, (size_t)d.R * sizeof(cs::Counters), st));
-    HIPCHK(e, hipMemsetAsync(d.due_n, 0, CS_DUE_RING * sizeof(unsigned), st));
+    HIPCHK(e, hipMemsetAsync(d.due_n, 0, CS_DUE_RING * CS_DUE_STRIDE * sizeof(unsigned), st));
     if (d.due_on != (e->due_ok ? 1 : 0)) {  // fresh state: every latent person is listed from here on
         HIPCHK(e, hipStreamSynchronize(st));
         drop_graph(e);
@@ -1071,7 +1071,7 @@ int cs_set_due_capacity(cs_engine* e, uint32_t per_step)
     if (rc) return rc;
     d.due_cap = per_step;
     // the lists held so far are gone: scan until the next reset lists afresh
-    HIPCHK(e, hipMemset(d.due_n, 0, CS_DUE_RING * sizeof(unsigned)));
+    HIPCHK(e, hipMemset(d.due_n, 0, CS_DUE_RING * CS_DUE_STRIDE * sizeof(unsigned)));
     d.due_on = 0;
     return 0;
 }

@@ -107,7 +107,7 @@ __device__ __forceinline__ Chunk chunk_at(const Chunk* l, unsigned ch)
 // this step's latent persons are all on its due list (not overflowed)
 __device__ __forceinline__ bool due_listed(const Dev& d)
 {
-    return d.due_on && d.due_n[d.sc->ts & (CS_DUE_RING - 1u)] <= d.due_cap;
+    return d.due_on && d.due_n[(d.sc->ts & (CS_DUE_RING - 1u)) * CS_DUE_STRIDE] <= d.due_cap;
 }
 
 __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v)
@@ -514,7 +514,7 @@ __device__ __forceinline__ void tally_scalar(Counters* cn, unsigned long long n,
 __device__ __forceinline__ void due_add(const Dev& d, unsigned lat, long long gp)
 {
     const unsigned b = lat & (CS_DUE_RING - 1u);
-    const unsigned idx = atomicAdd(&d.due_n[b], 1u);
+    const unsigned idx = atomicAdd(&d.due_n[b * CS_DUE_STRIDE], 1u);
     if (idx < d.due_cap) d.due[(size_t)b * d.due_cap + idx] = gp;
 }
 

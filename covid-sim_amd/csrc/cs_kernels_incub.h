@@ -170,7 +170,7 @@ __global__ void __launch_bounds__(256) k_latent_due(Dev d)
     if (halted(d) || !due_listed(d)) return;
     const unsigned ts = d.sc->ts;
     const unsigned b = ts & (CS_DUE_RING - 1u);
-    const unsigned n = d.due_n[b];
+    const unsigned n = d.due_n[b * CS_DUE_STRIDE];
     const long long* list = d.due + (size_t)b * d.due_cap;
     unsigned long long nt = 0;
     for (unsigned k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
@@ -189,7 +189,7 @@ __global__ void __launch_bounds__(256) k_latent_due(Dev d)
 // this step's due list is consumed: clear it for the step CS_DUE_RING later
 __global__ void k_due_clear(Dev d)
 {
-    if (threadIdx.x == 0 && !halted(d) && d.due_on) d.due_n[d.sc->ts & (CS_DUE_RING - 1u)] = 0u;
+    if (threadIdx.x == 0 && !halted(d) && d.due_on) d.due_n[(d.sc->ts & (CS_DUE_RING - 1u)) * CS_DUE_STRIDE] = 0u;
 }
 
 __global__ void __launch_bounds__(256) k_latent_scan(Dev d)

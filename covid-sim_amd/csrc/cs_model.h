@@ -81,6 +81,10 @@ __host__ __device__ __forceinline__ bool no_place_links(const int32_t* lk, int n
 // time is ts, appended when they are infected (every latent time lies less
 // than CS_DUE_RING steps ahead, checked from the latent-period table)
 #define CS_DUE_RING 256
+// a bucket's count sits alone on a 128-byte line (CS_DUE_STRIDE words apart):
+// a step's infections append to the few buckets of their latent times, and
+// adds from the whole GPU to one line serialise at the memory side
+#define CS_DUE_STRIDE 32
 
 struct Hot {  // per (realisation, person): 16 bytes, one vector load
     int8_t inf;       // InfStat code
@@ -275,7 +279,7 @@ struct Dev {
     int ts_max;
     long long* last_tot;  // [R][CS_TS_LAST] cumI, cumC, D, severity incidence at the previous sample
     long long* due;       // [CS_DUE_RING][due_cap] persons (gp) by latent time
-    unsigned* due_n;      // [CS_DUE_RING] appended; above due_cap: that step scans every latent person
+    unsigned* due_n;      // [CS_DUE_RING * CS_DUE_STRIDE], bucket b at b * CS_DUE_STRIDE: appended; above due_cap: that step scans every latent person
     unsigned due_cap;
     int due_on;           // 0: every step scans every latent person
 };

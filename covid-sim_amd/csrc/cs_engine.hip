@@ -203,7 +203,7 @@ int launch_infect_events(cs_engine* e)
     const unsigned nrc = (unsigned)((long long)d.R * d.NC);
     {
         ProfScope ps(e, CS_K_OTHER);
-        cs::k_zero_counts<<<1, 64, 0, st>>>(d.ev_count, 1);
+        cs::k_zero_counts<<<1, CS_EV_SHARDS * CS_EV_STRIDE, 0, st>>>(d.ev_count, CS_EV_SHARDS * CS_EV_STRIDE);
         cs::k_zero_counts<<<1, 64, 0, st>>>(d.wl, cs::WL_N);
         cs::k_compact<<<grid_for(nrc, 256), 256, 0, st>>>(d, 1, 1);
     }
@@ -291,7 +291,7 @@ void drop_graph(cs_engine* e)
 int grow_events(cs_engine* e)
 {
     Dev& d = e->d;
-    unsigned cap = d.ev_cap * 2u;
+    unsigned cap = d.ev_cap * 2u;  // stays a multiple of CS_EV_SHARDS
     if (cap < d.ev_cap) return set_err(e, CS_EOVERFLOW, "event queue cannot grow further");
     HIPCHK(e, hipStreamSynchronize(e->stream));
     drop_graph(e);  // the graph holds the old buffers
@@ -712,8 +712,9 @@ int cs_alloc_realisations(cs_engine* e, int n_real, const uint64_t* seeds)
     d.draw_cap = d.seg_cap;
     AL(segl, d.seg_cap); AL(drawl, d.draw_cap); AL(cells, RC); AL(wl, cs::WL_N);
     size_t cap = std::min<size_t>(std::max<size_t>(RN / 16, (size_t)1 << 20), (size_t)1 << 28);
+    cap -= cap % CS_EV_SHARDS;
     d.ev_cap = (unsigned)cap;
-    AL(ev_key, cap); AL(ev_gp, cap); AL(ev_infector, cap); AL(ev_type, cap); AL(ev_count, 1);
+    AL(ev_key, cap); AL(ev_gp, cap); AL(ev_infector, cap); AL(ev_type, cap); AL(ev_count, CS_EV_SHARDS * CS_EV_STRIDE);
     AL(clr, RN); AL(batch, RN); AL(bcount, RC);
     d.bind_clr(d.clr);
     AL(care_gp, RN); AL(care_k, RN); AL(care_abs, RN); AL(care_count, 1);
@@ -1040,6 +1041,8 @@ int cs_set_queue_capacity(cs_engine* e, uint32_t events, uint32_t draw_chunks)
     auto drop = [&](void* p) { e->real_allocs.erase(std::remove(e->real_allocs.begin(), e->real_allocs.end(), p), e->real_allocs.end()); };
     int rc;
     if (events) {
+        if (events > 0xffffffffu - CS_EV_SHARDS) return set_err(e, CS_EINVAL, "event capacity %u too large", events);
+        events += (CS_EV_SHARDS - events % CS_EV_SHARDS) % CS_EV_SHARDS;  // whole shards
         (void)hipFree(d.ev_key); (void)hipFree(d.ev_gp); (void)hipFree(d.ev_infector); (void)hipFree(d.ev_type);
         drop(d.ev_key); drop(d.ev_gp); drop(d.ev_infector); drop(d.ev_type);
         if ((rc = dalloc(e, e->real_allocs, &d.ev_key, events)) || (rc = dalloc(e, e->real_allocs, &d.ev_gp, events)) ||
@@ -1157,8 +1160,9 @@ int cs_autotune(cs_engine* e, double t, int reps, cs_tuning* out)
     float best = 1e30f;
     for (int i = 0; i < 4; i++) best = std::min(best, ms_c[i]);
     // the queue holds 4 times the events of this state (it still grows on demand)
-    unsigned nev = 0;
-    HIPCHK(e, hipMemcpy(&nev, e->d.ev_count, sizeof(nev), hipMemcpyDeviceToHost));
+    unsigned cnt[CS_EV_SHARDS * CS_EV_STRIDE], nev = 0;
+    HIPCHK(e, hipMemcpy(cnt, e->d.ev_count, sizeof(cnt), hipMemcpyDeviceToHost));
+    for (int s = 0; s < CS_EV_SHARDS; s++) nev += cnt[s * CS_EV_STRIDE];
     const uint64_t want = std::min<uint64_t>((uint64_t)nev * 4u, 0xffffffffull);
     if (want > e->d.ev_cap && (rc = cs_set_queue_capacity(e, (uint32_t)want, 0))) { e->prof = prof; return rc; }
     e->ev_pool.push_back(a);

@@ -34,12 +34,14 @@ __device__ __forceinline__ uint64_t ev_key(int infector, int pass, uint32_t sub)
 
 __device__ __forceinline__ void add_event(const Dev& d, uint64_t key, size_t gp_infectee, int infector, int type)
 {
-    unsigned idx = atomicAdd(d.ev_count, 1u);
-    if (idx < d.ev_cap) {
-        d.ev_key[idx] = key;
-        d.ev_gp[idx] = (long long)gp_infectee;
-        d.ev_infector[idx] = infector;
-        d.ev_type[idx] = (int16_t)type;
+    const unsigned sh = blockIdx.x & (CS_EV_SHARDS - 1u), scap = d.ev_cap / CS_EV_SHARDS;
+    const unsigned idx = atomicAdd(&d.ev_count[sh * CS_EV_STRIDE], 1u);
+    if (idx < scap) {
+        const size_t e = (size_t)sh * scap + idx;
+        d.ev_key[e] = key;
+        d.ev_gp[e] = (long long)gp_infectee;
+        d.ev_infector[e] = infector;
+        d.ev_type[e] = (int16_t)type;
     } else {
         atomicOr(d.status, (unsigned)FLAG_OVERFLOW);
     }
@@ -971,11 +973,32 @@ __global__ void __launch_bounds__(256) k_spatial_draws(Dev d)
     }
 }
 
+// the queue's events as one sequence: pre[s] events lie in the shards
+// before s (the counts are capped at the shard size, as the appends are)
+struct EvSeq {
+    unsigned pre[CS_EV_SHARDS + 1], scap;
+    __device__ __forceinline__ explicit EvSeq(const Dev& d) : scap(d.ev_cap / CS_EV_SHARDS)
+    {
+        pre[0] = 0;
+        for (int s = 0; s < CS_EV_SHARDS; s++) pre[s + 1] = pre[s] + min(d.ev_count[s * CS_EV_STRIDE], scap);
+    }
+    __device__ __forceinline__ unsigned n() const { return pre[CS_EV_SHARDS]; }
+    // the queue slot of the k-th event (k < n())
+    __device__ __forceinline__ size_t slot(unsigned k) const
+    {
+        int s = 0;
+        while (k >= pre[s + 1]) s++;
+        return (size_t)s * scap + (k - pre[s]);
+    }
+};
+
 __global__ void k_resolve_claim(Dev d)
 {
     if (halted(d)) return;
-    unsigned n = min(*d.ev_count, d.ev_cap);
-    for (unsigned e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+    const EvSeq q(d);
+    const unsigned n = q.n();
+    for (unsigned k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        const size_t e = q.slot(k);
         size_t gp = (size_t)d.ev_gp[e];
         unsigned long long key = d.ev_key[e];
         atomicMin(&d.claim[gp], key);
@@ -987,17 +1010,19 @@ __global__ void k_resolve_apply(Dev d)
 {
     if (halted(d)) return;
     const StepScalars sc = *d.sc;
-    const unsigned n = min(*d.ev_count, d.ev_cap);
+    const EvSeq q(d);
+    const unsigned n = q.n();
     const int lane = threadIdx.x & 63;
     unsigned long long nn = 0;
     // whole waves step through the events, so every lane takes part in the
     // wave reductions of the counters below
     for (unsigned base = grid_wave() * 64u; base < n; base += grid_nwaves() * 64u) {
-        const unsigned e = base + lane;
+        const unsigned k = base + lane;
+        const size_t e = (k < n) ? q.slot(k) : 0;
         bool won = false;
         int r = 0;
         InfectTally tl;
-        if (e < n) {
+        if (k < n) {
             const size_t gp = (size_t)d.ev_gp[e];
             const unsigned long long key = d.ev_key[e];
             r = (int)(gp / d.N);
@@ -1037,9 +1062,10 @@ __global__ void k_resolve_apply(Dev d)
 __global__ void k_resolve_reset(Dev d)
 {
     if (halted(d)) return;
-    unsigned n = min(*d.ev_count, d.ev_cap);
-    for (unsigned e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
-        size_t gp = (size_t)d.ev_gp[e];
+    const EvSeq q(d);
+    const unsigned n = q.n();
+    for (unsigned k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        size_t gp = (size_t)d.ev_gp[q.slot(k)];
         d.claim[gp] = ~0ull;
         d.lastk[gp] = 0ull;
     }

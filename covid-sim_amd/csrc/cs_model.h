@@ -86,6 +86,13 @@ __host__ __device__ __forceinline__ bool no_place_links(const int32_t* lk, int n
 // adds from the whole GPU to one line serialise at the memory side
 #define CS_DUE_STRIDE 32
 
+// the event queue is CS_EV_SHARDS shards: a block appends to shard
+// blockIdx.x % CS_EV_SHARDS, whose count sits alone on a 128-byte line and
+// whose events fill its own 1/CS_EV_SHARDS of the queue (appends from the
+// whole GPU to one count serialise at the memory side)
+#define CS_EV_SHARDS 8
+#define CS_EV_STRIDE 32
+
 struct Hot {  // per (realisation, person): 16 bytes, one vector load
     int8_t inf;       // InfStat code
     uint8_t flags;    // HOT_HH_DONE, HOT_NO_LINKS, household size << HOT_NH_SHIFT
@@ -246,8 +253,8 @@ struct Dev {
     long long* ev_gp;
     int32_t* ev_infector;
     int16_t* ev_type;
-    unsigned int* ev_count;
-    unsigned int ev_cap;
+    unsigned int* ev_count;  // [CS_EV_SHARDS * CS_EV_STRIDE], shard s's count at s * CS_EV_STRIDE
+    unsigned int ev_cap;     // a multiple of CS_EV_SHARDS; shard s holds [s, s + 1) * ev_cap / CS_EV_SHARDS
     ClaimRec* clr;        // [R][N]
     CS_FLD(ClaimRec, claim) claim;
     CS_FLD(ClaimRec, lastk) lastk;

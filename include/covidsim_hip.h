@@ -369,9 +369,11 @@ int  cs_run_steps(cs_engine* e, double t0, int n_steps, int steps_per_sample);
  * every kernel of every step from the host */
 int  cs_set_step_graphs(cs_engine* e, int enable);
 /* capacity of the infection event queue and of the spatial draw list (in
- * chunks of up to 256 draws of one source cell); 0 keeps a value.  Both grow by doubling when a step
- * overflows them (the step is re-run, with the same result), so capacities
- * only trade memory against re-runs. */
+ * chunks of up to 256 draws of one source cell); 0 keeps a value.  The
+ * event queue is 8 equal shards (appends spread over 8 counts), so its
+ * capacity is rounded up to a multiple of 8.  Both grow by doubling when a
+ * step overflows them (the step is re-run, with the same result), so
+ * capacities only trade memory against re-runs. */
 int  cs_set_queue_capacity(cs_engine* e, uint32_t events, uint32_t draw_chunks);
 int  cs_get_queue_capacity(cs_engine* e, uint32_t* events, uint32_t* draw_chunks);
 /* capacity of each step's latent due list: the persons whose latency ends in

@@ -54,11 +54,18 @@ struct Chunk {  // up to 64 consecutive entries starting at `first` of cell rc's
 };
 enum { WL_SEG = 0, WL_CELLS, WL_DRAWS, WL_N };
 
-struct Work {  // event tallies for the algorithmic-bytes figure
-    unsigned long long n_inf, n_hh_inf, n_hh_member, n_place_link, n_place_cand;
-    unsigned long long n_sp_draw, n_sp_susc, n_new, n_active, n_transition, n_cells;
-    unsigned long long n_sp_scan, n_sp_probe;
+// event tallies for the algorithmic-bytes figure.  Every wave of a kernel
+// adds its sums at the end, and adds from the whole GPU to one 128-byte line
+// serialise at the memory side, so each tally has a line of its own.
+#define CS_WORK_TALLY(name) alignas(128) unsigned long long name
+struct Work {
+    CS_WORK_TALLY(n_inf); CS_WORK_TALLY(n_hh_inf); CS_WORK_TALLY(n_hh_member);
+    CS_WORK_TALLY(n_place_link); CS_WORK_TALLY(n_place_cand);
+    CS_WORK_TALLY(n_sp_draw); CS_WORK_TALLY(n_sp_susc); CS_WORK_TALLY(n_new);
+    CS_WORK_TALLY(n_active); CS_WORK_TALLY(n_transition); CS_WORK_TALLY(n_cells);
+    CS_WORK_TALLY(n_sp_scan); CS_WORK_TALLY(n_sp_probe);
 };
+#undef CS_WORK_TALLY
 
 enum { FLAG_OVERFLOW = 1, FLAG_RANGE = 2, FLAG_BATCH = 4, FLAG_STUCK = 8 };
 // Hot.flags: HOT_HH_DONE, the person's household has no susceptible member

@@ -1255,11 +1255,12 @@ int cs_get_work_counts(cs_engine* e, cs_work_counts* out, int reset)
     HIPCHK(e, hipStreamSynchronize(e->stream));
     cs::Work w;
     HIPCHK(e, hipMemcpy(&w, e->d.wc, sizeof(w), hipMemcpyDeviceToHost));
-    out->n_inf = (int64_t)w.n_inf; out->n_hh_inf = (int64_t)w.n_hh_inf; out->n_hh_member = (int64_t)w.n_hh_member;
-    out->n_place_link = (int64_t)w.n_place_link; out->n_place_cand = (int64_t)w.n_place_cand;
-    out->n_sp_draw = (int64_t)w.n_sp_draw; out->n_sp_susc = (int64_t)w.n_sp_susc; out->n_new = (int64_t)w.n_new;
-    out->n_active = (int64_t)w.n_active; out->n_transition = (int64_t)w.n_transition; out->n_cells = (int64_t)e->NC;
-    out->n_sp_scan = (int64_t)w.n_sp_scan; out->n_sp_probe = (int64_t)w.n_sp_probe;
+    using cs::wsum;
+    out->n_inf = (int64_t)wsum(w.n_inf); out->n_hh_inf = (int64_t)wsum(w.n_hh_inf); out->n_hh_member = (int64_t)wsum(w.n_hh_member);
+    out->n_place_link = (int64_t)wsum(w.n_place_link); out->n_place_cand = (int64_t)wsum(w.n_place_cand);
+    out->n_sp_draw = (int64_t)wsum(w.n_sp_draw); out->n_sp_susc = (int64_t)wsum(w.n_sp_susc); out->n_new = (int64_t)wsum(w.n_new);
+    out->n_active = (int64_t)wsum(w.n_active); out->n_transition = (int64_t)wsum(w.n_transition); out->n_cells = (int64_t)e->NC;
+    out->n_sp_scan = (int64_t)wsum(w.n_sp_scan); out->n_sp_probe = (int64_t)wsum(w.n_sp_probe);
     if (reset) HIPCHK(e, hipMemset(e->d.wc, 0, sizeof(cs::Work)));
     return 0;
 }

@@ -197,7 +197,7 @@ __global__ void __launch_bounds__(256) k_compact(Dev d, int seg, int zero_draws)
     if (seg == 1 && n > 0) d.cells[base_cell + ex_cell] = (int32_t)rc;
     if (seg == 0) {
         act = wave_sum(act);
-        if ((threadIdx.x & 63) == 0 && act) atomicAdd(&d.wc->n_active, act);
+        if ((threadIdx.x & 63) == 0 && act) wtally(d.wc->n_active, act);
     }
 }
 
@@ -667,8 +667,8 @@ __global__ void __launch_bounds__(256) k_infect_households(Dev d)
     w_hh_inf = wave_sum(w_hh_inf);
     w_hh_mem = wave_sum(w_hh_mem);
     if ((threadIdx.x & 63) == 0 && (w_hh_inf | w_hh_mem)) {
-        atomicAdd(&d.wc->n_hh_inf, w_hh_inf);
-        atomicAdd(&d.wc->n_hh_member, w_hh_mem);
+        wtally(d.wc->n_hh_inf, w_hh_inf);
+        wtally(d.wc->n_hh_member, w_hh_mem);
     }
 }
 
@@ -781,8 +781,8 @@ __global__ void __launch_bounds__(256) k_infect_places(Dev d)
     }
     const unsigned long long s_link = wave_sum(w_link), s_cand = wave_sum(w_cand);
     if ((threadIdx.x & 63) == 0 && (s_link | s_cand)) {
-        atomicAdd(&d.wc->n_place_link, s_link);
-        atomicAdd(&d.wc->n_place_cand, s_cand);
+        wtally(d.wc->n_place_link, s_link);
+        wtally(d.wc->n_place_cand, s_cand);
     }
 }
 
@@ -858,7 +858,7 @@ __global__ void __launch_bounds__(CS_SUM_BLOCK) k_cell_spatial_sum(Dev d)
     }
     if (lane == 0 && w_inf) atomicAdd(&blk_inf, w_inf);
     __syncthreads();
-    if (threadIdx.x == 0 && blk_inf) atomicAdd(&d.wc->n_inf, blk_inf);
+    if (threadIdx.x == 0 && blk_inf) wtally(d.wc->n_inf, blk_inf);
 }
 
 // one spatial draw per lane; a wave takes a chunk of up to 64 draws of one
@@ -966,10 +966,10 @@ __global__ void __launch_bounds__(256) k_spatial_draws(Dev d)
     const unsigned long long s_draw = wave_sum(w_draw), s_susc = wave_sum(w_susc);
     const unsigned long long s_scan = wave_sum(w_scan), s_probe = wave_sum(w_probe);
     if ((threadIdx.x & 63) == 0 && s_draw) {
-        atomicAdd(&d.wc->n_sp_draw, s_draw);
-        atomicAdd(&d.wc->n_sp_susc, s_susc);
-        atomicAdd(&d.wc->n_sp_scan, s_scan);
-        atomicAdd(&d.wc->n_sp_probe, s_probe);
+        wtally(d.wc->n_sp_draw, s_draw);
+        wtally(d.wc->n_sp_susc, s_susc);
+        wtally(d.wc->n_sp_scan, s_scan);
+        wtally(d.wc->n_sp_probe, s_probe);
     }
 }
 
@@ -1056,7 +1056,7 @@ __global__ void k_resolve_apply(Dev d)
         }
     }
     nn = wave_sum(nn);
-    if (lane == 0 && nn) atomicAdd(&d.wc->n_new, nn);
+    if (lane == 0 && nn) wtally(d.wc->n_new, nn);
 }
 
 __global__ void k_resolve_reset(Dev d)

@@ -183,7 +183,7 @@ __global__ void __launch_bounds__(256) k_latent_due(Dev d)
         }
     }
     nt = wave_sum(nt);
-    if ((threadIdx.x & 63) == 0 && nt) atomicAdd(&d.wc->n_transition, nt);
+    if ((threadIdx.x & 63) == 0 && nt) wtally(d.wc->n_transition, nt);
 }
 
 // this step's due list is consumed: clear it for the step CS_DUE_RING later
@@ -214,7 +214,7 @@ __global__ void __launch_bounds__(256) k_latent_scan(Dev d)
         }
     }
     nt = wave_sum(nt);
-    if ((threadIdx.x & 63) == 0 && nt) atomicAdd(&d.wc->n_transition, nt);
+    if ((threadIdx.x & 63) == 0 && nt) wtally(d.wc->n_transition, nt);
 }
 
 // DoCase (src/Update.cpp:794-894) without interventions; returns 1 when the
@@ -359,7 +359,7 @@ __global__ void __launch_bounds__(256) k_infectious_scan(Dev d)
         d.gate[sl] = (uint16_t)g;
     }
     nt = wave_sum(nt);
-    if ((threadIdx.x & 63) == 0 && nt) atomicAdd(&d.wc->n_transition, nt);
+    if ((threadIdx.x & 63) == 0 && nt) wtally(d.wc->n_transition, nt);
 }
 
 // child-care absence (src/Update.cpp:836-868), evaluated for every request

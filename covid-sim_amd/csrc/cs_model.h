@@ -56,16 +56,27 @@ enum { WL_SEG = 0, WL_CELLS, WL_DRAWS, WL_N };
 
 // event tallies for the algorithmic-bytes figure.  Every wave of a kernel
 // adds its sums at the end, and adds from the whole GPU to one 128-byte line
-// serialise at the memory side, so each tally has a line of its own.
-#define CS_WORK_TALLY(name) alignas(128) unsigned long long name
-struct Work {
-    CS_WORK_TALLY(n_inf); CS_WORK_TALLY(n_hh_inf); CS_WORK_TALLY(n_hh_member);
-    CS_WORK_TALLY(n_place_link); CS_WORK_TALLY(n_place_cand);
-    CS_WORK_TALLY(n_sp_draw); CS_WORK_TALLY(n_sp_susc); CS_WORK_TALLY(n_new);
-    CS_WORK_TALLY(n_active); CS_WORK_TALLY(n_transition); CS_WORK_TALLY(n_cells);
-    CS_WORK_TALLY(n_sp_scan); CS_WORK_TALLY(n_sp_probe);
+// serialise at the memory side, so each tally has CS_WORK_COPIES copies, a
+// line each: a block adds to copy blockIdx.x % CS_WORK_COPIES (wtally), and
+// the host sums the copies.
+#define CS_WORK_COPIES 8
+struct alignas(128) WorkCell {
+    unsigned long long v;
 };
-#undef CS_WORK_TALLY
+struct Work {
+    WorkCell n_inf[CS_WORK_COPIES], n_hh_inf[CS_WORK_COPIES], n_hh_member[CS_WORK_COPIES];
+    WorkCell n_place_link[CS_WORK_COPIES], n_place_cand[CS_WORK_COPIES];
+    WorkCell n_sp_draw[CS_WORK_COPIES], n_sp_susc[CS_WORK_COPIES], n_new[CS_WORK_COPIES];
+    WorkCell n_active[CS_WORK_COPIES], n_transition[CS_WORK_COPIES];
+    WorkCell n_sp_scan[CS_WORK_COPIES], n_sp_probe[CS_WORK_COPIES];
+};
+__device__ __forceinline__ void wtally(WorkCell* t, unsigned long long v) { atomicAdd(&t[blockIdx.x & (CS_WORK_COPIES - 1u)].v, v); }
+inline unsigned long long wsum(const WorkCell* t)
+{
+    unsigned long long s = 0;
+    for (int i = 0; i < CS_WORK_COPIES; i++) s += t[i].v;
+    return s;
+}
 
 enum { FLAG_OVERFLOW = 1, FLAG_RANGE = 2, FLAG_BATCH = 4, FLAG_STUCK = 8 };
 // Hot.flags: HOT_HH_DONE, the person's household has no susceptible member

@@ -675,7 +675,9 @@ __global__ void __launch_bounds__(256) k_infect_households(Dev d)
 // Within-group and between-group place passes of one infectious person per
 // lane (src/Sweep.cpp:381-640).  Floyd's sample is kept in LDS, one column
 // per thread (bank-conflict free).
-__global__ void __launch_bounds__(256) k_infect_places(Dev d)
+// at most 128 VGPRs (4 waves per SIMD): the kernel waits on dependent
+// gathers, and the extra wave in flight outweighs the few spilled registers
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_infect_places(Dev d)
 {
     const StepScalars sc = *d.sc;
     __shared__ int picked[CS_FLOYD_MAX][256];

@@ -204,6 +204,19 @@ __device__ __forceinline__ double fc_corr(int32_t k)
     return (1.0 / 12.0 - (1.0 / 360.0 - ik2 / 1260.0) * ik2) * ik;
 }
 
+// BTRS's exact acceptance test, taken only when the squeeze fails: kept out
+// of line so that its logarithms do not add to the caller's registers
+__device__ __noinline__ bool btrs_accept(double v, double alpha, double a, double b, double us, double m, double r, int32_t n, double kf)
+{
+    double lv = clog(v * alpha / (a / (us * us) + b));
+    double ub = (m + 0.5) * clog((m + 1.0) / (r * ((double)n - m + 1.0)))
+              + ((double)n + 1.0) * clog(((double)n - m + 1.0) / ((double)n - kf + 1.0))
+              + (kf + 0.5) * clog(r * ((double)n - kf + 1.0) / (kf + 1.0))
+              + fc_corr((int32_t)m) + fc_corr(n - (int32_t)m)
+              - fc_corr((int32_t)kf) - fc_corr(n - (int32_t)kf);
+    return lv <= ub;
+}
+
 // Binomial(n, p): inversion when n*min(p,1-p) < 10, else BTRS (Hormann 1993)
 __device__ __forceinline__ int32_t binomial(Stream& s, int32_t n, double pp)
 {
@@ -239,13 +252,7 @@ __device__ __forceinline__ int32_t binomial(Stream& s, int32_t n, double pp)
             double kf = __builtin_floor((2.0 * a / us + b) * u + c);
             if (kf < 0.0 || kf > (double)n) continue;
             if (us >= 0.07 && v <= vr) { x = (int32_t)kf; break; }
-            double lv = clog(v * alpha / (a / (us * us) + b));
-            double ub = (m + 0.5) * clog((m + 1.0) / (r * ((double)n - m + 1.0)))
-                      + ((double)n + 1.0) * clog(((double)n - m + 1.0) / ((double)n - kf + 1.0))
-                      + (kf + 0.5) * clog(r * ((double)n - kf + 1.0) / (kf + 1.0))
-                      + fc_corr((int32_t)m) + fc_corr(n - (int32_t)m)
-                      - fc_corr((int32_t)kf) - fc_corr(n - (int32_t)kf);
-            if (lv <= ub) { x = (int32_t)kf; break; }
+            if (btrs_accept(v, alpha, a, b, us, m, r, n, kf)) { x = (int32_t)kf; break; }
         }
     }
     return pp > 0.5 ? n - x : x;

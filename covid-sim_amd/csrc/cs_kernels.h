@@ -865,7 +865,9 @@ __global__ void __launch_bounds__(CS_SUM_BLOCK) k_cell_spatial_sum(Dev d)
 
 // one spatial draw per lane; a wave takes a chunk of up to 64 draws of one
 // source cell (src/Sweep.cpp:712-855)
-__global__ void __launch_bounds__(256) k_spatial_draws(Dev d)
+// at most 80 VGPRs (6 waves per SIMD, no spills): each attempt is a chain of
+// dependent gathers, hidden only by waves in flight
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) k_spatial_draws(Dev d)
 {
     const StepScalars sc = *d.sc;
     const unsigned nch = min(d.wl[WL_DRAWS], d.draw_cap);

@@ -987,12 +987,15 @@ struct EvSeq {
         for (int s = 0; s < CS_EV_SHARDS; s++) pre[s + 1] = pre[s] + min(d.ev_count[s * CS_EV_STRIDE], scap);
     }
     __device__ __forceinline__ unsigned n() const { return pre[CS_EV_SHARDS]; }
-    // the queue slot of the k-th event (k < n())
+    // the queue slot of the k-th event (k < n()); constant indices only, so
+    // pre stays in registers (a dynamically indexed copy would go to LDS)
     __device__ __forceinline__ size_t slot(unsigned k) const
     {
-        int s = 0;
-        while (k >= pre[s + 1]) s++;
-        return (size_t)s * scap + (k - pre[s]);
+        unsigned sh = 0, base = 0;
+#pragma unroll
+        for (int j = 1; j < CS_EV_SHARDS; j++)
+            if (k >= pre[j]) { sh = (unsigned)j; base = pre[j]; }
+        return (size_t)sh * scap + (k - base);
     }
 };
 

@@ -411,6 +411,60 @@ __device__ void list_move_cell(const Dev& d, int mode, int rc, int32_t* pos, int
     }
 }
 
+// the same move by one wave for a cell with 0 < k <= 64 moves: lane i holds
+// batch entry i; ballots and shuffles stand in for the LDS arrays and the
+// barriers, and the pairing rule is the same (the j-th hole by position takes
+// the j-th free window position in ascending order)
+__device__ void list_move_cell_wave(const Dev& d, int mode, int rc, int k)
+{
+    const int lane = threadIdx.x & 63;
+    const int r = rc / d.NC, c = rc % d.NC;
+    const size_t rN = (size_t)r * d.N;
+    const size_t base = rN + d.cstart[c];
+    const int S = d.cS[rc], L = d.cL[rc], I = d.cI[rc];
+    const int hi = (mode == 0) ? S : (mode == 1 ? S + L : S + L + I);
+    const int bnd = hi - k;
+    const bool mine = lane < k;
+    const int pos = mine ? d.listpos[rN + d.batch[base + lane]] : 0;
+    if (__ballot(mine && (pos < 0 || pos >= hi))) {  // a list position outside the segment: report, touch nothing
+        if (lane == 0) atomicOr(d.status, (unsigned)FLAG_BATCH);
+        return;
+    }
+    // window positions [bnd, hi) taken by batch entries (bit j: bnd + j)
+    unsigned long long occ = 0;
+    for (unsigned long long m = __ballot(mine && pos >= bnd); m; m &= m - 1)
+        occ |= 1ull << (__shfl(pos, __builtin_ctzll(m), 64) - bnd);
+    const unsigned long long freem = ~occ & ((k == 64) ? ~0ull : ((1ull << k) - 1ull));
+    // holes: batch entries below the window; rank among them by position
+    const bool hole = mine && pos < bnd;
+    int rank = 0;
+    for (unsigned long long m = __ballot(hole); m; m &= m - 1)
+        rank += (__shfl(pos, __builtin_ctzll(m), 64) < pos) ? 1 : 0;
+    if (hole) {
+        unsigned long long f = freem;
+        for (int t = 0; t < rank; t++) f &= f - 1;
+        if (f == 0) {
+            atomicOr(d.status, (unsigned)FLAG_BATCH);  // holes and free positions disagree: cannot happen for a valid batch
+        } else {
+            const int ph = pos, q = bnd + __builtin_ctzll(f);
+            const int pa = d.perm[base + ph], pb = d.perm[base + q];
+            d.perm[base + ph] = pb;
+            d.listpos[rN + pb] = ph;
+            d.perm[base + q] = pa;
+            d.listpos[rN + pa] = q;
+            if (mode == 2) d.gate[base + ph] = d.gate[base + q];  // an infectious person stays: its gate moves with it
+        }
+    }
+    if (mode == 1 && mine) d.gate[base + bnd + lane] = 0;  // [bnd, hi) now heads the infectious list
+    const unsigned long long nd = wave_sum((unsigned long long)((mode == 2 && mine) ? is_dead_code(d.inf[rN + d.batch[base + lane]]) : 0));
+    if (lane == 0) {
+        if (mode == 0) { d.cS[rc] = S - k; d.cL[rc] = L + k; }
+        else if (mode == 1) { d.cL[rc] = L - k; d.cI[rc] = I + k; }
+        else { d.cI[rc] = I - k; d.cD[rc] += (int)nd; d.cR[rc] += k - (int)nd; }
+        d.bcount[rc] = 0;
+    }
+}
+
 __global__ void __launch_bounds__(256) k_list_move(Dev d, int mode)
 {
     if (halted(d)) return;
@@ -421,19 +475,27 @@ __global__ void __launch_bounds__(256) k_list_move(Dev d, int mode)
     __shared__ int dead_count, bad;
     // each block owns a contiguous range of (realisation, cell) pairs and
     // reads their pending-move counts 64 at a time (every wave the same 64,
-    // so the ballot is the same in all of them); a cell's own move clears its
-    // count only after every wave has read it (the move begins with a barrier)
+    // so the ballots are the same in all of them; the barrier after the read
+    // keeps any wave from clearing a count another has yet to read).  Cells
+    // with at most 64 moves go to the block's waves in turn, a wave each;
+    // larger ones take the whole block.
     const unsigned rcn = (unsigned)d.R * (unsigned)d.NC;
     const unsigned per = (rcn + gridDim.x - 1) / gridDim.x;
     const unsigned c0 = min(rcn, blockIdx.x * per), c1 = min(rcn, c0 + per);
+    const int w = (int)(threadIdx.x >> 6), nw = (int)(blockDim.x >> 6);
     for (unsigned b = c0; b < c1; b += 64) {
         const unsigned rc = b + (threadIdx.x & 63);
-        unsigned long long m = __ballot(rc < c1 && d.bcount[rc] != 0);
-        while (m) {
+        const int kk = (rc < c1) ? d.bcount[rc] : 0;
+        __syncthreads();
+        int t = 0;
+        for (unsigned long long m = __ballot(kk > 0 && kk <= 64); m; m &= m - 1, t++) {
+            if (t % nw != w) continue;
             const int j = __builtin_ctzll(m);
-            m &= m - 1;
+            list_move_cell_wave(d, mode, (int)(b + j), __shfl(kk, j, 64));
+        }
+        for (unsigned long long m = __ballot(kk > 64); m; m &= m - 1) {
             __syncthreads();  // the previous cell's use of the shared arrays is over
-            list_move_cell(d, mode, (int)(b + j), pos, fpos, occ, part, &dead_count, &bad);
+            list_move_cell(d, mode, (int)(b + __builtin_ctzll(m)), pos, fpos, occ, part, &dead_count, &bad);
         }
     }
 }

@@ -275,7 +275,8 @@ __device__ int do_case(const Dev& d, int r, int p, unsigned ts)
 
 // infectious lists: case onset, severity and outcome timers
 // (src/Sweep.cpp:929-977); a wave takes a chunk of up to 64 infectious persons
-__global__ void __launch_bounds__(256) k_infectious_scan(Dev d)
+// at most 72 VGPRs: 7 waves per SIMD, no spills
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) k_infectious_scan(Dev d)
 {
     if (halted(d)) return;
     const unsigned ts = d.sc->ts;

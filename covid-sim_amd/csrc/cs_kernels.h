@@ -648,7 +648,8 @@ __device__ __forceinline__ int select_dest(const Dev& d, size_t rc, double u, un
 // infectious persons of one cell.  The place passes are a kernel of their
 // own so that each keeps few registers (many waves in flight hide the
 // latency of the scattered person reads).
-__global__ void __launch_bounds__(256) k_infect_households(Dev d)
+// 8 waves per SIMD (78 SGPRs, no spills): more scattered reads in flight
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) k_infect_households(Dev d)
 {
     const StepScalars sc = *d.sc;
     const unsigned nch = min(d.wl[WL_SEG], d.seg_cap);
@@ -739,6 +740,7 @@ __global__ void __launch_bounds__(256) k_infect_households(Dev d)
 // per thread (bank-conflict free).
 // at most 128 VGPRs (4 waves per SIMD): the kernel waits on dependent
 // gathers, and the extra wave in flight outweighs the few spilled registers
+// (at 5 waves the spills grow to 192 bytes per lane and the kernel slows)
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_infect_places(Dev d)
 {
     const StepScalars sc = *d.sc;

@@ -20,11 +20,28 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "../../include/covidsim_hip.h"
 
+// Device memory of the allocation-free forward (cs_mlp_forward_noalloc): the
+// BF16 input and the two BF16 hidden activations of one batch.  Every GPU the
+// library runs on holds its own copy for the life of the process, and every
+// call on one GPU uses that GPU's copy (calls that overlap on one GPU race).
+// Host code never passes the name `scratch` to a kernel: on the host it is a
+// stand-in, not a device address.  cs_mlp_create looks up the device address
+// once per device (hipGetSymbolAddress, with that device current) and the
+// forward uses only that address.
+static __device__ __align__(256) unsigned char scratch[CS_MLP_NOALLOC_SCRATCH_BYTES];
+
 namespace {
+
+#define CS_MAX_DEVICES 64
+std::mutex g_scratch_mu;
+void* g_scratch_addr[CS_MAX_DEVICES];  // device address of `scratch` on each device, once looked up
+
+size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
@@ -221,6 +238,8 @@ struct cs_mlp {
     uint16_t* X = nullptr;
     float* T = nullptr;
     int n_data = 0;
+    // this device's address of the allocation-free forward's scratch array
+    unsigned char* scratch = nullptr;
 };
 
 namespace {
@@ -283,19 +302,47 @@ int ensure_batch(cs_mlp* m, int b)
     return 0;
 }
 
-// forward of the staged batch xb (b rows): z1, h1, z2, h2, y
-void forward(cs_mlp* m, int b)
+// forward of a BF16 batch xb (b rows) into the given buffers: the FP32
+// pre-activations z1, z2 (either may be null: not kept), the BF16
+// activations h1, h2 and the FP32 output y
+void forward_into(cs_mlp* m, int b, const uint16_t* xb, float* z1, uint16_t* h1, float* z2, uint16_t* h2, float* y)
 {
     const int I = m->n_in, H = m->n_h, O = m->n_out;
-    GemmArgs a = ga(b, H, I, m->xb, I, 1, m->w1b, 1, I);
-    a.bias = m->p + m->off[1]; a.act = 1; a.Cf = m->z1; a.cf_ld = H; a.Cb = m->h1; a.cb_ld = H;
+    GemmArgs a = ga(b, H, I, xb, I, 1, m->w1b, 1, I);
+    a.bias = m->p + m->off[1]; a.act = 1; a.Cf = z1; a.cf_ld = H; a.Cb = h1; a.cb_ld = H;
     gemm(m, a);
-    a = ga(b, H, H, m->h1, H, 1, m->w2b, 1, H);
-    a.bias = m->p + m->off[3]; a.act = 1; a.Cf = m->z2; a.cf_ld = H; a.Cb = m->h2; a.cb_ld = H;
+    a = ga(b, H, H, h1, H, 1, m->w2b, 1, H);
+    a.bias = m->p + m->off[3]; a.act = 1; a.Cf = z2; a.cf_ld = H; a.Cb = h2; a.cb_ld = H;
     gemm(m, a);
-    a = ga(b, O, H, m->h2, H, 1, m->w3b, 1, H);
-    a.bias = m->p + m->off[5]; a.Cf = m->y; a.cf_ld = O;
+    a = ga(b, O, H, h2, H, 1, m->w3b, 1, H);
+    a.bias = m->p + m->off[5]; a.Cf = y; a.cf_ld = O;
     gemm(m, a);
+}
+
+// forward of the staged batch xb (b rows): z1, h1, z2, h2, y
+void forward(cs_mlp* m, int b) { forward_into(m, b, m->xb, m->z1, m->h1, m->z2, m->h2, m->y); }
+
+// bytes of scratch the allocation-free forward needs for b rows: the BF16
+// input, then the two BF16 hidden activations, each 256-byte aligned
+size_t noalloc_bytes(const cs_mlp* m, long long b)
+{
+    return align256((size_t)b * m->n_in * 2) + 2 * align256((size_t)b * m->n_h * 2);
+}
+
+// this device's address of `scratch`, looked up once per device with the
+// device current (the caller has made it current)
+int scratch_addr(cs_mlp* m)
+{
+    if (m->device >= CS_MAX_DEVICES) return mset(m, CS_EINVAL, "device %d beyond the %d the library tracks", m->device, CS_MAX_DEVICES);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_addr[m->device]) {
+        void* p = nullptr;
+        const hipError_t st = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (st != hipSuccess || !p) return mset(m, CS_EHIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(st));
+        g_scratch_addr[m->device] = p;
+    }
+    m->scratch = (unsigned char*)g_scratch_addr[m->device];
+    return 0;
 }
 
 // loss and gradients of the staged batch (targets in tb) into m->g; returns the loss
@@ -399,12 +446,16 @@ int cs_mlp_create(cs_mlp** out, int device, int n_in, int n_hidden, int n_out, u
     m->device = device;
     m->n_in = n_in; m->n_h = n_hidden; m->n_out = n_out;
     if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&m->stream) != hipSuccess) { delete m; return CS_EHIP; }
+    int rc;
+    if ((rc = scratch_addr(m))) {  // set-up of the allocation-free forward on this device
+        *out = m;
+        return rc;
+    }
     const long long sz[6] = {(long long)n_hidden * n_in, n_hidden, (long long)n_hidden * n_hidden, n_hidden,
                              (long long)n_out * n_hidden, n_out};
     long long o = 0;
     for (int k = 0; k < 6; k++) { m->off[k] = o; o += sz[k]; }
     m->n_par = o;
-    int rc;
     if ((rc = malloc_n(m, &m->p, o)) || (rc = malloc_n(m, &m->m, o)) || (rc = malloc_n(m, &m->v, o)) || (rc = malloc_n(m, &m->g, o)) ||
         (rc = malloc_n(m, &m->w1b, sz[0])) || (rc = malloc_n(m, &m->w2b, sz[2])) || (rc = malloc_n(m, &m->w3b, sz[4])) ||
         (rc = malloc_n(m, &m->part, 8192))) {
@@ -472,6 +523,30 @@ int cs_mlp_forward(cs_mlp* m, const float* x, int batch, float* y)
     forward(m, batch);
     MCHK(m, hipGetLastError());
     MCHK(m, hipMemcpyAsync(y, m->y, (size_t)batch * m->n_out * 4, hipMemcpyDeviceToHost, m->stream));
+    MCHK(m, hipStreamSynchronize(m->stream));
+    return 0;
+}
+
+int cs_mlp_forward_noalloc(cs_mlp* m, const float* x, int batch, float* y)
+{
+    if (!m || !x || !y || batch <= 0) return CS_EINVAL;
+    if (!m->scratch) return mset(m, CS_ESTATE, "the allocation-free forward was not set up on device %d", m->device);
+    const size_t need = noalloc_bytes(m, batch);
+    if (need > (size_t)CS_MLP_NOALLOC_SCRATCH_BYTES) {
+        long long most = 0;
+        while (noalloc_bytes(m, most + 1) <= (size_t)CS_MLP_NOALLOC_SCRATCH_BYTES) most++;
+        return mset(m, CS_EINVAL, "batch %d needs %zu bytes of scratch; the allocation-free forward holds %d (at most %lld rows at these layer widths)",
+                    batch, need, CS_MLP_NOALLOC_SCRATCH_BYTES, most);
+    }
+    MCHK(m, hipSetDevice(m->device));
+    const long long nx = (long long)batch * m->n_in;
+    unsigned char* s = m->scratch;
+    uint16_t* xb = (uint16_t*)s;
+    uint16_t* h1 = (uint16_t*)(s + align256((size_t)nx * 2));
+    uint16_t* h2 = (uint16_t*)((unsigned char*)h1 + align256((size_t)batch * m->n_h * 2));
+    k_to_bf16<<<blocks_for(nx), 256, 0, m->stream>>>(x, xb, nx);  // the rounding stage_x applies on the host
+    forward_into(m, batch, xb, nullptr, h1, nullptr, h2, y);
+    MCHK(m, hipGetLastError());
     MCHK(m, hipStreamSynchronize(m->stream));
     return 0;
 }

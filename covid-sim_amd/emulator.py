@@ -31,6 +31,8 @@ def _lib():
         L.cs_mlp_get_params.argtypes = [vp, P(F)]
         L.cs_mlp_set_params.argtypes = [vp, P(F)]
         L.cs_mlp_forward.argtypes = [vp, P(F), I, P(F)]
+        L.cs_mlp_forward_noalloc.argtypes = [vp, vp, I, vp]
+        L.cs_mlp_forward_noalloc.restype = I
         L.cs_mlp_forward_trace.argtypes = [vp, P(F), I, P(F), P(F), P(C.c_uint16), P(C.c_uint16), P(F)]
         L.cs_mlp_loss_grad.argtypes = [vp, P(F), P(F), I, P(D), P(F)]
         L.cs_mlp_set_data.argtypes = [vp, P(F), P(F), I]
@@ -90,6 +92,11 @@ class Emulator:
         y = np.zeros((len(x), self.n_out), np.float32)
         self._check(self.L.cs_mlp_forward(self.h, _p(x, C.c_float), len(x), _p(y, C.c_float)))
         return y
+
+    def forward_device(self, x_dev, batch, y_dev):
+        """The allocation-free forward: x_dev and y_dev are device addresses
+        ([batch][n_in] and [batch][n_out] FP32) the caller owns."""
+        self._check(self.L.cs_mlp_forward_noalloc(self.h, C.c_void_p(int(x_dev)), int(batch), C.c_void_p(int(y_dev))))
 
     def forward_trace(self, x):
         x = np.ascontiguousarray(x, np.float32).reshape(-1, self.n_in)

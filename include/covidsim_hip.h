@@ -480,6 +480,21 @@ int  cs_mlp_forward(cs_mlp* m, const float* x, int batch, float* y);
  * BF16 activations h1, h2 (raw bits), [batch][n_hidden] each; any may be NULL */
 int  cs_mlp_forward_trace(cs_mlp* m, const float* x, int batch, float* z1, float* z2,
                           uint16_t* h1, uint16_t* h2, float* y);
+/* Allocation-free forward.  No counterpart in the reference; it is for
+ * callers that cannot allocate device memory (tight serving loops such as a
+ * calibration that scores candidates one batch at a time, harnesses that
+ * forbid hipMalloc).  Same result as cs_mlp_forward, bit for bit, but x
+ * ([batch][n_in] FP32) and y ([batch][n_out] FP32) are DEVICE pointers owned
+ * by the caller, and the batch's intermediate activations live in a fixed
+ * device array of CS_MLP_NOALLOC_SCRATCH_BYTES that each GPU holds for the
+ * life of the process (set up by cs_mlp_create).  The call allocates and
+ * frees nothing; it runs on the emulator's stream and returns when y is
+ * written.  A batch needing more than the array holds (batch x (2 n_in +
+ * 4 n_hidden) bytes, each part rounded up to 256) is refused with CS_EINVAL.
+ * All calls on one GPU, from any emulator, share that GPU's array: calls that
+ * overlap on one GPU race. */
+#define CS_MLP_NOALLOC_SCRATCH_BYTES 16777216
+int  cs_mlp_forward_noalloc(cs_mlp* m, const float* x, int batch, float* y);
 /* loss and its gradient with respect to every parameter, for one batch */
 int  cs_mlp_loss_grad(cs_mlp* m, const float* x, const float* t, int batch, double* loss, float* grad);
 /* training set kept in HBM (inputs stored as BF16) */

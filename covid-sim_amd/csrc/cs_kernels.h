@@ -855,16 +855,18 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
 // Per (realisation, cell): the cumulative spatial infectiousness by slot,
 // signed by the closure flag and normalised when draws follow, and the
 // Poisson number of spatial draws (src/Sweep.cpp:649-708).  The running
-// sum is kept in exact 64.64 fixed point, so this block-parallel scan gives
-// the same prefixes as a sequential pass; each prefix is rounded to double
-// once, then stored as float as the reference does.
+// sum is kept in exact 64.64 fixed point, so every prefix is the exact sum
+// of its terms rounded once to double, whatever the order of the parallel
+// scan; the reference instead adds in double one term at a time
+// (src/Sweep.cpp:680-686), so a prefix here can differ from its in the last
+// bit.  Each prefix is then stored as float, as the reference does.
 #define CS_SUM_BLOCK 256
 
 // one wave per cell with I > 0; the slots are read in rounds of 64, one per
 // lane (coalesced).  Pass 1 sums the cell's total (the draw count needs it),
 // pass 2 scans each round across the wave with the carry of the rounds
 // before: 128-bit integer adds are exact and associative, so every prefix
-// equals the sequential pass's.
+// is the same for any order of the adds.
 __global__ void __launch_bounds__(CS_SUM_BLOCK) k_cell_spatial_sum(Dev d)
 {
     const StepScalars sc = *d.sc;

@@ -30,3 +30,28 @@ def hz_population():
     popa = dict(pop.a)
     popa["p_flags"] = arrays["hz_p_flags"]
     return csa.population.Population(popa)
+
+
+@functools.lru_cache(maxsize=None)
+def c1():
+    """(Population, params) of the 200k-person C1 region, as the reference's
+    set-up builds it from covid-sim_amd/inputs/c1 (/c:1)."""
+    csa = importlib.import_module("covid-sim_amd")
+    z = np.load(os.path.join(GOLDEN, "c1_population.npz"))
+    pop = csa.population.Population({k: z[k] for k in z.files})
+    params = csa.population.load_params(os.path.join(os.path.dirname(HERE), "covid-sim_amd", "inputs", "c1", "params.bin"))
+    return pop, params
+
+
+@functools.lru_cache(maxsize=None)
+def mini_care():
+    """(Population, params) of the 20k-person region with care homes: mini's
+    population with the reference's care-home resident flags, and the
+    parameters the reference derives with the care-home keys
+    (tests/golden/make_fixtures.py CARE_KEYS)."""
+    csa = importlib.import_module("covid-sim_amd")
+    arrays, pop, params, _ = mini()
+    popa = dict(pop.a)
+    popa["p_flags"] = np.load(os.path.join(GOLDEN, "mini_care_flags.npz"))["p_flags"]
+    params = csa.population.load_params(os.path.join(GOLDEN, "mini_care_params.bin"))
+    return csa.population.Population(popa), params

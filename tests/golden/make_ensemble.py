@@ -12,7 +12,7 @@ reduces them to daily series per realisation:
 
 The realisations differ only in their run seeds (the reference chains them
 with setall, src/CovidSim.cpp:356-369); the population is the same one the
-fixtures hold.  Usage:  python make_ensemble.py mini 200
+fixtures hold.  Usage:  python make_ensemble.py mini|mini_care|c1 200
 """
 import os
 import subprocess
@@ -24,14 +24,16 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, HERE)
-from make_fixtures import run_harness  # noqa: E402
+from make_fixtures import care_pre_params, run_harness  # noqa: E402
 
 
 def main(name, runs, threads=1, n_seeds=10):
-    inputs = os.path.join(ROOT, "covid-sim_amd", "inputs", name)
+    # mini_care: the 20k-person region with care homes (make_fixtures.CARE_KEYS)
+    inputs = os.path.join(ROOT, "covid-sim_amd", "inputs", "mini" if name == "mini_care" else name)
     tmp = tempfile.mkdtemp()
     series = os.path.join(tmp, "series.bin")
-    run_harness(inputs, tmp, steps="", golden=False, threads=threads, series=series, nr=runs)
+    pre = care_pre_params(tmp) if name == "mini_care" else None
+    run_harness(inputs, tmp, steps="", golden=False, threads=threads, series=series, nr=runs, pre_params=pre)
     rec = np.fromfile(series, dtype=np.int32).reshape(-1, 26)
     steps_per_day = 4
     ndays = (rec[:, 1].max() + 1) // steps_per_day

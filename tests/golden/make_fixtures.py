@@ -37,13 +37,54 @@ HARNESS = os.path.join(ROOT, "oracle", "_ref", "golden_harness")
 SEEDS = ["98798150", "729101", "17389101", "4797132"]
 
 
-def run_harness(inputs, out, steps="0", golden=True, threads=1, extra=(), series=None, nr=None, extra_env=None):
+# the care-home variant of the 20k-person fixture region: the reference's
+# own parameter files never set a care-home place type, so this one makes the
+# workplaces (place type 3) care homes whose members aged 55 and over are
+# residents, with every care-home scaling away from 1 so that each branch of
+# the sweep that reads them (src/Sweep.cpp:483, :574-576, :792), the hazards'
+# scalings and DoIncub's hospitalisation override (src/Update.cpp:446) change
+# the outcome
+CARE_KEYS = """
+[Place type number for care homes]
+3
+
+[Minimum age of care home residents]
+55
+
+[Scaling of household contacts for care home residents]
+0.6
+
+[Scaling of spatial contacts for care home residents]
+0.4
+
+[Scaling of between group (home) contacts for care home residents]
+3.0
+
+[Scaling of within group (home) contacts for care home workers]
+0.5
+
+[Relative probability that care home residents are hospitalised]
+0.3
+"""
+
+
+def care_pre_params(out_dir):
+    """mini's pre-parameter file with the care-home keys appended"""
+    src = os.path.join(ROOT, "covid-sim_amd", "inputs", "mini", "pre-params.txt")
+    dst = os.path.join(out_dir, "care-pre-params.txt")
+    with open(src) as f, open(dst, "w") as g:
+        g.write(f.read().rstrip("\n") + "\n" + CARE_KEYS)
+    return dst
+
+
+def run_harness(inputs, out, steps="0", golden=True, threads=1, extra=(), series=None, nr=None, extra_env=None,
+                pre_params=None):
     env = dict(os.environ, CS_DUMP_DIR=out, CS_DUMP_STEPS=steps, CS_DUMP_GOLDEN="1" if golden else "0")
     env.update(extra_env or {})
     if series:
         env["CS_SERIES"] = series
     cmd = [HARNESS, "/c:%d" % threads,
-           "/PP:" + os.path.join(inputs, "pre-params.txt"),
+           "/PP:" + (pre_params or os.path.join(inputs, "pre-params.txt")),
            "/P:" + os.path.join(inputs, "input-noint-params.txt"),
            "/O:" + os.path.join(out, "out"),
            "/D:" + os.path.join(inputs, "density.txt"),
@@ -56,6 +97,24 @@ def run_harness(inputs, out, steps="0", golden=True, threads=1, extra=(), series
         sys.stderr.write(r.stderr[-3000:])
         raise SystemExit("harness failed")
     return r
+
+
+def population_fixture(name, out_npz, params_out=None, extra_env=None, steps=""):
+    """The population the reference's set-up builds for a workload's inputs
+    (/c:1), and the state dumps of `steps`, as one npz (kernel tables left
+    out: they are rebuilt from the kernel formula)."""
+    inputs = os.path.join(ROOT, "covid-sim_amd", "inputs", name)
+    tmp = tempfile.mkdtemp()
+    try:
+        run_harness(inputs, tmp, steps=steps, golden=False, threads=1, extra_env=extra_env)
+        arrays = {fn[:-4]: np.load(os.path.join(tmp, fn)) for fn in sorted(os.listdir(tmp))
+                  if fn.endswith(".npy") and not fn.startswith("kernel_")}
+        np.savez_compressed(out_npz, **arrays)
+        if params_out:
+            shutil.copy(os.path.join(tmp, "params.bin"), params_out)
+        print("wrote", out_npz, len(arrays), "arrays")
+    finally:
+        shutil.rmtree(tmp)
 
 
 def main():
@@ -94,5 +153,32 @@ def main():
         shutil.rmtree(tmp)
 
 
+def care():
+    """mini_care_params.bin (cs_params) and mini_care_flags.npz (the person
+    flags with the care-home residents marked); the rest of the population
+    is mini's, checked here"""
+    inputs = os.path.join(ROOT, "covid-sim_amd", "inputs", "mini")
+    base = np.load(os.path.join(HERE, "mini_fixture.npz"))
+    tmp = tempfile.mkdtemp()
+    try:
+        run_harness(inputs, tmp, steps="", golden=False, threads=1, pre_params=care_pre_params(tmp))
+        for fn in sorted(os.listdir(tmp)):
+            if fn.endswith(".npy") and not fn.startswith("kernel_") and fn != "p_flags.npy":
+                assert np.array_equal(np.load(os.path.join(tmp, fn)), base[fn[:-4]]), fn
+        flags = np.load(os.path.join(tmp, "p_flags.npy"))
+        assert (flags & 1).sum() > 1000
+        np.savez_compressed(os.path.join(HERE, "mini_care_flags.npz"), p_flags=flags)
+        shutil.copy(os.path.join(tmp, "params.bin"), os.path.join(HERE, "mini_care_params.bin"))
+        print("care-home residents", int((flags & 1).sum()))
+    finally:
+        shutil.rmtree(tmp)
+
+
 if __name__ == "__main__":
-    main()
+    if len(sys.argv) > 1 and sys.argv[1] == "care":
+        care()
+    elif len(sys.argv) > 1 and sys.argv[1] == "c1":
+        # the 200k-person C1 population (ensemble and block-path parity tests)
+        population_fixture("c1", os.path.join(HERE, "c1_population.npz"))
+    else:
+        main()

@@ -12,7 +12,7 @@ import importlib
 import numpy as np
 import pytest
 
-from fixtures import hz_population, mini
+from fixtures import c1, hz_population, mini
 from oracle_lib import OracleEngine
 
 pytestmark = pytest.mark.gpu
@@ -347,3 +347,83 @@ def test_household_beyond_the_size_table_is_rejected(engmod, csa):
         eng.reset(10)
         eng.run_steps(0.0, 10 * tspd, tspd)
     assert np.array_equal(g.timeseries(10), o.timeseries(10))
+
+
+def test_per_realisation_params_match_oracle(engmod, csa):
+    """Distinct LocalBeta and seed counts per realisation
+    (cs_set_realisation_params, the parameter sweep of a batched run): the
+    60-day series, states and counters equal the oracle's with the same
+    overrides, and the overrides really change the epidemics."""
+    arrays, pop, params, _ = mini()
+    seeds = np.array([51, 52, 53, 54], np.uint64)
+    lb = np.array([0.5, 1.0, 1.6, 2.2]) * params.local_beta
+    ns = np.array([1, 10, 40, 5], np.int32)
+    g = engmod.Engine(0)
+    g.set_params(params)
+    g.set_population(pop)
+    g.alloc(seeds)
+    o = OracleEngine(params, pop)
+    o.alloc(seeds)
+    tspd = int(params.time_steps_per_day)
+    for eng in (g, o):
+        eng.set_realisation_params(local_beta=lb, n_seed=ns)
+        eng.reset(10)
+        eng.run_steps(0.0, 60 * tspd, tspd)
+    tg = g.timeseries(60)
+    assert np.array_equal(tg, o.timeseries(60))
+    for r in range(len(seeds)):
+        a, b = csa.population.State(pop), csa.population.State(pop)
+        g.store_state(r, a)
+        o.store_state(r, b)
+        assert a.compare(b) == []
+        _compare_counters(g.counters(r), o.counters(r))
+    # the seed counts took effect (the first sample, after day 1, has at
+    # least the seeds infected) and beta orders the growth
+    assert (pop.n_people - tg[:, 0, 0] >= ns).all() and pop.n_people - tg[2, 0, 0] >= 40
+    assert tg[3, -1, 0] < tg[0, -1, 0]
+
+
+def test_block_list_moves_match_oracle(engmod, csa):
+    """Cells with more than 64 list moves in one step take the block path of
+    the batched move (a wave takes smaller ones).  On the 200k-person C1
+    region with its LocalBeta raised fourfold (so the peak is sharp), step by
+    step through days 34-37: every state equal to the oracle's, and the
+    per-cell move counts of each kind (S->L, L->I, I->R/D), derived from the
+    cell totals, exceed 64 in some cell."""
+    pop, params = c1()
+    seeds = np.array([61, 62], np.uint64)
+    g = engmod.Engine(0)
+    g.set_params(params)
+    g.set_population(pop)
+    g.alloc(seeds)
+    o = OracleEngine(params, pop)
+    o.alloc(seeds)
+    tspd = int(params.time_steps_per_day)
+    day0 = 34
+    for eng in (g, o):
+        eng.set_realisation_params(local_beta=np.full(2, 4.0 * params.local_beta))
+        eng.reset(10)
+        eng.run_steps(0.0, day0 * tspd, tspd)
+
+    def cells(eng, r):
+        st = csa.population.State(pop)
+        eng.store_state(r, st)
+        a = st.a
+        return st, np.stack([a["cell_S"], a["cell_L"], a["cell_I"], a["cell_R"] + a["cell_D"]]).astype(np.int64)
+
+    most = np.zeros(3, np.int64)
+    prev = [cells(o, r)[1] for r in range(2)]
+    for k in range(12):
+        t = day0 + k / tspd
+        for eng in (g, o):
+            eng.run_steps(t, 1, 1)
+        for r in range(2):
+            a, ca = cells(g, r)
+            b, cb = cells(o, r)
+            assert a.compare(b) == [], (k, r)
+            s_l = prev[r][0] - cb[0]
+            l_i = prev[r][1] + s_l - cb[1]
+            i_r = cb[3] - prev[r][3]
+            most = np.maximum(most, [s_l.max(), l_i.max(), i_r.max()])
+            prev[r] = cb
+    assert (most > 64).all(), most

@@ -1,52 +1,72 @@
-"""The oracle's stochastic trajectories against the reference's, as ensembles.
+"""The oracle's and the GPU's stochastic trajectories against the
+reference's, as ensembles.
 
 The random streams differ by design (counter-based streams replace the
 reference's per-thread generators), so single trajectories cannot match;
-the distribution of trajectories must.  The reference ensemble
-(tests/golden/mini_ensemble.npz) is 200 realisations of the unmodified
-reference on the 20k-person fixture region (tests/golden/make_ensemble.py);
-the oracle runs 200 realisations on the same population and parameters.
+the distribution of trajectories must.  Each reference ensemble is 200
+realisations of the unmodified reference (tests/golden/make_ensemble.py) on
+one of three workloads, all with the population the reference's own set-up
+builds (/c:1):
+
+  mini       the 20k-person fixture region, 120 days
+  mini_care  the same region with care homes (workplaces whose members aged
+             55 and over are residents, every care-home scaling away from 1;
+             tests/golden/make_fixtures.py CARE_KEYS), 120 days
+  c1         the 200k-person C1 region, 200 days
 
 Compared: attack rate, peak day, peak daily incidence, cumulative cases and
 deaths, each with a two-sample Kolmogorov-Smirnov test (p > 0.01) and
 agreement of the two ensembles' 95% confidence intervals for the mean (the
-intervals overlap); and the daily infections and cases of every day with a
-per-day KS test,
-Bonferroni-corrected over the days (family-wise level 0.01; 120
-uncorrected tests at 0.01 would reject a true null in 70% of suites).
-A 1000-versus-1000 run of the same comparison (not part of the suite)
-gave |z| < 1.8 for every statistic.
+intervals overlap); the daily infections and cases of every day with a
+per-day KS test, Bonferroni-corrected over the days (family-wise level 0.01;
+120 uncorrected tests at 0.01 would reject a true null in 70% of suites); and
+the final infections by route.  A 1000-versus-1000 run of the mini
+comparison (not part of the suite) gave |z| < 1.8 for every statistic.
 
-On the GPU the same 200 seeds, advanced together by one batched run, must
-give the oracle's 200 trajectories bit for bit and pass the same tests
-against the reference's 200.
+On the CPU the oracle's 200 realisations are compared (mini, mini_care).  On
+the GPU the same 200 seeds, advanced together by one batched run, must give
+the oracle's 200 trajectories bit for bit and pass the same tests.  For c1,
+where 200 oracle realisations of 200k persons would take minutes, the GPU's
+200 are compared with the reference's, and its first two realisations with
+the oracle's bit for bit.
 """
-import ctypes as C
 import functools
+import importlib
 import os
 
 import numpy as np
 import pytest
 from scipy import stats
 
-from fixtures import GOLDEN, mini
+from fixtures import GOLDEN, c1, mini, mini_care
 from oracle_lib import OracleEngine
 
 N_RUNS = 200
 ALPHA = 0.01
+SEEDS = np.arange(1, N_RUNS + 1, dtype=np.uint64) * np.uint64(2654435761)
+
+
+def workload(name):
+    if name == "mini":
+        _, pop, params, _ = mini()
+    elif name == "mini_care":
+        pop, params = mini_care()
+    else:
+        pop, params = c1()
+    z = np.load(os.path.join(GOLDEN, "%s_ensemble.npz" % name))
+    return pop, params, z["daily"], z["itype"].astype(float)
 
 
 @functools.lru_cache(maxsize=None)
-def ensembles():
-    ref = np.load(os.path.join(GOLDEN, "mini_ensemble.npz"))["daily"]
-    arrays, pop, params, _ = mini()
+def oracle_ensemble(name):
+    pop, params, ref, _ = workload(name)
     days = ref.shape[1]
     o = OracleEngine(params, pop)
-    o.alloc(np.arange(1, N_RUNS + 1, dtype=np.uint64) * np.uint64(2654435761))
+    o.alloc(SEEDS)
     o.reset(10)
     o.run_steps(0.0, days * int(params.time_steps_per_day), int(params.time_steps_per_day))
-    ensembles.itype = np.array([list(o.counters(r).cumItype) for r in range(N_RUNS)], float)
-    return ref, o.timeseries(days), pop.n_people
+    itype = np.array([list(o.counters(r).cumItype) for r in range(N_RUNS)], float)
+    return o.timeseries(days), itype
 
 
 SUMMARIES = ["attack rate", "peak day", "peak incidence", "cumulative cases", "deaths"]
@@ -73,65 +93,82 @@ def cis_overlap(a, b, level=0.95):
     return abs(a.mean() - b.mean()) <= ha + hb + 1e-12
 
 
-def test_population_conserved():
-    ref, orc, n = ensembles()
-    assert (orc[:, :, :5].sum(axis=2) == n).all()
+def check_against_reference(ref, ref_it, got, got_it, n):
+    """Every ensemble comparison of the module docstring."""
+    assert (got[:, :, :5].sum(axis=2) == n).all()
     assert (ref[:, :, :5].sum(axis=2) == n).all()
-
-
-@pytest.mark.parametrize("name", SUMMARIES)
-def test_summary_statistics(name):
-    ref, orc, n = ensembles()
-    idx = SUMMARIES.index(name)
-    a, b = summary(ref, n)[idx].astype(float), summary(orc, n)[idx].astype(float)
-    assert stats.ks_2samp(a, b).pvalue > ALPHA, (a.mean(), b.mean())
-    assert cis_overlap(a, b), (a.mean(), b.mean())
-
-
-@pytest.mark.parametrize("field,name", [(5, "infections"), (6, "cases")])
-def test_daily_incidence_per_day_ks(field, name):
-    ref, orc, n = ensembles()
+    for idx, name in enumerate(SUMMARIES):
+        a, b = summary(ref, n)[idx].astype(float), summary(got, n)[idx].astype(float)
+        assert stats.ks_2samp(a, b).pvalue > ALPHA, (name, a.mean(), b.mean())
+        assert cis_overlap(a, b), (name, a.mean(), b.mean())
     days = ref.shape[1]
-    pv = np.array([stats.ks_2samp(ref[:, d, field], orc[:, d, field]).pvalue for d in range(days)])
-    assert pv.min() > ALPHA / days, (name, pv.argmin(), pv.min())
-
-
-def test_infection_routes():
-    """Final infections by route (household, within-group and between-group
-    place per place type, spatial; StateT.cumItype) agree in mean: each
-    route's oracle mean lies within a Welch 99.9% interval of the
-    reference's."""
-    ref_it = np.load(os.path.join(GOLDEN, "mini_ensemble.npz"))["itype"].astype(float)
-    ensembles()
-    orc_it = ensembles.itype
-    for q in range(1, 15):  # 0 unused, 15 = seeds (cleared in the reference before the first sample)
-        if ref_it[:, q].sum() == 0 and orc_it[:, q].sum() == 0:
+    for field in (5, 6):
+        pv = np.array([stats.ks_2samp(ref[:, d, field], got[:, d, field]).pvalue for d in range(days)])
+        assert pv.min() > ALPHA / days, (field, pv.argmin(), pv.min())
+    # final infections by route: household, within- and between-group place
+    # per place type, spatial (StateT.cumItype); each route's mean within a
+    # Welch 99.9% interval of the reference's (0 is unused, 15 = seeds,
+    # cleared in the reference before the first sample)
+    for q in range(1, 15):
+        if ref_it[:, q].sum() == 0 and got_it[:, q].sum() == 0:
             continue
-        assert welch_ci_contains_zero(ref_it[:, q], orc_it[:, q], level=0.999), (q, ref_it[:, q].mean(), orc_it[:, q].mean())
+        assert welch_ci_contains_zero(ref_it[:, q], got_it[:, q], level=0.999), (q, ref_it[:, q].mean(), got_it[:, q].mean())
 
 
-@pytest.mark.gpu
-def test_gpu_ensemble_is_the_oracle_ensemble_and_matches_the_reference():
-    """The GPU's 200 realisations, advanced together by the batched run,
-    reproduce the oracle's 200 bit for bit, and pass the same ensemble tests
-    against the reference's 200 (summaries by KS and overlapping 95% CIs,
-    daily incidence by per-day KS)."""
-    import importlib
-    ref, orc, n = ensembles()
-    arrays, pop, params, _ = mini()
+@pytest.mark.parametrize("name", ["mini", "mini_care"])
+def test_oracle_ensemble_matches_reference(name):
+    pop, params, ref, ref_it = workload(name)
+    got, got_it = oracle_ensemble(name)
+    check_against_reference(ref, ref_it, got, got_it, pop.n_people)
+
+
+def test_care_homes_change_the_epidemic():
+    """The care-home fixture is not inert: its reference ensemble differs
+    from the plain region's in the infections within care homes (route of
+    place type 3) and in attack rate."""
+    _, _, ref, ref_it = workload("mini")
+    _, _, cref, cref_it = workload("mini_care")
+    assert stats.ks_2samp(ref_it[:, 2 + 3], cref_it[:, 2 + 3]).pvalue < 1e-6
+    assert stats.ks_2samp(summary(ref, 20000)[0], summary(cref, 20000)[0]).pvalue < 1e-3
+
+
+def gpu_run(pop, params, seeds, days):
     g = importlib.import_module("covid-sim_amd.engine").Engine(0)
     g.set_params(params)
     g.set_population(pop)
-    g.alloc(np.arange(1, N_RUNS + 1, dtype=np.uint64) * np.uint64(2654435761))
+    g.alloc(seeds)
     g.reset(10)
-    days = ref.shape[1]
     tspd = int(params.time_steps_per_day)
     g.run_steps(0.0, days * tspd, tspd)
-    gpu = g.timeseries(days)
+    itype = np.array([list(g.counters(r).cumItype) for r in range(len(seeds))], float)
+    return g.timeseries(days), itype
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["mini", "mini_care"])
+def test_gpu_ensemble_is_the_oracle_ensemble_and_matches_the_reference(name):
+    """The GPU's 200 realisations, advanced together by the batched run,
+    reproduce the oracle's 200 bit for bit, and pass the ensemble tests
+    against the reference's 200."""
+    pop, params, ref, ref_it = workload(name)
+    orc, orc_it = oracle_ensemble(name)
+    gpu, gpu_it = gpu_run(pop, params, SEEDS, ref.shape[1])
     assert gpu.shape == orc.shape and np.array_equal(gpu, orc)
-    for idx, name in enumerate(SUMMARIES):
-        a, b = summary(ref, n)[idx].astype(float), summary(gpu, n)[idx].astype(float)
-        assert stats.ks_2samp(a, b).pvalue > ALPHA and cis_overlap(a, b), (name, a.mean(), b.mean())
-    for field in (5, 6):
-        pv = np.array([stats.ks_2samp(ref[:, d, field], gpu[:, d, field]).pvalue for d in range(days)])
-        assert pv.min() > ALPHA / days, (field, pv.argmin(), pv.min())
+    assert np.array_equal(gpu_it, orc_it)
+    check_against_reference(ref, ref_it, gpu, gpu_it, pop.n_people)
+
+
+@pytest.mark.gpu
+def test_gpu_c1_ensemble_matches_the_reference():
+    """C1 (200k persons, 200 days): the GPU's 200 realisations against the
+    reference's 200, and its first two realisations equal to the oracle's."""
+    pop, params, ref, ref_it = workload("c1")
+    days = ref.shape[1]
+    gpu, gpu_it = gpu_run(pop, params, SEEDS, days)
+    check_against_reference(ref, ref_it, gpu, gpu_it, pop.n_people)
+    o = OracleEngine(params, pop)
+    o.alloc(SEEDS[:2])
+    o.reset(10)
+    tspd = int(params.time_steps_per_day)
+    o.run_steps(0.0, days * tspd, tspd)
+    assert np.array_equal(gpu[:2], o.timeseries(days))

@@ -48,11 +48,16 @@ HBM_PEAK_GBS = 8000.0
 
 # minimum bytes touched per event (the per-event costs of the step's
 # algorithmic-bytes formula); fixed, never tuned to the measurement
-BYTES = {"n_inf": 64, "n_hh_inf": 16, "n_hh_member": 40, "n_place_link": 32, "n_place_cand": 44,
+BYTES = {"n_inf": 64, "n_hh_inf": 16, "n_hh_member": 40, "n_hh_scan": 40, "n_place_link": 32, "n_place_cand": 44,
          "n_sp_draw": 48, "n_sp_susc": 40, "n_new": 40, "n_active": 16, "n_transition": 30,
          "n_cell_steps": 32, "n_sp_scan": 4, "n_sp_probe": 4}
+# the events each kernel's bytes are counted from.  The household kernel is
+# charged 40 B only per member its scans actually read (n_hh_scan): a
+# household already found to hold no susceptible member is skipped, and its
+# infector then reads no household record either, so neither the skipped
+# members (n_hh_member counts them) nor a per-household 16 B is charged
 KERNEL_UNITS = {
-    "infect_households": ["n_inf", "n_hh_inf", "n_hh_member"],
+    "infect_households": ["n_inf", "n_hh_scan"],
     "infect_places": ["n_place_link", "n_place_cand"],
     "cell_spatial_sum": ["n_cell_steps"],
     "spatial_draws": ["n_sp_draw", "n_sp_susc", "n_sp_scan", "n_sp_probe"],
@@ -258,17 +263,27 @@ def main():
     total = float(sum(times))
     person_days = float(args.realisations) * args.people * args.days * args.steps
     value = person_days / total
-    # dominant kernel and its roofline
-    dom = max((k for k in prof if k in KERNEL_UNITS), key=lambda k: prof[k][0])
+    # the roofline of every kernel with counted work, the dominant one first
+    def roof(k):
+        ms, launches = prof[k]
+        b = sum(BYTES[u] * wc[u] for u in KERNEL_UNITS[k]) / max(launches, 1)
+        a = b / (ms / max(launches, 1) * 1e-3) / 1e9
+        tr = measured_traffic(k, args)
+        return {"kernel": k, "ms_total": round(ms, 1), "launches": launches, "bytes_per_launch": round(b),
+                "ms_per_launch": round(ms / max(launches, 1), 4), "achieved_gbs": round(a, 2),
+                "frac": round(a / HBM_PEAK_GBS, 5), "traffic_bytes_per_launch": tr,
+                "traffic_over_algorithmic": round(tr / b, 2) if tr and b else None}
+    ranked = sorted((k for k in prof if k in KERNEL_UNITS), key=lambda k: -prof[k][0])
+    top = [roof(k) for k in ranked[:5]]
+    dom = ranked[0]
     ms, launches = prof[dom]
-    alg_bytes = sum(BYTES[u] * wc[u] for u in KERNEL_UNITS[dom]) / max(launches, 1)
-    achieved = alg_bytes / (ms / max(launches, 1) * 1e-3) / 1e9
+    alg_bytes, achieved = top[0]["bytes_per_launch"], top[0]["achieved_gbs"]
     attack = float(np.mean((ts[:, -1, 3] + ts[:, -1, 4]) / args.people)) if ts.size else 0.0
     log("kernel ms: " + ", ".join("%s %.0f/%d" % (k, v[0], v[1]) for k, v in prof.items()))
     log("work: " + json.dumps(wc))
     log("mean attack rate %.3f" % attack)
     total_alg = sum(BYTES[u] * wc[u] for u in BYTES)
-    traffic = measured_traffic(dom, args)
+    traffic = top[0]["traffic_bytes_per_launch"]
     base = None if args.no_cpu_baseline else cpu_baseline()
     out = {
         "metric": "person-days/s, 64-run ensemble",
@@ -292,7 +307,8 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                      "bytes_per_launch": round(alg_bytes), "ms_per_launch": round(ms / max(launches, 1), 4),
                      "measured": "HIP events around every launch of a profiled pass of the same workload "
-                                 "(%.2f s, direct launches)" % t_prof},
+                                 "(%.2f s, direct launches)" % t_prof,
+                     "top_kernels": top},
         "cpu_baseline": base,
     }
     print(json.dumps(out), flush=True)

@@ -93,6 +93,7 @@ cs_counters = _make("cs_counters")
 cs_work_counts = _make("cs_work_counts")
 cs_transmission_coeffs = _make("cs_transmission_coeffs")
 cs_tuning = _make("cs_tuning")
+cs_intervention_state = _make("cs_intervention_state")
 
 globals().update({k: v for k, v in CONSTS.items()})
 
@@ -120,6 +121,7 @@ def declare(lib, prefix):
         "set_queue_capacity": (I, [vp, C.c_uint32, C.c_uint32]),
         "get_queue_capacity": (I, [vp, P(C.c_uint32), P(C.c_uint32)]),
         "get_counters": (I, [vp, I, P(cs_counters), I]),
+        "get_intervention_state": (I, [vp, I, P(cs_intervention_state)]),
         "get_timeseries": (I, [vp, P(C.c_int32), I, P(I)]),
         "get_work_counts": (I, [vp, P(cs_work_counts), I]),
         "kernel_num": (I, [vp, P(D), P(D), I]),

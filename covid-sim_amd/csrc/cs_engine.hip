@@ -11,8 +11,8 @@
 #include <vector>
 
 #include "../../include/covidsim_hip.h"
-#include "cs_kernels_incub.h"
 #include "cs_kernels_itc.h"
+#include "cs_kernels_itv.h"
 
 using cs::Dev;
 
@@ -172,14 +172,52 @@ unsigned grid_for(long long n, int block, unsigned cap = 65535u * 16)
     return (unsigned)g;
 }
 
+// the parameter sets this library serves: the path without interventions,
+// or with the intervention subset (case isolation and household quarantine
+// of detected cases, place closure by microcell, on global detected-case or
+// ICU triggers, with the outbreak alert of CalibrationThresholdCheck when no
+// calibration is asked for); anything else is refused, never approximated
+const char* unsupported(const cs_params& p)
+{
+    const double never = 10000.0;
+    if (p.false_positive_rate != 0.0) return "false-positive infections are outside this path";
+    if (p.do_digital_contact_tracing) return "digital contact tracing is outside this path";
+    if (p.do_blanket_move_restr) return "movement restrictions are outside this path";
+    if (p.absenteeism_place_closure) return "absenteeism place closure is outside this path";
+    if (!p.interventions_active) return nullptr;
+    if (!p.do_global_triggers || p.do_admin_triggers) return "interventions need global triggers (admin-unit and microcell triggers are outside this path)";
+    if (p.do_per_capita_triggers) return "per-capita triggers are outside this path";
+    if (p.do_no_calibration || p.do_alert_trigger_after_interv || p.date_trigger_reached_cal_time >= 0)
+        return "calibration to a trigger date is outside this path";
+    if (p.place_close_by_admin_unit || p.do_intervention_delays_by_adunit) return "admin-unit place closure and delays are outside this path";
+    if (p.vary_efficacies_over_time) return "efficacies varying over time are outside this path";
+    if (p.do_mass_vacc || p.treat_time_start_base < never || p.vacc_time_start_base < never || p.vacc_time_start_geo < never ||
+        p.soc_dist_time_start_base < never || p.move_restr_time_start_base < never || p.key_worker_proph_time_start_base < never ||
+        p.airport_close_time_start_base < never)
+        return "treatment, vaccination, social distancing, movement restrictions, key-worker prophylaxis and airport closure are outside this path";
+    if (p.place_close_time_start_base2 < never) return "a second place closure is outside this path";
+    if (p.place_close_indep_thresh > 0) return "the independent place-closure threshold is outside this path";
+    if (p.place_close_cell_inc_stop_thresh != 0) return "reopening places on a stop threshold is outside this path";
+    if (p.triggers_sampling_interval < 1 || p.window_to_evaluate_trigger_alert < 1) return "trigger windows must be at least one sample";
+    return nullptr;
+}
+
+// a whole run (reset, cs_run_steps): the served parameter sets
 int check_path(cs_engine* e)
 {
-    const cs_params& P = e->P;
-    if (P.interventions_active) return set_err(e, CS_EUNSUPPORTED, "interventions are outside this path");
-    if (P.false_positive_rate != 0.0) return set_err(e, CS_EUNSUPPORTED, "false-positive infections are outside this path");
-    if (P.do_digital_contact_tracing) return set_err(e, CS_EUNSUPPORTED, "digital contact tracing is outside this path");
-    if (P.do_blanket_move_restr) return set_err(e, CS_EUNSUPPORTED, "movement restrictions are outside this path");
-    if (P.absenteeism_place_closure) return set_err(e, CS_EUNSUPPORTED, "absenteeism place closure is outside this path");
+    const char* why = unsupported(e->P);
+    return why ? set_err(e, CS_EUNSUPPORTED, "%s", why) : 0;
+}
+
+// one sweep called on its own (the drop-in's InfectSweep and
+// IncubRecoverySweep): interventions need the triggers and TreatSweep that
+// RunModel runs between sweeps, so only cs_run_steps serves them
+int check_sweep(cs_engine* e)
+{
+    int rc = check_path(e);
+    if (rc) return rc;
+    if (e->P.interventions_active)
+        return set_err(e, CS_EUNSUPPORTED, "interventions run only in cs_run_steps (the triggers and TreatSweep run between sweeps)");
     return 0;
 }
 
@@ -192,6 +230,8 @@ int check_flags(cs_engine* e)
     if (f & cs::FLAG_OVERFLOW) return set_err(e, CS_EOVERFLOW, "infection event queue overflow (capacity %u)", e->d.ev_cap);
     if (f & cs::FLAG_BATCH) return set_err(e, CS_EUNSUPPORTED, "more than %d list moves in one cell in one step", CS_MOVE_CAP);
     if (f & cs::FLAG_STUCK) return set_err(e, CS_ERANGE, "spatial draw found no live candidate in %d attempts", CS_MAX_ATTEMPTS);
+    if (f & cs::FLAG_TRIG) return set_err(e, CS_EUNSUPPORTED, "a trigger window reaches back more than %d samples", CS_TRIG_RING - 1);
+    if (f & cs::FLAG_LIST) return set_err(e, CS_EOVERFLOW, "intervention lists full (detected cases %u, closing places %u)", e->d.det_cap, e->d.close_cap);
     return 0;
 }
 
@@ -258,9 +298,15 @@ int launch_incub(cs_engine* e)
         ProfScope ps(e, CS_K_OTHER);
         cs::k_zero_counts<<<1, 64, 0, st>>>(d.care_count, 1);
         cs::k_zero_counts<<<1, 64, 0, st>>>(d.wl, cs::WL_N);
+        if (e->P.interventions_active) cs::k_zero_counts<<<1, 64, 0, st>>>(d.itv_n, 3);
         cs::k_compact<<<grid_for(nrc, 256), 256, 0, st>>>(d, 1, 0);
     }
     { ProfScope ps(e, CS_K_INFECTIOUS_SCAN); cs::k_infectious_scan<<<e->grid_waves, 256, 0, st>>>(d); }
+    if (e->P.interventions_active) {  // DoDetectedCase: quarantine, then isolation
+        ProfScope ps(e, CS_K_OTHER);
+        cs::k_detected_hq<<<256, 256, 0, st>>>(d);
+        cs::k_detected_ci<<<256, 256, 0, st>>>(d);
+    }
     {
         ProfScope ps(e, CS_K_OTHER);
         cs::k_care_eval<<<256, 256, 0, st>>>(d);
@@ -343,6 +389,14 @@ int launch_step_tail(cs_engine* e, bool sample_done)
     Dev& d = e->d;
     int rc;
     if ((rc = launch_incub(e))) return rc;
+    if (e->P.interventions_active) {  // TreatSweep and UpdateHostClosure (src/CovidSim.cpp:1498-1503)
+        ProfScope ps(e, CS_K_OTHER);
+        cs::k_zero_counts<<<1, 64, 0, e->stream>>>(d.itv_n + 1, 2);
+        cs::k_treat_mcells<<<grid_for((long long)d.R * d.NM, 256), 256, 0, e->stream>>>(d);
+        cs::k_close_members<<<e->grid_waves, 256, 0, e->stream>>>(d, 0);
+        cs::k_close_members<<<e->grid_waves, 256, 0, e->stream>>>(d, 1);
+        cs::k_host_closure<<<256, 256, 0, e->stream>>>(d);
+    }
     cs::k_sdrop<<<grid_for(d.R, 64), 64, 0, e->stream>>>(d, e->d_sel);
     if ((rc = launch_update_probs(e, e->d_sel))) return rc;
     if (sample_done) cs::k_record<<<d.R, 256, 0, e->stream>>>(d, -1);
@@ -400,6 +454,24 @@ int run_direct(cs_engine* e, double t, int from, int n_steps, int sps)
         if ((rc = launch_step_tail(e, (step + 1) % sps == 0))) return rc;
         t += e->P.model_time_step;
     }
+    return 0;
+}
+
+// the lists of the intervention kernels, sized on the first run with an
+// intervention set: the step's detected cases (at most one case per person;
+// an eighth of the ensemble's persons, at least 2^16) and its closing places
+int ensure_itv(cs_engine* e)
+{
+    Dev& d = e->d;
+    if (!e->P.interventions_active || d.det) return 0;
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    drop_graph(e);
+    const size_t RN = (size_t)d.R * d.N;
+    d.det_cap = (unsigned)std::min<size_t>(std::max<size_t>(RN / 8, (size_t)1 << 16), std::min<size_t>(RN, 0x7fffffff));
+    d.close_cap = (unsigned)std::min<size_t>(std::max<size_t>((size_t)d.R * d.np_off[CS_MAX_PLACE_TYPES], 1), 0x7fffffff);
+    int rc;
+    if ((rc = dalloc(e, e->real_allocs, &d.det, d.det_cap))) return rc;
+    if ((rc = dalloc(e, e->real_allocs, &d.close_l, d.close_cap))) return rc;
     return 0;
 }
 
@@ -656,6 +728,22 @@ int cs_set_population(cs_engine* e, const cs_population* pop)
         d.pl_gst[t].base = d.pl_gsz[t].base = d.pl_grp[t];
     }
     d.np_off[CS_MAX_PLACE_TYPES] = off;
+    {   // the places of each microcell (Microcell.places), by type then index
+        std::vector<int32_t> ms(e->NM + 1, 0), ml(std::max(off, 1));
+        for (int t = 0; t < e->NPT; t++)
+            for (int k = 0; k < e->np[t]; k++) {
+                const int m = pop->place_mcell[t] ? pop->place_mcell[t][k] : -1;
+                if (m < 0 || m >= e->NM) return set_err(e, CS_EINVAL, "place %d of type %d: microcell out of range", k, t);
+                if (k >= (1 << 27)) return set_err(e, CS_EINVAL, "more than 2^27 places of type %d", t);
+                ms[m + 1]++;
+            }
+        for (int m = 0; m < e->NM; m++) ms[m + 1] += ms[m];
+        std::vector<int32_t> cur(ms.begin(), ms.end() - 1);
+        for (int t = 0; t < e->NPT; t++)
+            for (int k = 0; k < e->np[t]; k++) ml[cur[pop->place_mcell[t][k]]++] = (t << 27) | k;
+        if ((rc = dupload(e, L, &d.mpl_start, ms.data(), e->NM + 1))) return rc;
+        if ((rc = dupload(e, L, &d.mpl, ml.data(), std::max(off, 1)))) return rc;
+    }
     for (int i = 0; i < N; i++)
         for (int t = 0; t < e->NPT; t++) {
             int lk = pop->place_links[(size_t)i * CS_MAX_PLACE_TYPES + t];
@@ -706,6 +794,8 @@ int cs_alloc_realisations(cs_engine* e, int n_real, const uint64_t* seeds)
     AL(cS, RC); AL(cL, RC); AL(cI, RC); AL(cR, RC); AL(cD, RC); AL(cumTC, RC); AL(S0, RC);
     AL(tot_prob, RC); AL(cum_trans, RC * NC); AL(invcdf, RC * 1025); AL(perm, RN);
     AL(pl_cs, PL); AL(pl_ce, PL); AL(m_socdist, R * NM); AL(m_moverest, R * NM); AL(m_trig, R * NM * 6);
+    AL(ist, R); AL(trig_hist, R * CS_TRIG_RING * 3); AL(mc_state, R * NM); AL(mc_end, R * NM); AL(pl_once, PL); AL(itv_n, 4);
+    d.det = nullptr; d.close_l = nullptr; d.det_cap = d.close_cap = 0;  // sized when interventions first run (ensure_itv)
     AL(cnt, R); AL(prev_prop, R); AL(samp_prop, R); AL(loc_init, 2 * R);
     AL(si_buf, RN); AL(ci_buf, RN); AL(spK, RC);
     d.seg_cap = (unsigned)std::min<size_t>(RN / 64 + RC + 1, 0xffffffffu);
@@ -734,6 +824,8 @@ int cs_alloc_realisations(cs_engine* e, int n_real, const uint64_t* seeds)
     HIPCHK(e, hipMemset(d.wl, 0, cs::WL_N * 4));
     HIPCHK(e, hipMemset(d.cnt, 0, R * sizeof(cs::Counters)));
     HIPCHK(e, hipMemset(d.m_trig, 0, R * NM * 6 * 4));
+    HIPCHK(e, hipMemset(d.itv_n, 0, 4 * sizeof(unsigned)));
+    cs::k_reset_interventions<<<grid_for(d.R, 64), 64, 0, e->stream>>>(d);
     d.ts_buf = nullptr;
     d.ts_max = 0;
     e->ts_n = 0;
@@ -780,6 +872,8 @@ int cs_reset_realisations(cs_engine* e, int n_seed)
     cs::k_reset_persons<<<grid_for((long long)d.R * d.N, 256), 256, 0, st>>>(d);
     cs::k_reset_cells<<<grid_for((long long)d.R * d.NC, 64), 64, 0, st>>>(d);
     cs::k_reset_misc<<<grid_for(std::max((long long)d.R * d.np_off[CS_MAX_PLACE_TYPES], (long long)d.R * d.NM), 256), 256, 0, st>>>(d);
+    cs::k_reset_interventions<<<grid_for(d.R, 64), 64, 0, st>>>(d);
+    if ((rc = ensure_itv(e))) return rc;
     if ((rc = launch_update_probs(e, nullptr))) return rc;
     cs::k_zero_counts<<<1, 64, 0, st>>>(d.wl, cs::WL_N);
     cs::k_seed<<<grid_for(d.R, 64), 64, 0, st>>>(d, n_seed);
@@ -927,7 +1021,7 @@ int cs_infect_sweep(cs_engine* e, double t)
 {
     if (!e) return CS_EINVAL;
     if (e->R == 0) return set_err(e, CS_ESTATE, "no realisations allocated");
-    int rc = check_path(e);
+    int rc = check_sweep(e);
     if (rc) return rc;
     HIPCHK(e, hipSetDevice(e->device));
     return infect_sweep(e, t);
@@ -937,7 +1031,7 @@ int cs_incub_recovery_sweep(cs_engine* e, double t)
 {
     if (!e) return CS_EINVAL;
     if (e->R == 0) return set_err(e, CS_ESTATE, "no realisations allocated");
-    int rc = check_path(e);
+    int rc = check_sweep(e);
     if (rc) return rc;
     HIPCHK(e, hipSetDevice(e->device));
     cs::k_step_begin<<<1, 64, 0, e->stream>>>(e->d, t, 0);
@@ -963,11 +1057,12 @@ int cs_run_steps(cs_engine* e, double t0, int n_steps, int sps)
     int rc = check_path(e);
     if (rc) return rc;
     HIPCHK(e, hipSetDevice(e->device));
+    if ((rc = ensure_itv(e))) return rc;
     Dev& d = e->d;
     const int ns = n_steps / sps;
     if (e->ts_fresh) {  // a new series: RunModel samples before its first timestep
         e->ts_n = 0;
-        cs::k_record_init<<<grid_for(d.R, 64), 64, 0, e->stream>>>(d);
+        cs::k_record_init<<<grid_for(d.R, 64), 64, 0, e->stream>>>(d, t0);
         e->ts_fresh = false;
     }
     if (e->ts_n + ns > d.ts_max || !d.ts_buf) {  // grow, keeping the samples taken so far
@@ -1201,6 +1296,8 @@ int cs_get_counters(cs_engine* e, int real, cs_counters* out, int reset)
     out->cumI = (int64_t)c.cumI; out->cumC = (int64_t)c.cumC; out->cumDC = (int64_t)c.cumDC; out->cumFC = (int64_t)c.cumFC;
     out->cumTG = (int64_t)c.cumTG; out->cumSI = (int64_t)c.cumSI; out->nTG = (int64_t)c.nTG;
     out->cumAC = (int64_t)c.cumAC; out->cumACS = (int64_t)c.cumACS; out->cumAA = (int64_t)c.cumAA;
+    out->cumHQ = (int64_t)c.cumHQ; out->cumAH = (int64_t)c.cumAH; out->cumAPC = (int64_t)c.cumAPC;
+    out->cumAPA = (int64_t)c.cumAPA; out->cumAPCS = (int64_t)c.cumAPCS;
     out->sumRad2 = c.sumRad2;
     memcpy(&out->maxRad2, &c.maxRad2_bits, 8);
     for (int i = 0; i < CS_INFECT_TYPE_MASK; i++) out->cumItype[i] = (int64_t)c.cumItype[i];
@@ -1234,6 +1331,28 @@ int cs_get_counters(cs_engine* e, int real, cs_counters* out, int reset)
     return 0;
 }
 
+int cs_get_intervention_state(cs_engine* e, int real, cs_intervention_state* out)
+{
+    if (!e || !out) return CS_EINVAL;
+    if (real < 0 || real >= e->R) return set_err(e, CS_EINVAL, "realisation %d out of range", real);
+    HIPCHK(e, hipSetDevice(e->device));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    Dev& d = e->d;
+    cs::IntState s;
+    HIPCHK(e, hipMemcpy(&s, d.ist + real, sizeof(s), hipMemcpyDeviceToHost));
+    std::vector<uint8_t> mc(e->NM);
+    HIPCHK(e, hipMemcpy(mc.data(), d.mc_state + (size_t)real * e->NM, e->NM, hipMemcpyDeviceToHost));
+    memset(out, 0, sizeof(*out));
+    out->alert = s.alert;
+    out->trigger = s.trig;
+    out->control_prop_cases_id = e->P.interventions_active ? s.ctrl : e->P.control_prop_cases_id;
+    out->case_isolation_start = s.ci_start;
+    out->hquarantine_start = s.hq_start;
+    out->place_close_start = s.pc_start;
+    for (uint8_t v : mc) out->mcells_closed += v == cs::MC_TREATED;
+    return 0;
+}
+
 int cs_get_timeseries(cs_engine* e, int32_t* out, int max_samples, int* n_samples)
 {
     if (!e || !out || !n_samples) return CS_EINVAL;
@@ -1257,6 +1376,7 @@ int cs_get_work_counts(cs_engine* e, cs_work_counts* out, int reset)
     HIPCHK(e, hipMemcpy(&w, e->d.wc, sizeof(w), hipMemcpyDeviceToHost));
     using cs::wsum;
     out->n_inf = (int64_t)wsum(w.n_inf); out->n_hh_inf = (int64_t)wsum(w.n_hh_inf); out->n_hh_member = (int64_t)wsum(w.n_hh_member);
+    out->n_hh_scan = (int64_t)wsum(w.n_hh_scan);
     out->n_place_link = (int64_t)wsum(w.n_place_link); out->n_place_cand = (int64_t)wsum(w.n_place_cand);
     out->n_sp_draw = (int64_t)wsum(w.n_sp_draw); out->n_sp_susc = (int64_t)wsum(w.n_sp_susc); out->n_new = (int64_t)wsum(w.n_new);
     out->n_active = (int64_t)wsum(w.n_active); out->n_transition = (int64_t)wsum(w.n_transition); out->n_cells = (int64_t)e->NC;

@@ -656,7 +656,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) k
     const cs_params* P = d.P;
     const unsigned ts = sc.ts;
     const int lane = threadIdx.x & 63;
-    unsigned long long w_hh_inf = 0, w_hh_mem = 0;
+    unsigned long long w_hh_inf = 0, w_hh_mem = 0, w_hh_scan = 0;
     for (unsigned ch = grid_wave(); ch < nch; ch += grid_nwaves()) {
         const Chunk e = chunk_at(d.segl, ch);
         const int rc = e.rc;
@@ -683,6 +683,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) k
                 unsigned smask = 0;
                 int f0 = 0;
                 if (!(fl & HOT_HH_DONE)) {
+                    w_hh_scan += nh;
                     f0 = d.hfirst[d.hh[i]];
                     for (int m = f0; m < f0 + nh; m++)
                         if (d.inf[rN + m] == CS_SUSCEPTIBLE) smask |= 1u << (m - f0);
@@ -729,9 +730,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) k
         }
     w_hh_inf = wave_sum(w_hh_inf);
     w_hh_mem = wave_sum(w_hh_mem);
+    w_hh_scan = wave_sum(w_hh_scan);
     if ((threadIdx.x & 63) == 0 && (w_hh_inf | w_hh_mem)) {
         wtally(d.wc->n_hh_inf, w_hh_inf);
         wtally(d.wc->n_hh_member, w_hh_mem);
+        wtally(d.wc->n_hh_scan, w_hh_scan);
     }
 }
 

@@ -155,7 +155,8 @@ __device__ void do_incub(const Dev& d, int r, int p, unsigned ts)
         }
 #undef CH
     }
-    if (st == CS_INF_ALMOST_SYMPT && ((P->control_prop_cases_id == 1) || (s.next() < P->control_prop_cases_id))) {
+    const double ctrl = control_prop(d, r);
+    if (st == CS_INF_ALMOST_SYMPT && ((ctrl == 1) || (s.next() < ctrl))) {
         d.bits[gp] |= 2;
         d.t_det[gp] = (uint16_t)(ts + (uint16_t)(P->latent_to_sympt_delay * tspd));
     }
@@ -301,8 +302,15 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) k
         uint16_t ct = (uint16_t)(d.t_lat[gp] + ((int)(P->latent_to_sympt_delay / P->model_time_step)));
         if (P->do_symptoms && ts == ct) {
             if (do_case(d, r, p, ts)) {
-                unsigned idx = atomicAdd(d.care_count, 1u);
-                d.care_gp[idx] = (int32_t)gp;  // capacity N*R is never exceeded (one request per case)
+                const unsigned idx = atomicAdd(d.care_count, 1u);
+                if (idx < (unsigned)((size_t)d.R * d.N)) d.care_gp[idx] = (int32_t)gp;  // a child case makes at most this and one isolation request
+                else atomicOr(d.status, (unsigned)FLAG_LIST);
+            }
+            // a detected case: DoDetectedCase's interventions follow the scan
+            if (P->interventions_active && (d.bits[gp] & 2)) {
+                const unsigned idx = atomicAdd(d.itv_n, 1u);
+                if (idx < d.det_cap) d.det[idx] = (int32_t)gp;
+                else atomicOr(d.status, (unsigned)FLAG_LIST);
             }
             nt++;
         }
@@ -363,16 +371,20 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) k
     if ((threadIdx.x & 63) == 0 && nt) wtally(d.wc->n_transition, nt);
 }
 
-// child-care absence (src/Update.cpp:836-868), evaluated for every request
-// of the step against the state after all timers, then applied
+// child-care absence (src/Update.cpp:836-868, and case isolation's form
+// :752-776), evaluated for every request of the step against the state after
+// all timers, then applied.  A request is gp (DoCase's form) or ~gp (case
+// isolation's form, which sets the carer's absence start even when the
+// carer is already absent).
 __global__ void k_care_eval(Dev d)
 {
     if (halted(d)) return;
     const unsigned ts = d.sc->ts;
     const cs_params* P = d.P;
-    unsigned n = *d.care_count;
+    const unsigned n = min(*d.care_count, (unsigned)((size_t)d.R * d.N));
     for (unsigned a = blockIdx.x * blockDim.x + threadIdx.x; a < n; a += gridDim.x * blockDim.x) {
-        size_t gp = (size_t)d.care_gp[a];
+        const int32_t v = d.care_gp[a];
+        size_t gp = (size_t)(v >= 0 ? v : ~v);
         int r = (int)(gp / d.N), p = (int)(gp % d.N);
         const size_t rN = (size_t)r * d.N;
         int h = d.hh[p], j1 = d.hfirst[h], j2 = j1 + d.hnh[h];
@@ -397,13 +409,14 @@ __global__ void k_care_apply(Dev d)
     if (halted(d)) return;
     const unsigned ts = d.sc->ts;
     const cs_params* P = d.P;
-    unsigned n = *d.care_count;
+    const unsigned n = min(*d.care_count, (unsigned)((size_t)d.R * d.N));
     for (unsigned a = blockIdx.x * blockDim.x + threadIdx.x; a < n; a += gridDim.x * blockDim.x) {
         int kk = d.care_k[a];
         if (kk < 0) continue;
-        size_t gp = (size_t)d.care_gp[a];
+        const int32_t v = d.care_gp[a];
+        size_t gp = (size_t)(v >= 0 ? v : ~v);
         size_t gk = (gp / d.N) * d.N + kk;
-        if (!d.care_abs[a]) d.t_as[gk] = (uint16_t)(ts + P->us_case_isolation_delay);
+        if (!d.care_abs[a] || v < 0) d.t_as[gk] = (uint16_t)(ts + P->us_case_isolation_delay);
         d.t_ae[gk] = (uint16_t)(ts + P->us_case_isolation_delay + P->us_case_isolation_duration);
     }
 }
@@ -472,11 +485,15 @@ __global__ void k_reset_misc(Dev d)
         d.pl_cs[g] = CS_UNSET_TIME;
         d.pl_ce[g] = 0;
     }
+    for (long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x; g < n; g += (long long)gridDim.x * blockDim.x)
+        d.pl_once[g] = 0;
     const long long nm = (long long)d.R * d.NM;
     for (long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x; g < nm; g += (long long)gridDim.x * blockDim.x) {
         d.m_socdist[g] = 0;
         d.m_moverest[g] = 0;
         for (int q = 0; q < 6; q++) d.m_trig[g * 6 + q] = 0;
+        d.mc_state[g] = MC_UNTREATED;
+        d.mc_end[g] = 0;
     }
 }
 
@@ -518,6 +535,10 @@ __global__ void k_seed(Dev d, int n_seed)
     }
 }
 
+// the trigger logic run at each sample when interventions are active
+// (defined with the intervention kernels)
+__device__ void itv_triggers(const Dev& d, int r, int m, double t, long long D);
+
 // daily totals (one block per realisation)
 // sample: the index to write, or -1 for the device clock's sample count
 __global__ void k_record(Dev d, int sample)
@@ -539,6 +560,9 @@ __global__ void k_record(Dev d, int sample)
         __syncthreads();
     }
     if (threadIdx.x == 0) d.samp_prop[r] = (double)part[0][0] / (double)d.N;
+    // the next output step's CalibrationThresholdCheck: this is sample
+    // number sample + 1 of the series, at the time after the step
+    if (threadIdx.x == 0 && d.P->interventions_active) itv_triggers(d, r, sample + 1, d.sc->t + d.P->model_time_step, part[4][0]);
     if (threadIdx.x == 0 && sample < d.ts_max) {
         int32_t* o = d.ts_buf + ((size_t)r * d.ts_max + sample) * CS_TS_FIELDS;
         const Counters& cn = d.cnt[r];
@@ -558,7 +582,7 @@ __global__ void k_record(Dev d, int sample)
     }
 }
 
-__global__ void k_record_init(Dev d)
+__global__ void k_record_init(Dev d, double t0)
 {
     int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= d.R) return;
@@ -573,6 +597,8 @@ __global__ void k_record_init(Dev d)
     lt[1] = (long long)d.cnt[r].cumC;
     lt[2] = D;
     for (int k = 0; k < CS_NUM_SEV_KINDS; k++) lt[3 + k] = (long long)d.cnt[r].sev_cum[k];
+    // the series' first sample and its CalibrationThresholdCheck (src/CovidSim.cpp:1413-1414)
+    if (d.P->interventions_active) itv_triggers(d, r, 0, t0, D);
 }
 
 // ----------------------------------------------- single-function kernels

@@ -37,7 +37,23 @@ struct Counters {  // device mirror of cs_counters, updated with atomics
     unsigned long long sev_prev[CS_NUM_SEV_KINDS], sev_cum[CS_NUM_SEV_KINDS];
     unsigned long long sev_prev_age[CS_NUM_SEV_KINDS][CS_NUM_AGE_GROUPS];
     unsigned long long sev_cum_age[CS_NUM_SEV_KINDS][CS_NUM_AGE_GROUPS];
+    unsigned long long cumHQ, cumAH, cumAPC, cumAPA, cumAPCS;
 };
+
+// one realisation's copy of the P and AdUnits fields the trigger logic sets
+// (src/CovidSim.cpp:3485-3689); written at samples by one thread
+struct IntState {
+    int alert;                 // the alert condition has held at a sample
+    int trig;                  // State.trigDetectedCases of the last sample
+    double ctrl;               // P.ControlPropCasesId
+    double ci_start, hq_start, pc_start, pc_prev;
+};
+// samples of trigger history kept per realisation (cumDC, cumCritical, D at
+// sample m in slot m % CS_TRIG_RING); a trigger window that reaches back
+// further within the run is an error
+#define CS_TRIG_RING 256
+// microcell place-closure states (TreatStat, src/Model.h)
+enum { MC_UNTREATED = 0, MC_TREATED = 1, MC_DONT_TREAT_AGAIN = 2 };
 
 struct StepScalars {  // per-timestep constants of InfectSweep (src/Sweep.cpp:294-301)
     double t, fp, seas, beta_s, beta_h;
@@ -69,6 +85,7 @@ struct Work {
     WorkCell n_sp_draw[CS_WORK_COPIES], n_sp_susc[CS_WORK_COPIES], n_new[CS_WORK_COPIES];
     WorkCell n_active[CS_WORK_COPIES], n_transition[CS_WORK_COPIES];
     WorkCell n_sp_scan[CS_WORK_COPIES], n_sp_probe[CS_WORK_COPIES];
+    WorkCell n_hh_scan[CS_WORK_COPIES];
 };
 __device__ __forceinline__ void wtally(WorkCell* t, unsigned long long v) { atomicAdd(&t[blockIdx.x & (CS_WORK_COPIES - 1u)].v, v); }
 inline unsigned long long wsum(const WorkCell* t)
@@ -78,7 +95,7 @@ inline unsigned long long wsum(const WorkCell* t)
     return s;
 }
 
-enum { FLAG_OVERFLOW = 1, FLAG_RANGE = 2, FLAG_BATCH = 4, FLAG_STUCK = 8 };
+enum { FLAG_OVERFLOW = 1, FLAG_RANGE = 2, FLAG_BATCH = 4, FLAG_STUCK = 8, FLAG_TRIG = 16, FLAG_LIST = 32 };
 // Hot.flags: HOT_HH_DONE, the person's household has no susceptible member
 // left (none can return to susceptible, so the household pass may skip it
 // for good); HOT_NO_LINKS, the person has no place link (a copy of PLinks
@@ -252,6 +269,18 @@ struct Dev {
     uint16_t *pl_cs, *pl_ce;  // [R][total places]
     uint8_t *m_socdist, *m_moverest;
     uint32_t* m_trig;
+    // interventions (launched only when P->interventions_active)
+    IntState* ist;        // [R]
+    int32_t* trig_hist;   // [R][CS_TRIG_RING][3]
+    uint8_t* mc_state;    // [R][NM] Microcell.placeclose
+    uint16_t* mc_end;     // [R][NM] Microcell.place_end_time
+    uint8_t* pl_once;     // [R][total places] closed under once-only (Place.control_trig at USHRT_MAX - 1)
+    const int32_t* mpl_start;  // [NM + 1] places by microcell (Microcell.places) ...
+    const int32_t* mpl;        // ... each type << 27 | place
+    int32_t* det;         // this step's detected cases (gp)
+    int32_t* close_l;     // places closing in this step's TreatSweep: r * total places + place offset
+    unsigned* itv_n;      // counts: [0] det, [1] close_l, [2] carers queued by closures (in care_k)
+    unsigned det_cap, close_cap;
     Counters* cnt;
     double* prev_prop;
     double* samp_prop;  // State.S / PopSize as of the last sample, per realisation
@@ -348,6 +377,12 @@ __device__ __forceinline__ bool isolated(const Dev& d, size_t gp, unsigned ts)
            (int)d.t_is[gp] + (int)P->us_case_isolation_delay + (int)P->us_case_isolation_duration > (int)ts;
 }
 __device__ __forceinline__ bool dct_traced(const Dev& d, size_t gp) { return d.P->interventions_active && d.dct[gp] == 1; }
+// P.ControlPropCasesId of realisation r: its own when interventions run (the
+// alert switches it), else the run's
+__device__ __forceinline__ double control_prop(const Dev& d, int r)
+{
+    return d.P->interventions_active ? d.ist[r].ctrl : d.P->control_prop_cases_id;
+}
 __device__ __forceinline__ bool socdist(const Dev& d, int r, int p)
 {
     return d.P->interventions_active && d.m_socdist[(size_t)r * d.NM + d.mcell[p]] == 2;

@@ -180,6 +180,12 @@ class Engine:
         self.call("get_counters", int(r), C.byref(c), int(bool(reset)))
         return c
 
+    def intervention_state(self, r):
+        """cs_get_intervention_state of realisation r"""
+        st = abi.cs_intervention_state()
+        self._check(self.L.cs_get_intervention_state(self.h, int(r), C.byref(st)))
+        return st
+
     def timeseries(self, max_samples):
         """[n_real, n_samples, CS_TS_FIELDS] int32: per sample S, L, I, R, D,
         incidence of infections, cases and deaths, incidence of each severity

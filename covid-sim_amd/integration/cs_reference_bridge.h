@@ -183,6 +183,61 @@ inline void fill_params(cs_params* c)
     c->got_waifw_spatial = P.Got_WAIFW_Matrix_Spatial;
     for (int a = 0; a < NUM_AGE_GROUPS; a++) c->initial_immunity[a] = P.InitialImmunity[a];
     c->care_home_prop_residents = P.CareHomePropResidents;
+    // the intervention subset and the switches of everything outside it
+    c->do_global_triggers = P.DoGlobalTriggers;
+    c->do_icu_triggers = P.DoICUTriggers;
+    c->do_admin_triggers = P.DoAdminTriggers;
+    c->do_per_capita_triggers = P.DoPerCapitaTriggers;
+    c->triggers_sampling_interval = P.TriggersSamplingInterval;
+    c->trigger_alert_on_deaths = P.TriggerAlertOnDeaths;
+    c->case_or_death_threshold_before_alert = P.CaseOrDeathThresholdBeforeAlert;
+    c->window_to_evaluate_trigger_alert = P.WindowToEvaluateTriggerAlert;
+    c->do_no_calibration = P.DoNoCalibration;
+    c->do_alert_trigger_after_interv = P.DoAlertTriggerAfterInterv;
+    c->date_trigger_reached_cal_time = P.DateTriggerReached_CalTime;
+    c->pre_alert_control_prop_cases_id = P.PreAlertControlPropCasesId;
+    c->post_alert_control_prop_cases_id = P.PostAlertControlPropCasesId;
+    c->case_isolation_time_start_base = P.CaseIsolationTimeStartBase;
+    c->case_isolation_prop = P.CaseIsolationProp;
+    c->case_isolation_policy_duration = P.CaseIsolationPolicyDuration;
+    c->case_isolation_cell_inc_thresh = P.CaseIsolation_CellIncThresh;
+    c->hquarantine_time_start_base = P.HQuarantineTimeStartBase;
+    c->hquarantine_delay = P.HQuarantineDelay;
+    c->hquarantine_policy_duration = P.HQuarantinePolicyDuration;
+    c->hquarantine_prop_house_compliant = P.HQuarantinePropHouseCompliant;
+    c->hquarantine_prop_indiv_compliant = P.HQuarantinePropIndivCompliant;
+    c->hhquar_cell_inc_thresh = P.HHQuar_CellIncThresh;
+    c->do_hq_retrigger = P.DoHQretrigger;
+    c->do_place_close_once_only = P.DoPlaceCloseOnceOnly;
+    c->place_close_time_start_base = P.PlaceCloseTimeStartBase;
+    c->place_close_delay_mean = P.PlaceCloseDelayMean;
+    c->place_close_duration = P.PlaceCloseDurationBase;
+    c->place_close_cell_inc_thresh = P.PlaceCloseCellIncThresh1;
+    c->place_close_cell_inc_stop_thresh = P.PlaceCloseCellIncStopThresh;
+    c->place_close_inc_trig = P.PlaceCloseIncTrig1;
+    c->place_close_by_admin_unit = P.PlaceCloseByAdminUnit;
+    c->place_close_frac_inc_trig = P.PlaceCloseFracIncTrig;
+    for (int i = 0; i < MAX_NUM_PLACE_TYPES; i++) c->place_close_prop_attending[i] = P.PlaceClosePropAttending[i];
+    c->treat_time_start_base = P.TreatTimeStartBase;
+    c->vacc_time_start_base = P.VaccTimeStartBase;
+    c->soc_dist_time_start_base = P.SocDistTimeStartBase;
+    c->move_restr_time_start_base = P.MoveRestrTimeStartBase;
+    c->key_worker_proph_time_start_base = P.KeyWorkerProphTimeStartBase;
+    c->airport_close_time_start_base = P.AirportCloseTimeStartBase;
+    c->place_close_time_start_base2 = P.PlaceCloseTimeStartBase2;
+    c->place_close_indep_thresh = P.PlaceCloseIndepThresh;
+    c->vacc_time_start_geo = P.VaccTimeStartGeo;
+    c->do_intervention_delays_by_adunit = P.DoInterventionDelaysByAdUnit;
+    c->vary_efficacies_over_time = P.VaryEfficaciesOverTime;
+    c->do_mass_vacc = P.DoMassVacc;
+    // any intervention configured to start at all: the library then serves
+    // the subset above or refuses the run
+    const double never = 10000.0;
+    c->interventions_active = (P.CaseIsolationTimeStartBase < never || P.HQuarantineTimeStartBase < never ||
+                               P.PlaceCloseTimeStartBase < never || P.TreatTimeStartBase < never ||
+                               P.VaccTimeStartBase < never || P.SocDistTimeStartBase < never ||
+                               P.MoveRestrTimeStartBase < never || P.KeyWorkerProphTimeStartBase < never ||
+                               P.AirportCloseTimeStartBase < never || P.VaccTimeStartGeo < never || P.DoMassVacc) ? 1 : 0;
 }
 
 // InfStat code of a person, read through the public predicates

@@ -116,7 +116,8 @@ typedef struct cs_params {
     int    do_real_sympt_withdrawal, absenteeism_place_closure;
     int    no_infectiousness_sd_in_hh, do_death, do_ad_units, output_ad_unit_age;
     int    include_stepdown_to_death, do_digital_contact_tracing;
-    int    do_blanket_move_restr, interventions_active;  /* must be 0: interventions are outside this path */
+    int    do_blanket_move_restr;
+    int    interventions_active;       /* 1: an intervention is configured (the subset below is served, any other refused) */
     double place_type_trans[CS_MAX_PLACE_TYPES];
     double place_type_group_size_param1[CS_MAX_PLACE_TYPES];
     double place_type_prop_between_group_links[CS_MAX_PLACE_TYPES];
@@ -183,6 +184,42 @@ typedef struct cs_params {
     int    got_waifw_spatial;          /* P.Got_WAIFW_Matrix_Spatial */
     double initial_immunity[CS_NUM_AGE_GROUPS];
     double care_home_prop_residents;   /* P.CareHomePropResidents */
+    /* ---- the intervention subset, served when interventions_active != 0 ----
+     * Case isolation and household quarantine of detected cases
+     * (DoDetectedCase, src/Update.cpp:525-792), place closure by microcell on
+     * global triggers (TreatSweep, src/Sweep.cpp:1651-1740; DoPlaceClose and
+     * DoPlaceOpen, src/Update.cpp:1095-1295), and the alert, trigger and
+     * start-time logic of RecordSample and CalibrationThresholdCheck
+     * (src/CovidSim.cpp:3150-3160, :3485-3689).  Field names follow Param;
+     * times are in days, trigger windows in output samples.  cs_run_steps is
+     * the only entry point that runs them (the single-sweep entry points
+     * refuse an active intervention set: TreatSweep and the triggers run
+     * between sweeps, in RunModel). */
+    int    do_global_triggers, do_icu_triggers, do_admin_triggers, do_per_capita_triggers;
+    int    triggers_sampling_interval;          /* P.TriggersSamplingInterval */
+    int    trigger_alert_on_deaths;             /* P.TriggerAlertOnDeaths */
+    int    case_or_death_threshold_before_alert;
+    int    window_to_evaluate_trigger_alert;    /* P.WindowToEvaluateTriggerAlert */
+    int    do_no_calibration, do_alert_trigger_after_interv;
+    double date_trigger_reached_cal_time;       /* P.DateTriggerReached_CalTime (< 0: none) */
+    double pre_alert_control_prop_cases_id, post_alert_control_prop_cases_id;
+    double case_isolation_time_start_base, case_isolation_prop, case_isolation_policy_duration;
+    double case_isolation_cell_inc_thresh;      /* P.CaseIsolation_CellIncThresh */
+    double hquarantine_time_start_base, hquarantine_delay, hquarantine_policy_duration;
+    double hquarantine_prop_house_compliant, hquarantine_prop_indiv_compliant;
+    double hhquar_cell_inc_thresh;              /* P.HHQuar_CellIncThresh */
+    int    do_hq_retrigger, do_place_close_once_only;
+    double place_close_time_start_base, place_close_delay_mean, place_close_duration;  /* duration: P.PlaceCloseDurationBase */
+    int    place_close_cell_inc_thresh;         /* P.PlaceCloseCellIncThresh1 */
+    int    place_close_cell_inc_stop_thresh, place_close_inc_trig, place_close_by_admin_unit;
+    double place_close_frac_inc_trig;
+    double place_close_prop_attending[CS_MAX_PLACE_TYPES];
+    /* interventions outside the subset: each must be off (start times of
+     * 10000 days or more, switches 0); cs_run_steps refuses anything else */
+    double treat_time_start_base, vacc_time_start_base, soc_dist_time_start_base, move_restr_time_start_base;
+    double key_worker_proph_time_start_base, airport_close_time_start_base, place_close_time_start_base2;
+    double place_close_indep_thresh, vacc_time_start_geo;
+    int    do_intervention_delays_by_adunit, vary_efficacies_over_time, do_mass_vacc, pad5;
 } cs_params;
 
 /*
@@ -300,6 +337,11 @@ typedef struct cs_counters {
     int64_t sev_cum_age[CS_NUM_SEV_KINDS][CS_NUM_AGE_GROUPS];
     /* population totals after the step */
     int64_t S, L, I, R, D;
+    /* intervention counters (StateT.cumHQ, cumAH, cumAPC, cumAPA, cumAPCS):
+     * households quarantined, adults absent for a quarantined household,
+     * adults and children absent from closed places, adults absent to look
+     * after children of closed places */
+    int64_t cumHQ, cumAH, cumAPC, cumAPA, cumAPCS;
 } cs_counters;
 
 /* Event-count tallies used for the algorithmic-bytes roofline figure. */
@@ -308,6 +350,9 @@ typedef struct cs_work_counts {
     int64_t n_sp_draw, n_sp_susc, n_new, n_active, n_transition, n_cells;
     int64_t n_sp_scan;   /* cum_trans entries read by the destination-cell scans */
     int64_t n_sp_probe;  /* cumulative-infectiousness entries read by the infector searches */
+    int64_t n_hh_scan;   /* household members read by the household scans: a household whose scan found no
+                            susceptible member is not scanned again for that infector (no one returns to
+                            susceptible), so this is at most n_hh_member */
 } cs_work_counts;
 
 typedef struct cs_engine cs_engine;
@@ -404,6 +449,20 @@ int  cs_autotune(cs_engine* e, double t, int reps, cs_tuning* out);
 
 /* results */
 int  cs_get_counters(cs_engine* e, int real, cs_counters* out, int reset);
+/* the intervention state of realisation real (the P and AdUnits fields that
+ * CalibrationThresholdCheck sets, src/CovidSim.cpp:3485-3689, one set per
+ * realisation here): whether the alert has been reached, the start times of
+ * case isolation, household quarantine and place closure (days; 1e10: not
+ * set) and the trigger value of the last sample (State.trigDetectedCases) */
+typedef struct cs_intervention_state {
+    int32_t alert;                     /* the alert condition has held at a sample */
+    int32_t trigger;                   /* State.trigDetectedCases of the last sample */
+    double  control_prop_cases_id;     /* P.ControlPropCasesId now */
+    double  case_isolation_start, hquarantine_start, place_close_start;
+    int32_t mcells_closed;             /* microcells whose places are closed now (TreatStat::Treated) */
+    int32_t pad;
+} cs_intervention_state;
+int  cs_get_intervention_state(cs_engine* e, int real, cs_intervention_state* out);
 /* series [n_real][n_samples][CS_TS_FIELDS] recorded by cs_run_steps since the
  * series last started; out holds max_samples per realisation.  Per sample
  * (RecordSample, src/CovidSim.cpp:3024-3483):

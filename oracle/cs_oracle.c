@@ -196,7 +196,8 @@ static double fx_to_double(fx128 a) { return (double)a.hi + (double)a.lo * 5.421
 
 /* pass identifiers (high half of counter word 0) */
 enum { PS_HH = 1, PS_PW_SAMPLE, PS_PW_ACCEPT, PS_PB_SAMPLE, PS_PB_ACCEPT, PS_SP_COUNT,
-       PS_SP_DRAW, PS_INFECT, PS_INCUB, PS_CASE, PS_RECOVER, PS_HOLIDAY, PS_INIT, PS_SEED, PS_ITC };
+       PS_SP_DRAW, PS_INFECT, PS_INCUB, PS_CASE, PS_RECOVER, PS_HOLIDAY, PS_INIT, PS_SEED, PS_ITC,
+       PS_HQ, PS_ISO, PS_PLACE };
 /* pass identifiers inside the event order key */
 enum { KP_HH = 0, KP_PW = 1, KP_PB = 6, KP_SP = 11 };
 
@@ -379,6 +380,7 @@ typedef struct {
     uint64_t seed;
     int8_t* inf;
     uint8_t *sev_cur, *sev_final, *bits, *qc, *dct;
+    uint8_t* hh_done; /* the person's household pass found no susceptible member (work count only) */
     int32_t *listpos, *infector;
     int16_t* itype;
     float *infness, *susc, *prob_care;
@@ -399,7 +401,22 @@ typedef struct {
     double local_beta;
     int n_seed;            /* per-realisation seed count, or -1 */
     double loc_init[2];
+    /* interventions (one copy of the P / AdUnits fields the trigger logic
+     * sets, src/CovidSim.cpp:3485-3689, per realisation) */
+    int alert;             /* the alert condition has held at a sample */
+    int32_t trig;          /* State.trigDetectedCases of the last sample */
+    double ctrl;           /* P.ControlPropCasesId */
+    double ci_start, hq_start, pc_start, pc_prev;
+    int32_t* hist;         /* [OR_TRIG_RING][3]: cumDC, cumCritical, D at sample m (slot m % OR_TRIG_RING) */
+    uint8_t* mc_state;     /* [NM] Microcell.placeclose: 0 untreated, 1 treated, 2 don't treat again */
+    uint16_t* mc_end;      /* [NM] Microcell.place_end_time */
+    uint8_t* pl_once[CS_MAX_PLACE_TYPES]; /* Place.control_trig at USHRT_MAX - 1 (closed under once-only) */
 } or_real;
+
+/* samples of trigger history kept per realisation: a trigger window
+ * (TriggersSamplingInterval, WindowToEvaluateTriggerAlert) that reaches back
+ * further than this, within the run, is refused */
+#define OR_TRIG_RING 256
 
 struct or_engine {
     char err[256];
@@ -423,6 +440,7 @@ struct or_engine {
         *pl_mem[CS_MAX_PLACE_TYPES], *pl_gs0[CS_MAX_PLACE_TYPES], *pl_gst[CS_MAX_PLACE_TYPES],
         *pl_gsz[CS_MAX_PLACE_TYPES];
     float* max_trans;
+    int32_t *mpl_start, *mpl; /* places by microcell (Microcell.places): type << 27 | place */
     int nreal;
     or_real* R;
     /* scratch */
@@ -795,6 +813,20 @@ int or_create(or_engine** out, const cs_params* p, const double* kl, const doubl
         for (int k = 0; k < np; k++) if (e->pl_n[t][k] > maxplace) maxplace = e->pl_n[t][k];
     }
     e->samp = (int32_t*)malloc(sizeof(int32_t) * (size_t)maxplace);
+    /* the places of each microcell (Microcell.places), by type then index */
+    e->mpl_start = (int32_t*)calloc((size_t)e->NM + 1, 4);
+    {
+        int tot = 0;
+        for (int t = 0; t < e->NPT; t++)
+            for (int k = 0; k < e->np[t]; k++) { e->mpl_start[e->pl_mcell[t][k] + 1]++; tot++; }
+        for (int m = 0; m < e->NM; m++) e->mpl_start[m + 1] += e->mpl_start[m];
+        e->mpl = (int32_t*)malloc(sizeof(int32_t) * (size_t)(tot > 0 ? tot : 1));
+        int32_t* cur = (int32_t*)malloc((size_t)e->NM * 4 + 4);
+        memcpy(cur, e->mpl_start, (size_t)e->NM * 4);
+        for (int t = 0; t < e->NPT; t++)
+            for (int k = 0; k < e->np[t]; k++) e->mpl[cur[e->pl_mcell[t][k]]++] = (t << 27) | k;
+        free(cur);
+    }
     int maxcell = 1;
     for (int j = 0; j < NC; j++) if (e->cn[j] > maxcell) maxcell = e->cn[j];
     e->cell_inf = (float*)malloc(sizeof(float) * (size_t)maxcell);
@@ -809,12 +841,13 @@ int or_create(or_engine** out, const cs_params* p, const double* kl, const doubl
 
 static void free_real(or_engine* e, or_real* r)
 {
-    void* ptrs[] = {r->inf, r->sev_cur, r->sev_final, r->bits, r->qc, r->dct, r->listpos, r->infector, r->itype,
+    void* ptrs[] = {r->inf, r->sev_cur, r->sev_final, r->bits, r->qc, r->dct, r->hh_done, r->listpos, r->infector, r->itype,
         r->infness, r->susc, r->prob_care, r->t_inf, r->t_lat, r->t_rod, r->t_sari, r->t_crit, r->t_step, r->t_det,
         r->t_as, r->t_ae, r->t_ts, r->t_te, r->t_vs, r->t_is, r->t_qs, r->cS, r->cL, r->cI, r->cR, r->cD, r->cumTC,
         r->S0, r->tot_prob, r->cum_trans, r->invcdf, r->perm, r->m_socdist, r->m_moverest, r->m_trig};
     for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) free(ptrs[i]);
-    for (int t = 0; t < CS_MAX_PLACE_TYPES; t++) { free(r->pl_cs[t]); free(r->pl_ce[t]); }
+    for (int t = 0; t < CS_MAX_PLACE_TYPES; t++) { free(r->pl_cs[t]); free(r->pl_ce[t]); free(r->pl_once[t]); }
+    free(r->hist); free(r->mc_state); free(r->mc_end);
     (void)e;
 }
 
@@ -831,6 +864,7 @@ void or_destroy(or_engine* e)
         free(e->pl_gs0[t]); free(e->pl_gst[t]); free(e->pl_gsz[t]);
     }
     free(e->samp); free(e->cell_inf); free(e->max_trans); free(e->ev); free(e->ts_buf); free(e->ts_last);
+    free(e->mpl_start); free(e->mpl);
     free(e);
 }
 
@@ -849,7 +883,7 @@ int or_alloc_realisations(or_engine* e, int n_real, const uint64_t* seeds)
         r->n_seed = -1;
 #define AL(f, n, T) r->f = (T*)calloc((size_t)(n), sizeof(T))
         AL(inf, N, int8_t); AL(sev_cur, N, uint8_t); AL(sev_final, N, uint8_t); AL(bits, N, uint8_t);
-        AL(qc, N, uint8_t); AL(dct, N, uint8_t); AL(listpos, N, int32_t); AL(infector, N, int32_t);
+        AL(qc, N, uint8_t); AL(dct, N, uint8_t); AL(hh_done, N, uint8_t); AL(listpos, N, int32_t); AL(infector, N, int32_t);
         AL(itype, N, int16_t); AL(infness, N, float); AL(susc, N, float); AL(prob_care, N, float);
         AL(t_inf, N, uint16_t); AL(t_lat, N, uint16_t); AL(t_rod, N, uint16_t); AL(t_sari, N, uint16_t);
         AL(t_crit, N, uint16_t); AL(t_step, N, uint16_t); AL(t_det, N, uint16_t); AL(t_as, N, uint16_t);
@@ -858,7 +892,8 @@ int or_alloc_realisations(or_engine* e, int n_real, const uint64_t* seeds)
         AL(cS, NC, int32_t); AL(cL, NC, int32_t); AL(cI, NC, int32_t); AL(cR, NC, int32_t); AL(cD, NC, int32_t);
         AL(cumTC, NC, int32_t); AL(S0, NC, int32_t); AL(tot_prob, NC, float);
         AL(cum_trans, (size_t)NC * NC, float); AL(invcdf, (size_t)NC * 1025, int32_t); AL(perm, N, int32_t);
-        for (int t = 0; t < e->NPT; t++) { AL(pl_cs[t], e->np[t], uint16_t); AL(pl_ce[t], e->np[t], uint16_t); }
+        for (int t = 0; t < e->NPT; t++) { AL(pl_cs[t], e->np[t], uint16_t); AL(pl_ce[t], e->np[t], uint16_t); AL(pl_once[t], e->np[t], uint8_t); }
+        AL(hist, (size_t)OR_TRIG_RING * 3, int32_t); AL(mc_state, e->NM, uint8_t); AL(mc_end, e->NM, uint16_t);
         AL(m_socdist, e->NM, uint8_t); AL(m_moverest, e->NM, uint8_t); AL(m_trig, (size_t)e->NM * 6, uint32_t);
 #undef AL
     }
@@ -1104,7 +1139,7 @@ static void resolve(or_engine* e, or_real* r, double t, unsigned ts)
     free(newl);
 }
 
-int or_infect_sweep(or_engine* e, double t)
+static int infect_sweep(or_engine* e, double t)
 {
     const cs_params* P = &e->P;
     if (P->false_positive_rate != 0.0) return fail(e, CS_EUNSUPPORTED, "false positives are outside this path");
@@ -1141,8 +1176,12 @@ int or_infect_sweep(or_engine* e, double t)
                             }
                         if (absent) HI *= P->efficacies[2][CS_HOUSE];
                         e->wc.n_hh_inf++;
+                        e->wc.n_hh_member += nh;
+                        int any_susc = 0;
+                        for (int m = f0; m < f0 + nh; m++) any_susc |= is_susceptible(r, m);
+                        if (!r->hh_done[i]) e->wc.n_hh_scan += nh;
+                        if (!any_susc) r->hh_done[i] = 1;
                         for (int m = f0; m < f0 + nh; m++) {
-                            e->wc.n_hh_member++;
                             if (is_susceptible(r, m)) {
                                 double foi = HI * calc_house_susc(e, r, m, ts, i);
                                 rs s;
@@ -1287,6 +1326,12 @@ int or_infect_sweep(or_engine* e, double t)
     return e->error;
 }
 
+int or_infect_sweep(or_engine* e, double t)
+{
+    if (e->P.interventions_active) return fail(e, CS_EUNSUPPORTED, "interventions run only in the day loop (or_run_steps)");
+    return infect_sweep(e, t);
+}
+
 /* ChooseFinalDiseaseSeverity (src/Update.cpp:1377-1389) */
 static int choose_final_severity(const cs_params* P, int ag, double x)
 {
@@ -1368,7 +1413,9 @@ static void do_incub(or_engine* e, or_real* r, int p, unsigned ts)
         }
 #undef CH
     }
-    if (r->inf[p] == CS_INF_ALMOST_SYMPT && ((P->control_prop_cases_id == 1) || (rs_next(&s) < P->control_prop_cases_id))) {
+    /* P.ControlPropCasesId: the realisation's own, switched at the alert, when interventions run */
+    const double ctrl = P->interventions_active ? r->ctrl : P->control_prop_cases_id;
+    if (r->inf[p] == CS_INF_ALMOST_SYMPT && ((ctrl == 1) || (rs_next(&s) < ctrl))) {
         r->bits[p] |= 2;
         r->t_det[p] = (uint16_t)(ts + (uint16_t)(P->latent_to_sympt_delay * tspd));
     }
@@ -1501,7 +1548,300 @@ static int infectious_timers(or_engine* e, or_real* r, int p, unsigned ts, int* 
     return leave;
 }
 
-int or_incub_recovery_sweep(or_engine* e, double t)
+
+/* ------------------------------------------------------ interventions */
+
+static void totals(const or_engine* e, const or_real* r, int64_t* S, int64_t* L, int64_t* I, int64_t* R, int64_t* D);
+
+/* The intervention subset served: case isolation and household quarantine
+ * of detected cases, and place closure by microcell, all on global
+ * (detected-case or ICU) triggers, with the alert logic of
+ * CalibrationThresholdCheck when no calibration is asked for.  NULL when p
+ * lies inside it (or configures no intervention), else the reason. */
+const char* or_interventions_unsupported(const cs_params* p)
+{
+    const double never = 10000.0;
+    if (p->false_positive_rate != 0.0) return "false-positive infections are outside this path";
+    if (p->do_digital_contact_tracing) return "digital contact tracing is outside this path";
+    if (p->do_blanket_move_restr) return "movement restrictions are outside this path";
+    if (p->absenteeism_place_closure) return "absenteeism place closure is outside this path";
+    if (!p->interventions_active) return NULL;
+    if (!p->do_global_triggers || p->do_admin_triggers) return "interventions need global triggers (admin-unit and microcell triggers are outside this path)";
+    if (p->do_per_capita_triggers) return "per-capita triggers are outside this path";
+    if (p->do_no_calibration || p->do_alert_trigger_after_interv || p->date_trigger_reached_cal_time >= 0)
+        return "calibration to a trigger date is outside this path";
+    if (p->place_close_by_admin_unit || p->do_intervention_delays_by_adunit) return "admin-unit place closure and delays are outside this path";
+    if (p->vary_efficacies_over_time) return "efficacies varying over time are outside this path";
+    if (p->do_mass_vacc || p->treat_time_start_base < never || p->vacc_time_start_base < never || p->vacc_time_start_geo < never ||
+        p->soc_dist_time_start_base < never || p->move_restr_time_start_base < never || p->key_worker_proph_time_start_base < never ||
+        p->airport_close_time_start_base < never)
+        return "treatment, vaccination, social distancing, movement restrictions, key-worker prophylaxis and airport closure are outside this path";
+    if (p->place_close_time_start_base2 < never) return "a second place closure is outside this path";
+    if (p->place_close_indep_thresh > 0) return "the independent place-closure threshold is outside this path";
+    if (p->place_close_cell_inc_stop_thresh != 0) return "reopening places on a stop threshold is outside this path";
+    if (p->triggers_sampling_interval < 1 || p->window_to_evaluate_trigger_alert < 1) return "trigger windows must be at least one sample";
+    return NULL;
+}
+
+/* RecordSample's trigger value (src/CovidSim.cpp:3150-3160) and
+ * CalibrationThresholdCheck's alert and start times
+ * (src/CovidSim.cpp:3485-3689) for the realisation's sample m at time t */
+static void itv_triggers(or_engine* e, or_real* r, int m, double t)
+{
+    const cs_params* P = &e->P;
+    int64_t S, L, I, R, D;
+    totals(e, r, &S, &L, &I, &R, &D);
+    int32_t* h = r->hist + (size_t)(m % OR_TRIG_RING) * 3;
+    h[0] = (int32_t)r->cnt.cumDC;
+    h[1] = (int32_t)r->cnt.sev_cum[CS_SK_CRITICAL];
+    h[2] = (int32_t)D;
+    const int tsi = P->triggers_sampling_interval, win = P->window_to_evaluate_trigger_alert;
+    if ((m >= tsi && tsi >= OR_TRIG_RING) || (m >= win && win >= OR_TRIG_RING)) {
+        e->error = CS_EUNSUPPORTED;
+        snprintf(e->err, sizeof(e->err), "a trigger window reaches back more than %d samples", OR_TRIG_RING - 1);
+        return;
+    }
+    const int iv = P->do_icu_triggers ? 1 : 0;
+    int32_t trig = h[iv];
+    if (m >= tsi) trig -= r->hist[(size_t)((m - tsi) % OR_TRIG_RING) * 3 + iv];
+    r->trig = trig;
+    int32_t alert_cases = h[0];
+    if (m >= win) alert_cases -= r->hist[(size_t)((m - win) % OR_TRIG_RING) * 3];
+    int32_t alert = alert_cases;
+    if (P->trigger_alert_on_deaths) {
+        alert = h[2];
+        if (m >= win) alert -= r->hist[(size_t)((m - win) % OR_TRIG_RING) * 3 + 2];
+    }
+    if (alert >= P->case_or_death_threshold_before_alert) {
+        r->alert = 1;
+        r->ctrl = P->post_alert_control_prop_cases_id;
+        /* DO_OR_DONT_AMEND_START_TIME: a start time is set once */
+        if (trig > P->case_isolation_cell_inc_thresh && r->ci_start >= 1e10) r->ci_start = t + P->case_isolation_time_start_base;
+        if (trig > P->hhquar_cell_inc_thresh && r->hq_start >= 1e10) r->hq_start = t + P->hquarantine_time_start_base;
+        if (trig >= P->place_close_cell_inc_thresh && r->pc_start >= 1e10) r->pc_start = t + P->place_close_time_start_base;
+    }
+    /* the first closure ends after its duration when nothing re-triggers it */
+    if (t == r->pc_start + P->place_close_duration) {
+        r->pc_prev = r->pc_start;
+        if (P->place_close_inc_trig == 0 && P->place_close_frac_inc_trig == 0 && P->place_close_cell_inc_thresh == 0) r->pc_start = 9e9;
+    }
+}
+
+static int host_to_be_quarantined(const or_engine* e, const or_real* r, int x, unsigned ts)
+{
+    return ((int)r->t_qs[x] + (int)e->P.us_hquarantine_house_duration > (int)ts) && (r->qc[x] < 2);
+}
+
+/* the household-member loop of a carer search (src/Update.cpp:836-868,
+ * :752-776, :1227-1246): is some member alive, adult and free (no link to the
+ * place type PlaceTypeNoAirNum, or absent, or quarantined)?  `with_absent`
+ * selects whether absence counts, as the three call sites differ */
+static int itv_household_has_free_adult(const or_engine* e, const or_real* r, int p, unsigned ts, int with_absent)
+{
+    const cs_params* P = &e->P;
+    const int h = e->hh[p], j1 = e->hfirst[h], j2 = j1 + e->hnh[h];
+    for (int j = j1; j < j2; j++)
+        if (!is_dead(r, j) && ((int)e->age[j] >= P->case_absent_child_age_cutoff) &&
+            ((e->plinks[(size_t)j * CS_MAX_PLACE_TYPES + P->place_type_no_air_num - 1] < 0) ||
+             (with_absent && host_absent(r, j, ts)) || host_quarantined(e, r, j, ts)))
+            return 1;
+    return 0;
+}
+static int itv_first_adult(const or_engine* e, const or_real* r, int p)
+{
+    const int h = e->hh[p], j1 = e->hfirst[h], j2 = j1 + e->hnh[h];
+    for (int j = j1; j < j2; j++)
+        if (((int)e->age[j] >= e->P.case_absent_child_age_cutoff) && !is_dead(r, j)) return j;
+    return -1;
+}
+
+/* the detected-case onset test: p became a detected case at this step */
+static int itv_detected_now(const or_engine* e, const or_real* r, int p, unsigned ts)
+{
+    const uint16_t ct = (uint16_t)(r->t_lat[p] + ((int)(e->P.latent_to_sympt_delay / e->P.model_time_step)));
+    return e->P.do_symptoms && ct == ts && DETECTED(r, p);
+}
+
+/* household quarantine of DoDetectedCase (src/Update.cpp:612-639) for
+ * detected case p.  One application per household per step: the draws are
+ * keyed by the household (its first member), so the household's first
+ * detected case of the step (lowest index) applies it, with the result any
+ * other order would give. */
+static void itv_quarantine(or_engine* e, or_real* r, int p, double t, unsigned ts)
+{
+    const cs_params* P = &e->P;
+    if (!P->do_households) return;
+    if (!(t >= r->hq_start && t < r->hq_start + P->hquarantine_policy_duration)) return;
+    const int h = e->hh[p], j1 = e->hfirst[h], j2 = j1 + e->hnh[h];
+    for (int j = j1; j < p; j++)
+        if (itv_detected_now(e, r, j, ts)) return; /* an earlier member applies it */
+    if (host_to_be_quarantined(e, r, j1, ts) && !P->do_hq_retrigger) return;
+    const uint16_t start = (uint16_t)(ts + (uint16_t)(P->time_steps_per_day * P->hquarantine_delay));
+    rs s;
+    rs_init(&s, r->seed, cw0(ts, PS_HQ), (uint32_t)j1, 0);
+    const int k = (rs_next(&s) < P->hquarantine_prop_house_compliant) ? 1 : 0;
+    if (k) r->cnt.cumHQ++;
+    for (int j = j1; j < j2; j++) {
+        r->t_qs[j] = start;
+        r->qc[j] = (uint8_t)((k == 0) ? 0 : ((rs_next(&s) < P->hquarantine_prop_indiv_compliant) ? 1 : 0));
+        if (r->qc[j] && !host_absent(r, j, ts)) {
+            if ((int)e->age[j] >= P->case_absent_child_age_cutoff) {
+                if (e->plinks[(size_t)j * CS_MAX_PLACE_TYPES + P->place_type_no_air_num - 1] >= 0) r->cnt.cumAH++;
+            } else
+                r->cnt.cumACS++;
+        }
+    }
+}
+
+/* case isolation of DoDetectedCase (src/Update.cpp:641-692) for detected
+ * case p; returns 1 when an adult carer is requested (applied with the
+ * child-care requests of the step, isolation form) */
+static int itv_isolate(or_engine* e, or_real* r, int p, double t, unsigned ts)
+{
+    const cs_params* P = &e->P;
+    if (!(t >= r->ci_start && t < r->ci_start + P->case_isolation_policy_duration)) return 0;
+    rs s;
+    rs_init(&s, r->seed, cw0(ts, PS_ISO), (uint32_t)p, 0);
+    if (!((P->case_isolation_prop == 1) || (rs_next(&s) < P->case_isolation_prop))) return 0;
+    r->t_is[p] = (uint16_t)ts;
+    if (host_absent(r, p, ts)) {
+        const int stop = (int)ts + P->us_case_absenteeism_delay + P->us_case_isolation_duration;
+        if ((int)r->t_ae[p] < stop) r->t_ae[p] = (uint16_t)stop;
+        return 0;
+    }
+    if (!P->do_real_sympt_withdrawal) return 0;
+    r->t_as[p] = (uint16_t)(ts + P->us_case_isolation_delay);
+    r->t_ae[p] = (uint16_t)(ts + P->us_case_isolation_delay + P->us_case_isolation_duration);
+    const int child = (int)e->age[p] < P->case_absent_child_age_cutoff;
+    if (P->do_places && !host_quarantined(e, r, p, ts) && e->plinks[(size_t)p * CS_MAX_PLACE_TYPES + P->place_type_no_air_num - 1] >= 0 && !child)
+        r->cnt.cumAC++;
+    if (P->do_households && P->do_places && child) {
+        if (!host_quarantined(e, r, p, ts)) r->cnt.cumACS++;
+        if (r->prob_care[p] < P->case_absent_child_prop_adult_carers) return 1;
+    }
+    return 0;
+}
+
+/* Place.ProbClose (src/CovidSim.cpp:1120) and Person.ProbAbsent
+ * (src/CovidSim.cpp:1017), drawn at the realisation's reset from their own
+ * streams and recomputed where read */
+static float itv_prob_close(const or_real* r, int pt, int k)
+{
+    rs s;
+    rs_init(&s, r->seed, cw0(0, PS_PLACE), (uint32_t)k, (uint32_t)pt);
+    return (float)rs_next(&s);
+}
+static float itv_prob_absent(const or_real* r, int p)
+{
+    rs s;
+    rs_init(&s, r->seed, cw0(0, PS_INIT), (uint32_t)p, 0);
+    return (float)rs_next(&s);
+}
+
+/* DoPlaceClose with DoAnyway (src/Update.cpp:1095-1190): the place's
+ * closure window; 1 when the place closes (its members' absences follow) */
+static int itv_close_place(or_engine* e, or_real* r, int pt, int k, unsigned ts, uint16_t t_start, uint16_t t_stop)
+{
+    const cs_params* P = &e->P;
+    if (!(r->pl_ce[pt][k] < t_stop) || r->pl_once[pt][k]) return 0;
+    if (P->do_place_close_once_only) r->pl_once[pt][k] = 1;
+    (void)ts;
+    if ((double)itv_prob_close(r, pt, k) >= P->efficacies[2][pt]) {
+        if (r->pl_cs[pt][k] > t_start) r->pl_cs[pt][k] = t_start;
+        r->pl_ce[pt][k] = t_stop;
+        return 1;
+    }
+    r->pl_cs[pt][k] = r->pl_ce[pt][k] = t_stop;
+    return 0;
+}
+
+/* TreatSweep's place-closure block on global triggers
+ * (src/Sweep.cpp:1422-1437, :1651-1740) after the step's IncubRecoverySweep,
+ * then the members' absences (src/Update.cpp:1192-1260) and the carers'
+ * (UpdateHostClosure, src/Update.cpp:1263-1279).  Every member test of
+ * every closing place reads the state before any absence of this sweep is
+ * written. */
+static void itv_treat_sweep(or_engine* e, or_real* r, double t, unsigned ts)
+{
+    const cs_params* P = &e->P;
+    if (!(t >= r->pc_start)) return;
+    const uint16_t t_end = (uint16_t)ceil(P->time_steps_per_day * (t + P->place_close_delay_mean + P->place_close_duration));
+    const uint16_t t_start = (uint16_t)(ts + (uint16_t)(P->time_steps_per_day * P->place_close_delay_mean));
+    const uint16_t t_stop = (uint16_t)(ts + (uint16_t)(P->time_steps_per_day * (P->place_close_delay_mean + P->place_close_duration)));
+    int32_t* closed = NULL;
+    int nclosed = 0, cap = 0;
+    for (int m = 0; m < e->NM; m++) {
+        if (e->mstart[m + 1] == e->mstart[m]) continue; /* unpopulated */
+        if (r->mc_state[m] == 1 && ts >= r->mc_end[m]) {
+            r->mc_state[m] = P->do_place_close_once_only ? 2 : 0;
+            r->mc_end[m] = (uint16_t)ts;
+        }
+        if (P->do_places && t >= r->pc_start && r->mc_state[m] == 0 && r->trig >= P->place_close_cell_inc_thresh) {
+            r->mc_end[m] = t_end;
+            r->mc_state[m] = 1;
+            for (int q = e->mpl_start[m]; q < e->mpl_start[m + 1]; q++) {
+                const int pt = e->mpl[q] >> 27, k = e->mpl[q] & ((1 << 27) - 1);
+                if (pt == P->hotel_place_type) continue;
+                if (itv_close_place(e, r, pt, k, ts, t_start, t_stop)) {
+                    if (nclosed == cap) { cap = cap ? 2 * cap : 64; closed = (int32_t*)realloc(closed, sizeof(int32_t) * (size_t)cap); }
+                    closed[nclosed++] = e->mpl[q];
+                }
+            }
+        }
+    }
+    if (!nclosed) { free(closed); return; }
+    if (P->do_real_sympt_withdrawal) {
+        int32_t* carers = (int32_t*)malloc(sizeof(int32_t) * (size_t)e->N);
+        int ncarer = 0;
+        for (int pass = 0; pass < 2; pass++) /* 0: tests and counters, 1: absences */
+            for (int a = 0; a < nclosed; a++) {
+                const int pt = closed[a] >> 27, k = closed[a] & ((1 << 27) - 1);
+                const double att = P->place_close_prop_attending[pt];
+                for (int q = e->pl_ms[pt][k]; q < e->pl_ms[pt][k + 1]; q++) {
+                    const int w = e->pl_mem[pt][q];
+                    if (!((att == 0) || ((double)itv_prob_absent(r, w) >= att))) continue;
+                    const int child = (int)e->age[w] < P->case_absent_child_age_cutoff;
+                    if (pass == 0) {
+                        if (!host_absent(r, w, ts) && !host_quarantined(e, r, w, ts) && child) {
+                            r->cnt.cumAPCS++;
+                            if (r->prob_care[w] < P->case_absent_child_prop_adult_carers && !itv_household_has_free_adult(e, r, w, ts, 0)) {
+                                const int c = itv_first_adult(e, r, w);
+                                if (c >= 0 && ncarer < e->N) { carers[ncarer++] = c; r->cnt.cumAPA++; }
+                            }
+                        }
+                        if (!child && e->plinks[(size_t)w * CS_MAX_PLACE_TYPES + P->place_type_no_air_num - 1] >= 0) r->cnt.cumAPC++;
+                    } else {
+                        if (r->t_as[w] > t_start) r->t_as[w] = t_start;
+                        if (r->t_ae[w] < t_stop) r->t_ae[w] = t_stop;
+                    }
+                }
+            }
+        for (int a = 0; a < ncarer; a++) {
+            const int c = carers[a];
+            if (r->t_as[c] > t_start) r->t_as[c] = t_start;
+            if (r->t_ae[c] < t_stop) r->t_ae[c] = t_stop;
+        }
+        free(carers);
+    }
+    free(closed);
+}
+
+int or_get_intervention_state(or_engine* e, int ri, cs_intervention_state* out)
+{
+    if (ri < 0 || ri >= e->nreal) return fail(e, CS_EINVAL, "bad realisation");
+    const or_real* r = &e->R[ri];
+    memset(out, 0, sizeof(*out));
+    out->alert = r->alert;
+    out->trigger = r->trig;
+    out->control_prop_cases_id = e->P.interventions_active ? r->ctrl : e->P.control_prop_cases_id;
+    out->case_isolation_start = r->ci_start;
+    out->hquarantine_start = r->hq_start;
+    out->place_close_start = r->pc_start;
+    for (int m = 0; m < e->NM; m++) out->mcells_closed += r->mc_state[m] == 1;
+    return 0;
+}
+
+static int incub_recovery_sweep(or_engine* e, double t)
 {
     const cs_params* P = &e->P;
     unsigned ts = (unsigned)(uint16_t)(P->time_steps_per_day * t);
@@ -1549,9 +1889,11 @@ int or_incub_recovery_sweep(or_engine* e, double t)
         }
         e->wc.n_transition += nl;
         batch_move(e, r, lst, nl, 1);
-        /* infectious timers (src/Sweep.cpp:929-977) */
-        int32_t* care = (int32_t*)malloc(sizeof(int32_t) * (size_t)e->N);
-        int nc = 0;
+        /* infectious timers (src/Sweep.cpp:929-977); a child-care request is
+         * p (DoCase's form) or ~p (case isolation's form) */
+        int32_t* care = (int32_t*)malloc(sizeof(int32_t) * 2 * (size_t)e->N);
+        int32_t* det = (int32_t*)malloc(sizeof(int32_t) * (size_t)e->N);
+        int nc = 0, nd = 0;
         nl = 0;
         for (int c = 0; c < e->NC; c++) {
             int base = e->cstart[c], lo = r->cS[c] + r->cL[c];
@@ -1560,14 +1902,22 @@ int or_incub_recovery_sweep(or_engine* e, double t)
                 int req = 0;
                 if (infectious_timers(e, r, p, ts, &req)) lst[nl++] = p;
                 if (req) care[nc++] = p;
+                if (P->interventions_active && itv_detected_now(e, r, p, ts)) det[nd++] = p;
             }
         }
+        /* DoDetectedCase's interventions (src/Update.cpp:525-792) for the
+         * step's detected cases: household quarantine for all of them, then
+         * case isolation, so that isolation sees every quarantine of the step */
+        for (int a = 0; a < nd; a++) itv_quarantine(e, r, det[a], t, ts);
+        for (int a = 0; a < nd; a++)
+            if (itv_isolate(e, r, det[a], t, ts)) care[nc++] = ~det[a];
+        free(det);
         /* child-care absence (src/Update.cpp:836-868): evaluated against the
          * state after every timer of the step, written after all requests */
         if (nc) {
             int32_t* kk = (int32_t*)malloc(sizeof(int32_t) * (size_t)nc);
             for (int a = 0; a < nc; a++) {
-                int p = care[a], h = e->hh[p], j1 = e->hfirst[h], j2 = j1 + e->hnh[h], f = 0;
+                int p = care[a] >= 0 ? care[a] : ~care[a], h = e->hh[p], j1 = e->hfirst[h], j2 = j1 + e->hnh[h], f = 0;
                 kk[a] = -1;
                 for (int j3 = j1; (j3 < j2) && !f; j3++)
                     f = (!is_dead(r, j3) && ((int)e->age[j3] >= P->case_absent_child_age_cutoff) &&
@@ -1585,7 +1935,8 @@ int or_incub_recovery_sweep(or_engine* e, double t)
             for (int a = 0; a < nc; a++) {
                 int k = kk[a];
                 if (k < 0) continue;
-                if (!was_absent[a]) r->t_as[k] = (uint16_t)(ts + P->us_case_isolation_delay);
+                /* DoCase's form keeps an absence already running; isolation's sets it anew */
+                if (!was_absent[a] || care[a] < 0) r->t_as[k] = (uint16_t)(ts + P->us_case_isolation_delay);
                 r->t_ae[k] = (uint16_t)(ts + P->us_case_isolation_delay + P->us_case_isolation_duration);
             }
             free(was_absent);
@@ -1596,6 +1947,12 @@ int or_incub_recovery_sweep(or_engine* e, double t)
         free(care);
     }
     return e->error;
+}
+
+int or_incub_recovery_sweep(or_engine* e, double t)
+{
+    if (e->P.interventions_active) return fail(e, CS_EUNSUPPORTED, "interventions run only in the day loop (or_run_steps)");
+    return incub_recovery_sweep(e, t);
 }
 
 /* ------------------------------------------------- realisation start */
@@ -1618,7 +1975,8 @@ int or_set_realisation_params(or_engine* e, const double* local_beta, const int3
 int or_reset_realisations(or_engine* e, int n_seed)
 {
     const cs_params* P = &e->P;
-    if (P->interventions_active) return fail(e, CS_EUNSUPPORTED, "interventions are outside this path");
+    const char* why = or_interventions_unsupported(P);
+    if (why) return fail(e, CS_EUNSUPPORTED, why);
     e->ts_fresh = 1;
     for (int ri = 0; ri < e->nreal; ri++) {
         or_real* r = &e->R[ri];
@@ -1631,6 +1989,7 @@ int or_reset_realisations(or_engine* e, int n_seed)
             r->t_lat[k] = r->t_rod[k] = 0; r->t_inf[k] = 0;
             r->infector[k] = -1; r->itype[k] = 0;
             r->qc[k] = 2; r->t_qs[k] = CS_UNSET_TIME;
+            r->hh_done[k] = 0;
             rs s;
             rs_init(&s, r->seed, cw0(0, PS_INIT), (uint32_t)k, 0);
             (void)rs_next(&s); /* ProbAbsent (unused on this path) */
@@ -1660,6 +2019,15 @@ int or_reset_realisations(or_engine* e, int n_seed)
         memset(r->m_moverest, 0, (size_t)e->NM);
         memset(r->m_trig, 0, (size_t)e->NM * 6 * 4);
         memset(&r->cnt, 0, sizeof(r->cnt));
+        /* intervention state (InitModel, src/CovidSim.cpp:1104, :1116-1122, :1216-1232) */
+        r->alert = 0;
+        r->trig = 0;
+        r->ctrl = P->pre_alert_control_prop_cases_id;
+        r->ci_start = r->hq_start = r->pc_start = r->pc_prev = 1e10;
+        memset(r->hist, 0, sizeof(int32_t) * OR_TRIG_RING * 3);
+        memset(r->mc_state, 0, (size_t)e->NM);
+        memset(r->mc_end, 0, (size_t)e->NM * 2);
+        for (int t = 0; t < e->NPT; t++) memset(r->pl_once[t], 0, (size_t)e->np[t]);
         update_probs_real(e, r);
         r->prev_prop_susc = 1.0;
         r->loc_init[0] = P->location_initial_infection[0];
@@ -1726,6 +2094,7 @@ int or_load_state(or_engine* e, int ri, const cs_state* s)
     }
     CP(m_socdist, s->mcell_socdist, (size_t)e->NM); CP(m_moverest, s->mcell_moverest, (size_t)e->NM);
 #undef CP
+    memset(r->hh_done, 0, N);
     r->loc_init[0] = e->P.location_initial_infection[0];
     r->loc_init[1] = e->P.location_initial_infection[1];
     int64_t S = sum_S(e, r);
@@ -1788,8 +2157,15 @@ int or_run_steps(or_engine* e, double t0, int n_steps, int sps)
             lt[0] = e->R[ri].cnt.cumI; lt[1] = e->R[ri].cnt.cumC; lt[2] = D;
             for (int k = 0; k < CS_NUM_SEV_KINDS; k++) lt[3 + k] = e->R[ri].cnt.sev_cum[k];
             e->R[ri].samp_prop_susc = (double)S / (double)e->N;
+            /* the first sample's CalibrationThresholdCheck (src/CovidSim.cpp:1414) */
+            if (e->P.interventions_active) itv_triggers(e, &e->R[ri], 0, t0);
         }
         e->ts_fresh = 0;
+        if (e->error) return e->error;
+    }
+    {
+        const char* why = or_interventions_unsupported(&e->P);
+        if (why) return fail(e, CS_EUNSUPPORTED, why);
     }
     if (e->ts_n + ns > e->ts_cap || !e->ts_buf) { /* grow, keeping the samples taken so far */
         int cap = e->ts_n + ns > 2 * e->ts_cap ? e->ts_n + ns : 2 * e->ts_cap;
@@ -1803,10 +2179,15 @@ int or_run_steps(or_engine* e, double t0, int n_steps, int sps)
         e->ts_cap = cap;
     }
     for (int st = 0; st < n_steps; st++) {
-        int rc = or_infect_sweep(e, t);
+        int rc = infect_sweep(e, t);
         if (rc) return rc;
-        rc = or_incub_recovery_sweep(e, t);
+        rc = incub_recovery_sweep(e, t);
         if (rc) return rc;
+        /* TreatSweep and UpdateHostClosure (src/CovidSim.cpp:1498-1503) */
+        if (e->P.interventions_active) {
+            const unsigned ts = (unsigned)(uint16_t)(e->P.time_steps_per_day * t);
+            for (int ri = 0; ri < e->nreal; ri++) itv_treat_sweep(e, &e->R[ri], t, ts);
+        }
         t += e->P.model_time_step;
         /* the S-drop test reads State.S, which only RecordSample refreshes
          * (src/CovidSim.cpp:1515, :3163), i.e. S as of the last sample */
@@ -1840,6 +2221,11 @@ int or_run_steps(or_engine* e, double t0, int n_steps, int sps)
                 memcpy(lt, now, sizeof(now));
             }
             e->ts_n++;
+            /* the next output step's RecordSample and CalibrationThresholdCheck
+             * (src/CovidSim.cpp:1413-1414): sample number ts_n of the series */
+            if (e->P.interventions_active)
+                for (int ri = 0; ri < e->nreal; ri++) itv_triggers(e, &e->R[ri], e->ts_n, t);
+            if (e->error) return e->error;
         }
     }
     return e->error;

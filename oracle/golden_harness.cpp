@@ -24,7 +24,12 @@
 //                       (itc_*.npy) and exit; the run does not continue
 //   CS_SERIES=<file>    append one record per step and realisation:
 //                       run, TimeStepNow, S, L, I, R, D, cumI, cumC, cumD,
-//                       cumItype[16] (int32 each), read after IncubRecoverySweep
+//                       cumItype[16], then the start times (in timesteps,
+//                       -1 while unset) of place closure (P.PlaceCloseTimeStart),
+//                       case isolation and household quarantine (of admin
+//                       unit 0: with global triggers every unit has the same)
+//                       and State.trigDetectedCases (int32 each, 30 in all),
+//                       read after IncubRecoverySweep
 //
 // Output arrays are .npy files; cs_params is written as its raw bytes
 // (params.bin) with the layout of include/covidsim_hip.h.
@@ -430,7 +435,7 @@ void IncubRecoverySweep(double t)
 {
     IncubRecoverySweep_cpu(t);
     if (g_series) {
-        int32_t rec[10 + INFECT_TYPE_MASK];
+        int32_t rec[10 + INFECT_TYPE_MASK + 4];
         int S = 0, L = 0, I = 0, R = 0, D = 0;
         for (int j = 0; j < P.NumPopulatedCells; j++) {
             S += CellLookup[j]->S; L += CellLookup[j]->L; I += CellLookup[j]->I;
@@ -455,6 +460,11 @@ void IncubRecoverySweep(double t)
             itype_prev[q] = cur;
             rec[10 + q] = itype_acc[q];
         }
+        auto steps = [](double x) { return (int32_t)(x < 1e9 ? x * P.TimeStepsPerDay : -1); };
+        rec[10 + INFECT_TYPE_MASK] = steps(P.PlaceCloseTimeStart);
+        rec[11 + INFECT_TYPE_MASK] = P.NumAdunits > 0 ? steps(AdUnits[0].CaseIsolationTimeStart) : -1;
+        rec[12 + INFECT_TYPE_MASK] = P.NumAdunits > 0 ? steps(AdUnits[0].HQuarantineTimeStart) : -1;
+        rec[13 + INFECT_TYPE_MASK] = State.trigDetectedCases;
         fwrite(rec, sizeof(rec), 1, g_series);
         fflush(g_series);
     }

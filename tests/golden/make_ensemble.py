@@ -12,7 +12,7 @@ reduces them to daily series per realisation:
 
 The realisations differ only in their run seeds (the reference chains them
 with setall, src/CovidSim.cpp:356-369); the population is the same one the
-fixtures hold.  Usage:  python make_ensemble.py mini|mini_care|c1 200
+fixtures hold.  Usage:  python make_ensemble.py mini|mini_care|mini_int|c1 200
 """
 import os
 import subprocess
@@ -24,21 +24,27 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, HERE)
-from make_fixtures import care_pre_params, run_harness  # noqa: E402
+from make_fixtures import care_pre_params, int_param_files, run_harness  # noqa: E402
 
 
 def main(name, runs, threads=1, n_seeds=10):
-    # mini_care: the 20k-person region with care homes (make_fixtures.CARE_KEYS)
-    inputs = os.path.join(ROOT, "covid-sim_amd", "inputs", "mini" if name == "mini_care" else name)
+    # mini_care: the 20k-person region with care homes (make_fixtures.CARE_KEYS);
+    # mini_int: the same region with interventions (make_fixtures.INT_PRE, INT_PARAMS)
+    inputs = os.path.join(ROOT, "covid-sim_amd", "inputs", "mini" if name in ("mini_care", "mini_int") else name)
     tmp = tempfile.mkdtemp()
     series = os.path.join(tmp, "series.bin")
-    pre = care_pre_params(tmp) if name == "mini_care" else None
-    run_harness(inputs, tmp, steps="", golden=False, threads=threads, series=series, nr=runs, pre_params=pre)
-    rec = np.fromfile(series, dtype=np.int32).reshape(-1, 26)
+    pre, par = None, None
+    if name == "mini_care":
+        pre = care_pre_params(tmp)
+    elif name == "mini_int":
+        pre, par = int_param_files(tmp)
+    run_harness(inputs, tmp, steps="", golden=False, threads=threads, series=series, nr=runs, pre_params=pre, params=par)
+    rec = np.fromfile(series, dtype=np.int32).reshape(-1, 30)
     steps_per_day = 4
     ndays = (rec[:, 1].max() + 1) // steps_per_day
     out = np.zeros((runs, ndays, 8), np.int32)
     itype = np.zeros((runs, 16), np.int64)
+    starts = np.full((runs, 3), -1, np.int32)  # place closure, case isolation, household quarantine (timesteps)
     for r in range(runs):
         x = rec[rec[:, 0] == r]
         x = x[np.argsort(x[:, 1], kind="stable")]
@@ -48,6 +54,8 @@ def main(name, runs, threads=1, n_seeds=10):
         for row in x:
             last[int(row[1])] = row
         itype[r] = x[-1, 10:26] if len(x) else 0
+        if len(x):
+            starts[r] = x[-1, 26:29]
         prevI, prevC, prevD = n_seeds, 0, 0
         for d in range(ndays):
             row = last.get(steps_per_day * d + steps_per_day - 1)
@@ -59,7 +67,7 @@ def main(name, runs, threads=1, n_seeds=10):
             out[r, d, 6] = row[8] - prevC
             out[r, d, 7] = row[6] - prevD
             prevI, prevC, prevD = row[7], row[8], row[6]
-    np.savez_compressed(os.path.join(HERE, "%s_ensemble.npz" % name), daily=out, itype=itype)
+    np.savez_compressed(os.path.join(HERE, "%s_ensemble.npz" % name), daily=out, itype=itype, starts=starts)
     print(name, "runs", runs, "days", ndays, "mean attack", (out[:, -1, 3] + out[:, -1, 4]).mean() / out[0, -1, :5].sum())
 
 

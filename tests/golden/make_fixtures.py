@@ -77,15 +77,90 @@ def care_pre_params(out_dir):
     return dst
 
 
+# the intervention variant of the 20k-person fixture region (C2's kind of
+# configuration at this size): global triggers on cumulative ICU cases; an
+# outbreak alert at 5 deaths that raises case detection from 0.3 to 0.8;
+# case isolation of 70% of detected cases and household quarantine (half of
+# households comply) once ICU cases pass 5; closure of schools and
+# universities (a quarter of universities stay open, a fifth of their members
+# still attend) for 30-day periods once ICU cases reach 10.  Keys the
+# parameter files already hold are overwritten, the others appended.
+INT_PRE = {
+    "Use global triggers for interventions": "1",
+    "Use ICU case triggers for interventions": "1",
+    "Number of deaths accummulated before alert": "5",
+    "Proportion of cases detected before outbreak alert": "0.3",
+    "Proportion of cases detected for treatment": "0.8",
+}
+INT_PARAMS = {
+    "Case isolation start time": "0",
+    "Case isolation trigger incidence per cell": "5",
+    "Proportion of detected cases isolated": "0.7",
+    "Delay to start case isolation": "1",
+    "Duration of case isolation": "7",
+    "Duration of case isolation policy": "364",
+    "Residual contacts after case isolation": "0.25",
+    "Residual household contacts after case isolation": "0.5",
+    "Household quarantine start time": "0",
+    "Household quarantine trigger incidence per cell": "5",
+    "Delay to start household quarantine": "1",
+    "Length of time households are quarantined": "14",
+    "Duration of household quarantine policy": "364",
+    "Household level compliance with quarantine": "0.5",
+    "Individual level compliance with quarantine": "1.0",
+    "Place closure start time": "0",
+    "Trigger incidence per cell for place closure": "10",
+    "Delay to start place closure": "1",
+    "Duration of place closure": "30",
+    "Proportion of places remaining open after closure by place type": "0 0 0.25 1",
+    "Proportional attendance after closure by place type": "0 0 0.2 0",
+    "Relative household contact rate after closure": "1.5",
+    "Relative spatial contact rate after closure": "1.25",
+}
+
+
+def override_params(src, overrides, dst):
+    """src with the value line after each overridden key replaced; keys src
+    does not hold are appended"""
+    lines = open(src).read().split("\n")
+    left = dict(overrides)
+    out, i = [], 0
+    while i < len(lines):
+        line = lines[i]
+        out.append(line)
+        key = line.strip()[1:-1] if line.strip().startswith("[") and line.strip().endswith("]") else None
+        if key in left:
+            j = i + 1
+            while j < len(lines) and not lines[j].strip():
+                out.append(lines[j])
+                j += 1
+            out.append(left.pop(key))
+            i = j + 1
+            continue
+        i += 1
+    for k, v in left.items():
+        out += ["", "[%s]" % k, v]
+    with open(dst, "w") as f:
+        f.write("\n".join(out) + "\n")
+    return dst
+
+
+def int_param_files(out_dir):
+    """(pre-params, params) of the intervention variant, written to out_dir"""
+    mini = os.path.join(ROOT, "covid-sim_amd", "inputs", "mini")
+    return (override_params(os.path.join(mini, "pre-params.txt"), INT_PRE, os.path.join(out_dir, "int-pre-params.txt")),
+            override_params(os.path.join(mini, "input-noint-params.txt"), INT_PARAMS, os.path.join(out_dir, "int-params.txt")))
+
+
 def run_harness(inputs, out, steps="0", golden=True, threads=1, extra=(), series=None, nr=None, extra_env=None,
-                pre_params=None):
+                pre_params=None, params=None):
     env = dict(os.environ, CS_DUMP_DIR=out, CS_DUMP_STEPS=steps, CS_DUMP_GOLDEN="1" if golden else "0")
     env.update(extra_env or {})
     if series:
         env["CS_SERIES"] = series
     cmd = [HARNESS, "/c:%d" % threads,
            "/PP:" + (pre_params or os.path.join(inputs, "pre-params.txt")),
-           "/P:" + os.path.join(inputs, "input-noint-params.txt"),
+           "/P:" + (params or os.path.join(inputs, "input-noint-params.txt")),
            "/O:" + os.path.join(out, "out"),
            "/D:" + os.path.join(inputs, "density.txt"),
            "/A:" + os.path.join(inputs, "admin-params.txt")]
@@ -174,8 +249,28 @@ def care():
         shutil.rmtree(tmp)
 
 
+def interventions():
+    """mini_int_params.bin: cs_params of the intervention variant (the
+    population is mini's, checked here)"""
+    inputs = os.path.join(ROOT, "covid-sim_amd", "inputs", "mini")
+    base = np.load(os.path.join(HERE, "mini_fixture.npz"))
+    tmp = tempfile.mkdtemp()
+    try:
+        pre, par = int_param_files(tmp)
+        run_harness(inputs, tmp, steps="", golden=False, threads=1, pre_params=pre, params=par)
+        for fn in sorted(os.listdir(tmp)):
+            if fn.endswith(".npy") and not fn.startswith("kernel_"):
+                assert np.array_equal(np.load(os.path.join(tmp, fn)), base[fn[:-4]]), fn
+        shutil.copy(os.path.join(tmp, "params.bin"), os.path.join(HERE, "mini_int_params.bin"))
+        print("wrote mini_int_params.bin")
+    finally:
+        shutil.rmtree(tmp)
+
+
 if __name__ == "__main__":
-    if len(sys.argv) > 1 and sys.argv[1] == "care":
+    if len(sys.argv) > 1 and sys.argv[1] == "interventions":
+        interventions()
+    elif len(sys.argv) > 1 and sys.argv[1] == "care":
         care()
     elif len(sys.argv) > 1 and sys.argv[1] == "c1":
         # the 200k-person C1 population (ensemble and block-path parity tests)

@@ -117,6 +117,11 @@ class OracleEngine(Oracle):
         self.call("get_counters", int(r), C.byref(c), int(bool(reset)))
         return c
 
+    def intervention_state(self, r):
+        s = self.abi.cs_intervention_state()
+        self.call("get_intervention_state", int(r), C.byref(s))
+        return s
+
     def timeseries(self, max_samples):
         out = np.zeros((self.n_real, max_samples, self.abi.CS_TS_FIELDS), np.int32)
         n = C.c_int()

@@ -12,7 +12,7 @@ import importlib
 import numpy as np
 import pytest
 
-from fixtures import c1, hz_population, mini
+from fixtures import c1, hz_population, mini, mini_int
 from oracle_lib import OracleEngine
 
 pytestmark = pytest.mark.gpu
@@ -427,3 +427,68 @@ def test_block_list_moves_match_oracle(engmod, csa):
             most = np.maximum(most, [s_l.max(), l_i.max(), i_r.max()])
             prev[r] = cb
     assert (most > 64).all(), most
+
+
+@pytest.mark.parametrize("graphs", [True, False])
+def test_interventions_match_oracle(engmod, csa, graphs):
+    """The intervention subset (global ICU triggers, an alert that raises
+    case detection, case isolation, household quarantine, school and
+    university closure with absenteeism and carers) over 120 days of three
+    realisations, with replayed step graphs or direct launches: series,
+    states, counters and the intervention state (alert, trigger, start
+    times, closed microcells) equal to the oracle's."""
+    pop, params = mini_int()
+    seeds = np.array([71, 72, 73], np.uint64)
+    g = engmod.Engine(0)
+    g.set_params(params)
+    g.set_population(pop)
+    g.alloc(seeds)
+    g.set_step_graphs(graphs)
+    o = OracleEngine(params, pop)
+    o.alloc(seeds)
+    tspd = int(params.time_steps_per_day)
+    for eng in (g, o):
+        eng.reset(10)
+        eng.run_steps(0.0, 120 * tspd, tspd)
+    assert np.array_equal(g.timeseries(120), o.timeseries(120))
+    for r in range(3):
+        a, b = csa.population.State(pop), csa.population.State(pop)
+        g.store_state(r, a)
+        o.store_state(r, b)
+        assert a.compare(b) == [], r
+        ca, cb = g.counters(r), o.counters(r)
+        _compare_counters(ca, cb)
+        sa, sb = g.intervention_state(r), o.intervention_state(r)
+        for f, _ in sa._fields_:
+            assert getattr(sa, f) == getattr(sb, f), (r, f)
+        # every intervention ran: the alert, all three start times, and
+        # quarantined households, closure absences and isolations
+        assert sa.alert == 1 and sa.place_close_start < 1e9 and sa.case_isolation_start < 1e9
+        assert ca.cumHQ > 0 and ca.cumAPCS > 0
+        assert (a.a["isolation_start_time"] != 65534).sum() > 0
+
+
+def test_interventions_refused_outside_the_subset(engmod, csa):
+    """Parameter sets outside the intervention subset are refused with
+    CS_EUNSUPPORTED (never approximated), and a single sweep refuses an
+    intervention set (its triggers and TreatSweep run between sweeps)."""
+    pop, params = mini_int()
+    g = engmod.Engine(0)
+    g.set_params(params)
+    g.set_population(pop)
+    g.alloc(np.array([5], np.uint64))
+    g.reset(10)
+    with pytest.raises(engmod.CSError, match="only in cs_run_steps"):
+        g.infect_sweep(0.0)
+    for field, value, msg in [("do_global_triggers", 0, "global triggers"), ("soc_dist_time_start_base", 5.0, "social distancing"),
+                              ("place_close_cell_inc_stop_thresh", 3, "stop threshold"), ("do_per_capita_triggers", 1, "per-capita")]:
+        bad = csa.abi.cs_params()
+        C_ = __import__("ctypes")
+        C_.memmove(C_.addressof(bad), C_.addressof(params), C_.sizeof(bad))
+        setattr(bad, field, value)
+        g.set_params(bad)
+        with pytest.raises(engmod.CSError, match=msg):
+            g.reset(10)
+    g.set_params(params)
+    g.reset(10)
+    g.run_steps(0.0, 8, 4)

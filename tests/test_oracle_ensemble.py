@@ -5,13 +5,17 @@ The random streams differ by design (counter-based streams replace the
 reference's per-thread generators), so single trajectories cannot match;
 the distribution of trajectories must.  Each reference ensemble is 200
 realisations of the unmodified reference (tests/golden/make_ensemble.py) on
-one of three workloads, all with the population the reference's own set-up
+one of four workloads, all with the population the reference's own set-up
 builds (/c:1):
 
   mini       the 20k-person fixture region, 120 days
   mini_care  the same region with care homes (workplaces whose members aged
              55 and over are residents, every care-home scaling away from 1;
              tests/golden/make_fixtures.py CARE_KEYS), 120 days
+  mini_int   the same region with the intervention subset: global triggers
+             on ICU cases, case isolation, household quarantine, school and
+             university closure, an alert that raises case detection
+             (tests/golden/make_fixtures.py INT_PRE, INT_PARAMS), 120 days
   c1         the 200k-person C1 region, 200 days
 
 Compared: attack rate, peak day, peak daily incidence, cumulative cases and
@@ -20,10 +24,12 @@ agreement of the two ensembles' 95% confidence intervals for the mean (the
 intervals overlap); the daily infections and cases of every day with a
 per-day KS test, Bonferroni-corrected over the days (family-wise level 0.01;
 120 uncorrected tests at 0.01 would reject a true null in 70% of suites); and
-the final infections by route.  A 1000-versus-1000 run of the mini
+the final infections by route; with interventions, the start times of
+place closure, case isolation and household quarantine.  A 1000-versus-1000 run of the mini
 comparison (not part of the suite) gave |z| < 1.8 for every statistic.
 
-On the CPU the oracle's 200 realisations are compared (mini, mini_care).  On
+On the CPU the oracle's 200 realisations are compared (mini, mini_care,
+mini_int).  On
 the GPU the same 200 seeds, advanced together by one batched run, must give
 the oracle's 200 trajectories bit for bit and pass the same tests.  For c1,
 where 200 oracle realisations of 200k persons would take minutes, the GPU's
@@ -38,7 +44,7 @@ import numpy as np
 import pytest
 from scipy import stats
 
-from fixtures import GOLDEN, c1, mini, mini_care
+from fixtures import GOLDEN, c1, mini, mini_care, mini_int
 from oracle_lib import OracleEngine
 
 N_RUNS = 200
@@ -51,9 +57,12 @@ def workload(name):
         _, pop, params, _ = mini()
     elif name == "mini_care":
         pop, params = mini_care()
+    elif name == "mini_int":
+        pop, params = mini_int()
     else:
         pop, params = c1()
     z = np.load(os.path.join(GOLDEN, "%s_ensemble.npz" % name))
+    workload.starts = z["starts"] if "starts" in z.files else None
     return pop, params, z["daily"], z["itype"].astype(float)
 
 
@@ -66,7 +75,33 @@ def oracle_ensemble(name):
     o.reset(10)
     o.run_steps(0.0, days * int(params.time_steps_per_day), int(params.time_steps_per_day))
     itype = np.array([list(o.counters(r).cumItype) for r in range(N_RUNS)], float)
+    oracle_ensemble.starts[name] = start_times(o, N_RUNS)
     return o.timeseries(days), itype
+
+
+oracle_ensemble.starts = {}
+
+
+def start_times(eng, n):
+    """[run, 3]: place closure, case isolation, household quarantine start
+    (days; 1e10 while unset)"""
+    out = np.zeros((n, 3))
+    for r in range(n):
+        s = eng.intervention_state(r)
+        out[r] = (s.place_close_start, s.case_isolation_start, s.hquarantine_start)
+    return out
+
+
+def check_start_times(ref_steps, got_days, tspd=4):
+    """The intervention start times: set in the same share of runs, and the
+    same distribution where set (KS p > 0.01, overlapping 95% CIs)"""
+    for k in range(3):
+        a = ref_steps[:, k][ref_steps[:, k] >= 0] / float(tspd)
+        b = got_days[:, k][got_days[:, k] < 1e9]
+        assert abs(len(a) - len(b)) <= 3 * np.sqrt(max(len(a), 1)) + 2, (k, len(a), len(b))
+        if len(a) > 10 and len(b) > 10:
+            assert stats.ks_2samp(a, b).pvalue > ALPHA, (k, a.mean(), b.mean())
+            assert cis_overlap(a, b), (k, a.mean(), b.mean())
 
 
 SUMMARIES = ["attack rate", "peak day", "peak incidence", "cumulative cases", "deaths"]
@@ -115,11 +150,24 @@ def check_against_reference(ref, ref_it, got, got_it, n):
         assert welch_ci_contains_zero(ref_it[:, q], got_it[:, q], level=0.999), (q, ref_it[:, q].mean(), got_it[:, q].mean())
 
 
-@pytest.mark.parametrize("name", ["mini", "mini_care"])
+@pytest.mark.parametrize("name", ["mini", "mini_care", "mini_int"])
 def test_oracle_ensemble_matches_reference(name):
     pop, params, ref, ref_it = workload(name)
     got, got_it = oracle_ensemble(name)
     check_against_reference(ref, ref_it, got, got_it, pop.n_people)
+    if name == "mini_int":
+        check_start_times(workload.starts, oracle_ensemble.starts[name])
+
+
+def test_interventions_change_the_epidemic():
+    """The intervention fixture is not inert: in the reference's own
+    ensemble every run triggers all three interventions, and they lower the
+    attack rate against the same region without them."""
+    _, _, ref, _ = workload("mini")
+    _, _, iref, _ = workload("mini_int")
+    starts = workload.starts
+    assert (starts >= 0).all()
+    assert summary(iref, 20000)[0].mean() < summary(ref, 20000)[0].mean() - 0.05
 
 
 def test_care_homes_change_the_epidemic():
@@ -141,11 +189,12 @@ def gpu_run(pop, params, seeds, days):
     tspd = int(params.time_steps_per_day)
     g.run_steps(0.0, days * tspd, tspd)
     itype = np.array([list(g.counters(r).cumItype) for r in range(len(seeds))], float)
+    gpu_run.starts = start_times(g, len(seeds))
     return g.timeseries(days), itype
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("name", ["mini", "mini_care"])
+@pytest.mark.parametrize("name", ["mini", "mini_care", "mini_int"])
 def test_gpu_ensemble_is_the_oracle_ensemble_and_matches_the_reference(name):
     """The GPU's 200 realisations, advanced together by the batched run,
     reproduce the oracle's 200 bit for bit, and pass the ensemble tests
@@ -156,6 +205,9 @@ def test_gpu_ensemble_is_the_oracle_ensemble_and_matches_the_reference(name):
     assert gpu.shape == orc.shape and np.array_equal(gpu, orc)
     assert np.array_equal(gpu_it, orc_it)
     check_against_reference(ref, ref_it, gpu, gpu_it, pop.n_people)
+    if name == "mini_int":
+        assert np.array_equal(gpu_run.starts, oracle_ensemble.starts[name])
+        check_start_times(workload.starts, gpu_run.starts)
 
 
 @pytest.mark.gpu

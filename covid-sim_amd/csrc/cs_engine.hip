@@ -265,6 +265,7 @@ int launch_resolve(cs_engine* e)
         cs::k_resolve_claim<<<e->grid_waves, 256, 0, st>>>(d);
         cs::k_resolve_apply<<<e->grid_waves, 256, 0, st>>>(d);
         cs::k_resolve_reset<<<e->grid_waves, 256, 0, st>>>(d);
+        cs::k_tally_fold<<<d.R, 64, 0, st>>>(d);
     }
     { ProfScope ps(e, CS_K_LIST_MOVE); cs::k_list_move<<<e->grid_moves, 256, 0, st>>>(d, 0); }
     HIPCHK(e, hipGetLastError());
@@ -809,8 +810,9 @@ int cs_alloc_realisations(cs_engine* e, int n_real, const uint64_t* seeds)
     d.bind_clr(d.clr);
     AL(care_gp, RN); AL(care_k, RN); AL(care_abs, RN); AL(care_count, 1);
     AL(last_tot, CS_TS_LAST * R);
-    d.due_cap = (unsigned)std::min<size_t>(std::max<size_t>(RN / 64, 4096), 0x7fffffff);
-    AL(due, (size_t)CS_DUE_RING * d.due_cap); AL(due_n, CS_DUE_RING * CS_DUE_STRIDE);
+    AL(tally, (size_t)CS_TALLY_COPIES * R * cs::TF_STRIDE);
+    d.due_scap = (unsigned)std::min<size_t>(std::max<size_t>(RN / 64, 4096) / CS_DUE_SHARDS + 1, 0x7fffffff);
+    AL(due, (size_t)CS_DUE_RING * CS_DUE_SHARDS * d.due_scap); AL(due_n, CS_DUE_RING * CS_DUE_SHARDS * CS_DUE_STRIDE); AL(due_ovf, CS_DUE_RING);
     d.due_on = 0;
 #undef AL
     if ((rc = dalloc(e, A, &e->d_sel, R))) return rc;
@@ -823,6 +825,7 @@ int cs_alloc_realisations(cs_engine* e, int n_real, const uint64_t* seeds)
     HIPCHK(e, hipMemset(d.bcount, 0, RC * 4));
     HIPCHK(e, hipMemset(d.wl, 0, cs::WL_N * 4));
     HIPCHK(e, hipMemset(d.cnt, 0, R * sizeof(cs::Counters)));
+    HIPCHK(e, hipMemset(d.tally, 0, (size_t)CS_TALLY_COPIES * R * cs::TF_STRIDE * sizeof(unsigned long long)));
     HIPCHK(e, hipMemset(d.m_trig, 0, R * NM * 6 * 4));
     HIPCHK(e, hipMemset(d.itv_n, 0, 4 * sizeof(unsigned)));
     cs::k_reset_interventions<<<grid_for(d.R, 64), 64, 0, e->stream>>>(d);
@@ -863,7 +866,8 @@ int cs_reset_realisations(cs_engine* e, int n_seed)
     hipStream_t st = e->stream;
     e->ts_fresh = true;
     HIPCHK(e, hipMemsetAsync(d.cnt, 0, (size_t)d.R * sizeof(cs::Counters), st));
-    HIPCHK(e, hipMemsetAsync(d.due_n, 0, CS_DUE_RING * CS_DUE_STRIDE * sizeof(unsigned), st));
+    HIPCHK(e, hipMemsetAsync(d.due_n, 0, CS_DUE_RING * CS_DUE_SHARDS * CS_DUE_STRIDE * sizeof(unsigned), st));
+    HIPCHK(e, hipMemsetAsync(d.due_ovf, 0, CS_DUE_RING * sizeof(unsigned), st));
     if (d.due_on != (e->due_ok ? 1 : 0)) {  // fresh state: every latent person is listed from here on
         HIPCHK(e, hipStreamSynchronize(st));
         drop_graph(e);
@@ -1165,11 +1169,13 @@ int cs_set_due_capacity(cs_engine* e, uint32_t per_step)
     Dev& d = e->d;
     (void)hipFree(d.due);
     e->real_allocs.erase(std::remove(e->real_allocs.begin(), e->real_allocs.end(), (void*)d.due), e->real_allocs.end());
-    int rc = dalloc(e, e->real_allocs, &d.due, (size_t)CS_DUE_RING * per_step);
+    const unsigned scap = (per_step + CS_DUE_SHARDS - 1) / CS_DUE_SHARDS;
+    int rc = dalloc(e, e->real_allocs, &d.due, (size_t)CS_DUE_RING * CS_DUE_SHARDS * scap);
     if (rc) return rc;
-    d.due_cap = per_step;
+    d.due_scap = scap;
     // the lists held so far are gone: scan until the next reset lists afresh
-    HIPCHK(e, hipMemset(d.due_n, 0, CS_DUE_RING * CS_DUE_STRIDE * sizeof(unsigned)));
+    HIPCHK(e, hipMemset(d.due_n, 0, CS_DUE_RING * CS_DUE_SHARDS * CS_DUE_STRIDE * sizeof(unsigned)));
+    HIPCHK(e, hipMemset(d.due_ovf, 0, CS_DUE_RING * sizeof(unsigned)));
     d.due_on = 0;
     return 0;
 }

@@ -109,7 +109,32 @@ __device__ __forceinline__ Chunk chunk_at(const Chunk* l, unsigned ch)
 // this step's latent persons are all on its due list (not overflowed)
 __device__ __forceinline__ bool due_listed(const Dev& d)
 {
-    return d.due_on && d.due_n[(d.sc->ts & (CS_DUE_RING - 1u)) * CS_DUE_STRIDE] <= d.due_cap;
+    return d.due_on && d.due_ovf[d.sc->ts & (CS_DUE_RING - 1u)] == 0u;
+}
+
+__device__ __forceinline__ unsigned* due_count(const Dev& d, unsigned b, unsigned sh) { return d.due_n + ((size_t)b * CS_DUE_SHARDS + sh) * CS_DUE_STRIDE; }
+
+// a returning atomicAdd(a, 1) from every lane with on set, merged: the lanes
+// that add to one address make one atomic between them (one instruction for
+// all the wave's addresses) and take consecutive results in lane order.
+// Every lane of the wave must reach the call.
+__device__ __forceinline__ unsigned slot_merged(bool on, unsigned* a)
+{
+    const uint64_t me = (uint64_t)(uintptr_t)a;
+    const unsigned lane = threadIdx.x & 63u;
+    unsigned long long mine = 0;
+    for (unsigned long long todo = __ballot(on); todo;) {
+        const int lead = __builtin_ctzll(todo);
+        const uint64_t la = ((uint64_t)(uint32_t)__shfl((int)(uint32_t)(me >> 32), lead, 64) << 32) | (uint32_t)__shfl((int)(uint32_t)me, lead, 64);
+        const unsigned long long same = __ballot(on && me == la);
+        if (on && me == la) mine = same;
+        todo &= ~same;
+    }
+    const int leader = on ? __builtin_ctzll(mine) : (int)lane;
+    unsigned base = 0;
+    if (on && (int)lane == leader) base = atomicAdd(a, (unsigned)__popcll(mine));
+    base = (unsigned)__shfl((int)base, leader, 64);
+    return base + (unsigned)__popcll(mine & ((1ull << lane) - 1ull));
 }
 
 __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v)
@@ -508,6 +533,7 @@ __global__ void __launch_bounds__(256) k_list_move(Dev d, int mode)
 struct InfectTally {
     int kind, ag, kw;        // cumItype, cumIa, cumI_keyworker indices
     unsigned lat;            // latent time (the due list p joins)
+    int c;                   // p's cell (the batch p joins)
     double rad2;
     long long tg, si;        // generation and serial intervals
     int ntg;                 // 1 when the infector is a person
@@ -541,6 +567,7 @@ __device__ bool do_infect_state(const Dev& d, int r, int p, double t, unsigned t
         lat = (uint16_t)(t * P->time_steps_per_day);
     d.t_lat[gp] = lat;
     tl->lat = lat;
+    tl->c = d.pc[p];
     tl->tg = tl->si = 0;
     tl->ntg = 0;
     if (infector >= 0) {
@@ -549,7 +576,6 @@ __device__ bool do_infect_state(const Dev& d, int r, int p, double t, unsigned t
         tl->si = ((int)lat) - ((int)d.t_lat[gi]);
         tl->ntg = 1;
     }
-    batch_add(d, r, d.pc[p], p);
     return true;
 }
 
@@ -574,19 +600,41 @@ __device__ __forceinline__ void tally_scalar(Counters* cn, unsigned long long n,
     }
 }
 
-// list gp on the due list of the step that ends its latency
-__device__ __forceinline__ void due_add(const Dev& d, unsigned lat, long long gp)
+// the staged record of realisation r for this block
+__device__ __forceinline__ unsigned long long* tally_rec(const Dev& d, int r)
 {
-    const unsigned b = lat & (CS_DUE_RING - 1u);
-    const unsigned idx = atomicAdd(&d.due_n[b * CS_DUE_STRIDE], 1u);
-    if (idx < d.due_cap) d.due[(size_t)b * d.due_cap + idx] = gp;
+    return d.tally + ((size_t)(blockIdx.x & (CS_TALLY_COPIES - 1u)) * d.R + r) * TF_STRIDE;
+}
+__device__ __forceinline__ void stage_scalar(unsigned long long* t, unsigned long long n, double rad2, unsigned long long rad2max_bits,
+                                             long long tg, long long si, unsigned long long ntg)
+{
+    atomicAdd(&t[TF_CUMI], n);
+    atomicAdd((double*)&t[TF_SUMR2], rad2);
+    atomicMax(&t[TF_MAXR2], rad2max_bits);
+    if (ntg) {
+        atomicAdd(&t[TF_TG], (unsigned long long)tg);
+        atomicAdd(&t[TF_SI], (unsigned long long)si);
+        atomicAdd(&t[TF_NTG], ntg);
+    }
+}
+
+// gp at index idx of its shard of the due list of the step that ends its
+// latency; a full shard marks the bucket overflowed
+__device__ __forceinline__ void due_put(const Dev& d, unsigned b, unsigned sh, unsigned idx, long long gp)
+{
+    if (idx < d.due_scap) d.due[((size_t)b * CS_DUE_SHARDS + sh) * d.due_scap + idx] = gp;
+    else if (d.due_ovf[b] == 0u) atomicOr(&d.due_ovf[b], 1u);
 }
 
 __device__ void do_infect(const Dev& d, int r, int p, double t, unsigned ts, int infector, int type)
 {
     InfectTally tl;
     if (!do_infect_state(d, r, p, t, ts, infector, type, &tl)) return;
-    if (d.due_on) due_add(d, tl.lat, (long long)r * d.N + p);
+    batch_add(d, r, tl.c, p);
+    if (d.due_on) {
+        const unsigned b = tl.lat & (CS_DUE_RING - 1u), sh = blockIdx.x & (CS_DUE_SHARDS - 1u);
+        due_put(d, b, sh, atomicAdd(due_count(d, b, sh), 1u), (long long)r * d.N + p);
+    }
     Counters* cn = d.cnt + r;
     tally_indexed(cn, tl);
     tally_scalar(cn, 1, tl.rad2, double_to_bits(tl.rad2), tl.tg, tl.si, (unsigned long long)tl.ntg);
@@ -1096,13 +1144,14 @@ __global__ void k_resolve_apply(Dev d)
         const unsigned k = base + lane;
         const size_t e = (k < n) ? q.slot(k) : 0;
         bool won = false;
-        int r = 0;
+        int r = 0, p = 0;
         InfectTally tl;
+        tl.c = 0;
         if (k < n) {
             const size_t gp = (size_t)d.ev_gp[e];
             const unsigned long long key = d.ev_key[e];
             r = (int)(gp / d.N);
-            const int p = (int)(gp % d.N);
+            p = (int)(gp % d.N);
             if (key == d.claim[gp]) {
                 won = do_infect_state(d, r, p, sc.t, sc.ts, d.ev_infector[e], d.ev_type[e], &tl);
                 nn++;
@@ -1112,27 +1161,76 @@ __global__ void k_resolve_apply(Dev d)
                 d.itype[gp] = d.ev_type[e];
             }
         }
-        if (d.due_on && won) due_add(d, tl.lat, (long long)d.ev_gp[e]);
-        // counters: events are appended by whole producing waves, so a wave's
-        // infections nearly always share one realisation; then the scalar
-        // counters take one atomic per wave instead of one per infection
+        // the winners join their cells' batches and their latent times' due
+        // lists, lanes of one cell (of one bucket) sharing one atomic
+        {
+            const int rcw = won ? r * d.NC + tl.c : 0;
+            const unsigned bi = slot_merged(won, (unsigned*)&d.bcount[rcw]);
+            if (won) d.batch[(size_t)r * d.N + d.cstart[tl.c] + bi] = p;
+        }
+        if (d.due_on) {
+            const unsigned b = won ? (tl.lat & (CS_DUE_RING - 1u)) : 0u, sh = blockIdx.x & (CS_DUE_SHARDS - 1u);
+            const unsigned di = slot_merged(won, due_count(d, b, sh));
+            if (won) due_put(d, b, sh, di, (long long)r * d.N + p);
+        }
+        // counters, staged in the block's copy of each realisation's record
+        // (k_tally_fold adds them up): events are appended by whole producing
+        // waves, so a wave's infections nearly always share one realisation;
+        // then the scalar counters take one atomic per wave instead of one
+        // per infection, and the indexed ones one per distinct index
         const unsigned long long wm = __ballot(won);
         if (wm == 0) continue;
         const int r0 = __shfl(r, __ffsll((long long)wm) - 1, 64);
-        if (won) tally_indexed(d.cnt + r, tl);
+        if (won) {
+            unsigned long long* t = tally_rec(d, r);
+            cadd_merged(&t[TF_ITYPE + tl.kind], 1);
+            cadd_merged(&t[TF_IA + tl.ag], 1);
+            cadd_merged(&t[TF_KW + tl.kw], 1);
+        }
         if (__ballot(won && r != r0) == 0) {
             const double rad2 = wave_sum_d(won ? tl.rad2 : 0.0);
             const unsigned long long mx = wave_max_u64(won ? double_to_bits(tl.rad2) : 0ull);
             const long long tg = (long long)wave_sum(won ? (unsigned long long)tl.tg : 0ull);
             const long long si = (long long)wave_sum(won ? (unsigned long long)tl.si : 0ull);
             const unsigned long long ntg = wave_sum(won ? (unsigned long long)tl.ntg : 0ull);
-            if (lane == __ffsll((long long)wm) - 1) tally_scalar(d.cnt + r0, __popcll(wm), rad2, mx, tg, si, ntg);
+            if (lane == __ffsll((long long)wm) - 1) stage_scalar(tally_rec(d, r0), __popcll(wm), rad2, mx, tg, si, ntg);
         } else if (won) {
-            tally_scalar(d.cnt + r, 1, tl.rad2, double_to_bits(tl.rad2), tl.tg, tl.si, (unsigned long long)tl.ntg);
+            stage_scalar(tally_rec(d, r), 1, tl.rad2, double_to_bits(tl.rad2), tl.tg, tl.si, (unsigned long long)tl.ntg);
         }
     }
     nn = wave_sum(nn);
     if (lane == 0 && nn) wtally(d.wc->n_new, nn);
+}
+
+// the staged counters of the resolution into Counters, clearing the stage
+// (a block per realisation, a thread per field; the copies in order)
+__global__ void k_tally_fold(Dev d)
+{
+    if (halted(d)) return;
+    const int r = blockIdx.x, f = threadIdx.x;
+    if (r >= d.R || f >= TF_N) return;
+    unsigned long long s = 0, mx = 0;
+    double sd = 0.0;
+    for (int c = 0; c < CS_TALLY_COPIES; c++) {
+        unsigned long long* t = d.tally + ((size_t)c * d.R + r) * TF_STRIDE + f;
+        const unsigned long long v = *t;
+        if (v == 0ull) continue;
+        *t = 0ull;
+        if (f == TF_SUMR2) sd += __longlong_as_double((long long)v);
+        else if (f == TF_MAXR2) mx = (v > mx) ? v : mx;
+        else s += v;
+    }
+    Counters& cn = d.cnt[r];
+    if (f == TF_SUMR2) { if (sd != 0.0) cn.sumRad2 += sd; return; }
+    if (f == TF_MAXR2) { if (mx > cn.maxRad2_bits) cn.maxRad2_bits = mx; return; }
+    if (s == 0ull) return;
+    if (f == TF_CUMI) cn.cumI += s;
+    else if (f == TF_TG) cn.cumTG += s;
+    else if (f == TF_SI) cn.cumSI += s;
+    else if (f == TF_NTG) cn.nTG += s;
+    else if (f < TF_IA) cn.cumItype[f - TF_ITYPE] += s;
+    else if (f < TF_KW) cn.cumIa[f - TF_IA] += s;
+    else cn.cumI_keyworker[f - TF_KW] += s;
 }
 
 __global__ void k_resolve_reset(Dev d)

@@ -171,16 +171,37 @@ __global__ void __launch_bounds__(256) k_latent_due(Dev d)
     if (halted(d) || !due_listed(d)) return;
     const unsigned ts = d.sc->ts;
     const unsigned b = ts & (CS_DUE_RING - 1u);
-    const unsigned n = d.due_n[b * CS_DUE_STRIDE];
-    const long long* list = d.due + (size_t)b * d.due_cap;
+    // the bucket's shards as one sequence: pre[s] persons lie in the shards
+    // before s (each count is at most due_scap: no shard overflowed);
+    // constant indices only, so pre stays in registers
+    unsigned pre[CS_DUE_SHARDS + 1];
+    pre[0] = 0;
+#pragma unroll
+    for (int s = 0; s < CS_DUE_SHARDS; s++) pre[s + 1] = pre[s] + uni(*due_count(d, b, (unsigned)s));
+    const unsigned n = pre[CS_DUE_SHARDS];
+    const long long* list = d.due + (size_t)b * CS_DUE_SHARDS * d.due_scap;
     unsigned long long nt = 0;
-    for (unsigned k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
-        const long long gp = list[k];
-        if (d.inf[gp] == CS_LATENT && d.t_lat[gp] == ts) {
-            const int r = (int)(gp / d.N), p = (int)(gp % d.N);
-            do_incub(d, r, p, ts);
-            batch_add(d, r, d.pc[p], p);
-            nt++;
+    {
+        // whole waves step through the sequence, every lane reaching the merged batch append
+        for (unsigned k0 = (blockIdx.x * blockDim.x + threadIdx.x) & ~63u; k0 < n; k0 += gridDim.x * blockDim.x) {
+            const unsigned k = k0 + (threadIdx.x & 63u);
+            bool go = false;
+            int r = 0, p = 0, c = 0;
+            if (k < n) {
+                unsigned sh = 0, base = 0;
+#pragma unroll
+                for (int j = 1; j < CS_DUE_SHARDS; j++)
+                    if (k >= pre[j]) { sh = (unsigned)j; base = pre[j]; }
+                const long long gp = list[(size_t)sh * d.due_scap + (k - base)];
+                if (d.inf[gp] == CS_LATENT && d.t_lat[gp] == ts) {
+                    r = (int)(gp / d.N), p = (int)(gp % d.N), c = d.pc[p];
+                    do_incub(d, r, p, ts);
+                    go = true;
+                    nt++;
+                }
+            }
+            const unsigned bi = slot_merged(go, (unsigned*)&d.bcount[r * d.NC + c]);
+            if (go) d.batch[(size_t)r * d.N + d.cstart[c] + bi] = p;
         }
     }
     nt = wave_sum(nt);
@@ -190,7 +211,10 @@ __global__ void __launch_bounds__(256) k_latent_due(Dev d)
 // this step's due list is consumed: clear it for the step CS_DUE_RING later
 __global__ void k_due_clear(Dev d)
 {
-    if (threadIdx.x == 0 && !halted(d) && d.due_on) d.due_n[(d.sc->ts & (CS_DUE_RING - 1u)) * CS_DUE_STRIDE] = 0u;
+    if (halted(d) || !d.due_on) return;
+    const unsigned b = d.sc->ts & (CS_DUE_RING - 1u);
+    if (threadIdx.x < CS_DUE_SHARDS) *due_count(d, b, threadIdx.x) = 0u;
+    if (threadIdx.x == 0) d.due_ovf[b] = 0u;
 }
 
 __global__ void __launch_bounds__(256) k_latent_scan(Dev d)

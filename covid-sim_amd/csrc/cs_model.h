@@ -40,6 +40,17 @@ struct Counters {  // device mirror of cs_counters, updated with atomics
     unsigned long long cumHQ, cumAH, cumAPC, cumAPA, cumAPCS;
 };
 
+// DoInfect's counters as resolution stages them: CS_TALLY_COPIES copies of
+// each realisation's record (a block adds to copy blockIdx.x %
+// CS_TALLY_COPIES), each record TF_STRIDE words (three 128-byte lines), folded
+// into Counters after the resolution.  Every wave adds to its realisation's
+// few counters, and adds from the whole GPU to one line serialise at the
+// memory side.
+#define CS_TALLY_COPIES 16
+enum { TF_CUMI = 0, TF_TG, TF_SI, TF_NTG, TF_MAXR2, TF_SUMR2, TF_ITYPE,
+       TF_IA = TF_ITYPE + CS_INFECT_TYPE_MASK, TF_KW = TF_IA + CS_NUM_AGE_GROUPS, TF_N = TF_KW + 2, TF_STRIDE = 48 };
+static_assert(TF_N <= TF_STRIDE, "a staged record fits its stride");
+
 // one realisation's copy of the P and AdUnits fields the trigger logic sets
 // (src/CovidSim.cpp:3485-3689); written at samples by one thread
 struct IntState {
@@ -116,10 +127,13 @@ __host__ __device__ __forceinline__ bool no_place_links(const int32_t* lk, int n
 // time is ts, appended when they are infected (every latent time lies less
 // than CS_DUE_RING steps ahead, checked from the latent-period table)
 #define CS_DUE_RING 256
-// a bucket's count sits alone on a 128-byte line (CS_DUE_STRIDE words apart):
-// a step's infections append to the few buckets of their latent times, and
-// adds from the whole GPU to one line serialise at the memory side
+// a bucket is CS_DUE_SHARDS shards, a block appending to shard
+// blockIdx.x % CS_DUE_SHARDS; each shard's count sits alone on a 128-byte
+// line (CS_DUE_STRIDE words apart): a step's infections append to the few
+// buckets of their latent times, and adds from the whole GPU to one line
+// serialise at the memory side
 #define CS_DUE_STRIDE 32
+#define CS_DUE_SHARDS 16
 
 // the event queue is CS_EV_SHARDS shards: a block appends to shard
 // blockIdx.x % CS_EV_SHARDS, whose count sits alone on a 128-byte line and
@@ -282,6 +296,7 @@ struct Dev {
     unsigned* itv_n;      // counts: [0] det, [1] close_l, [2] carers queued by closures (in care_k)
     unsigned det_cap, close_cap;
     Counters* cnt;
+    unsigned long long* tally;  // [CS_TALLY_COPIES][R][TF_STRIDE] staged DoInfect counters (zero between steps)
     double* prev_prop;
     double* samp_prop;  // State.S / PopSize as of the last sample, per realisation
     double* loc_init;  // [R][2]
@@ -332,9 +347,10 @@ struct Dev {
     int32_t* ts_buf;      // [R][max_samples][CS_TS_FIELDS]
     int ts_max;
     long long* last_tot;  // [R][CS_TS_LAST] cumI, cumC, D, severity incidence at the previous sample
-    long long* due;       // [CS_DUE_RING][due_cap] persons (gp) by latent time
-    unsigned* due_n;      // [CS_DUE_RING * CS_DUE_STRIDE], bucket b at b * CS_DUE_STRIDE: appended; above due_cap: that step scans every latent person
-    unsigned due_cap;
+    long long* due;       // [CS_DUE_RING][CS_DUE_SHARDS][due_scap] persons (gp) by latent time
+    unsigned* due_n;      // [CS_DUE_RING][CS_DUE_SHARDS] counts, CS_DUE_STRIDE words apart
+    unsigned* due_ovf;    // [CS_DUE_RING] a shard of the bucket overflowed: that step scans every latent person
+    unsigned due_scap;    // persons per shard
     int due_on;           // 0: every step scans every latent person
 };
 

@@ -424,8 +424,9 @@ int  cs_get_queue_capacity(cs_engine* e, uint32_t* events, uint32_t* draw_chunks
 /* capacity of each step's latent due list: the persons whose latency ends in
  * that step, listed when they are infected, so that the step visits only
  * them (after a reset; a loaded state is scanned in full until the next
- * reset).  A step whose list overflows scans every latent person instead,
- * with the same result; 0 keeps the value.  No counterpart in the reference,
+ * reset).  The list is 16 shards of per_step / 16 persons (rounded up), a
+ * block appending to one shard; a step one of whose shards overflows scans
+ * every latent person instead, with the same result; 0 keeps the value.  No counterpart in the reference,
  * which scans every latent person every step (src/Sweep.cpp:922-940). */
 int  cs_set_due_capacity(cs_engine* e, uint32_t per_step);
 /* launch shapes: blocks of the per-person passes (4 waves each, one wave

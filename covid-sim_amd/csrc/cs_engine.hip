@@ -265,7 +265,7 @@ int launch_resolve(cs_engine* e)
         cs::k_resolve_claim<<<e->grid_waves, 256, 0, st>>>(d);
         cs::k_resolve_apply<<<e->grid_waves, 256, 0, st>>>(d);
         cs::k_resolve_reset<<<e->grid_waves, 256, 0, st>>>(d);
-        cs::k_tally_fold<<<d.R, 64, 0, st>>>(d);
+        cs::k_cnt_fold<<<dim3(d.R, (cs::CS_CNT_WORDS + 63) / 64), 64, 0, st>>>(d);
     }
     { ProfScope ps(e, CS_K_LIST_MOVE); cs::k_list_move<<<e->grid_moves, 256, 0, st>>>(d, 0); }
     HIPCHK(e, hipGetLastError());
@@ -314,6 +314,7 @@ int launch_incub(cs_engine* e)
         cs::k_care_apply<<<256, 256, 0, st>>>(d);
     }
     { ProfScope ps(e, CS_K_LIST_MOVE); cs::k_list_move<<<e->grid_moves, 256, 0, st>>>(d, 2); }
+    { ProfScope ps(e, CS_K_OTHER); cs::k_cnt_fold<<<dim3(d.R, (cs::CS_CNT_WORDS + 63) / 64), 64, 0, st>>>(d); }
     HIPCHK(e, hipGetLastError());
     return 0;
 }
@@ -810,7 +811,7 @@ int cs_alloc_realisations(cs_engine* e, int n_real, const uint64_t* seeds)
     d.bind_clr(d.clr);
     AL(care_gp, RN); AL(care_k, RN); AL(care_abs, RN); AL(care_count, 1);
     AL(last_tot, CS_TS_LAST * R);
-    AL(tally, (size_t)CS_TALLY_COPIES * R * cs::TF_STRIDE);
+    AL(cstage, (size_t)CS_TALLY_COPIES * R);
     d.due_scap = (unsigned)std::min<size_t>(std::max<size_t>(RN / 64, 4096) / CS_DUE_SHARDS + 1, 0x7fffffff);
     AL(due, (size_t)CS_DUE_RING * CS_DUE_SHARDS * d.due_scap); AL(due_n, CS_DUE_RING * CS_DUE_SHARDS * CS_DUE_STRIDE); AL(due_ovf, CS_DUE_RING);
     d.due_on = 0;
@@ -825,7 +826,7 @@ int cs_alloc_realisations(cs_engine* e, int n_real, const uint64_t* seeds)
     HIPCHK(e, hipMemset(d.bcount, 0, RC * 4));
     HIPCHK(e, hipMemset(d.wl, 0, cs::WL_N * 4));
     HIPCHK(e, hipMemset(d.cnt, 0, R * sizeof(cs::Counters)));
-    HIPCHK(e, hipMemset(d.tally, 0, (size_t)CS_TALLY_COPIES * R * cs::TF_STRIDE * sizeof(unsigned long long)));
+    HIPCHK(e, hipMemset(d.cstage, 0, (size_t)CS_TALLY_COPIES * R * sizeof(cs::Counters)));
     HIPCHK(e, hipMemset(d.m_trig, 0, R * NM * 6 * 4));
     HIPCHK(e, hipMemset(d.itv_n, 0, 4 * sizeof(unsigned)));
     cs::k_reset_interventions<<<grid_for(d.R, 64), 64, 0, e->stream>>>(d);

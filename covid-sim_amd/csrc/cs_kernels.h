@@ -137,6 +137,12 @@ __device__ __forceinline__ unsigned slot_merged(bool on, unsigned* a)
     return base + (unsigned)__popcll(mine & ((1ull << lane) - 1ull));
 }
 
+// this block's staged copy of realisation r's counters
+__device__ __forceinline__ Counters* cstage(const Dev& d, int r)
+{
+    return d.cstage + (size_t)(blockIdx.x & (CS_TALLY_COPIES - 1u)) * d.R + r;
+}
+
 __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v)
 {
     for (int w = 32; w > 0; w >>= 1) v += __shfl_xor(v, w, 64);
@@ -597,24 +603,6 @@ __device__ __forceinline__ void tally_scalar(Counters* cn, unsigned long long n,
         cadd(&cn->cumTG, tg);
         cadd(&cn->cumSI, si);
         cadd(&cn->nTG, (long long)ntg);
-    }
-}
-
-// the staged record of realisation r for this block
-__device__ __forceinline__ unsigned long long* tally_rec(const Dev& d, int r)
-{
-    return d.tally + ((size_t)(blockIdx.x & (CS_TALLY_COPIES - 1u)) * d.R + r) * TF_STRIDE;
-}
-__device__ __forceinline__ void stage_scalar(unsigned long long* t, unsigned long long n, double rad2, unsigned long long rad2max_bits,
-                                             long long tg, long long si, unsigned long long ntg)
-{
-    atomicAdd(&t[TF_CUMI], n);
-    atomicAdd((double*)&t[TF_SUMR2], rad2);
-    atomicMax(&t[TF_MAXR2], rad2max_bits);
-    if (ntg) {
-        atomicAdd(&t[TF_TG], (unsigned long long)tg);
-        atomicAdd(&t[TF_SI], (unsigned long long)si);
-        atomicAdd(&t[TF_NTG], ntg);
     }
 }
 
@@ -1173,8 +1161,8 @@ __global__ void k_resolve_apply(Dev d)
             const unsigned di = slot_merged(won, due_count(d, b, sh));
             if (won) due_put(d, b, sh, di, (long long)r * d.N + p);
         }
-        // counters, staged in the block's copy of each realisation's record
-        // (k_tally_fold adds them up): events are appended by whole producing
+        // counters, staged in the block's copy of each realisation's
+        // Counters (k_cnt_fold adds them up): events are appended by whole producing
         // waves, so a wave's infections nearly always share one realisation;
         // then the scalar counters take one atomic per wave instead of one
         // per infection, and the indexed ones one per distinct index
@@ -1182,10 +1170,10 @@ __global__ void k_resolve_apply(Dev d)
         if (wm == 0) continue;
         const int r0 = __shfl(r, __ffsll((long long)wm) - 1, 64);
         if (won) {
-            unsigned long long* t = tally_rec(d, r);
-            cadd_merged(&t[TF_ITYPE + tl.kind], 1);
-            cadd_merged(&t[TF_IA + tl.ag], 1);
-            cadd_merged(&t[TF_KW + tl.kw], 1);
+            Counters* t = cstage(d, r);
+            cadd_merged(&t->cumItype[tl.kind], 1);
+            cadd_merged(&t->cumIa[tl.ag], 1);
+            cadd_merged(&t->cumI_keyworker[tl.kw], 1);
         }
         if (__ballot(won && r != r0) == 0) {
             const double rad2 = wave_sum_d(won ? tl.rad2 : 0.0);
@@ -1193,44 +1181,47 @@ __global__ void k_resolve_apply(Dev d)
             const long long tg = (long long)wave_sum(won ? (unsigned long long)tl.tg : 0ull);
             const long long si = (long long)wave_sum(won ? (unsigned long long)tl.si : 0ull);
             const unsigned long long ntg = wave_sum(won ? (unsigned long long)tl.ntg : 0ull);
-            if (lane == __ffsll((long long)wm) - 1) stage_scalar(tally_rec(d, r0), __popcll(wm), rad2, mx, tg, si, ntg);
+            if (lane == __ffsll((long long)wm) - 1) tally_scalar(cstage(d, r0), __popcll(wm), rad2, mx, tg, si, ntg);
         } else if (won) {
-            stage_scalar(tally_rec(d, r), 1, tl.rad2, double_to_bits(tl.rad2), tl.tg, tl.si, (unsigned long long)tl.ntg);
+            tally_scalar(cstage(d, r), 1, tl.rad2, double_to_bits(tl.rad2), tl.tg, tl.si, (unsigned long long)tl.ntg);
         }
     }
     nn = wave_sum(nn);
     if (lane == 0 && nn) wtally(d.wc->n_new, nn);
 }
 
-// the staged counters of the resolution into Counters, clearing the stage
-// (a block per realisation, a thread per field; the copies in order)
-__global__ void k_tally_fold(Dev d)
+// the staged counters into each realisation's Counters, clearing the stage:
+// block (r, y) takes words [64y, 64y + 64) of realisation r's Counters, a
+// thread per 8-byte word; the copies are read together, then summed in
+// order (sums wrap as the atomics do; sumRad2 is a double sum, maxRad2_bits
+// a maximum)
+constexpr int CS_CNT_WORDS = (int)(sizeof(Counters) / 8);
+__global__ void __launch_bounds__(64) k_cnt_fold(Dev d)
 {
     if (halted(d)) return;
-    const int r = blockIdx.x, f = threadIdx.x;
-    if (r >= d.R || f >= TF_N) return;
-    unsigned long long s = 0, mx = 0;
+    const int r = blockIdx.x, f = blockIdx.y * 64 + threadIdx.x;
+    if (r >= d.R || f >= CS_CNT_WORDS) return;
+    constexpr int WSUM = (int)(offsetof(Counters, sumRad2) / 8), WMAX = (int)(offsetof(Counters, maxRad2_bits) / 8);
+    unsigned long long v[CS_TALLY_COPIES];
+#pragma unroll
+    for (int c = 0; c < CS_TALLY_COPIES; c++) v[c] = ((const unsigned long long*)(d.cstage + (size_t)c * d.R + r))[f];
+    unsigned long long s = 0, mx = 0, any = 0;
     double sd = 0.0;
+#pragma unroll
     for (int c = 0; c < CS_TALLY_COPIES; c++) {
-        unsigned long long* t = d.tally + ((size_t)c * d.R + r) * TF_STRIDE + f;
-        const unsigned long long v = *t;
-        if (v == 0ull) continue;
-        *t = 0ull;
-        if (f == TF_SUMR2) sd += __longlong_as_double((long long)v);
-        else if (f == TF_MAXR2) mx = (v > mx) ? v : mx;
-        else s += v;
+        any |= v[c];
+        if (f == WSUM) sd += __longlong_as_double((long long)v[c]);
+        else if (f == WMAX) mx = (v[c] > mx) ? v[c] : mx;
+        else s += v[c];
     }
-    Counters& cn = d.cnt[r];
-    if (f == TF_SUMR2) { if (sd != 0.0) cn.sumRad2 += sd; return; }
-    if (f == TF_MAXR2) { if (mx > cn.maxRad2_bits) cn.maxRad2_bits = mx; return; }
-    if (s == 0ull) return;
-    if (f == TF_CUMI) cn.cumI += s;
-    else if (f == TF_TG) cn.cumTG += s;
-    else if (f == TF_SI) cn.cumSI += s;
-    else if (f == TF_NTG) cn.nTG += s;
-    else if (f < TF_IA) cn.cumItype[f - TF_ITYPE] += s;
-    else if (f < TF_KW) cn.cumIa[f - TF_IA] += s;
-    else cn.cumI_keyworker[f - TF_KW] += s;
+    if (any == 0ull) return;
+#pragma unroll
+    for (int c = 0; c < CS_TALLY_COPIES; c++)
+        if (v[c]) ((unsigned long long*)(d.cstage + (size_t)c * d.R + r))[f] = 0ull;
+    unsigned long long* dst = (unsigned long long*)(d.cnt + r);
+    if (f == WSUM) ((double*)dst)[f] += sd;
+    else if (f == WMAX) dst[f] = (mx > dst[f]) ? mx : dst[f];
+    else dst[f] += s;
 }
 
 __global__ void k_resolve_reset(Dev d)

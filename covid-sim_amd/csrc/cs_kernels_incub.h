@@ -252,7 +252,7 @@ __device__ int do_case(const Dev& d, int r, int p, unsigned ts)
     if (d.inf[gp] != CS_INF_ALMOST_SYMPT) return 0;
     int ag = age_group_g(d, gp);
     if (ag >= CS_NUM_AGE_GROUPS) ag = CS_NUM_AGE_GROUPS - 1;
-    Counters* cn = d.cnt + r;
+    Counters* cn = cstage(d, r);
     d.inf[gp] = CS_CASE;
     const int adel = P->us_case_absenteeism_delay, adur = P->us_case_absenteeism_duration;
     if (host_absent(d, gp, ts)) {
@@ -316,7 +316,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) k
         if (q >= uni(d.cI[rc])) continue;
         const int r = rc / d.NC, c = rc % d.NC;
         const size_t rN = (size_t)r * d.N;
-        Counters* cn = d.cnt + r;
+        Counters* cn = cstage(d, r);
         const size_t sl = rN + uni(d.cstart[c] + d.cS[rc] + d.cL[rc]) + q;  // the person's list slot
         if (d.gate[sl] > ts) continue;  // every timer of the slot's person lies after this step
         const int p = d.perm[sl];

@@ -40,16 +40,14 @@ struct Counters {  // device mirror of cs_counters, updated with atomics
     unsigned long long cumHQ, cumAH, cumAPC, cumAPA, cumAPCS;
 };
 
-// DoInfect's counters as resolution stages them: CS_TALLY_COPIES copies of
-// each realisation's record (a block adds to copy blockIdx.x %
-// CS_TALLY_COPIES), each record TF_STRIDE words (three 128-byte lines), folded
-// into Counters after the resolution.  Every wave adds to its realisation's
-// few counters, and adds from the whole GPU to one line serialise at the
+// the per-transition counters of the infection and incubation phases are
+// staged: CS_TALLY_COPIES copies of each realisation's Counters (a block adds
+// to copy blockIdx.x % CS_TALLY_COPIES), folded into the realisation's
+// Counters at the end of each phase.  Every wave adds to its realisation's
+// few hot counters, and adds from the whole GPU to one line serialise at the
 // memory side.
 #define CS_TALLY_COPIES 16
-enum { TF_CUMI = 0, TF_TG, TF_SI, TF_NTG, TF_MAXR2, TF_SUMR2, TF_ITYPE,
-       TF_IA = TF_ITYPE + CS_INFECT_TYPE_MASK, TF_KW = TF_IA + CS_NUM_AGE_GROUPS, TF_N = TF_KW + 2, TF_STRIDE = 48 };
-static_assert(TF_N <= TF_STRIDE, "a staged record fits its stride");
+static_assert(sizeof(Counters) % 8 == 0, "Counters is a sequence of 8-byte words");
 
 // one realisation's copy of the P and AdUnits fields the trigger logic sets
 // (src/CovidSim.cpp:3485-3689); written at samples by one thread
@@ -296,7 +294,7 @@ struct Dev {
     unsigned* itv_n;      // counts: [0] det, [1] close_l, [2] carers queued by closures (in care_k)
     unsigned det_cap, close_cap;
     Counters* cnt;
-    unsigned long long* tally;  // [CS_TALLY_COPIES][R][TF_STRIDE] staged DoInfect counters (zero between steps)
+    Counters* cstage;     // [CS_TALLY_COPIES][R] staged counters (zero between phases)
     double* prev_prop;
     double* samp_prop;  // State.S / PopSize as of the last sample, per realisation
     double* loc_init;  // [R][2]

@@ -23,7 +23,7 @@ namespace cs {
 // bound on the spatial rejection loop (src/Sweep.cpp:753-854 has none)
 #define CS_MAX_ATTEMPTS (1 << 20)
 // spatial draws per work-list chunk (a wave's 64 lanes share them)
-#define CS_DRAW_CHUNK 256
+#define CS_DRAW_CHUNK 64
 // InvCDF entry flag: the bucket's draws all stop at the entry's cell
 #define CS_INV_PURE 0x80000000u
 
@@ -968,11 +968,51 @@ __global__ void __launch_bounds__(CS_SUM_BLOCK) k_cell_spatial_sum(Dev d)
     if (threadIdx.x == 0 && blk_inf) wtally(d.wc->n_inf, blk_inf);
 }
 
-// one spatial draw per lane; a wave takes a chunk of up to 64 draws of one
-// source cell (src/Sweep.cpp:712-855)
-// at most 80 VGPRs (6 waves per SIMD, no spills): each attempt is a chain of
-// dependent gathers, hidden only by waves in flight
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) k_spatial_draws(Dev d)
+// the position of the q-th set bit of m (q < popcount(m))
+__device__ __forceinline__ int nth_set(unsigned long long m, int q)
+{
+    int pos = 0, c;
+    c = __popc((unsigned)m);
+    if (q >= c) { q -= c; m >>= 32; pos += 32; }
+    c = __popc((unsigned)(m & 0xffffu));
+    if (q >= c) { q -= c; m >>= 16; pos += 16; }
+    c = __popc((unsigned)(m & 0xffu));
+    if (q >= c) { q -= c; m >>= 8; pos += 8; }
+    c = __popc((unsigned)(m & 0xfu));
+    if (q >= c) { q -= c; m >>= 4; pos += 4; }
+    c = __popc((unsigned)(m & 0x3u));
+    if (q >= c) { q -= c; m >>= 2; pos += 2; }
+    if (q >= (int)(m & 1u)) pos += 1;
+    return pos;
+}
+
+// the stream of (a, b, c) from its value number pos on (two values per
+// Philox block)
+__device__ __forceinline__ Stream stream_at(uint64_t key, uint32_t a, uint32_t b, uint32_t c, unsigned pos)
+{
+    Stream t(key, a, b, c);
+    t.c3 = pos >> 1;
+    if (pos & 1u) (void)t.next();
+    return t;
+}
+
+// the spatial draws of one source cell per chunk of up to CS_DRAW_CHUNK
+// draws, a wave per chunk (src/Sweep.cpp:712-855).  A draw takes its
+// infector from value 0 of its stream (no value when the cell has one
+// infectious person: v0 = 0, else 1) and makes attempts until one is
+// accepted; a rejected attempt always uses exactly three values (the
+// destination, the slot, the acceptance test), so attempt j reads values
+// v0 + 3j .. v0 + 3j + 2 whatever came before, and the draw's outcome is its
+// lowest-numbered accepted attempt.  Each round the wave's draws in progress
+// share its lanes, G = 64 / (draws in progress) lanes each, lane g of a
+// draw's group making attempt (attempts so far) + g: a long draw's chain of
+// dependent gathers is cut G-fold once most of the chunk's draws are over.
+// The group's first accepting lane finishes the draw (the susceptibility
+// test reads value v0 + 3j + 3).  The work tallies count the attempts the
+// sequential loop makes (those up to the accepted one).
+// at most 96 VGPRs (5 waves per SIMD, no spills): each attempt is a chain of
+// dependent gathers, hidden by the waves in flight and by the groups
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k_spatial_draws(Dev d)
 {
     const StepScalars sc = *d.sc;
     const unsigned nch = min(d.wl[WL_DRAWS], d.draw_cap);
@@ -989,75 +1029,105 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) k
         const size_t rN = (size_t)r * d.N;
         const size_t sbase = rN + uni(d.cstart[c]) + S + L;
         const uint64_t seed = uni(d.seed[r]);
-        // Each lane runs one draw's rejection loop (the attempts of a draw
-        // follow its own stream, so they stay with one lane); a lane whose
-        // draw is over takes the chunk's next draw, so no lane idles while
-        // a slow draw of the wave finishes.
-        int k = e.first + lane;      // this lane's draw (>= kend: none)
-        int next = e.first + 64;     // the chunk's next unassigned draw (wave-uniform)
+        const uint32_t sw0 = cw0(ts, PS_SP_DRAW);
+        const unsigned v0 = (I == 1) ? 0u : 1u;  // the first value of attempt 0
+        // the draw this lane owns (>= kend: none); a lane whose draw is over
+        // takes the chunk's next unassigned draw (next is wave-uniform)
+        int k = e.first + lane;
+        int next = e.first + 64;
         bool fresh = true;
-        Stream s(seed, 0, 0, 0);
-        int i = 0, hh_i = 0, att = 0, ag_i = 0;
-        bool closed = false, care_i = false;
+        int i = 0, hh_i = 0, att = 0, ag_i = 0, fl_i = 0;  // fl_i: 1 closed, 2 care home
         float2 loc_i = make_float2(0.f, 0.f);
         for (;;) {
             const bool active = k < kend;
-            if (__ballot(active) == 0) break;
+            const unsigned long long act = __ballot(active);
+            if (act == 0) break;
+            if (active && fresh) {  // the draw's infector (src/Sweep.cpp:716-742)
+                Stream s0(seed, sw0, (uint32_t)c, (uint32_t)k);
+                const int jj = (I == 1) ? 0 : select_infector(d.ci_buf + sbase, I, s0.next(), &w_probe);
+                fl_i = (d.ci_buf[sbase + jj] < 0) ? 1 : 0;
+                i = d.perm[sbase + jj];
+                const PStat pi = d.pst[i];
+                loc_i = pi.loc;
+                hh_i = pi.hh;
+                ag_i = (int)pi.age / CS_AGE_GROUP_WIDTH;  // the infector's, kept for every attempt
+                fl_i |= (pi.pflags & 1) ? 2 : 0;
+                att = 0;
+                fresh = false;
+            }
+            // draw q (the q-th lane of act) takes lanes [q*G, q*G + G)
+            const int nact = __popcll(act);
+            const int G = 64 / nact;
+            const int q = lane / G;
+            const bool inq = q < nact;
+            const int own = inq ? nth_set(act, q) : lane;
+            const int k_o = __shfl(k, own, 64), att_o = __shfl(att, own, 64), i_o = __shfl(i, own, 64);
+            const int hh_o = __shfl(hh_i, own, 64), ag_o = __shfl(ag_i, own, 64), fl_o = __shfl(fl_i, own, 64);
+            const float2 loc_o = make_float2(__shfl(loc_i.x, own, 64), __shfl(loc_i.y, own, 64));
+            const int j = att_o + (lane - q * G);  // this lane's attempt of the draw
+            const bool tries = inq && j < CS_MAX_ATTEMPTS;
+            bool accepted = false;
+            unsigned scanned = 0;
+            Stream t(seed, sw0, (uint32_t)c, (uint32_t)k_o);
+            int l = 0, slot = 0, m = 0;
+            PStat pm{};
+            int8_t inf_m = 0;
+            if (tries) {  // attempt j (src/Sweep.cpp:753-788)
+                t = stream_at(seed, sw0, (uint32_t)c, (uint32_t)k_o, v0 + 3u * (unsigned)j);
+                l = select_dest(d, rc, t.next(), &scanned);
+                const size_t rl = (size_t)r * d.NC + l;
+                const float mt = d.max_trans[(size_t)c * d.NC + l];  // issued with the slot's loads
+                slot = (int)(t.next() * ((double)d.S0[rl]));
+                m = d.perm[rN + d.cstart[l] + slot];
+                pm = d.pst[m];
+                inf_m = d.inf[rN + m];  // read with the candidate's record, not after the test
+                const double d2 = dist2_ll(P, pm.loc, loc_o);
+                const double acc = kernel_num(d, d2) / mt;
+                accepted = !((t.next() >= acc) || is_dead_code(inf_m));
+            }
+            // the first accepting lane of each group
+            const unsigned long long am = __ballot(accepted);
+            const unsigned long long gmask = (G == 64) ? ~0ull : ((1ull << G) - 1ull);
+            const unsigned long long ga = inq ? (am & (gmask << (q * G))) : 0ull;
+            const int first = ga ? __builtin_ctzll(ga) : 64;
+            if (tries && lane <= first) {
+                w_draw++;
+                w_scan += scanned;
+            }
+            if (accepted && lane == first && ((c != l) || (pm.hh != hh_o))) {  // src/Sweep.cpp:789-854
+                const size_t rl = (size_t)r * d.NC + l;
+                double SS = spatial_susc(d, r, m, ts);
+                if ((pm.pflags & 1) || (fl_o & 2)) SS *= P->care_home_resident_spatial_scaling;
+                if (slot < d.cS[rl]) {
+                    w_susc++;
+                    const int ag_m = (int)pm.age / CS_AGE_GROUP_WIDTH;
+                    SS *= person_susc_ag(d, rN + m, ag_m, ag_o, ts);
+                    SS *= P->waifw_spatial[ag_m][ag_o];
+                    if (!(fl_o & 1) && host_absent(d, rN + m, ts)) {
+                        bool f2 = false;
+                        for (int pt = 0; pt < d.NPT && !f2; pt++) {
+                            int lk = d.plinks[(size_t)m * CS_MAX_PLACE_TYPES + pt];
+                            if (lk >= 0) f2 = place_closed(d, r, pt, lk, ts);
+                        }
+                        if (f2) SS *= P->efficacies[2][CS_SPATIAL];
+                    }
+                    if ((SS == 1) || (t.next() < SS))
+                        if (inf_m == CS_SUSCEPTIBLE)
+                            add_event(d, ev_key(i_o, KP_SP, (uint32_t)k_o), rN + m, i_o,
+                                      2 + 2 * CS_MAX_PLACE_TYPES + CS_INFECT_TYPE_MASK * (1 + d.itype[rN + i_o] / CS_INFECT_TYPE_MASK));
+                }
+            }
+            // the owners: a draw is over once its group accepted, or its
+            // attempts ran out (no live candidate: report, never spin)
             bool done = false;
             if (active) {
-                if (fresh) {  // the draw's infector (src/Sweep.cpp:716-742)
-                    s = Stream(seed, cw0(ts, PS_SP_DRAW), (uint32_t)c, (uint32_t)k);
-                    const int jj = (I == 1) ? 0 : select_infector(d.ci_buf + sbase, I, s.next(), &w_probe);
-                    closed = (d.ci_buf[sbase + jj] < 0);
-                    i = d.perm[sbase + jj];
-                    const PStat pi = d.pst[i];
-                    loc_i = pi.loc;
-                    hh_i = pi.hh;
-                    ag_i = (int)pi.age / CS_AGE_GROUP_WIDTH;  // the infector's, kept for every attempt
-                    care_i = (pi.pflags & 1) != 0;
-                    att = 0;
-                    fresh = false;
-                }
-                if (att >= CS_MAX_ATTEMPTS) {  // no live candidate: report, never spin
-                    atomicOr(d.status, (unsigned)FLAG_STUCK);
-                    done = true;
-                } else {  // one attempt (src/Sweep.cpp:753-854)
-                    att++;
-                    w_draw++;
-                    int l = select_dest(d, rc, s.next(), &w_scan);
-                    const size_t rl = (size_t)r * d.NC + l;
-                    const float mt = d.max_trans[(size_t)c * d.NC + l];  // issued with the slot's loads
-                    int slot = (int)(s.next() * ((double)d.S0[rl]));
-                    int m = d.perm[rN + d.cstart[l] + slot];
-                    const PStat pm = d.pst[m];
-                    const int8_t inf_m = d.inf[rN + m];  // read with the candidate's record, not after the test
-                    double d2 = dist2_ll(P, pm.loc, loc_i);
-                    double acc = kernel_num(d, d2) / mt;
-                    if (!((s.next() >= acc) || is_dead_code(inf_m))) {
+                const int qm = __popcll(act & ((1ull << lane) - 1ull));
+                if (am & (gmask << (qm * G))) done = true;
+                else {
+                    att += G;
+                    if (att >= CS_MAX_ATTEMPTS) {
+                        atomicOr(d.status, (unsigned)FLAG_STUCK);
                         done = true;
-                        if ((c != l) || (pm.hh != hh_i)) {
-                            double SS = spatial_susc(d, r, m, ts);
-                            if ((pm.pflags & 1) || care_i) SS *= P->care_home_resident_spatial_scaling;
-                            if (slot < d.cS[rl]) {
-                                w_susc++;
-                                const int ag_m = (int)pm.age / CS_AGE_GROUP_WIDTH;
-                                SS *= person_susc_ag(d, rN + m, ag_m, ag_i, ts);
-                                SS *= P->waifw_spatial[ag_m][ag_i];
-                                if (!closed && host_absent(d, rN + m, ts)) {
-                                    bool f2 = false;
-                                    for (int pt = 0; pt < d.NPT && !f2; pt++) {
-                                        int lk = d.plinks[(size_t)m * CS_MAX_PLACE_TYPES + pt];
-                                        if (lk >= 0) f2 = place_closed(d, r, pt, lk, ts);
-                                    }
-                                    if (f2) SS *= P->efficacies[2][CS_SPATIAL];
-                                }
-                                if ((SS == 1) || (s.next() < SS))
-                                    if (inf_m == CS_SUSCEPTIBLE)
-                                        add_event(d, ev_key(i, KP_SP, (uint32_t)k), rN + m, i,
-                                                  2 + 2 * CS_MAX_PLACE_TYPES +
-                                                      CS_INFECT_TYPE_MASK * (1 + d.itype[rN + i] / CS_INFECT_TYPE_MASK));
-                            }
-                        }
                     }
                 }
             }

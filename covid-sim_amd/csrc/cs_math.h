@@ -266,7 +266,7 @@ __device__ __forceinline__ double log_fact(int32_t k)
 }
 
 // Poisson(mu): inversion for mu < 10, else PTRS (Hormann 1993)
-__device__ __noinline__ int32_t poisson(Stream& s, double mu)
+__device__ __forceinline__ int32_t poisson(Stream& s, double mu)
 {
     if (!(mu > 0.0)) return 0;
     if (mu < 10.0) {
@@ -312,7 +312,7 @@ __device__ __forceinline__ double normal(Stream& s)
 
 // gen_gamma_mt(beta = rate, alpha = shape): Marsaglia-Tsang with the
 // acceptance test of src/Rand.cpp:1025-1086
-__device__ __noinline__ double gamma(Stream& s, double beta, double alpha)
+__device__ __forceinline__ double gamma(Stream& s, double beta, double alpha)
 {
     double al = alpha >= 1.0 ? alpha : alpha + 1.0;
     double d = al - 1.0 / 3.0;

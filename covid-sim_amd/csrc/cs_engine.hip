@@ -244,11 +244,16 @@ int launch_infect_events(cs_engine* e)
     {
         ProfScope ps(e, CS_K_OTHER);
         cs::k_zero_counts<<<1, CS_EV_SHARDS * CS_EV_STRIDE, 0, st>>>(d.ev_count, CS_EV_SHARDS * CS_EV_STRIDE);
+        cs::k_zero_counts<<<1, CS_EV_SHARDS * CS_EV_STRIDE, 0, st>>>(d.pc_count, CS_EV_SHARDS * CS_EV_STRIDE);
         cs::k_zero_counts<<<1, 64, 0, st>>>(d.wl, cs::WL_N);
         cs::k_compact<<<grid_for(nrc, 256), 256, 0, st>>>(d, 1, 1);
     }
     { ProfScope ps(e, CS_K_INFECT_HOUSEHOLDS); cs::k_infect_households<<<e->grid_waves, 256, 0, st>>>(d); }
-    { ProfScope ps(e, CS_K_INFECT_PLACES); cs::k_infect_places<<<e->grid_waves, 256, 0, st>>>(d); }
+    {
+        ProfScope ps(e, CS_K_INFECT_PLACES);
+        cs::k_infect_places<<<e->grid_waves, 256, 0, st>>>(d);
+        cs::k_place_contacts<<<e->grid_waves, 256, 0, st>>>(d);
+    }
     { ProfScope ps(e, CS_K_CELL_SPATIAL_SUM); cs::k_cell_spatial_sum<<<e->grid_cells, CS_SUM_BLOCK, 0, st>>>(d); }
     { ProfScope ps(e, CS_K_OTHER); cs::k_compact<<<grid_for(nrc, 256), 256, 0, st>>>(d, 2, 0); }
     { ProfScope ps(e, CS_K_SPATIAL_DRAWS); cs::k_spatial_draws<<<e->grid_waves, 256, 0, st>>>(d); }
@@ -336,6 +341,25 @@ void drop_graph(cs_engine* e)
     e->gexec = nullptr;
 }
 
+// the place contact list: a quarter of the event queue's capacity (at least
+// 8192 entries), whole shards; sized afresh whenever the queue is
+int size_contacts(cs_engine* e)
+{
+    Dev& d = e->d;
+    unsigned cap = std::max(d.ev_cap / 4u, 8192u);
+    cap -= cap % CS_EV_SHARDS;
+    if (d.pcl && cap == d.pc_cap) return 0;
+    if (d.pcl) {
+        (void)hipFree(d.pcl);
+        e->real_allocs.erase(std::remove(e->real_allocs.begin(), e->real_allocs.end(), (void*)d.pcl), e->real_allocs.end());
+        d.pcl = nullptr;
+    }
+    int rc = dalloc(e, e->real_allocs, &d.pcl, cap);
+    if (rc) return rc;
+    d.pc_cap = cap;
+    return 0;
+}
+
 int grow_events(cs_engine* e)
 {
     Dev& d = e->d;
@@ -352,6 +376,7 @@ int grow_events(cs_engine* e)
     if ((rc = dalloc(e, e->real_allocs, &d.ev_infector, cap))) return rc;
     if ((rc = dalloc(e, e->real_allocs, &d.ev_type, cap))) return rc;
     d.ev_cap = cap;
+    if ((rc = size_contacts(e))) return rc;  // the place contact list shares the flag
     // the spatial draw list shares the overflow flag: grow it too
     const unsigned dcap = d.draw_cap * 2u > d.draw_cap ? d.draw_cap * 2u : 0xffffffffu;
     (void)hipFree(d.drawl);
@@ -807,6 +832,9 @@ int cs_alloc_realisations(cs_engine* e, int n_real, const uint64_t* seeds)
     cap -= cap % CS_EV_SHARDS;
     d.ev_cap = (unsigned)cap;
     AL(ev_key, cap); AL(ev_gp, cap); AL(ev_infector, cap); AL(ev_type, cap); AL(ev_count, CS_EV_SHARDS * CS_EV_STRIDE);
+    AL(pc_count, CS_EV_SHARDS * CS_EV_STRIDE);
+    d.pcl = nullptr;
+    d.pc_cap = 0;
     AL(clr, RN); AL(batch, RN); AL(bcount, RC);
     d.bind_clr(d.clr);
     AL(care_gp, RN); AL(care_k, RN); AL(care_abs, RN); AL(care_count, 1);
@@ -816,6 +844,7 @@ int cs_alloc_realisations(cs_engine* e, int n_real, const uint64_t* seeds)
     AL(due, (size_t)CS_DUE_RING * CS_DUE_SHARDS * d.due_scap); AL(due_n, CS_DUE_RING * CS_DUE_SHARDS * CS_DUE_STRIDE); AL(due_ovf, CS_DUE_RING);
     d.due_on = 0;
 #undef AL
+    if ((rc = size_contacts(e))) return rc;
     if ((rc = dalloc(e, A, &e->d_sel, R))) return rc;
     if ((rc = dalloc(e, A, &e->rbeta_buf, R))) return rc;
     if ((rc = dalloc(e, A, &e->rseed_buf, R))) return rc;
@@ -1149,6 +1178,7 @@ int cs_set_queue_capacity(cs_engine* e, uint32_t events, uint32_t draw_chunks)
             (rc = dalloc(e, e->real_allocs, &d.ev_infector, events)) || (rc = dalloc(e, e->real_allocs, &d.ev_type, events)))
             return rc;
         d.ev_cap = events;
+        if ((rc = size_contacts(e))) return rc;
     }
     if (draw_chunks) {
         (void)hipFree(d.drawl);

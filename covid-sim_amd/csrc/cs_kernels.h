@@ -774,26 +774,56 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) k
     }
 }
 
-// Within-group and between-group place passes of one infectious person per
-// lane (src/Sweep.cpp:381-640).  Floyd's sample is kept in LDS, one column
-// per thread (bank-conflict free).
-// at most 128 VGPRs (4 waves per SIMD): the kernel waits on dependent
-// gathers, and the extra wave in flight outweighs the few spilled registers
-// (at 5 waves the spills grow to 192 bytes per lane and the kernel slows)
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_infect_places(Dev d)
+// the position of the q-th set bit of m (q < popcount(m))
+__device__ __forceinline__ int nth_set(unsigned long long m, int q)
+{
+    int pos = 0, c;
+    c = __popc((unsigned)m);
+    if (q >= c) { q -= c; m >>= 32; pos += 32; }
+    c = __popc((unsigned)(m & 0xffffu));
+    if (q >= c) { q -= c; m >>= 16; pos += 16; }
+    c = __popc((unsigned)(m & 0xffu));
+    if (q >= c) { q -= c; m >>= 8; pos += 8; }
+    c = __popc((unsigned)(m & 0xfu));
+    if (q >= c) { q -= c; m >>= 4; pos += 4; }
+    c = __popc((unsigned)(m & 0x3u));
+    if (q >= c) { q -= c; m >>= 2; pos += 2; }
+    if (q >= (int)(m & 1u)) pos += 1;
+    return pos;
+}
+
+// the stream of (a, b, c) from its value number pos on (two values per
+// Philox block)
+__device__ __forceinline__ Stream stream_at(uint64_t key, uint32_t a, uint32_t b, uint32_t c, unsigned pos)
+{
+    Stream t(key, a, b, c);
+    t.c3 = pos >> 1;
+    if (pos & 1u) (void)t.next();
+    return t;
+}
+
+// Within-group and between-group place passes (src/Sweep.cpp:381-640) in two
+// kernels.  k_infect_places takes one infectious person per lane: for each
+// place link and pass, the capped infectiousness and the binomial number of
+// contacts; the rare links with contacts go to the place contact list with
+// the stream position after the count.  k_place_contacts takes one contact
+// list entry per lane: the members sampled (Floyd's sample in LDS, one
+// column per thread, or selection sampling) and their infection tests.  The
+// first kernel carries none of the second's state, so it runs with neither
+// LDS nor spills.
+__global__ void __launch_bounds__(256) k_infect_places(Dev d)
 {
     const StepScalars sc = *d.sc;
-    __shared__ int picked[CS_FLOYD_MAX][256];
     const unsigned nch = min(d.wl[WL_SEG], d.seg_cap);
     const cs_params* P = d.P;
     const unsigned ts = sc.ts;
-    const int lane = threadIdx.x & 63;
-    unsigned w_link = 0, w_cand = 0;  // per lane; summed per wave in 64 bits
+    unsigned w_link = 0;  // per lane; summed per wave in 64 bits
     if (!P->do_places) return;
+    const unsigned sh = blockIdx.x & (CS_EV_SHARDS - 1u), scap = d.pc_cap / CS_EV_SHARDS;
     for (unsigned ch = grid_wave(); ch < nch; ch += grid_nwaves()) {
         const Chunk e = chunk_at(d.segl, ch);
         const int rc = e.rc;
-        const int j = e.first + lane;
+        const int j = e.first + (int)(threadIdx.x & 63);
         if (j >= uni(d.cI[rc])) continue;
         const int r = rc / d.NC, c = rc % d.NC;
         const uint64_t seed = uni(d.seed[r]);
@@ -801,94 +831,143 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
         const size_t sbase = rN + uni(d.cstart[c] + d.cS[rc] + d.cL[rc]);  // slot of infectious index 0
         const int i = d.perm[sbase + j];
         const size_t gi = rN + i;
+        if ((d.hflags[gi] & HOT_NO_LINKS) || host_absent(d, gi, ts)) continue;
+        for (int pt = 0; pt < d.NPT; pt++) {
+            const int lk = d.plinks[(size_t)i * CS_MAX_PLACE_TYPES + pt];
+            if (lk < 0) continue;
+            w_link++;
+            double PI = sc.fp * sc.seas * place_inf(d, r, i, pt, ts);
+            const int mstart = d.pl_ms[pt][lk];
+            for (int pass = 0; pass < 2; pass++) {
+                // Pc is both the capped place infectiousness and the
+                // sampling probability: the reference keeps two copies that
+                // differ only under digital contact tracing
+                // (src/Sweep.cpp:425-447), which this path does not serve
+                int gsize, goff;
+                double Pc;
+                if (pass == 0) {
+                    if (pt == P->hotel_place_type) continue;
+                    const int g2 = d.pglinks[(size_t)i * CS_MAX_PLACE_TYPES + pt];
+                    Pc = PI;
+                    if (Pc > 1) Pc = 1;
+                    const int gix = d.pl_gs0[pt][lk] + g2;
+                    goff = d.pl_gst[pt][gix];
+                    gsize = d.pl_gsz[pt][gix];
+                } else {
+                    PI *= P->place_type_prop_between_group_links[pt] * P->place_type_group_size_param1[pt] / ((double)d.pl_n[pt][lk]);
+                    if (PI > 1) PI = 1;
+                    Pc = PI;
+                    goff = 0;
+                    gsize = d.pl_n[pt][lk];
+                }
+                Stream s(seed, cw0(ts, pass == 0 ? PS_PW_SAMPLE : PS_PB_SAMPLE), (uint32_t)i, (uint32_t)pt);
+                const int n = (Pc >= 1) ? gsize : binomial(s, gsize, Pc);
+                if (n <= 0) continue;
+                // values used: two per Philox block, less the buffered one
+                const unsigned v = 2u * s.c3 - (s.idx == 1 ? 1u : 0u);
+                const unsigned idx = atomicAdd(&d.pc_count[sh * CS_EV_STRIDE], 1u);
+                if (idx < scap)
+                    d.pcl[(size_t)sh * scap + idx] = PlContact{r, i, n, gsize, goff, mstart, Pc, v, (uint8_t)pt, (uint8_t)pass, 0};
+                else
+                    atomicOr(d.status, (unsigned)FLAG_OVERFLOW);
+            }
+        }
+    }
+    const unsigned long long s_link = wave_sum(w_link);
+    if ((threadIdx.x & 63) == 0 && s_link) wtally(d.wc->n_place_link, s_link);
+}
+
+// a sharded list as one sequence: pre[s] entries lie in the shards before s
+// (the counts capped at the shard size, as the appends are); constant
+// indices only, so pre stays in registers
+struct ShardSeq {
+    unsigned pre[CS_EV_SHARDS + 1], scap;
+    __device__ __forceinline__ ShardSeq(const unsigned* count, unsigned cap) : scap(cap / CS_EV_SHARDS)
+    {
+        pre[0] = 0;
+        for (int s = 0; s < CS_EV_SHARDS; s++) pre[s + 1] = pre[s] + min(count[s * CS_EV_STRIDE], scap);
+    }
+    __device__ __forceinline__ unsigned n() const { return pre[CS_EV_SHARDS]; }
+    __device__ __forceinline__ size_t slot(unsigned k) const
+    {
+        unsigned sh = 0, base = 0;
+#pragma unroll
+        for (int j = 1; j < CS_EV_SHARDS; j++)
+            if (k >= pre[j]) { sh = (unsigned)j; base = pre[j]; }
+        return (size_t)sh * scap + (k - base);
+    }
+};
+
+// the members sampled for one place contact list entry per lane and their
+// infection tests (src/Sweep.cpp:449-640); the pass's stream resumes where
+// the count left it
+__global__ void __launch_bounds__(256) k_place_contacts(Dev d)
+{
+    if (halted(d)) return;
+    const StepScalars sc = *d.sc;
+    __shared__ int picked[CS_FLOYD_MAX][256];
+    const cs_params* P = d.P;
+    const unsigned ts = sc.ts;
+    const ShardSeq q(d.pc_count, d.pc_cap);
+    const unsigned nq = q.n();
+    unsigned w_cand = 0;
+    for (unsigned k = blockIdx.x * blockDim.x + threadIdx.x; k < nq; k += gridDim.x * blockDim.x) {
+        const PlContact w = d.pcl[q.slot(k)];
+        const int r = w.r, i = w.i, pt = w.pt, pass = w.pass, gsize = w.gsize, goff = w.goff, mstart = w.mstart;
+        int n = w.n;
+        const double Pc = w.Pc;
+        const size_t rN = (size_t)r * d.N, gi = rN + i;
+        const uint64_t seed = d.seed[r];
         const int gen = d.itype[gi] / CS_INFECT_TYPE_MASK;
-        // places (src/Sweep.cpp:381-640)
-        if (P->do_places && !(d.hflags[gi] & HOT_NO_LINKS) && !host_absent(d, gi, ts)) {
-            for (int pt = 0; pt < d.NPT; pt++) {
-                const int lk = d.plinks[(size_t)i * CS_MAX_PLACE_TYPES + pt];
-                if (lk < 0) continue;
-                w_link++;
-                double PI = sc.fp * sc.seas * place_inf(d, r, i, pt, ts);
-                const int mstart = d.pl_ms[pt][lk];
-                for (int pass = 0; pass < 2; pass++) {
-                    // Pc is both the capped place infectiousness and the
-                    // sampling probability: the reference keeps two copies
-                    // that differ only under digital contact tracing
-                    // (src/Sweep.cpp:425-447), which this path does not serve
-                    int gsize, goff;
-                    double Pc;
-                    if (pass == 0) {
-                        if (pt == P->hotel_place_type) continue;
-                        const int g2 = d.pglinks[(size_t)i * CS_MAX_PLACE_TYPES + pt];
-                        Pc = PI;
-                        if (Pc > 1) Pc = 1;
-                        const int gix = d.pl_gs0[pt][lk] + g2;
-                        goff = d.pl_gst[pt][gix];
-                        gsize = d.pl_gsz[pt][gix];
-                    } else {
-                        PI *= P->place_type_prop_between_group_links[pt] * P->place_type_group_size_param1[pt] / ((double)d.pl_n[pt][lk]);
-                        if (PI > 1) PI = 1;
-                        Pc = PI;
-                        goff = 0;
-                        gsize = d.pl_n[pt][lk];
-                    }
-                    Stream s(seed, cw0(ts, pass == 0 ? PS_PW_SAMPLE : PS_PB_SAMPLE), (uint32_t)i, (uint32_t)pt);
-                    int n = (Pc >= 1) ? gsize : binomial(s, gsize, Pc);
-                    if (n <= 0) continue;
-                    const bool all = n >= gsize;
-                    const bool floyd = !all && n <= CS_FLOYD_MAX;
-                    if (floyd) {
-                        int mm = 0;
-                        for (int j2 = gsize - n; j2 < gsize; j2++) {
-                            int tt = (int)(s.next() * (double)(j2 + 1));
-                            bool dup = false;
-                            for (int q = 0; q < mm; q++) dup |= (picked[q][threadIdx.x] == tt);
-                            picked[mm++][threadIdx.x] = dup ? j2 : tt;
-                        }
-                    }
-                    if (all) n = gsize;
-                    int sel = 0;
-                    for (int cidx = 0, scan = 0; cidx < n; cidx++) {
-                        int idx;
-                        if (all) idx = cidx;
-                        else if (floyd) idx = picked[cidx][threadIdx.x];
-                        else {
-                            // selection sampling over [scan, gsize)
-                            for (;;) {
-                                double u = s.next();
-                                int cur = scan++;
-                                if ((double)(gsize - cur) * u < (double)(n - sel)) { idx = cur; sel++; break; }
-                            }
-                        }
-                        const int m = d.pl_mem[pt][mstart + goff + idx];
-                        w_cand++;
-                        double PS = place_susc(d, r, m, pt, ts);
-                        if (pass == 1 && care_g(d, gi) && care_g(d, rN + m) &&
-                            d.pglinks[(size_t)i * CS_MAX_PLACE_TYPES + pt] != d.pglinks[(size_t)m * CS_MAX_PLACE_TYPES + pt])
-                            PS *= P->care_home_resident_place_scaling;
-                        if ((pt == P->care_home_place_type) && ((!care_g(d, gi)) || (!care_g(d, rN + m)))) PS *= P->care_home_worker_group_scaling;
-                        const size_t gm = rN + m;
-                        if (d.inf[gm] == CS_SUSCEPTIBLE && !host_absent(d, gm, ts)) {
-                            PS *= person_susc(d, r, m, ts, i) * Pc / Pc;  // (susc * Pc) / Pc, rounded as the reference does
-                            Stream s2(seed, cw0(ts, pass == 0 ? PS_PW_ACCEPT : PS_PB_ACCEPT), (uint32_t)i, ((uint32_t)pt << 24) | (uint32_t)cidx);
-                            if ((PS == 1) || (s2.next() < PS)) {
-                                if (pass == 0)
-                                    add_event(d, ev_key(i, KP_PW + pt, (uint32_t)cidx), gm, i, 2 + pt + CS_INFECT_TYPE_MASK * (1 + gen));
-                                else
-                                    add_event(d, ev_key(i, KP_PB + pt, (uint32_t)cidx), gm, i,
-                                              2 + pt + CS_MAX_PLACE_TYPES + CS_INFECT_TYPE_MASK * (1 + gen));
-                            }
-                        }
-                    }
+        Stream s = stream_at(seed, cw0(ts, pass == 0 ? PS_PW_SAMPLE : PS_PB_SAMPLE), (uint32_t)i, (uint32_t)pt, w.v);
+        const bool all = n >= gsize;
+        const bool floyd = !all && n <= CS_FLOYD_MAX;
+        if (floyd) {
+            int mm = 0;
+            for (int j2 = gsize - n; j2 < gsize; j2++) {
+                int tt = (int)(s.next() * (double)(j2 + 1));
+                bool dup = false;
+                for (int q2 = 0; q2 < mm; q2++) dup |= (picked[q2][threadIdx.x] == tt);
+                picked[mm++][threadIdx.x] = dup ? j2 : tt;
+            }
+        }
+        if (all) n = gsize;
+        int sel = 0;
+        for (int cidx = 0, scan = 0; cidx < n; cidx++) {
+            int idx;
+            if (all) idx = cidx;
+            else if (floyd) idx = picked[cidx][threadIdx.x];
+            else {
+                // selection sampling over [scan, gsize)
+                for (;;) {
+                    double u = s.next();
+                    int cur = scan++;
+                    if ((double)(gsize - cur) * u < (double)(n - sel)) { idx = cur; sel++; break; }
+                }
+            }
+            const int m = d.pl_mem[pt][mstart + goff + idx];
+            w_cand++;
+            double PS = place_susc(d, r, m, pt, ts);
+            if (pass == 1 && care_g(d, gi) && care_g(d, rN + m) &&
+                d.pglinks[(size_t)i * CS_MAX_PLACE_TYPES + pt] != d.pglinks[(size_t)m * CS_MAX_PLACE_TYPES + pt])
+                PS *= P->care_home_resident_place_scaling;
+            if ((pt == P->care_home_place_type) && ((!care_g(d, gi)) || (!care_g(d, rN + m)))) PS *= P->care_home_worker_group_scaling;
+            const size_t gm = rN + m;
+            if (d.inf[gm] == CS_SUSCEPTIBLE && !host_absent(d, gm, ts)) {
+                PS *= person_susc(d, r, m, ts, i) * Pc / Pc;  // (susc * Pc) / Pc, rounded as the reference does
+                Stream s2(seed, cw0(ts, pass == 0 ? PS_PW_ACCEPT : PS_PB_ACCEPT), (uint32_t)i, ((uint32_t)pt << 24) | (uint32_t)cidx);
+                if ((PS == 1) || (s2.next() < PS)) {
+                    if (pass == 0)
+                        add_event(d, ev_key(i, KP_PW + pt, (uint32_t)cidx), gm, i, 2 + pt + CS_INFECT_TYPE_MASK * (1 + gen));
+                    else
+                        add_event(d, ev_key(i, KP_PB + pt, (uint32_t)cidx), gm, i,
+                                  2 + pt + CS_MAX_PLACE_TYPES + CS_INFECT_TYPE_MASK * (1 + gen));
                 }
             }
         }
     }
-    const unsigned long long s_link = wave_sum(w_link), s_cand = wave_sum(w_cand);
-    if ((threadIdx.x & 63) == 0 && (s_link | s_cand)) {
-        wtally(d.wc->n_place_link, s_link);
-        wtally(d.wc->n_place_cand, s_cand);
-    }
+    const unsigned long long s_cand = wave_sum(w_cand);
+    if ((threadIdx.x & 63) == 0 && s_cand) wtally(d.wc->n_place_cand, s_cand);
 }
 
 // Per (realisation, cell): the cumulative spatial infectiousness by slot,
@@ -966,34 +1045,6 @@ __global__ void __launch_bounds__(CS_SUM_BLOCK) k_cell_spatial_sum(Dev d)
     if (lane == 0 && w_inf) atomicAdd(&blk_inf, w_inf);
     __syncthreads();
     if (threadIdx.x == 0 && blk_inf) wtally(d.wc->n_inf, blk_inf);
-}
-
-// the position of the q-th set bit of m (q < popcount(m))
-__device__ __forceinline__ int nth_set(unsigned long long m, int q)
-{
-    int pos = 0, c;
-    c = __popc((unsigned)m);
-    if (q >= c) { q -= c; m >>= 32; pos += 32; }
-    c = __popc((unsigned)(m & 0xffffu));
-    if (q >= c) { q -= c; m >>= 16; pos += 16; }
-    c = __popc((unsigned)(m & 0xffu));
-    if (q >= c) { q -= c; m >>= 8; pos += 8; }
-    c = __popc((unsigned)(m & 0xfu));
-    if (q >= c) { q -= c; m >>= 4; pos += 4; }
-    c = __popc((unsigned)(m & 0x3u));
-    if (q >= c) { q -= c; m >>= 2; pos += 2; }
-    if (q >= (int)(m & 1u)) pos += 1;
-    return pos;
-}
-
-// the stream of (a, b, c) from its value number pos on (two values per
-// Philox block)
-__device__ __forceinline__ Stream stream_at(uint64_t key, uint32_t a, uint32_t b, uint32_t c, unsigned pos)
-{
-    Stream t(key, a, b, c);
-    t.c3 = pos >> 1;
-    if (pos & 1u) (void)t.next();
-    return t;
 }
 
 // the spatial draws of one source cell per chunk of up to CS_DRAW_CHUNK
@@ -1152,32 +1203,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k
     }
 }
 
-// the queue's events as one sequence: pre[s] events lie in the shards
-// before s (the counts are capped at the shard size, as the appends are)
-struct EvSeq {
-    unsigned pre[CS_EV_SHARDS + 1], scap;
-    __device__ __forceinline__ explicit EvSeq(const Dev& d) : scap(d.ev_cap / CS_EV_SHARDS)
-    {
-        pre[0] = 0;
-        for (int s = 0; s < CS_EV_SHARDS; s++) pre[s + 1] = pre[s] + min(d.ev_count[s * CS_EV_STRIDE], scap);
-    }
-    __device__ __forceinline__ unsigned n() const { return pre[CS_EV_SHARDS]; }
-    // the queue slot of the k-th event (k < n()); constant indices only, so
-    // pre stays in registers (a dynamically indexed copy would go to LDS)
-    __device__ __forceinline__ size_t slot(unsigned k) const
-    {
-        unsigned sh = 0, base = 0;
-#pragma unroll
-        for (int j = 1; j < CS_EV_SHARDS; j++)
-            if (k >= pre[j]) { sh = (unsigned)j; base = pre[j]; }
-        return (size_t)sh * scap + (k - base);
-    }
-};
-
 __global__ void k_resolve_claim(Dev d)
 {
     if (halted(d)) return;
-    const EvSeq q(d);
+    const ShardSeq q(d.ev_count, d.ev_cap);
     const unsigned n = q.n();
     for (unsigned k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
         const size_t e = q.slot(k);
@@ -1192,7 +1221,7 @@ __global__ void k_resolve_apply(Dev d)
 {
     if (halted(d)) return;
     const StepScalars sc = *d.sc;
-    const EvSeq q(d);
+    const ShardSeq q(d.ev_count, d.ev_cap);
     const unsigned n = q.n();
     const int lane = threadIdx.x & 63;
     unsigned long long nn = 0;
@@ -1297,7 +1326,7 @@ __global__ void __launch_bounds__(64) k_cnt_fold(Dev d)
 __global__ void k_resolve_reset(Dev d)
 {
     if (halted(d)) return;
-    const EvSeq q(d);
+    const ShardSeq q(d.ev_count, d.ev_cap);
     const unsigned n = q.n();
     for (unsigned k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
         size_t gp = (size_t)d.ev_gp[q.slot(k)];

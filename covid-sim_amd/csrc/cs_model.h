@@ -140,6 +140,20 @@ __host__ __device__ __forceinline__ bool no_place_links(const int32_t* lk, int n
 #define CS_EV_SHARDS 8
 #define CS_EV_STRIDE 32
 
+// a place link of the place passes whose binomial number of contacts is
+// positive (src/Sweep.cpp:425-449), with what the member sampling needs:
+// the group, the capped infectiousness and the number of values of the
+// pass's stream the count used.  The list is CS_EV_SHARDS shards, as the
+// event queue.
+struct PlContact {
+    int32_t r, i, n, gsize, goff, mstart;
+    double Pc;
+    uint32_t v;
+    uint8_t pt, pass;
+    uint16_t pad;
+};
+static_assert(sizeof(PlContact) == 40, "PlContact layout");
+
 struct Hot {  // per (realisation, person): 16 bytes, one vector load
     int8_t inf;       // InfStat code
     uint8_t flags;    // HOT_HH_DONE, HOT_NO_LINKS, household size << HOT_NH_SHIFT
@@ -294,6 +308,9 @@ struct Dev {
     unsigned* itv_n;      // counts: [0] det, [1] close_l, [2] carers queued by closures (in care_k)
     unsigned det_cap, close_cap;
     Counters* cnt;
+    PlContact* pcl;       // [pc_cap] place contacts of this step, CS_EV_SHARDS shards
+    unsigned* pc_count;   // [CS_EV_SHARDS * CS_EV_STRIDE]
+    unsigned pc_cap;
     Counters* cstage;     // [CS_TALLY_COPIES][R] staged counters (zero between phases)
     double* prev_prop;
     double* samp_prop;  // State.S / PopSize as of the last sample, per realisation

@@ -666,7 +666,13 @@ int cs_set_population(cs_engine* e, const cs_population* pop)
             return set_err(e, CS_EINVAL, "household %d out of range", h);
     e->flags0.assign(N, 0);
     e->h_age.assign(pop->age, pop->age + N);
-    e->h_flags.assign(pop->flags, pop->flags + N);
+    e->h_flags.assign(N, 0);
+    for (int i = 0; i < N; i++) {  // the person flags with the household position (cs::PF_*)
+        const int h = pop->hh[i], pos = i - pop->hh_first[h];
+        if (pos < 0 || pos >= pop->hh_nh[h]) return set_err(e, CS_EINVAL, "person %d lies outside its household's member range", i);
+        e->h_flags[i] = (uint8_t)((pop->flags[i] & cs::PF_INPUT) | (pos << cs::PF_HPOS_SHIFT) |
+                                  (pop->hh_nhr[h] != pop->hh_nh[h] ? cs::PF_NHR_DIFFERS : 0));
+    }
     for (int i = 0; i < N; i++)
         e->flags0[i] = (uint8_t)((no_place_links_host(pop->place_links + (size_t)i * CS_MAX_PLACE_TYPES, e->NPT) ? cs::HOT_NO_LINKS : 0) |
                                  (pop->hh_nh[pop->hh[i]] << cs::HOT_NH_SHIFT));
@@ -689,7 +695,7 @@ int cs_set_population(cs_engine* e, const cs_population* pop)
     {
         std::vector<cs::PStat> ps(N);
         for (int i = 0; i < N; i++)
-            ps[i] = cs::PStat{pop->hh[i], pop->age[i], pop->flags[i], 0,
+            ps[i] = cs::PStat{pop->hh[i], pop->age[i], e->h_flags[i], 0,
                               make_float2(pop->hh_loc[2 * (size_t)pop->hh[i]], pop->hh_loc[2 * (size_t)pop->hh[i] + 1])};
         std::vector<cs::HRec> hr(e->NH);
         for (int h = 0; h < e->NH; h++) hr[h] = cs::HRec{pop->hh_first[h], pop->hh_nh[h], pop->hh_nhr[h]};

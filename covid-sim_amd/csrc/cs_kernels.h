@@ -720,7 +720,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) k
                 int f0 = 0;
                 if (!(fl & HOT_HH_DONE)) {
                     w_hh_scan += nh;
-                    f0 = d.hfirst[d.hh[i]];
+                    f0 = i - (int)((d.hpfl[gi] >> PF_HPOS_SHIFT) & PF_HPOS_MASK);  // the household's first member
                     for (int m = f0; m < f0 + nh; m++)
                         if (d.inf[rN + m] == CS_SUSCEPTIBLE) smask |= 1u << (m - f0);
                     if (!smask) d.hflags[gi] = (uint8_t)(fl | HOT_HH_DONE);

@@ -171,6 +171,15 @@ struct Tim {  // per (realisation, person): the disease timers and severity, 16 
 struct ClaimRec {  // per (realisation, person): this step's smallest and largest event keys
     unsigned long long claim, lastk;
 };
+// PStat.pflags (copied into Hot.pflags): bit 0 care-home resident, bit 1 key
+// worker, bit 2 essential (the population's flags; its bit 3, the contact
+// tracing app user, has no use here, digital contact tracing being refused
+// at the boundary), bits 3-6 the person's
+// position in its household (its first member is p - position), bit 7 the
+// household's resident count (nhr) differs from its size (else the size
+// in Hot.flags gives it)
+enum { PF_INPUT = 7, PF_HPOS_SHIFT = 3, PF_HPOS_MASK = 15, PF_NHR_DIFFERS = 0x80 };
+static_assert(CS_MAX_HOUSEHOLD_SIZE <= PF_HPOS_MASK + 1, "a household position fits its 4 bits");
 struct PStat {  // per person, shared by all realisations
     int32_t hh;
     uint8_t age, pflags;
@@ -555,6 +564,13 @@ __device__ __forceinline__ double person_inf(const Dev& d, size_t gp, unsigned t
          * inf_profile(d, gp, ts);
 }
 
+// the resident count of p's household: its size (in p's Hot record) unless
+// the flags say otherwise
+__device__ __forceinline__ int hh_nhr(const Dev& d, size_t gp, int p)
+{
+    return (d.hpfl[gp] & PF_NHR_DIFFERS) ? (int)d.hnhr[d.hh[p]] : (int)(d.hflags[gp] >> HOT_NH_SHIFT);
+}
+
 __device__ __forceinline__ double house_inf(const Dev& d, int r, int p, unsigned ts)
 {
     const cs_params* P = d.P;
@@ -563,7 +579,7 @@ __device__ __forceinline__ double house_inf(const Dev& d, int r, int p, unsigned
     return ((iso && !dc) ? P->efficacies[0][CS_HOUSE] : 1.0)
          * (dc ? P->efficacies[5][CS_HOUSE] : 1.0)
          * ((qu && !dc && !iso) ? P->efficacies[1][CS_HOUSE] : 1.0)
-         * P->household_denom_lookup[d.hnhr[d.hh[p]] - 1]
+         * P->household_denom_lookup[hh_nhr(d, gp, p) - 1]
          * (care_g(d, gp) ? P->care_home_resident_household_scaling : 1.0)
          * (treated(d, gp, ts) ? P->treat_inf_drop : 1.0)
          * (vacced(d, gp, ts) ? P->vacc_inf_drop : 1.0)

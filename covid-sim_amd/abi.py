@@ -109,6 +109,7 @@ def declare(lib, prefix):
         "alloc_realisations": (I, [vp, I, P(C.c_uint64)]),
         "reset_realisations": (I, [vp, I]),
         "set_realisation_params": (I, [vp, P(D), P(C.c_int32)]),
+        "set_realisation_triggers": (I, [vp, P(D)]),
         "load_state": (I, [vp, I, P(cs_state)]),
         "store_state": (I, [vp, I, P(cs_state)]),
         "infect_sweep": (I, [vp, D]),

@@ -33,6 +33,7 @@ struct cs_engine {
     bool adunits_valid = false;      // every person's administrative unit lies in [0, n_adunits)
     double* rbeta_buf = nullptr;     // [R] storage behind Dev.rbeta
     int32_t* rseed_buf = nullptr;    // [R] storage behind Dev.rseed_n
+    double* rthr_buf = nullptr;      // [R] storage behind Dev.rthr
     int ts_max = 0, ts_n = 0;
     std::vector<uint8_t> flags0;     // [N] each person's Hot flags after a reset (no links, household size), for loaded states
     std::vector<uint8_t> h_age, h_flags;  // [N] the copies a loaded state's Hot records take
@@ -43,6 +44,8 @@ struct cs_engine {
     // replayed step graphs (cs_run_steps): one graph per output step
     bool use_graphs = true;
     hipGraphExec_t gexec = nullptr;
+    hipGraph_t graph = nullptr;      // the captured graph behind gexec (a batch embeds it)
+    unsigned graph_gen = 0;          // captures so far: a batch rebuilds when it changes
     int graph_sps = 0;
 
     // profiling: event pairs per kernel kind, resolved on cs_get_profile
@@ -338,7 +341,10 @@ int launch_update_probs(cs_engine* e, const int* sel)
 void drop_graph(cs_engine* e)
 {
     if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
+    if (e->graph) (void)hipGraphDestroy(e->graph);
     e->gexec = nullptr;
+    e->graph = nullptr;
+    e->graph_gen++;
 }
 
 // the place contact list: a quarter of the event queue's capacity (at least
@@ -454,11 +460,13 @@ int capture_graph(cs_engine* e, int sps)
     }
     if (st != hipSuccess) return set_err(e, CS_EHIP, "step graph capture: %s", hipGetErrorString(st));
     st = hipGraphInstantiate(&e->gexec, g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
     if (st != hipSuccess) {
+        (void)hipGraphDestroy(g);
         e->gexec = nullptr;
         return set_err(e, CS_EHIP, "step graph instantiation: %s", hipGetErrorString(st));
     }
+    e->graph = g;
+    e->graph_gen++;
     e->graph_sps = sps;
     return 0;
 }
@@ -854,8 +862,10 @@ int cs_alloc_realisations(cs_engine* e, int n_real, const uint64_t* seeds)
     if ((rc = dalloc(e, A, &e->d_sel, R))) return rc;
     if ((rc = dalloc(e, A, &e->rbeta_buf, R))) return rc;
     if ((rc = dalloc(e, A, &e->rseed_buf, R))) return rc;
+    if ((rc = dalloc(e, A, &e->rthr_buf, R))) return rc;
     d.rbeta = nullptr;
     d.rseed_n = nullptr;
+    d.rthr = nullptr;
     cs::k_claim_init<<<grid_for((long long)RN, 256), 256, 0, e->stream>>>(d.clr, RN);
     HIPCHK(e, hipGetLastError());
     HIPCHK(e, hipMemset(d.bcount, 0, RC * 4));
@@ -888,6 +898,20 @@ int cs_set_realisation_params(cs_engine* e, const double* local_beta, const int3
     d.rbeta = local_beta ? e->rbeta_buf : nullptr;
     d.rseed_n = n_seed ? e->rseed_buf : nullptr;
     drop_graph(e);  // the captured launches hold the old pointers
+    return 0;
+}
+
+int cs_set_realisation_triggers(cs_engine* e, const double* threshold)
+{
+    if (!e) return CS_EINVAL;
+    if (e->R == 0) return set_err(e, CS_ESTATE, "no realisations allocated");
+    for (int r = 0; threshold && r < e->R; r++)
+        if (!(threshold[r] >= 0)) return set_err(e, CS_EINVAL, "threshold[%d] must be >= 0", r);
+    HIPCHK(e, hipSetDevice(e->device));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    if (threshold) HIPCHK(e, hipMemcpy(e->rthr_buf, threshold, (size_t)e->R * 8, hipMemcpyHostToDevice));
+    e->d.rthr = threshold ? e->rthr_buf : nullptr;
+    drop_graph(e);  // the captured launches hold the old pointer
     return 0;
 }
 
@@ -1090,7 +1114,9 @@ int cs_update_probs(cs_engine* e)
     return 0;
 }
 
-int cs_run_steps(cs_engine* e, double t0, int n_steps, int sps)
+// a run's set-up (cs_run_steps and batches): the first sample of a new
+// series, room for the run's samples, the device clock at t0
+static int run_prepare(cs_engine* e, double t0, int n_steps, int sps)
 {
     if (!e || n_steps < 0 || sps <= 0) return CS_EINVAL;
     if (e->R == 0) return set_err(e, CS_ESTATE, "no realisations allocated");
@@ -1124,6 +1150,36 @@ int cs_run_steps(cs_engine* e, double t0, int n_steps, int sps)
     }
     const cs::Clock c0 = {t0, 0, e->ts_n};
     HIPCHK(e, hipMemcpyAsync(d.clock, &c0, sizeof(c0), hipMemcpyHostToDevice, e->stream));
+    return 0;
+}
+
+// after replays that ended in the halt an overflow causes (the device clock
+// at c): grow the lists and finish the run's steps to `upto` directly
+static int run_recover(cs_engine* e, const cs::Clock& c, int upto, int sps)
+{
+    int rc;
+    if ((rc = grow_events(e))) return rc;
+    return run_direct(e, c.t, c.step, upto, sps);
+}
+
+// the run's last steps (fewer than an output step, or all of them without
+// replays) from step `done` at time t, and the series' sample count
+static int run_finish(cs_engine* e, double t, int done, int n_steps, int sps)
+{
+    int rc;
+    if ((rc = run_direct(e, t, done, n_steps, sps))) return rc;
+    cs::Clock c;
+    if ((rc = read_clock(e, &c))) return rc;
+    e->ts_n = c.sample;
+    HIPCHK(e, hipGetLastError());
+    return check_flags(e);
+}
+
+int cs_run_steps(cs_engine* e, double t0, int n_steps, int sps)
+{
+    int rc = run_prepare(e, t0, n_steps, sps);
+    if (rc) return rc;
+    Dev& d = e->d;
     const int periods = n_steps / sps;
     int done = 0;
     double t = t0;
@@ -1141,9 +1197,8 @@ int cs_run_steps(cs_engine* e, double t0, int n_steps, int sps)
                 // the replay halted at step c.step: grow the lists, finish that
                 // output step directly, then replay again
                 guard++;
-                if ((rc = grow_events(e))) return rc;
                 const int end = (c.step / sps + 1) * sps;
-                if ((rc = run_direct(e, c.t, c.step, end, sps))) return rc;
+                if ((rc = run_recover(e, c, end, sps))) return rc;
                 p = end / sps;
                 if ((rc = capture_graph(e, sps))) return rc;
                 continue;
@@ -1157,12 +1212,7 @@ int cs_run_steps(cs_engine* e, double t0, int n_steps, int sps)
         done = c.step;
         t = c.t;
     }
-    if ((rc = run_direct(e, t, done, n_steps, sps))) return rc;
-    cs::Clock c;
-    if ((rc = read_clock(e, &c))) return rc;
-    e->ts_n = c.sample;
-    HIPCHK(e, hipGetLastError());
-    return check_flags(e);
+    return run_finish(e, t, done, n_steps, sps);
 }
 
 int cs_set_queue_capacity(cs_engine* e, uint32_t events, uint32_t draw_chunks)
@@ -1710,6 +1760,173 @@ int cs_sample(cs_engine* e, int which, uint64_t key, uint32_t c0, uint32_t c1, d
     if (n) cs::k_t_sample<<<grid_for(n, 256, 1u << 30), 256, 0, e->stream>>>(which, key, c0, c1, a, b, o.p, n);
     HIPCHK(e, hipStreamSynchronize(e->stream));
     HIPCHK(e, hipMemcpy(out, o.p, (size_t)n * 8, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------ batches
+// engines of one device (a region each: its own population, parameters and
+// realisations) advanced together: an output step of the batch is one
+// launch of one graph whose branches are the engines' captured step graphs
+
+struct cs_batch {
+    std::vector<cs_engine*> eng;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    std::vector<unsigned> gens;  // the engines' capture generations the graph holds
+    int sps = 0;
+    char err[512] = {0};
+};
+
+namespace {
+
+int bset_err(cs_batch* b, int code, const char* fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(b->err, sizeof(b->err), fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+// an engine's error, as the batch's
+int bfrom(cs_batch* b, size_t i, int code)
+{
+    return bset_err(b, code, "engine %zu: %s", i, b->eng[i]->err);
+}
+
+void bdrop(cs_batch* b)
+{
+    if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->gexec = nullptr;
+    b->graph = nullptr;
+}
+
+// the batch graph for output steps of sps timesteps: every engine's step
+// graph as a child node, no edges between them
+int bbuild(cs_batch* b, int sps)
+{
+    int rc;
+    for (size_t i = 0; i < b->eng.size(); i++) {
+        cs_engine* e = b->eng[i];
+        if ((!e->gexec || e->graph_sps != sps) && (rc = capture_graph(e, sps))) return bfrom(b, i, rc);
+    }
+    bool fresh = b->gexec && b->sps == sps && b->gens.size() == b->eng.size();
+    for (size_t i = 0; fresh && i < b->eng.size(); i++) fresh = b->gens[i] == b->eng[i]->graph_gen;
+    if (fresh) return 0;
+    bdrop(b);
+    hipError_t st = hipGraphCreate(&b->graph, 0);
+    if (st != hipSuccess) return bset_err(b, CS_EHIP, "batch graph: %s", hipGetErrorString(st));
+    b->gens.assign(b->eng.size(), 0);
+    for (size_t i = 0; i < b->eng.size(); i++) {
+        hipGraphNode_t node;
+        st = hipGraphAddChildGraphNode(&node, b->graph, nullptr, 0, b->eng[i]->graph);
+        if (st != hipSuccess) return bset_err(b, CS_EHIP, "batch graph node %zu: %s", i, hipGetErrorString(st));
+        b->gens[i] = b->eng[i]->graph_gen;
+    }
+    st = hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0);
+    if (st != hipSuccess) {
+        b->gexec = nullptr;
+        return bset_err(b, CS_EHIP, "batch graph instantiation: %s", hipGetErrorString(st));
+    }
+    b->sps = sps;
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cs_batch_create(cs_batch** out, cs_engine* const* engines, int n)
+{
+    if (!out || !engines || n < 1) return CS_EINVAL;
+    *out = nullptr;
+    for (int i = 0; i < n; i++) {
+        if (!engines[i]) return CS_EINVAL;
+        if (engines[i]->device != engines[0]->device) return CS_EINVAL;
+        for (int j = 0; j < i; j++)
+            if (engines[j] == engines[i]) return CS_EINVAL;
+    }
+    cs_batch* b = new cs_batch;
+    b->eng.assign(engines, engines + n);
+    b->device = engines[0]->device;
+    if (hipSetDevice(b->device) != hipSuccess || hipStreamCreate(&b->stream) != hipSuccess) {
+        delete b;
+        return CS_EHIP;
+    }
+    *out = b;
+    return 0;
+}
+
+void cs_batch_destroy(cs_batch* b)
+{
+    if (!b) return;
+    (void)hipSetDevice(b->device);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    bdrop(b);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+const char* cs_batch_last_error(const cs_batch* b) { return b ? b->err : "null batch"; }
+
+int cs_batch_run_steps(cs_batch* b, double t0, int n_steps, int sps)
+{
+    if (!b || n_steps < 0 || sps <= 0) return CS_EINVAL;
+    const size_t n = b->eng.size();
+    int rc;
+    for (size_t i = 0; i < n; i++)
+        if ((rc = run_prepare(b->eng[i], t0, n_steps, sps))) return bfrom(b, i, rc);
+    const int periods = n_steps / sps;
+    bool graphs = periods > 0;
+    for (size_t i = 0; i < n; i++) graphs = graphs && b->eng[i]->use_graphs && !b->eng[i]->prof;
+    std::vector<int> done(n, 0);
+    std::vector<double> tt(n, t0);
+    if (graphs) {
+        if (hipSetDevice(b->device) != hipSuccess) return bset_err(b, CS_EHIP, "hipSetDevice");
+        for (size_t i = 0; i < n; i++)  // the engines' set-up precedes the batch's launches
+            if (hipStreamSynchronize(b->eng[i]->stream) != hipSuccess) return bset_err(b, CS_EHIP, "engine %zu stream", i);
+        int p = 0;
+        for (int guard = 0; p < periods && guard < 64;) {
+            if ((rc = bbuild(b, sps))) return rc;
+            const int nb = std::min(periods - p, 8);  // output steps between flag checks
+            for (int k = 0; k < nb; k++)
+                if (hipGraphLaunch(b->gexec, b->stream) != hipSuccess) return bset_err(b, CS_EHIP, "batch graph launch");
+            hipError_t st = hipStreamSynchronize(b->stream);
+            if (st != hipSuccess) return bset_err(b, CS_EHIP, "batch replay: %s", hipGetErrorString(st));
+            for (size_t i = 0; i < n; i++) {
+                cs_engine* e = b->eng[i];
+                cs::Clock c;
+                if ((rc = read_clock(e, &c))) return bfrom(b, i, rc);
+                unsigned f = 0;
+                if (hipMemcpy(&f, e->d.status, sizeof(f), hipMemcpyDeviceToHost) != hipSuccess) return bset_err(b, CS_EHIP, "status");
+                if (f & cs::FLAG_OVERFLOW) {
+                    // e halted at step c.step and its later replays did
+                    // nothing: grow its lists, bring it to the others' step
+                    // directly, capture it afresh
+                    guard++;
+                    if ((rc = run_recover(e, c, (p + nb) * sps, sps))) return bfrom(b, i, rc);
+                    if ((rc = capture_graph(e, sps))) return bfrom(b, i, rc);
+                } else if ((rc = check_flags(e))) {
+                    return bfrom(b, i, rc);
+                }
+            }
+            p += nb;
+        }
+        if (p < periods) return bset_err(b, CS_EOVERFLOW, "work lists keep overflowing");
+        for (size_t i = 0; i < n; i++) {
+            cs::Clock c;
+            if ((rc = read_clock(b->eng[i], &c))) return bfrom(b, i, rc);
+            done[i] = c.step;
+            tt[i] = c.t;
+        }
+    }
+    for (size_t i = 0; i < n; i++)
+        if ((rc = run_finish(b->eng[i], tt[i], done[i], n_steps, sps))) return bfrom(b, i, rc);
     return 0;
 }
 

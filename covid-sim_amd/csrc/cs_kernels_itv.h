@@ -30,6 +30,11 @@ __device__ __forceinline__ float prob_absent(const Dev& d, int r, int p)
     return (float)s.next();
 }
 
+// a trigger threshold of realisation r: its own where the realisations have
+// theirs (cs_set_realisation_triggers, for the three thresholds at once),
+// else the run's value v
+__device__ __forceinline__ double itv_thresh(const Dev& d, int r, double v) { return d.rthr ? d.rthr[r] : v; }
+
 // RecordSample's trigger value and CalibrationThresholdCheck's alert and
 // start times for realisation r at its sample m, time t; cumDC and
 // cumCritical from the counters, D the sample's dead total
@@ -62,14 +67,14 @@ __device__ void itv_triggers(const Dev& d, int r, int m, double t, long long D)
         s.alert = 1;
         s.ctrl = P->post_alert_control_prop_cases_id;
         // DO_OR_DONT_AMEND_START_TIME: each start time is set once
-        if (trig > P->case_isolation_cell_inc_thresh && s.ci_start >= 1e10) s.ci_start = t + P->case_isolation_time_start_base;
-        if (trig > P->hhquar_cell_inc_thresh && s.hq_start >= 1e10) s.hq_start = t + P->hquarantine_time_start_base;
-        if (trig >= P->place_close_cell_inc_thresh && s.pc_start >= 1e10) s.pc_start = t + P->place_close_time_start_base;
+        if (trig > itv_thresh(d, r, P->case_isolation_cell_inc_thresh) && s.ci_start >= 1e10) s.ci_start = t + P->case_isolation_time_start_base;
+        if (trig > itv_thresh(d, r, P->hhquar_cell_inc_thresh) && s.hq_start >= 1e10) s.hq_start = t + P->hquarantine_time_start_base;
+        if (trig >= itv_thresh(d, r, P->place_close_cell_inc_thresh) && s.pc_start >= 1e10) s.pc_start = t + P->place_close_time_start_base;
     }
     // the first closure ends after its duration when nothing re-triggers it
     if (t == s.pc_start + P->place_close_duration) {
         s.pc_prev = s.pc_start;
-        if (P->place_close_inc_trig == 0 && P->place_close_frac_inc_trig == 0 && P->place_close_cell_inc_thresh == 0) s.pc_start = 9e9;
+        if (P->place_close_inc_trig == 0 && P->place_close_frac_inc_trig == 0 && itv_thresh(d, r, P->place_close_cell_inc_thresh) == 0) s.pc_start = 9e9;
     }
 }
 
@@ -236,7 +241,7 @@ __global__ void k_treat_mcells(Dev d)
             d.mc_state[g] = P->do_place_close_once_only ? MC_DONT_TREAT_AGAIN : MC_UNTREATED;
             d.mc_end[g] = (uint16_t)ts;
         }
-        if (P->do_places && d.mc_state[g] == MC_UNTREATED && s.trig >= P->place_close_cell_inc_thresh) {
+        if (P->do_places && d.mc_state[g] == MC_UNTREATED && s.trig >= itv_thresh(d, r, P->place_close_cell_inc_thresh)) {
             d.mc_end[g] = w.t_end;
             d.mc_state[g] = MC_TREATED;
             for (int q = d.mpl_start[m]; q < d.mpl_start[m + 1]; q++) {

@@ -271,6 +271,7 @@ struct Dev {
     const uint64_t* seed;
     const double* rbeta;     // [R] per-realisation LocalBeta, or null (cs_params.local_beta)
     const int32_t* rseed_n;  // [R] per-realisation seed infections, or null
+    const double* rthr;      // [R] per-realisation intervention trigger thresholds, or null
     Hot* hot;             // [R][N]
     CS_FLD(Hot, inf) inf;
     CS_FLD(Hot, flags) hflags;

@@ -53,6 +53,14 @@ def lib():
         L.cs_version.restype = C.c_char_p
         L.cs_last_error.restype = C.c_char_p
         L.cs_last_error.argtypes = [C.c_void_p]
+        L.cs_batch_create.argtypes = [P(C.c_void_p), P(C.c_void_p), C.c_int]
+        L.cs_batch_create.restype = C.c_int
+        L.cs_batch_destroy.argtypes = [C.c_void_p]
+        L.cs_batch_destroy.restype = None
+        L.cs_batch_last_error.argtypes = [C.c_void_p]
+        L.cs_batch_last_error.restype = C.c_char_p
+        L.cs_batch_run_steps.argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_int]
+        L.cs_batch_run_steps.restype = C.c_int
         _lib = L
     return _lib
 
@@ -112,6 +120,10 @@ class Engine:
         ns = None if n_seed is None else np.ascontiguousarray(n_seed, np.int32)
         self.call("set_realisation_params", None if lb is None else _p(lb, C.c_double),
                   None if ns is None else _p(ns, C.c_int32))
+
+    def set_realisation_triggers(self, threshold=None):
+        th = None if threshold is None else np.ascontiguousarray(threshold, np.float64)
+        self.call("set_realisation_triggers", None if th is None else _p(th, C.c_double))
 
     def reset(self, n_seed):
         self.call("reset_realisations", int(n_seed))
@@ -284,3 +296,35 @@ class Engine:
         self._check(self.L.cs_sample(self.h, int(which), int(key), int(c0), int(c1), float(a), float(b),
                                      _p(out, C.c_double), int(n)))
         return out
+
+
+class Batch:
+    """Engines of one device (a region each) advanced together by one graph
+    launch per output step (cs_batch_*).  The engines stay the caller's and
+    keep their own results."""
+
+    def __init__(self, engines):
+        self.L = lib()
+        self.engines = list(engines)
+        hs = (C.c_void_p * len(self.engines))(*[e.h.value for e in self.engines])
+        self.h = C.c_void_p()
+        rc = self.L.cs_batch_create(C.byref(self.h), hs, len(self.engines))
+        if rc:
+            raise CSError(rc, "cs_batch_create")
+
+    def run_steps(self, t0, n_steps, steps_per_sample):
+        rc = self.L.cs_batch_run_steps(self.h, float(t0), int(n_steps), int(steps_per_sample))
+        if rc:
+            raise CSError(rc, self.L.cs_batch_last_error(self.h).decode())
+
+    def close(self):
+        if self.h:
+            self.L.cs_batch_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+

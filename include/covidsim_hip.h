@@ -389,6 +389,15 @@ int  cs_reset_realisations(cs_engine* e, int n_seed_infections);
  * realisation r.  Either may be NULL (no override); each call replaces the
  * previous overrides; cs_alloc_realisations clears them. */
 int  cs_set_realisation_params(cs_engine* e, const double* local_beta, const int32_t* n_seed);
+/* per-realisation intervention trigger thresholds: threshold[r] replaces
+ * the three thresholds the global trigger value is compared with
+ * (cs_params.case_isolation_cell_inc_thresh, hhquar_cell_inc_thresh and
+ * place_close_cell_inc_thresh; src/CovidSim.cpp:3640-3660 and
+ * src/Sweep.cpp:1426-1437 read P.CaseIsolation_CellIncThresh,
+ * P.HHQuar_CellIncThresh and P.PlaceCloseCellIncThresh) for realisation r,
+ * as a threshold sweep of the reference would run one process per value.
+ * NULL clears the overrides; cs_alloc_realisations clears them too. */
+int  cs_set_realisation_triggers(cs_engine* e, const double* threshold);
 
 /* host-initialised state in and out (drop-in use under CovidSim's RunModel) */
 int  cs_load_state(cs_engine* e, int real, const cs_state* s);
@@ -410,6 +419,24 @@ int  cs_update_probs(cs_engine* e);
  * are captured once into a HIP graph and replayed on a device-side clock
  * (results are identical either way; see cs_set_step_graphs). */
 int  cs_run_steps(cs_engine* e, double t0, int n_steps, int steps_per_sample);
+
+/* A batch: engines of one device, each a region of its own (population,
+ * parameters, realisations, per-realisation parameters), advanced together
+ * by cs_batch_run_steps: each output step of the batch is one launch of one
+ * graph whose independent branches are the engines' step graphs, so regions
+ * of uneven size run side by side.  The reference runs one region per
+ * process and its realisations one after another (/NR, src/CovidSim.cpp:
+ * 333-391).  Every engine's results equal those of cs_run_steps on it alone
+ * (a region's streams are its own).  The batch does not own the engines;
+ * each stays usable alone (its series, counters and state as after
+ * cs_run_steps), and must outlive the batch.  An engine that overflows a
+ * work list is grown and brought level directly, as cs_run_steps does.
+ * Engines with step graphs off or profiling on run one after another. */
+typedef struct cs_batch cs_batch;
+int  cs_batch_create(cs_batch** out, cs_engine* const* engines, int n);
+void cs_batch_destroy(cs_batch* b);
+const char* cs_batch_last_error(const cs_batch* b);
+int  cs_batch_run_steps(cs_batch* b, double t0, int n_steps, int steps_per_sample);
 /* 1 (default): cs_run_steps replays captured step graphs; 0: it launches
  * every kernel of every step from the host */
 int  cs_set_step_graphs(cs_engine* e, int enable);

@@ -398,6 +398,8 @@ typedef struct {
     double prev_prop_susc;
     double samp_prop_susc; /* State.S / PopSize as of the last sample */
     int has_beta;          /* per-realisation LocalBeta override */
+    int has_thr;           /* per-realisation trigger threshold override */
+    double thr;
     double local_beta;
     int n_seed;            /* per-realisation seed count, or -1 */
     double loc_init[2];
@@ -1586,6 +1588,9 @@ const char* or_interventions_unsupported(const cs_params* p)
 /* RecordSample's trigger value (src/CovidSim.cpp:3150-3160) and
  * CalibrationThresholdCheck's alert and start times
  * (src/CovidSim.cpp:3485-3689) for the realisation's sample m at time t */
+/* a trigger threshold of realisation r: its own, else the run's value v */
+#define THR(r, v) ((r)->has_thr ? (r)->thr : (v))
+
 static void itv_triggers(or_engine* e, or_real* r, int m, double t)
 {
     const cs_params* P = &e->P;
@@ -1616,14 +1621,14 @@ static void itv_triggers(or_engine* e, or_real* r, int m, double t)
         r->alert = 1;
         r->ctrl = P->post_alert_control_prop_cases_id;
         /* DO_OR_DONT_AMEND_START_TIME: a start time is set once */
-        if (trig > P->case_isolation_cell_inc_thresh && r->ci_start >= 1e10) r->ci_start = t + P->case_isolation_time_start_base;
-        if (trig > P->hhquar_cell_inc_thresh && r->hq_start >= 1e10) r->hq_start = t + P->hquarantine_time_start_base;
-        if (trig >= P->place_close_cell_inc_thresh && r->pc_start >= 1e10) r->pc_start = t + P->place_close_time_start_base;
+        if (trig > THR(r, P->case_isolation_cell_inc_thresh) && r->ci_start >= 1e10) r->ci_start = t + P->case_isolation_time_start_base;
+        if (trig > THR(r, P->hhquar_cell_inc_thresh) && r->hq_start >= 1e10) r->hq_start = t + P->hquarantine_time_start_base;
+        if (trig >= THR(r, P->place_close_cell_inc_thresh) && r->pc_start >= 1e10) r->pc_start = t + P->place_close_time_start_base;
     }
     /* the first closure ends after its duration when nothing re-triggers it */
     if (t == r->pc_start + P->place_close_duration) {
         r->pc_prev = r->pc_start;
-        if (P->place_close_inc_trig == 0 && P->place_close_frac_inc_trig == 0 && P->place_close_cell_inc_thresh == 0) r->pc_start = 9e9;
+        if (P->place_close_inc_trig == 0 && P->place_close_frac_inc_trig == 0 && THR(r, P->place_close_cell_inc_thresh) == 0) r->pc_start = 9e9;
     }
 }
 
@@ -1776,7 +1781,7 @@ static void itv_treat_sweep(or_engine* e, or_real* r, double t, unsigned ts)
             r->mc_state[m] = P->do_place_close_once_only ? 2 : 0;
             r->mc_end[m] = (uint16_t)ts;
         }
-        if (P->do_places && t >= r->pc_start && r->mc_state[m] == 0 && r->trig >= P->place_close_cell_inc_thresh) {
+        if (P->do_places && t >= r->pc_start && r->mc_state[m] == 0 && r->trig >= THR(r, P->place_close_cell_inc_thresh)) {
             r->mc_end[m] = t_end;
             r->mc_state[m] = 1;
             for (int q = e->mpl_start[m]; q < e->mpl_start[m + 1]; q++) {
@@ -1968,6 +1973,18 @@ int or_set_realisation_params(or_engine* e, const double* local_beta, const int3
         r->n_seed = n_seed ? n_seed[ri] : -1;
         if (local_beta && !(local_beta[ri] >= 0)) return fail(e, CS_EINVAL, "local_beta must be >= 0");
         if (n_seed && n_seed[ri] < 0) return fail(e, CS_EINVAL, "n_seed must be >= 0");
+    }
+    return 0;
+}
+
+/* per-realisation trigger thresholds (the three cell-incidence thresholds
+ * at once); NULL clears them */
+int or_set_realisation_triggers(or_engine* e, const double* threshold)
+{
+    for (int ri = 0; ri < e->nreal; ri++) {
+        if (threshold && !(threshold[ri] >= 0)) return fail(e, CS_EINVAL, "threshold must be >= 0");
+        e->R[ri].has_thr = threshold != NULL;
+        e->R[ri].thr = threshold ? threshold[ri] : 0.0;
     }
     return 0;
 }

@@ -29,6 +29,7 @@ void or_destroy(or_engine* e);
 const char* or_last_error(const or_engine* e);
 int  or_alloc_realisations(or_engine* e, int n_real, const uint64_t* seeds);
 int  or_set_realisation_params(or_engine* e, const double* local_beta, const int32_t* n_seed);
+int  or_set_realisation_triggers(or_engine* e, const double* threshold);
 int  or_reset_realisations(or_engine* e, int n_seed_infections);
 int  or_load_state(or_engine* e, int real, const cs_state* s);
 int  or_store_state(or_engine* e, int real, cs_state* s);

@@ -91,6 +91,10 @@ class OracleEngine(Oracle):
         self.call("set_realisation_params", None if lb is None else _p(lb, C.c_double),
                   None if ns is None else _p(ns, C.c_int32))
 
+    def set_realisation_triggers(self, threshold=None):
+        th = None if threshold is None else np.ascontiguousarray(threshold, np.float64)
+        self.call("set_realisation_triggers", None if th is None else _p(th, C.c_double))
+
     def reset(self, n_seed):
         self.call("reset_realisations", int(n_seed))
 

@@ -468,6 +468,39 @@ def test_interventions_match_oracle(engmod, csa, graphs):
         assert (a.a["isolation_start_time"] != 65534).sum() > 0
 
 
+@pytest.mark.parametrize("graphs", [True, False])
+def test_per_realisation_triggers_match_oracle(engmod, csa, graphs):
+    """Per-realisation trigger thresholds (the same value for the three
+    thresholds of realisation r: the run's own, a higher one, one never
+    reached) over 120 days: series, counters and intervention state equal to
+    the oracle's, and the start times ordered by threshold."""
+    pop, params = mini_int()
+    seeds = np.array([81, 81, 81], np.uint64)
+    thr = np.array([5.0, 40.0, 1e9])
+    g = engmod.Engine(0)
+    g.set_params(params)
+    g.set_population(pop)
+    g.alloc(seeds)
+    g.set_step_graphs(graphs)
+    o = OracleEngine(params, pop)
+    o.alloc(seeds)
+    tspd = int(params.time_steps_per_day)
+    for eng in (g, o):
+        eng.set_realisation_triggers(thr)
+        eng.reset(10)
+        eng.run_steps(0.0, 120 * tspd, tspd)
+    assert np.array_equal(g.timeseries(120), o.timeseries(120))
+    starts = []
+    for r in range(3):
+        _compare_counters(g.counters(r), o.counters(r))
+        sa, sb = g.intervention_state(r), o.intervention_state(r)
+        for f, _ in sa._fields_:
+            assert getattr(sa, f) == getattr(sb, f), (r, f)
+        starts.append(sa.case_isolation_start)
+    # one seed, so the runs agree until the lowest threshold is crossed
+    assert starts[0] < starts[1] < 1e9 and starts[2] >= 1e10, starts
+
+
 def test_interventions_refused_outside_the_subset(engmod, csa):
     """Parameter sets outside the intervention subset are refused with
     CS_EUNSUPPORTED (never approximated), and a single sweep refuses an

@@ -23,6 +23,16 @@ The CPU baseline runs the reference itself (oracle/_ref/covidsim_ref, built
 from the reference sources) on its 200k-person workload with all host
 threads, for a bounded number of realisations.
 
+--config c3 prints the line of the uneven-regions ensemble instead: 64
+realisations over five generated regions of 1M, 2M, 5M, 10M and 20M persons
+(13/13/13/13/12 realisations), each realisation with its own R0 (LocalBeta
+from one InitTransmissionCoeffs pass per region) and intervention trigger
+threshold (covid-sim_amd/inputs/c3/realisations.json), the intervention
+subset on (covid-sim_amd/inputs/c3/params.bin), 300 days; the regions advance
+as one batch (every output step, each region's step graph launched on its
+own stream before any wait, so the regions run side by side); `value` = the
+sum over realisations of the region's persons x days / wall seconds.
+
 --calibration prints a second kind of line instead: the wall time of
 fitting R0 and the seeding scale to a death curve with the emulator
 (covid-sim_amd/calibrate.py) against plain simulator-only calibration on
@@ -165,6 +175,8 @@ def main():
     ap.add_argument("--no-graphs", action="store_true", help="launch every kernel directly in the timed steps too "
                                                             "(for runs under rocprofv3 tracing)")
     ap.add_argument("--calibration", action="store_true", help="measure emulator-guided vs plain calibration instead")
+    ap.add_argument("--config", choices=["headline", "c3"], default="headline",
+                    help="c3: the 64-realisation ensemble over five regions of uneven size (see above)")
     ap.add_argument("--summarise-pmc", metavar="DIR", help="print per-kernel sums of a rocprofv3 --pmc output "
                                                             "directory (counter_collection.csv) and exit")
     ap.add_argument("--summarise-trace", metavar="DIR", help="print a per-kernel table of a rocprofv3 --kernel-trace "
@@ -195,6 +207,8 @@ def main():
                               "copy_probe_bytes": args.copy_probe_mb << 20})
     if args.calibration:
         return calibration(args)
+    if args.config == "c3":
+        return uneven_regions(args)
     import importlib
     csa = importlib.import_module("covid-sim_amd")
     eng_mod = importlib.import_module("covid-sim_amd.engine")
@@ -310,6 +324,150 @@ def main():
                                  "(%.2f s, direct launches)" % t_prof,
                      "top_kernels": top},
         "cpu_baseline": base,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def uneven_regions(args):
+    """The uneven-regions ensemble (--config c3) as one batch."""
+    import importlib
+    csa = importlib.import_module("covid-sim_amd")
+    eng_mod = importlib.import_module("covid-sim_amd.engine")
+    popgen = importlib.import_module("covid-sim_amd.popgen")
+    inp = os.path.join(ROOT, "covid-sim_amd", "inputs", "c3")
+    with open(os.path.join(inp, "realisations.json")) as f:
+        spec = json.load(f)
+    regions = spec["regions"]
+    reg = np.array(spec["realisation_region"])
+    r0 = np.array(spec["r0"], np.float64)
+    thr = np.array(spec["trigger_threshold"], np.float64)
+    seeds_all = np.arange(1, len(reg) + 1, dtype=np.uint64) * np.uint64(0x9E3779B97F4A7C15)
+    t0 = time.perf_counter()
+    engines, sizes, ncells, betas = [], [], [], []
+    for k, n in enumerate(regions):
+        arrays, geom = popgen.generate(n, seed=2024 + k)
+        pop = csa.population.Population(arrays)
+        params = csa.population.load_params(os.path.join(inp, "params.bin"))
+        popgen.apply_geometry(params, geom)
+        eng = eng_mod.Engine(0)
+        eng.set_params(params)
+        eng.set_population(pop)
+        c = eng.transmission_coeffs(77 + k)  # this region's R0 components (its LocalBeta for params.r0 too)
+        mine = np.where(reg == k)[0]
+        beta = np.maximum(0.0, (r0[mine] - c.r0_household - c.r0_places) / c.spatial_r0)
+        eng.alloc(seeds_all[mine])
+        eng.set_realisation_params(local_beta=beta)
+        eng.set_realisation_triggers(thr[mine])
+        if args.no_graphs:
+            eng.set_step_graphs(False)
+        engines.append(eng)
+        sizes.append(len(mine) * n)
+        ncells.append(pop.n_cells)
+        betas.append([float(x) for x in beta])
+        log("region %d: %d persons, %d cells, %d realisations, R0 %.2f-%.2f (%.1f s)"
+            % (k, n, pop.n_cells, len(mine), r0[mine].min(), r0[mine].max(), time.perf_counter() - t0))
+    batch = eng_mod.Batch(engines)
+    tspd = int(round(params.time_steps_per_day))
+    steps = args.days * tspd
+    for _ in range(args.warmup):
+        for e in engines:
+            e.reset(10)
+        batch.run_steps(0.0, args.warmup_days * tspd, tspd)
+    if args.warmup and not args.no_autotune:
+        for k, e in enumerate(engines):
+            log("autotune region %d: %s" % (k, json.dumps(e.autotune(float(args.warmup_days)))))
+    for e in engines:
+        e.sync()
+    chunk = max(1, args.chunk_days) * tspd
+
+    def one_step(k, label):
+        for e in engines:
+            e.sync()
+        ta = time.perf_counter()
+        for e in engines:
+            e.reset(10)
+        for s0 in range(0, steps, chunk):
+            n = min(chunk, steps - s0)
+            batch.run_steps(s0 / tspd, n, tspd)
+            for e in engines:
+                e.sync()
+            log("  %s %d: day %d (%.1f s)" % (label, k, (s0 + n) // tspd, time.perf_counter() - ta))
+        dt = time.perf_counter() - ta
+        log("%s %d: %.2f s" % (label, k, dt))
+        return dt
+
+    times = [one_step(k, "step") for k in range(args.steps)]
+    # the profiled pass: every launch timed, so the engines run one after
+    # another with direct launches; per-kernel times and work summed
+    for e in engines:
+        e.work_counts(reset=True)
+        e.set_profiling(True)
+    t_prof = one_step(0, "profiled step")
+    prof, wc = {}, {}
+    for e, nc, (k, n) in zip(engines, ncells, enumerate(regions)):
+        e.set_profiling(False)
+        pr = e.profile()
+        for name, (ms, launches) in pr.items():
+            a = prof.setdefault(name, [0.0, 0])
+            a[0] += ms
+            a[1] += launches
+        w = e.work_counts()
+        w["n_cell_steps"] = int((reg == k).sum()) * nc * pr["cell_spatial_sum"][1]  # cells visited per launch
+        for u, v in w.items():
+            wc[u] = wc.get(u, 0) + v
+    total = float(sum(times))
+    person_days = float(sum(sizes)) * args.days * args.steps
+    value = person_days / total
+    ranked = sorted((k for k in prof if k in KERNEL_UNITS), key=lambda k: -prof[k][0])
+    top = []
+    for name in ranked[:5]:
+        ms, launches = prof[name]
+        b = sum(BYTES[u] * wc.get(u, 0) for u in KERNEL_UNITS[name]) / max(launches, 1)
+        a = b / (ms / max(launches, 1) * 1e-3) / 1e9
+        top.append({"kernel": name, "ms_total": round(ms, 1), "launches": launches, "bytes_per_launch": round(b),
+                    "ms_per_launch": round(ms / max(launches, 1), 4), "achieved_gbs": round(a, 2),
+                    "frac": round(a / HBM_PEAK_GBS, 5)})
+    attack = []
+    for e, n in zip(engines, regions):
+        ts = e.timeseries(args.days)
+        attack.append(round(float(np.mean((ts[:, -1, 3] + ts[:, -1, 4]) / n)), 4))
+    starts = []
+    for e in engines:
+        for r in range(e.n_real):
+            s = e.intervention_state(r)
+            starts.append(s.place_close_start < 1e9)
+    log("kernel ms: " + ", ".join("%s %.0f/%d" % (k, v[0], v[1]) for k, v in prof.items()))
+    log("mean attack rate by region: %s; interventions started in %d of %d realisations" % (attack, sum(starts), len(starts)))
+    dom = top[0]
+    out = {
+        "metric": "person-days/s, 64-run ensemble",
+        "value": value,
+        "unit": "person-days/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": total / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "uneven-regions ensemble: 64 realisations over 5 synthetic regions (1M/2M/5M/10M/20M "
+                               "persons, 13/13/13/13/12 realisations), per-realisation R0 ~ U[1.8, 3.0] and ICU "
+                               "trigger threshold ~ U[20, 500], case isolation, household quarantine and place "
+                               "closure on, %d days, 4 steps/day, 10 seeds; the regions' step graphs replayed side by side"
+                               % args.days,
+                   "realisations": int(len(reg)), "regions": regions, "cells": ncells, "days": args.days,
+                   "person_realisations": int(sum(sizes)), "mean_attack_rate_by_region": attack,
+                   "interventions_started": int(sum(starts)),
+                   "launches": "direct launches" if args.no_graphs else "replayed step graphs, one stream per region"},
+        "roofline": {"bound": "hbm", "kernel": dom["kernel"], "achieved": dom["achieved_gbs"], "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": dom["frac"], "traffic": None,
+                     "bytes_per_launch": dom["bytes_per_launch"], "ms_per_launch": dom["ms_per_launch"],
+                     "measured": "HIP events around every launch of a profiled pass of the same workload "
+                                 "(%.2f s, engines one after another, direct launches)" % t_prof,
+                     "top_kernels": top},
+        "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)
 

@@ -44,8 +44,6 @@ struct cs_engine {
     // replayed step graphs (cs_run_steps): one graph per output step
     bool use_graphs = true;
     hipGraphExec_t gexec = nullptr;
-    hipGraph_t graph = nullptr;      // the captured graph behind gexec (a batch embeds it)
-    unsigned graph_gen = 0;          // captures so far: a batch rebuilds when it changes
     int graph_sps = 0;
 
     // profiling: event pairs per kernel kind, resolved on cs_get_profile
@@ -341,10 +339,7 @@ int launch_update_probs(cs_engine* e, const int* sel)
 void drop_graph(cs_engine* e)
 {
     if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
-    if (e->graph) (void)hipGraphDestroy(e->graph);
     e->gexec = nullptr;
-    e->graph = nullptr;
-    e->graph_gen++;
 }
 
 // the place contact list: a quarter of the event queue's capacity (at least
@@ -460,13 +455,11 @@ int capture_graph(cs_engine* e, int sps)
     }
     if (st != hipSuccess) return set_err(e, CS_EHIP, "step graph capture: %s", hipGetErrorString(st));
     st = hipGraphInstantiate(&e->gexec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
     if (st != hipSuccess) {
-        (void)hipGraphDestroy(g);
         e->gexec = nullptr;
         return set_err(e, CS_EHIP, "step graph instantiation: %s", hipGetErrorString(st));
     }
-    e->graph = g;
-    e->graph_gen++;
     e->graph_sps = sps;
     return 0;
 }
@@ -769,6 +762,7 @@ int cs_set_population(cs_engine* e, const cs_population* pop)
         d.pl_gst[t].base = d.pl_gsz[t].base = d.pl_grp[t];
     }
     d.np_off[CS_MAX_PLACE_TYPES] = off;
+    if ((rc = dupload(e, L, &d.np_off_d, (const int32_t*)d.np_off, CS_MAX_PLACE_TYPES + 1))) return rc;
     {   // the places of each microcell (Microcell.places), by type then index
         std::vector<int32_t> ms(e->NM + 1, 0), ml(std::max(off, 1));
         for (int t = 0; t < e->NPT; t++)
@@ -816,6 +810,10 @@ int cs_alloc_realisations(cs_engine* e, int n_real, const uint64_t* seeds)
     Dev& d = e->d;
     const size_t R = (size_t)n_real, N = (size_t)e->N, NC = (size_t)e->NC, NM = (size_t)e->NM;
     const size_t RN = R * N, RC = R * NC, PL = R * (size_t)d.np_off[CS_MAX_PLACE_TYPES];
+    // the per-(realisation, microcell) and per-(realisation, place) passes
+    // index in 32 bits
+    if (R * NM >= ((size_t)1 << 32) || PL >= ((size_t)1 << 32))
+        return set_err(e, CS_EUNSUPPORTED, "%d realisations x %zu microcells (or places) exceed 2^32", n_real, NM);
     e->R = d.R = n_real;
     auto& A = e->real_allocs;
     int rc = 0;
@@ -1767,17 +1765,13 @@ int cs_sample(cs_engine* e, int which, uint64_t key, uint32_t c0, uint32_t c1, d
 
 // ------------------------------------------------------------ batches
 // engines of one device (a region each: its own population, parameters and
-// realisations) advanced together: an output step of the batch is one
-// launch of one graph whose branches are the engines' captured step graphs
+// realisations) advanced together: every output step, each engine's
+// captured step graph is launched on the engine's own stream, all before
+// any wait, so the device's hardware queues run the regions side by side
 
 struct cs_batch {
     std::vector<cs_engine*> eng;
     int device = 0;
-    hipStream_t stream = nullptr;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t gexec = nullptr;
-    std::vector<unsigned> gens;  // the engines' capture generations the graph holds
-    int sps = 0;
     char err[512] = {0};
 };
 
@@ -1798,45 +1792,6 @@ int bfrom(cs_batch* b, size_t i, int code)
     return bset_err(b, code, "engine %zu: %s", i, b->eng[i]->err);
 }
 
-void bdrop(cs_batch* b)
-{
-    if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
-    if (b->graph) (void)hipGraphDestroy(b->graph);
-    b->gexec = nullptr;
-    b->graph = nullptr;
-}
-
-// the batch graph for output steps of sps timesteps: every engine's step
-// graph as a child node, no edges between them
-int bbuild(cs_batch* b, int sps)
-{
-    int rc;
-    for (size_t i = 0; i < b->eng.size(); i++) {
-        cs_engine* e = b->eng[i];
-        if ((!e->gexec || e->graph_sps != sps) && (rc = capture_graph(e, sps))) return bfrom(b, i, rc);
-    }
-    bool fresh = b->gexec && b->sps == sps && b->gens.size() == b->eng.size();
-    for (size_t i = 0; fresh && i < b->eng.size(); i++) fresh = b->gens[i] == b->eng[i]->graph_gen;
-    if (fresh) return 0;
-    bdrop(b);
-    hipError_t st = hipGraphCreate(&b->graph, 0);
-    if (st != hipSuccess) return bset_err(b, CS_EHIP, "batch graph: %s", hipGetErrorString(st));
-    b->gens.assign(b->eng.size(), 0);
-    for (size_t i = 0; i < b->eng.size(); i++) {
-        hipGraphNode_t node;
-        st = hipGraphAddChildGraphNode(&node, b->graph, nullptr, 0, b->eng[i]->graph);
-        if (st != hipSuccess) return bset_err(b, CS_EHIP, "batch graph node %zu: %s", i, hipGetErrorString(st));
-        b->gens[i] = b->eng[i]->graph_gen;
-    }
-    st = hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0);
-    if (st != hipSuccess) {
-        b->gexec = nullptr;
-        return bset_err(b, CS_EHIP, "batch graph instantiation: %s", hipGetErrorString(st));
-    }
-    b->sps = sps;
-    return 0;
-}
-
 }  // namespace
 
 extern "C" {
@@ -1854,23 +1809,11 @@ int cs_batch_create(cs_batch** out, cs_engine* const* engines, int n)
     cs_batch* b = new cs_batch;
     b->eng.assign(engines, engines + n);
     b->device = engines[0]->device;
-    if (hipSetDevice(b->device) != hipSuccess || hipStreamCreate(&b->stream) != hipSuccess) {
-        delete b;
-        return CS_EHIP;
-    }
     *out = b;
     return 0;
 }
 
-void cs_batch_destroy(cs_batch* b)
-{
-    if (!b) return;
-    (void)hipSetDevice(b->device);
-    if (b->stream) (void)hipStreamSynchronize(b->stream);
-    bdrop(b);
-    if (b->stream) (void)hipStreamDestroy(b->stream);
-    delete b;
-}
+void cs_batch_destroy(cs_batch* b) { delete b; }
 
 const char* cs_batch_last_error(const cs_batch* b) { return b ? b->err : "null batch"; }
 
@@ -1888,16 +1831,23 @@ int cs_batch_run_steps(cs_batch* b, double t0, int n_steps, int sps)
     std::vector<double> tt(n, t0);
     if (graphs) {
         if (hipSetDevice(b->device) != hipSuccess) return bset_err(b, CS_EHIP, "hipSetDevice");
-        for (size_t i = 0; i < n; i++)  // the engines' set-up precedes the batch's launches
-            if (hipStreamSynchronize(b->eng[i]->stream) != hipSuccess) return bset_err(b, CS_EHIP, "engine %zu stream", i);
         int p = 0;
         for (int guard = 0; p < periods && guard < 64;) {
-            if ((rc = bbuild(b, sps))) return rc;
+            for (size_t i = 0; i < n; i++) {
+                cs_engine* e = b->eng[i];
+                if ((!e->gexec || e->graph_sps != sps) && (rc = capture_graph(e, sps))) return bfrom(b, i, rc);
+            }
             const int nb = std::min(periods - p, 8);  // output steps between flag checks
+            // every engine's replays are issued before any wait (each on its
+            // own stream, behind that engine's set-up)
             for (int k = 0; k < nb; k++)
-                if (hipGraphLaunch(b->gexec, b->stream) != hipSuccess) return bset_err(b, CS_EHIP, "batch graph launch");
-            hipError_t st = hipStreamSynchronize(b->stream);
-            if (st != hipSuccess) return bset_err(b, CS_EHIP, "batch replay: %s", hipGetErrorString(st));
+                for (size_t i = 0; i < n; i++)
+                    if (hipGraphLaunch(b->eng[i]->gexec, b->eng[i]->stream) != hipSuccess)
+                        return bset_err(b, CS_EHIP, "engine %zu graph launch", i);
+            for (size_t i = 0; i < n; i++) {
+                hipError_t st = hipStreamSynchronize(b->eng[i]->stream);
+                if (st != hipSuccess) return bset_err(b, CS_EHIP, "engine %zu replay: %s", i, hipGetErrorString(st));
+            }
             for (size_t i = 0; i < n; i++) {
                 cs_engine* e = b->eng[i];
                 cs::Clock c;
@@ -1906,11 +1856,11 @@ int cs_batch_run_steps(cs_batch* b, double t0, int n_steps, int sps)
                 if (hipMemcpy(&f, e->d.status, sizeof(f), hipMemcpyDeviceToHost) != hipSuccess) return bset_err(b, CS_EHIP, "status");
                 if (f & cs::FLAG_OVERFLOW) {
                     // e halted at step c.step and its later replays did
-                    // nothing: grow its lists, bring it to the others' step
-                    // directly, capture it afresh
+                    // nothing: grow its lists (which drops its graph, to be
+                    // captured afresh above) and bring it to the others'
+                    // step directly
                     guard++;
                     if ((rc = run_recover(e, c, (p + nb) * sps, sps))) return bfrom(b, i, rc);
-                    if ((rc = capture_graph(e, sps))) return bfrom(b, i, rc);
                 } else if ((rc = check_flags(e))) {
                     return bfrom(b, i, rc);
                 }

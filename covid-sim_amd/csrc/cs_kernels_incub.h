@@ -42,9 +42,9 @@ __device__ void holiday_places(const Dev& d, int h, double ht)
 {
     const cs_params* P = d.P;
     const int tot = d.np_off[CS_MAX_PLACE_TYPES];
-    const long long n = (long long)d.R * tot;
-    for (long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x; g < n; g += (long long)gridDim.x * blockDim.x) {
-        int r = (int)(g / tot), f = (int)(g % tot);
+    const unsigned n = (unsigned)d.R * (unsigned)tot;  // below 2^32 (checked at allocation)
+    for (unsigned g = blockIdx.x * blockDim.x + threadIdx.x; g < n; g += gridDim.x * blockDim.x) {
+        int r = (int)(g / (unsigned)tot), f = (int)(g - (unsigned)r * (unsigned)tot);
         int pt = 0;
         while (pt + 1 < CS_MAX_PLACE_TYPES && f >= d.np_off[pt + 1]) pt++;
         int k = f - d.np_off[pt];

@@ -193,9 +193,9 @@ __global__ void k_detected_ci(Dev d)
 
 // DoPlaceClose with DoAnyway (src/Update.cpp:1095-1190): the place's closure
 // window; true when the place closes (its members' absences follow)
-__device__ bool close_place(const Dev& d, int r, int pt, int k, uint16_t t_start, uint16_t t_stop)
+__device__ __forceinline__ bool close_place(const Dev& d, int r, int pt, int k, uint16_t t_start, uint16_t t_stop)
 {
-    const size_t pi = (size_t)r * d.np_off[CS_MAX_PLACE_TYPES] + d.np_off[pt] + k;
+    const size_t pi = (size_t)r * d.np_off[CS_MAX_PLACE_TYPES] + np_off_of(d, pt) + k;
     if (!(d.pl_ce[pi] < t_stop) || d.pl_once[pi]) return false;
     if (d.P->do_place_close_once_only) d.pl_once[pi] = 1;
     if ((double)prob_close(d, r, pt, k) >= d.P->efficacies[2][pt]) {
@@ -230,10 +230,10 @@ __global__ void k_treat_mcells(Dev d)
     const StepScalars sc = *d.sc;
     const unsigned ts = sc.ts;
     const CloseWindow w = close_window(P, sc.t, ts);
-    const long long n = (long long)d.R * d.NM;
+    const unsigned nm = (unsigned)d.NM, n = (unsigned)d.R * nm;  // below 2^32 (checked at allocation)
     const int tot = d.np_off[CS_MAX_PLACE_TYPES];
-    for (long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x; g < n; g += (long long)gridDim.x * blockDim.x) {
-        const int r = (int)(g / d.NM), m = (int)(g % d.NM);
+    for (unsigned g = blockIdx.x * blockDim.x + threadIdx.x; g < n; g += gridDim.x * blockDim.x) {
+        const int r = (int)(g / nm), m = (int)(g - (unsigned)r * nm);  // 32-bit division: a 64-bit one costs the pass most of its time
         const IntState& s = d.ist[r];
         if (!(sc.t >= s.pc_start)) continue;
         if (d.mstart[m + 1] == d.mstart[m]) continue;  // unpopulated
@@ -248,7 +248,7 @@ __global__ void k_treat_mcells(Dev d)
                 const int pt = d.mpl[q] >> 27, k = d.mpl[q] & ((1 << 27) - 1);
                 if (pt == P->hotel_place_type) continue;
                 if (close_place(d, r, pt, k, w.t_start, w.t_stop))
-                    itv_list_add(d, d.close_l, d.itv_n + 1, d.close_cap, (int32_t)((long long)r * tot + d.np_off[pt] + k));
+                    itv_list_add(d, d.close_l, d.itv_n + 1, d.close_cap, (int32_t)((long long)r * tot + np_off_of(d, pt) + k));
             }
         }
     }

@@ -258,6 +258,7 @@ struct Dev {
     const int32_t* padunit;  // administrative unit of each person's microcell
     int np[CS_MAX_PLACE_TYPES];
     int np_off[CS_MAX_PLACE_TYPES + 1];  // offsets of each type in the per-realisation place arrays
+    const int32_t* np_off_d;  // a device copy of np_off, for run-time type indices
     const PlRec* pl_rec[CS_MAX_PLACE_TYPES];
     const GRec* pl_grp[CS_MAX_PLACE_TYPES];
     CS_CFLD(PlRec, n) pl_n[CS_MAX_PLACE_TYPES];
@@ -392,9 +393,13 @@ __device__ __forceinline__ bool esoc_g(const Dev& d, size_t gp) { return (d.hpfl
 __device__ __forceinline__ int keyw_g(const Dev& d, size_t gp) { return (d.hpfl[gp] >> 1) & 1; }
 __device__ __forceinline__ bool is_dead_code(int8_t s) { return s == CS_DEAD_ASYMP || s == CS_DEAD_SYMP; }
 __device__ __forceinline__ bool host_absent(const Dev& d, size_t gp, unsigned ts) { return d.t_as[gp] <= ts && d.t_ae[gp] > ts; }
+// d.np_off[t] for a run-time t, from the device copy: a run-time index into
+// the by-value Dev makes the compiler copy the whole struct to every lane's
+// scratch first
+__device__ __forceinline__ int np_off_of(const Dev& d, int t) { return d.np_off_d[t]; }
 __device__ __forceinline__ bool place_closed(const Dev& d, int r, int t, int k, unsigned ts)
 {
-    size_t i = (size_t)r * d.np_off[CS_MAX_PLACE_TYPES] + d.np_off[t] + k;
+    size_t i = (size_t)r * d.np_off[CS_MAX_PLACE_TYPES] + np_off_of(d, t) + k;
     return d.pl_cs[i] <= ts && d.pl_ce[i] > ts;
 }
 __device__ __forceinline__ bool treated(const Dev& d, size_t gp, unsigned ts)

@@ -422,9 +422,10 @@ int  cs_run_steps(cs_engine* e, double t0, int n_steps, int steps_per_sample);
 
 /* A batch: engines of one device, each a region of its own (population,
  * parameters, realisations, per-realisation parameters), advanced together
- * by cs_batch_run_steps: each output step of the batch is one launch of one
- * graph whose independent branches are the engines' step graphs, so regions
- * of uneven size run side by side.  The reference runs one region per
+ * by cs_batch_run_steps: every output step, each engine's captured step
+ * graph is launched on the engine's own stream, all of them before any
+ * wait, so that regions of uneven size run side by side on the device's
+ * hardware queues.  The reference runs one region per
  * process and its realisations one after another (/NR, src/CovidSim.cpp:
  * 333-391).  Every engine's results equal those of cs_run_steps on it alone
  * (a region's streams are its own).  The batch does not own the engines;

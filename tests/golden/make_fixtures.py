@@ -145,11 +145,12 @@ def override_params(src, overrides, dst):
     return dst
 
 
-def int_param_files(out_dir):
-    """(pre-params, params) of the intervention variant, written to out_dir"""
-    mini = os.path.join(ROOT, "covid-sim_amd", "inputs", "mini")
-    return (override_params(os.path.join(mini, "pre-params.txt"), INT_PRE, os.path.join(out_dir, "int-pre-params.txt")),
-            override_params(os.path.join(mini, "input-noint-params.txt"), INT_PARAMS, os.path.join(out_dir, "int-params.txt")))
+def int_param_files(out_dir, region="mini"):
+    """(pre-params, params) of the intervention variant of a region's inputs,
+    written to out_dir"""
+    src = os.path.join(ROOT, "covid-sim_amd", "inputs", region)
+    return (override_params(os.path.join(src, "pre-params.txt"), INT_PRE, os.path.join(out_dir, "int-pre-params.txt")),
+            override_params(os.path.join(src, "input-noint-params.txt"), INT_PARAMS, os.path.join(out_dir, "int-params.txt")))
 
 
 def run_harness(inputs, out, steps="0", golden=True, threads=1, extra=(), series=None, nr=None, extra_env=None,
@@ -267,9 +268,31 @@ def interventions():
         shutil.rmtree(tmp)
 
 
+def c3_params():
+    """covid-sim_amd/inputs/c3/params.bin: cs_params of the C1 inputs with
+    the intervention subset (global ICU triggers, case isolation, household
+    quarantine, school and university closure; INT_PRE and INT_PARAMS), as
+    the reference reads them: the parameter set of the uneven-regions
+    ensemble, whose regions are generated and whose trigger thresholds are
+    set per realisation at run time"""
+    inputs = os.path.join(ROOT, "covid-sim_amd", "inputs", "c1")
+    dst = os.path.join(ROOT, "covid-sim_amd", "inputs", "c3")
+    os.makedirs(dst, exist_ok=True)
+    tmp = tempfile.mkdtemp()
+    try:
+        pre, par = int_param_files(tmp, "c1")
+        run_harness(inputs, tmp, steps="", golden=False, threads=1, pre_params=pre, params=par)
+        shutil.copy(os.path.join(tmp, "params.bin"), os.path.join(dst, "params.bin"))
+        print("wrote", os.path.join(dst, "params.bin"))
+    finally:
+        shutil.rmtree(tmp)
+
+
 if __name__ == "__main__":
     if len(sys.argv) > 1 and sys.argv[1] == "interventions":
         interventions()
+    elif len(sys.argv) > 1 and sys.argv[1] == "c3":
+        c3_params()
     elif len(sys.argv) > 1 and sys.argv[1] == "care":
         care()
     elif len(sys.argv) > 1 and sys.argv[1] == "c1":

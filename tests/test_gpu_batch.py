@@ -1,5 +1,6 @@
-"""Regions of uneven size advanced together (cs_batch_*): one graph launch per
-output step runs every region's step graph.  Each region's realisations
+"""Regions of uneven size advanced together (cs_batch_*): every output step,
+each region's step graph is launched on its own stream before any wait.
+Each region's realisations
 must come out exactly as cs_run_steps gives them on that region alone, and
 as the oracle gives them; an engine whose work lists overflow inside the
 batch must be grown and brought level without changing any result."""

@@ -137,7 +137,35 @@ def cpu_baseline(max_seconds=25.0):
         nr, loop = 1, loop1
     return {"value": nr * n_people * days / loop, "unit": "person-days/s", "cores": threads, "kind": "reference",
             "sample": "reference covidsim_ref, its 200k-person 200-day workload (4 steps/day, no interventions), "
-                      "%d realisation(s), day-loop wall %.2f s from stderr markers, /c:%d" % (nr, loop, threads)}
+                      "%d realisation(s), day-loop wall %.2f s from stderr markers, /c:%d" % (nr, loop, threads),
+            "realisations": nr}
+
+
+def gpu_same_workload(nr, csa, eng_mod):
+    """The GPU on the CPU baseline's own workload: the 200k-person region as
+    the reference's set-up builds it from covid-sim_amd/inputs/c1
+    (tests/golden/c1_population.npz, built with /c:1), its parameters, 200
+    days at 4 steps/day, as many realisations as the baseline ran, one
+    launch; person-days/s of the second of two runs (the first captures the
+    step graphs)."""
+    z = np.load(os.path.join(ROOT, "tests", "golden", "c1_population.npz"))
+    pop = csa.population.Population({k: z[k] for k in z.files})
+    params = csa.population.load_params(os.path.join(ROOT, "covid-sim_amd", "inputs", "c1", "params.bin"))
+    eng = eng_mod.Engine(0)
+    eng.set_params(params)
+    eng.set_population(pop)
+    eng.alloc(np.arange(1, nr + 1, dtype=np.uint64) * np.uint64(0x9E3779B97F4A7C15))
+    tspd = int(round(params.time_steps_per_day))
+    dt = 0.0
+    for _ in range(2):
+        eng.sync()
+        ta = time.perf_counter()
+        eng.reset(10)
+        eng.run_steps(0.0, 200 * tspd, tspd)
+        eng.sync()
+        dt = time.perf_counter() - ta
+    eng.close()
+    return nr * pop.n_people * 200 / dt, dt
 
 
 def copy_probe(nbytes):
@@ -299,6 +327,12 @@ def main():
     total_alg = sum(BYTES[u] * wc[u] for u in BYTES)
     traffic = top[0]["traffic_bytes_per_launch"]
     base = None if args.no_cpu_baseline else cpu_baseline()
+    if base:  # the GPU on the baseline's own sample: the like-for-like ratio
+        g, dt = gpu_same_workload(base["realisations"], csa, eng_mod)
+        base["gpu_same_workload"] = {"value": g, "unit": "person-days/s", "seconds": round(dt, 3),
+                                     "ratio_to_cpu": round(g / base["value"], 2),
+                                     "workload": "the reference's own sample: the 200k-person region its set-up "
+                                                 "builds, 200 days, %d realisations in one launch" % base["realisations"]}
     out = {
         "metric": "person-days/s, 64-run ensemble",
         "value": value,
@@ -472,11 +506,18 @@ def uneven_regions(args):
     print(json.dumps(out), flush=True)
 
 
+# the kernels behind each timed kind whose traffic is summed (the profiler
+# measures kernels; the library times kinds)
+KIND_KERNELS = {"infect_places": ["infect_places", "place_contacts"],
+                "resolve": ["resolve_claim", "resolve_apply", "resolve_reset"]}
+
+
 def measured_traffic(kernel, args):
-    """HBM bytes per launch of `kernel` from the committed profiler summary
-    (profiles/traffic.json: rocprofv3 FETCH_SIZE and WRITE_SIZE per dispatch,
-    scaled by the same counters' reading for a copy of known size), when it
-    was measured on this workload; else None."""
+    """HBM bytes per launch of the timed kind `kernel` from the committed
+    profiler summary (profiles/traffic.json: rocprofv3 FETCH_SIZE and
+    WRITE_SIZE per dispatch, scaled by the same counters' reading for a copy
+    of known size), summed over the kind's kernels the summary holds, when
+    it was measured on this workload; else None."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(path):
         return None
@@ -485,8 +526,8 @@ def measured_traffic(kernel, args):
     w = t.get("workload", {})
     if (w.get("realisations"), w.get("persons"), w.get("days")) != (args.realisations, args.people, args.days):
         return None
-    k = t.get("kernels", {}).get(kernel)
-    return None if k is None else k.get("hbm_bytes_per_launch")
+    got = [t["kernels"][k]["hbm_bytes_per_launch"] for k in KIND_KERNELS.get(kernel, [kernel]) if k in t.get("kernels", {})]
+    return sum(got) if got else None
 
 
 def write_traffic(fetch_summary, write_summary, last, out, workload):

@@ -541,7 +541,7 @@ def write_traffic(fetch_summary, write_summary, last, out, workload):
         with open(path) as f:
             for line in f:
                 parts = [x.strip() for x in line.split("|")]
-                name = parts[0].split("(")[0]
+                name = short_name(parts[0])
                 m = re.search(counter + r" total \S+ mean \S+ max (\S+)(?: mean_last\d+ (\S+))?", line)
                 if m:
                     rows[name] = (float(m.group(1)), float(m.group(2)) if m.group(2) else None)
@@ -597,6 +597,13 @@ def summarise_pmc(d, last=0):
         print("%s | dispatches %d | %s" % (k[:80], len(per[min(per)]), " | ".join(parts)))
 
 
+def short_name(k):
+    """A kernel's name without its argument list ("(anonymous namespace)::"
+    kept out of the cut)."""
+    k = k.replace("(anonymous namespace)::", "")
+    return k.split("(")[0]
+
+
 def summarise_trace(d, last=0):
     """Per-kernel dispatch count, total and mean duration from a rocprofv3
     --kernel-trace CSV output directory (kernel_trace.csv), as a markdown
@@ -621,7 +628,7 @@ def summarise_trace(d, last=0):
     for k in sorted(runs, key=lambda k: -sum(x[1] for x in runs[k])):
         v = sorted(runs[k])
         tot = sum(x[1] for x in v)
-        line = "| %s | %d | %.1f | %.2f%% | %.1f |" % (k.split("(")[0], len(v), tot / 1e6, 100.0 * tot / tot_all,
+        line = "| %s | %d | %.1f | %.2f%% | %.1f |" % (short_name(k), len(v), tot / 1e6, 100.0 * tot / tot_all,
                                                       tot / len(v) / 1e3)
         if last:
             tail = v[-last:]

@@ -3,7 +3,7 @@
 // trained and evaluated on the GPU with hand-written BF16 MFMA kernels.
 //
 // Numerics: weights and activations are BF16 (round to nearest even);
-// every product is accumulated in FP32 by v_mfma_f32_16x16x16_bf16; the
+// every product is accumulated in FP32 by v_mfma_f32_16x16x32_bf16; the
 // pre-activations, the loss and the gradients of the weights are FP32; the
 // optimiser (Adam) keeps FP32 master weights and moments and refreshes the
 // BF16 copies after every step.
@@ -11,8 +11,11 @@
 // GEMM: C[M x N] = A[M x K] . B[K x N] with arbitrary element strides for A
 // and B, so one kernel serves the forward product (X . W^T), the input
 // gradient (dZ . W) and the weight gradient (dZ^T . X).  A 256-thread block
-// computes a 64 x 64 tile with four waves of 2 x 2 MFMA 16x16x16 tiles; the
-// K loop stages 64 x 16 slices of A and B through LDS.
+// computes a 64 x 64 tile with four waves of 2 x 2 MFMA 16x16x32 tiles (the
+// gfx950 BF16 shape: eight K values per lane); the K loop stages 64 x 32
+// slices of A and B through LDS, with one 16-byte load per thread and
+// operand where the operand is contiguous along K (X and W^T of the forward
+// product, dZ of the input gradient) and element loads elsewhere.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -43,7 +46,7 @@ void* g_scratch_addr[CS_MAX_DEVICES];  // device address of `scratch` on each de
 
 size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
 
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 __host__ __device__ __forceinline__ uint16_t f2bf(float f)
@@ -88,41 +91,50 @@ struct GemmArgs {
 
 #define TM 64
 #define TN 64
-#define TK 16
-#define LDK (TK + 4)  // padded row of a staged slice (keeps 8-byte fragment reads aligned)
+#define TK 32
+#define LDK (TK + 8)  // padded row of a staged slice: 80 bytes, so every 16-byte fragment read stays aligned
+
+// one thread's 8 K values of a staged row: a 16-byte load where the operand
+// is contiguous along K and the 8 values lie inside it, else element loads
+// (zero outside the matrix)
+__device__ __forceinline__ uint4 stage8(const uint16_t* p, long long ld_row, long long ld_k, int row, int rows, int k, int K,
+                                        bool vec)
+{
+    if (vec && row < rows && k + 8 <= K) return *reinterpret_cast<const uint4*>(p + (long long)row * ld_row + k);
+    uint16_t v[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) v[i] = (row < rows && k + i < K) ? p[(long long)row * ld_row + (long long)(k + i) * ld_k] : (uint16_t)0;
+    uint4 u;
+    memcpy(&u, v, 16);
+    return u;
+}
 
 __global__ void __launch_bounds__(256) k_gemm_bf16(GemmArgs g)
 {
-    __shared__ uint16_t As[TM][LDK];   // As[m][k]
-    __shared__ uint16_t Bt[TN][LDK];   // Bt[n][k]
+    __shared__ __align__(16) uint16_t As[TM][LDK];   // As[m][k]
+    __shared__ __align__(16) uint16_t Bt[TN][LDK];   // Bt[n][k]
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int bm = blockIdx.y * TM, bn = blockIdx.x * TN;
     const int wm = (w >> 1) * 32, wn = (w & 1) * 32;
     f32x4 acc[2][2];
     for (int a = 0; a < 2; a++)
         for (int b = 0; b < 2; b++) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
-    const int lr = tid >> 2, lk = (tid & 3) * 4;  // this thread stages row/column lr, k = lk..lk+3
+    // an operand takes 16-byte loads when it is contiguous along K with rows
+    // starting on 16-byte boundaries (k0 and lk are multiples of 8)
+    const bool a_vec = g.a_k == 1 && g.a_m % 8 == 0 && ((uintptr_t)g.A & 15) == 0;
+    const bool b_vec = g.b_k == 1 && g.b_n % 8 == 0 && ((uintptr_t)g.B & 15) == 0;
+    const int lr = tid >> 2, lk = (tid & 3) * 8;  // this thread stages row/column lr, k = lk..lk+7
     for (int k0 = 0; k0 < g.K; k0 += TK) {
-        {
-            const int m = bm + lr;
-            for (int i = 0; i < 4; i++) {
-                const int k = k0 + lk + i;
-                As[lr][lk + i] = (m < g.M && k < g.K) ? g.A[(long long)m * g.a_m + (long long)k * g.a_k] : (uint16_t)0;
-            }
-            const int n = bn + lr;
-            for (int i = 0; i < 4; i++) {
-                const int k = k0 + lk + i;
-                Bt[lr][lk + i] = (n < g.N && k < g.K) ? g.B[(long long)k * g.b_k + (long long)n * g.b_n] : (uint16_t)0;
-            }
-        }
+        *reinterpret_cast<uint4*>(&As[lr][lk]) = stage8(g.A, g.a_m, g.a_k, bm + lr, g.M, k0 + lk, g.K, a_vec);
+        *reinterpret_cast<uint4*>(&Bt[lr][lk]) = stage8(g.B, g.b_n, g.b_k, bn + lr, g.N, k0 + lk, g.K, b_vec);
         __syncthreads();
-        // fragments: lane l holds row/column (l % 16) and k = 4 * (l / 16) + 0..3
-        const int fr = lane & 15, fk = (lane >> 4) * 4;
-        bf16x4 af[2], bfr[2];
-        for (int a = 0; a < 2; a++) af[a] = *reinterpret_cast<const bf16x4*>(&As[wm + a * 16 + fr][fk]);
-        for (int b = 0; b < 2; b++) bfr[b] = *reinterpret_cast<const bf16x4*>(&Bt[wn + b * 16 + fr][fk]);
+        // fragments: lane l holds row/column (l % 16) and k = 8 * (l / 16) + 0..7
+        const int fr = lane & 15, fk = (lane >> 4) * 8;
+        bf16x8 af[2], bfr[2];
+        for (int a = 0; a < 2; a++) af[a] = *reinterpret_cast<const bf16x8*>(&As[wm + a * 16 + fr][fk]);
+        for (int b = 0; b < 2; b++) bfr[b] = *reinterpret_cast<const bf16x8*>(&Bt[wn + b * 16 + fr][fk]);
         for (int a = 0; a < 2; a++)
-            for (int b = 0; b < 2; b++) acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af[a], bfr[b], acc[a][b], 0, 0, 0);
+            for (int b = 0; b < 2; b++) acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[a], bfr[b], acc[a][b], 0, 0, 0);
         __syncthreads();
     }
     // result: lane l holds column (l % 16), rows 4 * (l / 16) + 0..3 of each 16 x 16 tile

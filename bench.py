@@ -688,6 +688,11 @@ def calibration(args):
         "fitted_r0_difference": abs(plain["theta"][0] - emul["theta"][0]),
         "realisation_ratio": plain["realisations"] / max(emul["realisations"], 1),
         "training_data_seconds": t_data, "training_seconds": t_train,
+        # the whole cost: one calibration that also pays for its emulator's
+        # training data and training, and the number of calibrations on one
+        # region and parameter family after which the emulator has paid off
+        "speedup_with_training": plain["seconds"] / (emul["seconds"] + t_data + t_train),
+        "break_even_calibrations": (t_data + t_train) / max(plain["seconds"] - emul["seconds"], 1e-9),
         "emulator_heldout_rel_l2": err,
     }
     print(json.dumps(out), flush=True)

@@ -117,6 +117,8 @@ def declare(lib, prefix):
         "update_probs": (I, [vp]),
         "run_steps": (I, [vp, D, I, I]),
         "set_step_graphs": (I, [vp, I]),
+        "set_dest_tables": (I, [vp, I]),
+        "get_dest_tables": (I, [vp, P(I)]),
         "set_launch_shape": (I, [vp, C.c_uint32, C.c_uint32]),
         "autotune": (I, [vp, D, I, P(cs_tuning)]),
         "set_queue_capacity": (I, [vp, C.c_uint32, C.c_uint32]),

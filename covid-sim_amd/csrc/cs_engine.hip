@@ -16,6 +16,12 @@
 
 using cs::Dev;
 
+// destination tables chosen at allocation: the full NC x NC max_trans and
+// per-realisation cum_trans rows while (realisations + 1) x NC^2 floats stay
+// within this many bytes, else checkpoints every CS_DEFAULT_CK_STRIDE cells
+#define CS_FULL_TABLE_BYTES (16ull << 30)
+#define CS_DEFAULT_CK_STRIDE 128
+
 struct cs_engine {
     char err[512];
     int device = 0;
@@ -34,6 +40,8 @@ struct cs_engine {
     double* rbeta_buf = nullptr;     // [R] storage behind Dev.rbeta
     int32_t* rseed_buf = nullptr;    // [R] storage behind Dev.rseed_n
     double* rthr_buf = nullptr;      // [R] storage behind Dev.rthr
+    int dest_stride = -1;            // destination tables asked for: -1 chosen at allocation, 0 full, > 0 checkpoint stride
+    float* mt_buf = nullptr;         // [NC][NC] max_trans (full tables only; built when first needed)
     int ts_max = 0, ts_n = 0;
     std::vector<uint8_t> flags0;     // [N] each person's Hot flags after a reset (no links, household size), for loaded states
     std::vector<uint8_t> h_age, h_flags;  // [N] the copies a loaded state's Hot records take
@@ -257,7 +265,11 @@ int launch_infect_events(cs_engine* e)
     }
     { ProfScope ps(e, CS_K_CELL_SPATIAL_SUM); cs::k_cell_spatial_sum<<<e->grid_cells, CS_SUM_BLOCK, 0, st>>>(d); }
     { ProfScope ps(e, CS_K_OTHER); cs::k_compact<<<grid_for(nrc, 256), 256, 0, st>>>(d, 2, 0); }
-    { ProfScope ps(e, CS_K_SPATIAL_DRAWS); cs::k_spatial_draws<<<e->grid_waves, 256, 0, st>>>(d); }
+    {
+        ProfScope ps(e, CS_K_SPATIAL_DRAWS);
+        if (d.ck) cs::k_spatial_draws<true><<<e->grid_waves, 256, 0, st>>>(d);
+        else cs::k_spatial_draws<false><<<e->grid_waves, 256, 0, st>>>(d);
+    }
     HIPCHK(e, hipGetLastError());
     return 0;
 }
@@ -325,13 +337,31 @@ int launch_incub(cs_engine* e)
     return 0;
 }
 
+// the full max_trans table (src/Kernels.cpp:48-63), built once per
+// population and kernel
+int ensure_max_trans(cs_engine* e)
+{
+    if (e->mt_buf) return 0;
+    const size_t NC = (size_t)e->NC;
+    if (hipMalloc(&e->mt_buf, NC * NC * sizeof(float)) != hipSuccess) {
+        (void)hipGetLastError();
+        e->mt_buf = nullptr;
+        return set_err(e, CS_ENOMEM, "max_trans: %zu x %zu cells (use checkpointed destination tables)", NC, NC);
+    }
+    e->d.max_trans = e->mt_buf;
+    cs::k_build_max_trans<<<grid_for((long long)NC * NC, 256, 1u << 30), 256, 0, e->stream>>>(e->d, e->mt_buf);
+    HIPCHK(e, hipGetLastError());
+    return check_flags(e);
+}
+
 int launch_update_probs(cs_engine* e, const int* sel)
 {
     Dev& d = e->d;
     long long n = (long long)d.R * d.NC;
     ProfScope ps(e, CS_K_UPDATE_PROBS);
     cs::k_update_s0<<<grid_for(n, 256), 256, 0, e->stream>>>(d, sel);
-    cs::k_update_rows<<<grid_for(n, 64), 64, 0, e->stream>>>(d, sel);
+    if (d.ck) cs::k_update_rows_ck<<<grid_for(n, 64), 64, 0, e->stream>>>(d, sel);
+    else cs::k_update_rows<<<grid_for(n, 64), 64, 0, e->stream>>>(d, sel);
     HIPCHK(e, hipGetLastError());
     return 0;
 }
@@ -549,6 +579,7 @@ void cs_destroy(cs_engine* e)
     drop_graph(e);
     free_list(e->real_allocs);
     free_list(e->pop_allocs);
+    if (e->mt_buf) (void)hipFree(e->mt_buf);
     if (e->dP) (void)hipFree(e->dP);
     if (e->dkl) (void)hipFree(e->dkl);
     if (e->dkh) (void)hipFree(e->dkh);
@@ -604,8 +635,8 @@ int cs_set_params(cs_engine* e, const cs_params* p, const double* kl, const doub
         e->due_ok = !p->do_latent || (cmin > 0 && steps + 3.0 < (double)CS_DUE_RING);
         if (!e->due_ok) e->d.due_on = 0;
     }
-    if (e->have_pop && new_tables) {
-        cs::k_build_max_trans<<<grid_for((long long)e->NC * e->NC, 256, 1u << 30), 256, 0, e->stream>>>(e->d);
+    if (e->have_pop && new_tables && e->mt_buf) {
+        cs::k_build_max_trans<<<grid_for((long long)e->NC * e->NC, 256, 1u << 30), 256, 0, e->stream>>>(e->d, e->mt_buf);
         HIPCHK(e, hipGetLastError());
         int rc = check_flags(e);
         if (rc) return rc;
@@ -789,12 +820,15 @@ int cs_set_population(cs_engine* e, const cs_population* pop)
                     return set_err(e, CS_EINVAL, "person %d group link beyond the place's groups", i);
             }
         }
-    float* mt;
-    if ((rc = dalloc(e, L, &mt, (size_t)NC * NC))) return rc;
-    d.max_trans = mt;
+    // max_trans is built when full tables are chosen (cs_alloc_realisations)
+    if (e->mt_buf) (void)hipFree(e->mt_buf);
+    e->mt_buf = nullptr;
+    d.max_trans = nullptr;
+    d.ck = d.ck_nb = 0;
+    d.ck_pre = nullptr;
+    d.cum_trans = nullptr;
+    d.invcdf = nullptr;
     HIPCHK(e, hipMemset(d.status, 0, sizeof(unsigned)));
-    cs::k_build_max_trans<<<grid_for((long long)NC * NC, 256, 1u << 30), 256, 0, e->stream>>>(d);
-    HIPCHK(e, hipGetLastError());
     e->have_pop = true;
     return check_flags(e);
 }
@@ -831,7 +865,20 @@ int cs_alloc_realisations(cs_engine* e, int n_real, const uint64_t* seeds)
     AL(listpos, RN); AL(infector, RN); AL(susc, RN); AL(prob_care, RN);
     AL(t_ts, RN); AL(t_te, RN); AL(t_vs, RN); AL(t_is, RN); AL(t_qs, RN);
     AL(cS, RC); AL(cL, RC); AL(cI, RC); AL(cR, RC); AL(cD, RC); AL(cumTC, RC); AL(S0, RC);
-    AL(tot_prob, RC); AL(cum_trans, RC * NC); AL(invcdf, RC * 1025); AL(perm, RN);
+    AL(tot_prob, RC); AL(perm, RN);
+    {   // destination tables: full NC x NC rows while they fit, else checkpoints
+        int stride = e->dest_stride;
+        if (stride < 0) stride = ((R + 1) * NC * NC * 4 <= CS_FULL_TABLE_BYTES) ? 0 : CS_DEFAULT_CK_STRIDE;
+        d.ck = stride;
+        d.ck_nb = stride ? (int)((NC + stride - 1) / stride) : 0;
+        d.cum_trans = nullptr; d.invcdf = nullptr; d.ck_pre = nullptr;
+        if (stride) {
+            AL(ck_pre, RC * d.ck_nb);
+        } else {
+            AL(cum_trans, RC * NC); AL(invcdf, RC * 1025);
+            if ((rc = ensure_max_trans(e))) return rc;
+        }
+    }
     AL(pl_cs, PL); AL(pl_ce, PL); AL(m_socdist, R * NM); AL(m_moverest, R * NM); AL(m_trig, R * NM * 6);
     AL(ist, R); AL(trig_hist, R * CS_TRIG_RING * 3); AL(mc_state, R * NM); AL(mc_end, R * NM); AL(pl_once, PL); AL(itv_n, 4);
     d.det = nullptr; d.close_l = nullptr; d.det_cap = d.close_cap = 0;  // sized when interventions first run (ensure_itv)
@@ -1366,6 +1413,20 @@ int cs_autotune(cs_engine* e, double t, int reps, cs_tuning* out)
     return 0;
 }
 
+int cs_set_dest_tables(cs_engine* e, int stride)
+{
+    if (!e || stride < -1) return CS_EINVAL;
+    e->dest_stride = stride;
+    return 0;
+}
+
+int cs_get_dest_tables(const cs_engine* e, int* stride)
+{
+    if (!e || !stride) return CS_EINVAL;
+    *stride = e->R ? e->d.ck : e->dest_stride;
+    return 0;
+}
+
 int cs_set_step_graphs(cs_engine* e, int enable)
 {
     if (!e) return CS_EINVAL;
@@ -1558,6 +1619,31 @@ int cs_get_cell_tables(cs_engine* e, int real, float* mt, float* ct, int32_t* in
     HIPCHK(e, hipStreamSynchronize(e->stream));
     const size_t NC = e->NC;
     Dev& d = e->d;
+    if (d.ck || (mt && !e->mt_buf && e->R == 0)) {  // no stored tables: expand them
+        DBuf<float> m, c;
+        DBuf<int32_t> v;
+        if (mt) {
+            TMP_UP(e, m, (const float*)nullptr, NC * NC);
+            cs::k_build_max_trans<<<grid_for((long long)NC * NC, 256, 1u << 30), 256, 0, e->stream>>>(d, m.p);
+            HIPCHK(e, hipStreamSynchronize(e->stream));
+            HIPCHK(e, hipMemcpy(mt, m.p, NC * NC * 4, hipMemcpyDeviceToHost));
+        }
+        if (real >= 0 && real < e->R && (ct || inv)) {
+            TMP_UP(e, c, (const float*)nullptr, NC * NC);
+            TMP_UP(e, v, (const int32_t*)nullptr, NC * 1025);
+            cs::k_expand_rows<<<grid_for((long long)NC, 64), 64, 0, e->stream>>>(d, real, nullptr, c.p, v.p);
+            HIPCHK(e, hipStreamSynchronize(e->stream));
+            if (ct) HIPCHK(e, hipMemcpy(ct, c.p, NC * NC * 4, hipMemcpyDeviceToHost));
+            if (inv) HIPCHK(e, hipMemcpy(inv, v.p, NC * 1025 * 4, hipMemcpyDeviceToHost));
+        }
+        if (real >= 0 && real < e->R) {
+            const size_t oc = (size_t)real * NC;
+            if (tp) HIPCHK(e, hipMemcpy(tp, d.tot_prob + oc, NC * 4, hipMemcpyDeviceToHost));
+            if (S0) HIPCHK(e, hipMemcpy(S0, d.S0 + oc, NC * 4, hipMemcpyDeviceToHost));
+        }
+        int rc = check_flags(e);
+        return rc;
+    }
     if (mt) HIPCHK(e, hipMemcpy(mt, d.max_trans, NC * NC * 4, hipMemcpyDeviceToHost));
     if (real >= 0 && real < e->R) {
         const size_t oc = (size_t)real * NC;

@@ -272,14 +272,21 @@ __global__ void k_step_end(Dev d, int sample_done)
 
 // ---------------------------------------------------------------- tables
 
-// max_trans[l][m] = (float)num(dist2_cc_min(l, m)) (src/Kernels.cpp:48-63)
-__global__ void k_build_max_trans(Dev d)
+// max_trans[l][m] = (float)num(dist2_cc_min(l, m)) (src/Kernels.cpp:48-63),
+// with l and m grid indices of populated cells
+__device__ __forceinline__ float max_trans_of(const Dev& d, int gl, int gm)
+{
+    return (float)kernel_num(d, dist2_cc_min(d.P, gl, gm));
+}
+
+// the whole table, into out ([NC][NC])
+__global__ void k_build_max_trans(Dev d, float* out)
 {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     size_t n = (size_t)d.NC * d.NC;
     if (i >= n) return;
     int l = (int)(i / d.NC), m = (int)(i % d.NC);
-    const_cast<float*>(d.max_trans)[i] = (float)kernel_num(d, dist2_cc_min(d.P, d.cidx[l], d.cidx[m]));
+    out[i] = max_trans_of(d, d.cidx[l], d.cidx[m]);
 }
 
 // UpdateProbs(0), S0 part (src/CovidSim.cpp:2756-2768); only realisations
@@ -331,6 +338,66 @@ __global__ void k_update_rows(Dev d, const int* sel)
         // in select_dest need not read cum_trans (the same cell either way)
         const bool pure = k < 1024 && (double)ct[m] >= (double)(k + 1) / 1024.0;
         inv[k] = pure ? (int32_t)((uint32_t)m | CS_INV_PURE) : m;
+    }
+}
+
+// UpdateProbs(0) rows with checkpointed tables: the same sequential float
+// sums as k_update_rows (src/CovidSim.cpp:2780-2800), with max_trans
+// evaluated term by term; the running prefix is kept at the last cell of
+// every block of ck cells (the last block's entry is the row total), so a
+// destination search can redo any block's prefixes exactly from the entry
+// before it
+__global__ void k_update_rows_ck(Dev d, const int* sel)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)d.R * d.NC) return;
+    const int NC = d.NC, B = d.ck;
+    int r = (int)(i / NC), j = (int)(i % NC);
+    if (sel && !sel[r]) return;
+    const int32_t* S0 = d.S0 + (size_t)r * NC;
+    float* pre = d.ck_pre + i * d.ck_nb;
+    const int gj = d.cidx[j];
+    float c = 0.f, t = 0.f;  // 0 + x == x: the first term is the reference's ct[0] and t
+    for (int m0 = 0, b = 0; m0 < NC; m0 += B, b++) {
+        const int m1 = min(NC, m0 + B);
+        for (int m = m0; m < m1; m++) {
+            const float mt = max_trans_of(d, gj, d.cidx[m]);
+            c = c + ((float)S0[m]) * mt;
+            t = t + ((float)d.cn[m]) * mt;
+        }
+        pre[b] = c;
+    }
+    d.tot_prob[i] = c / t;
+}
+
+// cum_trans, InvCDF and max_trans of realisation r expanded from the
+// checkpoints (each block's prefixes redone from the entry before it, as the
+// destination search redoes them), for cs_get_cell_tables
+__global__ void k_expand_rows(Dev d, int r, float* mt_out, float* ct_out, int32_t* inv_out)
+{
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= d.NC) return;
+    const int NC = d.NC, B = d.ck, nb = d.ck_nb;
+    const size_t rc = (size_t)r * NC + j;
+    const float* pre = d.ck_pre + rc * nb;
+    const int32_t* S0 = d.S0 + (size_t)r * NC;
+    const float tp = pre[nb - 1];
+    const int gj = d.cidx[j];
+    float* ct = ct_out + (size_t)j * NC;
+    for (int b = 0; b < nb; b++) {
+        float c = b ? pre[b - 1] : 0.f;
+        for (int m = b * B; m < min(NC, b * B + B); m++) {
+            const float mt = max_trans_of(d, gj, d.cidx[m]);
+            c = c + ((float)S0[m]) * mt;
+            ct[m] = c / tp;
+            if (mt_out) mt_out[(size_t)j * NC + m] = mt;
+        }
+    }
+    int32_t* inv = inv_out + (size_t)j * 1025;
+    int m = 0;
+    for (int k = 0; k <= 1024; k++) {
+        while (m < NC - 1 && ct[m] * 1024 < ((float)k)) m++;  // the last prefix is 1: the bound never binds
+        inv[k] = m;
     }
 }
 
@@ -674,6 +741,41 @@ __device__ __forceinline__ int select_dest(const Dev& d, size_t rc, double u, un
     // the reference reads cum_trans[l0 .. l] (one entry for a pure bucket)
     if (scanned) *scanned += (unsigned)(l - l0 + 1);
     return l;
+}
+
+// destination cell with checkpointed tables: the first cell whose
+// normalised prefix reaches u, which is where the InvCDF start and the
+// cum_trans scan stop (src/Sweep.cpp:757-762: every cell before
+// InvCDF[floor(1024 u)] has a prefix below floor(1024 u) / 1024 <= u).  A
+// binary search over the blocks' last prefixes finds the block, whose
+// prefixes are then redone from the entry before it in the order
+// k_update_rows_ck summed them, so each is the flat table's value.  *mt is
+// max_trans of (source, destination), the acceptance test's divisor.
+__device__ __forceinline__ int select_dest_ck(const Dev& d, int r, int c, double u, float* mt, unsigned* scanned = nullptr)
+{
+    const int NC = d.NC, B = d.ck, nb = d.ck_nb;
+    const float* pre = d.ck_pre + ((size_t)r * NC + c) * nb;
+    const float tp = pre[nb - 1];
+    int lo = 0, hi = nb - 1;  // the last block's prefix is tp: tp / tp = 1 > u
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if ((double)(pre[mid] / tp) >= u) hi = mid;
+        else lo = mid + 1;
+    }
+    float s = lo ? pre[lo - 1] : 0.f;
+    const int32_t* S0 = d.S0 + (size_t)r * NC;
+    const int gc = d.cidx[c];
+    const int m1 = min(NC, lo * B + B) - 1;
+    int m = lo * B;
+    float w;
+    for (;; m++) {
+        w = max_trans_of(d, gc, d.cidx[m]);
+        s = s + ((float)S0[m]) * w;
+        if (m == m1 || (double)(s / tp) >= u) break;  // the block's last prefix reaches u
+    }
+    if (scanned) *scanned += (unsigned)(m - lo * B + 1);
+    *mt = w;
+    return m;
 }
 
 // SampleWithoutReplacement replacement: all / Floyd (k <= 32) / selection
@@ -1063,6 +1165,8 @@ __global__ void __launch_bounds__(CS_SUM_BLOCK) k_cell_spatial_sum(Dev d)
 // sequential loop makes (those up to the accepted one).
 // at most 96 VGPRs (5 waves per SIMD, no spills): each attempt is a chain of
 // dependent gathers, hidden by the waves in flight and by the groups
+// CK: checkpointed destination tables (max_trans evaluated where read)
+template <bool CK>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k_spatial_draws(Dev d)
 {
     const StepScalars sc = *d.sc;
@@ -1125,9 +1229,14 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k
             int8_t inf_m = 0;
             if (tries) {  // attempt j (src/Sweep.cpp:753-788)
                 t = stream_at(seed, sw0, (uint32_t)c, (uint32_t)k_o, v0 + 3u * (unsigned)j);
-                l = select_dest(d, rc, t.next(), &scanned);
+                float mt;
+                if constexpr (CK) {
+                    l = select_dest_ck(d, r, c, t.next(), &mt, &scanned);
+                } else {
+                    l = select_dest(d, rc, t.next(), &scanned);
+                    mt = d.max_trans[(size_t)c * d.NC + l];  // issued with the slot's loads
+                }
                 const size_t rl = (size_t)r * d.NC + l;
-                const float mt = d.max_trans[(size_t)c * d.NC + l];  // issued with the slot's loads
                 slot = (int)(t.next() * ((double)d.S0[rl]));
                 m = d.perm[rN + d.cstart[l] + slot];
                 pm = d.pst[m];

@@ -673,7 +673,13 @@ __global__ void k_t_select_infector(const float* ci, int n_inf, const double* u,
 __global__ void k_t_select_dest(Dev d, int r, int c, const double* u, int32_t* out, int n)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) out[i] = select_dest(d, (size_t)r * d.NC + c, u[i]);
+    if (i >= n) return;
+    if (d.ck) {
+        float mt;
+        out[i] = select_dest_ck(d, r, c, u[i], &mt);
+    } else {
+        out[i] = select_dest(d, (size_t)r * d.NC + c, u[i]);
+    }
 }
 
 __global__ void k_t_philox(const uint32_t* ck, uint32_t* out, int n)

@@ -300,8 +300,13 @@ struct Dev {
     uint16_t *t_ts, *t_te, *t_vs, *t_is, *t_qs;
     int32_t *cS, *cL, *cI, *cR, *cD, *cumTC, *S0;
     float* tot_prob;
-    float* cum_trans;
-    int32_t* invcdf;
+    float* cum_trans;     // full tables (ck == 0): [R][NC][NC] normalised prefixes
+    int32_t* invcdf;      // full tables: [R][NC][1025]
+    // checkpointed tables (ck > 0), for regions whose NC x NC tables do not
+    // fit: max_trans is evaluated where it is read, and each row keeps its
+    // sequential float prefix only at the last cell of every ck cells
+    float* ck_pre;        // [R][NC][ck_nb], the last entry being the row total
+    int ck, ck_nb;
     int32_t* perm;
     uint16_t *pl_cs, *pl_ce;  // [R][total places]
     uint8_t *m_socdist, *m_moverest;

@@ -167,6 +167,15 @@ class Engine:
         self.call("autotune", float(t), int(reps), C.byref(tu))
         return {f: getattr(tu, f) for f, _ in tu._fields_}
 
+    def set_dest_tables(self, stride):
+        """Destination tables at the next alloc: 0 full, > 0 checkpoint stride, -1 automatic."""
+        self.call("set_dest_tables", int(stride))
+
+    def dest_tables(self):
+        s = C.c_int()
+        self.call("get_dest_tables", C.byref(s))
+        return s.value
+
     def set_step_graphs(self, on):
         self.call("set_step_graphs", int(bool(on)))
 

@@ -438,6 +438,21 @@ int  cs_batch_create(cs_batch** out, cs_engine* const* engines, int n);
 void cs_batch_destroy(cs_batch* b);
 const char* cs_batch_last_error(const cs_batch* b);
 int  cs_batch_run_steps(cs_batch* b, double t0, int n_steps, int steps_per_sample);
+/* destination tables of the spatial draws, chosen at the next
+ * cs_alloc_realisations.  stride 0: the reference's full tables, max_trans
+ * [NC][NC] and per realisation cum_trans [NC][NC] with InvCDF [NC][1025]
+ * (src/CovidSim.cpp:2752-2802, src/Kernels.cpp:48-63); stride > 0:
+ * checkpointed tables, for regions whose NC x NC tables do not fit (NC =
+ * 224,448 cells makes one table 201 GB): max_trans is evaluated where it is
+ * read and each cum_trans row keeps its sequential float prefix at every
+ * stride-th cell only (R x NC x ceil(NC / stride) floats), a destination
+ * search redoing one block's prefixes in the same order.  Both give the
+ * same destinations for the same uniforms, bit for bit.  -1 (default):
+ * full tables while (realisations + 1) x NC^2 floats fit 16 GiB, else
+ * stride 128.  cs_get_dest_tables gives the stride in use after an
+ * allocation (the requested value before). */
+int  cs_set_dest_tables(cs_engine* e, int stride);
+int  cs_get_dest_tables(const cs_engine* e, int* stride);
 /* 1 (default): cs_run_steps replays captured step graphs; 0: it launches
  * every kernel of every step from the host */
 int  cs_set_step_graphs(cs_engine* e, int enable);

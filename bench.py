@@ -203,7 +203,9 @@ def main():
     ap.add_argument("--no-graphs", action="store_true", help="launch every kernel directly in the timed steps too "
                                                             "(for runs under rocprofv3 tracing)")
     ap.add_argument("--calibration", action="store_true", help="measure emulator-guided vs plain calibration instead")
-    ap.add_argument("--config", choices=["headline", "c3"], default="headline",
+    ap.add_argument("--cells", type=int, nargs=2, default=[668, 336], metavar=("WIDE", "HIGH"),
+                    help="with --config large: the region's cells (0.075 degrees each)")
+    ap.add_argument("--config", choices=["headline", "c3", "large"], default="headline",
                     help="c3: the 64-realisation ensemble over five regions of uneven size (see above)")
     ap.add_argument("--summarise-pmc", metavar="DIR", help="print per-kernel sums of a rocprofv3 --pmc output "
                                                             "directory (counter_collection.csv) and exit")
@@ -237,6 +239,8 @@ def main():
         return calibration(args)
     if args.config == "c3":
         return uneven_regions(args)
+    if args.config == "large":
+        return large_region(args)
     import importlib
     csa = importlib.import_module("covid-sim_amd")
     eng_mod = importlib.import_module("covid-sim_amd.engine")
@@ -501,6 +505,84 @@ def uneven_regions(args):
                      "measured": "HIP events around every launch of a profiled pass of the same workload "
                                  "(%.2f s, engines one after another, direct launches)" % t_prof,
                      "top_kernels": top},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def large_region(args):
+    """One large region (--config large): its population generated on the
+    device (cs_popgen_*: SetupPopulation, AssignHouseholdAges,
+    AssignPeopleToPlaces, StratifyPlaces) from a synthetic density, then one
+    realisation simulated with the destination tables the region's size
+    calls for (checkpointed beyond NC x NC that fit)."""
+    import importlib
+    csa = importlib.import_module("covid-sim_amd")
+    eng_mod = importlib.import_module("covid-sim_amd.engine")
+    sm = importlib.import_module("covid-sim_amd.setup_model")
+    inp = os.path.join(ROOT, "covid-sim_amd", "inputs", "c1")
+    base_pg = sm.load_popgen_params(os.path.join(inp, "popgen_params.bin"))
+    base_kp = csa.population.load_params(os.path.join(inp, "params.bin"))
+    ncw, nch = args.cells
+    pg, kp, dens, country, adunit = sm.synthetic_region(args.people, ncw, nch, 2025, base_pg, base_kp)
+    log("region: %d persons, %d x %d cells, %d microcells" % (args.people, ncw, nch, pg.mcw * pg.mch))
+    gen = sm.PopGen(0)
+    t0 = time.perf_counter()
+    sizes = gen.run(pg, kp, dens, country, adunit, 77)
+    t_gen = time.perf_counter() - t0
+    log("generated on the device in %.2f s (%.3g persons/s): %d households, %d populated cells, places %s, "
+        "unplaced %s" % (t_gen, args.people / t_gen, sizes.n_households, sizes.n_cells, list(sizes.n_places)[:4],
+                         list(sizes.n_unplaced)[:4]))
+    t1 = time.perf_counter()
+    arrays = gen.fetch(pg)
+    gen.close()
+    pop = csa.population.Population(arrays)
+    eng = eng_mod.Engine(0)
+    eng.set_params(kp)
+    eng.set_population(pop)
+    c = eng.transmission_coeffs(5)
+    eng.alloc(np.array([0x9E3779B97F4A7C15], np.uint64))
+    stride = eng.dest_tables()
+    t_setup = time.perf_counter() - t1
+    log("engine set-up %.1f s (host copy and checks); R0 %.2f (household %.2f, places %.2f), LocalBeta %.4g; "
+        "destination tables: %s" % (t_setup, c.r0, c.r0_household, c.r0_places, c.local_beta,
+                                    "full" if stride == 0 else "checkpoints every %d cells" % stride))
+    tspd = int(round(kp.time_steps_per_day))
+    eng.reset(10)
+    eng.sync()
+    ta = time.perf_counter()
+    chunk = max(1, args.chunk_days) * tspd
+    steps = args.days * tspd
+    for s0 in range(0, steps, chunk):
+        n = min(chunk, steps - s0)
+        eng.run_steps(s0 / tspd, n, tspd)
+        eng.sync()
+        log("  day %d (%.1f s)" % ((s0 + n) // tspd, time.perf_counter() - ta))
+    dt = time.perf_counter() - ta
+    ts = eng.timeseries(args.days)
+    attack = float((ts[0, -1, 3] + ts[0, -1, 4]) / args.people)
+    out = {
+        "metric": "person-days/s, one large region",
+        "value": args.people * args.days / dt,
+        "unit": "person-days/s",
+        "n_gpus": 1,
+        "steps": 1,
+        "warmup": 0,
+        "ms_per_step": dt * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "one region of %d persons on %d x %d cells (%d populated), population generated on "
+                               "the device, 1 realisation, %d days, 4 steps/day, 10 seeds, no interventions"
+                               % (args.people, ncw, nch, sizes.n_cells, args.days),
+                   "persons": args.people, "cells": int(sizes.n_cells), "days": args.days,
+                   "generation_seconds": round(t_gen, 3), "generation_persons_per_s": args.people / t_gen,
+                   "engine_setup_seconds": round(t_setup, 1),
+                   "dest_tables": "full" if stride == 0 else "checkpoints every %d cells" % stride,
+                   "r0": round(c.r0, 3), "attack_rate": round(attack, 4)},
+        "roofline": None,
         "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)

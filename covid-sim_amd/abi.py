@@ -94,6 +94,9 @@ cs_work_counts = _make("cs_work_counts")
 cs_transmission_coeffs = _make("cs_transmission_coeffs")
 cs_tuning = _make("cs_tuning")
 cs_intervention_state = _make("cs_intervention_state")
+cs_popgen_params = _make("cs_popgen_params")
+cs_popgen_sizes = _make("cs_popgen_sizes")
+cs_popgen_out = _make("cs_popgen_out")
 
 globals().update({k: v for k, v in CONSTS.items()})
 

@@ -111,20 +111,6 @@ void free_list(std::vector<void*>& list)
 
 // KernelStruct families (src/Kernels.cpp:68-106) with the host's libm, so
 // the tables equal KernelLookup::init's (src/Kernels.cpp:17-45, norm 1.0)
-double kernel_fn(const cs_params& P, double r2)
-{
-    const double sh = P.kernel_shape, sc = P.kernel_scale, p3 = P.kernel_p3, p4 = P.kernel_p4;
-    double t, dd;
-    switch (P.kernel_type) {
-    case 1: return exp(-sqrt(r2) / sc);
-    case 2: t = -sh * log(sqrt(r2) / sc + 1.0); return (t < -690.0) ? 0.0 : exp(t);
-    case 3: return exp(-r2 / (sc * sc));
-    case 4: return (r2 > sc * sc) ? 0.0 : 1.0;
-    case 5: t = 0.5 * sh * log(r2 / (sc * sc)); return (t > 690.0) ? 0.0 : (1.0 / (exp(t) + 1.0));
-    case 6: t = log(sqrt(r2) / sc + 1.0); return (t < -690.0) ? 0.0 : (exp(-sh * t) + p3 * exp(-p4 * t)) / (1.0 + p3);
-    default: dd = sqrt(r2); t = -sh * log(dd / sc + 1.0); return (t < -690.0) ? 0.0 : exp(t - pow(dd / p3, p4));
-    }
-}
 
 hipEvent_t pool_event(cs_engine* e)
 {
@@ -612,8 +598,8 @@ int cs_set_params(cs_engine* e, const cs_params* p, const double* kl, const doub
             hl.resize(n);
             hh.resize(n);
             for (size_t i = 0; i < n; i++) {
-                hl[i] = kernel_fn(*p, (int)i * p->kernel_delta) / 1.0;
-                hh[i] = kernel_fn(*p, (int)i * p->kernel_delta / p->kernel_expansion) / 1.0;
+                hl[i] = cs::kernel_fn(*p, (int)i * p->kernel_delta) / 1.0;
+                hh[i] = cs::kernel_fn(*p, (int)i * p->kernel_delta / p->kernel_expansion) / 1.0;
             }
         }
         if (e->dkl) (void)hipFree(e->dkl);

@@ -15,7 +15,7 @@
 //    positions of the leaving batch below hi-k are paired in order with the
 //    sorted free positions in [hi-k, hi).
 #pragma once
-#include "cs_model.h"
+#include "cs_rows.h"
 
 namespace cs {
 
@@ -25,7 +25,6 @@ namespace cs {
 // spatial draws per work-list chunk (a wave's 64 lanes share them)
 #define CS_DRAW_CHUNK 64
 // InvCDF entry flag: the bucket's draws all stop at the entry's cell
-#define CS_INV_PURE 0x80000000u
 
 __device__ __forceinline__ uint64_t ev_key(int infector, int pass, uint32_t sub)
 {
@@ -272,13 +271,6 @@ __global__ void k_step_end(Dev d, int sample_done)
 
 // ---------------------------------------------------------------- tables
 
-// max_trans[l][m] = (float)num(dist2_cc_min(l, m)) (src/Kernels.cpp:48-63),
-// with l and m grid indices of populated cells
-__device__ __forceinline__ float max_trans_of(const Dev& d, int gl, int gm)
-{
-    return (float)kernel_num(d, dist2_cc_min(d.P, gl, gm));
-}
-
 // the whole table, into out ([NC][NC])
 __global__ void k_build_max_trans(Dev d, float* out)
 {
@@ -307,67 +299,24 @@ __global__ void k_update_s0(Dev d, const int* sel)
 }
 
 // UpdateProbs(0), per-row sequential float prefix (src/CovidSim.cpp:2780-2800).
-// One thread per (realisation, source cell); the row is summed in the
-// reference's order so the float results are identical.
+// One thread per (realisation, source cell).
 __global__ void k_update_rows(Dev d, const int* sel)
 {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (size_t)d.R * d.NC) return;
-    const int NC = d.NC;
-    int r = (int)(i / NC), j = (int)(i % NC);
+    int r = (int)(i / d.NC), j = (int)(i % d.NC);
     if (sel && !sel[r]) return;
-    float* ct = d.cum_trans + i * NC;
-    const float* mt = d.max_trans + (size_t)j * NC;
-    const int32_t* S0 = d.S0 + (size_t)r * NC;
-    float t;
-    ct[0] = ((float)S0[0]) * mt[0];
-    t = ((float)d.cn[0]) * mt[0];
-    for (int m = 1; m < NC; m++) {
-        ct[m] = ct[m - 1] + ((float)S0[m]) * mt[m];
-        t += ((float)d.cn[m]) * mt[m];
-    }
-    float tp = ct[NC - 1];
-    for (int m = 0; m < NC; m++) ct[m] /= tp;
-    d.tot_prob[i] = tp / t;
-    int32_t* inv = d.invcdf + i * 1025;
-    int m = 0;
-    for (int k = 0; k <= 1024; k++) {
-        while (ct[m] * 1024 < ((float)k)) m++;
-        // bit 31: every u of bucket k (u < (k+1)/1024) stops at m, because
-        // cum_trans[m] already reaches the bucket's upper end, so the scan
-        // in select_dest need not read cum_trans (the same cell either way)
-        const bool pure = k < 1024 && (double)ct[m] >= (double)(k + 1) / 1024.0;
-        inv[k] = pure ? (int32_t)((uint32_t)m | CS_INV_PURE) : m;
-    }
+    update_row(d, r, j);
 }
 
-// UpdateProbs(0) rows with checkpointed tables: the same sequential float
-// sums as k_update_rows (src/CovidSim.cpp:2780-2800), with max_trans
-// evaluated term by term; the running prefix is kept at the last cell of
-// every block of ck cells (the last block's entry is the row total), so a
-// destination search can redo any block's prefixes exactly from the entry
-// before it
+// the same with checkpointed tables
 __global__ void k_update_rows_ck(Dev d, const int* sel)
 {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (size_t)d.R * d.NC) return;
-    const int NC = d.NC, B = d.ck;
-    int r = (int)(i / NC), j = (int)(i % NC);
+    int r = (int)(i / d.NC), j = (int)(i % d.NC);
     if (sel && !sel[r]) return;
-    const int32_t* S0 = d.S0 + (size_t)r * NC;
-    float* pre = d.ck_pre + i * d.ck_nb;
-    const int gj = d.cidx[j];
-    float c = 0.f, t = 0.f;  // 0 + x == x: the first term is the reference's ct[0] and t
-    for (int m0 = 0, b = 0; m0 < NC; m0 += B, b++) {
-        const int m1 = min(NC, m0 + B);
-        for (int m = m0; m < m1; m++) {
-            const float mt = max_trans_of(d, gj, d.cidx[m]);
-            c = c + ((float)S0[m]) * mt;
-            t = t + ((float)d.cn[m]) * mt;
-        }
-        pre[b] = c;
-    }
-    d.tot_prob[i] = c / t;
+    update_row_ck(d, r, j);
 }
 
 // cum_trans, InvCDF and max_trans of realisation r expanded from the
@@ -726,56 +675,6 @@ __device__ __forceinline__ int select_infector(const float* ci, int n, double u,
     } while (keep);
     if (probes) *probes += np;
     return idx;
-}
-
-// destination cell: InvCDF then linear scan of cum_trans (src/Sweep.cpp:757-762)
-__device__ __forceinline__ int select_dest(const Dev& d, size_t rc, double u, unsigned* scanned = nullptr)
-{
-    const int32_t v = d.invcdf[rc * 1025 + (int)__builtin_floor(u * 1024)];
-    int l = (int)((uint32_t)v & ~CS_INV_PURE);
-    const int l0 = l;
-    if (v >= 0) {  // not a pure bucket: the reference's scan
-        const float* ct = d.cum_trans + rc * d.NC;
-        while (ct[l] < u) l++;
-    }
-    // the reference reads cum_trans[l0 .. l] (one entry for a pure bucket)
-    if (scanned) *scanned += (unsigned)(l - l0 + 1);
-    return l;
-}
-
-// destination cell with checkpointed tables: the first cell whose
-// normalised prefix reaches u, which is where the InvCDF start and the
-// cum_trans scan stop (src/Sweep.cpp:757-762: every cell before
-// InvCDF[floor(1024 u)] has a prefix below floor(1024 u) / 1024 <= u).  A
-// binary search over the blocks' last prefixes finds the block, whose
-// prefixes are then redone from the entry before it in the order
-// k_update_rows_ck summed them, so each is the flat table's value.  *mt is
-// max_trans of (source, destination), the acceptance test's divisor.
-__device__ __forceinline__ int select_dest_ck(const Dev& d, int r, int c, double u, float* mt, unsigned* scanned = nullptr)
-{
-    const int NC = d.NC, B = d.ck, nb = d.ck_nb;
-    const float* pre = d.ck_pre + ((size_t)r * NC + c) * nb;
-    const float tp = pre[nb - 1];
-    int lo = 0, hi = nb - 1;  // the last block's prefix is tp: tp / tp = 1 > u
-    while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if ((double)(pre[mid] / tp) >= u) hi = mid;
-        else lo = mid + 1;
-    }
-    float s = lo ? pre[lo - 1] : 0.f;
-    const int32_t* S0 = d.S0 + (size_t)r * NC;
-    const int gc = d.cidx[c];
-    const int m1 = min(NC, lo * B + B) - 1;
-    int m = lo * B;
-    float w;
-    for (;; m++) {
-        w = max_trans_of(d, gc, d.cidx[m]);
-        s = s + ((float)S0[m]) * w;
-        if (m == m1 || (double)(s / tp) >= u) break;  // the block's last prefix reaches u
-    }
-    if (scanned) *scanned += (unsigned)(m - lo * B + 1);
-    *mt = w;
-    return m;
 }
 
 // SampleWithoutReplacement replacement: all / Floyd (k <= 32) / selection

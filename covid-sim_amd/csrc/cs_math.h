@@ -179,7 +179,7 @@ struct Stream {
 // pass identifiers (high half of counter word 0)
 enum { PS_HH = 1, PS_PW_SAMPLE, PS_PW_ACCEPT, PS_PB_SAMPLE, PS_PB_ACCEPT, PS_SP_COUNT,
        PS_SP_DRAW, PS_INFECT, PS_INCUB, PS_CASE, PS_RECOVER, PS_HOLIDAY, PS_INIT, PS_SEED, PS_ITC,
-       PS_HQ, PS_ISO, PS_PLACE };
+       PS_HQ, PS_ISO, PS_PLACE, PS_POPGEN };
 // pass identifiers inside the event order key
 enum { KP_HH = 0, KP_PW = 1, KP_PB = 6, KP_SP = 11 };
 

@@ -439,6 +439,24 @@ __device__ __forceinline__ bool socdist(const Dev& d, int r, int p)
     return d.P->interventions_active && d.m_socdist[(size_t)r * d.NM + d.mcell[p]] == 2;
 }
 
+// the kernel families of KernelStruct (src/Kernels.cpp:68-106), on the host:
+// the lookup tables are lookup_[i] = k(i delta), hi_res_[i] = k(i delta / E)
+// (src/Kernels.cpp:17-45)
+inline double kernel_fn(const cs_params& P, double r2)
+{
+    const double sh = P.kernel_shape, sc = P.kernel_scale, p3 = P.kernel_p3, p4 = P.kernel_p4;
+    double t, dd;
+    switch (P.kernel_type) {
+    case 1: return exp(-sqrt(r2) / sc);
+    case 2: t = -sh * log(sqrt(r2) / sc + 1.0); return (t < -690.0) ? 0.0 : exp(t);
+    case 3: return exp(-r2 / (sc * sc));
+    case 4: return (r2 > sc * sc) ? 0.0 : 1.0;
+    case 5: t = 0.5 * sh * log(r2 / (sc * sc)); return (t > 690.0) ? 0.0 : (1.0 / (exp(t) + 1.0));
+    case 6: t = log(sqrt(r2) / sc + 1.0); return (t < -690.0) ? 0.0 : (exp(-sh * t) + p3 * exp(-p4 * t)) / (1.0 + p3);
+    default: dd = sqrt(r2); t = -sh * log(dd / sc + 1.0); return (t < -690.0) ? 0.0 : exp(t - pow(dd / p3, p4));
+    }
+}
+
 // ---- kernel lookup (src/Kernels.cpp:108-129) ----
 __device__ __forceinline__ double kernel_num(const Dev& d, double r2)
 {

@@ -275,6 +275,89 @@ inline void set_infstat(Person& a, int code)
     }
 }
 
+// cs_popgen_params from Param P as ReadParams leaves it (the generator's
+// inputs: src/SetupModel.cpp:848-2601 read these fields).  ReadParams stores
+// "Prob Youngest Child Under Five" into TwoChildrenProbYoungestUnderFive
+// and never sets ProbYoungestChildUnderFive (src/ReadParams.cpp:2289-2290);
+// the fields are copied as the run has them.
+inline void fill_popgen_params(cs_popgen_params* g)
+{
+    memset(g, 0, sizeof(*g));
+    g->pop_size = P.PopSize;
+    g->mcw = P.total_microcells_wide_;
+    g->mch = P.total_microcells_high_;
+    g->nmcl = P.NMCL;
+    g->ncw = P.ncw;
+    g->nch = P.nch;
+    g->mc_width = P.in_microcells_.width;
+    g->mc_height = P.in_microcells_.height;
+    g->bbox_x0 = P.SpatialBoundingBox.bottom_left().x;
+    g->bbox_y0 = P.SpatialBoundingBox.bottom_left().y;
+    g->longitude_cut_line = P.LongitudeCutLine;
+    g->do_ad_units = P.DoAdUnits;
+    g->do_adunit_boundaries = P.DoAdunitBoundaries;
+    g->adunit_level1_mask = P.AdunitLevel1Mask;
+    g->adunit_level1_divisor = P.AdunitLevel1Divisor;
+    g->do_households = P.DoHouseholds;
+    g->do_correct_age_dist = P.DoCorrectAgeDist;
+    for (int i = 0; i < MAX_HOUSEHOLD_SIZE; i++) g->household_size_cdf[i] = P.HouseholdSizeDistrib[0][i];
+    for (int i = 0; i < NUM_AGE_GROUPS; i++) g->prop_age_group[i] = P.PropAgeGroup[0][i];
+    g->one_pers_house_prob_old = P.OnePersHouseProbOld;
+    g->one_pers_house_prob_young = P.OnePersHouseProbYoung;
+    g->two_pers_house_prob_old = P.TwoPersHouseProbOld;
+    g->two_pers_house_prob_young = P.TwoPersHouseProbYoung;
+    g->one_child_two_pers_prob = P.OneChildTwoPersProb;
+    g->zero_child_three_pers_prob = P.ZeroChildThreePersProb;
+    g->two_child_three_pers_prob = P.TwoChildThreePersProb;
+    g->one_child_four_pers_prob = P.OneChildFourPersProb;
+    g->three_child_five_pers_prob = P.ThreeChildFivePersProb;
+    g->one_child_prob_youngest_under_five = P.OneChildProbYoungestChildUnderFive;
+    g->two_children_prob_youngest_under_five = P.TwoChildrenProbYoungestUnderFive;
+    g->prob_youngest_child_under_five = P.ProbYoungestChildUnderFive;
+    g->young_and_single_slope = P.YoungAndSingleSlope;
+    g->no_child_pers_age = P.NoChildPersAge;
+    g->old_pers_age = P.OldPersAge;
+    g->young_and_single = P.YoungAndSingle;
+    g->min_adult_age = P.MinAdultAge;
+    g->max_mf_partner_age_gap = P.MaxMFPartnerAgeGap;
+    g->max_fm_partner_age_gap = P.MaxFMPartnerAgeGap;
+    g->min_parent_age_gap = P.MinParentAgeGap;
+    g->max_parent_age_gap = P.MaxParentAgeGap;
+    g->max_child_age = P.MaxChildAge;
+    g->mean_child_age_gap = P.MeanChildAgeGap;
+    g->older_gen_gap = P.OlderGenGap;
+    g->num_place_types = P.NumPlaceTypes;
+    g->hotel_place_type = P.HotelPlaceType;
+    g->nsp = P.nsp;
+    for (int t = 0; t < MAX_NUM_PLACE_TYPES; t++) {
+        g->place_age_min[0][t] = P.PlaceTypeAgeMin[t];
+        g->place_age_max[0][t] = P.PlaceTypeAgeMax[t];
+        g->place_prop[0][t] = P.PlaceTypePropAgeGroup[t];
+        g->place_age_min[1][t] = P.PlaceTypeAgeMin2[t];
+        g->place_age_max[1][t] = P.PlaceTypeAgeMax2[t];
+        g->place_prop[1][t] = P.PlaceTypePropAgeGroup2[t];
+        g->place_age_min[2][t] = P.PlaceTypeAgeMin3[t];
+        g->place_age_max[2][t] = P.PlaceTypeAgeMax3[t];
+        g->place_prop[2][t] = P.PlaceTypePropAgeGroup3[t];
+        g->place_mean_size[t] = P.PlaceTypeMeanSize[t];
+        g->place_size_power[t] = P.PlaceTypeSizePower[t];
+        g->place_size_offset[t] = P.PlaceTypeSizeOffset[t];
+        g->place_size_sd[t] = P.PlaceTypeSizeSD[t];
+        g->place_size_min[t] = P.PlaceTypeSizeMin[t];
+        g->place_size_max[t] = P.PlaceTypeSizeMax[t];
+        g->place_nearest[t] = P.PlaceTypeNearestNeighb[t];
+        g->place_kernel_type[t] = P.PlaceTypeKernelType[t];
+        g->place_kernel_scale[t] = P.PlaceTypeKernelScale[t];
+        g->place_kernel_shape[t] = P.PlaceTypeKernelShape[t];
+        g->place_kernel_p3[t] = P.PlaceTypeKernelP3[t];
+        g->place_kernel_p4[t] = P.PlaceTypeKernelP4[t];
+        g->place_group_size[t] = P.PlaceTypeGroupSizeParam1[t];
+        g->inhibit_inter_adunit[t] = (P.DoAdUnits && P.InhibitInterAdunitPlaceAssignment[t] > 0) ? 1 : 0;
+    }
+    g->setup_seed1 = P.setupSeed1;
+    g->setup_seed2 = P.setupSeed2;
+}
+
 inline int n_mcells_total() { return P.NMCL * P.NMCL * P.NumCells; }
 
 // The static population as flat arrays (the layout cs_population describes).

@@ -560,6 +560,124 @@ int  cs_transmission_coeffs_from_draws(cs_engine* e, const double* prop_age_grou
                                        const uint16_t* infectious_steps, const int8_t* infstat,
                                        cs_transmission_coeffs* out);
 
+/* ---- population generator ----------------------------------------------
+ * The set-up the reference runs before its day loop, on the device:
+ * SetupPopulation (src/SetupModel.cpp:848-1642: microcell populations from
+ * the density, households, household locations, AssignHouseholdAges
+ * :1820-1988, the age correction, places and their locations),
+ * AssignPeopleToPlaces (:1989-2466: eligibility, place sizes, the capacity
+ * adjustment, the nearest-places choice and the cell-gravity assignment)
+ * and StratifyPlaces (:2468-2601: groups).  Its random streams are the
+ * library's counter-based ones, so a population equals the reference's in
+ * law, not draw for draw.  The concurrent pieces keep the reference's law
+ * where the reference is sequential: microcell and place counts are one
+ * multinomial, split as a tree of binomials; capacity changes draw their
+ * places independently; the cell-gravity assignment runs in rounds in which
+ * every unplaced person proposes slots as the reference's loop does
+ * (destination cell by the capacity-weighted cell tables, a uniform slot,
+ * acceptance by kernel over max_trans) and a slot goes to its first
+ * claimant, the tables being rebuilt from the slots left after each round.
+ * Unsupported (CS_EUNSUPPORTED): school files (nsp > 0), regional
+ * demography files, and InhibitInterAdunitPlaceAssignment. */
+typedef struct cs_popgen_params {
+    int32_t pop_size;                  /* P.PopSize */
+    int32_t mcw, mch;                  /* P.total_microcells_wide_/high_ */
+    int32_t nmcl;                      /* P.NMCL */
+    int32_t ncw, nch;                  /* P.ncw, P.nch */
+    double  mc_width, mc_height;       /* P.in_microcells_ (degrees) */
+    /* reading the density (src/SetupModel.cpp:73-103, :882-940): the box's
+     * bottom-left corner, the longitude cut and the administrative-unit codes */
+    double  bbox_x0, bbox_y0;          /* P.SpatialBoundingBox.bottom_left() */
+    double  longitude_cut_line;        /* P.LongitudeCutLine */
+    int32_t do_ad_units, do_adunit_boundaries;      /* P.DoAdUnits, P.DoAdunitBoundaries */
+    int32_t adunit_level1_mask, adunit_level1_divisor;  /* P.AdunitLevel1Mask, P.AdunitLevel1Divisor */
+    int32_t do_households;             /* P.DoHouseholds */
+    int32_t do_correct_age_dist;       /* P.DoCorrectAgeDist */
+    double  household_size_cdf[CS_MAX_HOUSEHOLD_SIZE];  /* P.HouseholdSizeDistrib[0], cumulative */
+    double  prop_age_group[CS_NUM_AGE_GROUPS];          /* P.PropAgeGroup[0] */
+    /* AssignHouseholdAges */
+    double  one_pers_house_prob_old, one_pers_house_prob_young;
+    double  two_pers_house_prob_old, two_pers_house_prob_young;
+    double  one_child_two_pers_prob, zero_child_three_pers_prob, two_child_three_pers_prob;
+    double  one_child_four_pers_prob, three_child_five_pers_prob;
+    double  one_child_prob_youngest_under_five, two_children_prob_youngest_under_five;
+    double  prob_youngest_child_under_five, young_and_single_slope;
+    int32_t no_child_pers_age, old_pers_age, young_and_single, min_adult_age;
+    int32_t max_mf_partner_age_gap, max_fm_partner_age_gap, min_parent_age_gap, max_parent_age_gap;
+    int32_t max_child_age, mean_child_age_gap, older_gen_gap;
+    /* places */
+    int32_t num_place_types;           /* P.NumPlaceTypes */
+    int32_t hotel_place_type;          /* P.HotelPlaceType (>= num_place_types: none) */
+    int32_t nsp;                       /* P.nsp, places read from a school file (must be 0) */
+    int32_t place_age_min[3][CS_MAX_PLACE_TYPES];   /* PlaceTypeAgeMin, ...Min2, ...Min3 */
+    int32_t place_age_max[3][CS_MAX_PLACE_TYPES];
+    double  place_prop[3][CS_MAX_PLACE_TYPES];      /* PlaceTypePropAgeGroup, 2, 3 */
+    double  place_mean_size[CS_MAX_PLACE_TYPES];
+    double  place_size_power[CS_MAX_PLACE_TYPES], place_size_offset[CS_MAX_PLACE_TYPES];
+    double  place_size_sd[CS_MAX_PLACE_TYPES];
+    double  place_size_min[CS_MAX_PLACE_TYPES], place_size_max[CS_MAX_PLACE_TYPES];
+    int32_t place_nearest[CS_MAX_PLACE_TYPES];      /* PlaceTypeNearestNeighb */
+    int32_t place_kernel_type[CS_MAX_PLACE_TYPES];
+    double  place_kernel_scale[CS_MAX_PLACE_TYPES], place_kernel_shape[CS_MAX_PLACE_TYPES];
+    double  place_kernel_p3[CS_MAX_PLACE_TYPES], place_kernel_p4[CS_MAX_PLACE_TYPES];
+    double  place_group_size[CS_MAX_PLACE_TYPES];   /* PlaceTypeGroupSizeParam1 */
+    int32_t inhibit_inter_adunit[CS_MAX_PLACE_TYPES];  /* InhibitInterAdunitPlaceAssignment != 0 (must be 0) */
+    int32_t setup_seed1, setup_seed2;  /* P.setupSeed1/2 (network file header only) */
+} cs_popgen_params;
+
+/* sizes of a generated population (the arrays of cs_population) */
+typedef struct cs_popgen_sizes {
+    int32_t n_people, n_households, n_cells, n_cells_total, n_mcells;
+    int32_t n_places[CS_MAX_PLACE_TYPES];
+    int64_t n_members[CS_MAX_PLACE_TYPES];   /* persons linked to each type */
+    int64_t n_groups[CS_MAX_PLACE_TYPES];
+    int64_t n_unplaced[CS_MAX_PLACE_TYPES];  /* eligible persons the assignment left without a place */
+} cs_popgen_sizes;
+
+/* host buffers of a generated population, sized by cs_popgen_sizes: the
+ * cs_population arrays (hh, mcell, pcell, age, place_links [n][CS_MAX_PLACE_TYPES],
+ * place_group_links, households, populated cells and their members,
+ * mcell_adunit, places by type) plus the place locations ([n][2] floats,
+ * relative to the bounding box as household locations are).  NULL
+ * fields are not copied. */
+typedef struct cs_popgen_out {
+    int32_t *hh, *mcell, *pcell;
+    uint8_t *age;
+    int32_t *place_links;
+    uint16_t *place_group_links;
+    int32_t *hh_first;
+    uint16_t *hh_nh;
+    float   *hh_loc;
+    int32_t *cell_index, *cell_n, *cell_member_start, *cell_members;
+    int32_t *mcell_adunit;
+    int32_t *place_n[CS_MAX_PLACE_TYPES], *place_mcell[CS_MAX_PLACE_TYPES];
+    int32_t *place_member_start[CS_MAX_PLACE_TYPES], *place_members[CS_MAX_PLACE_TYPES];
+    int32_t *place_groups_start[CS_MAX_PLACE_TYPES], *place_group_start[CS_MAX_PLACE_TYPES];
+    int32_t *place_group_size[CS_MAX_PLACE_TYPES];
+    float   *place_loc[CS_MAX_PLACE_TYPES];
+} cs_popgen_out;
+
+typedef struct cs_popgen cs_popgen;
+int  cs_popgen_create(cs_popgen** out, int device);
+void cs_popgen_destroy(cs_popgen* g);
+const char* cs_popgen_last_error(const cs_popgen* g);
+/* one population: kp supplies the distance geometry and the kernel table
+ * size (cs_params: do_utm_coords, periodic boundaries, region size,
+ * bounding-box latitude, kernel_size, kernel_delta, kernel_expansion; the
+ * place kernels come from pg); per microcell (mcw x mch, index x * mch + y)
+ * the mean density of its density points (0: none), its country and its
+ * administrative unit index (SetupPopulation's mcell_dens / mcell_num,
+ * mcell_country and mcell_adunits). */
+int  cs_popgen_run(cs_popgen* g, const cs_popgen_params* pg, const cs_params* kp, const double* mcell_density,
+                   const int32_t* mcell_country, const int32_t* mcell_adunit, uint64_t seed);
+int  cs_popgen_get_sizes(cs_popgen* g, cs_popgen_sizes* out);
+int  cs_popgen_fetch(cs_popgen* g, const cs_popgen_out* out);
+/* AssignHouseholdAges for given household sizes (household k takes the
+ * stream of household k of a population generated with `seed`); ages in
+ * household order, sum(sizes) entries.  For parity tests. */
+int  cs_popgen_household_ages(cs_popgen* g, const cs_popgen_params* pg, uint64_t seed, const int32_t* sizes,
+                              int n, uint8_t* ages);
+
 /* ---- calibration emulator ---------------------------------------------
  * No counterpart in the reference (its calibration re-runs the simulator,
  * src/CovidSim.cpp:3485-3689).  A multilayer perceptron

@@ -197,7 +197,7 @@ static double fx_to_double(fx128 a) { return (double)a.hi + (double)a.lo * 5.421
 /* pass identifiers (high half of counter word 0) */
 enum { PS_HH = 1, PS_PW_SAMPLE, PS_PW_ACCEPT, PS_PB_SAMPLE, PS_PB_ACCEPT, PS_SP_COUNT,
        PS_SP_DRAW, PS_INFECT, PS_INCUB, PS_CASE, PS_RECOVER, PS_HOLIDAY, PS_INIT, PS_SEED, PS_ITC,
-       PS_HQ, PS_ISO, PS_PLACE };
+       PS_HQ, PS_ISO, PS_PLACE, PS_POPGEN };
 /* pass identifiers inside the event order key */
 enum { KP_HH = 0, KP_PW = 1, KP_PB = 6, KP_SP = 11 };
 
@@ -2532,4 +2532,251 @@ int or_sample(int which, uint64_t key, uint32_t c0, uint32_t c1, uint32_t c2, do
         }
     }
     return n;
+}
+
+/* ---- population set-up: AssignHouseholdAges -------------------------------
+ * A restatement of src/SetupModel.cpp:1820-1988 on the library's streams:
+ * household k of a population generated with `seed` draws from stream
+ * (seed; pass PG_HHAGE of PS_POPGEN, k, 0) after its two location uniforms.
+ * Where the reference draws two uniforms in one expression
+ * (:1950, InvAgeDist[...] - a[(int)(ranf * nc)]) the left one comes first.
+ * Every rejection loop is bounded (the reference's are not); a household
+ * whose loop runs out is an error. */
+#define OR_PG_HHAGE 3
+#define OR_PG_AGES 85
+#define OR_PG_MAX_TRIES (1 << 20)
+
+static void or_inv_age(const cs_popgen_params* p, int32_t inv[1000])
+{
+    /* State.InvAgeDist[0] (src/SetupModel.cpp:1101-1111) */
+    double cum[CS_NUM_AGE_GROUPS + 1];
+    cum[0] = 0;
+    for (int i = 1; i <= CS_NUM_AGE_GROUPS; i++) cum[i] = cum[i - 1] + p->prop_age_group[i - 1];
+    for (int i = 0, j = 0; i < 1000; i++) {
+        double t = ((double)i) / 1000;
+        if (t >= cum[j + 1]) j++;
+        t = 5 * (((double)j) + (t - cum[j]) / (cum[j + 1] - cum[j]));
+        inv[i] = (int)t;
+    }
+    inv[999] = CS_NUM_AGE_GROUPS * 5 - 1;
+}
+
+static int pg_age(const int32_t* inv, rs* s) { return inv[(int)(1000.0 * rs_next(s))]; }
+
+static int pg_household_ages(const cs_popgen_params* P, const int32_t* inv, rs* s, int n, int* a)
+{
+    int tries = 0;
+#define OR_TRY if (++tries > OR_PG_MAX_TRIES) return -1
+    if (!P->do_households) {
+        for (int i = 0; i < n; i++) a[i] = pg_age(inv, s);
+        return 0;
+    }
+    const double span_old = (double)((long long)P->old_pers_age - P->no_child_pers_age + 1);
+    const double span_young = (double)((long long)P->young_and_single - P->min_adult_age);
+    if (n == 1) {
+        if (rs_next(s) < P->one_pers_house_prob_old) {
+            do { OR_TRY; a[0] = pg_age(inv, s); }
+            while ((a[0] < P->no_child_pers_age) || (rs_next(s) > (((double)a[0]) - P->no_child_pers_age + 1) / span_old));
+        } else if ((P->one_pers_house_prob_young > 0) && (rs_next(s) < P->one_pers_house_prob_young / (1 - P->one_pers_house_prob_old))) {
+            do { OR_TRY; a[0] = pg_age(inv, s); }
+            while ((a[0] > P->young_and_single) || (a[0] < P->min_adult_age) ||
+                   (rs_next(s) > 1 - P->young_and_single_slope * (((double)a[0]) - P->min_adult_age) / span_young));
+        } else {
+            do { OR_TRY; } while ((a[0] = pg_age(inv, s)) < P->min_adult_age);
+        }
+    } else if (n == 2) {
+        if (rs_next(s) < P->two_pers_house_prob_old) {
+            do { OR_TRY; a[0] = pg_age(inv, s); }
+            while ((a[0] < P->no_child_pers_age) || (rs_next(s) > (((double)a[0]) - P->no_child_pers_age + 1) / span_old));
+            do { OR_TRY; a[1] = pg_age(inv, s); }
+            while ((a[1] > a[0] + P->max_mf_partner_age_gap) || (a[1] < a[0] - P->max_fm_partner_age_gap) ||
+                   (a[1] < P->no_child_pers_age) || (rs_next(s) > (((double)a[1]) - P->no_child_pers_age + 1) / span_old));
+        } else if (rs_next(s) < P->one_child_two_pers_prob / (1 - P->two_pers_house_prob_old)) {
+            do { OR_TRY; } while ((a[0] = pg_age(inv, s)) > P->max_child_age);
+            do { OR_TRY; a[1] = pg_age(inv, s); }
+            while ((a[1] > a[0] + P->max_parent_age_gap) || (a[1] < a[0] + P->min_parent_age_gap) || (a[1] < P->min_adult_age));
+        } else if ((P->two_pers_house_prob_young > 0) &&
+                   (rs_next(s) < P->two_pers_house_prob_young / (1 - P->two_pers_house_prob_old - P->one_child_two_pers_prob))) {
+            do { OR_TRY; a[0] = pg_age(inv, s); }
+            while ((a[0] < P->min_adult_age) || (a[0] > P->young_and_single) ||
+                   (rs_next(s) > 1 - P->young_and_single_slope * (((double)a[0]) - P->min_adult_age) / span_young));
+            do { OR_TRY; a[1] = pg_age(inv, s); }
+            while ((a[1] > a[0] + P->max_mf_partner_age_gap) || (a[1] < a[0] - P->max_fm_partner_age_gap) || (a[1] < P->min_adult_age));
+        } else {
+            do { OR_TRY; a[0] = pg_age(inv, s); } while (a[0] < P->min_adult_age);
+            do { OR_TRY; a[1] = pg_age(inv, s); }
+            while ((a[1] > a[0] + P->max_mf_partner_age_gap) || (a[1] < a[0] - P->max_fm_partner_age_gap) || (a[1] < P->min_adult_age));
+        }
+    } else {
+        int nc;
+        if (n == 3) {
+            if ((P->zero_child_three_pers_prob > 0) || (P->two_child_three_pers_prob > 0))
+                nc = (rs_next(s) < P->zero_child_three_pers_prob) ? 0 : ((rs_next(s) < P->two_child_three_pers_prob) ? 2 : 1);
+            else
+                nc = 1;
+        } else if (n == 4) {
+            nc = (rs_next(s) < P->one_child_four_pers_prob) ? 1 : 2;
+        } else if (n == 5) {
+            nc = (rs_next(s) < P->three_child_five_pers_prob) ? 3 : 2;
+        } else {
+            nc = n - 2 - (int)(3 * rs_next(s));
+        }
+        if (nc <= 0) {
+            do { OR_TRY; a[0] = pg_age(inv, s); a[1] = pg_age(inv, s); }
+            while ((a[1] < P->min_adult_age) || (a[0] < P->min_adult_age));
+            do { OR_TRY; a[2] = pg_age(inv, s); }
+            while ((a[2] >= a[1] + P->max_mf_partner_age_gap) || (a[2] < a[1] - P->max_fm_partner_age_gap));
+        } else {
+            int k, j;
+            do {
+                OR_TRY;
+                a[0] = 0;
+                for (int i = 1; i < nc; i++)
+                    a[i] = a[i - 1] + 1 + (int)smp_poisson(s, (double)((long long)P->mean_child_age_gap - 1));
+                int base = pg_age(inv, s);
+                a[0] = base - a[(int)(rs_next(s) * ((double)nc))];
+                for (int i = 1; i < nc; i++) a[i] += a[0];
+                k = (((nc == 1) && (rs_next(s) < P->one_child_prob_youngest_under_five)) ||
+                     ((nc == 2) && (rs_next(s) < P->two_children_prob_youngest_under_five)) ||
+                     ((nc > 2) && (rs_next(s) < P->prob_youngest_child_under_five))) ? 5 : P->max_child_age;
+            } while ((a[0] < 0) || (a[0] > k) || (a[nc - 1] > P->max_child_age));
+            j = a[nc - 1] - a[0] - (P->max_parent_age_gap - P->min_parent_age_gap);
+            if (j > 0) j += P->max_parent_age_gap;
+            else j = P->max_parent_age_gap;
+            do { OR_TRY; a[nc] = pg_age(inv, s); k = a[nc - 1]; }
+            while ((a[nc] > a[0] + j) || (a[nc] < k + P->min_parent_age_gap) || (a[nc] < P->min_adult_age));
+            if ((n > nc + 1) && (rs_next(s) > 0.0)) {  /* PROP_OTHER_PARENT_AWAY = 0 (src/SetupModel.cpp:1818) */
+                do { OR_TRY; a[nc + 1] = pg_age(inv, s); }
+                while ((a[nc + 1] > a[nc] + P->max_mf_partner_age_gap) || (a[nc + 1] < a[nc] - P->max_fm_partner_age_gap) ||
+                       (a[nc + 1] > a[0] + j) || (a[nc + 1] < k + P->min_parent_age_gap) || (a[nc + 1] < P->min_adult_age));
+            }
+            if (n > nc + 2) {
+                j = ((a[nc + 1] > a[nc]) ? a[nc + 1] : a[nc]) + P->older_gen_gap;
+                if (j >= OR_PG_AGES) j = OR_PG_AGES - 1;
+                if (j < P->no_child_pers_age) j = P->no_child_pers_age;
+                for (int i = nc + 2; i < n; i++) {
+                    do { OR_TRY; } while ((a[i] = pg_age(inv, s)) < j);
+                }
+            }
+        }
+    }
+#undef OR_TRY
+    return 0;
+}
+
+int or_household_ages(const cs_popgen_params* p, uint64_t seed, const int32_t* sizes, int n, uint8_t* ages)
+{
+    int32_t inv[1000];
+    or_inv_age(p, inv);
+    size_t off = 0;
+    for (int h = 0; h < n; h++) {
+        if (sizes[h] < 1 || sizes[h] > CS_MAX_HOUSEHOLD_SIZE) return CS_EINVAL;
+        rs s;
+        rs_init(&s, seed, (uint32_t)OR_PG_HHAGE | ((uint32_t)PS_POPGEN << 16), (uint32_t)h, 0);
+        (void)rs_next(&s);
+        (void)rs_next(&s);
+        int a[CS_MAX_HOUSEHOLD_SIZE + 2];
+        if (pg_household_ages(p, inv, &s, sizes[h], a)) return CS_ERANGE;
+        for (int i = 0; i < sizes[h]; i++) ages[off + i] = (uint8_t)a[i];
+        off += (size_t)sizes[h];
+    }
+    return 0;
+}
+
+/* ---- population set-up: the nearest-places choice -------------------------
+ * A restatement of AssignPeopleToPlaces' choice among the nearest places
+ * (src/SetupModel.cpp:2250-2350) for given persons: a square spiral over
+ * microcells from the person's own (right, up (y - 1), left, down, the side
+ * growing every second turn), while fewer than nn places are held or the
+ * side is below 4, and the side below the grid's width; places of other
+ * countries are skipped; the nn largest kernel weights are kept (a new
+ * weight replaces the first smallest held one when larger); then one place
+ * with probability proportional to its weight, from the first uniform of
+ * stream (seed; pass PG_NEAR of PS_POPGEN for type tp, person, 0).  The
+ * place type's kernel tables are built as KernelLookup::init builds them.
+ * out[i] = -1 where no place lies within reach. */
+#define OR_PG_NEAR 12
+int or_nearest_places(const cs_params* kp, const cs_popgen_params* pg, int tp, uint64_t seed, const int32_t* persons,
+                      const float* hloc, const int32_t* pmcell, int n, const int64_t* pl_off, const float* pl_loc,
+                      const int32_t* mc_country, int32_t* out)
+{
+    const int nn = pg->place_nearest[tp];
+    if (nn <= 0 || nn > 64) return CS_EINVAL;
+    or_engine* e = (or_engine*)calloc(1, sizeof(or_engine));
+    e->P = *kp;
+    e->P.kernel_type = pg->place_kernel_type[tp];
+    e->P.kernel_scale = pg->place_kernel_scale[tp];
+    e->P.kernel_shape = pg->place_kernel_shape[tp];
+    e->P.kernel_p3 = pg->place_kernel_p3[tp];
+    e->P.kernel_p4 = pg->place_kernel_p4[tp];
+    const cs_params* P = &e->P;
+    const size_t ksz = (size_t)P->kernel_size + 1;
+    e->kl = (double*)malloc(ksz * 8);
+    e->kh = (double*)malloc(ksz * 8);
+    for (int i = 0; i <= P->kernel_size; i++) {
+        e->kl[i] = kernel_fn(P, i * P->kernel_delta) / 1.0;
+        e->kh[i] = kernel_fn(P, i * P->kernel_delta / P->kernel_expansion) / 1.0;
+    }
+    const int mcw = pg->mcw, mch = pg->mch;
+    for (int q = 0; q < n; q++) {
+        const int ic = pmcell[q];
+        int px = ic / mch, py = ic % mch;
+        const int country = mc_country[ic];
+        const double hx = hloc[2 * q], hy = hloc[2 * q + 1];
+        int id[64];
+        double w[64];
+        int dir = 0, side = 1, k = 0, turn = 0, step = 0;
+        while (((k < nn) || (side < 4)) && (side < mcw)) {
+            if (px >= 0 && py >= 0 && px < mcw && py < mch) {
+                const int c = px * mch + py;
+                if (mc_country[c] == country)
+                    for (int64_t pl = pl_off[c]; pl < pl_off[c + 1]; pl++) {
+                        const double lx = pl_loc[2 * pl], ly = pl_loc[2 * pl + 1];
+                        double d2;
+                        if (P->do_utm_coords) d2 = dist2_utm(P, hx, hy, lx, ly);
+                        else d2 = periodic_xy(P, fabs(hx - lx), fabs(hy - ly));
+                        const double s = k_num(e, d2);
+                        if (s > 0) {
+                            if (k < nn) {
+                                id[k] = (int)pl;
+                                w[k] = s;
+                                k++;
+                            } else {
+                                double t = 1e10;
+                                int j2 = 0;
+                                for (int i2 = 0; i2 < nn; i2++)
+                                    if (w[i2] < t) { t = w[i2]; j2 = i2; }
+                                if (s > t) { w[j2] = s; id[j2] = (int)pl; }
+                            }
+                        }
+                    }
+            }
+            switch (dir) {
+            case 0: px++; break;
+            case 1: py--; break;
+            case 2: px--; break;
+            default: py++; break;
+            }
+            step = (step + 1) % side;
+            if (step == 0) {
+                dir = (dir + 1) & 3;
+                turn = (turn + 1) % 2;
+                if (turn == 0) side++;
+            }
+        }
+        out[q] = -1;
+        if (k == 0) continue;
+        for (int i2 = 1; i2 < k; i2++) w[i2] += w[i2 - 1];
+        rs s;
+        rs_init(&s, seed, (uint32_t)(OR_PG_NEAR | (tp << 8)) | ((uint32_t)PS_POPGEN << 16), (uint32_t)persons[q], 0);
+        const double u = rs_next(&s);
+        out[q] = id[k - 1];
+        for (int i2 = 0; i2 < k; i2++)
+            if (u < w[i2] / w[k - 1]) { out[q] = id[i2]; break; }
+    }
+    const int err = e->error;
+    free(e->kl);
+    free(e->kh);
+    free(e);
+    return err;
 }

@@ -63,6 +63,15 @@ void or_philox(const uint32_t* ctr_key, uint32_t* out, int n);
 /* samplers exposed for distribution tests: draws from stream (key, c0..c2) */
 int  or_sample(int which, uint64_t key, uint32_t c0, uint32_t c1, uint32_t c2,
                double a, double b, int n, double* out);
+/* AssignHouseholdAges (src/SetupModel.cpp:1820-1988) on the generator's
+ * streams, for given household sizes */
+int  or_household_ages(const cs_popgen_params* p, uint64_t seed, const int32_t* sizes, int n, uint8_t* ages);
+/* the nearest-places choice of AssignPeopleToPlaces (src/SetupModel.cpp:2250-2350)
+ * for given persons (household location [n][2], microcell), places of type tp
+ * by microcell (pl_off over pg->mcw * pg->mch microcells, locations [][2]) */
+int  or_nearest_places(const cs_params* kp, const cs_popgen_params* pg, int tp, uint64_t seed, const int32_t* persons,
+                       const float* hloc, const int32_t* pmcell, int n, const int64_t* pl_off, const float* pl_loc,
+                       const int32_t* mc_country, int32_t* out);
 double or_log(double x);
 double or_exp(double x);
 

@@ -157,6 +157,24 @@ void dump_population()
     const int NC = P.NumPopulatedCells, nm = cs_bridge::n_mcells_total();
     std::vector<int32_t> meta = {P.PopSize, P.NumHouseholds, NC, P.NumCells, nm, P.NumAdunits, P.NumPlaceTypes, P.NMCL};
     save("meta", meta);
+    // the set-up's inputs and outputs the population generator is compared on:
+    // its parameters, place locations, microcell populations and countries
+    {
+        cs_popgen_params g;
+        cs_bridge::fill_popgen_params(&g);
+        std::string path = g_dir + "/popgen_params.bin";
+        FILE* f = fopen(path.c_str(), "wb");
+        fwrite(&g, sizeof(g), 1, f);
+        fclose(f);
+        for (int t = 0; t < P.NumPlaceTypes; t++) {
+            std::vector<float> loc((size_t)P.Nplace[t] * 2);
+            for (int i = 0; i < P.Nplace[t]; i++) { loc[2 * i] = Places[t][i].loc.x; loc[2 * i + 1] = Places[t][i].loc.y; }
+            save2("pl" + std::to_string(t) + "_loc", loc, P.Nplace[t], 2);
+        }
+        std::vector<int32_t> mn(nm), mcn(nm);
+        for (int i = 0; i < nm; i++) { mn[i] = Mcells[i].n; mcn[i] = mcell_country[i]; }
+        save("m_n", mn); save("m_country", mcn);
+    }
 
     // kernel tables
     std::vector<double> lk(P.KernelLookup.lookup_), hr(P.KernelLookup.hi_res_);

@@ -154,7 +154,7 @@ def int_param_files(out_dir, region="mini"):
 
 
 def run_harness(inputs, out, steps="0", golden=True, threads=1, extra=(), series=None, nr=None, extra_env=None,
-                pre_params=None, params=None):
+                pre_params=None, params=None, admin=None):
     env = dict(os.environ, CS_DUMP_DIR=out, CS_DUMP_STEPS=steps, CS_DUMP_GOLDEN="1" if golden else "0")
     env.update(extra_env or {})
     if series:
@@ -164,7 +164,7 @@ def run_harness(inputs, out, steps="0", golden=True, threads=1, extra=(), series
            "/P:" + (params or os.path.join(inputs, "input-noint-params.txt")),
            "/O:" + os.path.join(out, "out"),
            "/D:" + os.path.join(inputs, "density.txt"),
-           "/A:" + os.path.join(inputs, "admin-params.txt")]
+           "/A:" + (admin or os.path.join(inputs, "admin-params.txt"))]
     if nr:
         cmd.append("/NR:%d" % nr)
     cmd += list(extra) + SEEDS
@@ -288,8 +288,59 @@ def c3_params():
         shutil.rmtree(tmp)
 
 
+def setup():
+    """What the population generator is compared with, from runs of the
+    reference's own set-up: covid-sim_amd/inputs/{c1,mini}/popgen_params.bin
+    (cs_popgen_params as ReadParams leaves Param), c1_setup.npz (C1's place
+    locations, microcell populations and countries), and setup_files.json:
+    the SHA-256 and header fields of the network file (/S:) and binary
+    density file (/M:) the reference writes for the mini inputs, which the
+    generator's writers must reproduce byte for byte"""
+    import hashlib
+    import json
+    out = {}
+    for name in ("c1", "mini"):
+        inputs = os.path.join(ROOT, "covid-sim_amd", "inputs", name)
+        tmp = tempfile.mkdtemp()
+        try:
+            extra = ()
+            if name == "mini":
+                extra = ("/S:" + os.path.join(tmp, "net.bin"), "/M:" + os.path.join(tmp, "dens.bin"))
+            run_harness(inputs, tmp, steps="", golden=False, threads=1, extra=extra)
+            shutil.copy(os.path.join(tmp, "popgen_params.bin"), os.path.join(inputs, "popgen_params.bin"))
+            if name == "c1":
+                keep = [fn for fn in sorted(os.listdir(tmp))
+                        if fn.endswith(".npy") and (fn.startswith(("m_n", "m_country")) or (fn.startswith("pl") and fn.endswith("_loc.npy")))]
+                arrays = {fn[:-4]: np.load(os.path.join(tmp, fn)) for fn in keep}
+                # the same households' ages before the age correction: a run
+                # with the correction off draws the same set-up up to it
+                tmp2 = tempfile.mkdtemp()
+                try:
+                    adm = override_params(os.path.join(inputs, "admin-params.txt"),
+                                          {"Correct age distribution after household allocation to exactly match "
+                                           "specified demography": "0"}, os.path.join(tmp2, "admin.txt"))
+                    run_harness(inputs, tmp2, steps="", golden=False, threads=1, admin=adm)
+                    assert np.array_equal(np.load(os.path.join(tmp2, "h_nh.npy")), np.load(os.path.join(tmp, "h_nh.npy")))
+                    arrays["p_age_uncorrected"] = np.load(os.path.join(tmp2, "p_age.npy"))
+                finally:
+                    shutil.rmtree(tmp2)
+                np.savez_compressed(os.path.join(HERE, "c1_setup.npz"), **arrays)
+            else:
+                for key, fn in (("network", "net.bin"), ("density", "dens.bin")):
+                    raw = open(os.path.join(tmp, fn), "rb").read()
+                    out[key] = {"sha256": hashlib.sha256(raw).hexdigest(), "bytes": len(raw),
+                                "header": list(np.frombuffer(raw[:20], np.int32).tolist())}
+        finally:
+            shutil.rmtree(tmp)
+    with open(os.path.join(HERE, "setup_files.json"), "w") as f:
+        json.dump(out, f, indent=1)
+    print("wrote popgen_params.bin (c1, mini), c1_setup.npz, setup_files.json")
+
+
 if __name__ == "__main__":
-    if len(sys.argv) > 1 and sys.argv[1] == "interventions":
+    if len(sys.argv) > 1 and sys.argv[1] == "setup":
+        setup()
+    elif len(sys.argv) > 1 and sys.argv[1] == "interventions":
         interventions()
     elif len(sys.argv) > 1 and sys.argv[1] == "c3":
         c3_params()

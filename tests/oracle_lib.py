@@ -33,6 +33,12 @@ def load():
     lib.or_init_transmission_coeffs.argtypes = [C.c_void_p, P(C.c_double), C.c_uint64, P(abi.cs_transmission_coeffs),
                                                 P(C.c_float), P(C.c_float), P(C.c_uint16)]
     lib.or_init_transmission_coeffs.restype = C.c_int
+    lib.or_household_ages.argtypes = [P(abi.cs_popgen_params), C.c_uint64, P(C.c_int32), C.c_int, P(C.c_uint8)]
+    lib.or_household_ages.restype = C.c_int
+    lib.or_nearest_places.argtypes = [P(abi.cs_params), P(abi.cs_popgen_params), C.c_int, C.c_uint64, P(C.c_int32),
+                                      P(C.c_float), P(C.c_int32), C.c_int, P(C.c_int64), P(C.c_float), P(C.c_int32),
+                                      P(C.c_int32)]
+    lib.or_nearest_places.restype = C.c_int
     lib.or_log.argtypes = [C.c_double]
     lib.or_log.restype = C.c_double
     lib.or_exp.argtypes = [C.c_double]
@@ -225,3 +231,34 @@ class OracleEngine(Oracle):
             self.lib.or_sample(int(which), int(key), int(c0), int(c1), i, float(a), float(b), 1, _p(one, C.c_double))
             out[i] = one[0]
         return out
+
+
+def household_ages(pg, seed, sizes):
+    """AssignHouseholdAges of the oracle for the given household sizes."""
+    lib, _ = load()
+    sizes = np.ascontiguousarray(sizes, np.int32)
+    out = np.zeros(int(sizes.sum()), np.uint8)
+    rc = lib.or_household_ages(C.byref(pg), C.c_uint64(seed), iptr(sizes), len(sizes),
+                               out.ctypes.data_as(C.POINTER(C.c_uint8)))
+    if rc:
+        raise RuntimeError("or_household_ages: %d" % rc)
+    return out
+
+
+def nearest_places(kp, pg, tp, seed, persons, hloc, pmcell, pl_off, pl_loc, mc_country):
+    """The oracle's nearest-places choice for the given persons."""
+    lib, _ = load()
+    persons = np.ascontiguousarray(persons, np.int32)
+    hloc = np.ascontiguousarray(hloc, np.float32)
+    pmcell = np.ascontiguousarray(pmcell, np.int32)
+    pl_off = np.ascontiguousarray(pl_off, np.int64)
+    pl_loc = np.ascontiguousarray(pl_loc, np.float32)
+    mc_country = np.ascontiguousarray(mc_country, np.int32)
+    out = np.zeros(len(persons), np.int32)
+    fp = lambda a: a.ctypes.data_as(C.POINTER(C.c_float))
+    rc = lib.or_nearest_places(C.byref(kp), C.byref(pg), int(tp), C.c_uint64(seed), iptr(persons), fp(hloc), iptr(pmcell),
+                               len(persons), pl_off.ctypes.data_as(C.POINTER(C.c_int64)), fp(pl_loc), iptr(mc_country),
+                               iptr(out))
+    if rc:
+        raise RuntimeError("or_nearest_places: %d" % rc)
+    return out

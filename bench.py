@@ -203,6 +203,8 @@ def main():
     ap.add_argument("--no-graphs", action="store_true", help="launch every kernel directly in the timed steps too "
                                                             "(for runs under rocprofv3 tracing)")
     ap.add_argument("--calibration", action="store_true", help="measure emulator-guided vs plain calibration instead")
+    ap.add_argument("--profile-kinds", action="store_true", help="with --config large: time every launch by kind "
+                    "(HIP events, direct launches)")
     ap.add_argument("--cells", type=int, nargs=2, default=[668, 336], metavar=("WIDE", "HIGH"),
                     help="with --config large: the region's cells (0.075 degrees each)")
     ap.add_argument("--config", choices=["headline", "c3", "large"], default="headline",
@@ -548,6 +550,10 @@ def large_region(args):
         "destination tables: %s" % (t_setup, c.r0, c.r0_household, c.r0_places, c.local_beta,
                                     "full" if stride == 0 else "checkpoints every %d cells" % stride))
     tspd = int(round(kp.time_steps_per_day))
+    if args.no_graphs:
+        eng.set_step_graphs(False)
+    if args.profile_kinds:
+        eng.set_profiling(True)  # HIP events around every launch (direct launches)
     eng.reset(10)
     eng.sync()
     ta = time.perf_counter()
@@ -561,6 +567,10 @@ def large_region(args):
     dt = time.perf_counter() - ta
     ts = eng.timeseries(args.days)
     attack = float((ts[0, -1, 3] + ts[0, -1, 4]) / args.people)
+    kinds = None
+    if args.profile_kinds:
+        kinds = {k: [round(ms, 1), n] for k, (ms, n) in eng.profile().items() if n}
+        log("kernel ms by kind: " + ", ".join("%s %.0f/%d" % (k, v[0], v[1]) for k, v in kinds.items()))
     out = {
         "metric": "person-days/s, one large region",
         "value": args.people * args.days / dt,
@@ -581,7 +591,10 @@ def large_region(args):
                    "generation_seconds": round(t_gen, 3), "generation_persons_per_s": args.people / t_gen,
                    "engine_setup_seconds": round(t_setup, 1),
                    "dest_tables": "full" if stride == 0 else "checkpoints every %d cells" % stride,
-                   "r0": round(c.r0, 3), "attack_rate": round(attack, 4)},
+                   "r0": round(c.r0, 3), "attack_rate": round(attack, 4),
+                   "launches": "direct launches, every launch timed" if args.profile_kinds else
+                               ("direct launches" if args.no_graphs else "replayed step graphs"),
+                   "kernel_ms_by_kind": kinds},
         "roofline": None,
         "cpu_baseline": None,
     }

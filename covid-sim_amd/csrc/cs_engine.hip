@@ -758,7 +758,9 @@ int cs_set_population(cs_engine* e, const cs_population* pop)
         const int32_t* gs0 = pop->place_groups_start[t];
         if (np < 0 || (np && (!ms || !gs0 || ms[0] != 0 || gs0[0] != 0))) return set_err(e, CS_EINVAL, "place arrays of type %d", t);
         for (int k = 0; k < np; k++) {
-            if (ms[k + 1] - ms[k] != pop->place_n[t][k] || pop->place_n[t][k] <= 0)
+            // a place may have no members (StratifyPlaces leaves it without
+            // groups, src/SetupModel.cpp:2518); no link points to it
+            if (ms[k + 1] - ms[k] != pop->place_n[t][k] || pop->place_n[t][k] < 0)
                 return set_err(e, CS_EINVAL, "place %d of type %d: member range and n disagree", k, t);
             for (int g = gs0[k]; g < gs0[k + 1]; g++)
                 if (pop->place_group_start[t][g] < 0 || pop->place_group_size[t][g] < 0 ||

@@ -129,6 +129,9 @@ def declare(lib, prefix):
         "get_counters": (I, [vp, I, P(cs_counters), I]),
         "get_intervention_state": (I, [vp, I, P(cs_intervention_state)]),
         "get_timeseries": (I, [vp, P(C.c_int32), I, P(I)]),
+        "get_timeseries_age": (I, [vp, P(C.c_int32), I, P(I)]),
+        "get_timeseries_adunit": (I, [vp, P(C.c_int32), I, P(I)]),
+        "get_adunit_counters": (I, [vp, I, P(C.c_int64), I]),
         "get_work_counts": (I, [vp, P(cs_work_counts), I]),
         "kernel_num": (I, [vp, P(D), P(D), I]),
         "dist2_persons": (I, [vp, P(C.c_int32), P(C.c_int32), P(D), I]),

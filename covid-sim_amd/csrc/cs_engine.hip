@@ -18,9 +18,11 @@ using cs::Dev;
 
 // destination tables chosen at allocation: the full NC x NC max_trans and
 // per-realisation cum_trans rows while (realisations + 1) x NC^2 floats stay
-// within this many bytes, else checkpoints every CS_DEFAULT_CK_STRIDE cells
+// within CS_FULL_TABLE_BYTES, else checkpoints at the smallest power-of-two
+// stride (at least 8) whose R x NC x NC / stride floats stay within
+// CS_CK_TABLE_BYTES (a destination search redoes at most a stride of terms)
 #define CS_FULL_TABLE_BYTES (16ull << 30)
-#define CS_DEFAULT_CK_STRIDE 128
+#define CS_CK_TABLE_BYTES (16ull << 30)
 
 struct cs_engine {
     char err[512];
@@ -41,6 +43,7 @@ struct cs_engine {
     int32_t* rseed_buf = nullptr;    // [R] storage behind Dev.rseed_n
     double* rthr_buf = nullptr;      // [R] storage behind Dev.rthr
     int dest_stride = -1;            // destination tables asked for: -1 chosen at allocation, 0 full, > 0 checkpoint stride
+    int cxy_nch = 0;                 // the nch Dev.cxy was built with
     float* mt_buf = nullptr;         // [NC][NC] max_trans (full tables only; built when first needed)
     int ts_max = 0, ts_n = 0;
     std::vector<uint8_t> flags0;     // [N] each person's Hot flags after a reset (no links, household size), for loaded states
@@ -583,6 +586,8 @@ int cs_set_params(cs_engine* e, const cs_params* p, const double* kl, const doub
     if (p->kernel_size <= 0 || !(p->kernel_delta > 0)) return set_err(e, CS_EINVAL, "kernel table size/delta not set");
     if (p->num_place_types < 0 || p->num_place_types > CS_MAX_PLACE_TYPES) return set_err(e, CS_EINVAL, "bad number of place types");
     if (p->num_holidays < 0 || p->num_holidays > CS_MAX_HOLIDAYS) return set_err(e, CS_EINVAL, "bad number of holidays");
+    if (e->have_pop && p->nch != e->cxy_nch)
+        return set_err(e, CS_ESTATE, "nch changed to %d with a population of %d cells high loaded (set the population again)", p->nch, e->cxy_nch);
     drop_graph(e);  // a captured step follows the host's parameters (holiday launches)
     bool new_tables = !e->have_params || e->P.kernel_size != p->kernel_size || e->P.kernel_type != p->kernel_type ||
                       e->P.kernel_delta != p->kernel_delta || e->P.kernel_shape != p->kernel_shape ||
@@ -739,6 +744,12 @@ int cs_set_population(cs_engine* e, const cs_population* pop)
     if ((rc = dupload(e, L, &d.cstart, pop->cell_member_start, NC + 1))) return rc;
     if ((rc = dupload(e, L, &d.cn, pop->cell_n, NC))) return rc;
     if ((rc = dupload(e, L, &d.cidx, pop->cell_index, NC))) return rc;
+    {   // grid coordinates of the populated cells (dist2_cc_min without divisions)
+        std::vector<int2> xy(NC);
+        for (int j = 0; j < NC; j++) xy[j] = make_int2(pop->cell_index[j] / e->P.nch, pop->cell_index[j] % e->P.nch);
+        if ((rc = dupload(e, L, &d.cxy, xy.data(), NC))) return rc;
+        e->cxy_nch = e->P.nch;
+    }
     if ((rc = dupload(e, L, &d.cmem, pop->cell_members, N))) return rc;
     if ((rc = dupload(e, L, &d.mstart, mstart.data(), e->NM + 1))) return rc;
     if ((rc = dupload(e, L, &d.mmem, mmem.data(), N))) return rc;
@@ -856,7 +867,9 @@ int cs_alloc_realisations(cs_engine* e, int n_real, const uint64_t* seeds)
     AL(tot_prob, RC); AL(perm, RN);
     {   // destination tables: full NC x NC rows while they fit, else checkpoints
         int stride = e->dest_stride;
-        if (stride < 0) stride = ((R + 1) * NC * NC * 4 <= CS_FULL_TABLE_BYTES) ? 0 : CS_DEFAULT_CK_STRIDE;
+        if (stride < 0 && (R + 1) * NC * NC * 4 <= CS_FULL_TABLE_BYTES) stride = 0;
+        if (stride < 0)
+            for (stride = 8; (double)R * NC * NC * 4 / stride > (double)CS_CK_TABLE_BYTES; stride *= 2) {}
         d.ck = stride;
         d.ck_nb = stride ? (int)((NC + stride - 1) / stride) : 0;
         d.cum_trans = nullptr; d.invcdf = nullptr; d.ck_pre = nullptr;
@@ -905,6 +918,22 @@ int cs_alloc_realisations(cs_engine* e, int n_real, const uint64_t* seeds)
     HIPCHK(e, hipMemset(d.wl, 0, cs::WL_N * 4));
     HIPCHK(e, hipMemset(d.cnt, 0, R * sizeof(cs::Counters)));
     HIPCHK(e, hipMemset(d.cstage, 0, (size_t)CS_TALLY_COPIES * R * sizeof(cs::Counters)));
+    {   // counters and samples by administrative unit, when the units are known
+        const bool units = e->P.do_ad_units && e->adunits_valid && e->NA > 0;
+        d.NA = units ? e->NA : 0;
+        d.ad_cnt = nullptr;
+        if (units) {
+            unsigned long long* ac;
+            if ((rc = dalloc(e, A, &ac, R * (size_t)d.NA * CS_TSU_FIELDS))) return rc;
+            HIPCHK(e, hipMemset(ac, 0, R * (size_t)d.NA * CS_TSU_FIELDS * 8));
+            d.ad_cnt = ac;
+        }
+        d.ext_w = CS_TSA_FIELDS * CS_NUM_AGE_GROUPS + CS_TSU_FIELDS * d.NA;
+        long long* le;
+        if ((rc = dalloc(e, A, &le, R * (size_t)(CS_TSA_CUM * CS_NUM_AGE_GROUPS + CS_TSU_FIELDS * d.NA)))) return rc;
+        d.last_ext = le;
+        d.ts_ext = nullptr;
+    }
     HIPCHK(e, hipMemset(d.m_trig, 0, R * NM * 6 * 4));
     HIPCHK(e, hipMemset(d.itv_n, 0, 4 * sizeof(unsigned)));
     cs::k_reset_interventions<<<grid_for(d.R, 64), 64, 0, e->stream>>>(d);
@@ -959,6 +988,7 @@ int cs_reset_realisations(cs_engine* e, int n_seed)
     hipStream_t st = e->stream;
     e->ts_fresh = true;
     HIPCHK(e, hipMemsetAsync(d.cnt, 0, (size_t)d.R * sizeof(cs::Counters), st));
+    if (d.ad_cnt) HIPCHK(e, hipMemsetAsync(d.ad_cnt, 0, (size_t)d.R * d.NA * CS_TSU_FIELDS * 8, st));
     HIPCHK(e, hipMemsetAsync(d.due_n, 0, CS_DUE_RING * CS_DUE_SHARDS * CS_DUE_STRIDE * sizeof(unsigned), st));
     HIPCHK(e, hipMemsetAsync(d.due_ovf, 0, CS_DUE_RING * sizeof(unsigned), st));
     if (d.due_on != (e->due_ok ? 1 : 0)) {  // fresh state: every latent person is listed from here on
@@ -1166,19 +1196,27 @@ static int run_prepare(cs_engine* e, double t0, int n_steps, int sps)
     }
     if (e->ts_n + ns > d.ts_max || !d.ts_buf) {  // grow, keeping the samples taken so far
         const int cap = std::max(std::max(e->ts_n + ns, 2 * d.ts_max), 1);
-        int32_t* tb;
+        int32_t *tb, *tx;
         if ((rc = dalloc(e, e->real_allocs, &tb, (size_t)d.R * cap * CS_TS_FIELDS))) return rc;
+        if ((rc = dalloc(e, e->real_allocs, &tx, (size_t)d.R * cap * d.ext_w))) return rc;
         HIPCHK(e, hipMemsetAsync(tb, 0, (size_t)d.R * cap * CS_TS_FIELDS * 4, e->stream));
-        if (d.ts_buf && e->ts_n)
+        HIPCHK(e, hipMemsetAsync(tx, 0, (size_t)d.R * cap * d.ext_w * 4, e->stream));
+        if (d.ts_buf && e->ts_n) {
             HIPCHK(e, hipMemcpy2DAsync(tb, (size_t)cap * CS_TS_FIELDS * 4, d.ts_buf, (size_t)d.ts_max * CS_TS_FIELDS * 4,
                                        (size_t)e->ts_n * CS_TS_FIELDS * 4, (size_t)d.R, hipMemcpyDeviceToDevice, e->stream));
+            HIPCHK(e, hipMemcpy2DAsync(tx, (size_t)cap * d.ext_w * 4, d.ts_ext, (size_t)d.ts_max * d.ext_w * 4,
+                                       (size_t)e->ts_n * d.ext_w * 4, (size_t)d.R, hipMemcpyDeviceToDevice, e->stream));
+        }
         if (d.ts_buf) {
             HIPCHK(e, hipStreamSynchronize(e->stream));
-            drop_graph(e);  // the captured record launches hold the old buffer
-            (void)hipFree(d.ts_buf);
-            e->real_allocs.erase(std::remove(e->real_allocs.begin(), e->real_allocs.end(), (void*)d.ts_buf), e->real_allocs.end());
+            drop_graph(e);  // the captured record launches hold the old buffers
+            for (void* old : {(void*)d.ts_buf, (void*)d.ts_ext}) {
+                (void)hipFree(old);
+                e->real_allocs.erase(std::remove(e->real_allocs.begin(), e->real_allocs.end(), old), e->real_allocs.end());
+            }
         }
         d.ts_buf = tb;
+        d.ts_ext = tx;
         d.ts_max = cap;
     }
     const cs::Clock c0 = {t0, 0, e->ts_n};
@@ -1504,6 +1542,51 @@ int cs_get_timeseries(cs_engine* e, int32_t* out, int max_samples, int* n_sample
         if (n) HIPCHK(e, hipMemcpy(out + (size_t)r * max_samples * CS_TS_FIELDS, d.ts_buf + (size_t)r * d.ts_max * CS_TS_FIELDS,
                                    (size_t)n * CS_TS_FIELDS * 4, hipMemcpyDeviceToHost));
     *n_samples = n;
+    return 0;
+}
+
+int cs_get_timeseries_age(cs_engine* e, int32_t* out, int max_samples, int* n_samples)
+{
+    if (!e || !out || !n_samples || max_samples < 0) return CS_EINVAL;
+    HIPCHK(e, hipSetDevice(e->device));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    Dev& d = e->d;
+    const int n = std::min(e->ts_n, max_samples);
+    const size_t W = CS_TSA_FIELDS * CS_NUM_AGE_GROUPS;
+    for (int r = 0; r < d.R && n; r++)
+        HIPCHK(e, hipMemcpy2D(out + (size_t)r * max_samples * W, W * 4, d.ts_ext + (size_t)r * d.ts_max * d.ext_w,
+                              (size_t)d.ext_w * 4, W * 4, (size_t)n, hipMemcpyDeviceToHost));
+    *n_samples = n;
+    return 0;
+}
+
+int cs_get_timeseries_adunit(cs_engine* e, int32_t* out, int max_samples, int* n_samples)
+{
+    if (!e || !out || !n_samples || max_samples < 0) return CS_EINVAL;
+    Dev& d = e->d;
+    if (e->R == 0 || !d.ad_cnt) return set_err(e, CS_ESTATE, "no series by administrative unit (do_ad_units off or units unknown)");
+    HIPCHK(e, hipSetDevice(e->device));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    const int n = std::min(e->ts_n, max_samples);
+    const size_t W = (size_t)CS_TSU_FIELDS * d.NA, off = CS_TSA_FIELDS * CS_NUM_AGE_GROUPS;
+    for (int r = 0; r < d.R && n; r++)
+        HIPCHK(e, hipMemcpy2D(out + (size_t)r * max_samples * W, W * 4, d.ts_ext + (size_t)r * d.ts_max * d.ext_w + off,
+                              (size_t)d.ext_w * 4, W * 4, (size_t)n, hipMemcpyDeviceToHost));
+    *n_samples = n;
+    return 0;
+}
+
+int cs_get_adunit_counters(cs_engine* e, int real, int64_t* out, int reset)
+{
+    if (!e || !out) return CS_EINVAL;
+    Dev& d = e->d;
+    if (real < 0 || real >= e->R) return set_err(e, CS_EINVAL, "realisation %d out of range", real);
+    if (!d.ad_cnt) return set_err(e, CS_ESTATE, "no counters by administrative unit (do_ad_units off or units unknown)");
+    HIPCHK(e, hipSetDevice(e->device));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    const size_t n = (size_t)d.NA * CS_TSU_FIELDS;
+    HIPCHK(e, hipMemcpy(out, d.ad_cnt + (size_t)real * n, n * 8, hipMemcpyDeviceToHost));
+    if (reset) HIPCHK(e, hipMemset(d.ad_cnt + (size_t)real * n, 0, n * 8));
     return 0;
 }
 

@@ -278,7 +278,7 @@ __global__ void k_build_max_trans(Dev d, float* out)
     size_t n = (size_t)d.NC * d.NC;
     if (i >= n) return;
     int l = (int)(i / d.NC), m = (int)(i % d.NC);
-    out[i] = max_trans_of(d, d.cidx[l], d.cidx[m]);
+    out[i] = max_trans_of(d, l, m);
 }
 
 // UpdateProbs(0), S0 part (src/CovidSim.cpp:2756-2768); only realisations
@@ -331,12 +331,11 @@ __global__ void k_expand_rows(Dev d, int r, float* mt_out, float* ct_out, int32_
     const float* pre = d.ck_pre + rc * nb;
     const int32_t* S0 = d.S0 + (size_t)r * NC;
     const float tp = pre[nb - 1];
-    const int gj = d.cidx[j];
     float* ct = ct_out + (size_t)j * NC;
     for (int b = 0; b < nb; b++) {
         float c = b ? pre[b - 1] : 0.f;
         for (int m = b * B; m < min(NC, b * B + B); m++) {
-            const float mt = max_trans_of(d, gj, d.cidx[m]);
+            const float mt = max_trans_of(d, j, m);
             c = c + ((float)S0[m]) * mt;
             ct[m] = c / tp;
             if (mt_out) mt_out[(size_t)j * NC + m] = mt;
@@ -642,6 +641,7 @@ __device__ void do_infect(const Dev& d, int r, int p, double t, unsigned ts, int
     Counters* cn = d.cnt + r;
     tally_indexed(cn, tl);
     tally_scalar(cn, 1, tl.rad2, double_to_bits(tl.rad2), tl.tg, tl.si, (unsigned long long)tl.ntg);
+    if (d.ad_cnt) atomicAdd(ad_counter(d, r, p, CS_TSU_INC_I), 1ull);  // cumI_adunit (src/Update.cpp:159)
 }
 
 // ---------------------------------------------------------- InfectSweep
@@ -1281,6 +1281,7 @@ __global__ void k_resolve_apply(Dev d)
             cadd_merged(&t->cumItype[tl.kind], 1);
             cadd_merged(&t->cumIa[tl.ag], 1);
             cadd_merged(&t->cumI_keyworker[tl.kw], 1);
+            if (d.ad_cnt) cadd_merged(ad_counter(d, r, p, CS_TSU_INC_I), 1);  // cumI_adunit (src/Update.cpp:159)
         }
         if (__ballot(won && r != r0) == 0) {
             const double rad2 = wave_sum_d(won ? tl.rad2 : 0.0);

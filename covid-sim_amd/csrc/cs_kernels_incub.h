@@ -279,6 +279,7 @@ __device__ int do_case(const Dev& d, int r, int p, unsigned ts)
     }
     if (d.bits[gp] & 2) {
         cadd_merged(&cn->cumDC, 1);
+        if (d.ad_cnt) cadd_merged(ad_counter(d, r, p, CS_TSU_INC_DC), 1);  // cumDC_adunit (src/Update.cpp:872)
         // DoDetectedCase (src/Update.cpp:525-560): microcell trigger counts
         uint32_t* tg = d.m_trig + ((size_t)r * d.NM + d.mcell[p]) * 6;
         for (int q = 0; q < 5; q++) atomicAdd(&tg[q], 1u);
@@ -287,6 +288,7 @@ __device__ int do_case(const Dev& d, int r, int p, unsigned ts)
     if (treated(d, gp, ts)) atomicAdd(&d.cumTC[(size_t)r * d.NC + d.pc[p]], 1);
     cadd_merged(&cn->cumC, 1);
     cadd_merged(&cn->cumCa[ag], 1);
+    if (d.ad_cnt) cadd_merged(ad_counter(d, r, p, CS_TSU_INC_C), 1);  // cumC_adunit (src/Update.cpp:892)
     cadd_merged(&cn->cumC_keyworker[keyw_g(d, gp)], 1);
     if (P->do_severity) {
         if (d.sev_final[gp] == CS_SEV_MILD) {
@@ -375,6 +377,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) k
                 else {  // DoDeath
                     d.inf[gp] = (st == CS_CASE) ? CS_DEAD_SYMP : CS_DEAD_ASYMP;
                     cadd_merged(&cn->cumDa[ag], 1);
+                    if (d.ad_cnt) cadd_merged(ad_counter(d, r, p, CS_TSU_INC_DEATH), 1);  // cumD_adunit (src/Update.cpp:976)
                 }
                 batch_add(d, r, c, p);
                 nt++;
@@ -604,6 +607,30 @@ __global__ void k_record(Dev d, int sample)
         for (int k = 0; k <= CS_SK_CRITRECOV; k++) o[CS_TS_PREV_SEV + k] = (int32_t)(long long)cn.sev_prev[k];
         for (int q = 0; q < CS_TS_LAST; q++) lt[q] = now[q];
     }
+    if (sample < d.ts_max && d.ts_ext) {  // by age group and by unit
+        int32_t* o = d.ts_ext + ((size_t)r * d.ts_max + sample) * d.ext_w;
+        long long* le = d.last_ext + (size_t)r * (CS_TSA_CUM * CS_NUM_AGE_GROUPS + CS_TSU_FIELDS * d.NA);
+        const Counters& cn = d.cnt[r];
+        for (int q = threadIdx.x; q < CS_TSA_FIELDS * CS_NUM_AGE_GROUPS; q += blockDim.x) {
+            const int f = q / CS_NUM_AGE_GROUPS, a = q % CS_NUM_AGE_GROUPS;
+            if (f >= CS_TSA_PREV_SEV) {
+                o[q] = (int32_t)(long long)cn.sev_prev_age[f - CS_TSA_PREV_SEV][a];
+                continue;
+            }
+            const long long now = (long long)(f == CS_TSA_INC_I ? cn.cumIa[a] : f == CS_TSA_INC_C ? cn.cumCa[a] :
+                                              f == CS_TSA_INC_DEATH ? cn.cumDa[a] : cn.sev_cum_age[f - CS_TSA_INC_SEV][a]);
+            o[q] = (int32_t)(now - le[q]);
+            le[q] = now;
+        }
+        if (d.ad_cnt)
+            for (int q = threadIdx.x; q < CS_TSU_FIELDS * d.NA; q += blockDim.x) {
+                const int f = q / d.NA, u = q % d.NA;
+                const long long now = (long long)d.ad_cnt[((size_t)r * d.NA + u) * CS_TSU_FIELDS + f];
+                long long* l = le + CS_TSA_CUM * CS_NUM_AGE_GROUPS + q;
+                o[CS_TSA_FIELDS * CS_NUM_AGE_GROUPS + q] = (int32_t)(now - *l);
+                *l = now;
+            }
+    }
 }
 
 __global__ void k_record_init(Dev d, double t0)
@@ -621,6 +648,20 @@ __global__ void k_record_init(Dev d, double t0)
     lt[1] = (long long)d.cnt[r].cumC;
     lt[2] = D;
     for (int k = 0; k < CS_NUM_SEV_KINDS; k++) lt[3 + k] = (long long)d.cnt[r].sev_cum[k];
+    if (d.ts_ext || d.last_ext) {
+        const Counters& cn = d.cnt[r];
+        long long* le = d.last_ext + (size_t)r * (CS_TSA_CUM * CS_NUM_AGE_GROUPS + CS_TSU_FIELDS * d.NA);
+        for (int a = 0; a < CS_NUM_AGE_GROUPS; a++) {
+            le[CS_TSA_INC_I * CS_NUM_AGE_GROUPS + a] = (long long)cn.cumIa[a];
+            le[CS_TSA_INC_C * CS_NUM_AGE_GROUPS + a] = (long long)cn.cumCa[a];
+            le[CS_TSA_INC_DEATH * CS_NUM_AGE_GROUPS + a] = (long long)cn.cumDa[a];
+            for (int k = 0; k < CS_NUM_SEV_KINDS; k++) le[(CS_TSA_INC_SEV + k) * CS_NUM_AGE_GROUPS + a] = (long long)cn.sev_cum_age[k][a];
+        }
+        if (d.ad_cnt)
+            for (int q = 0; q < CS_TSU_FIELDS * d.NA; q++)
+                le[CS_TSA_CUM * CS_NUM_AGE_GROUPS + q] =
+                    (long long)d.ad_cnt[((size_t)r * d.NA + q % d.NA) * CS_TSU_FIELDS + q / d.NA];
+    }
     // the series' first sample and its CalibrationThresholdCheck (src/CovidSim.cpp:1413-1414)
     if (d.P->interventions_active) itv_triggers(d, r, 0, t0, D);
 }

@@ -255,6 +255,7 @@ struct Dev {
     LinkFld plinks;
     GLinkFld pglinks;
     const int32_t *pc, *mcell, *cstart, *cn, *cidx, *mstart, *mmem, *cmem;
+    const int2* cxy;      // [NC] grid coordinates (cidx / nch, cidx % nch) of each populated cell
     const int32_t* padunit;  // administrative unit of each person's microcell
     int np[CS_MAX_PLACE_TYPES];
     int np_off[CS_MAX_PLACE_TYPES + 1];  // offsets of each type in the per-realisation place arrays
@@ -378,6 +379,14 @@ struct Dev {
     int32_t* ts_buf;      // [R][max_samples][CS_TS_FIELDS]
     int ts_max;
     long long* last_tot;  // [R][CS_TS_LAST] cumI, cumC, D, severity incidence at the previous sample
+    // the samples by age group and by administrative unit: a row of
+    // ext_w = CS_TSA_FIELDS * CS_NUM_AGE_GROUPS + CS_TSU_FIELDS * NA words
+    // per (realisation, sample), differencing the cumulative counters kept
+    // in last_ext ([R][CS_TSA_CUM * CS_NUM_AGE_GROUPS + CS_TSU_FIELDS * NA])
+    int32_t* ts_ext;
+    long long* last_ext;
+    int ext_w, NA;
+    unsigned long long* ad_cnt;  // [R][NA][CS_TSU_FIELDS] cumI, cumC, cumDC, cumD by unit, or null
     long long* due;       // [CS_DUE_RING][CS_DUE_SHARDS][due_scap] persons (gp) by latent time
     unsigned* due_n;      // [CS_DUE_RING][CS_DUE_SHARDS] counts, CS_DUE_STRIDE words apart
     unsigned* due_ovf;    // [CS_DUE_RING] a shard of the bucket overflowed: that step scans every latent person
@@ -457,6 +466,12 @@ inline double kernel_fn(const cs_params& P, double r2)
     }
 }
 
+// a person's counters by administrative unit (StateT's *_adunit counters)
+__device__ __forceinline__ unsigned long long* ad_counter(const Dev& d, int r, int p, int field)
+{
+    return d.ad_cnt + ((size_t)r * d.NA + d.padunit[p]) * CS_TSU_FIELDS + field;
+}
+
 // ---- kernel lookup (src/Kernels.cpp:108-129) ----
 __device__ __forceinline__ double kernel_num(const Dev& d, double r2)
 {
@@ -530,40 +545,59 @@ __device__ __forceinline__ double dist2_ll(const cs_params* P, float2 la, float2
 
 __device__ __forceinline__ double dist2_pp(const Dev& d, int a, int b) { return dist2_ll(d.P, d.ploc[a], d.ploc[b]); }
 
-__device__ __noinline__ double dist2_cc_min(const cs_params* P, int l, int m)
+// dist2_cc_min (src/Dist.cpp:79-144) on cell grid coordinates: cell l is
+// (lx, ly) = (l / nch, l % nch).  The reference moves the corner cells by
+// whole indices (i += nch: the next column; i++: the next row, wrapping
+// into the next column at the top), then divides again; here the same
+// moves are made on the coordinates, so every double is formed from the
+// same integers as the reference's, with no integer division.
+__device__ __forceinline__ void cc_next_row(int& x, int& y, int nch)
 {
-    int nch = P->nch, i = l, j = m;
+    if (++y == nch) {
+        y = 0;
+        x++;
+    }
+}
+__device__ __forceinline__ double dist2_cc_min_xy(const cs_params* P, int lx, int ly, int mx, int my)
+{
+    const int nch = P->nch;
+    int ix = lx, iy = ly, jx = mx, jy = my;
     if (P->do_utm_coords) {
         const double PI_ = 3.1415926535;
-        if (P->cell_width * ((double)abs(m / nch - l / nch)) > PI_) {
-            if (m / nch > l / nch) j += nch;
-            else if (m / nch < l / nch) i += nch;
+        if (P->cell_width * ((double)abs(mx - lx)) > PI_) {
+            if (mx > lx) jx++;
+            else if (mx < lx) ix++;
         } else {
-            if (m / nch > l / nch) i += nch;
-            else if (m / nch < l / nch) j += nch;
+            if (mx > lx) ix++;
+            else if (mx < lx) jx++;
         }
-        if (m % nch > l % nch) i++;
-        else if (m % nch < l % nch) j++;
-        return dist2_utm(P, P->cell_width * __builtin_fabs((double)(i / nch)), P->cell_height * __builtin_fabs((double)(i % nch)),
-                         P->cell_width * __builtin_fabs((double)(j / nch)), P->cell_height * __builtin_fabs((double)(j % nch)));
+        if (my > ly) cc_next_row(ix, iy, nch);
+        else if (my < ly) cc_next_row(jx, jy, nch);
+        return dist2_utm(P, P->cell_width * __builtin_fabs((double)ix), P->cell_height * __builtin_fabs((double)iy),
+                         P->cell_width * __builtin_fabs((double)jx), P->cell_height * __builtin_fabs((double)jy));
     }
-    if (P->do_periodic_boundaries && (P->cell_width * ((double)abs(m / nch - l / nch)) > P->degrees_width * 0.5)) {
-        if (m / nch > l / nch) j += nch;
-        else if (m / nch < l / nch) i += nch;
+    if (P->do_periodic_boundaries && (P->cell_width * ((double)abs(mx - lx)) > P->degrees_width * 0.5)) {
+        if (mx > lx) jx++;
+        else if (mx < lx) ix++;
     } else {
-        if (m / nch > l / nch) i += nch;
-        else if (m / nch < l / nch) j += nch;
+        if (mx > lx) ix++;
+        else if (mx < lx) jx++;
     }
-    if (P->do_periodic_boundaries && (P->degrees_height * ((double)abs(m % nch - l % nch)) > P->degrees_height * 0.5)) {
-        if (m % nch > l % nch) j++;
-        else if (m % nch < l % nch) i++;
+    if (P->do_periodic_boundaries && (P->degrees_height * ((double)abs(my - ly)) > P->degrees_height * 0.5)) {
+        if (my > ly) cc_next_row(jx, jy, nch);
+        else if (my < ly) cc_next_row(ix, iy, nch);
     } else {
-        if (m % nch > l % nch) i++;
-        else if (m % nch < l % nch) j++;
+        if (my > ly) cc_next_row(ix, iy, nch);
+        else if (my < ly) cc_next_row(jx, jy, nch);
     }
-    double x = P->cell_width * __builtin_fabs((double)(i / nch - j / nch));
-    double y = P->cell_height * __builtin_fabs((double)(i % nch - j % nch));
+    double x = P->cell_width * __builtin_fabs((double)(ix - jx));
+    double y = P->cell_height * __builtin_fabs((double)(iy - jy));
     return periodic_xy(P, x, y);
+}
+
+__device__ __noinline__ double dist2_cc_min(const cs_params* P, int l, int m)
+{
+    return dist2_cc_min_xy(P, l / P->nch, l % P->nch, m / P->nch, m % P->nch);
 }
 
 // ---- InverseCdf::choose for a given uniform (src/InverseCdf.cpp:60-74) ----

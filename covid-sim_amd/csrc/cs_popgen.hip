@@ -596,7 +596,7 @@ __global__ void k_build_mt(Dev gd, float* out)
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (size_t)gd.NC * gd.NC) return;
     const int l = (int)(i / gd.NC), m = (int)(i % gd.NC);
-    out[i] = cs::max_trans_of(gd, gd.cidx[l], gd.cidx[m]);
+    out[i] = cs::max_trans_of(gd, l, m);
 }
 
 __global__ void k_grav_round(Dev gd, int64_t nact, const int32_t* act, int tp, int round, const int32_t* p_hh,
@@ -1239,8 +1239,13 @@ int cs_popgen_run(cs_popgen* g, const cs_popgen_params* p, const cs_params* kp, 
     DArr<uint8_t> d_elig;
     PGALLOC(g, d_elig, N);
     DArr<int32_t> d_country, d_cellmap, d_cidx, d_cn;
+    DArr<int2> d_cxy;
+    std::vector<int2> cxy(NC);
+    for (int c = 0; c < NC; c++) cxy[c] = make_int2(g->cell_index[c] / kp->nch, g->cell_index[c] % kp->nch);
+    if (kp->nch != p->nch) return pg_err(g, CS_EINVAL, "cs_params.nch %d and the generator's nch %d differ", kp->nch, p->nch);
     if ((rc = to_dev(g, d_country, mcell_country, NM)) || (rc = to_dev(g, d_cellmap, cell_map.data(), NCT)) ||
-        (rc = to_dev(g, d_cidx, g->cell_index.data(), NC)) || (rc = to_dev(g, d_cn, g->cell_n.data(), NC)))
+        (rc = to_dev(g, d_cidx, g->cell_index.data(), NC)) || (rc = to_dev(g, d_cn, g->cell_n.data(), NC)) ||
+        (rc = to_dev(g, d_cxy, cxy.data(), NC)))
         return rc;
     for (int t = 0; t < g->NPT; t++) {
         if (t == p->hotel_place_type) continue;
@@ -1326,6 +1331,7 @@ int cs_popgen_run(cs_popgen* g, const cs_popgen_params* p, const cs_params* kp, 
             gd.NC = NC;
             gd.R = 1;
             gd.cidx = d_cidx.p;
+            gd.cxy = d_cxy.p;
             gd.cn = d_cn.p;
             DArr<int32_t> S0;
             DArr<float> mt, ct, tot, ckp;
@@ -1344,8 +1350,9 @@ int cs_popgen_run(cs_popgen* g, const cs_popgen_params* p, const cs_params* kp, 
                 gd.invcdf = inv.p;
                 pg::k_build_mt<<<blocks((long long)NC * NC), 256, 0, st>>>(gd, mt.p);
             } else {
-                gd.ck = 128;
-                gd.ck_nb = (NC + 127) / 128;
+                gd.ck = 8;
+                while ((double)NC * NC * 4 / gd.ck > (double)(16ull << 30)) gd.ck *= 2;
+                gd.ck_nb = (NC + gd.ck - 1) / gd.ck;
                 PGALLOC(g, ckp, (size_t)NC * gd.ck_nb);
                 gd.ck_pre = ckp.p;
             }

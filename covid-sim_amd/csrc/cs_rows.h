@@ -13,10 +13,12 @@ namespace cs {
 #define CS_INV_PURE 0x80000000u
 
 // max_trans[l][m] = (float)num(dist2_cc_min(l, m)) (src/Kernels.cpp:48-63),
-// with l and m grid indices of populated cells
-__device__ __forceinline__ float max_trans_of(const Dev& d, int gl, int gm)
+// l and m populated cells (CellLookup order); their grid coordinates come
+// from d.cxy
+__device__ __forceinline__ float max_trans_of(const Dev& d, int l, int m)
 {
-    return (float)kernel_num(d, dist2_cc_min(d.P, gl, gm));
+    const int2 a = d.cxy[l], b = d.cxy[m];
+    return (float)kernel_num(d, dist2_cc_min_xy(d.P, a.x, a.y, b.x, b.y));
 }
 
 // row j of realisation r from d.S0 (src/CovidSim.cpp:2780-2800): summed in
@@ -50,30 +52,6 @@ __device__ inline void update_row(const Dev& d, int r, int j)
     }
 }
 
-// the same row with checkpointed tables: max_trans evaluated term by term;
-// the running prefix is kept at the last cell of every block of ck cells
-// (the last block's entry is the row total), so a destination search can
-// redo any block's prefixes exactly from the entry before it
-__device__ inline void update_row_ck(const Dev& d, int r, int j)
-{
-    const size_t i = (size_t)r * d.NC + j;
-    const int NC = d.NC, B = d.ck;
-    const int32_t* S0 = d.S0 + (size_t)r * NC;
-    float* pre = d.ck_pre + i * d.ck_nb;
-    const int gj = d.cidx[j];
-    float c = 0.f, t = 0.f;  // 0 + x == x: the first term is the reference's ct[0] and t
-    for (int m0 = 0, b = 0; m0 < NC; m0 += B, b++) {
-        const int m1 = min(NC, m0 + B);
-        for (int m = m0; m < m1; m++) {
-            const float mt = max_trans_of(d, gj, d.cidx[m]);
-            c = c + ((float)S0[m]) * mt;
-            t = t + ((float)d.cn[m]) * mt;
-        }
-        pre[b] = c;
-    }
-    d.tot_prob[i] = c / t;
-}
-
 // destination cell: InvCDF then linear scan of cum_trans (src/Sweep.cpp:757-762)
 __device__ __forceinline__ int select_dest(const Dev& d, size_t rc, double u, unsigned* scanned = nullptr)
 {
@@ -87,6 +65,30 @@ __device__ __forceinline__ int select_dest(const Dev& d, size_t rc, double u, un
     // the reference reads cum_trans[l0 .. l] (one entry for a pure bucket)
     if (scanned) *scanned += (unsigned)(l - l0 + 1);
     return l;
+}
+
+// row j of realisation r with checkpointed tables: update_row's sums (the
+// same float terms added in cell order), max_trans evaluated term by term;
+// the running prefix is kept only at the last cell of every block of ck
+// cells (the last block's entry is the row total), so a destination search
+// can redo any block's prefixes exactly from the entry before it
+__device__ inline void update_row_ck(const Dev& d, int r, int j)
+{
+    const size_t i = (size_t)r * d.NC + j;
+    const int NC = d.NC, B = d.ck;
+    const int32_t* S0 = d.S0 + (size_t)r * NC;
+    float* pre = d.ck_pre + i * d.ck_nb;
+    float c = 0.f, t = 0.f;  // 0 + x == x: the first term is the reference's ct[0] and t
+    for (int m0 = 0, b = 0; m0 < NC; m0 += B, b++) {
+        const int m1 = min(NC, m0 + B);
+        for (int m = m0; m < m1; m++) {
+            const float mt = max_trans_of(d, j, m);
+            c = c + ((float)S0[m]) * mt;
+            t = t + ((float)d.cn[m]) * mt;
+        }
+        pre[b] = c;
+    }
+    d.tot_prob[i] = c / t;
 }
 
 // destination cell with checkpointed tables: the first cell whose
@@ -110,12 +112,11 @@ __device__ __forceinline__ int select_dest_ck(const Dev& d, int r, int c, double
     }
     float s = lo ? pre[lo - 1] : 0.f;
     const int32_t* S0 = d.S0 + (size_t)r * NC;
-    const int gc = d.cidx[c];
     const int m1 = min(NC, lo * B + B) - 1;
     int m = lo * B;
     float w;
     for (;; m++) {
-        w = max_trans_of(d, gc, d.cidx[m]);
+        w = max_trans_of(d, c, m);
         s = s + ((float)S0[m]) * w;
         if (m == m1 || (double)(s / tp) >= u) break;  // the block's last prefix reaches u
     }

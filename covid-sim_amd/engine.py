@@ -217,6 +217,30 @@ class Engine:
         self.call("get_timeseries", _p(out, C.c_int32), int(max_samples), C.byref(n))
         return out[:, :n.value]
 
+    def timeseries_age(self, max_samples):
+        """[n_real, n_samples, CS_TSA_FIELDS, CS_NUM_AGE_GROUPS] int32: per
+        sample and age group the incidence of infections, cases, deaths and
+        each severity kind, and severity prevalence."""
+        out = np.zeros((self.n_real, max_samples, abi.CS_TSA_FIELDS, abi.CS_NUM_AGE_GROUPS), np.int32)
+        n = C.c_int()
+        self.call("get_timeseries_age", _p(out, C.c_int32), int(max_samples), C.byref(n))
+        return out[:, :n.value]
+
+    def timeseries_adunit(self, max_samples, n_adunits):
+        """[n_real, n_samples, CS_TSU_FIELDS, n_adunits] int32: per sample
+        and administrative unit the incidence of infections, cases,
+        detected cases and deaths."""
+        out = np.zeros((self.n_real, max_samples, abi.CS_TSU_FIELDS, n_adunits), np.int32)
+        n = C.c_int()
+        self.call("get_timeseries_adunit", _p(out, C.c_int32), int(max_samples), C.byref(n))
+        return out[:, :n.value]
+
+    def adunit_counters(self, r, n_adunits, reset=False):
+        """[n_adunits, CS_TSU_FIELDS] int64: cumI, cumC, cumDC, cumD by unit."""
+        out = np.zeros((n_adunits, abi.CS_TSU_FIELDS), np.int64)
+        self.call("get_adunit_counters", int(r), _p(out, C.c_int64), int(bool(reset)))
+        return out
+
     def work_counts(self, reset=False):
         w = abi.cs_work_counts()
         self.call("get_work_counts", C.byref(w), int(bool(reset)))

@@ -10,6 +10,9 @@
                                 src/SetupModel.cpp:944-987)
   write_network / read_network  the network file (/S:, /L:,
                                 src/SetupModel.cpp:2603-2693)
+  read_admin_units(path)        the units an admin file defines (code,
+                                country, name), as ReadParams keeps them
+                                (src/ReadParams.cpp:1814-1833)
   generate(...)                 a whole population from the device generator
                                 (cs_popgen_* of include/covidsim_hip.h), as
                                 the arrays of population.Population
@@ -33,6 +36,33 @@ def load_popgen_params(path):
         raise ValueError("%s: %d bytes, cs_popgen_params is %d" % (path, len(raw), C.sizeof(g)))
     C.memmove(C.addressof(g), raw, len(raw))
     return g
+
+
+def read_admin_units(path):
+    """[(code, country, name)] of the units an admin file defines: the
+    "Codes and country/province names for admin units" triples whose country
+    is among "List of names of countries to include" and whose code is not
+    zero, in file order (src/ReadParams.cpp:1814-1833).  A file without
+    country names defines no named units: []."""
+    blocks, key = {}, None
+    with open(path) as f:
+        for line in f:
+            line = line.strip()
+            if line.startswith("[") and line.endswith("]"):
+                key = line[1:-1]
+                blocks[key] = []
+            elif line and key is not None:
+                blocks[key] += line.split()
+    trip = blocks.get("Codes and country/province names for admin units", [])
+    countries = blocks.get("List of names of countries to include", [])
+    nc = abs(int(blocks.get("Number of countries to include", ["0"])[0]))
+    countries = countries[:nc]
+    units = []
+    for i in range(len(trip) // 3):
+        code, cnt, name = trip[3 * i:3 * i + 3]
+        if cnt in countries and int(code) != 0:
+            units.append((int(code), cnt, name))
+    return units
 
 
 def read_density(path, pg):

@@ -1,5 +1,6 @@
 """Ensemble summaries of a batched run, in the layout of CovidSim's summary
-outputs (SaveSummaryResults, src/CovidSim.cpp:1960-2400).
+outputs (SaveSummaryResults, src/CovidSim.cpp:1960-2400): write (totals and
+severity), write_age (.age.xls) and write_adunit (.adunit.xls).
 
 For every sample, each field's mean over the realisations, c * sum(x), and
 its variance, c * sum(x^2) - (c * sum(x))^2 with c = 1 / realisations (the
@@ -58,3 +59,76 @@ def read(path):
     with open(path) as f:
         names = f.readline().rstrip("\n").split("\t")
     return names, np.loadtxt(path, skiprows=1, ndmin=2)
+
+
+def _mean(x):
+    """c * sum over realisations (TSMean arithmetic), c = 1 / realisations"""
+    x = np.asarray(x, np.float64)
+    return (1.0 / x.shape[0]) * x.sum(axis=0)
+
+
+def _rows(series, start, sample_days, t0):
+    """(times, per-sample means) with the optional sample taken at t0 first"""
+    m = _mean(series)
+    if start is None:
+        return [t0 + (i + 1) * sample_days for i in range(m.shape[0])], m
+    return ([t0 + i * sample_days for i in range(m.shape[0] + 1)],
+            np.concatenate([_mean(start)[None], m], axis=0))
+
+
+def write_age(path, series_age, ages, sample_days=1.0, t0=0.0, start=None):
+    """The ensemble's age file (.age.xls of SaveSummaryResults,
+    src/CovidSim.cpp:2035-2063): per sample the mean incidence of infections,
+    cases and deaths of each 5-year age group, then a `dist` row with the
+    population's count in each group (AgeDist).  series_age is
+    [n_real, n_samples, CS_TSA_FIELDS, CS_NUM_AGE_GROUPS]
+    (Engine.timeseries_age); ages the persons' ages in years.  start, shaped
+    [n_real, CS_TSA_FIELDS, CS_NUM_AGE_GROUPS], is the sample the reference
+    takes at t0 itself (the seed infections: Engine.counters' cumIa after
+    the reset); given, it is the first row, as in the reference's file."""
+    g = abi.CS_NUM_AGE_GROUPS
+    ts, m = _rows(series_age, start, sample_days, t0)
+    cols = []
+    for tag in ("I", "C", "D"):
+        cols += ["%s%d-%d" % (tag, 5 * i, 5 * (i + 1)) for i in range(g)]
+    dist = np.bincount(np.minimum(np.asarray(ages, np.int64) // 5, g - 1), minlength=g)
+    with open(path, "w") as f:
+        f.write("t\t" + "\t".join(cols) + "\n")
+        for i, t in enumerate(ts):
+            row = [t]
+            for fld in (abi.CS_TSA_INC_I, abi.CS_TSA_INC_C, abi.CS_TSA_INC_DEATH):
+                row += list(m[i, fld])
+            f.write("\t".join("%.10f" % v for v in row) + "\n")
+        f.write("dist" + "".join("\t%.10f" % v for v in dist) + "\n")
+
+
+def write_adunit(path, series_adunit, names, unit_n, pop_by_adunit=None, sample_days=1.0, t0=0.0,
+                 start=None):
+    """The ensemble's administrative-unit file (.adunit.xls of
+    SaveSummaryResults, src/CovidSim.cpp:2065-2090): a header of columns
+    I_<name>, C_<name>, DC_<name>, T_<name> followed by each unit's person
+    count (unit_n, AdUnits[i].n) and its PopByAdunit[i][1] entry
+    (pop_by_adunit, zero when not given, as it stays unless a regional
+    demography file is read, src/SetupModel.cpp:1018-1045); then per sample the mean
+    incidence of infections, cases and detected cases of each unit and the
+    mean cumulative treated count, which is zero: treatment is not part of
+    this path.  series_adunit is [n_real, n_samples, CS_TSU_FIELDS, n_adunits]
+    (Engine.timeseries_adunit); start, [n_real, CS_TSU_FIELDS, n_adunits],
+    the sample taken at t0 (Engine.adunit_counters after the reset), written
+    first when given."""
+    na = len(names)
+    ts, m = _rows(series_adunit, start, sample_days, t0)
+    if pop_by_adunit is None:
+        pop_by_adunit = np.zeros(na)
+    cols = []
+    for tag in ("I", "C", "DC", "T"):
+        cols += ["%s_%s" % (tag, n) for n in names]
+    with open(path, "w") as f:
+        f.write("t\t" + "\t".join(cols) + "".join("\t%d" % int(v) for v in unit_n[:na])
+                + "".join("\t%.10f" % v for v in pop_by_adunit[:na]) + "\n")
+        for i, t in enumerate(ts):
+            row = [t]
+            for fld in (abi.CS_TSU_INC_I, abi.CS_TSU_INC_C, abi.CS_TSU_INC_DC):
+                row += list(m[i, fld, :na])
+            row += [0.0] * na
+            f.write("\t".join("%.10f" % v for v in row) + "\n")

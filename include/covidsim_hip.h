@@ -448,9 +448,10 @@ int  cs_batch_run_steps(cs_batch* b, double t0, int n_steps, int steps_per_sampl
  * stride-th cell only (R x NC x ceil(NC / stride) floats), a destination
  * search redoing one block's prefixes in the same order.  Both give the
  * same destinations for the same uniforms, bit for bit.  -1 (default):
- * full tables while (realisations + 1) x NC^2 floats fit 16 GiB, else
- * stride 128.  cs_get_dest_tables gives the stride in use after an
- * allocation (the requested value before). */
+ * full tables while (realisations + 1) x NC^2 floats fit 16 GiB, else the
+ * smallest power-of-two stride (at least 8) whose checkpoints fit 16 GiB.
+ * cs_get_dest_tables gives the stride in use after an allocation (the
+ * requested value before). */
 int  cs_set_dest_tables(cs_engine* e, int stride);
 int  cs_get_dest_tables(const cs_engine* e, int* stride);
 /* 1 (default): cs_run_steps replays captured step graphs; 0: it launches
@@ -526,6 +527,27 @@ enum { CS_TS_S = 0, CS_TS_L, CS_TS_I, CS_TS_R, CS_TS_D, CS_TS_INC_I, CS_TS_INC_C
        CS_TS_PREV_SEV = 16 }; /* + CS_SK_MILD .. CS_SK_CRITRECOV */
 #define CS_TS_LAST 11   /* cumulative counters a sample differences: cumI, cumC, D, 8 severity kinds */
 int  cs_get_timeseries(cs_engine* e, int32_t* out, int max_samples, int* n_samples);
+/* the same samples by age group and by administrative unit (RecordSample's
+ * by-age and by-unit fields, src/CovidSim.cpp:3024-3483, from StateT's
+ * cumIa/cumCa/cumDa, the severity counters by age, and cumI_adunit,
+ * cumC_adunit, cumDC_adunit and cumD_adunit, src/Update.cpp:159, :872,
+ * :892, :976).  By age: out [n_real][max_samples][CS_TSA_FIELDS]
+ * [CS_NUM_AGE_GROUPS]; by unit: out [n_real][max_samples][CS_TSU_FIELDS]
+ * [n_adunits], kept when do_ad_units is set and every person's unit lies in
+ * [0, n_adunits) (CS_ESTATE otherwise). */
+#define CS_TSA_FIELDS 16
+enum { CS_TSA_INC_I = 0, CS_TSA_INC_C, CS_TSA_INC_DEATH,
+       CS_TSA_INC_SEV = 3,     /* + CS_SK_* kind, 8 kinds */
+       CS_TSA_PREV_SEV = 11 }; /* + CS_SK_MILD .. CS_SK_CRITRECOV */
+#define CS_TSA_CUM 11   /* by-age fields a sample differences: the incidences before CS_TSA_PREV_SEV */
+#define CS_TSU_FIELDS 4
+enum { CS_TSU_INC_I = 0, CS_TSU_INC_C, CS_TSU_INC_DC, CS_TSU_INC_DEATH };
+int  cs_get_timeseries_age(cs_engine* e, int32_t* out, int max_samples, int* n_samples);
+int  cs_get_timeseries_adunit(cs_engine* e, int32_t* out, int max_samples, int* n_samples);
+/* realisation real's counters by administrative unit: out [n_adunits][4] =
+ * cumI, cumC, cumDC, cumD (StateT's *_adunit counters); reset != 0 zeroes
+ * them (as cs_get_counters, not to be interleaved with a series) */
+int  cs_get_adunit_counters(cs_engine* e, int real, int64_t* out, int reset);
 int  cs_get_work_counts(cs_engine* e, cs_work_counts* out, int reset);
 int  cs_sync(cs_engine* e);
 

@@ -395,6 +395,7 @@ typedef struct {
     uint8_t *m_socdist, *m_moverest;
     uint32_t* m_trig; /* [n_mcells][6] */
     cs_counters cnt;
+    int64_t* ad_cnt;       /* [NAU][CS_TSU_FIELDS] cumI, cumC, cumDC, cumD by administrative unit (StateT's *_adunit) */
     double prev_prop_susc;
     double samp_prop_susc; /* State.S / PopSize as of the last sample */
     int has_beta;          /* per-realisation LocalBeta override */
@@ -454,6 +455,11 @@ struct or_engine {
     int ts_n, ts_cap;
     int ts_fresh;           /* no sample since the last alloc/reset/load: the series restarts */
     int64_t* ts_last;       /* [nreal][3] cumI, cumC, D at the previous sample */
+    /* the samples by age group and by administrative unit: rows of ext_w
+     * words, differencing the cumulative counters kept in ext_last */
+    int32_t* ts_ext;
+    int64_t* ext_last;      /* [nreal][CS_TSA_CUM * CS_NUM_AGE_GROUPS + CS_TSU_FIELDS * NAU] */
+    int NAU, ext_w;         /* units counted (0: none), row width */
     cs_work_counts wc;
     int error;
 };
@@ -846,7 +852,7 @@ static void free_real(or_engine* e, or_real* r)
     void* ptrs[] = {r->inf, r->sev_cur, r->sev_final, r->bits, r->qc, r->dct, r->hh_done, r->listpos, r->infector, r->itype,
         r->infness, r->susc, r->prob_care, r->t_inf, r->t_lat, r->t_rod, r->t_sari, r->t_crit, r->t_step, r->t_det,
         r->t_as, r->t_ae, r->t_ts, r->t_te, r->t_vs, r->t_is, r->t_qs, r->cS, r->cL, r->cI, r->cR, r->cD, r->cumTC,
-        r->S0, r->tot_prob, r->cum_trans, r->invcdf, r->perm, r->m_socdist, r->m_moverest, r->m_trig};
+        r->S0, r->tot_prob, r->cum_trans, r->invcdf, r->perm, r->m_socdist, r->m_moverest, r->m_trig, r->ad_cnt};
     for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) free(ptrs[i]);
     for (int t = 0; t < CS_MAX_PLACE_TYPES; t++) { free(r->pl_cs[t]); free(r->pl_ce[t]); free(r->pl_once[t]); }
     free(r->hist); free(r->mc_state); free(r->mc_end);
@@ -866,6 +872,7 @@ void or_destroy(or_engine* e)
         free(e->pl_gs0[t]); free(e->pl_gst[t]); free(e->pl_gsz[t]);
     }
     free(e->samp); free(e->cell_inf); free(e->max_trans); free(e->ev); free(e->ts_buf); free(e->ts_last);
+    free(e->ts_ext); free(e->ext_last);
     free(e->mpl_start); free(e->mpl);
     free(e);
 }
@@ -874,8 +881,18 @@ int or_alloc_realisations(or_engine* e, int n_real, const uint64_t* seeds)
 {
     for (int i = 0; i < e->nreal; i++) free_real(e, &e->R[i]);
     free(e->R);
-    free(e->ts_buf); free(e->ts_last);
-    e->ts_buf = NULL; e->ts_last = NULL; e->ts_cap = e->ts_n = 0; e->ts_fresh = 1;
+    free(e->ts_buf); free(e->ts_last); free(e->ts_ext); free(e->ext_last);
+    e->ts_buf = NULL; e->ts_last = NULL; e->ts_ext = NULL; e->ext_last = NULL;
+    e->ts_cap = e->ts_n = 0; e->ts_fresh = 1;
+    {   /* units are counted when known for every person */
+        int ok = e->P.do_ad_units && e->NA > 0 && e->madunit != NULL;
+        for (int i = 0; i < e->N && ok; i++) {
+            const int u = e->madunit[e->mcell[i]];
+            if (u < 0 || u >= e->NA) ok = 0;
+        }
+        e->NAU = ok ? e->NA : 0;
+        e->ext_w = CS_TSA_FIELDS * CS_NUM_AGE_GROUPS + CS_TSU_FIELDS * e->NAU;
+    }
     e->nreal = n_real;
     e->R = (or_real*)calloc((size_t)n_real, sizeof(or_real));
     int N = e->N, NC = e->NC;
@@ -897,6 +914,7 @@ int or_alloc_realisations(or_engine* e, int n_real, const uint64_t* seeds)
         for (int t = 0; t < e->NPT; t++) { AL(pl_cs[t], e->np[t], uint16_t); AL(pl_ce[t], e->np[t], uint16_t); AL(pl_once[t], e->np[t], uint8_t); }
         AL(hist, (size_t)OR_TRIG_RING * 3, int32_t); AL(mc_state, e->NM, uint8_t); AL(mc_end, e->NM, uint16_t);
         AL(m_socdist, e->NM, uint8_t); AL(m_moverest, e->NM, uint8_t); AL(m_trig, (size_t)e->NM * 6, uint32_t);
+        AL(ad_cnt, (size_t)(e->NAU ? e->NAU : 1) * CS_TSU_FIELDS, int64_t);
 #undef AL
     }
     return 0;
@@ -1002,6 +1020,7 @@ static void do_infect(or_engine* e, or_real* r, int p, double t, unsigned ts)
     r->cnt.cumI++;
     r->cnt.cumItype[r->itype[p] % CS_INFECT_TYPE_MASK]++;
     r->cnt.cumIa[ag]++;
+    if (e->NAU) r->ad_cnt[(size_t)e->madunit[e->mcell[p]] * CS_TSU_FIELDS + CS_TSU_INC_I]++; /* cumI_adunit (src/Update.cpp:159) */
     r->cnt.sumRad2 += rad2;
     if (rad2 > r->cnt.maxRad2) r->cnt.maxRad2 = rad2;
     r->cnt.cumI_keyworker[KEYW(e, p)]++;
@@ -1462,6 +1481,7 @@ static int do_case(or_engine* e, or_real* r, int p, unsigned ts)
     }
     if (DETECTED(r, p)) {
         r->cnt.cumDC++;
+        if (e->NAU) r->ad_cnt[(size_t)e->madunit[e->mcell[p]] * CS_TSU_FIELDS + CS_TSU_INC_DC]++; /* cumDC_adunit (src/Update.cpp:872) */
         /* DoDetectedCase (src/Update.cpp:525-560): microcell trigger counts */
         uint32_t* tg = r->m_trig + (size_t)e->mcell[p] * 6;
         for (int q = 0; q < 5; q++) tg[q]++;
@@ -1470,6 +1490,7 @@ static int do_case(or_engine* e, or_real* r, int p, unsigned ts)
     if (host_treated(e, r, p, ts)) r->cumTC[e->pc[p]]++;
     r->cnt.cumC++;
     r->cnt.cumCa[ag]++;
+    if (e->NAU) r->ad_cnt[(size_t)e->madunit[e->mcell[p]] * CS_TSU_FIELDS + CS_TSU_INC_C]++; /* cumC_adunit (src/Update.cpp:892) */
     r->cnt.cumC_keyworker[KEYW(e, p)]++;
     if (P->do_severity) {
         if (r->sev_final[p] == CS_SEV_MILD) {
@@ -1542,6 +1563,7 @@ static int infectious_timers(or_engine* e, or_real* r, int p, unsigned ts, int* 
             } else { /* DoDeath (src/Update.cpp:953-998) */
                 r->inf[p] = (r->inf[p] == CS_CASE) ? CS_DEAD_SYMP : CS_DEAD_ASYMP;
                 r->cnt.cumDa[ag]++;
+                if (e->NAU) r->ad_cnt[(size_t)e->madunit[e->mcell[p]] * CS_TSU_FIELDS + CS_TSU_INC_DEATH]++; /* cumD_adunit (src/Update.cpp:976) */
             }
             leave = 1;
             e->wc.n_transition++;
@@ -2036,6 +2058,7 @@ int or_reset_realisations(or_engine* e, int n_seed)
         memset(r->m_moverest, 0, (size_t)e->NM);
         memset(r->m_trig, 0, (size_t)e->NM * 6 * 4);
         memset(&r->cnt, 0, sizeof(r->cnt));
+        memset(r->ad_cnt, 0, (size_t)(e->NAU ? e->NAU : 1) * CS_TSU_FIELDS * 8);
         /* intervention state (InitModel, src/CovidSim.cpp:1104, :1116-1122, :1216-1232) */
         r->alert = 0;
         r->trig = 0;
@@ -2157,6 +2180,22 @@ static void totals(const or_engine* e, const or_real* r, int64_t* S, int64_t* L,
 /* RunModel's timestep loop (src/CovidSim.cpp:1426-1522) for the two sweeps,
  * with the S-drop UpdateProbs check; sps = timesteps per output step (the
  * sampling interval, NumModelTimeStepsPerOutputTimeStep) */
+/* the cumulative counters a sample differences by age group (infections,
+ * cases, deaths, the 8 severity kinds) and by unit (cumI, cumC, cumDC,
+ * cumD), field-major: [field][age] then [field][unit] */
+static void ext_now(const or_engine* e, const or_real* r, int64_t* out)
+{
+    for (int a = 0; a < CS_NUM_AGE_GROUPS; a++) {
+        out[CS_TSA_INC_I * CS_NUM_AGE_GROUPS + a] = r->cnt.cumIa[a];
+        out[CS_TSA_INC_C * CS_NUM_AGE_GROUPS + a] = r->cnt.cumCa[a];
+        out[CS_TSA_INC_DEATH * CS_NUM_AGE_GROUPS + a] = r->cnt.cumDa[a];
+        for (int k = 0; k < CS_NUM_SEV_KINDS; k++) out[(CS_TSA_INC_SEV + k) * CS_NUM_AGE_GROUPS + a] = r->cnt.sev_cum_age[k][a];
+    }
+    for (int f = 0; f < CS_TSU_FIELDS; f++)
+        for (int u = 0; u < e->NAU; u++)
+            out[CS_TSA_CUM * CS_NUM_AGE_GROUPS + f * e->NAU + u] = r->ad_cnt[(size_t)u * CS_TSU_FIELDS + f];
+}
+
 int or_run_steps(or_engine* e, double t0, int n_steps, int sps)
 {
     if (sps <= 0) return fail(e, CS_EINVAL, "steps per sample must be positive");
@@ -2166,6 +2205,8 @@ int or_run_steps(or_engine* e, double t0, int n_steps, int sps)
         /* a new series: RunModel samples before its first timestep (src/CovidSim.cpp:1413) */
         free(e->ts_last);
         e->ts_last = (int64_t*)calloc((size_t)e->nreal * CS_TS_LAST, 8);
+        free(e->ext_last);
+        e->ext_last = (int64_t*)calloc((size_t)e->nreal * (CS_TSA_CUM * CS_NUM_AGE_GROUPS + CS_TSU_FIELDS * e->NAU), 8);
         e->ts_n = 0;
         for (int ri = 0; ri < e->nreal; ri++) {
             int64_t S, L, I, R, D;
@@ -2173,6 +2214,7 @@ int or_run_steps(or_engine* e, double t0, int n_steps, int sps)
             int64_t* lt = e->ts_last + (size_t)CS_TS_LAST * ri;
             lt[0] = e->R[ri].cnt.cumI; lt[1] = e->R[ri].cnt.cumC; lt[2] = D;
             for (int k = 0; k < CS_NUM_SEV_KINDS; k++) lt[3 + k] = e->R[ri].cnt.sev_cum[k];
+            ext_now(e, &e->R[ri], e->ext_last + (size_t)ri * (CS_TSA_CUM * CS_NUM_AGE_GROUPS + CS_TSU_FIELDS * e->NAU));
             e->R[ri].samp_prop_susc = (double)S / (double)e->N;
             /* the first sample's CalibrationThresholdCheck (src/CovidSim.cpp:1414) */
             if (e->P.interventions_active) itv_triggers(e, &e->R[ri], 0, t0);
@@ -2188,11 +2230,17 @@ int or_run_steps(or_engine* e, double t0, int n_steps, int sps)
         int cap = e->ts_n + ns > 2 * e->ts_cap ? e->ts_n + ns : 2 * e->ts_cap;
         if (cap < 1) cap = 1;
         int32_t* nb = (int32_t*)calloc((size_t)e->nreal * (size_t)cap * CS_TS_FIELDS, 4);
-        for (int ri = 0; ri < e->nreal && e->ts_buf; ri++)
+        int32_t* nx = (int32_t*)calloc((size_t)e->nreal * (size_t)cap * e->ext_w, 4);
+        for (int ri = 0; ri < e->nreal && e->ts_buf; ri++) {
             memcpy(nb + (size_t)ri * cap * CS_TS_FIELDS, e->ts_buf + (size_t)ri * e->ts_cap * CS_TS_FIELDS,
                    (size_t)e->ts_n * CS_TS_FIELDS * 4);
+            memcpy(nx + (size_t)ri * cap * e->ext_w, e->ts_ext + (size_t)ri * e->ts_cap * e->ext_w,
+                   (size_t)e->ts_n * e->ext_w * 4);
+        }
         free(e->ts_buf);
+        free(e->ts_ext);
         e->ts_buf = nb;
+        e->ts_ext = nx;
         e->ts_cap = cap;
     }
     for (int st = 0; st < n_steps; st++) {
@@ -2236,6 +2284,23 @@ int or_run_steps(or_engine* e, double t0, int n_steps, int sps)
                 for (int k = 0; k < CS_NUM_SEV_KINDS; k++) o[CS_TS_INC_SEV + k] = (int32_t)(now[3 + k] - lt[3 + k]);
                 for (int k = 0; k <= CS_SK_CRITRECOV; k++) o[CS_TS_PREV_SEV + k] = (int32_t)r->cnt.sev_prev[k];
                 memcpy(lt, now, sizeof(now));
+                /* by age group and by administrative unit (RecordSample's
+                 * incIa/incCa/incDa, severity by age and *_adunit fields) */
+                {
+                    const int nc = CS_TSA_CUM * CS_NUM_AGE_GROUPS + CS_TSU_FIELDS * e->NAU;
+                    int64_t* le = e->ext_last + (size_t)ri * nc;
+                    int64_t* cur = (int64_t*)malloc((size_t)nc * 8);
+                    ext_now(e, r, cur);
+                    int32_t* ox = e->ts_ext + ((size_t)ri * e->ts_cap + e->ts_n) * e->ext_w;
+                    for (int q = 0; q < CS_TSA_CUM * CS_NUM_AGE_GROUPS; q++) ox[q] = (int32_t)(cur[q] - le[q]);
+                    for (int k = 0; k <= CS_SK_CRITRECOV; k++)
+                        for (int a = 0; a < CS_NUM_AGE_GROUPS; a++)
+                            ox[(CS_TSA_PREV_SEV + k) * CS_NUM_AGE_GROUPS + a] = (int32_t)r->cnt.sev_prev_age[k][a];
+                    for (int q = 0; q < CS_TSU_FIELDS * e->NAU; q++)
+                        ox[CS_TSA_FIELDS * CS_NUM_AGE_GROUPS + q] = (int32_t)(cur[CS_TSA_CUM * CS_NUM_AGE_GROUPS + q] - le[CS_TSA_CUM * CS_NUM_AGE_GROUPS + q]);
+                    memcpy(le, cur, (size_t)nc * 8);
+                    free(cur);
+                }
             }
             e->ts_n++;
             /* the next output step's RecordSample and CalibrationThresholdCheck
@@ -2255,6 +2320,39 @@ int or_get_timeseries(or_engine* e, int32_t* out, int max_samples, int* n_sample
         memcpy(out + (size_t)ri * max_samples * CS_TS_FIELDS, e->ts_buf + (size_t)ri * e->ts_cap * CS_TS_FIELDS,
                (size_t)n * CS_TS_FIELDS * 4);
     *n_samples = n;
+    return 0;
+}
+
+int or_get_timeseries_age(or_engine* e, int32_t* out, int max_samples, int* n_samples)
+{
+    const int n = e->ts_n < max_samples ? e->ts_n : max_samples;
+    const size_t W = CS_TSA_FIELDS * CS_NUM_AGE_GROUPS;
+    for (int ri = 0; ri < e->nreal; ri++)
+        for (int k = 0; k < n; k++)
+            memcpy(out + ((size_t)ri * max_samples + k) * W, e->ts_ext + ((size_t)ri * e->ts_cap + k) * e->ext_w, W * 4);
+    *n_samples = n;
+    return 0;
+}
+
+int or_get_timeseries_adunit(or_engine* e, int32_t* out, int max_samples, int* n_samples)
+{
+    if (!e->NAU) return fail(e, CS_ESTATE, "no series by administrative unit");
+    const int n = e->ts_n < max_samples ? e->ts_n : max_samples;
+    const size_t W = (size_t)CS_TSU_FIELDS * e->NAU;
+    for (int ri = 0; ri < e->nreal; ri++)
+        for (int k = 0; k < n; k++)
+            memcpy(out + ((size_t)ri * max_samples + k) * W,
+                   e->ts_ext + ((size_t)ri * e->ts_cap + k) * e->ext_w + CS_TSA_FIELDS * CS_NUM_AGE_GROUPS, W * 4);
+    *n_samples = n;
+    return 0;
+}
+
+int or_get_adunit_counters(or_engine* e, int ri, int64_t* out, int reset)
+{
+    if (!e->NAU) return fail(e, CS_ESTATE, "no counters by administrative unit");
+    if (ri < 0 || ri >= e->nreal) return fail(e, CS_EINVAL, "bad realisation");
+    memcpy(out, e->R[ri].ad_cnt, (size_t)e->NAU * CS_TSU_FIELDS * 8);
+    if (reset) memset(e->R[ri].ad_cnt, 0, (size_t)e->NAU * CS_TSU_FIELDS * 8);
     return 0;
 }
 

@@ -39,6 +39,9 @@ int  or_update_probs(or_engine* e);
 int  or_run_steps(or_engine* e, double t0, int n_steps, int steps_per_sample);
 int  or_get_counters(or_engine* e, int real, cs_counters* out, int reset);
 int  or_get_timeseries(or_engine* e, int32_t* out, int max_samples, int* n_samples);
+int  or_get_timeseries_age(or_engine* e, int32_t* out, int max_samples, int* n_samples);
+int  or_get_timeseries_adunit(or_engine* e, int32_t* out, int max_samples, int* n_samples);
+int  or_get_adunit_counters(or_engine* e, int real, int64_t* out, int reset);
 int  or_get_work_counts(or_engine* e, cs_work_counts* out, int reset);
 /* the intervention subset: NULL when p lies inside it, else the reason */
 const char* or_interventions_unsupported(const cs_params* p);

@@ -337,8 +337,34 @@ def setup():
     print("wrote popgen_params.bin (c1, mini), c1_setup.npz, setup_files.json")
 
 
+def outputs():
+    """ref_outputs.json: the header lines and the age-distribution row of
+    the ensemble files the reference writes for the mini inputs (2
+    realisations, age and administrative-unit outputs on): what the
+    library's summary writers must reproduce for the same population"""
+    import json
+    inputs = os.path.join(ROOT, "covid-sim_amd", "inputs", "mini")
+    tmp = tempfile.mkdtemp()
+    try:
+        pre = override_params(os.path.join(inputs, "pre-params.txt"),
+                              {"Output incidence by administrative unit": "1", "OutputAge": "1"},
+                              os.path.join(tmp, "pre.txt"))
+        run_harness(inputs, tmp, steps="", golden=False, threads=1, nr=2, pre_params=pre)
+        out = {}
+        for suf in ("age", "adunit"):
+            lines = open(os.path.join(tmp, "out.avNE.%s.xls" % suf)).read().split("\n")
+            out[suf] = {"header": lines[0], "rows": len([l for l in lines[1:] if l]), "last": lines[-2]}
+        with open(os.path.join(HERE, "ref_outputs.json"), "w") as f:
+            json.dump(out, f, indent=1)
+        print("wrote ref_outputs.json")
+    finally:
+        shutil.rmtree(tmp)
+
+
 if __name__ == "__main__":
-    if len(sys.argv) > 1 and sys.argv[1] == "setup":
+    if len(sys.argv) > 1 and sys.argv[1] == "outputs":
+        outputs()
+    elif len(sys.argv) > 1 and sys.argv[1] == "setup":
         setup()
     elif len(sys.argv) > 1 and sys.argv[1] == "interventions":
         interventions()

@@ -1,0 +1,78 @@
+"""The age and administrative-unit summary files (.age.xls, .adunit.xls of
+SaveSummaryResults, src/CovidSim.cpp:2035-2090), written from the oracle's
+series for the mini region, against the files the reference writes for the
+same inputs (tests/golden/ref_outputs.json: header lines, row counts, and the
+age file's `dist` row, which depends on the population alone).  The values
+of the sample rows are the ensemble means of series pinned elsewhere
+(test_series_by_age_unit.py, test_gpu_series.py); here they are checked
+against the series they were written from."""
+import importlib
+import json
+import os
+
+import numpy as np
+
+from fixtures import GOLDEN, mini
+from oracle_lib import OracleEngine
+
+DAYS = 120
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_age_and_adunit_files_match_reference_layout(csa, tmp_path):
+    abi = csa.abi
+    summary = importlib.import_module("covid-sim_amd.summary")
+    sm = importlib.import_module("covid-sim_amd.setup_model")
+    ref = json.load(open(os.path.join(GOLDEN, "ref_outputs.json")))
+    arrays, pop, params, _ = mini()
+    units = sm.read_admin_units(os.path.join(ROOT, "covid-sim_amd", "inputs", "mini", "admin-params.txt"))
+    names = [u[2] for u in units]
+    na = len(names)
+    assert na == pop.n_adunits == 2
+    seeds = np.array([5, 6], np.uint64)
+    o = OracleEngine(params, pop)
+    o.alloc(seeds)
+    o.reset(10)
+    start_age = np.zeros((2, abi.CS_TSA_FIELDS, abi.CS_NUM_AGE_GROUPS), np.int64)
+    start_unit = np.zeros((2, abi.CS_TSU_FIELDS, na), np.int64)
+    for r in range(2):
+        c = o.counters(r)
+        start_age[r, abi.CS_TSA_INC_I] = np.array(c.cumIa)
+        start_age[r, abi.CS_TSA_INC_C] = np.array(c.cumCa)
+        start_unit[r] = o.adunit_counters(r, na).T
+    tspd = int(params.time_steps_per_day)
+    o.run_steps(0.0, DAYS * tspd, tspd)
+    ta = o.timeseries_age(DAYS)
+    tu = o.timeseries_adunit(DAYS, na)
+
+    unit_n = np.bincount(pop.a["m_adunit"][pop.a["p_mcell"]], minlength=na)
+    p_age = str(tmp_path / "out.age.xls")
+    p_unit = str(tmp_path / "out.adunit.xls")
+    summary.write_age(p_age, ta, pop.a["p_age"], start=start_age)
+    summary.write_adunit(p_unit, tu, names, unit_n, start=start_unit)
+
+    lines = open(p_age).read().split("\n")
+    assert lines[0] == ref["age"]["header"]
+    assert len([l for l in lines[1:] if l]) == int(ref["age"]["rows"])
+    assert lines[-2] == ref["age"]["last"]
+    lines = open(p_unit).read().split("\n")
+    assert lines[0] == ref["adunit"]["header"]
+    assert len([l for l in lines[1:] if l]) == int(ref["adunit"]["rows"])
+
+    # rows: t, then the ensemble means of the series they were written from
+    v = np.loadtxt(p_age, skiprows=1, max_rows=DAYS + 1)
+    assert np.array_equal(v[:, 0], np.arange(DAYS + 1))
+    g = abi.CS_NUM_AGE_GROUPS
+    assert np.allclose(v[0, 1:1 + g], start_age[:, abi.CS_TSA_INC_I].mean(axis=0))
+    assert v[0, 1:1 + g].sum() == 10
+    assert np.allclose(v[1:, 1:1 + g], ta[:, :, abi.CS_TSA_INC_I].mean(axis=0), atol=1e-10)
+    assert np.allclose(v[1:, 1 + g:1 + 2 * g], ta[:, :, abi.CS_TSA_INC_C].mean(axis=0), atol=1e-10)
+    assert np.allclose(v[1:, 1 + 2 * g:], ta[:, :, abi.CS_TSA_INC_DEATH].mean(axis=0), atol=1e-10)
+    u = np.loadtxt(p_unit, skiprows=1)
+    assert u.shape == (DAYS + 1, 1 + 4 * na)
+    for k, f in enumerate((abi.CS_TSU_INC_I, abi.CS_TSU_INC_C, abi.CS_TSU_INC_DC)):
+        assert np.allclose(u[1:, 1 + k * na:1 + (k + 1) * na], tu[:, :, f].mean(axis=0), atol=1e-10)
+    assert not u[:, 1 + 3 * na:].any()
+    # the infections over the whole file, seeds included, are the persons
+    # no longer susceptible at the end (every person started susceptible)
+    assert u[:, 1:1 + na].sum() * 2 == sum(o.adunit_counters(r, na)[:, abi.CS_TSU_INC_I].sum() for r in range(2))

@@ -24,8 +24,9 @@
 // convention for fatal errors.
 //
 // Scope: the no-intervention path; the library refuses parameter sets that
-// switch interventions on.  Per-administrative-unit counters are not
-// produced by the library and stay zero.
+// switch interventions on.  The counters by administrative unit (cumI, cumC,
+// cumDC, cumD per unit) are added to StateT[0] beside the others when the
+// parameters keep them (P.DoAdUnits with units known).
 #include <climits>
 #include <cmath>
 #include <cstdio>
@@ -65,6 +66,8 @@ struct Shim {
     cs_bridge::PopulationView pop;
     cs_bridge::StateView st;
     std::vector<int32_t> S0;  // cell S0 the library's tables were built from
+    std::vector<int64_t> adc; // counters by administrative unit, [NumAdunits][CS_TSU_FIELDS]
+    bool ad = false;          // the library keeps them
 };
 Shim g;
 
@@ -107,6 +110,8 @@ void push_state(int run)
     check(cs_load_state(g.eng, 0, &v), "cs_load_state");
     cs_counters c;
     check(cs_get_counters(g.eng, 0, &c, 1), "cs_get_counters");
+    g.adc.assign((size_t)P.NumAdunits * CS_TSU_FIELDS, 0);
+    g.ad = P.DoAdUnits && P.NumAdunits > 0 && cs_get_adunit_counters(g.eng, 0, g.adc.data(), 1) == 0;
     g.S0.resize(P.NumPopulatedCells);
     for (int j = 0; j < P.NumPopulatedCells; j++) g.S0[j] = CellLookup[j]->S0;
 }
@@ -149,6 +154,15 @@ void pull_state()
     cs_counters d;
     check(cs_get_counters(g.eng, 0, &d, 1), "cs_get_counters");
     add_counters(d);
+    if (g.ad) {
+        check(cs_get_adunit_counters(g.eng, 0, g.adc.data(), 1), "cs_get_adunit_counters");
+        PopVar& z = StateT[0];
+        for (int i = 0; i < P.NumAdunits; i++) {
+            const int64_t* a = &g.adc[(size_t)i * CS_TSU_FIELDS];
+            z.cumI_adunit[i] += (int)a[CS_TSU_INC_I]; z.cumC_adunit[i] += (int)a[CS_TSU_INC_C];
+            z.cumDC_adunit[i] += (int)a[CS_TSU_INC_DC]; z.cumD_adunit[i] += (int)a[CS_TSU_INC_DEATH];
+        }
+    }
 }
 
 }  // namespace

@@ -36,9 +36,9 @@ SEEDS = ["98798150", "729101", "17389101", "4797132"]
 PROGRESS = re.compile(rb"t=([0-9.e+-]+)\s+(-?\d+)\s+(-?\d+)\|(-?\d+)\s+(-?\d+)\s+(-?\d+) \[=(\d+)\]")
 
 
-def run(exe, out, nr, env_extra=None, timeout=900):
+def run(exe, out, nr, env_extra=None, timeout=900, pre=None):
     cmd = [exe, "/c:1", "/NR:%d" % nr,
-           "/PP:" + os.path.join(INPUTS, "pre-params.txt"), "/P:" + os.path.join(INPUTS, "input-noint-params.txt"),
+           "/PP:" + (pre or os.path.join(INPUTS, "pre-params.txt")), "/P:" + os.path.join(INPUTS, "input-noint-params.txt"),
            "/O:" + os.path.join(out, "out"), "/D:" + os.path.join(INPUTS, "density.txt"),
            "/A:" + os.path.join(INPUTS, "admin-params.txt")] + SEEDS
     env = dict(os.environ, **(env_extra or {}))
@@ -92,3 +92,28 @@ def test_dropin_gpu_ensemble_matches_reference():
     hb = z * ref_attack.std(ddof=1) / np.sqrt(len(ref_attack))
     assert abs(gpu_attack.mean() - ref_attack.mean()) <= ha + hb, (gpu_attack.mean(), ref_attack.mean())
     assert len(set(gpu_attack)) > 1  # realisations differ (per-run stream seeds)
+
+
+@pytest.mark.gpu
+def test_dropin_gpu_adunit_and_age_files():
+    """The reference's .adunit.xls and .age.xls written from a GPU run: the
+    counters by administrative unit (cs_get_adunit_counters) and by age
+    (cs_get_counters) come from separate device counters, and per sample
+    their sums over units and over age groups agree, for infections and
+    cases; the infections go beyond the seeds."""
+    need(DROPIN)
+    out = tempfile.mkdtemp()
+    pre = os.path.join(out, "pre.txt")
+    text = open(os.path.join(INPUTS, "pre-params.txt")).read()
+    key = "[Output incidence by administrative unit]\n0"
+    assert key in text
+    open(pre, "w").write(text.replace(key, key[:-1] + "1"))
+    run(DROPIN, out, 2, pre=pre)
+    unit = np.loadtxt(os.path.join(out, "out.avNE.adunit.xls"), skiprows=1, ndmin=2)
+    age = np.loadtxt(os.path.join(out, "out.avNE.age.xls"), skiprows=1, ndmin=2, max_rows=len(unit))
+    na = (unit.shape[1] - 1) // 4
+    g = (age.shape[1] - 1) // 3
+    assert np.array_equal(unit[:, 0], age[:, 0])
+    assert np.allclose(unit[:, 1:1 + na].sum(axis=1), age[:, 1:1 + g].sum(axis=1))
+    assert np.allclose(unit[:, 1 + na:1 + 2 * na].sum(axis=1), age[:, 1 + g:1 + 2 * g].sum(axis=1))
+    assert unit[1:, 1:1 + na].sum() > 100

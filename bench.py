@@ -205,6 +205,8 @@ def main():
     ap.add_argument("--calibration", action="store_true", help="measure emulator-guided vs plain calibration instead")
     ap.add_argument("--profile-kinds", action="store_true", help="with --config large: time every launch by kind "
                     "(HIP events, direct launches)")
+    ap.add_argument("--dest-stride", type=int, default=-1, help="destination tables (headline and large), 0 full, "
+                    "> 0 checkpoints every so many cells, -1 (default) chosen by the table memory")
     ap.add_argument("--cells", type=int, nargs=2, default=[668, 336], metavar=("WIDE", "HIGH"),
                     help="with --config large: the region's cells (0.075 degrees each)")
     ap.add_argument("--config", choices=["headline", "c3", "large"], default="headline",
@@ -258,6 +260,8 @@ def main():
     eng.set_params(params)
     eng.set_population(pop)
     seeds = np.arange(1, args.realisations + 1, dtype=np.uint64) * np.uint64(0x9E3779B97F4A7C15)
+    if args.dest_stride >= 0:
+        eng.set_dest_tables(args.dest_stride)
     eng.alloc(seeds)
     if args.no_graphs:
         eng.set_step_graphs(False)
@@ -356,6 +360,7 @@ def main():
                                "10 seeds, no interventions" % (args.realisations, args.people, args.days),
                    "realisations": args.realisations, "persons": args.people, "days": args.days, "cells": pop.n_cells,
                    "mean_attack_rate": round(attack, 4), "launches": "direct launches" if args.no_graphs else "replayed step graphs",
+                   "dest_tables": "full" if eng.dest_tables() == 0 else "checkpoints every %d cells" % eng.dest_tables(),
                    "all_kernel_algorithmic_gbs": round(total_alg / t_prof / 1e9, 2)},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
@@ -543,6 +548,8 @@ def large_region(args):
     eng.set_params(kp)
     eng.set_population(pop)
     c = eng.transmission_coeffs(5)
+    if args.dest_stride >= 0:
+        eng.set_dest_tables(args.dest_stride)
     eng.alloc(np.array([0x9E3779B97F4A7C15], np.uint64))
     stride = eng.dest_tables()
     t_setup = time.perf_counter() - t1

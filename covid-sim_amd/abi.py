@@ -123,6 +123,8 @@ def declare(lib, prefix):
         "set_dest_tables": (I, [vp, I]),
         "get_dest_tables": (I, [vp, P(I)]),
         "set_launch_shape": (I, [vp, C.c_uint32, C.c_uint32]),
+        "set_move_capacity": (I, [vp, C.c_int]),
+        "set_dest_index": (I, [vp, C.c_int]),
         "autotune": (I, [vp, D, I, P(cs_tuning)]),
         "set_queue_capacity": (I, [vp, C.c_uint32, C.c_uint32]),
         "get_queue_capacity": (I, [vp, P(C.c_uint32), P(C.c_uint32)]),

@@ -23,6 +23,9 @@ using cs::Dev;
 // CS_CK_TABLE_BYTES (a destination search redoes at most a stride of terms)
 #define CS_FULL_TABLE_BYTES (16ull << 30)
 #define CS_CK_TABLE_BYTES (16ull << 30)
+// the rows' bucket index of checkpointed tables (R x NC x 1025 block
+// numbers) is kept while it fits this
+#define CS_CK_INDEX_BYTES (4ull << 30)
 
 struct cs_engine {
     char err[512];
@@ -34,6 +37,8 @@ struct cs_engine {
     cs_params* dP = nullptr;
     double *dkl = nullptr, *dkh = nullptr;
     int N = 0, NH = 0, NC = 0, NCT = 0, NM = 0, NA = 0, NPT = 0, R = 0;
+    int max_cn = 0;                  // the largest cell's persons
+    int move_cap = CS_MOVE_CAP;      // Dev.move_cap (cs_set_move_capacity)
     int np[CS_MAX_PLACE_TYPES] = {0};
     std::vector<void*> pop_allocs, real_allocs;
     Dev d;
@@ -43,6 +48,7 @@ struct cs_engine {
     int32_t* rseed_buf = nullptr;    // [R] storage behind Dev.rseed_n
     double* rthr_buf = nullptr;      // [R] storage behind Dev.rthr
     int dest_stride = -1;            // destination tables asked for: -1 chosen at allocation, 0 full, > 0 checkpoint stride
+    bool dest_index = true;          // checkpointed tables keep the rows' bucket index (Dev.ck_inv) when it fits
     int cxy_nch = 0;                 // the nch Dev.cxy was built with
     float* mt_buf = nullptr;         // [NC][NC] max_trans (full tables only; built when first needed)
     int ts_max = 0, ts_n = 0;
@@ -226,14 +232,23 @@ int check_flags(cs_engine* e)
     HIPCHK(e, hipStreamSynchronize(e->stream));
     if (f & cs::FLAG_RANGE) return set_err(e, CS_ERANGE, "r too large in NumKernel (distance beyond the kernel table)");
     if (f & cs::FLAG_OVERFLOW) return set_err(e, CS_EOVERFLOW, "infection event queue overflow (capacity %u)", e->d.ev_cap);
-    if (f & cs::FLAG_BATCH) return set_err(e, CS_EUNSUPPORTED, "more than %d list moves in one cell in one step", CS_MOVE_CAP);
+    if (f & cs::FLAG_BATCH) return set_err(e, CS_ESTATE, "a batched list move found a list position outside its cell's segment");
     if (f & cs::FLAG_STUCK) return set_err(e, CS_ERANGE, "spatial draw found no live candidate in %d attempts", CS_MAX_ATTEMPTS);
     if (f & cs::FLAG_TRIG) return set_err(e, CS_EUNSUPPORTED, "a trigger window reaches back more than %d samples", CS_TRIG_RING - 1);
     if (f & cs::FLAG_LIST) return set_err(e, CS_EOVERFLOW, "intervention lists full (detected cases %u, closing places %u)", e->d.det_cap, e->d.close_cap);
     return 0;
 }
 
+// the batched list moves of one mode: cells within the LDS arrays' reach in
+// k_list_move, the cells it queues in k_list_move_big
+void launch_list_move(cs_engine* e, hipStream_t st, int mode)
+{
+    cs::k_list_move<<<e->grid_moves, 256, 0, st>>>(e->d, mode);
+    cs::k_list_move_big<<<CS_MOVE_BIG_BLOCKS, 256, 0, st>>>(e->d, mode);
+}
+
 // sweep kernels that append events; state is untouched until resolution
+
 int launch_infect_events(cs_engine* e)
 {
     Dev& d = e->d;
@@ -274,7 +289,7 @@ int launch_resolve(cs_engine* e)
         cs::k_resolve_reset<<<e->grid_waves, 256, 0, st>>>(d);
         cs::k_cnt_fold<<<dim3(d.R, (cs::CS_CNT_WORDS + 63) / 64), 64, 0, st>>>(d);
     }
-    { ProfScope ps(e, CS_K_LIST_MOVE); cs::k_list_move<<<e->grid_moves, 256, 0, st>>>(d, 0); }
+    { ProfScope ps(e, CS_K_LIST_MOVE); launch_list_move(e, st, 0); }
     HIPCHK(e, hipGetLastError());
     return 0;
 }
@@ -301,7 +316,7 @@ int launch_incub(cs_engine* e)
         cs::k_latent_scan<<<e->grid_waves, 256, 0, st>>>(d);
         cs::k_due_clear<<<1, 64, 0, st>>>(d);
     }
-    { ProfScope ps(e, CS_K_LIST_MOVE); cs::k_list_move<<<e->grid_moves, 256, 0, st>>>(d, 1); }
+    { ProfScope ps(e, CS_K_LIST_MOVE); launch_list_move(e, st, 1); }
     {
         ProfScope ps(e, CS_K_OTHER);
         cs::k_zero_counts<<<1, 64, 0, st>>>(d.care_count, 1);
@@ -320,7 +335,7 @@ int launch_incub(cs_engine* e)
         cs::k_care_eval<<<256, 256, 0, st>>>(d);
         cs::k_care_apply<<<256, 256, 0, st>>>(d);
     }
-    { ProfScope ps(e, CS_K_LIST_MOVE); cs::k_list_move<<<e->grid_moves, 256, 0, st>>>(d, 2); }
+    { ProfScope ps(e, CS_K_LIST_MOVE); launch_list_move(e, st, 2); }
     { ProfScope ps(e, CS_K_OTHER); cs::k_cnt_fold<<<dim3(d.R, (cs::CS_CNT_WORDS + 63) / 64), 64, 0, st>>>(d); }
     HIPCHK(e, hipGetLastError());
     return 0;
@@ -349,7 +364,10 @@ int launch_update_probs(cs_engine* e, const int* sel)
     long long n = (long long)d.R * d.NC;
     ProfScope ps(e, CS_K_UPDATE_PROBS);
     cs::k_update_s0<<<grid_for(n, 256), 256, 0, e->stream>>>(d, sel);
-    if (d.ck) cs::k_update_rows_ck<<<grid_for(n, 64), 64, 0, e->stream>>>(d, sel);
+    if (d.ck) {
+        cs::k_update_rows_ck<<<grid_for(n, 64), 64, 0, e->stream>>>(d, sel);
+        if (d.ck_inv) cs::k_ck_index<<<grid_for(n * 64, 256), 256, 0, e->stream>>>(d, sel);
+    }
     else cs::k_update_rows<<<grid_for(n, 64), 64, 0, e->stream>>>(d, sel);
     HIPCHK(e, hipGetLastError());
     return 0;
@@ -743,6 +761,8 @@ int cs_set_population(cs_engine* e, const cs_population* pop)
     }
     if ((rc = dupload(e, L, &d.cstart, pop->cell_member_start, NC + 1))) return rc;
     if ((rc = dupload(e, L, &d.cn, pop->cell_n, NC))) return rc;
+    e->max_cn = 0;
+    for (int j = 0; j < NC; j++) e->max_cn = std::max(e->max_cn, (int)pop->cell_n[j]);
     if ((rc = dupload(e, L, &d.cidx, pop->cell_index, NC))) return rc;
     {   // grid coordinates of the populated cells (dist2_cc_min without divisions)
         std::vector<int2> xy(NC);
@@ -868,13 +888,17 @@ int cs_alloc_realisations(cs_engine* e, int n_real, const uint64_t* seeds)
     {   // destination tables: full NC x NC rows while they fit, else checkpoints
         int stride = e->dest_stride;
         if (stride < 0 && (R + 1) * NC * NC * 4 <= CS_FULL_TABLE_BYTES) stride = 0;
+        const double inv_bytes = (double)RC * 1025 * 4;
+        const bool index = e->dest_index && inv_bytes <= (double)CS_CK_INDEX_BYTES;
         if (stride < 0)
-            for (stride = 8; (double)R * NC * NC * 4 / stride > (double)CS_CK_TABLE_BYTES; stride *= 2) {}
+            for (stride = 8; (double)R * NC * NC * 4 / stride + (index ? inv_bytes : 0.0) > (double)CS_CK_TABLE_BYTES;
+                 stride *= 2) {}
         d.ck = stride;
         d.ck_nb = stride ? (int)((NC + stride - 1) / stride) : 0;
-        d.cum_trans = nullptr; d.invcdf = nullptr; d.ck_pre = nullptr;
+        d.cum_trans = nullptr; d.invcdf = nullptr; d.ck_pre = nullptr; d.ck_inv = nullptr;
         if (stride) {
             AL(ck_pre, RC * d.ck_nb);
+            if (index) AL(ck_inv, RC * 1025);
         } else {
             AL(cum_trans, RC * NC); AL(invcdf, RC * 1025);
             if ((rc = ensure_max_trans(e))) return rc;
@@ -896,6 +920,9 @@ int cs_alloc_realisations(cs_engine* e, int n_real, const uint64_t* seeds)
     d.pcl = nullptr;
     d.pc_cap = 0;
     AL(clr, RN); AL(batch, RN); AL(bcount, RC);
+    d.mv_words = (e->max_cn + 31) / 32 + 1;
+    AL(mv_over, RC); AL(mv_over_n, 2); AL(mv_bm, (size_t)CS_MOVE_BIG_BLOCKS * 2 * d.mv_words);
+    d.move_cap = e->move_cap;
     d.bind_clr(d.clr);
     AL(care_gp, RN); AL(care_k, RN); AL(care_abs, RN); AL(care_count, 1);
     AL(last_tot, CS_TS_LAST * R);
@@ -915,6 +942,7 @@ int cs_alloc_realisations(cs_engine* e, int n_real, const uint64_t* seeds)
     cs::k_claim_init<<<grid_for((long long)RN, 256), 256, 0, e->stream>>>(d.clr, RN);
     HIPCHK(e, hipGetLastError());
     HIPCHK(e, hipMemset(d.bcount, 0, RC * 4));
+    HIPCHK(e, hipMemset(d.mv_over_n, 0, 2 * 4));
     HIPCHK(e, hipMemset(d.wl, 0, cs::WL_N * 4));
     HIPCHK(e, hipMemset(d.cnt, 0, R * sizeof(cs::Counters)));
     HIPCHK(e, hipMemset(d.cstage, 0, (size_t)CS_TALLY_COPIES * R * sizeof(cs::Counters)));
@@ -924,8 +952,8 @@ int cs_alloc_realisations(cs_engine* e, int n_real, const uint64_t* seeds)
         d.ad_cnt = nullptr;
         if (units) {
             unsigned long long* ac;
-            if ((rc = dalloc(e, A, &ac, R * (size_t)d.NA * CS_TSU_FIELDS))) return rc;
-            HIPCHK(e, hipMemset(ac, 0, R * (size_t)d.NA * CS_TSU_FIELDS * 8));
+            if ((rc = dalloc(e, A, &ac, (size_t)CS_AD_COPIES * R * d.NA * CS_TSU_FIELDS))) return rc;
+            HIPCHK(e, hipMemset(ac, 0, (size_t)CS_AD_COPIES * R * d.NA * CS_TSU_FIELDS * 8));
             d.ad_cnt = ac;
         }
         d.ext_w = CS_TSA_FIELDS * CS_NUM_AGE_GROUPS + CS_TSU_FIELDS * d.NA;
@@ -988,7 +1016,7 @@ int cs_reset_realisations(cs_engine* e, int n_seed)
     hipStream_t st = e->stream;
     e->ts_fresh = true;
     HIPCHK(e, hipMemsetAsync(d.cnt, 0, (size_t)d.R * sizeof(cs::Counters), st));
-    if (d.ad_cnt) HIPCHK(e, hipMemsetAsync(d.ad_cnt, 0, (size_t)d.R * d.NA * CS_TSU_FIELDS * 8, st));
+    if (d.ad_cnt) HIPCHK(e, hipMemsetAsync(d.ad_cnt, 0, (size_t)CS_AD_COPIES * d.R * d.NA * CS_TSU_FIELDS * 8, st));
     HIPCHK(e, hipMemsetAsync(d.due_n, 0, CS_DUE_RING * CS_DUE_SHARDS * CS_DUE_STRIDE * sizeof(unsigned), st));
     HIPCHK(e, hipMemsetAsync(d.due_ovf, 0, CS_DUE_RING * sizeof(unsigned), st));
     if (d.due_on != (e->due_ok ? 1 : 0)) {  // fresh state: every latent person is listed from here on
@@ -1004,7 +1032,7 @@ int cs_reset_realisations(cs_engine* e, int n_seed)
     if ((rc = launch_update_probs(e, nullptr))) return rc;
     cs::k_zero_counts<<<1, 64, 0, st>>>(d.wl, cs::WL_N);
     cs::k_seed<<<grid_for(d.R, 64), 64, 0, st>>>(d, n_seed);
-    cs::k_list_move<<<e->grid_moves, 256, 0, st>>>(d, 0);
+    launch_list_move(e, st, 0);
     HIPCHK(e, hipGetLastError());
     return check_flags(e);
 }
@@ -1355,6 +1383,16 @@ int cs_set_launch_shape(cs_engine* e, uint32_t grid_waves, uint32_t grid_cells)
     return 0;
 }
 
+int cs_set_move_capacity(cs_engine* e, int per_cell)
+{
+    if (!e) return CS_EINVAL;
+    if (per_cell < 0 || per_cell > CS_MOVE_CAP)
+        return set_err(e, CS_EINVAL, "move capacity %d outside [0, %d]", per_cell, CS_MOVE_CAP);
+    e->move_cap = e->d.move_cap = per_cell;
+    drop_graph(e);
+    return 0;
+}
+
 int cs_autotune(cs_engine* e, double t, int reps, cs_tuning* out)
 {
     if (!e || reps <= 0) return CS_EINVAL;
@@ -1443,6 +1481,13 @@ int cs_set_dest_tables(cs_engine* e, int stride)
 {
     if (!e || stride < -1) return CS_EINVAL;
     e->dest_stride = stride;
+    return 0;
+}
+
+int cs_set_dest_index(cs_engine* e, int enable)
+{
+    if (!e) return CS_EINVAL;
+    e->dest_index = enable != 0;
     return 0;
 }
 
@@ -1585,8 +1630,14 @@ int cs_get_adunit_counters(cs_engine* e, int real, int64_t* out, int reset)
     HIPCHK(e, hipSetDevice(e->device));
     HIPCHK(e, hipStreamSynchronize(e->stream));
     const size_t n = (size_t)d.NA * CS_TSU_FIELDS;
-    HIPCHK(e, hipMemcpy(out, d.ad_cnt + (size_t)real * n, n * 8, hipMemcpyDeviceToHost));
-    if (reset) HIPCHK(e, hipMemset(d.ad_cnt + (size_t)real * n, 0, n * 8));
+    std::vector<unsigned long long> h(n);
+    for (size_t i = 0; i < n; i++) out[i] = 0;
+    for (int k = 0; k < CS_AD_COPIES; k++) {  // the sum of the copies (ad_counter)
+        unsigned long long* src = d.ad_cnt + ((size_t)k * e->R + real) * n;
+        HIPCHK(e, hipMemcpy(h.data(), src, n * 8, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < n; i++) out[i] += (int64_t)h[i];
+        if (reset) HIPCHK(e, hipMemset(src, 0, n * 8));
+    }
     return 0;
 }
 

@@ -278,7 +278,7 @@ __global__ void k_build_max_trans(Dev d, float* out)
     size_t n = (size_t)d.NC * d.NC;
     if (i >= n) return;
     int l = (int)(i / d.NC), m = (int)(i % d.NC);
-    out[i] = max_trans_of(d, l, m);
+    out[i] = max_trans_of<true>(d, l, m);
 }
 
 // UpdateProbs(0), S0 part (src/CovidSim.cpp:2756-2768); only realisations
@@ -317,6 +317,15 @@ __global__ void k_update_rows_ck(Dev d, const int* sel)
     int r = (int)(i / d.NC), j = (int)(i % d.NC);
     if (sel && !sel[r]) return;
     update_row_ck(d, r, j);
+}
+
+// the rows' bucket index after k_update_rows_ck (ck_index_row): a wave per
+// (realisation, source cell)
+__global__ void __launch_bounds__(256) k_ck_index(Dev d, const int* sel)
+{
+    const size_t rows = (size_t)d.R * d.NC, waves = ((size_t)gridDim.x * blockDim.x) >> 6;
+    for (size_t i = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; i < rows; i += waves)
+        if (!sel || sel[i / d.NC]) ck_index_row(d, i);
 }
 
 // cum_trans, InvCDF and max_trans of realisation r expanded from the
@@ -365,7 +374,8 @@ __global__ void k_sdrop(Dev d, int* sel)
 
 // ------------------------------------------------------- list movement
 
-#define CS_MOVE_CAP 4096
+#define CS_MOVE_CAP 4096       // moves one cell's LDS arrays hold (k_list_move)
+#define CS_MOVE_BIG_BLOCKS 64  // blocks of k_list_move_big, each with its own bitmap
 
 // Move each cell's batch to the tail of its segment.  mode 0: S->L,
 // 1: L->I, 2: I->R/D.  One block per (realisation, cell).  The batch is
@@ -380,8 +390,8 @@ __device__ void list_move_cell(const Dev& d, int mode, int rc, int32_t* pos, int
     int& bad = *bad_p;
     const int r = rc / d.NC, c = rc % d.NC;
     const int k = d.bcount[rc];
-    if (k > CS_MOVE_CAP) {
-        if (threadIdx.x == 0) atomicOr(d.status, (unsigned)FLAG_BATCH);
+    if (k > d.move_cap) {  // more than the LDS arrays take: k_list_move_big moves this cell
+        if (threadIdx.x == 0) d.mv_over[atomicAdd(d.mv_over_n, 1u)] = rc;
         return;
     }
     const size_t rN = (size_t)r * d.N;
@@ -534,14 +544,134 @@ __global__ void __launch_bounds__(256) k_list_move(Dev d, int mode)
         const int kk = (rc < c1) ? d.bcount[rc] : 0;
         __syncthreads();
         int t = 0;
-        for (unsigned long long m = __ballot(kk > 0 && kk <= 64); m; m &= m - 1, t++) {
+        for (unsigned long long m = __ballot(kk > 0 && kk <= 64 && kk <= d.move_cap); m; m &= m - 1, t++) {
             if (t % nw != w) continue;
             const int j = __builtin_ctzll(m);
             list_move_cell_wave(d, mode, (int)(b + j), __shfl(kk, j, 64));
         }
-        for (unsigned long long m = __ballot(kk > 64); m; m &= m - 1) {
+        for (unsigned long long m = __ballot(kk > 64 || kk > d.move_cap); m; m &= m - 1) {
             __syncthreads();  // the previous cell's use of the shared arrays is over
             list_move_cell(d, mode, (int)(b + __builtin_ctzll(m)), pos, fpos, occ, part, &dead_count, &bad);
+        }
+    }
+}
+
+// one cell's batched move in global memory, for a cell with more moves than
+// the LDS arrays hold; the pairing rule is list_move_cell's (the j-th batch
+// position below the window, in ascending order, swaps with the j-th window
+// position not holding a batch member).  The batch's segment positions are
+// marked in a bitmap of the segment (bm, W words) whose per-word exclusive
+// bit counts (pre) give each hole its rank, set bits below it, and each free
+// window position its rank, found by binary search
+__device__ void list_move_cell_big(const Dev& d, int mode, int rc, uint32_t* bm, int32_t* pre, int32_t* part,
+                                   int* dead_count_p, int* bad_p)
+{
+    int& dead_count = *dead_count_p;
+    int& bad = *bad_p;
+    const int r = rc / d.NC, c = rc % d.NC;
+    const int k = d.bcount[rc];
+    const size_t rN = (size_t)r * d.N;
+    const size_t base = rN + d.cstart[c];
+    const int S = d.cS[rc], L = d.cL[rc], I = d.cI[rc];
+    const int hi = (mode == 0) ? S : (mode == 1 ? S + L : S + L + I);
+    const int bnd = hi - k;
+    const int W = (hi + 31) >> 5;
+    if (threadIdx.x == 0) { dead_count = 0; bad = (W > d.mv_words || bnd < 0) ? 1 : 0; }
+    __syncthreads();
+    if (bad) {  // a segment longer than the cell (cannot happen for a valid state)
+        if (threadIdx.x == 0) atomicOr(d.status, (unsigned)FLAG_BATCH);
+        return;
+    }
+    for (int w = threadIdx.x; w < W; w += blockDim.x) bm[w] = 0u;
+    __syncthreads();
+    for (int i = threadIdx.x; i < k; i += blockDim.x) {
+        const int p = d.listpos[rN + d.batch[base + i]];
+        if (p < 0 || p >= hi) { bad = 1; continue; }
+        atomicOr(&bm[p >> 5], 1u << (p & 31));
+    }
+    __syncthreads();
+    if (bad) {  // a list position outside the segment: report, touch nothing
+        if (threadIdx.x == 0) atomicOr(d.status, (unsigned)FLAG_BATCH);
+        return;
+    }
+    {   // pre[w] = set bits in words below w (chunked exclusive scan, as list_move_cell's)
+        const int per = (W + blockDim.x - 1) / blockDim.x;
+        const int a0 = min(W, (int)threadIdx.x * per), a1 = min(W, a0 + per);
+        int cnt = 0;
+        for (int w = a0; w < a1; w++) cnt += __popc(bm[w]);
+        const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+        int incl = cnt;
+        for (int o = 1; o < 64; o <<= 1) {
+            const int y = __shfl_up(incl, o, 64);
+            if (lane >= o) incl += y;
+        }
+        if (lane == 63) part[wv] = incl;
+        __syncthreads();
+        int run = incl - cnt;
+        for (int q = 0; q < wv; q++) run += part[q];
+        for (int w = a0; w < a1; w++) { pre[w] = run; run += __popc(bm[w]); }
+    }
+    __syncthreads();
+    // set bits below position x (x < hi)
+    auto below = [&](int x) { return pre[x >> 5] + __popc(bm[x >> 5] & ((1u << (x & 31)) - 1u)); };
+    const int sb = below(bnd);
+    for (int i = threadIdx.x; i < k; i += blockDim.x) {
+        const int ph = d.listpos[rN + d.batch[base + i]];  // written only by this thread below
+        if (ph >= bnd) continue;
+        const int j = below(ph);  // every batch position below ph is a hole: ph's rank among the holes
+        // the j-th free window position: the largest q in [bnd, hi) with
+        // g(q) = (q - bnd) - (below(q) - sb) <= j, g counting the free
+        // positions in [bnd, q)
+        int lo = bnd, up = hi - 1;
+        while (lo < up) {
+            const int mid = (lo + up + 1) >> 1;
+            if ((mid - bnd) - (below(mid) - sb) <= j) lo = mid;
+            else up = mid - 1;
+        }
+        const int q = lo;
+        const int pa = d.perm[base + ph], pb = d.perm[base + q];
+        d.perm[base + ph] = pb;
+        d.listpos[rN + pb] = ph;
+        d.perm[base + q] = pa;
+        d.listpos[rN + pa] = q;
+        if (mode == 2) d.gate[base + ph] = d.gate[base + q];  // an infectious person stays: its gate moves with it
+    }
+    if (mode == 1)  // [bnd, hi) now heads the infectious list: its new timers are all to be looked at
+        for (int i = threadIdx.x; i < k; i += blockDim.x) d.gate[base + bnd + i] = 0;
+    if (mode == 2) {
+        int nd = 0;
+        for (int i = threadIdx.x; i < k; i += blockDim.x) nd += is_dead_code(d.inf[rN + d.batch[base + i]]);
+        if (nd) atomicAdd(&dead_count, nd);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (mode == 0) { d.cS[rc] = S - k; d.cL[rc] = L + k; }
+        else if (mode == 1) { d.cL[rc] = L - k; d.cI[rc] = I + k; }
+        else { d.cI[rc] = I - k; d.cD[rc] += dead_count; d.cR[rc] += k - dead_count; }
+        d.bcount[rc] = 0;
+    }
+}
+
+// the cells k_list_move queued (more moves than its LDS arrays hold), a
+// block per cell in turn, each block with its own bitmap; the last block to
+// finish empties the queue for the next launch
+__global__ void __launch_bounds__(256) k_list_move_big(Dev d, int mode)
+{
+    __shared__ int32_t part[256 / 64];
+    __shared__ int dead_count, bad;
+    const unsigned n = halted(d) ? 0u : d.mv_over_n[0];
+    uint32_t* bm = d.mv_bm + (size_t)blockIdx.x * 2 * d.mv_words;
+    int32_t* pre = (int32_t*)(bm + d.mv_words);
+    for (unsigned it = blockIdx.x; it < n; it += gridDim.x) {
+        list_move_cell_big(d, mode, d.mv_over[it], bm, pre, part, &dead_count, &bad);
+        __syncthreads();  // the bitmap and the shared words are free again
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        if (atomicAdd(d.mv_over_n + 1, 1u) == gridDim.x - 1) {  // every block has read the length
+            d.mv_over_n[0] = 0u;
+            d.mv_over_n[1] = 0u;
         }
     }
 }

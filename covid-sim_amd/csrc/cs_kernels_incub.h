@@ -625,7 +625,7 @@ __global__ void k_record(Dev d, int sample)
         if (d.ad_cnt)
             for (int q = threadIdx.x; q < CS_TSU_FIELDS * d.NA; q += blockDim.x) {
                 const int f = q / d.NA, u = q % d.NA;
-                const long long now = (long long)d.ad_cnt[((size_t)r * d.NA + u) * CS_TSU_FIELDS + f];
+                const long long now = ad_total(d, r, u, f);
                 long long* l = le + CS_TSA_CUM * CS_NUM_AGE_GROUPS + q;
                 o[CS_TSA_FIELDS * CS_NUM_AGE_GROUPS + q] = (int32_t)(now - *l);
                 *l = now;
@@ -659,8 +659,7 @@ __global__ void k_record_init(Dev d, double t0)
         }
         if (d.ad_cnt)
             for (int q = 0; q < CS_TSU_FIELDS * d.NA; q++)
-                le[CS_TSA_CUM * CS_NUM_AGE_GROUPS + q] =
-                    (long long)d.ad_cnt[((size_t)r * d.NA + q % d.NA) * CS_TSU_FIELDS + q / d.NA];
+                le[CS_TSA_CUM * CS_NUM_AGE_GROUPS + q] = ad_total(d, r, q % d.NA, q / d.NA);
     }
     // the series' first sample and its CalibrationThresholdCheck (src/CovidSim.cpp:1413-1414)
     if (d.P->interventions_active) itv_triggers(d, r, 0, t0, D);

@@ -47,6 +47,7 @@ struct Counters {  // device mirror of cs_counters, updated with atomics
 // few hot counters, and adds from the whole GPU to one line serialise at the
 // memory side.
 #define CS_TALLY_COPIES 16
+#define CS_AD_COPIES 16  // copies of the counters by administrative unit (ad_counter)
 static_assert(sizeof(Counters) % 8 == 0, "Counters is a sequence of 8-byte words");
 
 // one realisation's copy of the P and AdUnits fields the trigger logic sets
@@ -307,6 +308,7 @@ struct Dev {
     // fit: max_trans is evaluated where it is read, and each row keeps its
     // sequential float prefix only at the last cell of every ck cells
     float* ck_pre;        // [R][NC][ck_nb], the last entry being the row total
+    int32_t* ck_inv;      // [R][NC][1025] or null: per row and k, the first block whose normalised last prefix reaches k / 1024
     int ck, ck_nb;
     int32_t* perm;
     uint16_t *pl_cs, *pl_ce;  // [R][total places]
@@ -369,6 +371,11 @@ struct Dev {
     void bind_clr(ClaimRec* p) { clr = p; claim.base = p; lastk.base = p; }
     int32_t* batch;       // [R][N] by permutation slot range of the cell
     int32_t* bcount;      // [R*NC]
+    int32_t* mv_over;     // [R*NC] cells whose moves exceed move_cap, queued by k_list_move
+    unsigned* mv_over_n;  // [2] queue length, blocks of k_list_move_big done
+    uint32_t* mv_bm;      // [CS_MOVE_BIG_BLOCKS][2 * mv_words] per block: position bitmap, word prefixes
+    int mv_words;         // words of one bitmap: a cell's largest segment / 32, rounded up
+    int move_cap;         // largest move count the LDS path takes (<= CS_MOVE_CAP)
     int32_t *care_gp, *care_k;
     uint8_t* care_abs;
     unsigned int* care_count;
@@ -386,7 +393,7 @@ struct Dev {
     int32_t* ts_ext;
     long long* last_ext;
     int ext_w, NA;
-    unsigned long long* ad_cnt;  // [R][NA][CS_TSU_FIELDS] cumI, cumC, cumDC, cumD by unit, or null
+    unsigned long long* ad_cnt;  // [CS_AD_COPIES][R][NA][CS_TSU_FIELDS] cumI, cumC, cumDC, cumD by unit, or null
     long long* due;       // [CS_DUE_RING][CS_DUE_SHARDS][due_scap] persons (gp) by latent time
     unsigned* due_n;      // [CS_DUE_RING][CS_DUE_SHARDS] counts, CS_DUE_STRIDE words apart
     unsigned* due_ovf;    // [CS_DUE_RING] a shard of the bucket overflowed: that step scans every latent person
@@ -466,10 +473,22 @@ inline double kernel_fn(const cs_params& P, double r2)
     }
 }
 
-// a person's counters by administrative unit (StateT's *_adunit counters)
+// a person's counters by administrative unit (StateT's *_adunit counters):
+// CS_AD_COPIES copies, a block adding to the copy its index picks, so that
+// regions of few units do not send every block's adds to one address;
+// readers sum the copies (ad_total)
 __device__ __forceinline__ unsigned long long* ad_counter(const Dev& d, int r, int p, int field)
 {
-    return d.ad_cnt + ((size_t)r * d.NA + d.padunit[p]) * CS_TSU_FIELDS + field;
+    const size_t copy = blockIdx.x & (CS_AD_COPIES - 1);
+    return d.ad_cnt + ((copy * d.R + r) * d.NA + d.padunit[p]) * CS_TSU_FIELDS + field;
+}
+
+__device__ __forceinline__ long long ad_total(const Dev& d, int r, int u, int field)
+{
+    long long s = 0;
+    for (int k = 0; k < CS_AD_COPIES; k++)
+        s += (long long)d.ad_cnt[(((size_t)k * d.R + r) * d.NA + u) * CS_TSU_FIELDS + field];
+    return s;
 }
 
 // ---- kernel lookup (src/Kernels.cpp:108-129) ----
@@ -507,7 +526,7 @@ __device__ __forceinline__ double periodic_xy(const cs_params* P, double x, doub
     return x * x + y * y;
 }
 
-__device__ __noinline__ double dist2_utm(const cs_params* P, double x1, double y1, double x2, double y2)
+__device__ __forceinline__ double dist2_utm_inl(const cs_params* P, double x1, double y1, double x2, double y2)
 {
     double x, y, cy1, cy2, yt, xi, yi;
     x = __builtin_fabs(x1 - x2) / 2;
@@ -534,6 +553,12 @@ __device__ __noinline__ double dist2_utm(const cs_params* P, double x1, double y
     return 4 * er * er * y;
 }
 
+// the same out of line, for kernels whose register budget is tight
+__device__ __noinline__ double dist2_utm(const cs_params* P, double x1, double y1, double x2, double y2)
+{
+    return dist2_utm_inl(P, x1, y1, x2, y2);
+}
+
 // dist2 of two household locations (src/Dist.cpp dist2 over Households[].loc)
 __device__ __forceinline__ double dist2_ll(const cs_params* P, float2 la, float2 lb)
 {
@@ -558,6 +583,8 @@ __device__ __forceinline__ void cc_next_row(int& x, int& y, int nch)
         x++;
     }
 }
+// INL: the UTM distance inlined (loops that evaluate many independent pairs)
+template <bool INL = false>
 __device__ __forceinline__ double dist2_cc_min_xy(const cs_params* P, int lx, int ly, int mx, int my)
 {
     const int nch = P->nch;
@@ -573,7 +600,7 @@ __device__ __forceinline__ double dist2_cc_min_xy(const cs_params* P, int lx, in
         }
         if (my > ly) cc_next_row(ix, iy, nch);
         else if (my < ly) cc_next_row(jx, jy, nch);
-        return dist2_utm(P, P->cell_width * __builtin_fabs((double)ix), P->cell_height * __builtin_fabs((double)iy),
+        return (INL ? dist2_utm_inl : dist2_utm)(P, P->cell_width * __builtin_fabs((double)ix), P->cell_height * __builtin_fabs((double)iy),
                          P->cell_width * __builtin_fabs((double)jx), P->cell_height * __builtin_fabs((double)jy));
     }
     if (P->do_periodic_boundaries && (P->cell_width * ((double)abs(mx - lx)) > P->degrees_width * 0.5)) {

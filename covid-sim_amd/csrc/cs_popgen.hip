@@ -591,12 +591,19 @@ __global__ void k_rows(Dev gd)
     else cs::update_row(gd, 0, j);
 }
 
+// the rows' bucket index after checkpointed rows: a wave per row
+__global__ void __launch_bounds__(256) k_rows_index(Dev gd)
+{
+    const size_t i = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    if (i < (size_t)gd.NC) cs::ck_index_row(gd, i);
+}
+
 __global__ void k_build_mt(Dev gd, float* out)
 {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (size_t)gd.NC * gd.NC) return;
     const int l = (int)(i / gd.NC), m = (int)(i % gd.NC);
-    out[i] = cs::max_trans_of(gd, l, m);
+    out[i] = cs::max_trans_of<true>(gd, l, m);
 }
 
 __global__ void k_grav_round(Dev gd, int64_t nact, const int32_t* act, int tp, int round, const int32_t* p_hh,
@@ -1351,10 +1358,12 @@ int cs_popgen_run(cs_popgen* g, const cs_popgen_params* p, const cs_params* kp, 
                 pg::k_build_mt<<<blocks((long long)NC * NC), 256, 0, st>>>(gd, mt.p);
             } else {
                 gd.ck = 8;
-                while ((double)NC * NC * 4 / gd.ck > (double)(16ull << 30)) gd.ck *= 2;
+                while ((double)NC * NC * 4 / gd.ck + (double)NC * 1025 * 4 > (double)(16ull << 30)) gd.ck *= 2;
                 gd.ck_nb = (NC + gd.ck - 1) / gd.ck;
                 PGALLOC(g, ckp, (size_t)NC * gd.ck_nb);
                 gd.ck_pre = ckp.p;
+                PGALLOC(g, inv, (size_t)NC * 1025);  // the rows' bucket index (select_dest_ck's brackets)
+                gd.ck_inv = inv.p;
             }
             DArr<int32_t> tests, act;
             PGALLOC(g, tests, N);
@@ -1373,6 +1382,7 @@ int cs_popgen_run(cs_popgen* g, const cs_popgen_params* p, const cs_params* kp, 
                     if (round > 0) pg::k_slot_compact<<<blocks(NC), 256, 0, st>>>(NC, cell_slot.p, slots.p, S.p);
                     PGCHK(g, hipMemcpyAsync(S0.p, S.p, (size_t)NC * 4, hipMemcpyDeviceToDevice, st));
                     pg::k_rows<<<blocks(NC, 64), 64, 0, st>>>(gd);
+                    if (gd.ck_inv) pg::k_rows_index<<<blocks((long long)NC * 64, 256), 256, 0, st>>>(gd);
                     PGCHK(g, hipMemsetAsync(claims.p, 0, 8, st));
                     std::vector<int32_t> hs;
                     if ((rc = to_host(g, hs, S.p, NC))) return rc;

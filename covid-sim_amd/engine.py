@@ -158,6 +158,11 @@ class Engine:
         self.call("get_queue_capacity", C.byref(a), C.byref(b))
         return a.value, b.value
 
+    def set_move_capacity(self, per_cell):
+        """cells with more list moves in a step than per_cell move through
+        device memory instead of LDS (0: every cell; results are the same)"""
+        self.call("set_move_capacity", int(per_cell))
+
     def set_launch_shape(self, grid_waves=0, grid_cells=0):
         self.call("set_launch_shape", int(grid_waves), int(grid_cells))
 
@@ -170,6 +175,10 @@ class Engine:
     def set_dest_tables(self, stride):
         """Destination tables at the next alloc: 0 full, > 0 checkpoint stride, -1 automatic."""
         self.call("set_dest_tables", int(stride))
+
+    def set_dest_index(self, enable):
+        """Checkpointed tables' per-row bucket index at the next alloc (default on)."""
+        self.call("set_dest_index", int(bool(enable)))
 
     def dest_tables(self):
         s = C.c_int()

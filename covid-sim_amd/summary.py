@@ -78,16 +78,30 @@ def _rows(series, start, sample_days, t0):
 
 def write_age(path, series_age, ages, sample_days=1.0, t0=0.0, start=None):
     """The ensemble's age file (.age.xls of SaveSummaryResults,
-    src/CovidSim.cpp:2035-2063): per sample the mean incidence of infections,
-    cases and deaths of each 5-year age group, then a `dist` row with the
-    population's count in each group (AgeDist).  series_age is
-    [n_real, n_samples, CS_TSA_FIELDS, CS_NUM_AGE_GROUPS]
-    (Engine.timeseries_age); ages the persons' ages in years.  start, shaped
-    [n_real, CS_TSA_FIELDS, CS_NUM_AGE_GROUPS], is the sample the reference
-    takes at t0 itself (the seed infections: Engine.counters' cumIa after
-    the reset); given, it is the first row, as in the reference's file."""
+    src/CovidSim.cpp:2035-2063): per sample the mean number of infections,
+    cases and deaths of each 5-year age group since the realisation began,
+    then a `dist` row with the population's count in each group (AgeDist).
+    The reference's columns are cumulative although its fields are named
+    incIa, incCa, incDa: RecordSample copies StateT's cumIa, cumCa, cumDa
+    (src/CovidSim.cpp:3392-3400), which only InitModel zeroes (:928, :954).
+    series_age is [n_real, n_samples, CS_TSA_FIELDS, CS_NUM_AGE_GROUPS]
+    (Engine.timeseries_age, per-sample incidence), accumulated here per
+    realisation; ages the persons' ages in years.  start, shaped
+    [n_real, CS_TSA_FIELDS, CS_NUM_AGE_GROUPS], holds the counts at t0 (the
+    seed infections: Engine.counters' cumIa and cumCa after the reset): given,
+    the accumulation starts from it and it is the first row, as in the
+    reference's file; without it the counts start from zero."""
     g = abi.CS_NUM_AGE_GROUPS
-    ts, m = _rows(series_age, start, sample_days, t0)
+    x = np.asarray(series_age, np.int64)
+    flds = [abi.CS_TSA_INC_I, abi.CS_TSA_INC_C, abi.CS_TSA_INC_DEATH]
+    base = np.zeros((x.shape[0], len(flds), g), np.int64) if start is None else np.asarray(start, np.int64)[:, flds]
+    cum = base[:, None] + np.cumsum(x[:, :, flds], axis=1)  # [n_real, n_samples, 3, groups]
+    if start is not None:
+        cum = np.concatenate([base[:, None], cum], axis=1)
+        ts = [t0 + i * sample_days for i in range(cum.shape[1])]
+    else:
+        ts = [t0 + (i + 1) * sample_days for i in range(cum.shape[1])]
+    m = _mean(cum)
     cols = []
     for tag in ("I", "C", "D"):
         cols += ["%s%d-%d" % (tag, 5 * i, 5 * (i + 1)) for i in range(g)]
@@ -95,10 +109,7 @@ def write_age(path, series_age, ages, sample_days=1.0, t0=0.0, start=None):
     with open(path, "w") as f:
         f.write("t\t" + "\t".join(cols) + "\n")
         for i, t in enumerate(ts):
-            row = [t]
-            for fld in (abi.CS_TSA_INC_I, abi.CS_TSA_INC_C, abi.CS_TSA_INC_DEATH):
-                row += list(m[i, fld])
-            f.write("\t".join("%.10f" % v for v in row) + "\n")
+            f.write("\t".join("%.10f" % v for v in [t] + list(m[i].reshape(-1))) + "\n")
         f.write("dist" + "".join("\t%.10f" % v for v in dist) + "\n")
 
 

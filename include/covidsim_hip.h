@@ -449,11 +449,18 @@ int  cs_batch_run_steps(cs_batch* b, double t0, int n_steps, int steps_per_sampl
  * search redoing one block's prefixes in the same order.  Both give the
  * same destinations for the same uniforms, bit for bit.  -1 (default):
  * full tables while (realisations + 1) x NC^2 floats fit 16 GiB, else the
- * smallest power-of-two stride (at least 8) whose checkpoints fit 16 GiB.
- * cs_get_dest_tables gives the stride in use after an allocation (the
- * requested value before). */
+ * smallest power-of-two stride (at least 8) whose checkpoints and bucket
+ * index fit 16 GiB.  cs_get_dest_tables gives the stride in use after an
+ * allocation (the requested value before).  A search finds the first cell
+ * (or block) whose normalised prefix reaches u by binary search within the
+ * bracket of u's bucket of 1/1024: InvCDF with full tables; with
+ * checkpoints, a per-row index of each bucket's first block (R x NC x 1025
+ * int32), kept while it fits 4 GiB (cs_set_dest_index(e, 0) drops it, the
+ * search then spanning the row's blocks).  Both choices, for the next
+ * cs_alloc_realisations, leave every destination unchanged. */
 int  cs_set_dest_tables(cs_engine* e, int stride);
 int  cs_get_dest_tables(const cs_engine* e, int* stride);
+int  cs_set_dest_index(cs_engine* e, int enable);
 /* 1 (default): cs_run_steps replays captured step graphs; 0: it launches
  * every kernel of every step from the host */
 int  cs_set_step_graphs(cs_engine* e, int enable);
@@ -478,6 +485,14 @@ int  cs_set_due_capacity(cs_engine* e, uint32_t per_step);
  * sums (4 waves each, one wave per cell); 0 keeps a value.  Results do not
  * depend on them. */
 int  cs_set_launch_shape(cs_engine* e, uint32_t grid_waves, uint32_t grid_cells);
+/* the batched list moves of a step (the reference's per-person list swaps
+ * in DoInfect, DoIncub, DoRecover and DoDeath, src/Update.cpp:100-400): a
+ * cell with at most per_cell moves (default and maximum 4096) is moved in
+ * LDS, a larger one through a bitmap of its list segment in device memory
+ * (sized at cs_alloc_realisations from the largest cell); both pair the
+ * positions alike, so results do not depend on per_cell (0 sends every
+ * cell through device memory).  No counterpart in the reference. */
+int  cs_set_move_capacity(cs_engine* e, int per_cell);
 /* autotuner: at the current state and time t, times the InfectSweep event
  * phase (it reads the state and writes only scratch, so it can be repeated)
  * for each candidate launch shape, best of reps, keeps the largest shape

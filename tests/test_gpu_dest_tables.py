@@ -26,11 +26,12 @@ def engmod():
     return importlib.import_module("covid-sim_amd.engine")
 
 
-def _engine(engmod, params, pop, seeds, stride):
+def _engine(engmod, params, pop, seeds, stride, index=True):
     g = engmod.Engine(0)
     g.set_params(params)
     g.set_population(pop)
     g.set_dest_tables(stride)
+    g.set_dest_index(index)
     g.alloc(np.asarray(seeds, np.uint64))
     assert g.dest_tables() == stride
     return g
@@ -60,14 +61,27 @@ def test_tables_expand_to_the_full_ones(engmod, stride):
         _tables_equal(full.cell_tables(r, pop.n_cells), ck.cell_tables(r, pop.n_cells))
 
 
-@pytest.mark.parametrize("stride", [1, 5, 16])
-def test_destinations_equal_at_the_boundaries(engmod, stride):
+def _edges_and_buckets(rng, prefixes):
+    """random uniforms, each prefix value and the doubles either side of it,
+    and the bucket edges k / 1024 and the doubles either side of them"""
+    edges = prefixes.astype(np.float64)
+    edges = edges[edges < 1.0]
+    k = np.arange(1, 1024) / 1024.0
+    u = np.concatenate([rng.random(4000), edges, np.nextafter(edges, 0.0), np.nextafter(edges, 1.0),
+                        k, np.nextafter(k, 0.0), np.nextafter(k, 1.0)])
+    return u[(u > 0.0) & (u < 1.0)]
+
+
+@pytest.mark.parametrize("stride,index", [(1, True), (5, True), (16, True), (5, False), (16, False)])
+def test_destinations_equal_at_the_boundaries(engmod, stride, index):
     """Every source cell of C1 after 40 days: random uniforms, every
     normalised prefix value itself and the doubles just below and above it
-    (where a rounding slip in a redone block would show)."""
+    (where a rounding slip in a redone block would show), and the bucket
+    edges of the brackets; checkpoints with and without the rows' bucket
+    index."""
     pop, params = c1()
     full = _engine(engmod, params, pop, [9], 0)
-    ck = _engine(engmod, params, pop, [9], stride)
+    ck = _engine(engmod, params, pop, [9], stride, index)
     tspd = int(params.time_steps_per_day)
     for eng in (full, ck):
         eng.reset(10)
@@ -76,10 +90,7 @@ def test_destinations_equal_at_the_boundaries(engmod, stride):
     _, ct, _, _, _ = full.cell_tables(0, pop.n_cells)
     rng = np.random.default_rng(8)
     for c in range(pop.n_cells):
-        edges = ct[c].astype(np.float64)
-        edges = edges[edges < 1.0]
-        u = np.concatenate([rng.random(4000), edges, np.nextafter(edges, 0.0), np.nextafter(edges, 1.0)])
-        u = u[(u > 0.0) & (u < 1.0)]
+        u = _edges_and_buckets(rng, ct[c])
         a, b = full.select_dest(0, c, u), ck.select_dest(0, c, u)
         assert np.array_equal(a, b), c
         # the inverse-CDF definition: the first cell whose prefix reaches u
@@ -113,8 +124,8 @@ def test_trajectory_with_checkpoints_matches_oracle(engmod, csa, graphs):
 
 
 def test_many_cell_region_equals_full_tables(engmod, csa):
-    """A generated region of 1,156 cells (300k persons): the automatic
-    stride (128: 10 blocks a row, the last partial) against full tables,
+    """A generated region of 1,156 cells (300k persons): checkpoints every
+    128 cells (10 blocks a row, the last partial) against full tables,
     2 realisations through 80 days with the S-drop table rebuilds; the
     series and every person's state equal."""
     popgen = importlib.import_module("covid-sim_amd.popgen")
@@ -139,6 +150,34 @@ def test_many_cell_region_equals_full_tables(engmod, csa):
         full.store_state(r, a)
         ck.store_state(r, b)
         assert a.compare(b) == []
+
+
+def test_many_cell_destinations(engmod, csa):
+    """The 1,156-cell region after 30 days: for 60 source cells, random
+    uniforms, prefix values and bucket edges give the same destination from
+    full tables (InvCDF brackets), checkpoints every 16 and every 128 cells
+    with the rows' bucket index, and checkpoints without it; each is the first
+    cell whose prefix reaches u."""
+    popgen = importlib.import_module("covid-sim_amd.popgen")
+    arrays, geom = popgen.generate(300000, side_deg=2.55, seed=5)
+    pop = csa.population.Population(arrays)
+    _, params = c1()
+    params = type(params).from_buffer_copy(params)
+    popgen.apply_geometry(params, geom)
+    engs = [_engine(engmod, params, pop, [3], 0), _engine(engmod, params, pop, [3], 16),
+            _engine(engmod, params, pop, [3], 128), _engine(engmod, params, pop, [3], 16, False)]
+    tspd = int(params.time_steps_per_day)
+    for eng in engs:
+        eng.reset(10)
+        eng.run_steps(0.0, 30 * tspd, tspd)
+        eng.update_probs()
+    _, ct, _, _, _ = engs[0].cell_tables(0, pop.n_cells)
+    rng = np.random.default_rng(4)
+    for c in rng.choice(pop.n_cells, 60, replace=False):
+        u = _edges_and_buckets(rng, ct[c])
+        want = np.argmax(ct[c][None, :].astype(np.float64) >= u[:, None], axis=1)
+        for eng in engs:
+            assert np.array_equal(eng.select_dest(0, int(c), u), want), c
 
 
 def test_automatic_choice(engmod):

@@ -98,9 +98,11 @@ def test_dropin_gpu_ensemble_matches_reference():
 def test_dropin_gpu_adunit_and_age_files():
     """The reference's .adunit.xls and .age.xls written from a GPU run: the
     counters by administrative unit (cs_get_adunit_counters) and by age
-    (cs_get_counters) come from separate device counters, and per sample
-    their sums over units and over age groups agree, for infections and
-    cases; the infections go beyond the seeds."""
+    (cs_get_counters) come from separate device counters.  The age file
+    counts since the start (its columns are StateT's cumIa and cumCa, never
+    zeroed between samples), the unit file per sample: per sample the unit
+    file's sum over units equals the age file's increment over all groups,
+    for infections and cases; the infections go beyond the seeds."""
     need(DROPIN)
     out = tempfile.mkdtemp()
     pre = os.path.join(out, "pre.txt")
@@ -114,6 +116,7 @@ def test_dropin_gpu_adunit_and_age_files():
     na = (unit.shape[1] - 1) // 4
     g = (age.shape[1] - 1) // 3
     assert np.array_equal(unit[:, 0], age[:, 0])
-    assert np.allclose(unit[:, 1:1 + na].sum(axis=1), age[:, 1:1 + g].sum(axis=1))
-    assert np.allclose(unit[:, 1 + na:1 + 2 * na].sum(axis=1), age[:, 1 + g:1 + 2 * g].sum(axis=1))
+    assert np.allclose(unit[0, 1:1 + na].sum(), age[0, 1:1 + g].sum())
+    assert np.allclose(unit[1:, 1:1 + na].sum(axis=1), np.diff(age[:, 1:1 + g].sum(axis=1)))
+    assert np.allclose(unit[1:, 1 + na:1 + 2 * na].sum(axis=1), np.diff(age[:, 1 + g:1 + 2 * g].sum(axis=1)))
     assert unit[1:, 1:1 + na].sum() > 100

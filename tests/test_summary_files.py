@@ -63,16 +63,24 @@ def test_age_and_adunit_files_match_reference_layout(csa, tmp_path):
     v = np.loadtxt(p_age, skiprows=1, max_rows=DAYS + 1)
     assert np.array_equal(v[:, 0], np.arange(DAYS + 1))
     g = abi.CS_NUM_AGE_GROUPS
+    # the age file's columns count since the start (the reference's cumIa,
+    # cumCa, cumDa), the unit file's per sample (its *_adunit fields, zeroed
+    # at every sample)
     assert np.allclose(v[0, 1:1 + g], start_age[:, abi.CS_TSA_INC_I].mean(axis=0))
     assert v[0, 1:1 + g].sum() == 10
-    assert np.allclose(v[1:, 1:1 + g], ta[:, :, abi.CS_TSA_INC_I].mean(axis=0), atol=1e-10)
-    assert np.allclose(v[1:, 1 + g:1 + 2 * g], ta[:, :, abi.CS_TSA_INC_C].mean(axis=0), atol=1e-10)
-    assert np.allclose(v[1:, 1 + 2 * g:], ta[:, :, abi.CS_TSA_INC_DEATH].mean(axis=0), atol=1e-10)
+    for k, f in enumerate((abi.CS_TSA_INC_I, abi.CS_TSA_INC_C, abi.CS_TSA_INC_DEATH)):
+        cum = start_age[:, None, f] + np.cumsum(ta[:, :, f], axis=1)
+        assert np.allclose(v[1:, 1 + k * g:1 + (k + 1) * g], cum.mean(axis=0), atol=1e-10)
     u = np.loadtxt(p_unit, skiprows=1)
     assert u.shape == (DAYS + 1, 1 + 4 * na)
     for k, f in enumerate((abi.CS_TSU_INC_I, abi.CS_TSU_INC_C, abi.CS_TSU_INC_DC)):
         assert np.allclose(u[1:, 1 + k * na:1 + (k + 1) * na], tu[:, :, f].mean(axis=0), atol=1e-10)
     assert not u[:, 1 + 3 * na:].any()
-    # the infections over the whole file, seeds included, are the persons
-    # no longer susceptible at the end (every person started susceptible)
+    # per sample, the unit file's infections and cases over all units are the
+    # age file's increments over all groups
+    assert np.allclose(u[0, 1:1 + na].sum(), v[0, 1:1 + g].sum())
+    assert np.allclose(u[1:, 1:1 + na].sum(axis=1), np.diff(v[:, 1:1 + g].sum(axis=1)))
+    assert np.allclose(u[1:, 1 + na:1 + 2 * na].sum(axis=1), np.diff(v[:, 1 + g:1 + 2 * g].sum(axis=1)))
+    # the infections over the whole file, seeds included, are every infection
+    # of the run (the counters count since the reset)
     assert u[:, 1:1 + na].sum() * 2 == sum(o.adunit_counters(r, na)[:, abi.CS_TSU_INC_I].sum() for r in range(2))

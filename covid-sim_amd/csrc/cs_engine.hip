@@ -877,6 +877,7 @@ int cs_alloc_realisations(cs_engine* e, int n_real, const uint64_t* seeds)
     d.seed = sd;
     AL(hot, RN);
     d.bind_hot(d.hot);
+    AL(shot, RN);
     AL(gate, RN);
     AL(tim, RN);
     d.bind_tim(d.tim);
@@ -1114,6 +1115,8 @@ int cs_load_state(cs_engine* e, int real, const cs_state* s)
     HIPCHK(e, hipMemcpy(e->d_sel, sel.data(), e->R * sizeof(int), hipMemcpyHostToDevice));
     int rc = launch_update_probs(e, e->d_sel);
     if (rc) return rc;
+    cs::k_shot_build<<<grid_for(N, 256), 256, 0, e->stream>>>(d, real);  // the infectious persons' slot copies
+    HIPCHK(e, hipGetLastError());
     HIPCHK(e, hipStreamSynchronize(e->stream));
     return 0;
 }

@@ -440,7 +440,10 @@ __device__ void list_move_cell(const Dev& d, int mode, int rc, int32_t* pos, int
     // holes: batch positions below bnd, paired with the free slot of equal rank
     for (int i = threadIdx.x; i < k; i += blockDim.x) {
         int ph = pos[i];
-        if (ph >= bnd) continue;
+        if (ph >= bnd) {  // already in the window
+            if (mode == 1) d.shot[base + ph] = d.hot[rN + d.batch[base + i]];  // its slot copy, as it joins the infectious
+            continue;
+        }
         int rank = 0;
         for (int j = 0; j < k; j++) rank += (pos[j] < ph);
         int q = fpos[rank];
@@ -449,7 +452,11 @@ __device__ void list_move_cell(const Dev& d, int mode, int rc, int32_t* pos, int
         d.listpos[rN + pb] = ph;
         d.perm[base + q] = pa;
         d.listpos[rN + pa] = q;
-        if (mode == 2) d.gate[base + ph] = d.gate[base + q];  // an infectious person stays: its gate moves with it
+        if (mode == 1) d.shot[base + q] = d.hot[rN + pa];
+        if (mode == 2) {  // an infectious person stays: its gate and slot copy move with it
+            d.gate[base + ph] = d.gate[base + q];
+            d.shot[base + ph] = d.shot[base + q];
+        }
     }
     if (mode == 1)  // [bnd, hi) now heads the infectious list: its new timers are all to be looked at
         for (int i = threadIdx.x; i < k; i += blockDim.x) d.gate[base + bnd + i] = 0;
@@ -508,9 +515,14 @@ __device__ void list_move_cell_wave(const Dev& d, int mode, int rc, int k)
             d.listpos[rN + pb] = ph;
             d.perm[base + q] = pa;
             d.listpos[rN + pa] = q;
-            if (mode == 2) d.gate[base + ph] = d.gate[base + q];  // an infectious person stays: its gate moves with it
+            if (mode == 1) d.shot[base + q] = d.hot[rN + pa];  // its slot copy, as it joins the infectious
+            if (mode == 2) {  // an infectious person stays: its gate and slot copy move with it
+                d.gate[base + ph] = d.gate[base + q];
+                d.shot[base + ph] = d.shot[base + q];
+            }
         }
     }
+    if (mode == 1 && mine && pos >= bnd) d.shot[base + pos] = d.hot[rN + d.batch[base + lane]];  // already in the window
     if (mode == 1 && mine) d.gate[base + bnd + lane] = 0;  // [bnd, hi) now heads the infectious list
     const unsigned long long nd = wave_sum((unsigned long long)((mode == 2 && mine) ? is_dead_code(d.inf[rN + d.batch[base + lane]]) : 0));
     if (lane == 0) {
@@ -617,7 +629,10 @@ __device__ void list_move_cell_big(const Dev& d, int mode, int rc, uint32_t* bm,
     const int sb = below(bnd);
     for (int i = threadIdx.x; i < k; i += blockDim.x) {
         const int ph = d.listpos[rN + d.batch[base + i]];  // written only by this thread below
-        if (ph >= bnd) continue;
+        if (ph >= bnd) {  // already in the window
+            if (mode == 1) d.shot[base + ph] = d.hot[rN + d.batch[base + i]];  // its slot copy, as it joins the infectious
+            continue;
+        }
         const int j = below(ph);  // every batch position below ph is a hole: ph's rank among the holes
         // the j-th free window position: the largest q in [bnd, hi) with
         // g(q) = (q - bnd) - (below(q) - sb) <= j, g counting the free
@@ -634,7 +649,11 @@ __device__ void list_move_cell_big(const Dev& d, int mode, int rc, uint32_t* bm,
         d.listpos[rN + pb] = ph;
         d.perm[base + q] = pa;
         d.listpos[rN + pa] = q;
-        if (mode == 2) d.gate[base + ph] = d.gate[base + q];  // an infectious person stays: its gate moves with it
+        if (mode == 1) d.shot[base + q] = d.hot[rN + pa];  // its slot copy, as it joins the infectious
+        if (mode == 2) {  // an infectious person stays: its gate and slot copy move with it
+            d.gate[base + ph] = d.gate[base + q];
+            d.shot[base + ph] = d.shot[base + q];
+        }
     }
     if (mode == 1)  // [bnd, hi) now heads the infectious list: its new timers are all to be looked at
         for (int i = threadIdx.x; i < k; i += blockDim.x) d.gate[base + bnd + i] = 0;
@@ -834,11 +853,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) k
         const size_t rN = (size_t)r * d.N;
         const size_t sbase = rN + uni(d.cstart[c] + d.cS[rc] + d.cL[rc]);  // slot of infectious index 0
         const int i = d.perm[sbase + j];
-        const size_t gi = rN + i;
-        const int gen = d.itype[gi] / CS_INFECT_TYPE_MASK;
+        const Hot h = d.shot[sbase + j];  // i's record, by slot: one coalesced load
+        const int gen = h.itype / CS_INFECT_TYPE_MASK;
         // household (src/Sweep.cpp:331-377)
         if (sc.beta_h > 0) {
-            const unsigned fl = d.hflags[gi];
+            const unsigned fl = h.flags;
             const int nh = (int)(fl >> HOT_NH_SHIFT);  // the household's size, from i's own record
             if (nh > 1) {
                 w_hh_inf++;
@@ -851,13 +870,13 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) k
                 int f0 = 0;
                 if (!(fl & HOT_HH_DONE)) {
                     w_hh_scan += nh;
-                    f0 = i - (int)((d.hpfl[gi] >> PF_HPOS_SHIFT) & PF_HPOS_MASK);  // the household's first member
+                    f0 = i - (int)((h.pflags >> PF_HPOS_SHIFT) & PF_HPOS_MASK);  // the household's first member
                     for (int m = f0; m < f0 + nh; m++)
                         if (d.inf[rN + m] == CS_SUSCEPTIBLE) smask |= 1u << (m - f0);
-                    if (!smask) d.hflags[gi] = (uint8_t)(fl | HOT_HH_DONE);
+                    if (!smask) d.shot[sbase + j].flags = (uint8_t)(fl | HOT_HH_DONE);
                 }
                 if (smask) {
-                    double HI = sc.beta_h * house_inf(d, r, i, ts);
+                    double HI = sc.beta_h * house_inf_t(d, r, i, HotRec{h}, ts);
                     bool absent = false;
                     // a member absent from a closed place (src/Sweep.cpp:348-355); the
                     // member's absence is tested first (cheaper, same result)
@@ -881,9 +900,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) k
         }
         // spatial infectiousness (src/Sweep.cpp:649-690); the closed flag is the sign
         if (beta_s_of(d, sc, r) > 0) {
-            double SI = spatial_inf(d, r, i, ts);
+            double SI = spatial_inf_t(d, r, i, HotRec{h}, ts);
             bool flag = false;  // a linked place is closed and i is absent (tested first: same result)
-            if (P->do_places && host_absent(d, gi, ts))
+            if (P->do_places && h.t_as <= ts && h.t_ae > ts)
                 for (int pt = 0; pt < d.NPT && !flag; pt++) {
                     int lk = d.plinks[(size_t)i * CS_MAX_PLACE_TYPES + pt];
                     if (lk >= 0) flag = place_closed(d, r, pt, lk, ts);
@@ -961,13 +980,13 @@ __global__ void __launch_bounds__(256) k_infect_places(Dev d)
         const size_t rN = (size_t)r * d.N;
         const size_t sbase = rN + uni(d.cstart[c] + d.cS[rc] + d.cL[rc]);  // slot of infectious index 0
         const int i = d.perm[sbase + j];
-        const size_t gi = rN + i;
-        if ((d.hflags[gi] & HOT_NO_LINKS) || host_absent(d, gi, ts)) continue;
+        const Hot h = d.shot[sbase + j];  // i's record, by slot: one coalesced load
+        if ((h.flags & HOT_NO_LINKS) || (h.t_as <= ts && h.t_ae > ts)) continue;
         for (int pt = 0; pt < d.NPT; pt++) {
             const int lk = d.plinks[(size_t)i * CS_MAX_PLACE_TYPES + pt];
             if (lk < 0) continue;
             w_link++;
-            double PI = sc.fp * sc.seas * place_inf(d, r, i, pt, ts);
+            double PI = sc.fp * sc.seas * place_inf_t(d, r, i, HotRec{h}, pt, ts);
             const int mstart = d.pl_ms[pt][lk];
             for (int pass = 0; pass < 2; pass++) {
                 // Pc is both the capped place infectiousness and the

@@ -64,9 +64,11 @@ __device__ void holiday_places(const Dev& d, int h, double ht)
         if (d.pl_ce[pi] < he) d.pl_ce[pi] = (uint16_t)he;
         const size_t rN = (size_t)r * d.N;
         for (int q = d.pl_ms[pt][k]; q < d.pl_ms[pt][k + 1]; q++) {
-            size_t gm = rN + d.pl_mem[pt][q];
+            const int m = d.pl_mem[pt][q];
+            size_t gm = rN + m;
             if (d.t_as[gm] > hs) d.t_as[gm] = (uint16_t)hs;
             if (d.t_ae[gm] < he) d.t_ae[gm] = (uint16_t)he;
+            shot_sync(d, gm, m);
         }
     }
 }
@@ -297,6 +299,7 @@ __device__ int do_case(const Dev& d, int r, int p, unsigned ts)
             if (d.sev_cur[gp] == CS_SEV_ASYMPTOMATIC) { d.sev_cur[gp] = CS_SEV_ILI; sev_to(cn, CS_SK_ILI, ag); }
         }
     }
+    shot_sync(d, gp, p);  // the changed status and absence, in its slot copy
     return req;
 }
 
@@ -445,6 +448,7 @@ __global__ void k_care_apply(Dev d)
         size_t gk = (gp / d.N) * d.N + kk;
         if (!d.care_abs[a] || v < 0) d.t_as[gk] = (uint16_t)(ts + P->us_case_isolation_delay);
         d.t_ae[gk] = (uint16_t)(ts + P->us_case_isolation_delay + P->us_case_isolation_duration);
+        shot_sync(d, gk, kk);
     }
 }
 
@@ -631,6 +635,14 @@ __global__ void k_record(Dev d, int sample)
                 *l = now;
             }
     }
+}
+
+// the slot copies of realisation r's infectious persons, from a loaded state
+__global__ void k_shot_build(Dev d, int r)
+{
+    const size_t rN = (size_t)r * d.N;
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < d.N; p += gridDim.x * blockDim.x)
+        if (in_infectious_list(d.inf[rN + p])) d.shot[rN + d.cstart[d.pc[p]] + d.listpos[rN + p]] = d.hot[rN + p];
 }
 
 __global__ void k_record_init(Dev d, double t0)

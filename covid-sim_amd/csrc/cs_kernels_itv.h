@@ -174,11 +174,13 @@ __global__ void k_detected_ci(Dev d)
         if (host_absent(d, gp, ts)) {
             const int stop = (int)ts + P->us_case_absenteeism_delay + P->us_case_isolation_duration;
             if ((int)d.t_ae[gp] < stop) d.t_ae[gp] = (uint16_t)stop;
+            shot_sync(d, gp, p);
             continue;
         }
         if (!P->do_real_sympt_withdrawal) continue;
         d.t_as[gp] = (uint16_t)(ts + P->us_case_isolation_delay);
         d.t_ae[gp] = (uint16_t)(ts + P->us_case_isolation_delay + P->us_case_isolation_duration);
+        shot_sync(d, gp, p);
         const bool child = (int)d.age[p] < P->case_absent_child_age_cutoff;
         Counters* cn = d.cnt + r;
         if (P->do_places && !quarantined(d, gp, ts) && d.plinks[(size_t)p * CS_MAX_PLACE_TYPES + P->place_type_no_air_num - 1] >= 0 && !child)
@@ -319,6 +321,7 @@ __global__ void __launch_bounds__(256) k_close_members(Dev d, int pass)
                 // unordered min and max give one result
                 if (d.t_as[gw] > w.t_start) d.t_as[gw] = w.t_start;
                 if (d.t_ae[gw] < w.t_stop) d.t_ae[gw] = w.t_stop;
+                shot_sync(d, gw, (int)(gw % d.N));
             }
         }
     }
@@ -337,6 +340,7 @@ __global__ void k_host_closure(Dev d)
         const size_t gc = (size_t)d.care_k[a];
         if (d.t_as[gc] > w.t_start) d.t_as[gc] = w.t_start;
         if (d.t_ae[gc] < w.t_stop) d.t_ae[gc] = w.t_stop;
+        shot_sync(d, gc, (int)(gc % d.N));
     }
 }
 

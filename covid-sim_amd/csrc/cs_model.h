@@ -276,6 +276,7 @@ struct Dev {
     const int32_t* rseed_n;  // [R] per-realisation seed infections, or null
     const double* rthr;      // [R] per-realisation intervention trigger thresholds, or null
     Hot* hot;             // [R][N]
+    Hot* shot;            // [R][N] by list slot: the Hot record of the person at each slot of an infectious segment (else stale)
     CS_FLD(Hot, inf) inf;
     CS_FLD(Hot, flags) hflags;
     CS_FLD(Hot, age) hage;
@@ -638,30 +639,60 @@ __device__ __forceinline__ uint16_t icdf_choose(const double* c, double mean, do
 }
 
 // ---- hazards (src/CalcInfSusc.cpp:12-80); r = realisation, p = person ----
-__device__ __forceinline__ double inf_profile(const Dev& d, size_t gp, unsigned ts)
+// The infector-side hazards read the person's Hot fields through an
+// accessor: HotById reads them from d.hot by person (one field at a time),
+// HotRec from a record already in registers (the slot copy, Dev.shot, of an
+// infectious person).  Both give the same doubles.
+struct HotById {
+    const Dev& d;
+    size_t gp;
+    __device__ __forceinline__ int8_t inf() const { return d.inf[gp]; }
+    __device__ __forceinline__ uint8_t flags() const { return d.hflags[gp]; }
+    __device__ __forceinline__ uint8_t pflags() const { return d.hpfl[gp]; }
+    __device__ __forceinline__ uint8_t age() const { return d.hage[gp]; }
+    __device__ __forceinline__ float infness() const { return d.infness[gp]; }
+    __device__ __forceinline__ uint16_t t_lat() const { return d.t_lat[gp]; }
+};
+struct HotRec {
+    const Hot& h;
+    __device__ __forceinline__ int8_t inf() const { return h.inf; }
+    __device__ __forceinline__ uint8_t flags() const { return h.flags; }
+    __device__ __forceinline__ uint8_t pflags() const { return h.pflags; }
+    __device__ __forceinline__ uint8_t age() const { return h.age; }
+    __device__ __forceinline__ float infness() const { return h.infness; }
+    __device__ __forceinline__ uint16_t t_lat() const { return h.t_lat; }
+};
+
+template <class H>
+__device__ __forceinline__ double inf_profile_t(const Dev& d, const H& h, unsigned ts)
 {
-    int idx = (int)ts - (int)d.t_lat[gp] - 1;
+    int idx = (int)ts - (int)h.t_lat() - 1;
     idx = idx < 0 ? 0 : (idx >= CS_MAX_INFECTIOUS_STEPS ? CS_MAX_INFECTIOUS_STEPS - 1 : idx);
     return d.P->infectiousness[idx];
 }
+__device__ __forceinline__ double inf_profile(const Dev& d, size_t gp, unsigned ts) { return inf_profile_t(d, HotById{d, gp}, ts); }
 
-__device__ __forceinline__ double person_inf(const Dev& d, size_t gp, unsigned ts)
+template <class H>
+__device__ __forceinline__ double person_inf_t(const Dev& d, size_t gp, const H& h, unsigned ts)
 {
     const cs_params* P = d.P;
     return (treated(d, gp, ts) ? P->treat_inf_drop : 1.0)
          * (vacced(d, gp, ts) ? P->vacc_inf_drop : 1.0)
-         * __builtin_fabs((double)d.infness[gp])
-         * inf_profile(d, gp, ts);
+         * __builtin_fabs((double)h.infness())
+         * inf_profile_t(d, h, ts);
 }
+__device__ __forceinline__ double person_inf(const Dev& d, size_t gp, unsigned ts) { return person_inf_t(d, gp, HotById{d, gp}, ts); }
 
 // the resident count of p's household: its size (in p's Hot record) unless
 // the flags say otherwise
-__device__ __forceinline__ int hh_nhr(const Dev& d, size_t gp, int p)
+template <class H>
+__device__ __forceinline__ int hh_nhr_t(const Dev& d, const H& h, int p)
 {
-    return (d.hpfl[gp] & PF_NHR_DIFFERS) ? (int)d.hnhr[d.hh[p]] : (int)(d.hflags[gp] >> HOT_NH_SHIFT);
+    return (h.pflags() & PF_NHR_DIFFERS) ? (int)d.hnhr[d.hh[p]] : (int)(h.flags() >> HOT_NH_SHIFT);
 }
 
-__device__ __forceinline__ double house_inf(const Dev& d, int r, int p, unsigned ts)
+template <class H>
+__device__ __forceinline__ double house_inf_t(const Dev& d, int r, int p, const H& h, unsigned ts)
 {
     const cs_params* P = d.P;
     size_t gp = (size_t)r * d.N + p;
@@ -669,38 +700,69 @@ __device__ __forceinline__ double house_inf(const Dev& d, int r, int p, unsigned
     return ((iso && !dc) ? P->efficacies[0][CS_HOUSE] : 1.0)
          * (dc ? P->efficacies[5][CS_HOUSE] : 1.0)
          * ((qu && !dc && !iso) ? P->efficacies[1][CS_HOUSE] : 1.0)
-         * P->household_denom_lookup[hh_nhr(d, gp, p) - 1]
-         * (care_g(d, gp) ? P->care_home_resident_household_scaling : 1.0)
+         * P->household_denom_lookup[hh_nhr_t(d, h, p) - 1]
+         * ((h.pflags() & 1) ? P->care_home_resident_household_scaling : 1.0)
          * (treated(d, gp, ts) ? P->treat_inf_drop : 1.0)
          * (vacced(d, gp, ts) ? P->vacc_inf_drop : 1.0)
-         * (P->no_infectiousness_sd_in_hh ? ((d.infness[gp] < 0) ? P->sympt_infectiousness : P->asympt_infectiousness)
-                                          : __builtin_fabs((double)d.infness[gp]))
-         * inf_profile(d, gp, ts);
+         * (P->no_infectiousness_sd_in_hh ? ((h.infness() < 0) ? P->sympt_infectiousness : P->asympt_infectiousness)
+                                          : __builtin_fabs((double)h.infness()))
+         * inf_profile_t(d, h, ts);
+}
+__device__ __forceinline__ double house_inf(const Dev& d, int r, int p, unsigned ts)
+{
+    return house_inf_t(d, r, p, HotById{d, (size_t)r * d.N + p}, ts);
 }
 
-__device__ __forceinline__ double place_inf(const Dev& d, int r, int p, int pt, unsigned ts)
+template <class H>
+__device__ __forceinline__ double place_inf_t(const Dev& d, int r, int p, const H& h, int pt, unsigned ts)
 {
     const cs_params* P = d.P;
     size_t gp = (size_t)r * d.N + p;
     bool iso = isolated(d, gp, ts), dc = dct_traced(d, gp), qu = quarantined(d, gp, ts);
+    const bool care = (h.pflags() & 1) != 0;
     return ((iso && !dc) ? P->efficacies[0][pt] : 1.0)
          * (dc ? P->efficacies[5][pt] : 1.0)
-         * ((qu && !care_g(d, gp) && !dc && !iso) ? P->efficacies[1][pt] : 1.0)
-         * ((d.inf[gp] == CS_CASE && !care_g(d, gp)) ? P->sympt_place_type_contact_rate[pt] : 1.0)
-         * P->place_type_trans[pt] / P->place_type_group_size_param1[pt] * person_inf(d, gp, ts);
+         * ((qu && !care && !dc && !iso) ? P->efficacies[1][pt] : 1.0)
+         * ((h.inf() == CS_CASE && !care) ? P->sympt_place_type_contact_rate[pt] : 1.0)
+         * P->place_type_trans[pt] / P->place_type_group_size_param1[pt] * person_inf_t(d, gp, h, ts);
+}
+__device__ __forceinline__ double place_inf(const Dev& d, int r, int p, int pt, unsigned ts)
+{
+    return place_inf_t(d, r, p, HotById{d, (size_t)r * d.N + p}, pt, ts);
 }
 
-__device__ __forceinline__ double spatial_inf(const Dev& d, int r, int p, unsigned ts)
+template <class H>
+__device__ __forceinline__ double spatial_inf_t(const Dev& d, int r, int p, const H& h, unsigned ts)
 {
     const cs_params* P = d.P;
     size_t gp = (size_t)r * d.N + p;
     bool iso = isolated(d, gp, ts), dc = dct_traced(d, gp), qu = quarantined(d, gp, ts);
     return ((iso && !dc) ? P->efficacies[0][CS_SPATIAL] : 1.0)
          * (dc ? P->efficacies[5][CS_SPATIAL] : 1.0)
-         * ((qu && !care_g(d, gp) && !dc && !iso) ? P->efficacies[1][CS_SPATIAL] : 1.0)
-         * ((d.inf[gp] == CS_CASE) ? P->sympt_spatial_contact_rate : 1.0)
-         * P->relative_spatial_contact[age_group_g(d, gp)]
-         * person_inf(d, gp, ts);
+         * ((qu && !(h.pflags() & 1) && !dc && !iso) ? P->efficacies[1][CS_SPATIAL] : 1.0)
+         * ((h.inf() == CS_CASE) ? P->sympt_spatial_contact_rate : 1.0)
+         * P->relative_spatial_contact[(int)h.age() / CS_AGE_GROUP_WIDTH]
+         * person_inf_t(d, gp, h, ts);
+}
+__device__ __forceinline__ double spatial_inf(const Dev& d, int r, int p, unsigned ts)
+{
+    return spatial_inf_t(d, r, p, HotById{d, (size_t)r * d.N + p}, ts);
+}
+
+// an infectious person's slot copy of its Hot record (Dev.shot) after a
+// change of its Hot fields by a writer that reaches it by person: the copy
+// takes the record, keeping its own HOT_HH_DONE bit (set on the copy only)
+__device__ __forceinline__ bool in_infectious_list(int8_t s)
+{
+    return s == CS_INF_ALMOST_SYMPT || s == CS_INF_ASYMPT || s == CS_CASE;
+}
+__device__ __forceinline__ void shot_sync(const Dev& d, size_t gp, int p)
+{
+    if (!in_infectious_list(d.inf[gp])) return;
+    const size_t slot = (gp - (size_t)p) + d.cstart[d.pc[p]] + d.listpos[gp];
+    Hot h = d.hot[gp];
+    h.flags = (uint8_t)(h.flags | (d.shot[slot].flags & HOT_HH_DONE));
+    d.shot[slot] = h;
 }
 
 // the same with the two age groups given (a caller that already holds them)

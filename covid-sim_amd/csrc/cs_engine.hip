@@ -368,7 +368,7 @@ int launch_update_probs(cs_engine* e, const int* sel)
         cs::k_update_rows_ck<<<grid_for(n, 64), 64, 0, e->stream>>>(d, sel);
         if (d.ck_inv) cs::k_ck_index<<<grid_for(n * 64, 256), 256, 0, e->stream>>>(d, sel);
     }
-    else cs::k_update_rows<<<grid_for(n, 64), 64, 0, e->stream>>>(d, sel);
+    else cs::k_update_rows_wave<<<grid_for(n * 64, 256, 4096), 256, 0, e->stream>>>(d, sel);
     HIPCHK(e, hipGetLastError());
     return 0;
 }

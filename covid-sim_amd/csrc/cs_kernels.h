@@ -298,15 +298,15 @@ __global__ void k_update_s0(Dev d, const int* sel)
     d.S0[i] = s0;
 }
 
-// UpdateProbs(0), per-row sequential float prefix (src/CovidSim.cpp:2780-2800).
-// One thread per (realisation, source cell).
-__global__ void k_update_rows(Dev d, const int* sel)
+// UpdateProbs(0), per-row sequential float prefix (src/CovidSim.cpp:2780-2800),
+// a wave per (realisation, source cell) row (update_row_wave)
+__global__ void __launch_bounds__(256) k_update_rows_wave(Dev d, const int* sel)
 {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (size_t)d.R * d.NC) return;
-    int r = (int)(i / d.NC), j = (int)(i % d.NC);
-    if (sel && !sel[r]) return;
-    update_row(d, r, j);
+    const size_t rows = (size_t)d.R * d.NC, waves = ((size_t)gridDim.x * blockDim.x) >> 6;
+    for (size_t i = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; i < rows; i += waves) {
+        const int r = (int)(i / d.NC), j = (int)(i % d.NC);
+        if (!sel || sel[r]) update_row_wave(d, r, j);
+    }
 }
 
 // the same with checkpointed tables
@@ -890,7 +890,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) k
                     if (absent) HI *= P->efficacies[2][CS_HOUSE];
                     for (unsigned sm = smask; sm; sm &= sm - 1) {
                         const int m = f0 + __builtin_ctz(sm);
-                        double foi = HI * house_susc(d, r, m, ts, i);
+                        double foi = HI * house_susc_ag(d, r, m, ts, (int)h.age / CS_AGE_GROUP_WIDTH);
                         Stream s(seed, cw0(ts, PS_HH), (uint32_t)i, (uint32_t)(m - f0));
                         if (s.next() < foi)
                             add_event(d, ev_key(i, KP_HH, (uint32_t)(m - f0)), rN + m, i, 1 + CS_INFECT_TYPE_MASK * (1 + gen));

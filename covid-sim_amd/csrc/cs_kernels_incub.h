@@ -245,8 +245,9 @@ __global__ void __launch_bounds__(256) k_latent_scan(Dev d)
 }
 
 // DoCase (src/Update.cpp:794-894) without interventions; returns 1 when the
-// case is a child who needs an adult at home (applied after the phase)
-__device__ int do_case(const Dev& d, int r, int p, unsigned ts)
+// case is a child who needs an adult at home (applied after the phase).  sl
+// is the person's list slot (its Hot record's slot copy is updated there)
+__device__ int do_case(const Dev& d, int r, int p, unsigned ts, size_t sl)
 {
     const cs_params* P = d.P;
     const size_t gp = (size_t)r * d.N + p;
@@ -299,7 +300,7 @@ __device__ int do_case(const Dev& d, int r, int p, unsigned ts)
             if (d.sev_cur[gp] == CS_SEV_ASYMPTOMATIC) { d.sev_cur[gp] = CS_SEV_ILI; sev_to(cn, CS_SK_ILI, ag); }
         }
     }
-    shot_sync(d, gp, p);  // the changed status and absence, in its slot copy
+    shot_sync_at(d, gp, sl);  // the changed status and absence, in its slot copy
     return req;
 }
 
@@ -330,7 +331,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) k
         if (ag >= CS_NUM_AGE_GROUPS) ag = CS_NUM_AGE_GROUPS - 1;
         uint16_t ct = (uint16_t)(d.t_lat[gp] + ((int)(P->latent_to_sympt_delay / P->model_time_step)));
         if (P->do_symptoms && ts == ct) {
-            if (do_case(d, r, p, ts)) {
+            if (do_case(d, r, p, ts, sl)) {
                 const unsigned idx = atomicAdd(d.care_count, 1u);
                 if (idx < (unsigned)((size_t)d.R * d.N)) d.care_gp[idx] = (int32_t)gp;  // a child case makes at most this and one isolation request
                 else atomicOr(d.status, (unsigned)FLAG_LIST);

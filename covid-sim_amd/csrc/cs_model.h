@@ -756,13 +756,16 @@ __device__ __forceinline__ bool in_infectious_list(int8_t s)
 {
     return s == CS_INF_ALMOST_SYMPT || s == CS_INF_ASYMPT || s == CS_CASE;
 }
-__device__ __forceinline__ void shot_sync(const Dev& d, size_t gp, int p)
+__device__ __forceinline__ void shot_sync_at(const Dev& d, size_t gp, size_t slot)
 {
-    if (!in_infectious_list(d.inf[gp])) return;
-    const size_t slot = (gp - (size_t)p) + d.cstart[d.pc[p]] + d.listpos[gp];
     Hot h = d.hot[gp];
     h.flags = (uint8_t)(h.flags | (d.shot[slot].flags & HOT_HH_DONE));
     d.shot[slot] = h;
+}
+__device__ __forceinline__ void shot_sync(const Dev& d, size_t gp, int p)
+{
+    if (!in_infectious_list(d.inf[gp])) return;
+    shot_sync_at(d, gp, (gp - (size_t)p) + d.cstart[d.pc[p]] + d.listpos[gp]);
 }
 
 // the same with the two age groups given (a caller that already holds them)
@@ -781,14 +784,20 @@ __device__ __forceinline__ double person_susc(const Dev& d, int r, int p, unsign
     return person_susc_ag(d, gp, age_group_g(d, gp), age_group_g(d, (size_t)r * d.N + infector), ts);
 }
 
-__device__ __forceinline__ double house_susc(const Dev& d, int r, int p, unsigned ts, int infector)
+// the infector's age group given (a caller that already holds its record)
+__device__ __forceinline__ double house_susc_ag(const Dev& d, int r, int p, unsigned ts, int ag_infector)
 {
     const cs_params* P = d.P;
     size_t gp = (size_t)r * d.N + p;
-    return person_susc(d, r, p, ts, infector)
+    return person_susc_ag(d, gp, age_group_g(d, gp), ag_infector, ts)
          * (socdist(d, r, p) ? (esoc_g(d, gp) ? P->efficacies[4][CS_HOUSE] : P->efficacies[3][CS_HOUSE]) : 1.0)
          * (dct_traced(d, gp) ? P->efficacies[5][CS_HOUSE] : 1.0)
          * (care_g(d, gp) ? P->care_home_resident_household_scaling : 1.0);
+}
+
+__device__ __forceinline__ double house_susc(const Dev& d, int r, int p, unsigned ts, int infector)
+{
+    return house_susc_ag(d, r, p, ts, age_group_g(d, (size_t)r * d.N + infector));
 }
 
 __device__ __forceinline__ double place_susc(const Dev& d, int r, int p, int pt, unsigned ts)

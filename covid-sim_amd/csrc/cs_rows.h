@@ -53,6 +53,59 @@ __device__ inline void update_row(const Dev& d, int r, int j)
     }
 }
 
+// update_row by one whole wave: the same float sums, added one term at a time
+// in cell order (each lane's term read into the running sums in turn, so
+// every lane holds the same sums), with the row's loads, the normalisation
+// and InvCDF spread over the lanes.  InvCDF[k] is the first m with
+// cum_trans[m] * 1024 >= k (exact in float), so cell m is InvCDF's for
+// floor(1024 cum_trans[m - 1]) < k <= floor(1024 cum_trans[m]) and the lane
+// holding m writes those k.  A row of no weight (tp = 0: NaN prefixes)
+// leaves every InvCDF entry at 0, as the sequential loop does.
+__device__ inline void update_row_wave(const Dev& d, int r, int j)
+{
+    const int lane = (int)(threadIdx.x & 63);
+    const size_t i = (size_t)r * d.NC + j;
+    const int NC = d.NC;
+    float* ct = d.cum_trans + i * NC;
+    const float* mt = d.max_trans + (size_t)j * NC;
+    const int32_t* S0 = d.S0 + (size_t)r * NC;
+    float c = 0.f, t = 0.f;  // 0 + x == x: the first terms are the reference's ct[0] and t
+    for (int m0 = 0; m0 < NC; m0 += 64) {
+        const int m = m0 + lane;
+        float xs = 0.f, xt = 0.f;
+        if (m < NC) {
+            const float w = mt[m];
+            xs = ((float)S0[m]) * w;
+            xt = ((float)d.cn[m]) * w;
+        }
+        const int cnt = min(64, NC - m0);
+        float mine = 0.f;
+        for (int k = 0; k < cnt; k++) {
+            c = c + __int_as_float(__builtin_amdgcn_readlane(__float_as_int(xs), k));
+            t = t + __int_as_float(__builtin_amdgcn_readlane(__float_as_int(xt), k));
+            if (lane == k) mine = c;
+        }
+        if (m < NC) ct[m] = mine;
+    }
+    const float tp = c;
+    for (int m = lane; m < NC; m += 64) ct[m] = ct[m] / tp;
+    if (lane == 0) d.tot_prob[i] = tp / t;
+    int32_t* inv = d.invcdf + i * 1025;
+    if (!(tp / tp == 1.0f)) {
+        for (int k = lane; k <= 1024; k += 64) inv[k] = 0;
+        return;
+    }
+    for (int m = lane; m < NC; m += 64) {
+        const float v = ct[m];
+        const int klo = m ? (int)__builtin_floorf(ct[m - 1] * 1024.0f) + 1 : 0;
+        const int khi = min(1024, (int)__builtin_floorf(v * 1024.0f));
+        for (int k = klo; k <= khi; k++) {
+            const bool pure = k < 1024 && (double)v >= (double)(k + 1) / 1024.0;
+            inv[k] = pure ? (int32_t)((uint32_t)m | CS_INV_PURE) : m;
+        }
+    }
+}
+
 // destination cell: the cell where the reference's InvCDF start and
 // cum_trans scan stop (src/Sweep.cpp:757-762), the first l from
 // InvCDF[k] = InvCDF[floor(1024 u)] with cum_trans[l] >= u.  It lies at or

@@ -701,9 +701,13 @@ def summarise_pmc(d, last=0):
 
 def short_name(k):
     """A kernel's name without its argument list ("(anonymous namespace)::"
-    kept out of the cut)."""
+    kept out of the cut) and without the return type and template arguments
+    the profiler prints for a template instance ("void cs::k_x<false>")."""
     k = k.replace("(anonymous namespace)::", "")
-    return k.split("(")[0]
+    k = k.split("(")[0]
+    if k.startswith("void "):
+        k = k[len("void "):]
+    return k.split("<")[0]
 
 
 def summarise_trace(d, last=0):

@@ -756,11 +756,17 @@ __device__ __forceinline__ bool in_infectious_list(int8_t s)
 {
     return s == CS_INF_ALMOST_SYMPT || s == CS_INF_ASYMPT || s == CS_CASE;
 }
+// (in words, stored as two 8-byte halves: a Hot local, or a 16-byte vector
+// store, here made the compiler give these kernels LDS they do not use,
+// costing them occupancy)
+static_assert(offsetof(Hot, flags) == 1, "Hot.flags is byte 1 of the record's first word");
 __device__ __forceinline__ void shot_sync_at(const Dev& d, size_t gp, size_t slot)
 {
-    Hot h = d.hot[gp];
-    h.flags = (uint8_t)(h.flags | (d.shot[slot].flags & HOT_HH_DONE));
-    d.shot[slot] = h;
+    uint4 v = reinterpret_cast<const uint4*>(d.hot)[gp];
+    v.x |= reinterpret_cast<const uint32_t*>(d.shot + slot)[0] & ((uint32_t)HOT_HH_DONE << 8);
+    uint2* o = reinterpret_cast<uint2*>(d.shot + slot);
+    o[0] = make_uint2(v.x, v.y);
+    o[1] = make_uint2(v.z, v.w);
 }
 __device__ __forceinline__ void shot_sync(const Dev& d, size_t gp, int p)
 {

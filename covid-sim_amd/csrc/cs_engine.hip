@@ -301,7 +301,9 @@ int launch_incub(cs_engine* e)
     const unsigned nrc = (unsigned)((long long)d.R * d.NC);
     if (e->P.do_places && e->P.num_holidays > 0) {
         long long n = (long long)d.R * d.np_off[CS_MAX_PLACE_TYPES];
-        cs::k_holidays<<<grid_for(n, 256), 256, 0, st>>>(d);
+        // a holiday starts in few steps: a modest grid (the places loop is
+        // grid-strided) keeps the other steps' launch to a uniform test
+        cs::k_holidays<<<grid_for(n, 256, 2048), 256, 0, st>>>(d);
     }
     {
         ProfScope ps(e, CS_K_OTHER);

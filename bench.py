@@ -205,6 +205,9 @@ def main():
     ap.add_argument("--calibration", action="store_true", help="measure emulator-guided vs plain calibration instead")
     ap.add_argument("--profile-kinds", action="store_true", help="with --config large: time every launch by kind "
                     "(HIP events, direct launches)")
+    ap.add_argument("--icu-trigger", type=float, default=0.0, help="with --config large: the uneven-regions "
+                    "intervention subset (case isolation, household quarantine, place closure) with global ICU "
+                    "triggers at this many cumulative ICU cases; 0 (default) no interventions")
     ap.add_argument("--dest-stride", type=int, default=-1, help="destination tables (headline and large), 0 full, "
                     "> 0 checkpoints every so many cells, -1 (default) chosen by the table memory")
     ap.add_argument("--cells", type=int, nargs=2, default=[668, 336], metavar=("WIDE", "HIGH"),
@@ -530,6 +533,8 @@ def large_region(args):
     inp = os.path.join(ROOT, "covid-sim_amd", "inputs", "c1")
     base_pg = sm.load_popgen_params(os.path.join(inp, "popgen_params.bin"))
     base_kp = csa.population.load_params(os.path.join(inp, "params.bin"))
+    if args.icu_trigger > 0:  # the intervention subset of the uneven-regions parameters
+        base_kp = csa.population.load_params(os.path.join(ROOT, "covid-sim_amd", "inputs", "c3", "params.bin"))
     ncw, nch = args.cells
     pg, kp, dens, country, adunit = sm.synthetic_region(args.people, ncw, nch, 2025, base_pg, base_kp)
     log("region: %d persons, %d x %d cells, %d microcells" % (args.people, ncw, nch, pg.mcw * pg.mch))
@@ -551,6 +556,8 @@ def large_region(args):
     if args.dest_stride >= 0:
         eng.set_dest_tables(args.dest_stride)
     eng.alloc(np.array([0x9E3779B97F4A7C15], np.uint64))
+    if args.icu_trigger > 0:
+        eng.set_realisation_triggers(np.array([args.icu_trigger], np.float64))
     stride = eng.dest_tables()
     t_setup = time.perf_counter() - t1
     log("engine set-up %.1f s (host copy and checks); R0 %.2f (household %.2f, places %.2f), LocalBeta %.4g; "
@@ -574,6 +581,16 @@ def large_region(args):
     dt = time.perf_counter() - ta
     ts = eng.timeseries(args.days)
     attack = float((ts[0, -1, 3] + ts[0, -1, 4]) / args.people)
+    itv = None
+    if args.icu_trigger > 0:
+        st = eng.intervention_state(0)
+        itv = {"icu_trigger": args.icu_trigger, "case_isolation_prop": kp.case_isolation_prop,
+               "household_quarantine_prop": kp.hquarantine_prop_house_compliant,
+               "alert": int(st.alert),
+               "case_isolation_start_day": st.case_isolation_start if st.case_isolation_start < 1e9 else None,
+               "household_quarantine_start_day": st.hquarantine_start if st.hquarantine_start < 1e9 else None,
+               "place_closure_start_day": st.place_close_start if st.place_close_start < 1e9 else None}
+        log("interventions (ICU trigger %g): %s" % (args.icu_trigger, itv))
     kinds = None
     if args.profile_kinds:
         kinds = {k: [round(ms, 1), n] for k, (ms, n) in eng.profile().items() if n}
@@ -592,8 +609,11 @@ def large_region(args):
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": "one region of %d persons on %d x %d cells (%d populated), population generated on "
-                               "the device, 1 realisation, %d days, 4 steps/day, 10 seeds, no interventions"
-                               % (args.people, ncw, nch, sizes.n_cells, args.days),
+                               "the device, 1 realisation, %d days, 4 steps/day, 10 seeds, %s"
+                               % (args.people, ncw, nch, sizes.n_cells, args.days,
+                                  "case isolation 70%%, household quarantine 50%%, school and university closure "
+                                  "when cumulative ICU cases pass %g" % args.icu_trigger if args.icu_trigger > 0
+                                  else "no interventions"),
                    "persons": args.people, "cells": int(sizes.n_cells), "days": args.days,
                    "generation_seconds": round(t_gen, 3), "generation_persons_per_s": args.people / t_gen,
                    "engine_setup_seconds": round(t_setup, 1),
@@ -601,7 +621,7 @@ def large_region(args):
                    "r0": round(c.r0, 3), "attack_rate": round(attack, 4),
                    "launches": "direct launches, every launch timed" if args.profile_kinds else
                                ("direct launches" if args.no_graphs else "replayed step graphs"),
-                   "kernel_ms_by_kind": kinds},
+                   "kernel_ms_by_kind": kinds, "interventions": itv},
         "roofline": None,
         "cpu_baseline": None,
     }

@@ -20,6 +20,10 @@
 // RecordSample and the output files see the GPU run.
 //
 // CS_SHIM=cpu in the environment runs the reference's own sweeps instead.
+// CS_SHIM_TIMING=1 prints, at exit, where the GPU path's time went: the two
+// sweeps on the device, the state copies back to the host arrays after each
+// timestep, the state loads at each realisation's start and the table
+// rebuilds.
 // Any library error ends the run through ERR_CRITICAL, the reference's
 // convention for fatal errors.
 //
@@ -27,6 +31,7 @@
 // switch interventions on.  The counters by administrative unit (cumI, cumC,
 // cumDC, cumD per unit) are added to StateT[0] beside the others when the
 // parameters keep them (P.DoAdUnits with units known).
+#include <chrono>
 #include <climits>
 #include <cmath>
 #include <cstdio>
@@ -68,8 +73,25 @@ struct Shim {
     std::vector<int32_t> S0;  // cell S0 the library's tables were built from
     std::vector<int64_t> adc; // counters by administrative unit, [NumAdunits][CS_TSU_FIELDS]
     bool ad = false;          // the library keeps them
+    bool timing = false;      // CS_SHIM_TIMING
+    double t_sweep = 0, t_copy = 0, t_load = 0, t_probs = 0;
+    long n_steps = 0, n_loads = 0;
 };
 Shim g;
+
+double now_s()
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+void report_timing()
+{
+    fprintf(stderr,
+            "covidsim_hip shim: %ld timesteps: sweeps on the device %.3f s, state copies to the host %.3f s "
+            "(%.3f ms a step), %ld state loads %.3f s, table rebuilds %.3f s\n",
+            g.n_steps, g.t_sweep, g.t_copy, g.n_steps ? 1e3 * g.t_copy / (double)g.n_steps : 0.0, g.n_loads, g.t_load,
+            g.t_probs);
+}
 
 void check(int rc, const char* what)
 {
@@ -90,6 +112,8 @@ void setup_engine()
     const char* mode = getenv("CS_SHIM");
     g.gpu = !(mode && strcmp(mode, "cpu") == 0);
     if (!g.gpu) return;
+    g.timing = getenv("CS_SHIM_TIMING") != nullptr;
+    if (g.timing) atexit(report_timing);
     check(cs_create(&g.eng, 0), "cs_create");
     cs_bridge::fill_params(&g.prm);
     check(cs_set_params(g.eng, &g.prm, P.KernelLookup.lookup_.data(), P.KernelLookup.hi_res_.data()), "cs_set_params");
@@ -174,8 +198,10 @@ void InfectSweep(double t, int run)
         InfectSweep_cpu(t, run);
         return;
     }
+    double t0 = g.timing ? now_s() : 0.0;
     if (t == 0) {
         push_state(run);
+        if (g.timing) { g.t_load += now_s() - t0; g.n_loads++; }
     } else {
         // the host's UpdateProbs rebuilt its cell tables: rebuild ours from the same counts
         bool changed = false;
@@ -183,9 +209,12 @@ void InfectSweep(double t, int run)
         if (changed) {
             check(cs_update_probs(g.eng), "cs_update_probs");
             for (int j = 0; j < P.NumPopulatedCells; j++) g.S0[j] = CellLookup[j]->S0;
+            if (g.timing) { check(cs_sync(g.eng), "cs_sync"); g.t_probs += now_s() - t0; }
         }
     }
+    t0 = g.timing ? now_s() : 0.0;
     check(cs_infect_sweep(g.eng, t), "cs_infect_sweep");
+    if (g.timing) { check(cs_sync(g.eng), "cs_sync"); g.t_sweep += now_s() - t0; }
 }
 
 void IncubRecoverySweep(double t)
@@ -194,6 +223,14 @@ void IncubRecoverySweep(double t)
         IncubRecoverySweep_cpu(t);
         return;
     }
+    double t0 = g.timing ? now_s() : 0.0;
     check(cs_incub_recovery_sweep(g.eng, t), "cs_incub_recovery_sweep");
+    if (g.timing) {
+        check(cs_sync(g.eng), "cs_sync");
+        const double t1 = now_s();
+        g.t_sweep += t1 - t0;
+        t0 = t1;
+    }
     pull_state();
+    if (g.timing) { g.t_copy += now_s() - t0; g.n_steps++; }
 }

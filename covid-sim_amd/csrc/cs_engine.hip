@@ -51,6 +51,8 @@ struct cs_engine {
     bool dest_index = true;          // checkpointed tables keep the rows' bucket index (Dev.ck_inv) when it fits
     int cxy_nch = 0;                 // the nch Dev.cxy was built with
     float* mt_buf = nullptr;         // [NC][NC] max_trans (full tables only; built when first needed)
+    uint8_t* pin = nullptr;          // pinned host staging of cs_store_state (grown as needed)
+    size_t pin_cap = 0;
     int ts_max = 0, ts_n = 0;
     std::vector<uint8_t> flags0;     // [N] each person's Hot flags after a reset (no links, household size), for loaded states
     std::vector<uint8_t> h_age, h_flags;  // [N] the copies a loaded state's Hot records take
@@ -589,6 +591,7 @@ void cs_destroy(cs_engine* e)
     free_list(e->real_allocs);
     free_list(e->pop_allocs);
     if (e->mt_buf) (void)hipFree(e->mt_buf);
+    if (e->pin) (void)hipHostFree(e->pin);
     if (e->dP) (void)hipFree(e->dP);
     if (e->dkl) (void)hipFree(e->dkl);
     if (e->dkh) (void)hipFree(e->dkh);
@@ -1128,37 +1131,22 @@ int cs_store_state(cs_engine* e, int real, cs_state* s)
     if (!e || !s) return CS_EINVAL;
     if (real < 0 || real >= e->R) return set_err(e, CS_EINVAL, "realisation %d out of range", real);
     HIPCHK(e, hipSetDevice(e->device));
-    HIPCHK(e, hipStreamSynchronize(e->stream));
     Dev& d = e->d;
     const size_t N = e->N, NC = e->NC, o = (size_t)real * N, oc = (size_t)real * NC;
-    {
-        std::vector<cs::Hot> h(N);
-        HIPCHK(e, hipMemcpy(h.data(), d.hot + o, N * sizeof(cs::Hot), hipMemcpyDeviceToHost));
-        for (size_t i = 0; i < N; i++) {
-            const cs::Hot& x = h[i];
-            if (s->inf) s->inf[i] = x.inf;
-            if (s->infect_type) s->infect_type[i] = x.itype;
-            if (s->infectiousness) s->infectiousness[i] = x.infness;
-            if (s->latent_time) s->latent_time[i] = x.t_lat;
-            if (s->absent_start_time) s->absent_start_time[i] = x.t_as;
-            if (s->absent_stop_time) s->absent_stop_time[i] = x.t_ae;
-        }
-        std::vector<cs::Tim> m(N);
-        HIPCHK(e, hipMemcpy(m.data(), d.tim + o, N * sizeof(cs::Tim), hipMemcpyDeviceToHost));
-        for (size_t i = 0; i < N; i++) {
-            const cs::Tim& x = m[i];
-            if (s->infection_time) s->infection_time[i] = x.t_inf;
-            if (s->recovery_or_death_time) s->recovery_or_death_time[i] = x.t_rod;
-            if (s->sari_time) s->sari_time[i] = x.t_sari;
-            if (s->critical_time) s->critical_time[i] = x.t_crit;
-            if (s->stepdown_time) s->stepdown_time[i] = x.t_step;
-            if (s->detected_time) s->detected_time[i] = x.t_det;
-            if (s->severity_current) s->severity_current[i] = x.sev_cur;
-            if (s->severity_final) s->severity_final[i] = x.sev_final;
-            if (s->bits) s->bits[i] = x.bits;
-        }
-    }
-#define DN(src, dst, off, n) if (dst) HIPCHK(e, hipMemcpy(dst, d.src + (off), (n) * sizeof(*d.src), hipMemcpyDeviceToHost))
+    const size_t tot = d.np_off[CS_MAX_PLACE_TYPES];
+    // every array the caller asked for is copied into one pinned staging
+    // buffer (one asynchronous copy each, one wait), then unpacked: a
+    // drop-in caller stores the state after every timestep
+    struct Part { const void* src; void* dst; size_t bytes, off; };
+    std::vector<Part> parts;
+    size_t need = 0;
+    auto add = [&](const void* src, void* dst, size_t bytes) {
+        parts.push_back(Part{src, dst, bytes, need});
+        need += (bytes + 255) & ~(size_t)255;
+    };
+    add(d.hot + o, nullptr, N * sizeof(cs::Hot));
+    add(d.tim + o, nullptr, N * sizeof(cs::Tim));
+#define DN(src, dst, off, n) if (dst) add(d.src + (off), dst, (n) * sizeof(*d.src))
     DN(listpos, s->listpos, o, N); DN(infector, s->infector, o, N);
     DN(susc, s->susc, o, N); DN(prob_care, s->prob_care, o, N);
     DN(t_ts, s->treat_start_time, o, N); DN(t_te, s->treat_stop_time, o, N); DN(t_vs, s->vacc_start_time, o, N);
@@ -1166,7 +1154,6 @@ int cs_store_state(cs_engine* e, int real, cs_state* s)
     DN(dct, s->dct_traced, o, N);
     DN(cS, s->cell_S, oc, NC); DN(cL, s->cell_L, oc, NC); DN(cI, s->cell_I, oc, NC); DN(cR, s->cell_R, oc, NC);
     DN(cD, s->cell_D, oc, NC); DN(cumTC, s->cell_cumTC, oc, NC); DN(perm, s->cell_perm, o, N);
-    const size_t tot = d.np_off[CS_MAX_PLACE_TYPES];
     for (int t = 0; t < e->NPT; t++) {
         DN(pl_cs, s->place_close_start[t], real * tot + d.np_off[t], (size_t)e->np[t]);
         DN(pl_ce, s->place_close_end[t], real * tot + d.np_off[t], (size_t)e->np[t]);
@@ -1174,6 +1161,41 @@ int cs_store_state(cs_engine* e, int real, cs_state* s)
     DN(m_socdist, s->mcell_socdist, (size_t)real * e->NM, (size_t)e->NM);
     DN(m_moverest, s->mcell_moverest, (size_t)real * e->NM, (size_t)e->NM);
 #undef DN
+    if (need > e->pin_cap) {
+        if (e->pin) HIPCHK(e, hipHostFree(e->pin));
+        e->pin = nullptr;
+        e->pin_cap = 0;
+        HIPCHK(e, hipHostMalloc((void**)&e->pin, need, hipHostMallocDefault));
+        e->pin_cap = need;
+    }
+    for (const Part& q : parts)
+        if (q.bytes) HIPCHK(e, hipMemcpyAsync(e->pin + q.off, q.src, q.bytes, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    const cs::Hot* h = reinterpret_cast<const cs::Hot*>(e->pin + parts[0].off);
+    for (size_t i = 0; i < N; i++) {
+        const cs::Hot& x = h[i];
+        if (s->inf) s->inf[i] = x.inf;
+        if (s->infect_type) s->infect_type[i] = x.itype;
+        if (s->infectiousness) s->infectiousness[i] = x.infness;
+        if (s->latent_time) s->latent_time[i] = x.t_lat;
+        if (s->absent_start_time) s->absent_start_time[i] = x.t_as;
+        if (s->absent_stop_time) s->absent_stop_time[i] = x.t_ae;
+    }
+    const cs::Tim* m = reinterpret_cast<const cs::Tim*>(e->pin + parts[1].off);
+    for (size_t i = 0; i < N; i++) {
+        const cs::Tim& x = m[i];
+        if (s->infection_time) s->infection_time[i] = x.t_inf;
+        if (s->recovery_or_death_time) s->recovery_or_death_time[i] = x.t_rod;
+        if (s->sari_time) s->sari_time[i] = x.t_sari;
+        if (s->critical_time) s->critical_time[i] = x.t_crit;
+        if (s->stepdown_time) s->stepdown_time[i] = x.t_step;
+        if (s->detected_time) s->detected_time[i] = x.t_det;
+        if (s->severity_current) s->severity_current[i] = x.sev_cur;
+        if (s->severity_final) s->severity_final[i] = x.sev_final;
+        if (s->bits) s->bits[i] = x.bits;
+    }
+    for (size_t k = 2; k < parts.size(); k++)
+        if (parts[k].bytes) memcpy(parts[k].dst, e->pin + parts[k].off, parts[k].bytes);
     return 0;
 }
 

@@ -152,14 +152,15 @@ __device__ inline void update_row_ck(const Dev& d, int r, int j)
     for (int m0 = 0, b = 0; m0 < NC; m0 += B, b++) {
         const int m1 = min(NC, m0 + B);
         int m = m0;
-        // four terms evaluated together (independent loads and lookups in
-        // flight), added in cell order
-        for (; m + 4 <= m1; m += 4) {
-            float mt[4];
+        // eight terms evaluated together (independent loads and lookups in
+        // flight: a row rebuild has few rows per SIMD to hide latency with),
+        // added in cell order
+        for (; m + 8 <= m1; m += 8) {
+            float mt[8];
 #pragma unroll
-            for (int q = 0; q < 4; q++) mt[q] = max_trans_of<true>(d, j, m + q);
+            for (int q = 0; q < 8; q++) mt[q] = max_trans_of<true>(d, j, m + q);
 #pragma unroll
-            for (int q = 0; q < 4; q++) {
+            for (int q = 0; q < 8; q++) {
                 c = c + ((float)S0[m + q]) * mt[q];
                 t = t + ((float)d.cn[m + q]) * mt[q];
             }

@@ -230,8 +230,8 @@ def main():
     ap.add_argument("--cal-train", type=int, default=512, help="training candidates (8 seeds each)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py runs on one GPU: this build has no multi-GPU path (each realisation batch "
-              "would shard by realisation; see the design notes)", file=sys.stderr)
+        print("bench.py runs on one GPU: the library has no multi-GPU path (an ensemble would shard "
+              "by realisation, one batch of realisations per GPU)", file=sys.stderr)
         sys.exit(2)
     if args.summarise_pmc:
         return summarise_pmc(args.summarise_pmc, args.pmc_last)

@@ -1288,7 +1288,13 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k
                 slot = (int)(t.next() * ((double)d.S0[rl]));
                 m = d.perm[rN + d.cstart[l] + slot];
                 pm = d.pst[m];
-                inf_m = d.inf[rN + m];  // read with the candidate's record, not after the test
+                // the candidate's status from its list segment: the cell's
+                // lists are [S | L | I | R and D] and stay as they are through
+                // the event phase, so only a slot past S + L + I (a recovered
+                // or dead person) needs the person's own status byte
+                const int seg = d.cS[rl] + d.cL[rl] + d.cI[rl];
+                inf_m = (slot < d.cS[rl]) ? (int8_t)CS_SUSCEPTIBLE
+                      : (slot < seg) ? (int8_t)CS_LATENT : d.inf[rN + m];
                 const double d2 = dist2_ll(P, pm.loc, loc_o);
                 const double acc = kernel_num(d, d2) / mt;
                 accepted = !((t.next() >= acc) || is_dead_code(inf_m));

@@ -418,18 +418,23 @@ def test_per_realisation_params_match_oracle(engmod, csa):
     assert tg[3, -1, 0] < tg[0, -1, 0]
 
 
-def test_block_list_moves_match_oracle(engmod, csa):
+@pytest.mark.parametrize("cap", [None, 0])
+def test_block_list_moves_match_oracle(engmod, csa, cap):
     """Cells with more than 64 list moves in one step take the block path of
-    the batched move (a wave takes smaller ones).  On the 200k-person C1
-    region with its LocalBeta raised fourfold (so the peak is sharp), step by
-    step through days 34-37: every state equal to the oracle's, and the
-    per-cell move counts of each kind (S->L, L->I, I->R/D), derived from the
-    cell totals, exceed 64 in some cell."""
+    the batched move (a wave takes smaller ones); with move capacity 0 every
+    moving cell takes the device-memory bitmap path instead, here over list
+    segments of thousands of persons (many bitmap words per thread).  On the
+    200k-person C1 region with its LocalBeta raised fourfold (so the peak is
+    sharp), step by step through days 34-37: every state equal to the
+    oracle's, and the per-cell move counts of each kind (S->L, L->I,
+    I->R/D), derived from the cell totals, exceed 64 in some cell."""
     pop, params = c1()
     seeds = np.array([61, 62], np.uint64)
     g = engmod.Engine(0)
     g.set_params(params)
     g.set_population(pop)
+    if cap is not None:
+        g.set_move_capacity(cap)
     g.alloc(seeds)
     o = OracleEngine(params, pop)
     o.alloc(seeds)
